@@ -1,0 +1,129 @@
+"""Benchmark of the StereoTgv solve on one MI355X.
+
+A step is one masked solve (solveStereoForwardMasked) of one synthetic 848x800 pair
+(5 levels, 10 warps, 50 iterations; inputs resident in HBM).  Prints one JSON line:
+frame-pairs/s, p50/p99 per-solve latency, mean disparity error against the FP64 oracle,
+the roofline of the dominant kernel work (one inner TGV iteration at level 0) and the
+CPU baseline (the FP64 oracle on the host cores, one solve).
+
+    python bench.py [--steps K] [--warmup W] [--workload tgv_848x800] [--no-cpu]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+# Algorithmic bytes per pixel and inner iteration (FP32 storage): read u, u_, v_(2),
+# p(2), q(4), a, b, c, Iu, Iz and write u, u_, v_(2), p(2), q(4) = 25 floats.
+B_ITER_FP32 = 100.0
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--workload", default="tgv_848x800")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU-oracle baseline leg")
+    ap.add_argument("--iter-reps", type=int, default=200)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build measures one GPU only (asked for {args.gpus}); the solver shards by "
+              f"frame pair / stream with no cross-GPU exchange, so N GPUs run N independent copies.",
+              file=sys.stderr)
+        sys.exit(2)
+
+    import vfs_amd
+    from vfs_amd import synth as S
+
+    wl = S.WORKLOADS[args.workload]
+    P = S.SOLVER_PARAMS
+    W, H = wl["W"], wl["H"]
+    print(f"bench: workload {args.workload} {W}x{H} L{wl['levels']} W{wl['warps']} I{wl['iters']}", file=sys.stderr,
+          flush=True)
+    pair = S.make_pair(W, H, wl["seed"], wl["radius"], wl["dmax"])
+
+    g = vfs_amd.StereoTgv()
+    g.initialize(W, H, P["beta"], P["gamma"], P["alpha0"], P["alpha1"], P["timestep_lambda"], P["lam"],
+                 wl["levels"], P["fScale"], wl["warps"], wl["iters"])
+    g.limitRange = P["limitRange"]
+    g.copyMaskToDevice(pair["mask"])
+    g.loadVectorFields(pair["tv"], pair["cv"])
+    g.copyImagesToDevice(pair["i0"], pair["i1"])
+
+    for _ in range(args.warmup):
+        g.time_solves(True, 1)
+    g.synchronize()
+    lat = []
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        lat.append(g.time_solves(True, 1))
+    g.synchronize()
+    wall = time.perf_counter() - t0
+    lat = np.array(lat)
+    pairs_per_s = args.steps / wall
+    u_gpu = g.copy1DDisparityToHost()
+    print(f"bench: {args.steps} solves in {wall * 1e3:.1f} ms; per-solve p50 {np.percentile(lat, 50):.3f} ms",
+          file=sys.stderr, flush=True)
+
+    # roofline of the dominant work: the inner TGV iteration at level 0
+    w0, h0, _ = g.level_dims(0)
+    ms_it, launches = g.time_inner_iterations(0, args.iter_reps)
+    t_iter = ms_it / args.iter_reps * 1e-3
+    bytes_iter = B_ITER_FP32 * w0 * h0
+    achieved = bytes_iter / t_iter / 1e9
+
+    out = {
+        "metric": "frame-pairs/sec + p99 latency @848x800, mean disp err <=0.05px vs FP64",
+        "value": round(pairs_per_s, 3),
+        "unit": "frame-pairs/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(wall / args.steps * 1e3, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": f"{args.workload}: masked solve, {W}x{H}, {wl['levels']} levels, "
+                               f"{wl['warps']} warps, {wl['iters']} iterations, one pair per step, FP32",
+                   "global_batch": 1},
+        "latency_ms": {"p50": round(float(np.percentile(lat, 50)), 4),
+                       "p99": round(float(np.percentile(lat, 99)), 4),
+                       "mean": round(float(lat.mean()), 4)},
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "kernel": f"inner TGV iteration at level 0 ({w0}x{h0}), {launches} launch(es)",
+                     "us_per_iter": round(t_iter * 1e6, 3), "bytes_per_iter": int(bytes_iter)},
+    }
+
+    if not args.no_cpu:
+        from oracle import oracle as O
+        threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+        t0 = time.perf_counter()
+        o = O.solve_pair(pair, wl["levels"], wl["warps"], wl["iters"], P, precision="f64")
+        cpu_s = time.perf_counter() - t0
+        u_ref = o.copy1DDisparityToHost()
+        m = pair["mask"] > 0
+        err = np.abs(u_gpu - u_ref)[m]
+        out["mean_disp_err_px"] = float(f"{err.mean():.3e}")
+        out["max_disp_err_px"] = float(f"{err.max():.3e}")
+        out["cpu_baseline"] = {"value": round(1.0 / cpu_s, 4), "unit": "frame-pairs/s", "cores": threads,
+                               "kind": "port",
+                               "sample": f"one full {W}x{H} L{wl['levels']} W{wl['warps']} I{wl['iters']} masked "
+                                         f"solve of the same pair by the FP64 C oracle (OpenMP), {cpu_s:.2f} s"}
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,139 @@
+/*
+ * libvfs: C ABI of the gfx950 StereoTgv TGV-L1 fisheye-stereo solver.
+ *
+ * Every entry point replaces one member of the reference class `StereoTgv`
+ * (menandro/vfs stereotgv/stereotgv.h:14-252); the comment on each names the
+ * reference interface (file:line) it stands in for.  Conventions:
+ *   - an opaque handle per solver instance; no process-global mutable state, so
+ *     one handle per thread is safe;
+ *   - caller-owned host buffers, row pitch given explicitly in ELEMENTS of the
+ *     buffer's scalar type (floats for float planes, bytes for u8 planes);
+ *     multi-channel planes are interleaved (2 floats per pixel for vector fields
+ *     and flow, 3 floats per pixel for X);
+ *   - every call returns VFS_OK (0) or a negative status and never exits the
+ *     process; vfs_last_error() describes the last failure of a handle.
+ * No HIP, CUDA or OpenCV type appears in this header.
+ */
+#ifndef VFS_H
+#define VFS_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define VFS_OK 0
+#define VFS_ERR_ARG (-1)     /* bad argument: null pointer, size, pitch, level */
+#define VFS_ERR_STATE (-2)   /* call out of order (e.g. solve before initialize) */
+#define VFS_ERR_DEVICE (-3)  /* HIP runtime / kernel failure */
+#define VFS_ERR_NOMEM (-4)   /* device or host allocation failed */
+
+typedef struct vfs_solver vfs_solver;
+
+/* StereoTgv::StereoTgv() (stereotgv.cu:32-36): block 32x1, stride alignment 32. */
+int vfs_create(vfs_solver **out);
+
+/* StereoTgv::StereoTgv(int blockWidth, int blockHeight, int strideAlignment)
+ * (stereotgv.cu:38-42).  blockWidth sets the column coverage of the tensor launch
+ * (calcTensor.cu:81,227 launch ceil(h/blockWidth) blocks of blockWidth columns);
+ * strideAlignment sets the row pitch of every plane (stereotgv.cu:571-580).
+ * blockHeight is accepted and ignored (launch shapes are the library's own). */
+int vfs_create_ex(vfs_solver **out, int blockWidth, int blockHeight, int strideAlignment);
+
+/* ~StereoTgv() (stereotgv.h:18).  Unlike the reference, releases every buffer. */
+int vfs_destroy(vfs_solver *s);
+
+/* StereoTgv::initialize (stereotgv.h:107-109, stereotgv.cu:44-168).  Level sizes
+ * follow the reference: h_{l+1} = (int)((float)h_l / fScale), same for w. */
+int vfs_initialize(vfs_solver *s, int width, int height, float beta, float gamma, float alpha0,
+                   float alpha1, float timestep_lambda, float lambda, int nLevels, float fScale,
+                   int nWarpIters, int nSolverIters);
+
+/* Public field StereoTgv::limitRange (stereotgv.h:45, default 1.0f). */
+int vfs_set_limit_range(vfs_solver *s, float limit);
+int vfs_get_limit_range(const vfs_solver *s, float *limit);
+
+/* StereoTgv::copyMaskToDevice(cv::Mat CV_32F) (stereotgv.h:115, stereotgv.cu:241-252):
+ * width x height floats, nonzero = solve here; builds the nearest-neighbour pyramid. */
+int vfs_copy_mask(vfs_solver *s, const float *mask, int pitch);
+
+/* StereoTgv::loadVectorFields(cv::Mat tv CV_32FC2, cv::Mat cv CV_32FC2)
+ * (stereotgv.h:110, stereotgv.cu:171-199): trajectory field (downscaled per level)
+ * and calibration offsets (level 0).  Pitches in floats (>= 2 * width). */
+int vfs_load_vector_fields(vfs_solver *s, const float *tv, int tv_pitch, const float *cv, int cv_pitch);
+
+/* StereoTgv::copyImagesToDevice(unsigned char*, unsigned char*) (stereotgv.h:112,
+ * stereotgv.cu:201-213) and the CV_8U branch of copyImagesToDevice(cv::Mat, cv::Mat)
+ * (stereotgv.cu:220-226): gray = u8 / 256. */
+int vfs_copy_images_u8(vfs_solver *s, const uint8_t *i0, const uint8_t *i1, int pitch);
+
+/* CV_32F branch of StereoTgv::copyImagesToDevice(cv::Mat, cv::Mat) (stereotgv.cu:227-230). */
+int vfs_copy_images_f32(vfs_solver *s, const float *i0, const float *i1, int pitch);
+
+/* CV_8UC3 branch of StereoTgv::copyImagesToDevice(cv::Mat, cv::Mat) (stereotgv.cu:231-237):
+ * gray = (c0 + c1 + c2) / 3 / 256.  pitch in bytes. */
+int vfs_copy_images_u8c3(vfs_solver *s, const uint8_t *i0, const uint8_t *i1, int pitch);
+
+/* StereoTgv::solveStereoForwardMasked() (stereotgv.h:117, stereotgv.cu:389-515).
+ * Asynchronous on the solver's stream; the copy-out calls wait for it. */
+int vfs_solve_masked(vfs_solver *s);
+
+/* StereoTgv::solveStereoForward() (stereotgv.h:116, stereotgv.cu:256-387): the
+ * unmasked form (circle gate r < width/2.2, no median). */
+int vfs_solve(vfs_solver *s);
+
+/* StereoTgv::copy1DDisparityToHost (stereotgv.h:121, stereotgv.cu:533-540): u, width x height. */
+int vfs_copy_disparity_1d(vfs_solver *s, float *out, int pitch);
+
+/* StereoTgv::copyDisparityToHost (stereotgv.h:122, stereotgv.cu:542-549): the 2-D flow
+ * warpUV, 2 floats per pixel. */
+int vfs_copy_flow(vfs_solver *s, float *out, int pitch);
+
+/* StereoTgv::copyStereoToHost (stereotgv.h:118-120, stereotgv.cu:517-531): ConvertKB
+ * (conversion.cu:3-69, evaluated in FP64 on the GPU) of warpUV into depth (1 float per
+ * pixel) and X (3 floats per pixel).  Either output may be NULL. */
+int vfs_copy_stereo(vfs_solver *s, float *depth, int depth_pitch, float *X, int X_pitch, float focalx,
+                    float focaly, float cx, float cy, float d1, float d2, float d3, float d4, float t1,
+                    float t2, float t3);
+
+/* StereoTgv::copyWarpedImageToHost (stereotgv.h:124, stereotgv.cu:561-568): the last
+ * warped I1 (valid at level 0 after a solve). */
+int vfs_copy_warped_image(vfs_solver *s, float *out, int pitch);
+
+/* No counterpart in the reference (its checkCudaErrors exits the process): text of the
+ * last failure on this handle, "" if none.  Valid until the next call on the handle. */
+const char *vfs_last_error(const vfs_solver *s);
+
+/* Pyramid geometry (the reference's public pW/pH/pS vectors, stereotgv.h:68-70). */
+int vfs_get_level_dims(const vfs_solver *s, int level, int *w, int *h, int *stride);
+
+/* Inspection of an internal plane at `level` dims into a compact host array of
+ * w*h*nch floats (the reference exposes these as public device pointers,
+ * stereotgv.h:47-100).  Names: "u", "u_", "u_last", "a", "b", "c", "Iu", "Iz",
+ * "i1warp", "I0", "I1", "mask", "warpUV", "tv2", "tv", "p", "q", "v_", "Tp".
+ * Returns the channel count (> 0) or a negative status. */
+int vfs_read_plane(vfs_solver *s, const char *name, int level, float *out);
+
+/* Blocks until all work queued on the solver's stream has finished. */
+int vfs_synchronize(vfs_solver *s);
+
+/* Execution options: "graph" (1 = replay the solve as a captured HIP graph, default;
+ * 0 = launch kernels one by one). */
+int vfs_set_option(vfs_solver *s, const char *key, int value);
+
+/* Timing with HIP events on the solver's stream: runs `n` solves (masked != 0:
+ * solveStereoForwardMasked) back to back on resident inputs and returns their total
+ * wall time in ms. */
+int vfs_time_solves(vfs_solver *s, int masked, int n, float *ms_total);
+
+/* Timing of the inner TGV iteration at `level` (the dominant work): `reps` iterations
+ * launched back to back on the current solver state, HIP events around them.
+ * Returns the total time in ms and the number of kernel launches per iteration. */
+int vfs_time_inner_iterations(vfs_solver *s, int level, int reps, float *ms_total, int *launches_per_iter);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* VFS_H */

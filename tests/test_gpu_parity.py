@@ -1,0 +1,184 @@
+"""GPU parity: libvfs (HIP, FP32) against the CPU oracle (FP64 restatement of the
+reference kernels) on the same seeded inputs, through the C ABI.
+
+Tolerances: single kernels and single iterations agree to FP32 rounding (relative
+1e-5 scale); full solves must meet mean |u_gpu - u_fp64| <= 0.05 px over mask > 0
+pixels (the parity bar), and the tighter 0.005 px regression bound.
+"""
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from vfs_amd import StereoTgv
+from vfs_amd import synth as S
+
+pytestmark = pytest.mark.gpu
+
+P = S.SOLVER_PARAMS
+
+
+def run_gpu(pair, levels, warps, iters, masked=True, limit=None, graph=True, u8=False):
+    H, W = pair["i0"].shape
+    g = StereoTgv()
+    g.initialize(W, H, P["beta"], P["gamma"], P["alpha0"], P["alpha1"], P["timestep_lambda"], P["lam"], levels,
+                 P["fScale"], warps, iters)
+    g.limitRange = P["limitRange"] if limit is None else limit
+    g.set_option("graph", 1 if graph else 0)
+    g.copyMaskToDevice(pair["mask"])
+    g.loadVectorFields(pair["tv"], pair["cv"])
+    if u8:
+        g.copyImagesToDevice(pair["i0u8"], pair["i1u8"])
+    else:
+        g.copyImagesToDevice(pair["i0"], pair["i1"])
+    if masked:
+        g.solveStereoForwardMasked()
+    else:
+        g.solveStereoForward()
+    return g
+
+
+def run_oracle(pair, levels, warps, iters, masked=True, limit=None, precision="f64", u8=False):
+    H, W = pair["i0"].shape
+    o = O.Oracle(W, H, P["beta"], P["gamma"], P["alpha0"], P["alpha1"], P["timestep_lambda"], P["lam"], levels,
+                 P["fScale"], warps, iters, precision)
+    o.set_limit_range(P["limitRange"] if limit is None else limit)
+    o.copyMaskToDevice(pair["mask"])
+    o.loadVectorFields(pair["tv"], pair["cv"])
+    if u8:
+        o.copyImagesToDevice(pair["i0u8"], pair["i1u8"])
+    else:
+        o.copyImagesToDevice(pair["i0"], pair["i1"])
+    if masked:
+        o.solveStereoForwardMasked()
+    else:
+        o.solveStereoForward()
+    return o
+
+
+def small_pair(W=96, H=80, seed=7):
+    return S.make_pair(W, H, seed, radius=H / 2, dmax=W / 21.2)
+
+
+def assert_close(name, got, ref, rtol, atol):
+    got = np.asarray(got, np.float64)
+    ref = np.asarray(ref, np.float64)
+    assert got.shape == ref.shape, (name, got.shape, ref.shape)
+    bad = ~np.isclose(got, ref, rtol=rtol, atol=atol)
+    if bad.any():
+        idx = np.argwhere(bad)[:5]
+        raise AssertionError(f"{name}: {bad.sum()} / {bad.size} differ; first {idx.tolist()} "
+                             f"got {got[bad][:5]} ref {ref[bad][:5]}")
+
+
+def test_pyramids_and_fields():
+    pair = small_pair(106, 100, seed=3)
+    L = 4
+    g = run_gpu(pair, L, 0, 0)
+    o = run_oracle(pair, L, 0, 0)
+    for lvl in range(L):
+        assert g.level_dims(lvl) == o.level_dims(lvl)
+        for name in ("I0", "I1", "tv"):
+            assert_close(f"{name}@{lvl}", g.read_plane(name, lvl), o.read_plane(name, lvl), 1e-5, 1e-6)
+        assert np.array_equal(g.read_plane("mask", lvl), o.read_plane("mask", lvl)), f"mask@{lvl}"
+
+
+@pytest.mark.parametrize("masked", [True, False])
+def test_one_iteration_planes(masked):
+    """L=1, W=1, I=1: every intermediate plane of one warp and one TGV iteration."""
+    pair = small_pair()
+    g = run_gpu(pair, 1, 1, 1, masked=masked)
+    o = run_oracle(pair, 1, 1, 1, masked=masked)
+    for name in ("a", "b", "c", "i1warp", "tv2", "Iu", "Iz", "p", "q", "Tp", "u_last"):
+        ref = o.read_plane(name, 0)
+        got = g.read_plane(name, 0)
+        if masked and name in ("p", "q", "Tp"):
+            m = pair["mask"] > 0
+            got, ref = got[m], ref[m]
+        assert_close(name, got, ref, 1e-4, 1e-5)
+    for name in ("u", "u_", "warpUV", "v_"):
+        assert_close(name, g.read_plane(name, 0), o.read_plane(name, 0), 1e-4, 1e-5)
+
+
+@pytest.mark.parametrize("masked", [True, False])
+def test_small_solve(masked):
+    pair = small_pair(128, 96, seed=11)
+    g = run_gpu(pair, 3, 3, 10, masked=masked)
+    o = run_oracle(pair, 3, 3, 10, masked=masked)
+    m = pair["mask"] > 0
+    du = np.abs(g.copy1DDisparityToHost() - o.copy1DDisparityToHost())[m]
+    duv = np.linalg.norm(g.copyDisparityToHost() - o.copyDisparityToHost(), axis=-1)[m]
+    assert du.mean() <= 5e-4, du.mean()
+    assert duv.mean() <= 5e-4, duv.mean()
+
+
+def test_graph_equals_direct_launch():
+    pair = small_pair(128, 96, seed=5)
+    a = run_gpu(pair, 3, 2, 8, graph=True).copyDisparityToHost()
+    b = run_gpu(pair, 3, 2, 8, graph=False).copyDisparityToHost()
+    assert np.array_equal(a, b)
+
+
+def test_resolve_without_new_images_recalibrates():
+    """A second solve without copyImagesToDevice warps the calibrated I1 again
+    (the pI1[0] / d_i1calibrated swap of stereotgv.cu:392-393)."""
+    pair = small_pair(96, 80, seed=9)
+    g = run_gpu(pair, 2, 2, 5)
+    o = run_oracle(pair, 2, 2, 5)
+    g.solveStereoForwardMasked()
+    o.solveStereoForwardMasked()
+    m = pair["mask"] > 0
+    assert np.abs(g.copy1DDisparityToHost() - o.copy1DDisparityToHost())[m].mean() <= 5e-4
+    assert_close("I1@0", g.read_plane("I1", 0), o.read_plane("I1", 0), 1e-5, 1e-6)
+
+
+def test_u8_input_path():
+    pair = small_pair(96, 80, seed=13)
+    rng = np.random.default_rng(1)
+    pair["i0u8"] = (pair["i0"] * 255).astype(np.uint8)
+    pair["i1u8"] = np.clip(pair["i1"] * 255 + rng.normal(0, 1, pair["i1"].shape), 0, 255).astype(np.uint8)
+    g = run_gpu(pair, 2, 2, 5, u8=True)
+    o = run_oracle(pair, 2, 2, 5, u8=True)
+    assert_close("I0@0", g.read_plane("I0", 0), o.read_plane("I0", 0), 0, 0)
+    m = pair["mask"] > 0
+    assert np.abs(g.copy1DDisparityToHost() - o.copy1DDisparityToHost())[m].mean() <= 5e-4
+
+
+def test_solve_is_repeatable():
+    pair = small_pair(96, 80, seed=21)
+    g = run_gpu(pair, 2, 2, 5)
+    u1 = g.copy1DDisparityToHost()
+    g.copyImagesToDevice(pair["i0"], pair["i1"])
+    g.solveStereoForwardMasked()
+    assert np.array_equal(u1, g.copy1DDisparityToHost())
+
+
+def _workload_parity(name, tol_mean):
+    wl = S.WORKLOADS[name]
+    pair = S.make_pair(wl["W"], wl["H"], wl["seed"], wl["radius"], wl["dmax"])
+    g = run_gpu(pair, wl["levels"], wl["warps"], wl["iters"])
+    o = run_oracle(pair, wl["levels"], wl["warps"], wl["iters"])
+    m = pair["mask"] > 0
+    ug, uo = g.copy1DDisparityToHost(), o.copy1DDisparityToHost()
+    err = np.abs(ug - uo)[m]
+    epe = np.linalg.norm(g.copyDisparityToHost() - o.copyDisparityToHost(), axis=-1)[m]
+    print(f"{name}: mean |du| {err.mean():.2e} px, max {err.max():.3f} px, warpUV EPE mean {epe.mean():.2e}")
+    assert np.isfinite(ug[m]).all()
+    assert err.mean() <= tol_mean, err.mean()
+    assert epe.mean() <= tol_mean, epe.mean()
+    return g, o, pair
+
+
+def test_parity_424x400():
+    _workload_parity("tgv_424x400", 0.005)
+
+
+def test_parity_848x800():
+    g, o, pair = _workload_parity("tgv_848x800", 0.005)
+    # depth / 3-D output (ConvertKB, FP64 on both sides) from the same flow
+    intr = (285.722, 286.759, 420.135, 403.394, -0.00659769, 0.0473251, -0.0458264, 0.00897725,
+            -0.0641854, -0.000218299, 0.000111253)
+    dg, Xg = g.copyStereoToHost(*intr)
+    flow = g.copyDisparityToHost()
+    do, Xo = O.k_convert_kb(flow, *intr)
+    m = (pair["mask"] > 0) & np.isfinite(do) & (np.abs(do) < 1e4)
+    assert_close("depth", dg[m], do[m], 1e-5, 1e-5)

@@ -1,0 +1,234 @@
+"""Python host mirror of the StereoTgv class surface over libvfs's C ABI.
+
+Method names, argument meaning and order follow the reference class
+(menandro/vfs stereotgv/stereotgv.h:107-124) so that scripts read like
+test_vfs/main.cpp.  All compute runs in libvfs.so (HIP kernels for gfx950); there is
+no CPU fallback: if the library is missing or no GPU is present, calls raise.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+import numpy as np
+
+__all__ = ["StereoTgv", "VfsError", "lib", "lib_path", "EXPORTS"]
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = None
+
+_c_float_p = ctypes.POINTER(ctypes.c_float)
+_c_u8_p = ctypes.POINTER(ctypes.c_uint8)
+_c_int_p = ctypes.POINTER(ctypes.c_int)
+
+# (name, restype, argtypes) for every entry point declared in include/vfs.h
+EXPORTS = [
+    ("vfs_create", ctypes.c_int, [ctypes.POINTER(ctypes.c_void_p)]),
+    ("vfs_create_ex", ctypes.c_int, [ctypes.POINTER(ctypes.c_void_p), ctypes.c_int, ctypes.c_int, ctypes.c_int]),
+    ("vfs_destroy", ctypes.c_int, [ctypes.c_void_p]),
+    ("vfs_initialize", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int] + [ctypes.c_float] * 6
+     + [ctypes.c_int, ctypes.c_float, ctypes.c_int, ctypes.c_int]),
+    ("vfs_set_limit_range", ctypes.c_int, [ctypes.c_void_p, ctypes.c_float]),
+    ("vfs_get_limit_range", ctypes.c_int, [ctypes.c_void_p, _c_float_p]),
+    ("vfs_copy_mask", ctypes.c_int, [ctypes.c_void_p, _c_float_p, ctypes.c_int]),
+    ("vfs_load_vector_fields", ctypes.c_int, [ctypes.c_void_p, _c_float_p, ctypes.c_int, _c_float_p, ctypes.c_int]),
+    ("vfs_copy_images_u8", ctypes.c_int, [ctypes.c_void_p, _c_u8_p, _c_u8_p, ctypes.c_int]),
+    ("vfs_copy_images_f32", ctypes.c_int, [ctypes.c_void_p, _c_float_p, _c_float_p, ctypes.c_int]),
+    ("vfs_copy_images_u8c3", ctypes.c_int, [ctypes.c_void_p, _c_u8_p, _c_u8_p, ctypes.c_int]),
+    ("vfs_solve_masked", ctypes.c_int, [ctypes.c_void_p]),
+    ("vfs_solve", ctypes.c_int, [ctypes.c_void_p]),
+    ("vfs_copy_disparity_1d", ctypes.c_int, [ctypes.c_void_p, _c_float_p, ctypes.c_int]),
+    ("vfs_copy_flow", ctypes.c_int, [ctypes.c_void_p, _c_float_p, ctypes.c_int]),
+    ("vfs_copy_stereo", ctypes.c_int, [ctypes.c_void_p, _c_float_p, ctypes.c_int, _c_float_p, ctypes.c_int]
+     + [ctypes.c_float] * 11),
+    ("vfs_copy_warped_image", ctypes.c_int, [ctypes.c_void_p, _c_float_p, ctypes.c_int]),
+    ("vfs_last_error", ctypes.c_char_p, [ctypes.c_void_p]),
+    ("vfs_get_level_dims", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, _c_int_p, _c_int_p, _c_int_p]),
+    ("vfs_read_plane", ctypes.c_int, [ctypes.c_void_p, ctypes.c_char_p, ctypes.c_int, _c_float_p]),
+    ("vfs_synchronize", ctypes.c_int, [ctypes.c_void_p]),
+    ("vfs_set_option", ctypes.c_int, [ctypes.c_void_p, ctypes.c_char_p, ctypes.c_int]),
+    ("vfs_time_solves", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, _c_float_p]),
+    ("vfs_time_inner_iterations", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, _c_float_p, _c_int_p]),
+]
+
+
+class VfsError(RuntimeError):
+    pass
+
+
+def lib_path() -> str:
+    return os.path.join(_HERE, "libvfs.so")
+
+
+def lib() -> ctypes.CDLL:
+    """Load libvfs.so (built in-tree by __graft_entry__.build()).  Raises if absent."""
+    global _LIB
+    if _LIB is None:
+        path = lib_path()
+        if not os.path.exists(path):
+            raise VfsError(f"libvfs.so not built at {path}; run __graft_entry__.build()")
+        L = ctypes.CDLL(path)
+        for name, res, args in EXPORTS:
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _LIB = L
+    return _LIB
+
+
+def _fptr(a: np.ndarray):
+    return a.ctypes.data_as(_c_float_p)
+
+
+def _u8ptr(a: np.ndarray):
+    return a.ctypes.data_as(_c_u8_p)
+
+
+class StereoTgv:
+    """stereotgv.h:14-252 over libvfs.  One instance owns one device solver."""
+
+    def __init__(self, blockWidth: int = 32, blockHeight: int = 1, strideAlignment: int = 32):
+        self._L = lib()
+        h = ctypes.c_void_p()
+        rc = self._L.vfs_create_ex(ctypes.byref(h), blockWidth, blockHeight, strideAlignment)
+        if rc != 0:
+            raise VfsError(f"vfs_create_ex failed ({rc}): {self._L.vfs_last_error(None).decode()}")
+        self._h = h
+        self.width = self.height = 0
+
+    # -- helpers
+    def _check(self, rc: int, what: str) -> int:
+        if rc < 0:
+            msg = self._L.vfs_last_error(self._h).decode()
+            raise VfsError(f"{what} failed ({rc}): {msg}")
+        return rc
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self._L.vfs_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # -- reference surface
+    def initialize(self, width, height, beta, gamma, alpha0, alpha1, timestep_lambda, lambda_, nLevels, fScale,
+                   nWarpIters, nSolverIters):
+        self._check(self._L.vfs_initialize(self._h, width, height, beta, gamma, alpha0, alpha1, timestep_lambda,
+                                           lambda_, nLevels, fScale, nWarpIters, nSolverIters), "initialize")
+        self.width, self.height = width, height
+        return 0
+
+    @property
+    def limitRange(self) -> float:
+        v = ctypes.c_float()
+        self._check(self._L.vfs_get_limit_range(self._h, ctypes.byref(v)), "get_limit_range")
+        return v.value
+
+    @limitRange.setter
+    def limitRange(self, v: float):
+        self._check(self._L.vfs_set_limit_range(self._h, float(v)), "set_limit_range")
+
+    def copyMaskToDevice(self, mask: np.ndarray):
+        m = np.ascontiguousarray(mask, dtype=np.float32)
+        self._shape(m, (self.height, self.width))
+        return self._check(self._L.vfs_copy_mask(self._h, _fptr(m), m.shape[1]), "copyMaskToDevice")
+
+    def loadVectorFields(self, translationVector: np.ndarray, calibrationVector: np.ndarray):
+        tv = np.ascontiguousarray(translationVector, dtype=np.float32)
+        cv = np.ascontiguousarray(calibrationVector, dtype=np.float32)
+        self._shape(tv, (self.height, self.width, 2))
+        self._shape(cv, (self.height, self.width, 2))
+        return self._check(self._L.vfs_load_vector_fields(self._h, _fptr(tv), 2 * self.width, _fptr(cv),
+                                                          2 * self.width), "loadVectorFields")
+
+    def copyImagesToDevice(self, i0: np.ndarray, i1: np.ndarray):
+        """uint8 (H, W) -> CV_8U path, float32 (H, W) -> CV_32F path, uint8 (H, W, 3) -> CV_8UC3 path."""
+        if i0.dtype == np.uint8 and i0.ndim == 3:
+            a = np.ascontiguousarray(i0)
+            b = np.ascontiguousarray(i1, dtype=np.uint8)
+            self._shape(a, (self.height, self.width, 3))
+            self._shape(b, (self.height, self.width, 3))
+            return self._check(self._L.vfs_copy_images_u8c3(self._h, _u8ptr(a), _u8ptr(b), 3 * self.width),
+                               "copyImagesToDevice")
+        if i0.dtype == np.uint8:
+            a = np.ascontiguousarray(i0)
+            b = np.ascontiguousarray(i1, dtype=np.uint8)
+            self._shape(a, (self.height, self.width))
+            self._shape(b, (self.height, self.width))
+            return self._check(self._L.vfs_copy_images_u8(self._h, _u8ptr(a), _u8ptr(b), self.width),
+                               "copyImagesToDevice")
+        a = np.ascontiguousarray(i0, dtype=np.float32)
+        b = np.ascontiguousarray(i1, dtype=np.float32)
+        self._shape(a, (self.height, self.width))
+        self._shape(b, (self.height, self.width))
+        return self._check(self._L.vfs_copy_images_f32(self._h, _fptr(a), _fptr(b), self.width),
+                           "copyImagesToDevice")
+
+    def solveStereoForwardMasked(self):
+        return self._check(self._L.vfs_solve_masked(self._h), "solveStereoForwardMasked")
+
+    def solveStereoForward(self):
+        return self._check(self._L.vfs_solve(self._h), "solveStereoForward")
+
+    def copy1DDisparityToHost(self) -> np.ndarray:
+        out = np.empty((self.height, self.width), np.float32)
+        self._check(self._L.vfs_copy_disparity_1d(self._h, _fptr(out), self.width), "copy1DDisparityToHost")
+        return out
+
+    def copyDisparityToHost(self) -> np.ndarray:
+        out = np.empty((self.height, self.width, 2), np.float32)
+        self._check(self._L.vfs_copy_flow(self._h, _fptr(out), 2 * self.width), "copyDisparityToHost")
+        return out
+
+    def copyWarpedImageToHost(self) -> np.ndarray:
+        out = np.empty((self.height, self.width), np.float32)
+        self._check(self._L.vfs_copy_warped_image(self._h, _fptr(out), self.width), "copyWarpedImageToHost")
+        return out
+
+    def copyStereoToHost(self, focalx, focaly, cx, cy, d1, d2, d3, d4, t1, t2, t3):
+        depth = np.empty((self.height, self.width), np.float32)
+        X = np.empty((self.height, self.width, 3), np.float32)
+        self._check(self._L.vfs_copy_stereo(self._h, _fptr(depth), self.width, _fptr(X), 3 * self.width,
+                                            focalx, focaly, cx, cy, d1, d2, d3, d4, t1, t2, t3), "copyStereoToHost")
+        return depth, X
+
+    # -- inspection / timing (no reference counterpart)
+    def level_dims(self, level: int):
+        w, h, s = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+        self._check(self._L.vfs_get_level_dims(self._h, level, ctypes.byref(w), ctypes.byref(h), ctypes.byref(s)),
+                    "get_level_dims")
+        return w.value, h.value, s.value
+
+    def read_plane(self, name: str, level: int = 0) -> np.ndarray:
+        w, h, _ = self.level_dims(level)
+        buf = np.empty(h * w * 4, np.float32)
+        nch = self._check(self._L.vfs_read_plane(self._h, name.encode(), level, _fptr(buf)), f"read_plane({name})")
+        arr = buf[: h * w * nch].reshape(h, w, nch)
+        return arr[..., 0] if nch == 1 else arr
+
+    def synchronize(self):
+        return self._check(self._L.vfs_synchronize(self._h), "synchronize")
+
+    def set_option(self, key: str, value: int):
+        return self._check(self._L.vfs_set_option(self._h, key.encode(), int(value)), "set_option")
+
+    def time_solves(self, masked: bool = True, n: int = 1) -> float:
+        ms = ctypes.c_float()
+        self._check(self._L.vfs_time_solves(self._h, 1 if masked else 0, n, ctypes.byref(ms)), "time_solves")
+        return ms.value
+
+    def time_inner_iterations(self, level: int, reps: int):
+        ms = ctypes.c_float()
+        nl = ctypes.c_int()
+        self._check(self._L.vfs_time_inner_iterations(self._h, level, reps, ctypes.byref(ms), ctypes.byref(nl)),
+                    "time_inner_iterations")
+        return ms.value, nl.value
+
+    @staticmethod
+    def _shape(a: np.ndarray, shape):
+        if a.shape != tuple(shape):
+            raise VfsError(f"expected array of shape {tuple(shape)}, got {a.shape}")
