@@ -35,6 +35,9 @@ def main():
     ap.add_argument("--workload", default="tgv_848x800")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU-oracle baseline leg")
     ap.add_argument("--iter-reps", type=int, default=200)
+    ap.add_argument("--no-graph", action="store_true", help="launch kernels one by one instead of a HIP graph")
+    ap.add_argument("--iter-only", type=int, default=0,
+                    help="profiling aid: run N level-0 inner iterations (after a 256 MiB reference copy) and exit")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU only (asked for {args.gpus}); the solver shards by "
@@ -56,9 +59,21 @@ def main():
     g.initialize(W, H, P["beta"], P["gamma"], P["alpha0"], P["alpha1"], P["timestep_lambda"], P["lam"],
                  wl["levels"], P["fScale"], wl["warps"], wl["iters"])
     g.limitRange = P["limitRange"]
+    if args.no_graph:
+        g.set_option("graph", 0)
     g.copyMaskToDevice(pair["mask"])
     g.loadVectorFields(pair["tv"], pair["cv"])
     g.copyImagesToDevice(pair["i0"], pair["i1"])
+
+    if args.iter_only:
+        # A plain device copy of known size (256 MiB read + 256 MiB written per dispatch)
+        # calibrates the HBM counters (FETCH_SIZE / WRITE_SIZE), then only level-0 inner
+        # iterations run.
+        g.time_copy(256 << 20, 3)
+        ms, nl = g.time_inner_iterations(0, args.iter_only)
+        print(f"iter-only: {args.iter_only} level-0 iterations, {ms / args.iter_only * 1e3:.2f} us each "
+              f"({nl} launches per iteration)", flush=True)
+        return
 
     for _ in range(args.warmup):
         g.time_solves(True, 1)
@@ -71,6 +86,10 @@ def main():
     wall = time.perf_counter() - t0
     lat = np.array(lat)
     pairs_per_s = args.steps / wall
+    # Re-solving without new images re-calibrates I1 (the reference's pI1[0] swap), so
+    # the disparity error is measured on a solve of freshly copied images.
+    g.copyImagesToDevice(pair["i0"], pair["i1"])
+    g.solveStereoForwardMasked()
     u_gpu = g.copy1DDisparityToHost()
     print(f"bench: {args.steps} solves in {wall * 1e3:.1f} ms; per-solve p50 {np.percentile(lat, 50):.3f} ms",
           file=sys.stderr, flush=True)
@@ -81,6 +100,10 @@ def main():
     t_iter = ms_it / args.iter_reps * 1e-3
     bytes_iter = B_ITER_FP32 * w0 * h0
     achieved = bytes_iter / t_iter / 1e9
+
+    # achievable-bandwidth reference: a plain copy kernel, 1 GiB moved per launch
+    copy_ms = g.time_copy(512 << 20, 10) / 10
+    copy_gbs = 2 * (512 << 20) / (copy_ms * 1e-3) / 1e9
 
     out = {
         "metric": "frame-pairs/sec + p99 latency @848x800, mean disp err <=0.05px vs FP64",
@@ -104,7 +127,8 @@ def main():
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
                      "kernel": f"inner TGV iteration at level 0 ({w0}x{h0}), {launches} launch(es)",
-                     "us_per_iter": round(t_iter * 1e6, 3), "bytes_per_iter": int(bytes_iter)},
+                     "us_per_iter": round(t_iter * 1e6, 3), "bytes_per_iter": int(bytes_iter),
+                     "copy_kernel_GBs": round(copy_gbs, 1)},
     }
 
     if not args.no_cpu:

@@ -17,6 +17,7 @@
 #ifndef VFS_H
 #define VFS_H
 
+#include <stddef.h>
 #include <stdint.h>
 
 #ifdef __cplusplus
@@ -119,7 +120,8 @@ int vfs_read_plane(vfs_solver *s, const char *name, int level, float *out);
 int vfs_synchronize(vfs_solver *s);
 
 /* Execution options: "graph" (1 = replay the solve as a captured HIP graph, default;
- * 0 = launch kernels one by one). */
+ * 0 = launch kernels one by one); "fused" (1 = one launch per inner TGV iteration,
+ * default; 0 = two launches, dual+Tp then threshold+primal). */
 int vfs_set_option(vfs_solver *s, const char *key, int value);
 
 /* Timing with HIP events on the solver's stream: runs `n` solves (masked != 0:
@@ -131,6 +133,10 @@ int vfs_time_solves(vfs_solver *s, int masked, int n, float *ms_total);
  * launched back to back on the current solver state, HIP events around them.
  * Returns the total time in ms and the number of kernel launches per iteration. */
 int vfs_time_inner_iterations(vfs_solver *s, int level, int reps, float *ms_total, int *launches_per_iter);
+
+/* Timing of a plain device-to-device copy kernel moving `bytes` (read + write), `reps`
+ * times: the achievable-bandwidth reference for the solver's kernels. */
+int vfs_time_copy(vfs_solver *s, size_t bytes, int reps, float *ms_total);
 
 #ifdef __cplusplus
 }

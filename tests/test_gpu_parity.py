@@ -17,9 +17,10 @@ pytestmark = pytest.mark.gpu
 P = S.SOLVER_PARAMS
 
 
-def run_gpu(pair, levels, warps, iters, masked=True, limit=None, graph=True, u8=False):
+def run_gpu(pair, levels, warps, iters, masked=True, limit=None, graph=True, u8=False, fused=True):
     H, W = pair["i0"].shape
     g = StereoTgv()
+    g.set_option("fused", 1 if fused else 0)
     g.initialize(W, H, P["beta"], P["gamma"], P["alpha0"], P["alpha1"], P["timestep_lambda"], P["lam"], levels,
                  P["fScale"], warps, iters)
     g.limitRange = P["limitRange"] if limit is None else limit
@@ -82,11 +83,12 @@ def test_pyramids_and_fields():
         assert np.array_equal(g.read_plane("mask", lvl), o.read_plane("mask", lvl)), f"mask@{lvl}"
 
 
+@pytest.mark.parametrize("fused", [True, False])
 @pytest.mark.parametrize("masked", [True, False])
-def test_one_iteration_planes(masked):
+def test_one_iteration_planes(masked, fused):
     """L=1, W=1, I=1: every intermediate plane of one warp and one TGV iteration."""
     pair = small_pair()
-    g = run_gpu(pair, 1, 1, 1, masked=masked)
+    g = run_gpu(pair, 1, 1, 1, masked=masked, fused=fused)
     o = run_oracle(pair, 1, 1, 1, masked=masked)
     for name in ("a", "b", "c", "i1warp", "tv2", "Iu", "Iz", "p", "q", "Tp", "u_last"):
         ref = o.read_plane(name, 0)
@@ -109,6 +111,18 @@ def test_small_solve(masked):
     duv = np.linalg.norm(g.copyDisparityToHost() - o.copyDisparityToHost(), axis=-1)[m]
     assert du.mean() <= 5e-4, du.mean()
     assert duv.mean() <= 5e-4, duv.mean()
+
+
+@pytest.mark.parametrize("masked", [True, False])
+def test_fused_iteration_matches_two_launch_form(masked):
+    pair = small_pair(160, 120, seed=17)
+    a = run_gpu(pair, 3, 3, 12, masked=masked, fused=True)
+    b = run_gpu(pair, 3, 3, 12, masked=masked, fused=False)
+    m = pair["mask"] > 0
+    d = np.abs(a.copy1DDisparityToHost() - b.copy1DDisparityToHost())[m]
+    assert d.mean() <= 1e-5, d.mean()
+    for name in ("p", "q", "v_", "u_"):
+        assert_close(name, a.read_plane(name, 0)[m], b.read_plane(name, 0)[m], 1e-3, 1e-4)
 
 
 def test_graph_equals_direct_launch():
@@ -179,6 +193,14 @@ def test_parity_848x800():
             -0.0641854, -0.000218299, 0.000111253)
     dg, Xg = g.copyStereoToHost(*intr)
     flow = g.copyDisparityToHost()
-    do, Xo = O.k_convert_kb(flow, *intr)
-    m = (pair["mask"] > 0) & np.isfinite(do) & (np.abs(do) < 1e4)
-    assert_close("depth", dg[m], do[m], 1e-5, 1e-5)
+    # the ABI takes the intrinsics as float (as stereotgv.h:118-120 does)
+    do, Xo = O.k_convert_kb(flow, *[float(np.float32(v)) for v in intr])
+    m = (pair["mask"] > 0) & np.isfinite(do)
+    assert np.isfinite(dg[m]).all()
+    # depth = |X| with Z = (t1 - x1 t3) / (x1 - x0): pixels where either difference
+    # cancels amplify libm ULP differences without bound, so a tiny outlier fraction
+    # is allowed; everywhere else the FP64 results agree to 1e-5.
+    bad = ~np.isclose(dg[m], do[m], rtol=1e-5, atol=1e-6)
+    assert bad.mean() < 2e-4, (bad.sum(), m.sum())
+    badX = ~np.isclose(Xg[m], Xo[m], rtol=1e-5, atol=1e-6).all(axis=-1)
+    assert badX.mean() < 2e-4, (badX.sum(), m.sum())

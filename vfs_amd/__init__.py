@@ -49,6 +49,7 @@ EXPORTS = [
     ("vfs_set_option", ctypes.c_int, [ctypes.c_void_p, ctypes.c_char_p, ctypes.c_int]),
     ("vfs_time_solves", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, _c_float_p]),
     ("vfs_time_inner_iterations", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, _c_float_p, _c_int_p]),
+    ("vfs_time_copy", ctypes.c_int, [ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int, _c_float_p]),
 ]
 
 
@@ -227,6 +228,11 @@ class StereoTgv:
         self._check(self._L.vfs_time_inner_iterations(self._h, level, reps, ctypes.byref(ms), ctypes.byref(nl)),
                     "time_inner_iterations")
         return ms.value, nl.value
+
+    def time_copy(self, nbytes: int, reps: int) -> float:
+        ms = ctypes.c_float()
+        self._check(self._L.vfs_time_copy(self._h, nbytes, reps, ctypes.byref(ms)), "time_copy")
+        return ms.value
 
     @staticmethod
     def _shape(a: np.ndarray, shape):

@@ -22,6 +22,7 @@ struct IterParams {
     float alpha0;
     float alpha1;
     int first;     // 1: p and q were just zeroed (stereotgv.cu:431-432), read them as 0
+    int write_tp;  // 1: also store Tp (read later only where a neighbour is outside the mask)
 };
 
 void launch_fill_zero(void *ptr, size_t bytes, hipStream_t st);
@@ -53,6 +54,11 @@ void launch_update(float *u, const float *u_last, const float2 *tv2, float limit
                    float2 *warpUV, Dims d, hipStream_t st);
 void launch_upscale3(const float *u, const float *u_, const float2 *uv, Dims coarse, float scale,
                      float *ou, float *ou_, float2 *ouv, Dims fine, hipStream_t st);
+void launch_tgv_iter(int masked, const float *mask, const float *u_, const float2 *v_, const float *a,
+                     const float *b, const float *c, const float *Iu, const float *Iz, const float *u,
+                     const float2 *p_in, const float4 *q_in, float2 *p_out, float4 *q_out, float2 *Tp, float *us,
+                     float *u_s, float2 *v_s, const IterParams &ip, Dims d, hipStream_t st);
+void launch_copy(const void *src, void *dst, size_t bytes, hipStream_t st);
 void launch_convert_kb(const float2 *uv, double focalx, double focaly, double cx, double cy, double d1,
                        double d2, double d3, double d4, double t1, double t2, double t3, float *X,
                        float *depth, Dims d, hipStream_t st);

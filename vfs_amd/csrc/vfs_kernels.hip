@@ -644,6 +644,205 @@ __global__ void k_convert_kb(const float2 *__restrict__ uv, double fx, double fy
     depth[pos] = (float)sqrt(X0 * X0 + X1 * X1 + X2 * X2);
 }
 
+
+// ------------------------------------------------------------ fused iteration
+//
+// One TGV-L1 inner iteration in one launch: dual ascent + projection
+// (updateDualVariables.cu:122-237), Tp (thresholdingL1.cu:234-244), L1 thresholding
+// (thresholdingL1.cu:128-216) and the primal step with over-relaxation
+// (updatePrimalVariables.cu:138-245).  Each workgroup owns a 64x8 tile.  Phase A
+// computes the new p, q and Tp on the tile plus its left column and top row (the
+// threshold and primal of a pixel read Tp and q of the left and upper neighbours) into
+// LDS; phase B finishes thresholding and the primal step of the tile.  Old p, q are
+// read from one buffer and new ones written to the other, so every workgroup sees the
+// previous iteration's values in its recomputed halo (the reference's Jacobi order).
+constexpr int FTX = 64, FTY = 8;
+constexpr int EXW = FTX + 1, EXH = FTY + 1;
+
+template <bool MASKED>
+__global__ __launch_bounds__(256) void k_tgv_iter(
+    const float *__restrict__ mask, const float *__restrict__ u_, const float2 *__restrict__ v_,
+    const float *__restrict__ at, const float *__restrict__ bt, const float *__restrict__ ct,
+    const float *__restrict__ Iu, const float *__restrict__ Iz, const float *__restrict__ u,
+    const float2 *__restrict__ p_in, const float4 *__restrict__ q_in, float2 *__restrict__ p_out,
+    float4 *__restrict__ q_out, float2 *__restrict__ Tp, float *__restrict__ us, float *__restrict__ u_s,
+    float2 *__restrict__ v_s, IterParams ip, int w, int h, int s)
+{
+    __shared__ float2 sP[EXH][EXW];
+    __shared__ float4 sQ[EXH][EXW];
+    __shared__ float2 sT[EXH][EXW];
+    const int x0 = blockIdx.x * FTX, y0 = blockIdx.y * FTY;
+    const int tid = threadIdx.y * BX + threadIdx.x;
+
+    // ---- phase A: dual update on the tile + left column + top row
+    for (int i = tid; i < EXH * EXW; i += BX * BY) {
+        const int ey = i / EXW, ex = i - ey * EXW;
+        const int x = x0 - 1 + ex, y = y0 - 1 + ey;
+        if (x < 0 || y < 0 || x >= w || y >= h) continue;
+        const long pos = (long)y * s + x;
+        const bool interior = ex >= 1 && ey >= 1;
+        const float a = at[pos], b = bt[pos], c = ct[pos];
+        float2 pn;
+        float4 qn;
+        bool hasR, hasD;
+        if (MASKED) {
+            if (mask[pos] == 0.f) continue;
+            hasR = (x + 1 < w) && (mask[pos + 1] != 0.f);
+            hasD = (y + 1 < h) && (mask[pos + s] != 0.f);
+        } else {
+            hasR = x + 1 < w;
+            hasD = y + 1 < h;
+        }
+        if (!MASKED && outside_circle(x, y, w, h)) {
+            pn = make_float2(0.f, 0.f);
+            qn = make_float4(0.f, 0.f, 0.f, 0.f);
+        } else {
+            const float u0 = u_[pos];
+            const float2 v0 = v_[pos];
+            const float ux = hasR ? (u_[pos + 1] - u0 - v0.x) : (u0 - ld0(u_, pos - 1) - v0.x);
+            const float uy = hasD ? (u_[pos + s] - u0 - v0.y) : (u0 - ld0(u_, pos - s) - v0.y);
+            const float dtx = a * ux + c * uy;
+            const float dty = c * ux + b * uy;
+            const float2 po = ip.first ? make_float2(0.f, 0.f) : p_in[pos];
+            const float px = po.x + ip.k1 * dtx;
+            const float py = po.y + ip.k1 * dty;
+            float rp = sqrtf(px * px + py * py);
+            if (rp < 1.0f) rp = 1.0f;
+            pn = make_float2(px / rp, py / rp);
+            float g0, g1, g2, g3;
+            if (hasR) {
+                const float2 vr = v_[pos + 1];
+                g0 = vr.x - v0.x;
+                g3 = vr.y - v0.y;
+            } else {
+                const float2 vl = ld0(v_, pos - 1);
+                g0 = v0.x - vl.x;
+                g3 = v0.y - vl.y;
+            }
+            if (hasD) {
+                const float2 vd = v_[pos + s];
+                g1 = vd.y - v0.y;
+                g2 = vd.x - v0.x;
+            } else {
+                const float2 vu = ld0(v_, pos - s);
+                g1 = v0.y - vu.y;
+                g2 = v0.x - vu.x;
+            }
+            const float4 qo = ip.first ? make_float4(0.f, 0.f, 0.f, 0.f) : q_in[pos];
+            const float q0 = qo.x + ip.k0 * g0, q1 = qo.y + ip.k0 * g1, q2 = qo.z + ip.k0 * g2,
+                        q3 = qo.w + ip.k0 * g3;
+            float rq = sqrtf(q0 * q0 + q1 * q1 + q2 * q2 + q3 * q3);
+            if (rq < 1.0f) rq = 1.0f;
+            qn = make_float4(q0 / rq, q1 / rq, q2 / rq, q3 / rq);
+        }
+        const float2 tpn = make_float2(a * pn.x + c * pn.y, c * pn.x + b * pn.y);
+        sP[ey][ex] = pn;
+        sQ[ey][ex] = qn;
+        sT[ey][ex] = tpn;
+        if (interior) {
+            p_out[pos] = pn;
+            q_out[pos] = qn;
+            if (ip.write_tp) Tp[pos] = tpn;
+        }
+    }
+    __syncthreads();
+
+    // ---- phase B: thresholding + primal step on the tile
+    const int x = x0 + threadIdx.x;
+    if (x >= w) return;
+    for (int ty = threadIdx.y; ty < FTY; ty += BY) {
+        const int y = y0 + ty;
+        if (y >= h) break;
+        const long pos = (long)y * s + x;
+        const int ex = threadIdx.x + 1, ey = ty + 1;
+        bool hasR, hasL, hasD, hasU;
+        if (MASKED) {
+            if (mask[pos] == 0.f) continue;
+            hasR = (x + 1 < w) && (mask[pos + 1] != 0.f);
+            hasL = (x >= 1) && (mask[pos - 1] != 0.f);
+            hasD = (y + 1 < h) && (mask[pos + s] != 0.f);
+            hasU = (y >= 1) && (mask[pos - s] != 0.f);
+        } else {
+            if (outside_circle(x, y, w, h)) {
+                us[pos] = 0.f;
+                u_s[pos] = 0.f;
+                v_s[pos] = make_float2(0.f, 0.f);
+                continue;
+            }
+            hasR = x + 1 < w;
+            hasL = x >= 1;
+            hasD = y + 1 < h;
+            hasU = y >= 1;
+        }
+        const float a = at[pos], b = bt[pos], c = ct[pos];
+        const float aa = ip.alpha1 * ip.alpha1;
+        const float2 tp0 = sT[ey][ex];
+        // a neighbour outside the mask (or the image) is read from the Tp buffer, which
+        // holds what earlier levels left there, as the reference does
+        const float tpLx = hasL ? sT[ey][ex - 1].x : ld0(Tp, pos - 1).x;
+        const float tpUy = hasU ? sT[ey - 1][ex].y : ld0(Tp, pos - s).y;
+        float dx, dy;
+        if (MASKED) {
+            if (hasL && hasR) dx = tp0.x - tpLx;
+            else if (!hasR) dx = -tpLx;
+            else dx = tp0.x;
+            if (hasU && hasD) dy = tp0.y - tpUy;
+            else if (!hasD) dy = -tpUy;
+            else dy = tp0.y;
+        } else {
+            if (hasL) dx = tp0.x - tpLx;
+            else if (x == w - 1) dx = -tpLx;
+            else dx = tp0.x;
+            if (hasU) dy = tp0.y - tpUy;
+            else if (y == h - 1) dy = -tpUy;
+            else dy = tp0.y;
+        }
+        const float etau = (a * a + b * b + 2 * c * c + (a + c) * (a + c) + (b + c) * (b + c)) * aa;
+        const float u_pos = u_[pos];
+        const float un = thresh_core(dx + dy, ip.tau, etau, u_pos, u[pos], Iu[pos], Iz[pos], ip.lambda);
+        us[pos] = un;
+        const float4 qc4 = sQ[ey][ex];
+        float qcx = 0.f, qcw = 0.f, qcy = 0.f, qcz = 0.f;
+        if (hasR) {
+            qcx = qc4.x;
+            qcw = qc4.w;
+        }
+        if (hasD) {
+            qcy = qc4.y;
+            qcz = qc4.z;
+        }
+        float qwx = 0.f, qww = 0.f, qny = 0.f, qnz = 0.f;
+        if (hasL) {
+            const float4 ql = sQ[ey][ex - 1];
+            qwx = ql.x;
+            qww = ql.w;
+        }
+        if (hasU) {
+            const float4 qu = sQ[ey - 1][ex];
+            qny = qu.y;
+            qnz = qu.z;
+        }
+        const float dqx = (qcx - qwx) + (qcz - qnz);
+        const float dqy = (qcw - qww) + (qcy - qny);
+        const float2 pp = sP[ey][ex];
+        const float tx = a * pp.x + c * pp.y;
+        const float tyy = c * pp.x + b * pp.y;
+        const float e1 = aa * (b * b + c * c) + 4 * ip.alpha0 * ip.alpha0;
+        const float e2 = aa * (a * a + c * c) + 4 * ip.alpha0 * ip.alpha0;
+        const float2 vo = v_[pos];
+        float vx, vy;
+        vx = vo.x + ((e1 != 0.f) ? ip.tau / e1 : ip.tau) * (ip.alpha1 * tx + ip.alpha0 * dqx);
+        if (e2 != 0.f) {
+            vy = vo.y + (ip.tau / e2) * (ip.alpha1 * tyy + ip.alpha0 * dqy);
+        } else {
+            vx = vo.x + ip.tau * (ip.alpha1 * tx + ip.alpha0 * dqx);
+            vy = 0.f;
+        }
+        u_s[pos] = un + ip.mu * (un - u_pos);
+        v_s[pos] = make_float2(vx + ip.mu * (vx - vo.x), vy + ip.mu * (vy - vo.y));
+    }
+}
+
 }  // namespace
 
 // ------------------------------------------------------------------ launchers
@@ -766,6 +965,36 @@ void launch_convert_kb(const float2 *uv, double focalx, double focaly, double cx
 {
     k_convert_kb<<<grid_for(d.w, d.h), block2d(), 0, st>>>(uv, focalx, focaly, cx, cy, d1, d2, d3, d4, t1, t2, t3,
                                                           X, depth, d.w, d.h, d.s);
+}
+
+
+void launch_tgv_iter(int masked, const float *mask, const float *u_, const float2 *v_, const float *a,
+                     const float *b, const float *c, const float *Iu, const float *Iz, const float *u,
+                     const float2 *p_in, const float4 *q_in, float2 *p_out, float4 *q_out, float2 *Tp, float *us,
+                     float *u_s, float2 *v_s, const IterParams &ip, Dims d, hipStream_t st)
+{
+    const dim3 grid((d.w + FTX - 1) / FTX, (d.h + FTY - 1) / FTY);
+    if (masked)
+        k_tgv_iter<true><<<grid, block2d(), 0, st>>>(mask, u_, v_, a, b, c, Iu, Iz, u, p_in, q_in, p_out, q_out, Tp,
+                                                    us, u_s, v_s, ip, d.w, d.h, d.s);
+    else
+        k_tgv_iter<false><<<grid, block2d(), 0, st>>>(mask, u_, v_, a, b, c, Iu, Iz, u, p_in, q_in, p_out, q_out,
+                                                     Tp, us, u_s, v_s, ip, d.w, d.h, d.s);
+}
+
+__global__ void k_copy(const float4 *__restrict__ src, float4 *__restrict__ dst, size_t n4)
+{
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const size_t stride = (size_t)gridDim.x * blockDim.x;
+    for (; i < n4; i += stride) dst[i] = src[i];
+}
+
+void launch_copy(const void *src, void *dst, size_t bytes, hipStream_t st)
+{
+    const size_t n4 = bytes / 16;
+    size_t blocks = (n4 + 255) / 256;
+    if (blocks > 16384) blocks = 16384;
+    k_copy<<<dim3((unsigned)blocks), dim3(256), 0, st>>>((const float4 *)src, (float4 *)dst, n4);
 }
 
 }  // namespace vfs

@@ -64,19 +64,26 @@ struct vfs_solver {
     size_t arena_bytes = 0;
     float *bufU[2] = {}, *bufU_[2] = {}, *u_last = nullptr, *i1warp = nullptr, *Iu = nullptr, *Iz = nullptr;
     float *a = nullptr, *b = nullptr, *c = nullptr;
-    float2 *bufV_[2] = {}, *p = nullptr, *Tp = nullptr, *tv2 = nullptr, *bufUV[2] = {};
-    float4 *q = nullptr;
+    float2 *bufV_[2] = {}, *bufP[2] = {}, *Tp = nullptr, *tv2 = nullptr, *bufUV[2] = {};
+    float4 *bufQ[2] = {};
 
     // roles after the last issued solve
     float *out_u = nullptr, *out_u_ = nullptr;
-    float2 *out_uv = nullptr, *out_v_ = nullptr;
+    float2 *out_uv = nullptr, *out_v_ = nullptr, *out_p = nullptr;
+    float4 *out_q = nullptr;
+    bool fused = true;  // one launch per inner iteration (else dual+Tp, threshold+primal)
 
     // graph replay
     bool use_graph = true;
     hipGraphExec_t graph[2] = {nullptr, nullptr};
     float graph_limit[2] = {0, 0};
-    float *graph_out_u[2] = {}, *graph_out_u_[2] = {};
-    float2 *graph_out_uv[2] = {}, *graph_out_v_[2] = {};
+    struct Roles {
+        float *u, *u_;
+        float2 *uv, *v_, *p;
+        float4 *q;
+    };
+    Roles graph_roles[2] = {};
+    bool graph_fused[2] = {true, true};
 
     std::vector<void *> allocs;
 };
@@ -161,8 +168,7 @@ int check_launch(vfs_solver *s)
 
 // Issues the whole solve (stereotgv.cu:389-515 when masked, :256-387 otherwise) on the
 // solver's stream, starting from the canonical buffer roles.  Records the final roles.
-int issue_solve(vfs_solver *s, int masked, float *fin_u[1], float *fin_u_[1], float2 *fin_uv[1],
-                float2 *fin_v_[1])
+int issue_solve(vfs_solver *s, int masked, vfs_solver::Roles *fin)
 {
     hipStream_t st = s->stream;
     const Dims d0 = s->lv[0];
@@ -178,6 +184,7 @@ int issue_solve(vfs_solver *s, int masked, float *fin_u[1], float *fin_u_[1], fl
     float *u_ = s->bufU_[0], *u_s = s->bufU_[1];
     float2 *v_ = s->bufV_[0], *v_s = s->bufV_[1];
     float2 *uv = s->bufUV[0], *uvs = s->bufUV[1];
+    int pq = 0;  // which p / q buffer holds the current duals
 
     for (int l = s->nLevels - 1; l >= 0; l--) {
         const Dims d = s->lv[l];
@@ -204,9 +211,18 @@ int issue_solve(vfs_solver *s, int masked, float *fin_u[1], float *fin_u_[1], fl
                 ip.alpha0 = s->alpha0;
                 ip.alpha1 = s->alpha1;
                 ip.first = (it == 0);
-                vfs::launch_dual_tp(masked, mask, u_, v_, s->a, s->b, s->c, ip, s->p, s->q, s->Tp, d, st);
-                vfs::launch_thresh_primal(masked, mask, s->Tp, u_, v_, s->Iu, s->Iz, s->a, s->b, s->c, s->p, s->q, u,
-                                          ip, us, u_s, v_s, d, st);
+                ip.write_tp = (it == s->nSolverIters - 1);
+                if (s->fused) {
+                    vfs::launch_tgv_iter(masked, mask, u_, v_, s->a, s->b, s->c, s->Iu, s->Iz, u, s->bufP[pq],
+                                         s->bufQ[pq], s->bufP[pq ^ 1], s->bufQ[pq ^ 1], s->Tp, us, u_s, v_s, ip, d,
+                                         st);
+                    pq ^= 1;
+                } else {
+                    vfs::launch_dual_tp(masked, mask, u_, v_, s->a, s->b, s->c, ip, s->bufP[pq], s->bufQ[pq], s->Tp,
+                                        d, st);
+                    vfs::launch_thresh_primal(masked, mask, s->Tp, u_, v_, s->Iu, s->Iz, s->a, s->b, s->c,
+                                              s->bufP[pq], s->bufQ[pq], u, ip, us, u_s, v_s, d, st);
+                }
                 std::swap(u, us);
                 std::swap(u_, u_s);
                 std::swap(v_, v_s);
@@ -227,10 +243,12 @@ int issue_solve(vfs_solver *s, int masked, float *fin_u[1], float *fin_u_[1], fl
             std::swap(uv, uvs);
         }
     }
-    fin_u[0] = u;
-    fin_u_[0] = u_;
-    fin_uv[0] = uv;
-    fin_v_[0] = v_;
+    fin->u = u;
+    fin->u_ = u_;
+    fin->uv = uv;
+    fin->v_ = v_;
+    fin->p = s->bufP[pq];
+    fin->q = s->bufQ[pq];
     return check_launch(s);
 }
 
@@ -251,25 +269,18 @@ int solve_common(vfs_solver *s, int masked)
         HIPCHK(s, hipMemcpyAsync(s->in_i1, s->pI1[0], sizeof(float) * (size_t)d0.s * d0.h,
                                  hipMemcpyDeviceToDevice, s->stream));
     }
+    vfs_solver::Roles roles{};
     if (!s->use_graph) {
-        float *fu, *fu_;
-        float2 *fuv, *fv_;
-        rc = issue_solve(s, masked, &fu, &fu_, &fuv, &fv_);
+        rc = issue_solve(s, masked, &roles);
         if (rc) return rc;
-        s->out_u = fu;
-        s->out_u_ = fu_;
-        s->out_uv = fuv;
-        s->out_v_ = fv_;
     } else {
-        if (s->graph[m] && s->graph_limit[m] != s->limitRange) {
+        if (s->graph[m] && (s->graph_limit[m] != s->limitRange || s->graph_fused[m] != s->fused)) {
             (void)hipGraphExecDestroy(s->graph[m]);
             s->graph[m] = nullptr;
         }
         if (!s->graph[m]) {
             HIPCHK(s, hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal));
-            float *fu, *fu_;
-            float2 *fuv, *fv_;
-            rc = issue_solve(s, masked, &fu, &fu_, &fuv, &fv_);
+            rc = issue_solve(s, masked, &roles);
             hipGraph_t g = nullptr;
             hipError_t e = hipStreamEndCapture(s->stream, &g);
             if (rc) {
@@ -284,17 +295,18 @@ int solve_common(vfs_solver *s, int masked)
                 return fail(s, VFS_ERR_DEVICE, "hipGraphInstantiate: %s", hipGetErrorString(e));
             }
             s->graph_limit[m] = s->limitRange;
-            s->graph_out_u[m] = fu;
-            s->graph_out_u_[m] = fu_;
-            s->graph_out_uv[m] = fuv;
-            s->graph_out_v_[m] = fv_;
+            s->graph_fused[m] = s->fused;
+            s->graph_roles[m] = roles;
         }
         HIPCHK(s, hipGraphLaunch(s->graph[m], s->stream));
-        s->out_u = s->graph_out_u[m];
-        s->out_u_ = s->graph_out_u_[m];
-        s->out_uv = s->graph_out_uv[m];
-        s->out_v_ = s->graph_out_v_[m];
+        roles = s->graph_roles[m];
     }
+    s->out_u = roles.u;
+    s->out_u_ = roles.u_;
+    s->out_uv = roles.uv;
+    s->out_v_ = roles.v_;
+    s->out_p = roles.p;
+    s->out_q = roles.q;
     s->raw_i1_fresh = false;
     return VFS_OK;
 }
@@ -421,10 +433,10 @@ int vfs_initialize(vfs_solver *s, int width, int height, float beta, float gamma
     if ((rc = dev_alloc(s, (void **)&s->cv, N * sizeof(float2)))) return rc;
     if ((rc = dev_alloc(s, (void **)&s->outX, 3 * N * sizeof(float)))) return rc;
     if ((rc = dev_alloc(s, (void **)&s->outDepth, N * sizeof(float)))) return rc;
-    // work arena: 31 floats per level-0 pixel, each plane 256 B aligned
+    // work arena: 37 floats per level-0 pixel, each plane 256 B aligned
     const size_t plane = (N * sizeof(float) + 255) & ~(size_t)255;
-    const int nplanes = 5 /*u,us,u_,u_s,u_last*/ + 4 /*i1warp,Iu,Iz,.*/ + 3 /*a,b,c*/ + 4 /*v_,v_s*/ + 2 /*p*/ +
-                        2 /*Tp*/ + 2 /*tv2*/ + 4 /*uv,uvs*/ + 4 /*q*/;
+    const int nplanes = 5 /*u,us,u_,u_s,u_last*/ + 4 /*i1warp,Iu,Iz,.*/ + 3 /*a,b,c*/ + 4 /*v_,v_s*/ + 4 /*p x2*/ +
+                        2 /*Tp*/ + 2 /*tv2*/ + 4 /*uv,uvs*/ + 8 /*q x2*/;
     s->arena_bytes = plane * nplanes;
     if ((rc = dev_alloc(s, &s->arena, s->arena_bytes))) return rc;
     char *ar = (char *)s->arena;
@@ -447,16 +459,20 @@ int vfs_initialize(vfs_solver *s, int width, int height, float beta, float gamma
     s->c = (float *)take(1);
     s->bufV_[0] = (float2 *)take(2);
     s->bufV_[1] = (float2 *)take(2);
-    s->p = (float2 *)take(2);
+    s->bufP[0] = (float2 *)take(2);
+    s->bufP[1] = (float2 *)take(2);
     s->Tp = (float2 *)take(2);
     s->tv2 = (float2 *)take(2);
     s->bufUV[0] = (float2 *)take(2);
     s->bufUV[1] = (float2 *)take(2);
-    s->q = (float4 *)take(4);
+    s->bufQ[0] = (float4 *)take(4);
+    s->bufQ[1] = (float4 *)take(4);
     s->out_u = s->bufU[0];
     s->out_u_ = s->bufU_[0];
     s->out_uv = s->bufUV[0];
     s->out_v_ = s->bufV_[0];
+    s->out_p = s->bufP[0];
+    s->out_q = s->bufQ[0];
     HIPCHK(s, hipStreamSynchronize(s->stream));
     s->initialized = true;
     return VFS_OK;
@@ -638,10 +654,10 @@ int vfs_read_plane(vfs_solver *s, const char *name, int level, float *out)
     else if (n == "warpUV") { src = s->out_uv; nch = 2; }
     else if (n == "tv2") { src = s->tv2; nch = 2; }
     else if (n == "tv") { src = s->pTv[level]; nch = 2; }
-    else if (n == "p") { src = s->p; nch = 2; }
+    else if (n == "p") { src = s->out_p; nch = 2; }
     else if (n == "v_") { src = s->out_v_; nch = 2; }
     else if (n == "Tp") { src = s->Tp; nch = 2; }
-    else if (n == "q") { src = s->q; nch = 4; }
+    else if (n == "q") { src = s->out_q; nch = 4; }
     if (!src) return fail(s, VFS_ERR_ARG, "unknown plane '%s'", name);
     int rc = set_device(s);
     if (rc) return rc;
@@ -663,6 +679,10 @@ int vfs_set_option(vfs_solver *s, const char *key, int value)
     if (!s || !key) return VFS_ERR_ARG;
     if (!strcmp(key, "graph")) {
         s->use_graph = value != 0;
+        return VFS_OK;
+    }
+    if (!strcmp(key, "fused")) {
+        s->fused = value != 0;
         return VFS_OK;
     }
     return fail(s, VFS_ERR_ARG, "unknown option '%s'", key);
@@ -707,10 +727,15 @@ int vfs_time_inner_iterations(vfs_solver *s, int level, int reps, float *ms_tota
     for (int it = 0; it < reps; it++) {
         const float mu = 1.0f / sqrtf(1 + 0.7f * tau * s->timestep_lambda);
         vfs::IterParams ip{s->alpha1 * sigma / 3.0f, s->alpha0 * sigma / 2.0f, tau, mu, s->lambda, s->alpha0,
-                           s->alpha1, 0};
-        vfs::launch_dual_tp(1, mask, u_, v_, s->a, s->b, s->c, ip, s->p, s->q, s->Tp, d, s->stream);
-        vfs::launch_thresh_primal(1, mask, s->Tp, u_, v_, s->Iu, s->Iz, s->a, s->b, s->c, s->p, s->q, u, ip, us, u_s,
-                                  v_s, d, s->stream);
+                           s->alpha1, 0, 0};
+        if (s->fused) {
+            vfs::launch_tgv_iter(1, mask, u_, v_, s->a, s->b, s->c, s->Iu, s->Iz, u, s->bufP[it & 1], s->bufQ[it & 1],
+                                 s->bufP[(it & 1) ^ 1], s->bufQ[(it & 1) ^ 1], s->Tp, us, u_s, v_s, ip, d, s->stream);
+        } else {
+            vfs::launch_dual_tp(1, mask, u_, v_, s->a, s->b, s->c, ip, s->bufP[0], s->bufQ[0], s->Tp, d, s->stream);
+            vfs::launch_thresh_primal(1, mask, s->Tp, u_, v_, s->Iu, s->Iz, s->a, s->b, s->c, s->bufP[0], s->bufQ[0],
+                                      u, ip, us, u_s, v_s, d, s->stream);
+        }
         std::swap(u, us);
         std::swap(u_, u_s);
         std::swap(v_, v_s);
@@ -730,7 +755,38 @@ int vfs_time_inner_iterations(vfs_solver *s, int level, int reps, float *ms_tota
     (void)hipEventDestroy(e1);
     if ((rc = check_launch(s))) return rc;
     *ms_total = ms;
-    if (launches_per_iter) *launches_per_iter = 2;
+    if (launches_per_iter) *launches_per_iter = s->fused ? 1 : 2;
+    return VFS_OK;
+}
+
+int vfs_time_copy(vfs_solver *s, size_t bytes, int reps, float *ms_total)
+{
+    if (!s || !ms_total || reps < 1 || bytes < 16) return VFS_ERR_ARG;
+    int rc = set_device(s);
+    if (rc) return rc;
+    void *src = nullptr, *dst = nullptr;
+    HIPCHK(s, hipMalloc(&src, bytes));
+    hipError_t e = hipMalloc(&dst, bytes);
+    if (e != hipSuccess) {
+        (void)hipFree(src);
+        return fail(s, VFS_ERR_NOMEM, "hipMalloc(%zu): %s", bytes, hipGetErrorString(e));
+    }
+    vfs::launch_fill_zero(src, bytes, s->stream);
+    hipEvent_t e0, e1;
+    HIPCHK(s, hipEventCreate(&e0));
+    HIPCHK(s, hipEventCreate(&e1));
+    HIPCHK(s, hipEventRecord(e0, s->stream));
+    for (int i = 0; i < reps; i++) vfs::launch_copy(src, dst, bytes, s->stream);
+    HIPCHK(s, hipEventRecord(e1, s->stream));
+    HIPCHK(s, hipEventSynchronize(e1));
+    float ms = 0;
+    HIPCHK(s, hipEventElapsedTime(&ms, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    (void)hipFree(src);
+    (void)hipFree(dst);
+    if ((rc = check_launch(s))) return rc;
+    *ms_total = ms;
     return VFS_OK;
 }
 
