@@ -119,10 +119,14 @@ int vfs_read_plane(vfs_solver *s, const char *name, int level, float *out);
 /* Blocks until all work queued on the solver's stream has finished. */
 int vfs_synchronize(vfs_solver *s);
 
-/* Execution options: "graph" (1 = replay the solve as a captured HIP graph, default;
- * 0 = launch kernels one by one); "fused" (1 = one launch per inner TGV iteration,
- * default; 0 = two launches, dual+Tp then threshold+primal). */
+/* Execution options:
+ *   "graph"   1 = replay the solve as a captured HIP graph (default), 0 = launch
+ *             kernels one by one;
+ *   "storage" 0 = FP32 (default), 1 = FP16, 2 = BF16 storage of the image pyramid,
+ *             warped image, derivatives, tensor and duals p, q; arithmetic stays FP32.
+ * No counterpart in the reference (FP32 only, stereotgv.h:47-100). */
 int vfs_set_option(vfs_solver *s, const char *key, int value);
+int vfs_get_option(const vfs_solver *s, const char *key, int *value);
 
 /* Timing with HIP events on the solver's stream: runs `n` solves (masked != 0:
  * solveStereoForwardMasked) back to back on resident inputs and returns their total
@@ -131,12 +135,29 @@ int vfs_time_solves(vfs_solver *s, int masked, int n, float *ms_total);
 
 /* Timing of the inner TGV iteration at `level` (the dominant work): `reps` iterations
  * launched back to back on the current solver state, HIP events around them.
- * Returns the total time in ms and the number of kernel launches per iteration. */
+ * Returns the total time in ms and the number of kernel launches per iteration (1).
+ * The solver state is left modified; the next solve starts fresh as always. */
 int vfs_time_inner_iterations(vfs_solver *s, int level, int reps, float *ms_total, int *launches_per_iter);
 
 /* Timing of a plain device-to-device copy kernel moving `bytes` (read + write), `reps`
  * times: the achievable-bandwidth reference for the solver's kernels. */
 int vfs_time_copy(vfs_solver *s, size_t bytes, int reps, float *ms_total);
+
+/* ---- batches: no counterpart in the reference, which solves one pair per call ----
+ * A batch groups initialized solvers (each with its own mask, fields, images and
+ * buffers; resolutions may differ) that share nLevels, nWarpIters, nSolverIters, the
+ * solver parameters, limitRange and the storage option.  vfs_batch_solve runs
+ * solveStereoForwardMasked (masked != 0) or solveStereoForward for all of them with
+ * every kernel launch covering all pairs; afterwards each member's copy-out calls
+ * return its own results.  The members must not be solved concurrently elsewhere. */
+typedef struct vfs_batch vfs_batch;
+int vfs_batch_create(vfs_batch **out, vfs_solver *const *solvers, int n);
+int vfs_batch_destroy(vfs_batch *b);
+int vfs_batch_solve(vfs_batch *b, int masked);
+int vfs_batch_synchronize(vfs_batch *b);
+/* `n` batch solves back to back, HIP events around them; total ms. */
+int vfs_batch_time_solves(vfs_batch *b, int masked, int n, float *ms_total);
+const char *vfs_batch_last_error(const vfs_batch *b);
 
 #ifdef __cplusplus
 }

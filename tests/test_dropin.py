@@ -86,9 +86,11 @@ def test_dropin_matches_python_mirror_and_oracle(tmp_path):
     g.solveStereoForwardMasked()
     assert np.array_equal(u, g.copy1DDisparityToHost())
     assert np.array_equal(flow, g.copyDisparityToHost())
-    s = 848.0 / W
-    dg, _ = g.copyStereoToHost(285.722 / s, 286.759 / s, 420.135 / s, 403.394 / s, -0.00659769, 0.0473251,
-                               -0.0458264, 0.00897725, -0.0641854, -0.000218299, 0.000111253)
+    # the consumer scales the intrinsics in float, as test_vfs/main.cpp:208-211 does
+    f32 = np.float32
+    s = f32(848.0) / f32(W)
+    dg, _ = g.copyStereoToHost(f32(285.722) / s, f32(286.759) / s, f32(420.135) / s, f32(403.394) / s, -0.00659769,
+                               0.0473251, -0.0458264, 0.00897725, -0.0641854, -0.000218299, 0.000111253)
     assert np.array_equal(depth, dg, equal_nan=True)
 
     o = O.Oracle(W, H, P["beta"], P["gamma"], P["alpha0"], P["alpha1"], P["timestep_lambda"], P["lam"], L, 2.0, WI,

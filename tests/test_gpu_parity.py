@@ -17,10 +17,27 @@ pytestmark = pytest.mark.gpu
 P = S.SOLVER_PARAMS
 
 
-def run_gpu(pair, levels, warps, iters, masked=True, limit=None, graph=True, u8=False, fused=True):
+def make_gpu(pair, levels, warps, iters, limit=None, graph=True, u8=False, storage="f32"):
     H, W = pair["i0"].shape
     g = StereoTgv()
-    g.set_option("fused", 1 if fused else 0)
+    g.initialize(W, H, P["beta"], P["gamma"], P["alpha0"], P["alpha1"], P["timestep_lambda"], P["lam"], levels,
+                 P["fScale"], warps, iters)
+    g.limitRange = P["limitRange"] if limit is None else limit
+    g.set_option("graph", 1 if graph else 0)
+    g.set_option("storage", storage)
+    g.copyMaskToDevice(pair["mask"])
+    g.loadVectorFields(pair["tv"], pair["cv"])
+    if u8:
+        g.copyImagesToDevice(pair["i0u8"], pair["i1u8"])
+    else:
+        g.copyImagesToDevice(pair["i0"], pair["i1"])
+    return g
+
+
+def run_gpu(pair, levels, warps, iters, masked=True, limit=None, graph=True, u8=False, storage="f32"):
+    H, W = pair["i0"].shape
+    g = StereoTgv()
+    g.set_option("storage", storage)
     g.initialize(W, H, P["beta"], P["gamma"], P["alpha0"], P["alpha1"], P["timestep_lambda"], P["lam"], levels,
                  P["fScale"], warps, iters)
     g.limitRange = P["limitRange"] if limit is None else limit
@@ -83,12 +100,11 @@ def test_pyramids_and_fields():
         assert np.array_equal(g.read_plane("mask", lvl), o.read_plane("mask", lvl)), f"mask@{lvl}"
 
 
-@pytest.mark.parametrize("fused", [True, False])
 @pytest.mark.parametrize("masked", [True, False])
-def test_one_iteration_planes(masked, fused):
+def test_one_iteration_planes(masked):
     """L=1, W=1, I=1: every intermediate plane of one warp and one TGV iteration."""
     pair = small_pair()
-    g = run_gpu(pair, 1, 1, 1, masked=masked, fused=fused)
+    g = run_gpu(pair, 1, 1, 1, masked=masked)
     o = run_oracle(pair, 1, 1, 1, masked=masked)
     for name in ("a", "b", "c", "i1warp", "tv2", "Iu", "Iz", "p", "q", "Tp", "u_last"):
         ref = o.read_plane(name, 0)
@@ -114,15 +130,21 @@ def test_small_solve(masked):
 
 
 @pytest.mark.parametrize("masked", [True, False])
-def test_fused_iteration_matches_two_launch_form(masked):
-    pair = small_pair(160, 120, seed=17)
-    a = run_gpu(pair, 3, 3, 12, masked=masked, fused=True)
-    b = run_gpu(pair, 3, 3, 12, masked=masked, fused=False)
-    m = pair["mask"] > 0
-    d = np.abs(a.copy1DDisparityToHost() - b.copy1DDisparityToHost())[m]
-    assert d.mean() <= 1e-5, d.mean()
-    for name in ("p", "q", "v_", "u_"):
-        assert_close(name, a.read_plane(name, 0)[m], b.read_plane(name, 0)[m], 1e-3, 1e-4)
+def test_two_iterations_planes(masked):
+    """L=1, W=1, I=2: the second iteration reads the first one's duals from the other
+    buffer and the recomputed halo must match the oracle everywhere."""
+    pair = small_pair(150, 90, seed=8)
+    g = run_gpu(pair, 1, 1, 2, masked=masked)
+    o = run_oracle(pair, 1, 1, 2, masked=masked)
+    m = pair["mask"] > 0 if masked else np.ones(pair["mask"].shape, bool)
+    for name in ("p", "q"):
+        assert_close(name, g.read_plane(name, 0)[m], o.read_plane(name, 0)[m], 1e-4, 1e-5)
+    # the threshold divides by Iu, so FP32 rounding of the first iteration is amplified
+    # where Iu is small: almost every pixel agrees to 1e-5, all of them to 2e-4
+    for name in ("u", "u_", "v_"):
+        got, ref = g.read_plane(name, 0), o.read_plane(name, 0)
+        assert_close(name, got, ref, 1e-3, 2e-4)
+        assert np.isclose(got, ref, rtol=1e-4, atol=1e-5).mean() > 0.998, name
 
 
 def test_graph_equals_direct_launch():
@@ -204,3 +226,37 @@ def test_parity_848x800():
     assert bad.mean() < 2e-4, (bad.sum(), m.sum())
     badX = ~np.isclose(Xg[m], Xo[m], rtol=1e-5, atol=1e-6).all(axis=-1)
     assert badX.mean() < 2e-4, (badX.sum(), m.sum())
+
+
+@pytest.mark.parametrize("storage", ["f16", "bf16"])
+def test_reduced_storage_424x400(storage):
+    """FP16 / BF16 storage of images, derivatives, tensor and duals (FP32 arithmetic):
+    mean disparity error against the FP64 oracle within the 0.05 px bar."""
+    wl = S.WORKLOADS["tgv_424x400"]
+    pair = S.make_pair(wl["W"], wl["H"], wl["seed"], wl["radius"], wl["dmax"])
+    g = run_gpu(pair, wl["levels"], wl["warps"], wl["iters"], storage=storage)
+    o = run_oracle(pair, wl["levels"], wl["warps"], wl["iters"])
+    m = pair["mask"] > 0
+    err = np.abs(g.copy1DDisparityToHost() - o.copy1DDisparityToHost())[m]
+    print(f"{storage} 424x400: mean |du| {err.mean():.3e} px, max {err.max():.3f} px")
+    assert err.mean() <= 0.05, err.mean()
+
+
+def test_batch_equals_alone_mixed_sizes():
+    """Pairs of different resolutions solved in one batch give bit-identical results to
+    the same pairs solved alone (no cross-pair interaction, deterministic kernels)."""
+    from vfs_amd import Batch
+    sizes = [(128, 96, 41), (192, 160, 42), (96, 80, 43)]
+    pairs = [S.make_pair(w, h, seed, radius=h / 2, dmax=w / 21.2) for w, h, seed in sizes]
+    alone = [run_gpu(p, 3, 3, 10).copy1DDisparityToHost() for p in pairs]
+    solvers = [make_gpu(p, 3, 3, 10) for p in pairs]
+    b = Batch(solvers)
+    b.solve(True)
+    for k, s in enumerate(solvers):
+        assert np.array_equal(s.copy1DDisparityToHost(), alone[k]), k
+    # a second batch solve on freshly copied images is identical again
+    for s, p in zip(solvers, pairs):
+        s.copyImagesToDevice(p["i0"], p["i1"])
+    b.solve(True)
+    for k, s in enumerate(solvers):
+        assert np.array_equal(s.copy1DDisparityToHost(), alone[k]), k

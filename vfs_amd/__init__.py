@@ -12,7 +12,7 @@ import os
 
 import numpy as np
 
-__all__ = ["StereoTgv", "VfsError", "lib", "lib_path", "EXPORTS"]
+__all__ = ["StereoTgv", "Batch", "VfsError", "lib", "lib_path", "EXPORTS", "STORAGE"]
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 _LIB = None
@@ -50,7 +50,16 @@ EXPORTS = [
     ("vfs_time_solves", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, _c_float_p]),
     ("vfs_time_inner_iterations", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, _c_float_p, _c_int_p]),
     ("vfs_time_copy", ctypes.c_int, [ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int, _c_float_p]),
+    ("vfs_get_option", ctypes.c_int, [ctypes.c_void_p, ctypes.c_char_p, _c_int_p]),
+    ("vfs_batch_create", ctypes.c_int, [ctypes.POINTER(ctypes.c_void_p), ctypes.POINTER(ctypes.c_void_p), ctypes.c_int]),
+    ("vfs_batch_destroy", ctypes.c_int, [ctypes.c_void_p]),
+    ("vfs_batch_solve", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
+    ("vfs_batch_synchronize", ctypes.c_int, [ctypes.c_void_p]),
+    ("vfs_batch_time_solves", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, _c_float_p]),
+    ("vfs_batch_last_error", ctypes.c_char_p, [ctypes.c_void_p]),
 ]
+
+STORAGE = {"f32": 0, "fp32": 0, "f16": 1, "fp16": 1, "bf16": 2}
 
 
 class VfsError(RuntimeError):
@@ -214,8 +223,15 @@ class StereoTgv:
     def synchronize(self):
         return self._check(self._L.vfs_synchronize(self._h), "synchronize")
 
-    def set_option(self, key: str, value: int):
+    def set_option(self, key: str, value):
+        if key == "storage" and isinstance(value, str):
+            value = STORAGE[value.lower()]
         return self._check(self._L.vfs_set_option(self._h, key.encode(), int(value)), "set_option")
+
+    def get_option(self, key: str) -> int:
+        v = ctypes.c_int()
+        self._check(self._L.vfs_get_option(self._h, key.encode(), ctypes.byref(v)), "get_option")
+        return v.value
 
     def time_solves(self, masked: bool = True, n: int = 1) -> float:
         ms = ctypes.c_float()
@@ -238,3 +254,44 @@ class StereoTgv:
     def _shape(a: np.ndarray, shape):
         if a.shape != tuple(shape):
             raise VfsError(f"expected array of shape {tuple(shape)}, got {a.shape}")
+
+
+class Batch:
+    """Several StereoTgv solvers solved together: every kernel launch covers all pairs."""
+
+    def __init__(self, solvers):
+        self._L = lib()
+        self.solvers = list(solvers)
+        arr = (ctypes.c_void_p * len(self.solvers))(*[s._h.value for s in self.solvers])
+        h = ctypes.c_void_p()
+        rc = self._L.vfs_batch_create(ctypes.byref(h), arr, len(self.solvers))
+        if rc != 0:
+            raise VfsError(f"vfs_batch_create failed ({rc}): {self._L.vfs_batch_last_error(None).decode()}")
+        self._h = h
+
+    def _check(self, rc, what):
+        if rc < 0:
+            raise VfsError(f"{what} failed ({rc}): {self._L.vfs_batch_last_error(self._h).decode()}")
+        return rc
+
+    def solve(self, masked: bool = True):
+        return self._check(self._L.vfs_batch_solve(self._h, 1 if masked else 0), "batch_solve")
+
+    def synchronize(self):
+        return self._check(self._L.vfs_batch_synchronize(self._h), "batch_synchronize")
+
+    def time_solves(self, masked: bool = True, n: int = 1) -> float:
+        ms = ctypes.c_float()
+        self._check(self._L.vfs_batch_time_solves(self._h, 1 if masked else 0, n, ctypes.byref(ms)), "batch_time")
+        return ms.value
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self._L.vfs_batch_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

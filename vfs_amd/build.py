@@ -20,7 +20,7 @@ HIPCC = os.environ.get("HIPCC", shutil.which("hipcc") or "/opt/rocm/bin/hipcc")
 HIP_FLAGS = ["--offload-arch=" + ARCH, "-O3", "-std=c++17", "-fPIC", "-Wall", "-Wno-unused-result",
              "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
 
-LIB_SOURCES = ["vfs_kernels.hip", "vfs_solver.hip"]
+LIB_SOURCES = ["kernels.hip", "solver.hip"]
 
 
 def _run(cmd, verbose):

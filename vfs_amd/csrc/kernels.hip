@@ -1,0 +1,1111 @@
+// gfx950 kernels of the StereoTgv TGV-L1 fisheye-stereo solver.
+//
+// Every per-pixel kernel of the solve works on a *set of frame pairs* at one pyramid
+// level (see vfs_engine.h): workgroups of 256 work-items (4 wave64) own 64x8 tiles,
+// each wave covering contiguous 64-pixel row segments, and a per-level prefix table maps
+// the flat workgroup index to (pair, tile).  Pairs of different resolutions therefore
+// share one launch.
+//
+// Texture fetches of the reference (linear filter, mirror addressing, normalized
+// coordinates) are evaluated in registers: exact bilinear weights at pixel-index
+// position p = x_norm * dim - 0.5 over the symmetric extension of the image.
+//
+// Storage type T (float, __half, __hip_bfloat16) applies to the image pyramid, the
+// warped image, the derivatives, the tensor and the duals p, q; all arithmetic is FP32.
+#include "vfs_engine.h"
+
+#include <math.h>
+
+namespace vfs {
+
+namespace {
+
+constexpr int BX = 64;
+constexpr int BY = 4;
+constexpr int NT = BX * BY;
+
+// ------------------------------------------------------------------ storage
+
+template <typename T>
+struct Cv;
+template <>
+struct Cv<float> {
+    static __device__ __forceinline__ float f(float v) { return v; }
+    static __device__ __forceinline__ float t(float v) { return v; }
+};
+template <>
+struct Cv<__half> {
+    static __device__ __forceinline__ float f(__half v) { return __half2float(v); }
+    static __device__ __forceinline__ __half t(float v) { return __float2half(v); }
+};
+template <>
+struct Cv<__hip_bfloat16> {
+    static __device__ __forceinline__ float f(__hip_bfloat16 v) { return __bfloat162float(v); }
+    static __device__ __forceinline__ __hip_bfloat16 t(float v) { return __float2bfloat16(v); }
+};
+
+template <typename T>
+struct alignas(2 * sizeof(T)) V2 {
+    T x, y;
+};
+template <typename T>
+struct alignas(4 * sizeof(T)) V4 {
+    T x, y, z, w;
+};
+
+template <typename T>
+__device__ __forceinline__ float ldT(const T *__restrict__ p, long i)
+{
+    return Cv<T>::f(p[i]);
+}
+template <typename T>
+__device__ __forceinline__ void stT(T *__restrict__ p, long i, float v)
+{
+    p[i] = Cv<T>::t(v);
+}
+template <typename T>
+__device__ __forceinline__ float2 ld2T(const V2<T> *__restrict__ p, long i)
+{
+    const V2<T> v = p[i];
+    return make_float2(Cv<T>::f(v.x), Cv<T>::f(v.y));
+}
+template <typename T>
+__device__ __forceinline__ void st2T(V2<T> *__restrict__ p, long i, float2 v)
+{
+    V2<T> o;
+    o.x = Cv<T>::t(v.x);
+    o.y = Cv<T>::t(v.y);
+    p[i] = o;
+}
+template <typename T>
+__device__ __forceinline__ float4 ld4T(const V4<T> *__restrict__ p, long i)
+{
+    const V4<T> v = p[i];
+    return make_float4(Cv<T>::f(v.x), Cv<T>::f(v.y), Cv<T>::f(v.z), Cv<T>::f(v.w));
+}
+template <typename T>
+__device__ __forceinline__ void st4T(V4<T> *__restrict__ p, long i, float4 v)
+{
+    V4<T> o;
+    o.x = Cv<T>::t(v.x);
+    o.y = Cv<T>::t(v.y);
+    o.z = Cv<T>::t(v.z);
+    o.w = Cv<T>::t(v.w);
+    p[i] = o;
+}
+
+// ------------------------------------------------------------------ sampling
+
+__device__ __forceinline__ int mir(int k, int n)
+{
+    if ((unsigned)k < (unsigned)n) return k;
+    if (k < 0) k = -1 - k;
+    if (k >= n) {
+        const int P = 2 * n;
+        k %= P;
+        if (k >= n) k = P - 1 - k;
+    }
+    return k;
+}
+
+__device__ __forceinline__ float clampc(float v) { return fminf(fmaxf(v, -1e8f), 1e8f); }
+
+// Bilinear fetch with mirror addressing at pixel-index position (px, py).
+template <typename T>
+__device__ __forceinline__ float bil(const T *__restrict__ img, int w, int h, int s, float px, float py)
+{
+    if (__builtin_isnan(px) || __builtin_isnan(py)) return __builtin_nanf("");
+    px = clampc(px);
+    py = clampc(py);
+    const float fx = floorf(px), fy = floorf(py);
+    const float ax = px - fx, ay = py - fy;
+    const int i = (int)fx, j = (int)fy;
+    const int i0 = mir(i, w), i1 = mir(i + 1, w);
+    const int j0 = mir(j, h), j1 = mir(j + 1, h);
+    const T *r0 = img + (size_t)j0 * s;
+    const T *r1 = img + (size_t)j1 * s;
+    const float top = (1.f - ax) * Cv<T>::f(r0[i0]) + ax * Cv<T>::f(r0[i1]);
+    const float bot = (1.f - ax) * Cv<T>::f(r1[i0]) + ax * Cv<T>::f(r1[i1]);
+    return (1.f - ay) * top + ay * bot;
+}
+
+__device__ __forceinline__ float2 bil2(const float2 *__restrict__ img, int w, int h, int s, float px, float py)
+{
+    if (__builtin_isnan(px) || __builtin_isnan(py)) {
+        const float n = __builtin_nanf("");
+        return make_float2(n, n);
+    }
+    px = clampc(px);
+    py = clampc(py);
+    const float fx = floorf(px), fy = floorf(py);
+    const float ax = px - fx, ay = py - fy;
+    const int i = (int)fx, j = (int)fy;
+    const int i0 = mir(i, w), i1 = mir(i + 1, w);
+    const int j0 = mir(j, h), j1 = mir(j + 1, h);
+    const float2 *r0 = img + (size_t)j0 * s;
+    const float2 *r1 = img + (size_t)j1 * s;
+    const float2 a = r0[i0], b = r0[i1], c = r1[i0], d = r1[i1];
+    float2 o;
+    o.x = (1.f - ay) * ((1.f - ax) * a.x + ax * b.x) + ay * ((1.f - ax) * c.x + ax * d.x);
+    o.y = (1.f - ay) * ((1.f - ax) * a.y + ax * b.y) + ay * ((1.f - ax) * c.y + ax * d.y);
+    return o;
+}
+
+template <typename T>
+__device__ __forceinline__ float texel(const T *__restrict__ img, int w, int h, int s, int x, int y)
+{
+    return Cv<T>::f(img[(size_t)mir(y, h) * s + mir(x, w)]);
+}
+
+// Circle gate of the unmasked solve (updateDualVariables.cu:13-22): true when
+// radius >= width / 2.2 around (w/2, h/2).  Evaluated in double on the float bound.
+__device__ __forceinline__ bool outside_circle(int x, int y, int w, int h)
+{
+    const float desired = (float)w / 2.20f;
+    const double hw = (double)((float)w / 2.0f), hh = (double)((float)h / 2.0f);
+    const double dx = (double)x - hw, dy = (double)y - hh;
+    return dx * dx + dy * dy >= (double)desired * (double)desired;
+}
+
+__device__ __forceinline__ float2 ld0(const float2 *__restrict__ b, long i)
+{
+    return i < 0 ? make_float2(0.f, 0.f) : b[i];
+}
+
+// ------------------------------------------------------------------ tiling
+
+struct Tile {
+    const PairDesc *d;
+    int x0, y0, w, h, s;
+};
+
+// Workgroup -> (pair, 64x8 tile) at `level` through the tile prefix table.
+__device__ __forceinline__ Tile tile_of(const PairDesc *__restrict__ D, const int *__restrict__ ts, int n,
+                                        int level)
+{
+    const int b = blockIdx.x;
+    int lo = 0, hi = n - 1;
+    while (lo < hi) {  // skipped for a single pair: no table load on the critical path
+        const int mid = (lo + hi + 1) >> 1;
+        if (ts[mid] <= b) lo = mid;
+        else hi = mid - 1;
+    }
+    Tile t;
+    t.d = D + lo;
+    t.w = t.d->w[level];
+    t.h = t.d->h[level];
+    t.s = t.d->s[level];
+    const int local = n == 1 ? b : b - ts[lo];
+    const int ntx = (t.w + kTileW - 1) / kTileW;
+    t.x0 = (local % ntx) * kTileW;
+    t.y0 = (local / ntx) * kTileH;
+    return t;
+}
+
+__device__ __forceinline__ float *slot(const PairDesc *d, int k) { return d->arena + (long long)k * d->plane; }
+
+template <typename P>
+__device__ __forceinline__ P *slotp(const PairDesc *d, int k)
+{
+    return reinterpret_cast<P *>(d->arena + (long long)k * d->plane);
+}
+
+// Iterates the calling work-item over its pixels of the tile (2 rows each).
+#define FOR_TILE_PIXELS(t, x, y)                                                      \
+    for (int y = (t).y0 + (int)threadIdx.y, x = (t).x0 + (int)threadIdx.x;            \
+         y < (t).y0 + kTileH && y < (t).h && x < (t).w; y += BY)
+
+inline dim3 block2d() { return dim3(BX, BY); }
+
+// ------------------------------------------------------------------ solve kernels
+
+__global__ void k_zero_arenas(const PairDesc *__restrict__ D)
+{
+    const PairDesc *d = D + blockIdx.y;
+    float4 *p = reinterpret_cast<float4 *>(d->arena);
+    const long long n4 = (long long)PL_COUNT * d->plane / 4;
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n4; i += (long long)gridDim.x * blockDim.x)
+        p[i] = make_float4(0.f, 0.f, 0.f, 0.f);
+}
+
+// Calibration warp of I1 (stereotgv.cu:392-393, warpImage.cu:9-23) into the level-0
+// pyramid slot; with 16-bit storage also narrows I0 into its pyramid slot.
+template <typename T>
+__global__ __launch_bounds__(NT) void k_calibrate(const PairDesc *__restrict__ D, const int *__restrict__ ts, int n)
+{
+    const Tile t = tile_of(D, ts, n, 0);
+    const float *i1 = t.d->in_i1;
+    T *o1 = (T *)t.d->I1[0];
+    FOR_TILE_PIXELS(t, x, y)
+    {
+        const long pos = (long)y * t.s + x;
+        const float2 c = t.d->cv[pos];
+        stT(o1, pos, bil(i1, t.w, t.h, t.s, (float)x + c.x, (float)y + c.y));
+        if (sizeof(T) != sizeof(float)) stT((T *)t.d->I0[0], pos, t.d->in_i0[pos]);
+    }
+}
+
+// downscale.cu:10-30 for I0 and I1 (stereotgv.cu:400-405): mean of 4 bilinear taps at
+// +-1/4 coarse pixel around the coarse pixel centre, fetched from the finer level.
+template <typename T>
+__global__ __launch_bounds__(NT) void k_downscale_images(const PairDesc *__restrict__ D, const int *__restrict__ ts,
+                                                         int n, int level)
+{
+    const Tile t = tile_of(D, ts, n, level);
+    const int W = t.d->w[level - 1], H = t.d->h[level - 1], S = t.d->s[level - 1];
+    const T *a = (const T *)t.d->I0[level - 1];
+    const T *b = (const T *)t.d->I1[level - 1];
+    T *oa = (T *)t.d->I0[level];
+    T *ob = (T *)t.d->I1[level];
+    const float rx = (float)W / (float)t.w, ry = (float)H / (float)t.h;
+    const float ox = 0.25f * rx, oy = 0.25f * ry;
+    FOR_TILE_PIXELS(t, x, y)
+    {
+        const float cx = ((float)x + 0.5f) * rx - 0.5f, cy = ((float)y + 0.5f) * ry - 0.5f;
+        const long pos = (long)y * t.s + x;
+        stT(oa, pos, 0.25f * (bil(a, W, H, S, cx - ox, cy) + bil(a, W, H, S, cx + ox, cy) +
+                              bil(a, W, H, S, cx, cy - oy) + bil(a, W, H, S, cx, cy + oy)));
+        stT(ob, pos, 0.25f * (bil(b, W, H, S, cx - ox, cy) + bil(b, W, H, S, cx + ox, cy) +
+                              bil(b, W, H, S, cx, cy - oy) + bil(b, W, H, S, cx, cy + oy)));
+    }
+}
+
+// Coarsest-level init warpUV = u * tv (stereotgv.cu:419-421, warpImage.cu:136-150).
+__global__ __launch_bounds__(NT) void k_coarsest_init(const PairDesc *__restrict__ D, const int *__restrict__ ts,
+                                                      int n, int level, int roles)
+{
+    const Tile t = tile_of(D, ts, n, level);
+    const float *u = slot(t.d, (roles & RB_U) ? PL_U1 : PL_U0);
+    float2 *uv = slotp<float2>(t.d, (roles & RB_UV) ? PL_UV1 : PL_UV0);
+    const float2 *tv = t.d->tv[level];
+    FOR_TILE_PIXELS(t, x, y)
+    {
+        const long pos = (long)y * t.s + x;
+        const float us = u[pos];
+        const float2 v = tv[pos];
+        uv[pos] = make_float2(us * v.x, us * v.y);
+    }
+}
+
+// 3x3 binomial of the level image at (x, y), mirror addressing (calcTensor.cu:245-273).
+template <typename T>
+__device__ __forceinline__ float gauss3(const T *__restrict__ I, int w, int h, int s, int x, int y)
+{
+    float r = 0.25f * texel(I, w, h, s, x, y);
+    r += 0.0625f * texel(I, w, h, s, x - 1, y - 1);
+    r += 0.0625f * texel(I, w, h, s, x - 1, y + 1);
+    r += 0.0625f * texel(I, w, h, s, x + 1, y - 1);
+    r += 0.0625f * texel(I, w, h, s, x + 1, y + 1);
+    r += 0.125f * texel(I, w, h, s, x - 1, y);
+    r += 0.125f * texel(I, w, h, s, x + 1, y);
+    r += 0.125f * texel(I, w, h, s, x, y - 1);
+    r += 0.125f * texel(I, w, h, s, x, y + 1);
+    return r;
+}
+
+// The smoothed image as the tensor kernel samples it: the masked blur writes only
+// mask != 0 pixels of a zeroed plane, and the texture fetch mirrors at the level edge.
+template <typename T, bool MASKED>
+__device__ __forceinline__ float smoothed(const T *__restrict__ I, const float *__restrict__ mask, int w, int h,
+                                          int s, int x, int y)
+{
+    const int mx = mir(x, w), my = mir(y, h);
+    if (MASKED && mask[(size_t)my * s + mx] == 0.f) return 0.f;
+    return gauss3(I, w, h, s, mx, my);
+}
+
+// Fused GaussianMasked + CalcTensorMasked (calcTensor.cu:154-273) or the unmasked pair
+// (calcTensor.cu:7-128).  Writes a, b, c for every in-image pixel: the tensor where the
+// reference computes it, 0 where the reference leaves its per-level memset (mask == 0,
+// or columns x >= xlimit: calcTensor.cu:81,227 size the launch by the HEIGHT).  The
+// preconditioners eta (preconditioning.cu) are recomputed from (a, b, c) where needed.
+template <typename T, bool MASKED>
+__global__ __launch_bounds__(NT) void k_tensor(const PairDesc *__restrict__ D, const int *__restrict__ ts, int n,
+                                               int level, int xblock, float beta, float gamma)
+{
+    const Tile t = tile_of(D, ts, n, level);
+    const T *I0 = (const T *)t.d->I0[level];
+    const float *mask = t.d->mask[level];
+    T *at = (T *)slot(t.d, PL_A);
+    T *bt = (T *)slot(t.d, PL_B);
+    T *ct = (T *)slot(t.d, PL_C);
+    const int xlimit = xblock * ((t.h + xblock - 1) / xblock);
+    FOR_TILE_PIXELS(t, x, y)
+    {
+        const long pos = (long)y * t.s + x;
+        float a = 0.f, b = 0.f, c = 0.f;
+        const bool active = MASKED ? (mask[pos] != 0.f) : true;
+        if (active && x < xlimit) {
+            const float s11 = smoothed<T, MASKED>(I0, mask, t.w, t.h, t.s, x + 1, y + 1);
+            const float s01 = smoothed<T, MASKED>(I0, mask, t.w, t.h, t.s, x, y + 1);
+            const float s10 = smoothed<T, MASKED>(I0, mask, t.w, t.h, t.s, x + 1, y);
+            const float gx = s11 - s01;
+            const float gy = s11 - s10;
+            const float absg = sqrtf(gx * gx + gy * gy);
+            float nx = gx / absg, ny = gy / absg;
+            if (absg < 1e-8f) {
+                nx = 1.f;
+                ny = 0.f;
+            }
+            const float tx = ny, ty = -nx;
+            float wt = expf(-beta * powf(absg, gamma));
+            if (!(wt > 1e-8f)) wt = 1e-8f;
+            a = wt * nx * nx + tx * tx;
+            c = wt * nx * ny + tx * ty;
+            b = wt * ny * ny + ty * ty;
+        }
+        stT(at, pos, a);
+        stT(bt, pos, b);
+        stT(ct, pos, c);
+    }
+}
+
+// Per-warp preparation (stereotgv.cu:431-447): tv2 = bilinear(tv, x + warpUV)
+// (warpImage.cu:96-111), I1w = bilinear(I1, x + warpUV) (warpImage.cu:9-23),
+// u_last = u (copyClone.cu:3-12), v_ = 0 (Clone of the zeroed v).
+template <typename T>
+__global__ __launch_bounds__(NT) void k_warp_prep(const PairDesc *__restrict__ D, const int *__restrict__ ts, int n,
+                                                  int level, int roles)
+{
+    const Tile t = tile_of(D, ts, n, level);
+    const float2 *uv = slotp<float2>(t.d, (roles & RB_UV) ? PL_UV1 : PL_UV0);
+    const float *u = slot(t.d, (roles & RB_U) ? PL_U1 : PL_U0);
+    float2 *v_ = slotp<float2>(t.d, (roles & RB_V) ? PL_V1 : PL_V0);
+    const float2 *tv = t.d->tv[level];
+    const T *I1 = (const T *)t.d->I1[level];
+    float2 *tv2 = slotp<float2>(t.d, PL_TV2);
+    T *i1w = (T *)slot(t.d, PL_I1W);
+    float *ulast = slot(t.d, PL_ULAST);
+    FOR_TILE_PIXELS(t, x, y)
+    {
+        const long pos = (long)y * t.s + x;
+        const float2 d = uv[pos];
+        const float px = (float)x + d.x, py = (float)y + d.y;
+        tv2[pos] = bil2(tv, t.w, t.h, t.s, px, py);
+        stT(i1w, pos, bil(I1, t.w, t.h, t.s, px, py));
+        ulast[pos] = u[pos];
+        v_[pos] = make_float2(0.f, 0.f);
+    }
+}
+
+// computeDerivatives.cu:88-129: 5-point derivative along t = tv/|tv| of I0 and I1w,
+// averaged; Iz = I1w - I0 at the texel centre.
+template <typename T>
+__global__ __launch_bounds__(NT) void k_deriv(const PairDesc *__restrict__ D, const int *__restrict__ ts, int n,
+                                              int level)
+{
+    const Tile t = tile_of(D, ts, n, level);
+    const T *I0 = (const T *)t.d->I0[level];
+    const T *I1w = (const T *)slot(t.d, PL_I1W);
+    const float2 *tv = t.d->tv[level];
+    T *Iu = (T *)slot(t.d, PL_IU);
+    T *Iz = (T *)slot(t.d, PL_IZ);
+    FOR_TILE_PIXELS(t, x, y)
+    {
+        const long pos = (long)y * t.s + x;
+        const float2 v = tv[pos];
+        const float r = sqrtf(v.x * v.x + v.y * v.y);
+        const float dx = v.x / r, dy = v.y / r;
+        const float X = (float)x, Y = (float)y;
+        const int w = t.w, h = t.h, s = t.s;
+        float t0 = bil(I0, w, h, s, X - 2.f * dx, Y - 2.f * dy);
+        t0 -= bil(I0, w, h, s, X - dx, Y - dy) * 8.f;
+        t0 += bil(I0, w, h, s, X + dx, Y + dy) * 8.f;
+        t0 -= bil(I0, w, h, s, X + 2.f * dx, Y + 2.f * dy);
+        t0 /= 12.f;
+        float t1 = bil(I1w, w, h, s, X - 2.f * dx, Y - 2.f * dy);
+        t1 -= bil(I1w, w, h, s, X - dx, Y - dy) * 8.f;
+        t1 += bil(I1w, w, h, s, X + dx, Y + dy) * 8.f;
+        t1 -= bil(I1w, w, h, s, X + 2.f * dx, Y + 2.f * dy);
+        t1 /= 12.f;
+        stT(Iu, pos, (t0 + t1) * 0.5f);
+        stT(Iz, pos, ldT(I1w, pos) - ldT(I0, pos));
+    }
+}
+
+// L1 proximal step (thresholdingL1.cu:180-215), linearised at the current u.
+__device__ __forceinline__ float thresh_core(float divp, float tau, float etau, float u_pos, float upos, float Iu,
+                                             float Iz, float lambda)
+{
+    const float te = (etau == 0.f) ? tau : tau / etau;
+    const float uhat = u_pos + te * divp;
+    const float dun = uhat - upos;
+    const float rho = Iu * dun + Iz;
+    const float upper = lambda * te * (Iu * Iu);
+    const float lower = -lambda * te * (Iu * Iu);
+    float du = 0.f;
+    if ((rho <= upper) && (rho >= lower)) {
+        du = (Iu == 0.f) ? dun : dun - rho / Iu;
+    } else if (rho < lower) {
+        du = dun + lambda * te * Iu;
+    } else if (rho > upper) {
+        du = dun - lambda * te * Iu;
+    }
+    return upos + du;
+}
+
+// One TGV-L1 inner iteration in one launch: dual ascent + projection
+// (updateDualVariables.cu:122-237 / :4-101), Tp (thresholdingL1.cu:234-244), L1
+// thresholding (thresholdingL1.cu:128-216 / :3-86) and the primal step with
+// over-relaxation (updatePrimalVariables.cu:138-245 / :3-113).
+// Phase A computes the new p, q and Tp on the 64x8 tile plus its left column and top row
+// (the threshold and primal of a pixel read Tp and q of its left and upper neighbours)
+// into LDS; phase B finishes thresholding and the primal step.  Old duals are read from
+// one buffer and new ones written to the other, so every workgroup recomputes its halo
+// from the previous iteration's values (the reference's Jacobi order).
+constexpr int EXW = kTileW + 1, EXH = kTileH + 1;
+constexpr int HALO = EXW + kTileH;  // top row (incl. corner) + left column positions
+
+// Operands of the dual update at one pixel, gathered with in-buffer (clamped) addresses
+// and no data-dependent branch, so that all loads of a work-item issue back to back.
+struct DualIn {
+    long pos;
+    bool ok;  // inside the level image
+    bool act; // processed: in the mask (masked form) / in the image (unmasked form)
+    bool hasR, hasD;
+    float a, b, c, u0, uR, uD, uL, uU;
+    float2 v0, vR, vD, vL, vU, po;
+    float4 qo;
+};
+
+template <bool MASKED, typename T>
+__device__ __forceinline__ void dual_load(DualIn &r, int x, int y, int w, int h, int s, const float *__restrict__ mask,
+                                          const float *__restrict__ u_, const float2 *__restrict__ v_,
+                                          const T *__restrict__ at, const T *__restrict__ bt, const T *__restrict__ ct,
+                                          const V2<T> *__restrict__ p_in, const V4<T> *__restrict__ q_in, bool first)
+{
+    r.ok = x >= 0 && y >= 0 && x < w && y < h;
+    const int xc = min(max(x, 0), w - 1), yc = min(max(y, 0), h - 1);
+    const long pos = (long)yc * s + xc;
+    r.pos = pos;
+    const long pR = (xc + 1 < w) ? pos + 1 : pos;
+    const long pD = (yc + 1 < h) ? pos + s : pos;
+    const long pL = pos >= 1 ? pos - 1 : pos;
+    const long pU = pos >= s ? pos - s : pos;
+    if (MASKED) {
+        const float m0 = mask[pos], mR = mask[pR], mD = mask[pD];
+        r.act = r.ok && m0 != 0.f;
+        r.hasR = (xc + 1 < w) && mR != 0.f;
+        r.hasD = (yc + 1 < h) && mD != 0.f;
+    } else {
+        r.act = r.ok;
+        r.hasR = xc + 1 < w;
+        r.hasD = yc + 1 < h;
+    }
+    r.a = ldT(at, pos);
+    r.b = ldT(bt, pos);
+    r.c = ldT(ct, pos);
+    r.u0 = u_[pos];
+    r.uR = u_[pR];
+    r.uD = u_[pD];
+    r.uL = pos >= 1 ? u_[pL] : 0.f;   // flat index -1 reads 0
+    r.uU = pos >= s ? u_[pU] : 0.f;
+    r.v0 = v_[pos];
+    r.vR = v_[pR];
+    r.vD = v_[pD];
+    r.vL = pos >= 1 ? v_[pL] : make_float2(0.f, 0.f);
+    r.vU = pos >= s ? v_[pU] : make_float2(0.f, 0.f);
+    if (first) {
+        r.po = make_float2(0.f, 0.f);
+        r.qo = make_float4(0.f, 0.f, 0.f, 0.f);
+    } else {
+        r.po = ld2T(p_in, pos);
+        r.qo = ld4T(q_in, pos);
+    }
+}
+
+// updateDualVariables.cu:152-236 (masked) / :28-98 (unmasked): ascent on p and q with
+// unit-ball projection, then Tp = T p (thresholdingL1.cu:242-243).
+template <bool MASKED>
+__device__ __forceinline__ void dual_step(const DualIn &r, int x, int y, int w, int h, const IterParams &ip,
+                                          float2 &pn, float4 &qn, float2 &tpn)
+{
+    if (!MASKED && outside_circle(x, y, w, h)) {
+        pn = make_float2(0.f, 0.f);
+        qn = make_float4(0.f, 0.f, 0.f, 0.f);
+    } else {
+        const float ux = r.hasR ? (r.uR - r.u0 - r.v0.x) : (r.u0 - r.uL - r.v0.x);
+        const float uy = r.hasD ? (r.uD - r.u0 - r.v0.y) : (r.u0 - r.uU - r.v0.y);
+        const float dtx = r.a * ux + r.c * uy;
+        const float dty = r.c * ux + r.b * uy;
+        const float px = r.po.x + ip.k1 * dtx;
+        const float py = r.po.y + ip.k1 * dty;
+        float rp = sqrtf(px * px + py * py);
+        if (rp < 1.0f) rp = 1.0f;
+        pn = make_float2(px / rp, py / rp);
+        float g0, g1, g2, g3;
+        if (r.hasR) {
+            g0 = r.vR.x - r.v0.x;
+            g3 = r.vR.y - r.v0.y;
+        } else {
+            g0 = r.v0.x - r.vL.x;
+            g3 = r.v0.y - r.vL.y;
+        }
+        if (r.hasD) {
+            g1 = r.vD.y - r.v0.y;
+            g2 = r.vD.x - r.v0.x;
+        } else {
+            g1 = r.v0.y - r.vU.y;
+            g2 = r.v0.x - r.vU.x;
+        }
+        const float q0 = r.qo.x + ip.k0 * g0, q1 = r.qo.y + ip.k0 * g1, q2 = r.qo.z + ip.k0 * g2,
+                    q3 = r.qo.w + ip.k0 * g3;
+        float rq = sqrtf(q0 * q0 + q1 * q1 + q2 * q2 + q3 * q3);
+        if (rq < 1.0f) rq = 1.0f;
+        qn = make_float4(q0 / rq, q1 / rq, q2 / rq, q3 / rq);
+    }
+    tpn = make_float2(r.a * pn.x + r.c * pn.y, r.c * pn.x + r.b * pn.y);
+}
+
+// One TGV-L1 inner iteration in one launch: dual ascent + projection
+// (updateDualVariables.cu:122-237 / :4-101), Tp (thresholdingL1.cu:234-244), L1
+// thresholding (thresholdingL1.cu:128-216 / :3-86) and the primal step with
+// over-relaxation (updatePrimalVariables.cu:138-245 / :3-113).
+// Phase A computes the new p, q and Tp of the 64x8 tile and of its left column and top
+// row (the threshold and primal of a pixel read Tp and q of its left and upper
+// neighbours) into LDS: each work-item takes the two tile pixels it finishes in phase B,
+// and 73 work-items also one halo position.  Phase B finishes thresholding and the primal
+// step.  Old duals are read from one buffer and new ones written to the other, so every
+// workgroup recomputes its halo from the previous iteration's values (the reference's
+// Jacobi order).  Every load is issued before any data-dependent branch.
+template <bool MASKED, typename T>
+__global__ __launch_bounds__(NT) void k_iter(const PairDesc *__restrict__ D, const int *__restrict__ ts, int n,
+                                             int level, int roles, IterParams ip)
+{
+    __shared__ float2 sP[EXH][EXW];
+    __shared__ float4 sQ[EXH][EXW];
+    __shared__ float2 sT[EXH][EXW];
+    const Tile t = tile_of(D, ts, n, level);
+    const int w = t.w, h = t.h, s = t.s, x0 = t.x0, y0 = t.y0;
+    const PairDesc *d = t.d;
+    const float *__restrict__ mask = d->mask[level];
+    const float *__restrict__ u = slot(d, (roles & RB_U) ? PL_U1 : PL_U0);
+    float *__restrict__ us = slot(d, (roles & RB_U) ? PL_U0 : PL_U1);
+    const float *__restrict__ u_ = slot(d, (roles & RB_UB) ? PL_UB1 : PL_UB0);
+    float *__restrict__ u_s = slot(d, (roles & RB_UB) ? PL_UB0 : PL_UB1);
+    const float2 *__restrict__ v_ = slotp<float2>(d, (roles & RB_V) ? PL_V1 : PL_V0);
+    float2 *__restrict__ v_s = slotp<float2>(d, (roles & RB_V) ? PL_V0 : PL_V1);
+    const V2<T> *__restrict__ p_in = slotp<V2<T>>(d, (roles & RB_PQ) ? PL_P1 : PL_P0);
+    V2<T> *__restrict__ p_out = slotp<V2<T>>(d, (roles & RB_PQ) ? PL_P0 : PL_P1);
+    const V4<T> *__restrict__ q_in = slotp<V4<T>>(d, (roles & RB_PQ) ? PL_Q1 : PL_Q0);
+    V4<T> *__restrict__ q_out = slotp<V4<T>>(d, (roles & RB_PQ) ? PL_Q0 : PL_Q1);
+    float2 *__restrict__ Tp = slotp<float2>(d, PL_TP);
+    const T *__restrict__ at = (const T *)slot(d, PL_A);
+    const T *__restrict__ bt = (const T *)slot(d, PL_B);
+    const T *__restrict__ ct = (const T *)slot(d, PL_C);
+    const T *__restrict__ Iu = (const T *)slot(d, PL_IU);
+    const T *__restrict__ Iz = (const T *)slot(d, PL_IZ);
+    const int tid = threadIdx.y * BX + threadIdx.x;
+    const int x = x0 + threadIdx.x;
+
+    // ---- loads: two tile pixels (rows threadIdx.y and threadIdx.y + 4) + one halo position
+    DualIn in[2];
+    float uu[2], iu[2], iz[2];
+    bool hasL[2], hasU[2];
+#pragma unroll
+    for (int r = 0; r < 2; r++) {
+        const int y = y0 + threadIdx.y + BY * r;
+        dual_load<MASKED, T>(in[r], x, y, w, h, s, mask, u_, v_, at, bt, ct, p_in, q_in, ip.first);
+        const long pos = in[r].pos;
+        uu[r] = u[pos];
+        iu[r] = ldT(Iu, pos);
+        iz[r] = ldT(Iz, pos);
+        const int xc = min(x, w - 1), yc = min(y, h - 1);
+        if (MASKED) {
+            hasL[r] = xc >= 1 && mask[xc >= 1 ? pos - 1 : pos] != 0.f;
+            hasU[r] = yc >= 1 && mask[yc >= 1 ? pos - s : pos] != 0.f;
+        } else {
+            hasL[r] = xc >= 1;
+            hasU[r] = yc >= 1;
+        }
+    }
+    DualIn hin;
+    int hx = 0, hy = 0, hex = 0, hey = 0;
+    const bool has_halo = tid < HALO;
+    if (has_halo) {
+        if (tid < EXW) {
+            hex = tid;
+            hey = 0;
+        } else {
+            hex = 0;
+            hey = tid - EXW + 1;
+        }
+        hx = x0 - 1 + hex;
+        hy = y0 - 1 + hey;
+        dual_load<MASKED, T>(hin, hx, hy, w, h, s, mask, u_, v_, at, bt, ct, p_in, q_in, ip.first);
+    }
+
+    // ---- phase A: new duals + Tp into LDS (and the tile's to global memory)
+#pragma unroll
+    for (int r = 0; r < 2; r++) {
+        const int y = y0 + threadIdx.y + BY * r;
+        if (!in[r].act) continue;
+        float2 pn, tpn;
+        float4 qn;
+        dual_step<MASKED>(in[r], x, y, w, h, ip, pn, qn, tpn);
+        const int ex = threadIdx.x + 1, ey = threadIdx.y + BY * r + 1;
+        sP[ey][ex] = pn;
+        sQ[ey][ex] = qn;
+        sT[ey][ex] = tpn;
+        st2T(p_out, in[r].pos, pn);
+        st4T(q_out, in[r].pos, qn);
+        if (ip.write_tp) Tp[in[r].pos] = tpn;
+    }
+    if (has_halo && hin.act) {
+        float2 pn, tpn;
+        float4 qn;
+        dual_step<MASKED>(hin, hx, hy, w, h, ip, pn, qn, tpn);
+        sP[hey][hex] = pn;
+        sQ[hey][hex] = qn;
+        sT[hey][hex] = tpn;
+    }
+    __syncthreads();
+
+    // ---- phase B: thresholding + primal step on the tile
+    const float aa = ip.alpha1 * ip.alpha1;
+#pragma unroll
+    for (int r = 0; r < 2; r++) {
+        const DualIn &q = in[r];
+        const int y = y0 + threadIdx.y + BY * r;
+        if (!q.ok) continue;
+        const long pos = q.pos;
+        if (!MASKED && outside_circle(x, y, w, h)) {
+            us[pos] = 0.f;
+            u_s[pos] = 0.f;
+            v_s[pos] = make_float2(0.f, 0.f);
+            continue;
+        }
+        if (!q.act) continue;
+        const bool hR = q.hasR, hD = q.hasD, hL = hasL[r], hU = hasU[r];
+        const int ex = threadIdx.x + 1, ey = threadIdx.y + BY * r + 1;
+        const float a = q.a, b = q.b, c = q.c;
+        const float2 tp0 = sT[ey][ex];
+        // a neighbour outside the mask (or the image) is read from the Tp plane, which
+        // holds what earlier levels left there, as the reference does
+        const float tpLx = hL ? sT[ey][ex - 1].x : ld0(Tp, pos - 1).x;
+        const float tpUy = hU ? sT[ey - 1][ex].y : ld0(Tp, pos - s).y;
+        float dx, dy;
+        if (MASKED) {
+            if (hL && hR) dx = tp0.x - tpLx;
+            else if (!hR) dx = -tpLx;
+            else dx = tp0.x;
+            if (hU && hD) dy = tp0.y - tpUy;
+            else if (!hD) dy = -tpUy;
+            else dy = tp0.y;
+        } else {
+            if (hL) dx = tp0.x - tpLx;
+            else if (x == w - 1) dx = -tpLx;
+            else dx = tp0.x;
+            if (hU) dy = tp0.y - tpUy;
+            else if (y == h - 1) dy = -tpUy;
+            else dy = tp0.y;
+        }
+        const float etau = (a * a + b * b + 2 * c * c + (a + c) * (a + c) + (b + c) * (b + c)) * aa;
+        const float u_pos = q.u0;
+        const float un = thresh_core(dx + dy, ip.tau, etau, u_pos, uu[r], iu[r], iz[r], ip.lambda);
+        us[pos] = un;
+        const float4 qc4 = sQ[ey][ex];
+        float qcx = 0.f, qcw = 0.f, qcy = 0.f, qcz = 0.f;
+        if (hR) {
+            qcx = qc4.x;
+            qcw = qc4.w;
+        }
+        if (hD) {
+            qcy = qc4.y;
+            qcz = qc4.z;
+        }
+        float qwx = 0.f, qww = 0.f, qny = 0.f, qnz = 0.f;
+        if (hL) {
+            const float4 ql = sQ[ey][ex - 1];
+            qwx = ql.x;
+            qww = ql.w;
+        }
+        if (hU) {
+            const float4 qu = sQ[ey - 1][ex];
+            qny = qu.y;
+            qnz = qu.z;
+        }
+        const float dqx = (qcx - qwx) + (qcz - qnz);
+        const float dqy = (qcw - qww) + (qcy - qny);
+        const float2 pp = sP[ey][ex];
+        const float tx = a * pp.x + c * pp.y;
+        const float tyy = c * pp.x + b * pp.y;
+        const float e1 = aa * (b * b + c * c) + 4 * ip.alpha0 * ip.alpha0;
+        const float e2 = aa * (a * a + c * c) + 4 * ip.alpha0 * ip.alpha0;
+        const float2 vo = q.v0;
+        // e1, e2 >= 4 alpha0^2 > 0 for alpha0 != 0; the reference's zero branches
+        // (updatePrimalVariables.cu:228-239, the second writes .x) are kept for alpha0 == 0.
+        float vx = vo.x + ((e1 != 0.f) ? ip.tau / e1 : ip.tau) * (ip.alpha1 * tx + ip.alpha0 * dqx);
+        float vy;
+        if (e2 != 0.f) {
+            vy = vo.y + (ip.tau / e2) * (ip.alpha1 * tyy + ip.alpha0 * dqy);
+        } else {
+            vx = vo.x + ip.tau * (ip.alpha1 * tx + ip.alpha0 * dqx);
+            vy = 0.f;
+        }
+        u_s[pos] = un + ip.mu * (un - u_pos);
+        v_s[pos] = make_float2(vx + ip.mu * (vx - vo.x), vy + ip.mu * (vy - vo.y));
+    }
+}
+
+// 13th smallest of 25 by the partial selection sort of medianFilter.cu:307-316.
+__device__ __forceinline__ float median25(float m[25])
+{
+#pragma unroll
+    for (int j = 0; j < 13; j++) {
+#pragma unroll
+        for (int i = j + 1; i < 25; i++) {
+            const float a = m[j], b = m[i];
+            const bool sw = a > b;
+            m[j] = sw ? b : a;
+            m[i] = sw ? a : b;
+        }
+    }
+    return m[12];
+}
+
+// 5x5 window as medianFilter.cu:282-304: edge replicate when exactly one of row / column
+// is outside the image, 0 when both are.
+__device__ __forceinline__ float median5_at(const float *__restrict__ u, int x, int y, int w, int h, int s)
+{
+    float m[25];
+#pragma unroll
+    for (int j = 0; j < 5; j++) {
+        const int row = y + j - 2;
+        const bool rin = row >= 0 && row < h;
+#pragma unroll
+        for (int i = 0; i < 5; i++) {
+            const int col = x + i - 2;
+            const bool cin = col >= 0 && col < w;
+            float v = 0.f;
+            if (rin || cin) {
+                const int rr = rin ? row : (row < 0 ? 0 : h - 1);
+                const int cc = cin ? col : (col < 0 ? 0 : w - 1);
+                v = u[(size_t)rr * s + cc];
+            }
+            m[j * 5 + i] = v;
+        }
+    }
+    return median25(m);
+}
+
+// End of a warp (stereotgv.cu:481-493 masked, :354-363 unmasked): [median of u
+// (medianFilter.cu:264-319) into the other u slot,] du = min(u - u_last, limit)
+// (scalar.cu:129-141, limitRange.cu:3-24), u = u_last + du (scalar.cu:77-89),
+// u_ = u (copyClone.cu:3-12), warpUV += du * tv2 (warpImage.cu:136-150, scalar.cu:102-115).
+template <bool MASKED>
+__global__ __launch_bounds__(NT) void k_warp_end(const PairDesc *__restrict__ D, const int *__restrict__ ts, int n,
+                                                 int level, int roles, float limit)
+{
+    const Tile t = tile_of(D, ts, n, level);
+    const float *u = slot(t.d, (roles & RB_U) ? PL_U1 : PL_U0);
+    float *out = MASKED ? slot(t.d, (roles & RB_U) ? PL_U0 : PL_U1) : (float *)u;
+    float *u_ = slot(t.d, (roles & RB_UB) ? PL_UB1 : PL_UB0);
+    float2 *uv = slotp<float2>(t.d, (roles & RB_UV) ? PL_UV1 : PL_UV0);
+    const float *ulast = slot(t.d, PL_ULAST);
+    const float2 *tv2 = slotp<float2>(t.d, PL_TV2);
+    FOR_TILE_PIXELS(t, x, y)
+    {
+        const long pos = (long)y * t.s + x;
+        const float uval = MASKED ? median5_at(u, x, y, t.w, t.h, t.s) : u[pos];
+        const float ul = ulast[pos];
+        float du = uval - ul;
+        du = (du > limit) ? limit : du;
+        const float un = ul + du;
+        out[pos] = un;
+        u_[pos] = un;
+        const float2 tt = tv2[pos];
+        const float2 o = uv[pos];
+        uv[pos] = make_float2(o.x + du * tt.x, o.y + du * tt.y);
+    }
+}
+
+// upscale.cu:7-21, 46-61 for u, u_ and warpUV (stereotgv.cu:497-510): bilinear at the
+// finer level's pixel centres, times fScale; writes the other slot of each pair.
+__global__ __launch_bounds__(NT) void k_upscale(const PairDesc *__restrict__ D, const int *__restrict__ ts, int n,
+                                                int level, int roles, float scale)
+{
+    const Tile t = tile_of(D, ts, n, level - 1);  // tiles of the finer level
+    const int w = t.d->w[level], h = t.d->h[level], s = t.d->s[level];
+    const float *u = slot(t.d, (roles & RB_U) ? PL_U1 : PL_U0);
+    float *ou = slot(t.d, (roles & RB_U) ? PL_U0 : PL_U1);
+    const float *u_ = slot(t.d, (roles & RB_UB) ? PL_UB1 : PL_UB0);
+    float *ou_ = slot(t.d, (roles & RB_UB) ? PL_UB0 : PL_UB1);
+    const float2 *uv = slotp<float2>(t.d, (roles & RB_UV) ? PL_UV1 : PL_UV0);
+    float2 *ouv = slotp<float2>(t.d, (roles & RB_UV) ? PL_UV0 : PL_UV1);
+    const float rx = (float)w / (float)t.w, ry = (float)h / (float)t.h;
+    FOR_TILE_PIXELS(t, x, y)
+    {
+        const float px = ((float)x + 0.5f) * rx - 0.5f, py = ((float)y + 0.5f) * ry - 0.5f;
+        const long pos = (long)y * t.s + x;
+        ou[pos] = bil(u, w, h, s, px, py) * scale;
+        ou_[pos] = bil(u_, w, h, s, px, py) * scale;
+        const float2 v = bil2(uv, w, h, s, px, py);
+        ouv[pos] = make_float2(v.x * scale, v.y * scale);
+    }
+}
+
+// ------------------------------------------------------------------ single-plane kernels
+
+__device__ __forceinline__ bool pixel(int &x, int &y, int w, int h)
+{
+    x = blockIdx.x * BX + threadIdx.x;
+    y = blockIdx.y * BY + threadIdx.y;
+    return x < w && y < h;
+}
+inline dim3 grid_for(int w, int h) { return dim3((w + BX - 1) / BX, (h + BY - 1) / BY); }
+
+__global__ void k_fill_zero(float4 *__restrict__ p, size_t n4)
+{
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n4; i += (size_t)gridDim.x * blockDim.x)
+        p[i] = make_float4(0.f, 0.f, 0.f, 0.f);
+}
+
+__global__ void k_copy(const float4 *__restrict__ src, float4 *__restrict__ dst, size_t n4)
+{
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n4; i += (size_t)gridDim.x * blockDim.x)
+        dst[i] = src[i];
+}
+
+// imageConverter.cu:3-16
+__global__ void k_cv8u(const uint8_t *__restrict__ in, int ip, float *__restrict__ out, int w, int h, int s)
+{
+    int x, y;
+    if (!pixel(x, y, w, h)) return;
+    out[(size_t)y * s + x] = (float)in[(size_t)y * ip + x] / 256.0f;
+}
+
+// imageConverter.cu:26-42
+__global__ void k_cv8uc3(const uint8_t *__restrict__ in, int ip, float *__restrict__ out, int w, int h, int s)
+{
+    int x, y;
+    if (!pixel(x, y, w, h)) return;
+    const uint8_t *px = in + (size_t)y * ip + 3 * (size_t)x;
+    const float g = ((float)px[0] + (float)px[1] + (float)px[2]) / 3;
+    out[(size_t)y * s + x] = g / 256.0f;
+}
+
+// downscale.cu:98-122 (trajectory pyramid, stereotgv.cu:187-194)
+__global__ void k_downscale_f2(const float2 *__restrict__ src, int W, int H, int S, float2 *__restrict__ out, int w,
+                               int h, int s)
+{
+    int x, y;
+    if (!pixel(x, y, w, h)) return;
+    const float rx = (float)W / (float)w, ry = (float)H / (float)h;
+    const float cx = ((float)x + 0.5f) * rx - 0.5f, cy = ((float)y + 0.5f) * ry - 0.5f;
+    const float ox = 0.25f * rx, oy = 0.25f * ry;
+    const float2 v0 = bil2(src, W, H, S, cx - ox, cy), v1 = bil2(src, W, H, S, cx + ox, cy);
+    const float2 v2 = bil2(src, W, H, S, cx, cy - oy), v3 = bil2(src, W, H, S, cx, cy + oy);
+    out[(size_t)y * s + x] = make_float2(0.25f * (v0.x + v1.x + v2.x + v3.x), 0.25f * (v0.y + v1.y + v2.y + v3.y));
+}
+
+// downscale.cu:52-73: point sample at the coarse pixel centre, texel floor((x+.5) W / w)
+__global__ void k_downscale_nn(const float *__restrict__ src, int W, int H, int S, float *__restrict__ out, int w,
+                               int h, int s)
+{
+    int x, y;
+    if (!pixel(x, y, w, h)) return;
+    const long fx = ((2L * x + 1) * W) / (2L * w);
+    const long fy = ((2L * y + 1) * H) / (2L * h);
+    out[(size_t)y * s + x] = texel(src, W, H, S, (int)fx, (int)fy);
+}
+
+__device__ double kb_theta(double r, double d1, double d2, double d3, double d4)
+{
+    double th = 0.0;
+    for (int it = 0; it < 5; it++) {
+        const double t2 = th * th, t3 = t2 * th, t4 = t2 * t2, t5 = t4 * th, t6 = t4 * t2, t7 = t6 * th;
+        const double t8 = t4 * t4, t9 = t8 * th;
+        const double td = th + d1 * t3 + d2 * t5 + d3 * t7 + d4 * t9;
+        const double dtd = 1.0 + 3.0 * d1 * t2 + 5.0 * d2 * t4 + 7.0 * d3 * t6 + 9.0 * d4 * t8;
+        th = th - (r - td) / (-dtd);
+    }
+    return th;
+}
+
+// conversion.cu:3-69 in FP64: Kannala-Brandt inversion of both frames (5 Newton steps
+// from theta = 0), Z from the x components, X = (x0 Z, y0 Z, Z), depth = |X|.
+__global__ void k_convert_kb(const float2 *__restrict__ uv, double fx, double fy, double cx, double cy, double d1,
+                             double d2, double d3, double d4, double t1, double t2, double t3, float *__restrict__ X,
+                             float *__restrict__ depth, int w, int h, int s)
+{
+    int x, y;
+    if (!pixel(x, y, w, h)) return;
+    const size_t pos = (size_t)y * s + x;
+    const double u0 = x, v0 = y;
+    const double xp0 = (u0 - fx) / cx, yp0 = (v0 - fy) / cy;
+    const float2 d = uv[pos];
+    const double xp1 = (u0 + (double)d.x - fx) / cx, yp1 = (v0 + (double)d.y - fy) / cy;
+    const double r0 = sqrt(xp0 * xp0 + yp0 * yp0);
+    const double th0 = kb_theta(r0, d1, d2, d3, d4);
+    const double x0o = tan(th0) * xp0 / r0, y0o = tan(th0) * yp0 / r0;
+    const double r1 = sqrt(xp1 * xp1 + yp1 * yp1);
+    const double th1 = kb_theta(r1, d1, d2, d3, d4);
+    const double x1o = tan(th1) * xp1 / r1;
+    (void)t2;
+    const double Z = (t1 - x1o * t3) / (x1o - x0o);
+    const double X0 = x0o * Z, X1 = y0o * Z, X2 = Z;
+    X[3 * pos] = (float)X0;
+    X[3 * pos + 1] = (float)X1;
+    X[3 * pos + 2] = (float)X2;
+    depth[pos] = (float)sqrt(X0 * X0 + X1 * X1 + X2 * X2);
+}
+
+template <typename T>
+__global__ void k_widen(const T *__restrict__ src, int nch, float *__restrict__ dst, int w, int h, int s)
+{
+    int x, y;
+    if (!pixel(x, y, w, h)) return;
+    for (int c = 0; c < nch; c++) {
+        const size_t i = ((size_t)y * s + x) * nch + c;
+        dst[i] = Cv<T>::f(src[i]);
+    }
+}
+
+}  // namespace
+
+// ------------------------------------------------------------------ launchers
+
+static inline const int *tiles_at(const Launch &L, int level) { return L.tiles + level * (L.npairs + 1); }
+static inline dim3 grid_at(const Launch &L, int level) { return dim3((unsigned)L.tiles_total[level]); }
+
+#define STORAGE_DISPATCH(storage, KERNEL_CALL)                  \
+    do {                                                        \
+        if ((storage) == kF16) {                                \
+            using T = __half;                                   \
+            KERNEL_CALL;                                        \
+        } else if ((storage) == kBF16) {                        \
+            using T = __hip_bfloat16;                           \
+            KERNEL_CALL;                                        \
+        } else {                                                \
+            using T = float;                                    \
+            KERNEL_CALL;                                        \
+        }                                                       \
+    } while (0)
+
+void launch_zero_arenas(const Launch &L, hipStream_t st)
+{
+    long long n4 = (long long)PL_COUNT * L.max_plane_floats / 4;
+    long long blocks = (n4 + 255) / 256;
+    if (blocks > 2048) blocks = 2048;
+    k_zero_arenas<<<dim3((unsigned)blocks, (unsigned)L.npairs), dim3(256), 0, st>>>(L.desc);
+}
+
+void launch_calibrate(const Launch &L, int storage)
+{
+    STORAGE_DISPATCH(storage, (k_calibrate<T><<<grid_at(L, 0), block2d(), 0, L.stream>>>(L.desc, tiles_at(L, 0),
+                                                                                          L.npairs)));
+}
+
+void launch_downscale_images(const Launch &L, int level, int storage)
+{
+    STORAGE_DISPATCH(storage, (k_downscale_images<T><<<grid_at(L, level), block2d(), 0, L.stream>>>(
+                                  L.desc, tiles_at(L, level), L.npairs, level)));
+}
+
+void launch_coarsest_init(const Launch &L, int level, int roles)
+{
+    k_coarsest_init<<<grid_at(L, level), block2d(), 0, L.stream>>>(L.desc, tiles_at(L, level), L.npairs, level,
+                                                                    roles);
+}
+
+void launch_tensor(const Launch &L, int level, int masked, int xblock, float beta, float gamma, int storage)
+{
+    if (masked)
+        STORAGE_DISPATCH(storage, (k_tensor<T, true><<<grid_at(L, level), block2d(), 0, L.stream>>>(
+                                      L.desc, tiles_at(L, level), L.npairs, level, xblock, beta, gamma)));
+    else
+        STORAGE_DISPATCH(storage, (k_tensor<T, false><<<grid_at(L, level), block2d(), 0, L.stream>>>(
+                                      L.desc, tiles_at(L, level), L.npairs, level, xblock, beta, gamma)));
+}
+
+void launch_warp_prep(const Launch &L, int level, int roles, int storage)
+{
+    STORAGE_DISPATCH(storage, (k_warp_prep<T><<<grid_at(L, level), block2d(), 0, L.stream>>>(
+                                  L.desc, tiles_at(L, level), L.npairs, level, roles)));
+}
+
+void launch_deriv(const Launch &L, int level, int storage)
+{
+    STORAGE_DISPATCH(storage, (k_deriv<T><<<grid_at(L, level), block2d(), 0, L.stream>>>(L.desc, tiles_at(L, level),
+                                                                                          L.npairs, level)));
+}
+
+void launch_iter(const Launch &L, int level, int masked, int roles, const IterParams &ip, int storage)
+{
+    if (masked)
+        STORAGE_DISPATCH(storage, (k_iter<true, T><<<grid_at(L, level), block2d(), 0, L.stream>>>(
+                                      L.desc, tiles_at(L, level), L.npairs, level, roles, ip)));
+    else
+        STORAGE_DISPATCH(storage, (k_iter<false, T><<<grid_at(L, level), block2d(), 0, L.stream>>>(
+                                      L.desc, tiles_at(L, level), L.npairs, level, roles, ip)));
+}
+
+void launch_warp_end(const Launch &L, int level, int masked, int roles, float limit)
+{
+    if (masked)
+        k_warp_end<true><<<grid_at(L, level), block2d(), 0, L.stream>>>(L.desc, tiles_at(L, level), L.npairs, level,
+                                                                         roles, limit);
+    else
+        k_warp_end<false><<<grid_at(L, level), block2d(), 0, L.stream>>>(L.desc, tiles_at(L, level), L.npairs, level,
+                                                                          roles, limit);
+}
+
+void launch_upscale(const Launch &L, int level, int roles, float scale)
+{
+    k_upscale<<<grid_at(L, level - 1), block2d(), 0, L.stream>>>(L.desc, tiles_at(L, level - 1), L.npairs, level,
+                                                                  roles, scale);
+}
+
+void launch_fill_zero(void *ptr, size_t bytes, hipStream_t st)
+{
+    const size_t n4 = bytes / 16;
+    if (n4 == 0) return;
+    size_t blocks = (n4 + 255) / 256;
+    if (blocks > 8192) blocks = 8192;
+    k_fill_zero<<<dim3((unsigned)blocks), dim3(256), 0, st>>>((float4 *)ptr, n4);
+}
+
+void launch_copy(const void *src, void *dst, size_t bytes, hipStream_t st)
+{
+    const size_t n4 = bytes / 16;
+    size_t blocks = (n4 + 255) / 256;
+    if (blocks > 16384) blocks = 16384;
+    k_copy<<<dim3((unsigned)blocks), dim3(256), 0, st>>>((const float4 *)src, (float4 *)dst, n4);
+}
+
+void launch_cv8u(const uint8_t *in, int ip, float *out, Dims d, hipStream_t st)
+{
+    k_cv8u<<<grid_for(d.w, d.h), block2d(), 0, st>>>(in, ip, out, d.w, d.h, d.s);
+}
+
+void launch_cv8uc3(const uint8_t *in, int ip, float *out, Dims d, hipStream_t st)
+{
+    k_cv8uc3<<<grid_for(d.w, d.h), block2d(), 0, st>>>(in, ip, out, d.w, d.h, d.s);
+}
+
+void launch_downscale_f2(const float2 *src, Dims f, float2 *out, Dims c, hipStream_t st)
+{
+    k_downscale_f2<<<grid_for(c.w, c.h), block2d(), 0, st>>>(src, f.w, f.h, f.s, out, c.w, c.h, c.s);
+}
+
+void launch_downscale_nn(const float *src, Dims f, float *out, Dims c, hipStream_t st)
+{
+    k_downscale_nn<<<grid_for(c.w, c.h), block2d(), 0, st>>>(src, f.w, f.h, f.s, out, c.w, c.h, c.s);
+}
+
+void launch_convert_kb(const float2 *uv, double focalx, double focaly, double cx, double cy, double d1, double d2,
+                       double d3, double d4, double t1, double t2, double t3, float *X, float *depth, Dims d,
+                       hipStream_t st)
+{
+    k_convert_kb<<<grid_for(d.w, d.h), block2d(), 0, st>>>(uv, focalx, focaly, cx, cy, d1, d2, d3, d4, t1, t2, t3, X,
+                                                          depth, d.w, d.h, d.s);
+}
+
+void launch_widen(const void *src, int storage, int nch, float *dst, Dims d, hipStream_t st)
+{
+    STORAGE_DISPATCH(storage, (k_widen<T><<<grid_for(d.w, d.h), block2d(), 0, st>>>((const T *)src, nch, dst, d.w,
+                                                                                     d.h, d.s)));
+}
+
+}  // namespace vfs

@@ -1,0 +1,110 @@
+// Internal interface between the host solver (vfs_solver.hip) and the gfx950 kernels
+// (vfs_kernels.hip) of the StereoTgv solve.
+//
+// A launch covers a *set of frame pairs* at one pyramid level: every pair has a
+// descriptor (PairDesc, in device memory) holding its level geometry, its pyramid planes
+// and the base of its work arena.  Workgroups own 64x8 tiles; a per-level prefix table
+// of tile counts maps the flat workgroup index to (pair, tile), so pairs of different
+// resolutions share one launch.  A single solve is the same code with one pair.
+#pragma once
+#include <hip/hip_bf16.h>
+#include <hip/hip_fp16.h>
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace vfs {
+
+constexpr int kMaxLevels = 16;
+constexpr int kTileW = 64;  // tile width = one wave64 per row segment
+constexpr int kTileH = 8;   // tile height; a 256-work-item workgroup covers it in 2 row passes
+
+// Storage precision of the image pyramid, warped image, derivatives, tensor and duals.
+// Arithmetic is FP32 in every mode.
+enum Storage : int { kF32 = 0, kF16 = 1, kBF16 = 2 };
+
+// Work planes of one pair: slot k starts at arena + k * plane (floats).  2- and 4-channel
+// planes take 2 / 4 consecutive slots.  Planes marked (st) hold the storage type.
+enum Plane : int {
+    PL_U0 = 0, PL_U1 = 1,    // u / us ping-pong
+    PL_UB0 = 2, PL_UB1 = 3,  // u_ / u_s ping-pong
+    PL_ULAST = 4,
+    PL_I1W = 5,              // warped I1 (st)
+    PL_IU = 6, PL_IZ = 7,    // derivatives (st)
+    PL_A = 8, PL_B = 9, PL_C = 10,  // tensor (st)
+    PL_V0 = 11, PL_V1 = 13,  // v_ / v_s ping-pong (float2)
+    PL_P0 = 15, PL_P1 = 17,  // p double buffer (2 x st)
+    PL_TP = 19,              // Tp (float2)
+    PL_TV2 = 21,             // tv2 (float2)
+    PL_UV0 = 23, PL_UV1 = 25,  // warpUV ping-pong (float2)
+    PL_Q0 = 27, PL_Q1 = 31,  // q double buffer (4 x st)
+    PL_COUNT = 35
+};
+
+struct PairDesc {
+    float *arena;
+    long long plane;       // floats per plane slot (>= s0 * h0, multiple of 64)
+    const float *in_i0;    // level-0 input images (float)
+    const float *in_i1;
+    const float2 *cv;      // calibration offsets (level 0)
+    int w[kMaxLevels], h[kMaxLevels], s[kMaxLevels];
+    void *I0[kMaxLevels];  // image pyramids (st); in FP32 storage I0[0] == in_i0
+    void *I1[kMaxLevels];
+    const float *mask[kMaxLevels];
+    const float2 *tv[kMaxLevels];
+};
+
+// Which member of each ping-pong pair is current (bit set = slot 1).
+enum RoleBit : int { RB_U = 1, RB_UB = 2, RB_V = 4, RB_PQ = 8, RB_UV = 16 };
+
+// Scalars of one inner iteration (host-computed schedule, stereotgv.cu:449-477).
+struct IterParams {
+    float k1;      // alpha1 * sigma / eta_p
+    float k0;      // alpha0 * sigma / eta_q
+    float tau;
+    float mu;
+    float lambda;
+    float alpha0;
+    float alpha1;
+    int first;     // p and q were just zeroed (stereotgv.cu:431-432): read them as 0
+    int write_tp;  // also store Tp (read later only where a neighbour is outside the mask)
+};
+
+// One batched launch: descriptors + per-level tile prefix table.
+struct Launch {
+    const PairDesc *desc;  // device array [npairs]
+    const int *tiles;      // device array [kMaxLevels][npairs + 1] of tile prefix sums
+    int npairs;
+    int tiles_total[kMaxLevels];
+    int max_plane_floats;  // largest arena plane of the set (for the zero fill grid)
+    hipStream_t stream;
+};
+
+// ---- batched kernels (vfs_kernels.hip) ----
+void launch_zero_arenas(const Launch &L, hipStream_t st);
+void launch_calibrate(const Launch &L, int storage);
+void launch_downscale_images(const Launch &L, int level, int storage);
+void launch_coarsest_init(const Launch &L, int level, int roles);
+void launch_tensor(const Launch &L, int level, int masked, int xlimit_block, float beta, float gamma, int storage);
+void launch_warp_prep(const Launch &L, int level, int roles, int storage);
+void launch_deriv(const Launch &L, int level, int storage);
+void launch_iter(const Launch &L, int level, int masked, int roles, const IterParams &ip, int storage);
+void launch_warp_end(const Launch &L, int level, int masked, int roles, float limit);
+void launch_upscale(const Launch &L, int level, int roles, float scale);
+
+// ---- single-plane kernels (load time, outputs) ----
+struct Dims {
+    int w, h, s;
+};
+void launch_fill_zero(void *ptr, size_t bytes, hipStream_t st);
+void launch_cv8u(const uint8_t *in, int in_pitch, float *out, Dims d, hipStream_t st);
+void launch_cv8uc3(const uint8_t *in, int in_pitch, float *out, Dims d, hipStream_t st);
+void launch_downscale_f2(const float2 *src, Dims fine, float2 *out, Dims coarse, hipStream_t st);
+void launch_downscale_nn(const float *src, Dims fine, float *out, Dims coarse, hipStream_t st);
+void launch_convert_kb(const float2 *uv, double focalx, double focaly, double cx, double cy, double d1,
+                       double d2, double d3, double d4, double t1, double t2, double t3, float *X,
+                       float *depth, Dims d, hipStream_t st);
+void launch_copy(const void *src, void *dst, size_t bytes, hipStream_t st);
+// Copies a storage-typed plane into a float plane (inspection).
+void launch_widen(const void *src, int storage, int nch, float *dst, Dims d, hipStream_t st);
+
+}  // namespace vfs
