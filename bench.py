@@ -36,6 +36,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU-oracle baseline leg")
     ap.add_argument("--iter-reps", type=int, default=200)
     ap.add_argument("--no-graph", action="store_true", help="launch kernels one by one instead of a HIP graph")
+    ap.add_argument("--quick", action="store_true", help="plain form only (no batched / reduced-storage legs)")
+    ap.add_argument("--batch-steps", type=int, default=2)
     ap.add_argument("--iter-only", type=int, default=0,
                     help="profiling aid: run N level-0 inner iterations (after a 256 MiB reference copy) and exit")
     args = ap.parse_args()
@@ -131,6 +133,10 @@ def main():
                      "copy_kernel_GBs": round(copy_gbs, 1)},
     }
 
+    if not args.quick:
+        out["batched"] = measure_batched(args, P, S, vfs_amd)
+        out["reduced_storage"] = measure_reduced(args, P, S, vfs_amd, pair_checker=not args.no_cpu)
+
     if not args.no_cpu:
         from oracle import oracle as O
         threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
@@ -147,6 +153,83 @@ def main():
                                "sample": f"one full {W}x{H} L{wl['levels']} W{wl['warps']} I{wl['iters']} masked "
                                          f"solve of the same pair by the FP64 C oracle (OpenMP), {cpu_s:.2f} s"}
     print(json.dumps(out), flush=True)
+
+
+def _solver(vfs_amd, P, pair, levels, warps, iters, storage="f32", graph=True):
+    H, W = pair["i0"].shape
+    g = vfs_amd.StereoTgv()
+    g.set_option("storage", storage)
+    g.set_option("graph", 1 if graph else 0)
+    g.initialize(W, H, P["beta"], P["gamma"], P["alpha0"], P["alpha1"], P["timestep_lambda"], P["lam"], levels,
+                 P["fScale"], warps, iters)
+    g.limitRange = P["limitRange"]
+    g.copyMaskToDevice(pair["mask"])
+    g.loadVectorFields(pair["tv"], pair["cv"])
+    g.copyImagesToDevice(pair["i0"], pair["i1"])
+    return g
+
+
+def _time_batch(args, b, n_pairs):
+    b.time_solves(True, 1)  # warm-up (graph capture)
+    b.synchronize()
+    t0 = time.perf_counter()
+    ms = b.time_solves(True, args.batch_steps)
+    b.synchronize()
+    wall = time.perf_counter() - t0
+    return {"pairs_per_s": round(n_pairs * args.batch_steps / wall, 2),
+            "ms_per_batch_solve": round(ms / args.batch_steps, 2)}
+
+
+def measure_batched(args, P, S, vfs_amd):
+    """64 pairs of uneven sizes (424x400, 640x480, 848x800; seeds 100+i), all solved by
+    one batch: every kernel launch of the L5 W10 I50 schedule covers all 64 pairs."""
+    sizes = [(424, 400), (640, 480), (848, 800)]
+    pairs, solvers = [], []
+    for i in range(64):
+        W, H = sizes[i % 3]
+        pair = S.make_pair(W, H, 100 + i, H / 2.0, W / 21.2)
+        solvers.append(_solver(vfs_amd, P, pair, 5, 10, 50))
+        pairs.append(pair)
+    print("bench: batched 64 mixed pairs ready", file=sys.stderr, flush=True)
+    b = vfs_amd.Batch(solvers)
+    res = _time_batch(args, b, 64)
+    mpx = sum(p["i0"].size for p in pairs) / 1e6
+    res["Mpx_per_s"] = round(res["pairs_per_s"] * mpx / 64, 2)
+    res["workload"] = "64 pairs, sizes 424x400 / 640x480 / 848x800 in turn, L5 W10 I50, FP32, one batch"
+    b.close()
+    for g in solvers:
+        g.close()
+    return res
+
+
+def measure_reduced(args, P, S, vfs_amd, pair_checker=True):
+    """16 pairs at 848x800 (seeds 200+i) solved as one batch with FP32, FP16 and BF16
+    storage; disparity error of pair 0 against the FP64 oracle per storage type."""
+    pairs = [S.make_pair(848, 800, 200 + i, 400.0, 40.0) for i in range(16)]
+    ref = None
+    if pair_checker:
+        from oracle import oracle as O
+        ref = O.solve_pair(pairs[0], 5, 10, 50, P, precision="f64").copy1DDisparityToHost()
+    m = pairs[0]["mask"] > 0
+    out = {"workload": "16 pairs 848x800, L5 W10 I50, one batch per storage type"}
+    for st in ("f32", "f16", "bf16"):
+        solvers = [_solver(vfs_amd, P, p, 5, 10, 50, storage=st) for p in pairs]
+        b = vfs_amd.Batch(solvers)
+        res = _time_batch(args, b, 16)
+        # bytes per pixel-iteration: FP32 100; 16-bit storage of p, q, a, b, c, Iu, Iz: 66
+        res["bytes_per_px_iter"] = 100 if st == "f32" else 66
+        if ref is not None:
+            solvers[0].copyImagesToDevice(pairs[0]["i0"], pairs[0]["i1"])
+            solvers[0].solveStereoForwardMasked()
+            err = np.abs(solvers[0].copy1DDisparityToHost() - ref)[m]
+            res["mean_err_px"] = float(f"{err.mean():.3e}")
+            res["max_err_px"] = float(f"{err.max():.3e}")
+        out[st] = res
+        print(f"bench: reduced storage {st}: {res}", file=sys.stderr, flush=True)
+        b.close()
+        for g in solvers:
+            g.close()
+    return out
 
 
 if __name__ == "__main__":

@@ -84,6 +84,14 @@ int vfs_solve_masked(vfs_solver *s);
  * unmasked form (circle gate r < width/2.2, no median). */
 int vfs_solve(vfs_solver *s);
 
+/* Warm-started re-solve (no counterpart in the reference, whose solves always restart
+ * from zero, stereotgv.cu:395-397): the previous solve's u, u_ and warpUV are downscaled
+ * to `start_level` (scaled by 1/fScale per level, the scaled Downscale of
+ * downscale.cu:148-237) and the masked (masked != 0) or unmasked solve runs from there
+ * to level 0 with `warps` warps of `iters` iterations per level.  Requires a previous
+ * solve of this handle; new images are normally copied in first. */
+int vfs_solve_warm(vfs_solver *s, int masked, int start_level, int warps, int iters);
+
 /* StereoTgv::copy1DDisparityToHost (stereotgv.h:121, stereotgv.cu:533-540): u, width x height. */
 int vfs_copy_disparity_1d(vfs_solver *s, float *out, int pitch);
 
@@ -154,10 +162,71 @@ typedef struct vfs_batch vfs_batch;
 int vfs_batch_create(vfs_batch **out, vfs_solver *const *solvers, int n);
 int vfs_batch_destroy(vfs_batch *b);
 int vfs_batch_solve(vfs_batch *b, int masked);
+/* vfs_solve_warm for every member (all members solved last by the same program). */
+int vfs_batch_solve_warm(vfs_batch *b, int masked, int start_level, int warps, int iters);
 int vfs_batch_synchronize(vfs_batch *b);
 /* `n` batch solves back to back, HIP events around them; total ms. */
 int vfs_batch_time_solves(vfs_batch *b, int masked, int n, float *ms_total);
 const char *vfs_batch_last_error(const vfs_batch *b);
+
+/* ---- serving: no counterpart in the reference ----
+ * A continuous-batching server for many camera streams on one device.  Each stream keeps
+ * its solver state resident (pyramids, duals, previous disparity) while the HBM budget
+ * allows; its first frame (and the first after an eviction) is a full solve, every later
+ * frame warm-starts from the stream's previous result (vfs_solve_warm semantics).  A
+ * submitted frame is admitted as soon as its stream is idle, an active slot is free and
+ * the stream is resident; each vfs_server_step launches one kernel that advances every
+ * admitted frame by one op of its own schedule, so frames arriving while others are on
+ * the GPU join the running work.  A frame's result does not depend on what else is
+ * running.  Solver parameters are per server; geometry and schedule are per stream. */
+typedef struct vfs_server vfs_server;
+
+typedef struct {
+    size_t hbm_budget_bytes;  /* device memory the resident streams may use */
+    int max_streams;          /* resident streams at most */
+    int max_active;           /* frames in flight at most (1..64) */
+    int storage;              /* 0 FP32, 1 FP16, 2 BF16 (vfs_set_option "storage") */
+    float beta, gamma, alpha0, alpha1, timestep_lambda, lambda, fScale, limitRange;
+} vfs_server_params;
+
+typedef struct {
+    int width, height;
+    int nLevels, nWarpIters, nSolverIters;  /* full (cold) solve schedule */
+    int warm_level, warm_warps, warm_iters; /* warm start schedule */
+    const float *mask;                      /* width*height, copied */
+    const float *tv, *cv;                   /* width*height*2 each, copied */
+} vfs_stream_params;
+
+typedef struct {
+    long long tag;
+    int stream;
+    int warm;           /* 1 if the frame warm-started */
+    int ops;            /* ops of its schedule */
+    double latency_ms;  /* submit -> result in the caller's buffer */
+} vfs_server_result;
+
+typedef struct {
+    long long ticks;    /* kernel launches */
+    long long entries;  /* (frame, op) entries over all launches */
+    size_t hbm_used;
+    int resident_streams;
+} vfs_server_stats_t;
+
+int vfs_server_create(vfs_server **out, const vfs_server_params *params);
+int vfs_server_destroy(vfs_server *v);
+int vfs_server_add_stream(vfs_server *v, const vfs_stream_params *params, int *stream_id);
+/* Queues a frame (float images width*height, copied).  If u_out is not NULL it receives
+ * the 1-D disparity (width*height floats) when the frame completes; it must stay valid
+ * until the frame is collected. */
+int vfs_server_submit(vfs_server *v, int stream_id, const float *i0, const float *i1, float *u_out,
+                      long long tag);
+/* One scheduling step: retire finished frames, admit queued ones, launch one tick (waits
+ * only when 8 ticks are already in flight).  Reports frames in flight and queued. */
+int vfs_server_step(vfs_server *v, int *active, int *queued);
+/* Moves up to `max` finished frames into `out`; *n receives the count. */
+int vfs_server_collect(vfs_server *v, vfs_server_result *out, int max, int *n);
+int vfs_server_stats(const vfs_server *v, vfs_server_stats_t *stats);
+const char *vfs_server_last_error(const vfs_server *v);
 
 #ifdef __cplusplus
 }

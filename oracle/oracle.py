@@ -39,7 +39,7 @@ def lib() -> ctypes.CDLL:
             getattr(_LIB, f"oracle_create_{sfx}").argtypes = (
                 [ctypes.c_int, ctypes.c_int] + [ctypes.c_float] * 6 + [ctypes.c_int, ctypes.c_float, ctypes.c_int,
                                                                        ctypes.c_int, ctypes.c_int, ctypes.c_int])
-            for fn in ("oracle_destroy", "oracle_solve", "oracle_copy_mask", "oracle_load_vector_fields",
+            for fn in ("oracle_destroy", "oracle_solve", "oracle_solve_warm", "oracle_copy_mask", "oracle_load_vector_fields",
                        "oracle_copy_images_f32", "oracle_copy_images_u8", "oracle_copy_images_u8c3",
                        "oracle_set_limit_range", "oracle_get_u", "oracle_get_warpuv", "oracle_get_i1warp",
                        "oracle_read_plane", "oracle_level_dims", "oracle_stereo"):
@@ -114,6 +114,9 @@ class Oracle:
 
     def solveStereoForward(self):
         return self._fn("oracle_solve")(self._h, 0)
+
+    def solve_warm(self, start_level=1, warps=2, iters=20, masked=True):
+        return self._fn("oracle_solve_warm")(self._h, 1 if masked else 0, start_level, warps, iters)
 
     def copy1DDisparityToHost(self):
         out = np.empty((self.height, self.width), self.dt)

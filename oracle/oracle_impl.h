@@ -821,101 +821,153 @@ static void FN(zero_work)(FN(oracle_t) *o)
     }
 }
 
-/* stereotgv.cu:389-515 (masked = 1) and :256-387 (masked = 0) */
-int FN(oracle_solve)(FN(oracle_t) *o, int masked)
+/* One level of the solve, from the tensor to the end of the last warp
+ * (stereotgv.cu:409-494 masked, :275-364 unmasked). */
+static void FN(solve_level)(FN(oracle_t) *o, int masked, int l, int warps, int iters)
 {
-    FN(zero_work)(o);
-    int W0 = o->width, H0 = o->height, S0 = o->stride;
-    /* calibration warp, then swap into the level-0 I1 slot (:392-393) */
-    FN(oracle_k_warp)(o->pI1[0], W0, H0, S0, o->cv, o->i1cal);
+    int w = o->pW[l], h = o->pH[l], s = o->pS[l];
+    const R *mask = o->pMask[l];
+    size_t N = (size_t)o->stride * o->height;
+    memset(o->a, 0, N * sizeof(R)); memset(o->b, 0, N * sizeof(R)); memset(o->c, 0, N * sizeof(R));
+    memset(o->etau, 0, N * sizeof(R)); memset(o->etav1, 0, N * sizeof(R)); memset(o->etav2, 0, N * sizeof(R));
+    memset(o->i0smooth, 0, N * sizeof(R));
+    R eta_p = 3, eta_q = 2;
+    int xlimit = o->blockWidth * ((h + o->blockWidth - 1) / o->blockWidth);
+    if (masked) {
+        FN(oracle_k_gauss)(o->pI0[l], mask, w, h, s, o->i0smooth);
+        FN(oracle_k_tensor)(o->i0smooth, mask, o->beta, o->gamma, w, h, s, xlimit, o->a, o->b, o->c);
+        FN(oracle_k_eta)(mask, o->alpha0, o->alpha1, o->a, o->b, o->c, w, h, s, o->etau, o->etav1, o->etav2);
+    } else {
+        FN(oracle_k_gauss)(o->pI0[l], NULL, w, h, s, o->i0smooth);
+        FN(oracle_k_tensor)(o->i0smooth, NULL, o->beta, o->gamma, w, h, s, xlimit, o->a, o->b, o->c);
+        FN(oracle_k_eta)(NULL, o->alpha0, o->alpha1, o->a, o->b, o->c, w, h, s, o->etau, o->etav1, o->etav2);
+    }
+    for (int wi = 0; wi < warps; wi++) {
+        memset(o->p, 0, 2 * N * sizeof(R)); memset(o->q, 0, 4 * N * sizeof(R));
+        memset(o->v, 0, 2 * N * sizeof(R));
+        FN(oracle_k_clone)(o->v_, 2, w, h, s, o->v);
+        memset(o->gradv, 0, 4 * N * sizeof(R)); memset(o->du, 0, N * sizeof(R));
+        FN(oracle_k_find_tv2)(o->warpUV, o->pTv[l], w, h, s, o->tv2);
+        FN(oracle_k_warp)(o->pI1[l], w, h, s, o->warpUV, o->i1warp);
+        FN(oracle_deriv)(o->pI0[l], o->i1warp, o->pTv[l], w, h, s, o->Iu, o->Iz);
+        FN(oracle_k_clone)(o->u_last, 1, w, h, s, o->u);
+        /* host step schedule in float (:449-477) */
+        float tau = 1.0f, sigma = 1.0f / tau;
+        for (int it = 0; it < iters; it++) {
+            float mu;
+            if (sigma < 1000.0f) mu = 1.0f / sqrtf(1 + 0.7f * tau * (float)o->timestep_lambda);
+            else mu = 1;
+            if (masked) {
+                FN(oracle_k_dual_masked)(mask, o->u_, o->v_, o->alpha0, o->alpha1, (R)sigma, eta_p, eta_q,
+                                         o->a, o->b, o->c, w, h, s, o->gradv, o->p, o->q);
+                FN(oracle_k_tp)(mask, o->a, o->b, o->c, o->p, w, h, s, o->Tp);
+                FN(oracle_k_thresh_masked)(o->Tp, o->u_, o->Iu, o->Iz, mask, o->lambda, (R)tau, o->etau,
+                                           o->u, o->us, w, h, s);
+                SWAP_PTR(o->u, o->us);
+                FN(oracle_k_primal_masked)(mask, o->u_, o->v_, o->p, o->q, o->a, o->b, o->c, (R)tau,
+                                           o->etav1, o->etav2, o->alpha0, o->alpha1, (R)mu, o->u,
+                                           o->v, o->u_s, o->v_s, w, h, s);
+            } else {
+                FN(oracle_k_dual)(o->u_, o->v_, o->alpha0, o->alpha1, (R)sigma, eta_p, eta_q,
+                                  o->a, o->b, o->c, w, h, s, o->gradv, o->p, o->q);
+                FN(oracle_k_tp)(NULL, o->a, o->b, o->c, o->p, w, h, s, o->Tp);
+                FN(oracle_k_thresh)(o->Tp, o->u_, o->Iu, o->Iz, o->lambda, (R)tau, o->etau,
+                                    o->u, o->us, w, h, s);
+                SWAP_PTR(o->u, o->us);
+                FN(oracle_k_primal)(o->u_, o->v_, o->p, o->q, o->a, o->b, o->c, (R)tau,
+                                    o->etav1, o->etav2, o->alpha0, o->alpha1, (R)mu, o->u,
+                                    o->v, o->u_s, o->v_s, w, h, s);
+            }
+            SWAP_PTR(o->u_, o->u_s);
+            SWAP_PTR(o->v_, o->v_s);
+            sigma = sigma / mu;
+            tau = tau * mu;
+        }
+        if (masked) {
+            FN(oracle_k_median5)(o->u, w, h, s, o->us);
+            SWAP_PTR(o->u, o->us);
+        }
+        FN(oracle_k_sub)(o->u, o->u_last, w, h, s, o->du);
+        FN(oracle_k_limit)(o->du, o->limitRange, w, h, s, o->du);
+        FN(oracle_k_add)(o->u_last, o->du, 1, w, h, s, o->u);
+        FN(oracle_k_clone)(o->u_, 1, w, h, s, o->u);
+        FN(oracle_k_flowvec)(o->du, o->tv2, w, h, s, o->dwarpUV);
+        FN(oracle_k_add)(o->warpUV, o->dwarpUV, 2, w, h, s, o->warpUV);
+    }
+}
+
+/* Upscale u, u_ and warpUV from level l to l-1, then swap (stereotgv.cu:497-510). */
+static void FN(upscale_level)(FN(oracle_t) *o, int l)
+{
+    int w = o->pW[l], h = o->pH[l], s = o->pS[l];
+    int W = o->pW[l - 1], H = o->pH[l - 1], S = o->pS[l - 1];
+    FN(oracle_k_upscale)(o->u, 1, w, h, s, W, H, S, o->fScale, o->us);
+    FN(oracle_k_upscale)(o->u_, 1, w, h, s, W, H, S, o->fScale, o->u_s);
+    FN(oracle_k_upscale)(o->warpUV, 2, w, h, s, W, H, S, o->fScale, o->warpUVs);
+    SWAP_PTR(o->u, o->us);
+    SWAP_PTR(o->u_, o->u_s);
+    SWAP_PTR(o->warpUV, o->warpUVs);
+}
+
+/* Calibration warp and image pyramid up to `top` (stereotgv.cu:392-405). */
+static void FN(prepare_images)(FN(oracle_t) *o, int top)
+{
+    FN(oracle_k_warp)(o->pI1[0], o->width, o->height, o->stride, o->cv, o->i1cal);
     SWAP_PTR(o->pI1[0], o->i1cal);
-    /* u, u_, warpUV were zeroed by zero_work (:395-397) */
-    for (int l = 1; l < o->nLevels; l++) {
+    for (int l = 1; l <= top; l++) {
         FN(oracle_k_downscale)(o->pI0[l - 1], 1, o->pW[l - 1], o->pH[l - 1], o->pS[l - 1],
                                o->pW[l], o->pH[l], o->pS[l], o->pI0[l]);
         FN(oracle_k_downscale)(o->pI1[l - 1], 1, o->pW[l - 1], o->pH[l - 1], o->pS[l - 1],
                                o->pW[l], o->pH[l], o->pS[l], o->pI1[l]);
     }
-    size_t N = (size_t)S0 * H0;
+}
+
+/* stereotgv.cu:389-515 (masked = 1) and :256-387 (masked = 0) */
+int FN(oracle_solve)(FN(oracle_t) *o, int masked)
+{
+    FN(zero_work)(o);
+    FN(prepare_images)(o, o->nLevels - 1);
+    /* u, u_, warpUV were zeroed by zero_work (:395-397) */
     for (int l = o->nLevels - 1; l >= 0; l--) {
+        if (l == o->nLevels - 1)
+            FN(oracle_k_flowvec)(o->u, o->pTv[l], o->pW[l], o->pH[l], o->pS[l], o->warpUV);
+        FN(solve_level)(o, masked, l, o->nWarpIters, o->nSolverIters);
+        if (l > 0) FN(upscale_level)(o, l);
+    }
+    return 0;
+}
+
+/* Warm-started re-solve (no reference counterpart; the serving schedule): the previous
+ * solve's u, u_ and warpUV are downscaled level by level to `start_level` with the
+ * 4-tap Downscale times 1/fScale (downscale.cu:148-237) into the ping-pong partners,
+ * then the solve runs from start_level to level 0 with `warps` x `iters`.  Nothing is
+ * zeroed. */
+int FN(oracle_solve_warm)(FN(oracle_t) *o, int masked, int start_level, int warps, int iters)
+{
+    if (start_level < 0 || start_level >= o->nLevels) return -1;
+    FN(prepare_images)(o, start_level);
+    R sc = (R)(1.0f / (float)o->fScale);
+    for (int l = 1; l <= start_level; l++) {
+        int W = o->pW[l - 1], H = o->pH[l - 1], S = o->pS[l - 1];
         int w = o->pW[l], h = o->pH[l], s = o->pS[l];
-        const R *mask = o->pMask[l];
-        memset(o->a, 0, N * sizeof(R)); memset(o->b, 0, N * sizeof(R)); memset(o->c, 0, N * sizeof(R));
-        memset(o->etau, 0, N * sizeof(R)); memset(o->etav1, 0, N * sizeof(R)); memset(o->etav2, 0, N * sizeof(R));
-        memset(o->i0smooth, 0, N * sizeof(R));
-        R eta_p = 3, eta_q = 2;
-        if (l == o->nLevels - 1) FN(oracle_k_flowvec)(o->u, o->pTv[l], w, h, s, o->warpUV);
-        int xlimit = o->blockWidth * ((h + o->blockWidth - 1) / o->blockWidth);
-        if (masked) {
-            FN(oracle_k_gauss)(o->pI0[l], mask, w, h, s, o->i0smooth);
-            FN(oracle_k_tensor)(o->i0smooth, mask, o->beta, o->gamma, w, h, s, xlimit, o->a, o->b, o->c);
-            FN(oracle_k_eta)(mask, o->alpha0, o->alpha1, o->a, o->b, o->c, w, h, s, o->etau, o->etav1, o->etav2);
-        } else {
-            FN(oracle_k_gauss)(o->pI0[l], NULL, w, h, s, o->i0smooth);
-            FN(oracle_k_tensor)(o->i0smooth, NULL, o->beta, o->gamma, w, h, s, xlimit, o->a, o->b, o->c);
-            FN(oracle_k_eta)(NULL, o->alpha0, o->alpha1, o->a, o->b, o->c, w, h, s, o->etau, o->etav1, o->etav2);
-        }
-        for (int wi = 0; wi < o->nWarpIters; wi++) {
-            memset(o->p, 0, 2 * N * sizeof(R)); memset(o->q, 0, 4 * N * sizeof(R));
-            memset(o->v, 0, 2 * N * sizeof(R));
-            FN(oracle_k_clone)(o->v_, 2, w, h, s, o->v);
-            memset(o->gradv, 0, 4 * N * sizeof(R)); memset(o->du, 0, N * sizeof(R));
-            FN(oracle_k_find_tv2)(o->warpUV, o->pTv[l], w, h, s, o->tv2);
-            FN(oracle_k_warp)(o->pI1[l], w, h, s, o->warpUV, o->i1warp);
-            FN(oracle_deriv)(o->pI0[l], o->i1warp, o->pTv[l], w, h, s, o->Iu, o->Iz);
-            FN(oracle_k_clone)(o->u_last, 1, w, h, s, o->u);
-            /* host step schedule in float (:449-477) */
-            float tau = 1.0f, sigma = 1.0f / tau;
-            for (int it = 0; it < o->nSolverIters; it++) {
-                float mu;
-                if (sigma < 1000.0f) mu = 1.0f / sqrtf(1 + 0.7f * tau * (float)o->timestep_lambda);
-                else mu = 1;
-                if (masked) {
-                    FN(oracle_k_dual_masked)(mask, o->u_, o->v_, o->alpha0, o->alpha1, (R)sigma, eta_p, eta_q,
-                                             o->a, o->b, o->c, w, h, s, o->gradv, o->p, o->q);
-                    FN(oracle_k_tp)(mask, o->a, o->b, o->c, o->p, w, h, s, o->Tp);
-                    FN(oracle_k_thresh_masked)(o->Tp, o->u_, o->Iu, o->Iz, mask, o->lambda, (R)tau, o->etau,
-                                               o->u, o->us, w, h, s);
-                    SWAP_PTR(o->u, o->us);
-                    FN(oracle_k_primal_masked)(mask, o->u_, o->v_, o->p, o->q, o->a, o->b, o->c, (R)tau,
-                                               o->etav1, o->etav2, o->alpha0, o->alpha1, (R)mu, o->u,
-                                               o->v, o->u_s, o->v_s, w, h, s);
-                } else {
-                    FN(oracle_k_dual)(o->u_, o->v_, o->alpha0, o->alpha1, (R)sigma, eta_p, eta_q,
-                                      o->a, o->b, o->c, w, h, s, o->gradv, o->p, o->q);
-                    FN(oracle_k_tp)(NULL, o->a, o->b, o->c, o->p, w, h, s, o->Tp);
-                    FN(oracle_k_thresh)(o->Tp, o->u_, o->Iu, o->Iz, o->lambda, (R)tau, o->etau,
-                                        o->u, o->us, w, h, s);
-                    SWAP_PTR(o->u, o->us);
-                    FN(oracle_k_primal)(o->u_, o->v_, o->p, o->q, o->a, o->b, o->c, (R)tau,
-                                        o->etav1, o->etav2, o->alpha0, o->alpha1, (R)mu, o->u,
-                                        o->v, o->u_s, o->v_s, w, h, s);
-                }
-                SWAP_PTR(o->u_, o->u_s);
-                SWAP_PTR(o->v_, o->v_s);
-                sigma = sigma / mu;
-                tau = tau * mu;
+        FN(oracle_k_downscale)(o->u, 1, W, H, S, w, h, s, o->us);
+        FN(oracle_k_downscale)(o->u_, 1, W, H, S, w, h, s, o->u_s);
+        FN(oracle_k_downscale)(o->warpUV, 2, W, H, S, w, h, s, o->warpUVs);
+        for (int y = 0; y < h; y++)
+            for (int x = 0; x < w; x++) {
+                size_t p = (size_t)y * s + x;
+                o->us[p] *= sc;
+                o->u_s[p] *= sc;
+                o->warpUVs[2 * p] *= sc;
+                o->warpUVs[2 * p + 1] *= sc;
             }
-            if (masked) {
-                FN(oracle_k_median5)(o->u, w, h, s, o->us);
-                SWAP_PTR(o->u, o->us);
-            }
-            FN(oracle_k_sub)(o->u, o->u_last, w, h, s, o->du);
-            FN(oracle_k_limit)(o->du, o->limitRange, w, h, s, o->du);
-            FN(oracle_k_add)(o->u_last, o->du, 1, w, h, s, o->u);
-            FN(oracle_k_clone)(o->u_, 1, w, h, s, o->u);
-            FN(oracle_k_flowvec)(o->du, o->tv2, w, h, s, o->dwarpUV);
-            FN(oracle_k_add)(o->warpUV, o->dwarpUV, 2, w, h, s, o->warpUV);
-        }
-        if (l > 0) {
-            int W = o->pW[l - 1], H = o->pH[l - 1], S = o->pS[l - 1];
-            FN(oracle_k_upscale)(o->u, 1, w, h, s, W, H, S, o->fScale, o->us);
-            FN(oracle_k_upscale)(o->u_, 1, w, h, s, W, H, S, o->fScale, o->u_s);
-            FN(oracle_k_upscale)(o->warpUV, 2, w, h, s, W, H, S, o->fScale, o->warpUVs);
-            SWAP_PTR(o->u, o->us);
-            SWAP_PTR(o->u_, o->u_s);
-            SWAP_PTR(o->warpUV, o->warpUVs);
-        }
+        SWAP_PTR(o->u, o->us);
+        SWAP_PTR(o->u_, o->u_s);
+        SWAP_PTR(o->warpUV, o->warpUVs);
+    }
+    for (int l = start_level; l >= 0; l--) {
+        FN(solve_level)(o, masked, l, warps, iters);
+        if (l > 0) FN(upscale_level)(o, l);
     }
     return 0;
 }

@@ -12,7 +12,7 @@ import os
 
 import numpy as np
 
-__all__ = ["StereoTgv", "Batch", "VfsError", "lib", "lib_path", "EXPORTS", "STORAGE"]
+__all__ = ["StereoTgv", "Batch", "Server", "VfsError", "lib", "lib_path", "EXPORTS", "STORAGE"]
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 _LIB = None
@@ -37,6 +37,7 @@ EXPORTS = [
     ("vfs_copy_images_u8c3", ctypes.c_int, [ctypes.c_void_p, _c_u8_p, _c_u8_p, ctypes.c_int]),
     ("vfs_solve_masked", ctypes.c_int, [ctypes.c_void_p]),
     ("vfs_solve", ctypes.c_int, [ctypes.c_void_p]),
+    ("vfs_solve_warm", ctypes.c_int, [ctypes.c_void_p] + [ctypes.c_int] * 4),
     ("vfs_copy_disparity_1d", ctypes.c_int, [ctypes.c_void_p, _c_float_p, ctypes.c_int]),
     ("vfs_copy_flow", ctypes.c_int, [ctypes.c_void_p, _c_float_p, ctypes.c_int]),
     ("vfs_copy_stereo", ctypes.c_int, [ctypes.c_void_p, _c_float_p, ctypes.c_int, _c_float_p, ctypes.c_int]
@@ -54,10 +55,42 @@ EXPORTS = [
     ("vfs_batch_create", ctypes.c_int, [ctypes.POINTER(ctypes.c_void_p), ctypes.POINTER(ctypes.c_void_p), ctypes.c_int]),
     ("vfs_batch_destroy", ctypes.c_int, [ctypes.c_void_p]),
     ("vfs_batch_solve", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
+    ("vfs_batch_solve_warm", ctypes.c_int, [ctypes.c_void_p] + [ctypes.c_int] * 4),
     ("vfs_batch_synchronize", ctypes.c_int, [ctypes.c_void_p]),
     ("vfs_batch_time_solves", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, _c_float_p]),
     ("vfs_batch_last_error", ctypes.c_char_p, [ctypes.c_void_p]),
+    ("vfs_server_create", ctypes.c_int, [ctypes.POINTER(ctypes.c_void_p), ctypes.c_void_p]),
+    ("vfs_server_destroy", ctypes.c_int, [ctypes.c_void_p]),
+    ("vfs_server_add_stream", ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, _c_int_p]),
+    ("vfs_server_submit", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, _c_float_p, _c_float_p, _c_float_p,
+                                         ctypes.c_longlong]),
+    ("vfs_server_step", ctypes.c_int, [ctypes.c_void_p, _c_int_p, _c_int_p]),
+    ("vfs_server_collect", ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, _c_int_p]),
+    ("vfs_server_stats", ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p]),
+    ("vfs_server_last_error", ctypes.c_char_p, [ctypes.c_void_p]),
 ]
+
+
+class _ServerParams(ctypes.Structure):
+    _fields_ = [("hbm_budget_bytes", ctypes.c_size_t), ("max_streams", ctypes.c_int), ("max_active", ctypes.c_int),
+                ("storage", ctypes.c_int)] + [(n, ctypes.c_float) for n in (
+                    "beta", "gamma", "alpha0", "alpha1", "timestep_lambda", "lambda_", "fScale", "limitRange")]
+
+
+class _StreamParams(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_int) for n in ("width", "height", "nLevels", "nWarpIters", "nSolverIters",
+                                            "warm_level", "warm_warps", "warm_iters")] + [
+        ("mask", _c_float_p), ("tv", _c_float_p), ("cv", _c_float_p)]
+
+
+class _ServerResult(ctypes.Structure):
+    _fields_ = [("tag", ctypes.c_longlong), ("stream", ctypes.c_int), ("warm", ctypes.c_int), ("ops", ctypes.c_int),
+                ("latency_ms", ctypes.c_double)]
+
+
+class _ServerStats(ctypes.Structure):
+    _fields_ = [("ticks", ctypes.c_longlong), ("entries", ctypes.c_longlong), ("hbm_used", ctypes.c_size_t),
+                ("resident_streams", ctypes.c_int)]
 
 STORAGE = {"f32": 0, "fp32": 0, "f16": 1, "fp16": 1, "bf16": 2}
 
@@ -184,6 +217,11 @@ class StereoTgv:
     def solveStereoForward(self):
         return self._check(self._L.vfs_solve(self._h), "solveStereoForward")
 
+    def solve_warm(self, start_level: int = 1, warps: int = 2, iters: int = 20, masked: bool = True):
+        """Warm-started re-solve from this handle's previous result (serving path)."""
+        return self._check(self._L.vfs_solve_warm(self._h, 1 if masked else 0, start_level, warps, iters),
+                           "solve_warm")
+
     def copy1DDisparityToHost(self) -> np.ndarray:
         out = np.empty((self.height, self.width), np.float32)
         self._check(self._L.vfs_copy_disparity_1d(self._h, _fptr(out), self.width), "copy1DDisparityToHost")
@@ -277,6 +315,10 @@ class Batch:
     def solve(self, masked: bool = True):
         return self._check(self._L.vfs_batch_solve(self._h, 1 if masked else 0), "batch_solve")
 
+    def solve_warm(self, start_level: int = 1, warps: int = 2, iters: int = 20, masked: bool = True):
+        return self._check(self._L.vfs_batch_solve_warm(self._h, 1 if masked else 0, start_level, warps, iters),
+                           "batch_solve_warm")
+
     def synchronize(self):
         return self._check(self._L.vfs_batch_synchronize(self._h), "batch_synchronize")
 
@@ -288,6 +330,99 @@ class Batch:
     def close(self):
         if getattr(self, "_h", None):
             self._L.vfs_batch_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Server:
+    """Continuous-batching server for camera streams (vfs_server_* of include/vfs.h)."""
+
+    def __init__(self, params: dict, hbm_budget_bytes: int = 64 << 30, max_streams: int = 64, max_active: int = 64,
+                 storage="f32"):
+        self._L = lib()
+        sp = _ServerParams()
+        sp.hbm_budget_bytes = int(hbm_budget_bytes)
+        sp.max_streams = max_streams
+        sp.max_active = max_active
+        sp.storage = STORAGE[storage] if isinstance(storage, str) else int(storage)
+        for k_py, k_c in (("beta", "beta"), ("gamma", "gamma"), ("alpha0", "alpha0"), ("alpha1", "alpha1"),
+                          ("timestep_lambda", "timestep_lambda"), ("lam", "lambda_"), ("fScale", "fScale"),
+                          ("limitRange", "limitRange")):
+            setattr(sp, k_c, float(params[k_py]))
+        h = ctypes.c_void_p()
+        rc = self._L.vfs_server_create(ctypes.byref(h), ctypes.byref(sp))
+        if rc != 0:
+            raise VfsError(f"vfs_server_create failed ({rc}): {self._L.vfs_server_last_error(None).decode()}")
+        self._h = h
+        self._keep = {}  # result buffers by tag
+        self.shapes = {}
+
+    def _check(self, rc, what):
+        if rc < 0:
+            raise VfsError(f"{what} failed ({rc}): {self._L.vfs_server_last_error(self._h).decode()}")
+        return rc
+
+    def add_stream(self, mask, tv, cv, levels, warps, iters, warm_level=1, warm_warps=2, warm_iters=20) -> int:
+        m = np.ascontiguousarray(mask, np.float32)
+        t = np.ascontiguousarray(tv, np.float32)
+        c = np.ascontiguousarray(cv, np.float32)
+        H, W = m.shape
+        p = _StreamParams(W, H, levels, warps, iters, warm_level, warm_warps, warm_iters, _fptr(m), _fptr(t), _fptr(c))
+        sid = ctypes.c_int()
+        self._check(self._L.vfs_server_add_stream(self._h, ctypes.byref(p), ctypes.byref(sid)), "add_stream")
+        self.shapes[sid.value] = (H, W)
+        return sid.value
+
+    def submit(self, stream: int, i0, i1, tag: int, want_result: bool = True):
+        a = np.ascontiguousarray(i0, np.float32)
+        b = np.ascontiguousarray(i1, np.float32)
+        out = np.empty(self.shapes[stream], np.float32) if want_result else None
+        if out is not None:
+            self._keep[tag] = out
+        self._check(self._L.vfs_server_submit(self._h, stream, _fptr(a), _fptr(b),
+                                              _fptr(out) if out is not None else None, tag), "submit")
+
+    def step(self):
+        act, q = ctypes.c_int(), ctypes.c_int()
+        self._check(self._L.vfs_server_step(self._h, ctypes.byref(act), ctypes.byref(q)), "step")
+        return act.value, q.value
+
+    def collect(self, max_n: int = 256):
+        buf = (_ServerResult * max_n)()
+        n = ctypes.c_int()
+        self._check(self._L.vfs_server_collect(self._h, buf, max_n, ctypes.byref(n)), "collect")
+        out = []
+        for r in buf[: n.value]:
+            out.append(dict(tag=r.tag, stream=r.stream, warm=r.warm, ops=r.ops, latency_ms=r.latency_ms,
+                            u=self._keep.pop(r.tag, None)))
+        return out
+
+    def stats(self):
+        st = _ServerStats()
+        self._check(self._L.vfs_server_stats(self._h, ctypes.byref(st)), "stats")
+        return dict(ticks=st.ticks, entries=st.entries, hbm_used=st.hbm_used, resident_streams=st.resident_streams)
+
+    def run_until_idle(self, timeout_s: float = 120.0):
+        import time
+        t0 = time.perf_counter()
+        results = []
+        while True:
+            act, q = self.step()
+            results += self.collect()
+            if act == 0 and q == 0:
+                results += self.collect()
+                return results
+            if time.perf_counter() - t0 > timeout_s:
+                raise VfsError("server did not drain in time")
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self._L.vfs_server_destroy(self._h)
             self._h = None
 
     def __del__(self):

@@ -179,29 +179,6 @@ struct Tile {
     int x0, y0, w, h, s;
 };
 
-// Workgroup -> (pair, 64x8 tile) at `level` through the tile prefix table.
-__device__ __forceinline__ Tile tile_of(const PairDesc *__restrict__ D, const int *__restrict__ ts, int n,
-                                        int level)
-{
-    const int b = blockIdx.x;
-    int lo = 0, hi = n - 1;
-    while (lo < hi) {  // skipped for a single pair: no table load on the critical path
-        const int mid = (lo + hi + 1) >> 1;
-        if (ts[mid] <= b) lo = mid;
-        else hi = mid - 1;
-    }
-    Tile t;
-    t.d = D + lo;
-    t.w = t.d->w[level];
-    t.h = t.d->h[level];
-    t.s = t.d->s[level];
-    const int local = n == 1 ? b : b - ts[lo];
-    const int ntx = (t.w + kTileW - 1) / kTileW;
-    t.x0 = (local % ntx) * kTileW;
-    t.y0 = (local / ntx) * kTileH;
-    return t;
-}
-
 __device__ __forceinline__ float *slot(const PairDesc *d, int k) { return d->arena + (long long)k * d->plane; }
 
 template <typename P>
@@ -217,23 +194,27 @@ __device__ __forceinline__ P *slotp(const PairDesc *d, int k)
 
 inline dim3 block2d() { return dim3(BX, BY); }
 
-// ------------------------------------------------------------------ solve kernels
+// ------------------------------------------------------------------ solve ops
+//
+// Each op works on one 64x8 tile of one pair; k_tick (below) dispatches them.
 
-__global__ void k_zero_arenas(const PairDesc *__restrict__ D)
+// Fresh state: zero every work plane of the pair at the tile's level-0 positions
+// (the tile grid spans the stride, so row padding is cleared too).
+__device__ __forceinline__ void op_zero(const Tile &t)
 {
-    const PairDesc *d = D + blockIdx.y;
-    float4 *p = reinterpret_cast<float4 *>(d->arena);
-    const long long n4 = (long long)PL_COUNT * d->plane / 4;
-    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n4; i += (long long)gridDim.x * blockDim.x)
-        p[i] = make_float4(0.f, 0.f, 0.f, 0.f);
+    FOR_TILE_PIXELS(t, x, y)
+    {
+        const long pos = (long)y * t.s + x;
+#pragma unroll 5
+        for (int k = 0; k < PL_COUNT; k++) slot(t.d, k)[pos] = 0.f;
+    }
 }
 
 // Calibration warp of I1 (stereotgv.cu:392-393, warpImage.cu:9-23) into the level-0
 // pyramid slot; with 16-bit storage also narrows I0 into its pyramid slot.
 template <typename T>
-__global__ __launch_bounds__(NT) void k_calibrate(const PairDesc *__restrict__ D, const int *__restrict__ ts, int n)
+__device__ __forceinline__ void op_calib(const Tile &t)
 {
-    const Tile t = tile_of(D, ts, n, 0);
     const float *i1 = t.d->in_i1;
     T *o1 = (T *)t.d->I1[0];
     FOR_TILE_PIXELS(t, x, y)
@@ -248,10 +229,8 @@ __global__ __launch_bounds__(NT) void k_calibrate(const PairDesc *__restrict__ D
 // downscale.cu:10-30 for I0 and I1 (stereotgv.cu:400-405): mean of 4 bilinear taps at
 // +-1/4 coarse pixel around the coarse pixel centre, fetched from the finer level.
 template <typename T>
-__global__ __launch_bounds__(NT) void k_downscale_images(const PairDesc *__restrict__ D, const int *__restrict__ ts,
-                                                         int n, int level)
+__device__ __forceinline__ void op_down(const Tile &t, int level)
 {
-    const Tile t = tile_of(D, ts, n, level);
     const int W = t.d->w[level - 1], H = t.d->h[level - 1], S = t.d->s[level - 1];
     const T *a = (const T *)t.d->I0[level - 1];
     const T *b = (const T *)t.d->I1[level - 1];
@@ -271,10 +250,8 @@ __global__ __launch_bounds__(NT) void k_downscale_images(const PairDesc *__restr
 }
 
 // Coarsest-level init warpUV = u * tv (stereotgv.cu:419-421, warpImage.cu:136-150).
-__global__ __launch_bounds__(NT) void k_coarsest_init(const PairDesc *__restrict__ D, const int *__restrict__ ts,
-                                                      int n, int level, int roles)
+__device__ __forceinline__ void op_cinit(const Tile &t, int level, int roles)
 {
-    const Tile t = tile_of(D, ts, n, level);
     const float *u = slot(t.d, (roles & RB_U) ? PL_U1 : PL_U0);
     float2 *uv = slotp<float2>(t.d, (roles & RB_UV) ? PL_UV1 : PL_UV0);
     const float2 *tv = t.d->tv[level];
@@ -284,6 +261,34 @@ __global__ __launch_bounds__(NT) void k_coarsest_init(const PairDesc *__restrict
         const float us = u[pos];
         const float2 v = tv[pos];
         uv[pos] = make_float2(us * v.x, us * v.y);
+    }
+}
+
+// Warm start: u, u_ and warpUV of the finer level downscaled to `level` and multiplied
+// by `scale` (= 1/fScale) into the other slot of each pair: the 4-tap Downscale with a
+// scale factor of downscale.cu:148-237.
+__device__ __forceinline__ void op_winit(const Tile &t, int level, int roles, float scale)
+{
+    const int W = t.d->w[level - 1], H = t.d->h[level - 1], S = t.d->s[level - 1];
+    const float *u = slot(t.d, (roles & RB_U) ? PL_U1 : PL_U0);
+    float *ou = slot(t.d, (roles & RB_U) ? PL_U0 : PL_U1);
+    const float *u_ = slot(t.d, (roles & RB_UB) ? PL_UB1 : PL_UB0);
+    float *ou_ = slot(t.d, (roles & RB_UB) ? PL_UB0 : PL_UB1);
+    const float2 *uv = slotp<float2>(t.d, (roles & RB_UV) ? PL_UV1 : PL_UV0);
+    float2 *ouv = slotp<float2>(t.d, (roles & RB_UV) ? PL_UV0 : PL_UV1);
+    const float rx = (float)W / (float)t.w, ry = (float)H / (float)t.h;
+    const float ox = 0.25f * rx, oy = 0.25f * ry;
+    FOR_TILE_PIXELS(t, x, y)
+    {
+        const float cx = ((float)x + 0.5f) * rx - 0.5f, cy = ((float)y + 0.5f) * ry - 0.5f;
+        const long pos = (long)y * t.s + x;
+        ou[pos] = scale * 0.25f * (bil(u, W, H, S, cx - ox, cy) + bil(u, W, H, S, cx + ox, cy) +
+                                   bil(u, W, H, S, cx, cy - oy) + bil(u, W, H, S, cx, cy + oy));
+        ou_[pos] = scale * 0.25f * (bil(u_, W, H, S, cx - ox, cy) + bil(u_, W, H, S, cx + ox, cy) +
+                                    bil(u_, W, H, S, cx, cy - oy) + bil(u_, W, H, S, cx, cy + oy));
+        const float2 v0 = bil2(uv, W, H, S, cx - ox, cy), v1 = bil2(uv, W, H, S, cx + ox, cy);
+        const float2 v2 = bil2(uv, W, H, S, cx, cy - oy), v3 = bil2(uv, W, H, S, cx, cy + oy);
+        ouv[pos] = make_float2(scale * 0.25f * (v0.x + v1.x + v2.x + v3.x), scale * 0.25f * (v0.y + v1.y + v2.y + v3.y));
     }
 }
 
@@ -320,10 +325,8 @@ __device__ __forceinline__ float smoothed(const T *__restrict__ I, const float *
 // or columns x >= xlimit: calcTensor.cu:81,227 size the launch by the HEIGHT).  The
 // preconditioners eta (preconditioning.cu) are recomputed from (a, b, c) where needed.
 template <typename T, bool MASKED>
-__global__ __launch_bounds__(NT) void k_tensor(const PairDesc *__restrict__ D, const int *__restrict__ ts, int n,
-                                               int level, int xblock, float beta, float gamma)
+__device__ __forceinline__ void op_tensor(const Tile &t, int level, int xblock, float beta, float gamma)
 {
-    const Tile t = tile_of(D, ts, n, level);
     const T *I0 = (const T *)t.d->I0[level];
     const float *mask = t.d->mask[level];
     T *at = (T *)slot(t.d, PL_A);
@@ -364,10 +367,8 @@ __global__ __launch_bounds__(NT) void k_tensor(const PairDesc *__restrict__ D, c
 // (warpImage.cu:96-111), I1w = bilinear(I1, x + warpUV) (warpImage.cu:9-23),
 // u_last = u (copyClone.cu:3-12), v_ = 0 (Clone of the zeroed v).
 template <typename T>
-__global__ __launch_bounds__(NT) void k_warp_prep(const PairDesc *__restrict__ D, const int *__restrict__ ts, int n,
-                                                  int level, int roles)
+__device__ __forceinline__ void op_prep(const Tile &t, int level, int roles)
 {
-    const Tile t = tile_of(D, ts, n, level);
     const float2 *uv = slotp<float2>(t.d, (roles & RB_UV) ? PL_UV1 : PL_UV0);
     const float *u = slot(t.d, (roles & RB_U) ? PL_U1 : PL_U0);
     float2 *v_ = slotp<float2>(t.d, (roles & RB_V) ? PL_V1 : PL_V0);
@@ -391,10 +392,8 @@ __global__ __launch_bounds__(NT) void k_warp_prep(const PairDesc *__restrict__ D
 // computeDerivatives.cu:88-129: 5-point derivative along t = tv/|tv| of I0 and I1w,
 // averaged; Iz = I1w - I0 at the texel centre.
 template <typename T>
-__global__ __launch_bounds__(NT) void k_deriv(const PairDesc *__restrict__ D, const int *__restrict__ ts, int n,
-                                              int level)
+__device__ __forceinline__ void op_deriv(const Tile &t, int level)
 {
-    const Tile t = tile_of(D, ts, n, level);
     const T *I0 = (const T *)t.d->I0[level];
     const T *I1w = (const T *)slot(t.d, PL_I1W);
     const float2 *tv = t.d->tv[level];
@@ -444,15 +443,6 @@ __device__ __forceinline__ float thresh_core(float divp, float tau, float etau, 
     return upos + du;
 }
 
-// One TGV-L1 inner iteration in one launch: dual ascent + projection
-// (updateDualVariables.cu:122-237 / :4-101), Tp (thresholdingL1.cu:234-244), L1
-// thresholding (thresholdingL1.cu:128-216 / :3-86) and the primal step with
-// over-relaxation (updatePrimalVariables.cu:138-245 / :3-113).
-// Phase A computes the new p, q and Tp on the 64x8 tile plus its left column and top row
-// (the threshold and primal of a pixel read Tp and q of its left and upper neighbours)
-// into LDS; phase B finishes thresholding and the primal step.  Old duals are read from
-// one buffer and new ones written to the other, so every workgroup recomputes its halo
-// from the previous iteration's values (the reference's Jacobi order).
 constexpr int EXW = kTileW + 1, EXH = kTileH + 1;
 constexpr int HALO = EXW + kTileH;  // top row (incl. corner) + left column positions
 
@@ -569,13 +559,9 @@ __device__ __forceinline__ void dual_step(const DualIn &r, int x, int y, int w, 
 // workgroup recomputes its halo from the previous iteration's values (the reference's
 // Jacobi order).  Every load is issued before any data-dependent branch.
 template <bool MASKED, typename T>
-__global__ __launch_bounds__(NT) void k_iter(const PairDesc *__restrict__ D, const int *__restrict__ ts, int n,
-                                             int level, int roles, IterParams ip)
+__device__ __forceinline__ void op_iter(const Tile &t, int level, int roles, const IterParams &ip, float2 (*sP)[EXW],
+                                        float4 (*sQ)[EXW], float2 (*sT)[EXW])
 {
-    __shared__ float2 sP[EXH][EXW];
-    __shared__ float4 sQ[EXH][EXW];
-    __shared__ float2 sT[EXH][EXW];
-    const Tile t = tile_of(D, ts, n, level);
     const int w = t.w, h = t.h, s = t.s, x0 = t.x0, y0 = t.y0;
     const PairDesc *d = t.d;
     const float *__restrict__ mask = d->mask[level];
@@ -794,10 +780,8 @@ __device__ __forceinline__ float median5_at(const float *__restrict__ u, int x, 
 // (scalar.cu:129-141, limitRange.cu:3-24), u = u_last + du (scalar.cu:77-89),
 // u_ = u (copyClone.cu:3-12), warpUV += du * tv2 (warpImage.cu:136-150, scalar.cu:102-115).
 template <bool MASKED>
-__global__ __launch_bounds__(NT) void k_warp_end(const PairDesc *__restrict__ D, const int *__restrict__ ts, int n,
-                                                 int level, int roles, float limit)
+__device__ __forceinline__ void op_end(const Tile &t, int roles, float limit)
 {
-    const Tile t = tile_of(D, ts, n, level);
     const float *u = slot(t.d, (roles & RB_U) ? PL_U1 : PL_U0);
     float *out = MASKED ? slot(t.d, (roles & RB_U) ? PL_U0 : PL_U1) : (float *)u;
     float *u_ = slot(t.d, (roles & RB_UB) ? PL_UB1 : PL_UB0);
@@ -822,10 +806,8 @@ __global__ __launch_bounds__(NT) void k_warp_end(const PairDesc *__restrict__ D,
 
 // upscale.cu:7-21, 46-61 for u, u_ and warpUV (stereotgv.cu:497-510): bilinear at the
 // finer level's pixel centres, times fScale; writes the other slot of each pair.
-__global__ __launch_bounds__(NT) void k_upscale(const PairDesc *__restrict__ D, const int *__restrict__ ts, int n,
-                                                int level, int roles, float scale)
-{
-    const Tile t = tile_of(D, ts, n, level - 1);  // tiles of the finer level
+__device__ __forceinline__ void op_up(const Tile &t, int level, int roles, float scale)
+{  // the tile belongs to the finer level level - 1
     const int w = t.d->w[level], h = t.d->h[level], s = t.d->s[level];
     const float *u = slot(t.d, (roles & RB_U) ? PL_U1 : PL_U0);
     float *ou = slot(t.d, (roles & RB_U) ? PL_U0 : PL_U1);
@@ -842,6 +824,96 @@ __global__ __launch_bounds__(NT) void k_upscale(const PairDesc *__restrict__ D, 
         ou_[pos] = bil(u_, w, h, s, px, py) * scale;
         const float2 v = bil2(uv, w, h, s, px, py);
         ouv[pos] = make_float2(v.x * scale, v.y * scale);
+    }
+}
+
+
+// ------------------------------------------------------------------ the tick kernel
+//
+// One launch executes one op for each entry of `A`: entry i owns workgroups
+// [A.e[i].tile_start, A.e[i+1].tile_start).  A batched solve passes the same op for every
+// pair; the serving loop passes each active request's next op, so requests at different
+// points of their schedules share launches.  Each pair's result depends only on its own
+// entries, never on what runs beside it.
+template <typename T>
+__global__ __launch_bounds__(NT) void k_tick(const PairDesc *__restrict__ D, const TickArgs A)
+{
+    __shared__ float2 sP[EXH][EXW];
+    __shared__ float4 sQ[EXH][EXW];
+    __shared__ float2 sT[EXH][EXW];
+    // Workgroups are dealt to the 8 XCDs round-robin; renumber them so that each XCD
+    // owns one contiguous run of tiles (whole bands of rows), keeping the halo rows that
+    // neighbouring tiles share in the same L2.
+    const int nb = gridDim.x, i = blockIdx.x, xcd = i & 7, q = nb >> 3, r = nb & 7;
+    const int b = (xcd < r ? xcd * (q + 1) : r * (q + 1) + (xcd - r) * q) + (i >> 3);
+    int lo = 0, hi = A.n - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (A.e[mid].tile_start <= b) lo = mid;
+        else hi = mid - 1;
+    }
+    const TickEntry &E = A.e[lo];
+    const int kind = E.kind, level = E.level;
+    const int tl = kind == OP_UP ? level - 1 : level;
+    Tile t;
+    t.d = D + E.pair;
+    t.w = kind == OP_ZERO ? t.d->s[0] : t.d->w[tl];
+    t.h = t.d->h[tl];
+    t.s = t.d->s[tl];
+    const int local = b - E.tile_start;
+    const int ntx = (t.w + kTileW - 1) / kTileW;
+    t.x0 = (local % ntx) * kTileW;
+    t.y0 = (local / ntx) * kTileH;
+    switch (kind) {
+    case OP_ZERO:
+        op_zero(t);
+        break;
+    case OP_CALIB:
+        op_calib<T>(t);
+        break;
+    case OP_DOWN:
+        op_down<T>(t, level);
+        break;
+    case OP_CINIT:
+        op_cinit(t, level, E.roles);
+        break;
+    case OP_WINIT:
+        op_winit(t, level, E.roles, A.scale);
+        break;
+    case OP_TENSOR:
+        if (E.masked) op_tensor<T, true>(t, level, A.xblock, A.beta, A.gamma);
+        else op_tensor<T, false>(t, level, A.xblock, A.beta, A.gamma);
+        break;
+    case OP_PREP:
+        op_prep<T>(t, level, E.roles);
+        break;
+    case OP_DERIV:
+        op_deriv<T>(t, level);
+        break;
+    case OP_ITER: {
+        IterParams ip;
+        ip.k1 = E.k1;
+        ip.k0 = E.k0;
+        ip.tau = E.tau;
+        ip.mu = E.mu;
+        ip.lambda = A.lambda;
+        ip.alpha0 = A.alpha0;
+        ip.alpha1 = A.alpha1;
+        ip.first = E.flags & 1;
+        ip.write_tp = (E.flags >> 1) & 1;
+        if (E.masked) op_iter<true, T>(t, level, E.roles, ip, sP, sQ, sT);
+        else op_iter<false, T>(t, level, E.roles, ip, sP, sQ, sT);
+        break;
+    }
+    case OP_END:
+        if (E.masked) op_end<true>(t, E.roles, A.limit);
+        else op_end<false>(t, E.roles, A.limit);
+        break;
+    case OP_UP:
+        op_up(t, level, E.roles, A.fscale);
+        break;
+    default:
+        break;
     }
 }
 
@@ -966,9 +1038,6 @@ __global__ void k_widen(const T *__restrict__ src, int nch, float *__restrict__ 
 
 // ------------------------------------------------------------------ launchers
 
-static inline const int *tiles_at(const Launch &L, int level) { return L.tiles + level * (L.npairs + 1); }
-static inline dim3 grid_at(const Launch &L, int level) { return dim3((unsigned)L.tiles_total[level]); }
-
 #define STORAGE_DISPATCH(storage, KERNEL_CALL)                  \
     do {                                                        \
         if ((storage) == kF16) {                                \
@@ -983,78 +1052,10 @@ static inline dim3 grid_at(const Launch &L, int level) { return dim3((unsigned)L
         }                                                       \
     } while (0)
 
-void launch_zero_arenas(const Launch &L, hipStream_t st)
+void launch_tick(const PairDesc *desc, const TickArgs &args, int storage, hipStream_t st)
 {
-    long long n4 = (long long)PL_COUNT * L.max_plane_floats / 4;
-    long long blocks = (n4 + 255) / 256;
-    if (blocks > 2048) blocks = 2048;
-    k_zero_arenas<<<dim3((unsigned)blocks, (unsigned)L.npairs), dim3(256), 0, st>>>(L.desc);
-}
-
-void launch_calibrate(const Launch &L, int storage)
-{
-    STORAGE_DISPATCH(storage, (k_calibrate<T><<<grid_at(L, 0), block2d(), 0, L.stream>>>(L.desc, tiles_at(L, 0),
-                                                                                          L.npairs)));
-}
-
-void launch_downscale_images(const Launch &L, int level, int storage)
-{
-    STORAGE_DISPATCH(storage, (k_downscale_images<T><<<grid_at(L, level), block2d(), 0, L.stream>>>(
-                                  L.desc, tiles_at(L, level), L.npairs, level)));
-}
-
-void launch_coarsest_init(const Launch &L, int level, int roles)
-{
-    k_coarsest_init<<<grid_at(L, level), block2d(), 0, L.stream>>>(L.desc, tiles_at(L, level), L.npairs, level,
-                                                                    roles);
-}
-
-void launch_tensor(const Launch &L, int level, int masked, int xblock, float beta, float gamma, int storage)
-{
-    if (masked)
-        STORAGE_DISPATCH(storage, (k_tensor<T, true><<<grid_at(L, level), block2d(), 0, L.stream>>>(
-                                      L.desc, tiles_at(L, level), L.npairs, level, xblock, beta, gamma)));
-    else
-        STORAGE_DISPATCH(storage, (k_tensor<T, false><<<grid_at(L, level), block2d(), 0, L.stream>>>(
-                                      L.desc, tiles_at(L, level), L.npairs, level, xblock, beta, gamma)));
-}
-
-void launch_warp_prep(const Launch &L, int level, int roles, int storage)
-{
-    STORAGE_DISPATCH(storage, (k_warp_prep<T><<<grid_at(L, level), block2d(), 0, L.stream>>>(
-                                  L.desc, tiles_at(L, level), L.npairs, level, roles)));
-}
-
-void launch_deriv(const Launch &L, int level, int storage)
-{
-    STORAGE_DISPATCH(storage, (k_deriv<T><<<grid_at(L, level), block2d(), 0, L.stream>>>(L.desc, tiles_at(L, level),
-                                                                                          L.npairs, level)));
-}
-
-void launch_iter(const Launch &L, int level, int masked, int roles, const IterParams &ip, int storage)
-{
-    if (masked)
-        STORAGE_DISPATCH(storage, (k_iter<true, T><<<grid_at(L, level), block2d(), 0, L.stream>>>(
-                                      L.desc, tiles_at(L, level), L.npairs, level, roles, ip)));
-    else
-        STORAGE_DISPATCH(storage, (k_iter<false, T><<<grid_at(L, level), block2d(), 0, L.stream>>>(
-                                      L.desc, tiles_at(L, level), L.npairs, level, roles, ip)));
-}
-
-void launch_warp_end(const Launch &L, int level, int masked, int roles, float limit)
-{
-    if (masked)
-        k_warp_end<true><<<grid_at(L, level), block2d(), 0, L.stream>>>(L.desc, tiles_at(L, level), L.npairs, level,
-                                                                         roles, limit);
-    else
-        k_warp_end<false><<<grid_at(L, level), block2d(), 0, L.stream>>>(L.desc, tiles_at(L, level), L.npairs, level,
-                                                                          roles, limit);
-}
-
-void launch_upscale(const Launch &L, int level, int roles, float scale)
-{
-    k_upscale<<<grid_at(L, level - 1), block2d(), 0, L.stream>>>(L.desc, tiles_at(L, level - 1), L.npairs, level,
-                                                                  roles, scale);
+    if (args.n < 1 || args.total < 1) return;
+    STORAGE_DISPATCH(storage, (k_tick<T><<<dim3((unsigned)args.total), block2d(), 0, st>>>(desc, args)));
 }
 
 void launch_fill_zero(void *ptr, size_t bytes, hipStream_t st)

@@ -20,8 +20,10 @@
 #include <stdarg.h>
 #include <stdio.h>
 #include <string.h>
+#include <time.h>
 
 #include <algorithm>
+#include <map>
 #include <new>
 #include <string>
 #include <vector>
@@ -36,23 +38,50 @@ namespace {
 
 thread_local std::string g_create_error;
 
-// Device-side launch set: descriptors + tile tables + captured graphs.
+// Device-side launch set: pair descriptors (device copy + host copy for tile counts)
+// and the graphs captured for the programs run on it.
+struct GraphKey {
+    int masked, warm_level, warps, iters, start_roles;
+    float limit;
+    bool operator<(const GraphKey &o) const
+    {
+        if (masked != o.masked) return masked < o.masked;
+        if (warm_level != o.warm_level) return warm_level < o.warm_level;
+        if (warps != o.warps) return warps < o.warps;
+        if (iters != o.iters) return iters < o.iters;
+        if (start_roles != o.start_roles) return start_roles < o.start_roles;
+        return limit < o.limit;
+    }
+};
+
+struct GraphEntry {
+    hipGraphExec_t exec = nullptr;
+    int final_roles = 0;
+};
+
 struct Engine {
     int device = 0;
     hipStream_t stream = nullptr;
     vfs::PairDesc *d_desc = nullptr;
-    int *d_tiles = nullptr;
-    int npairs = 0;
-    int tiles_total[kMaxLevels] = {};
-    int max_plane = 0;
-    hipGraphExec_t graph[2] = {nullptr, nullptr};
-    float graph_limit[2] = {0, 0};
-    int graph_roles[2] = {0, 0};
+    std::vector<vfs::PairDesc> h_desc;
+    std::map<GraphKey, GraphEntry> graphs;
 };
 
 struct Params {
     int nLevels, nWarpIters, nSolverIters, blockWidth, storage;
     float beta, gamma, alpha0, alpha1, timestep_lambda, lambda, fScale, limitRange;
+};
+
+// One op of a solve program (see vfs::OpKind), with the scalars it needs.
+struct ProgOp {
+    unsigned char kind, level, masked, flags;
+    int roles;
+    float k1, k0, tau, mu;
+};
+
+struct Program {
+    std::vector<ProgOp> ops;
+    int final_roles = 0;
 };
 
 }  // namespace
@@ -87,6 +116,7 @@ struct vfs_solver {
     float *arena = nullptr;
     long long plane = 0;
     int roles = 0;  // ping-pong roles after the last solve
+    bool solved = false;  // a solve has run since initialize
 
     Engine eng;  // single-pair launch set
     bool desc_dirty = true;
@@ -99,7 +129,8 @@ struct vfs_solver {
     }
     vfs::PairDesc desc() const
     {
-        vfs::PairDesc d{};
+        vfs::PairDesc d;
+        memset(&d, 0, sizeof d);  // descriptors are compared bytewise
         d.arena = arena;
         d.plane = plane;
         d.in_i0 = in_i0;
@@ -186,120 +217,174 @@ int dev_alloc(vfs_solver *s, void **p, size_t bytes)
 
 void engine_drop_graphs(Engine &e)
 {
-    for (int m = 0; m < 2; m++) {
-        if (e.graph[m]) (void)hipGraphExecDestroy(e.graph[m]);
-        e.graph[m] = nullptr;
-    }
+    for (auto &kv : e.graphs)
+        if (kv.second.exec) (void)hipGraphExecDestroy(kv.second.exec);
+    e.graphs.clear();
 }
 
 void engine_release(Engine &e)
 {
     engine_drop_graphs(e);
     if (e.d_desc) (void)hipFree(e.d_desc);
-    if (e.d_tiles) (void)hipFree(e.d_tiles);
     e.d_desc = nullptr;
-    e.d_tiles = nullptr;
-    e.npairs = 0;
+    e.h_desc.clear();
 }
 
-// Uploads descriptors and tile prefix tables for `descs` (all with nLevels levels).
-hipError_t engine_setup(Engine &e, const std::vector<vfs::PairDesc> &descs, int nLevels)
+// Uploads the pair descriptors of the launch set.
+hipError_t engine_setup(Engine &e, const std::vector<vfs::PairDesc> &descs)
 {
     engine_release(e);
-    const int n = (int)descs.size();
-    std::vector<int> tiles((size_t)kMaxLevels * (n + 1), 0);
-    e.max_plane = 0;
-    for (int l = 0; l < kMaxLevels; l++) e.tiles_total[l] = 0;
-    for (int l = 0; l < nLevels; l++) {
-        int acc = 0;
-        for (int i = 0; i < n; i++) {
-            tiles[(size_t)l * (n + 1) + i] = acc;
-            const int tx = (descs[i].w[l] + vfs::kTileW - 1) / vfs::kTileW;
-            const int ty = (descs[i].h[l] + vfs::kTileH - 1) / vfs::kTileH;
-            acc += tx * ty;
-        }
-        tiles[(size_t)l * (n + 1) + n] = acc;
-        e.tiles_total[l] = acc;
-    }
-    for (const auto &d : descs) e.max_plane = std::max(e.max_plane, (int)d.plane);
-    hipError_t r = hipMalloc(&e.d_desc, sizeof(vfs::PairDesc) * n);
+    hipError_t r = hipMalloc(&e.d_desc, sizeof(vfs::PairDesc) * descs.size());
     if (r != hipSuccess) return r;
-    r = hipMalloc(&e.d_tiles, sizeof(int) * tiles.size());
+    r = hipMemcpyAsync(e.d_desc, descs.data(), sizeof(vfs::PairDesc) * descs.size(), hipMemcpyHostToDevice,
+                       e.stream);
     if (r != hipSuccess) return r;
-    r = hipMemcpyAsync(e.d_desc, descs.data(), sizeof(vfs::PairDesc) * n, hipMemcpyHostToDevice, e.stream);
-    if (r != hipSuccess) return r;
-    r = hipMemcpyAsync(e.d_tiles, tiles.data(), sizeof(int) * tiles.size(), hipMemcpyHostToDevice, e.stream);
-    if (r != hipSuccess) return r;
-    e.npairs = n;
+    e.h_desc = descs;
     return hipStreamSynchronize(e.stream);
 }
 
-vfs::Launch engine_launch(const Engine &e)
+void push_op(Program &pr, int kind, int level, int masked, int roles)
 {
-    vfs::Launch L;
-    L.desc = e.d_desc;
-    L.tiles = e.d_tiles;
-    L.npairs = e.npairs;
-    for (int l = 0; l < kMaxLevels; l++) L.tiles_total[l] = e.tiles_total[l];
-    L.max_plane_floats = e.max_plane;
-    L.stream = e.stream;
-    return L;
+    ProgOp o{};
+    o.kind = (unsigned char)kind;
+    o.level = (unsigned char)level;
+    o.masked = (unsigned char)masked;
+    o.roles = roles;
+    pr.ops.push_back(o);
 }
 
-// Issues one solve of every pair of the launch set (stereotgv.cu:389-515 when masked,
-// :256-387 otherwise), starting from zeroed arenas and role bits 0.  Returns the roles
-// at the end.
-int issue_solve(const Engine &e, const Params &P, int masked)
+// One level of the solve from the tensor to the end of the last warp
+// (stereotgv.cu:409-494, masked; :275-364 unmasked).
+void push_level(Program &pr, const Params &P, int masked, int l, int warps, int iters, int &roles)
 {
-    const vfs::Launch L = engine_launch(e);
-    const int st = P.storage;
-    vfs::launch_zero_arenas(L, e.stream);
-    vfs::launch_calibrate(L, st);
-    for (int l = 1; l < P.nLevels; l++) vfs::launch_downscale_images(L, l, st);
-    int roles = 0;
-    for (int l = P.nLevels - 1; l >= 0; l--) {
-        if (l == P.nLevels - 1) vfs::launch_coarsest_init(L, l, roles);
-        vfs::launch_tensor(L, l, masked, P.blockWidth, P.beta, P.gamma, st);
-        for (int wi = 0; wi < P.nWarpIters; wi++) {
-            vfs::launch_warp_prep(L, l, roles, st);
-            vfs::launch_deriv(L, l, st);
-            float tau = 1.0f;
-            float sigma = 1.0f / tau;
-            for (int it = 0; it < P.nSolverIters; it++) {
-                float mu;
-                if (sigma < 1000.0f) mu = 1.0f / sqrtf(1 + 0.7f * tau * P.timestep_lambda);
-                else mu = 1;
-                vfs::IterParams ip;
-                ip.k1 = P.alpha1 * sigma / 3.0f;  // eta_p = 3 (stereotgv.cu:416)
-                ip.k0 = P.alpha0 * sigma / 2.0f;  // eta_q = 2 (stereotgv.cu:417)
-                ip.tau = tau;
-                ip.mu = mu;
-                ip.lambda = P.lambda;
-                ip.alpha0 = P.alpha0;
-                ip.alpha1 = P.alpha1;
-                ip.first = (it == 0);
-                ip.write_tp = (it == P.nSolverIters - 1);
-                vfs::launch_iter(L, l, masked, roles, ip, st);
-                roles ^= vfs::RB_U | vfs::RB_UB | vfs::RB_V | vfs::RB_PQ;
-                sigma = sigma / mu;
-                tau = tau * mu;
-            }
-            vfs::launch_warp_end(L, l, masked, roles, P.limitRange);
-            if (masked) roles ^= vfs::RB_U;
+    push_op(pr, vfs::OP_TENSOR, l, masked, roles);
+    for (int wi = 0; wi < warps; wi++) {
+        push_op(pr, vfs::OP_PREP, l, masked, roles);
+        push_op(pr, vfs::OP_DERIV, l, masked, roles);
+        float tau = 1.0f;
+        float sigma = 1.0f / tau;
+        for (int it = 0; it < iters; it++) {
+            float mu;
+            if (sigma < 1000.0f) mu = 1.0f / sqrtf(1 + 0.7f * tau * P.timestep_lambda);
+            else mu = 1;
+            ProgOp o{};
+            o.kind = vfs::OP_ITER;
+            o.level = (unsigned char)l;
+            o.masked = (unsigned char)masked;
+            o.flags = (unsigned char)((it == 0 ? 1 : 0) | (it == iters - 1 ? 2 : 0));
+            o.roles = roles;
+            o.k1 = P.alpha1 * sigma / 3.0f;  // eta_p = 3 (stereotgv.cu:416)
+            o.k0 = P.alpha0 * sigma / 2.0f;  // eta_q = 2 (stereotgv.cu:417)
+            o.tau = tau;
+            o.mu = mu;
+            pr.ops.push_back(o);
+            roles ^= vfs::RB_U | vfs::RB_UB | vfs::RB_V | vfs::RB_PQ;
+            sigma = sigma / mu;
+            tau = tau * mu;
         }
-        if (l > 0) {
-            vfs::launch_upscale(L, l, roles, P.fScale);
+        push_op(pr, vfs::OP_END, l, masked, roles);
+        if (masked) roles ^= vfs::RB_U;
+    }
+}
+
+// The solve program.  Cold (warm_level < 0): stereotgv.cu:389-515 (masked) or :256-387,
+// from zeroed buffers, all levels with the configured warps and iterations.
+// Warm (warm_level >= 0): the previous solve's u, u_ and warpUV are downscaled (x 1/fScale
+// per level) to warm_level and the solve runs from there to level 0 with `warps` and
+// `iters`; nothing is zeroed.
+Program build_program(const Params &P, int masked, int warm_level, int warps, int iters, int start_roles)
+{
+    Program pr;
+    int roles = 0;
+    if (warm_level < 0) {
+        push_op(pr, vfs::OP_ZERO, 0, masked, 0);
+        push_op(pr, vfs::OP_CALIB, 0, masked, 0);
+        for (int l = 1; l < P.nLevels; l++) push_op(pr, vfs::OP_DOWN, l, masked, 0);
+        for (int l = P.nLevels - 1; l >= 0; l--) {
+            if (l == P.nLevels - 1) push_op(pr, vfs::OP_CINIT, l, masked, roles);
+            push_level(pr, P, masked, l, P.nWarpIters, P.nSolverIters, roles);
+            if (l > 0) {
+                push_op(pr, vfs::OP_UP, l, masked, roles);
+                roles ^= vfs::RB_U | vfs::RB_UB | vfs::RB_UV;
+            }
+        }
+    } else {
+        roles = start_roles;
+        push_op(pr, vfs::OP_CALIB, 0, masked, roles);
+        for (int l = 1; l <= warm_level; l++) push_op(pr, vfs::OP_DOWN, l, masked, roles);
+        for (int l = 1; l <= warm_level; l++) {
+            push_op(pr, vfs::OP_WINIT, l, masked, roles);
             roles ^= vfs::RB_U | vfs::RB_UB | vfs::RB_UV;
         }
+        for (int l = warm_level; l >= 0; l--) {
+            push_level(pr, P, masked, l, warps, iters, roles);
+            if (l > 0) {
+                push_op(pr, vfs::OP_UP, l, masked, roles);
+                roles ^= vfs::RB_U | vfs::RB_UB | vfs::RB_UV;
+            }
+        }
     }
-    return roles;
+    pr.final_roles = roles;
+    return pr;
 }
 
-// Runs one solve on the engine's stream: replays (capturing first if needed) the graph
-// for this (masked, limit) or launches directly.  Returns the final roles through *roles.
-int engine_solve(Engine &e, const Params &P, int masked, bool use_graph, std::string *err, int *roles)
+vfs::TickArgs tick_header(const Params &P)
 {
-    const int m = masked ? 1 : 0;
+    vfs::TickArgs a;
+    memset(&a, 0, sizeof a);
+    a.lambda = P.lambda;
+    a.alpha0 = P.alpha0;
+    a.alpha1 = P.alpha1;
+    a.beta = P.beta;
+    a.gamma = P.gamma;
+    a.fscale = P.fScale;
+    a.scale = 1.0f / P.fScale;
+    a.limit = P.limitRange;
+    a.xblock = P.blockWidth;
+    return a;
+}
+
+void set_entry(vfs::TickEntry &e, const ProgOp &o, int pair, int tile_start)
+{
+    e.pair = pair;
+    e.tile_start = tile_start;
+    e.kind = o.kind;
+    e.level = o.level;
+    e.masked = o.masked;
+    e.flags = o.flags;
+    e.roles = o.roles;
+    e.k1 = o.k1;
+    e.k0 = o.k0;
+    e.tau = o.tau;
+    e.mu = o.mu;
+}
+
+// Issues every op of `pr` for every pair of the launch set: one k_tick launch per op per
+// group of up to kMaxEntries pairs.
+void issue_program(const Engine &e, const Params &P, const Program &pr)
+{
+    vfs::TickArgs a = tick_header(P);
+    const int n = (int)e.h_desc.size();
+    for (const ProgOp &o : pr.ops) {
+        for (int p0 = 0; p0 < n; p0 += vfs::kMaxEntries) {
+            const int cnt = std::min(vfs::kMaxEntries, n - p0);
+            int acc = 0;
+            for (int k = 0; k < cnt; k++) {
+                const vfs::PairDesc &d = e.h_desc[p0 + k];
+                set_entry(a.e[k], o, p0 + k, acc);
+                acc += vfs::op_tiles(o.kind, o.level, d.w, d.h, d.s);
+            }
+            a.n = cnt;
+            a.total = acc;
+            vfs::launch_tick(e.d_desc, a, P.storage, e.stream);
+        }
+    }
+}
+
+// Runs `pr` on the engine's stream: replays (capturing first if needed) the graph cached
+// under `key`, or launches directly.
+int engine_run(Engine &e, const Params &P, const Program &pr, const GraphKey &key, bool use_graph, std::string *err)
+{
     auto ferr = [&](const char *what, hipError_t r) {
         char buf[256];
         snprintf(buf, sizeof buf, "%s: %s", what, hipGetErrorString(r));
@@ -307,33 +392,27 @@ int engine_solve(Engine &e, const Params &P, int masked, bool use_graph, std::st
         return VFS_ERR_DEVICE;
     };
     if (!use_graph) {
-        *roles = issue_solve(e, P, masked);
+        issue_program(e, P, pr);
         hipError_t r = hipGetLastError();
         return r == hipSuccess ? VFS_OK : ferr("kernel launch", r);
     }
-    if (e.graph[m] && e.graph_limit[m] != P.limitRange) {
-        (void)hipGraphExecDestroy(e.graph[m]);
-        e.graph[m] = nullptr;
-    }
-    if (!e.graph[m]) {
+    auto it = e.graphs.find(key);
+    if (it == e.graphs.end()) {
         hipError_t r = hipStreamBeginCapture(e.stream, hipStreamCaptureModeThreadLocal);
         if (r != hipSuccess) return ferr("hipStreamBeginCapture", r);
-        const int fr = issue_solve(e, P, masked);
+        issue_program(e, P, pr);
         hipGraph_t g = nullptr;
         r = hipStreamEndCapture(e.stream, &g);
         if (r != hipSuccess) return ferr("hipStreamEndCapture", r);
-        r = hipGraphInstantiate(&e.graph[m], g, nullptr, nullptr, 0);
+        GraphEntry ge;
+        r = hipGraphInstantiate(&ge.exec, g, nullptr, nullptr, 0);
         (void)hipGraphDestroy(g);
-        if (r != hipSuccess) {
-            e.graph[m] = nullptr;
-            return ferr("hipGraphInstantiate", r);
-        }
-        e.graph_limit[m] = P.limitRange;
-        e.graph_roles[m] = fr;
+        if (r != hipSuccess) return ferr("hipGraphInstantiate", r);
+        ge.final_roles = pr.final_roles;
+        it = e.graphs.emplace(key, ge).first;
     }
-    hipError_t r = hipGraphLaunch(e.graph[m], e.stream);
+    hipError_t r = hipGraphLaunch(it->second.exec, e.stream);
     if (r != hipSuccess) return ferr("hipGraphLaunch", r);
-    *roles = e.graph_roles[m];
     return VFS_OK;
 }
 
@@ -402,25 +481,33 @@ int ensure_engine(vfs_solver *s)
 {
     if (!s->desc_dirty) return VFS_OK;
     std::vector<vfs::PairDesc> d{s->desc()};
-    hipError_t r = engine_setup(s->eng, d, s->nLevels);
+    hipError_t r = engine_setup(s->eng, d);
     if (r != hipSuccess) return fail(s, VFS_ERR_DEVICE, "engine setup: %s", hipGetErrorString(r));
     s->desc_dirty = false;
     return VFS_OK;
 }
 
-int solve_common(vfs_solver *s, int masked)
+// warm_level < 0: cold solve; otherwise a warm start at warm_level with warps/iters.
+int solve_common(vfs_solver *s, int masked, int warm_level = -1, int warps = 0, int iters = 0)
 {
     if (!s) return VFS_ERR_ARG;
     int rc = ready_to_solve(s, masked, &s->err);
     if (rc) return rc;
+    if (warm_level >= 0 && !s->solved)
+        return fail(s, VFS_ERR_STATE, "warm-started solve before any solve of this pair");
+    if (warm_level >= s->nLevels) return fail(s, VFS_ERR_ARG, "warm start level %d >= nLevels", warm_level);
     if ((rc = set_device(s))) return rc;
     if ((rc = ensure_engine(s))) return rc;
     HIPCHK(s, refresh_raw_i1(s, s->stream));
-    int roles = 0;
-    rc = engine_solve(s->eng, s->params(), masked, s->use_graph, &s->err, &roles);
+    const Params P = s->params();
+    const int start_roles = warm_level < 0 ? 0 : s->roles;
+    const Program pr = build_program(P, masked, warm_level, warps, iters, start_roles);
+    const GraphKey key{masked, warm_level, warps, iters, start_roles, P.limitRange};
+    rc = engine_run(s->eng, P, pr, key, s->use_graph, &s->err);
     if (rc) return rc;
-    s->roles = roles;
+    s->roles = pr.final_roles;
     s->raw_i1_fresh = false;
+    s->solved = true;
     HIPCHK(s, hipEventRecord(s->done, s->stream));
     return VFS_OK;
 }
@@ -595,6 +682,7 @@ int vfs_initialize(vfs_solver *s, int width, int height, float beta, float gamma
     s->plane = (long long)((N + 63) & ~(size_t)63);
     if ((rc = dev_alloc(s, (void **)&s->arena, sizeof(float) * (size_t)vfs::PL_COUNT * s->plane))) return rc;
     s->roles = 0;
+    s->solved = false;
     s->desc_dirty = true;
     HIPCHK(s, hipStreamSynchronize(s->stream));
     s->initialized = true;
@@ -697,6 +785,13 @@ int vfs_copy_images_u8c3(vfs_solver *s, const uint8_t *i0, const uint8_t *i1, in
 
 int vfs_solve_masked(vfs_solver *s) { return solve_common(s, 1); }
 int vfs_solve(vfs_solver *s) { return solve_common(s, 0); }
+
+int vfs_solve_warm(vfs_solver *s, int masked, int start_level, int warps, int iters)
+{
+    if (!s) return VFS_ERR_ARG;
+    if (start_level < 0 || warps < 0 || iters < 0) return fail(s, VFS_ERR_ARG, "bad warm schedule");
+    return solve_common(s, masked, start_level, warps, iters);
+}
 
 static float *slot_ptr(vfs_solver *s, int k) { return s->arena + (long long)k * s->plane; }
 
@@ -863,25 +958,24 @@ int vfs_time_inner_iterations(vfs_solver *s, int level, int reps, float *ms_tota
     if (rc) return rc;
     if ((rc = ensure_engine(s))) return rc;
     if ((rc = wait_done(s))) return rc;
-    const vfs::Launch L = engine_launch(s->eng);
-    hipEvent_t e0, e1;
-    if ((rc = time_events(s, s->stream, &e0, &e1))) return rc;
+    const Params P = s->params();
+    Program pr;
     int roles = 0;
-    float tau = 1.0f, sigma = 1.0f;
-    for (int it = 0; it < reps; it++) {
-        const float mu = 1.0f / sqrtf(1 + 0.7f * tau * s->timestep_lambda);
-        vfs::IterParams ip{s->alpha1 * sigma / 3.0f, s->alpha0 * sigma / 2.0f, tau, mu, s->lambda, s->alpha0,
-                           s->alpha1, 0, 0};
-        vfs::launch_iter(L, level, 1, roles, ip, s->storage);
-        roles ^= vfs::RB_U | vfs::RB_UB | vfs::RB_V | vfs::RB_PQ;
-        if ((it + 1) % 50 == 0) {
-            tau = 1.0f;
-            sigma = 1.0f;
-        } else {
-            sigma /= mu;
-            tau *= mu;
+    // `reps` iterations of the 50-iteration schedule, restarted every 50
+    {
+        Params Q = P;
+        Q.nSolverIters = 50;
+        Program w;
+        while ((int)pr.ops.size() < reps) {
+            w.ops.clear();
+            push_level(w, Q, 1, level, 1, 50, roles);
+            for (const ProgOp &o : w.ops)
+                if (o.kind == vfs::OP_ITER && (int)pr.ops.size() < reps) pr.ops.push_back(o);
         }
     }
+    hipEvent_t e0, e1;
+    if ((rc = time_events(s, s->stream, &e0, &e1))) return rc;
+    issue_program(s->eng, P, pr);
     HIPCHK(s, hipEventRecord(e1, s->stream));
     HIPCHK(s, hipEventSynchronize(e1));
     float ms = 0;
@@ -973,28 +1067,33 @@ int vfs_batch_destroy(vfs_batch *b)
 
 const char *vfs_batch_last_error(const vfs_batch *b) { return b ? b->err.c_str() : g_create_error.c_str(); }
 
-int vfs_batch_solve(vfs_batch *b, int masked)
+static int batch_solve(vfs_batch *b, int masked, int warm_level, int warps, int iters)
 {
     if (!b) return VFS_ERR_ARG;
     HIPCHKB(b, hipSetDevice(b->eng.device));
     const vfs_solver *f = b->members[0];
     Params P = f->params();
-    bool rebuild = b->eng.npairs != (int)b->members.size();
+    bool rebuild = b->eng.h_desc.size() != b->members.size();
     for (size_t i = 0; i < b->members.size(); i++) {
         vfs_solver *s = b->members[i];
         std::string why;
         if (ready_to_solve(s, masked, &why)) return failb(b, VFS_ERR_STATE, "member %zu: %s", i, why.c_str());
         if (s->storage != f->storage || s->limitRange != f->limitRange)
             return failb(b, VFS_ERR_ARG, "member %zu differs in storage or limitRange", i);
+        if (warm_level >= 0 && (!s->solved || s->roles != f->roles))
+            return failb(b, VFS_ERR_STATE, "member %zu: warm start needs every member solved with the same program",
+                         i);
         if (s->desc_dirty) rebuild = true;
+        if (!rebuild) {
+            const vfs::PairDesc d = s->desc();
+            if (memcmp(&b->eng.h_desc[i], &d, sizeof d) != 0) rebuild = true;
+        }
     }
+    if (warm_level >= P.nLevels) return failb(b, VFS_ERR_ARG, "warm start level %d >= nLevels", warm_level);
     if (rebuild) {
         std::vector<vfs::PairDesc> descs;
-        for (vfs_solver *s : b->members) {
-            descs.push_back(s->desc());
-            // a member's own single-pair launch set is rebuilt on its next own solve
-        }
-        hipError_t r = engine_setup(b->eng, descs, P.nLevels);
+        for (vfs_solver *s : b->members) descs.push_back(s->desc());
+        hipError_t r = engine_setup(b->eng, descs);
         if (r != hipSuccess) return failb(b, VFS_ERR_DEVICE, "engine setup: %s", hipGetErrorString(r));
     }
     // order after each member's pending work, then refresh re-used raw inputs
@@ -1002,15 +1101,26 @@ int vfs_batch_solve(vfs_batch *b, int masked)
         HIPCHKB(b, hipStreamWaitEvent(b->eng.stream, s->done, 0));
         HIPCHKB(b, refresh_raw_i1(s, b->eng.stream));
     }
-    int roles = 0;
-    int rc = engine_solve(b->eng, P, masked, f->use_graph, &b->err, &roles);
+    const int start_roles = warm_level < 0 ? 0 : f->roles;
+    const Program pr = build_program(P, masked, warm_level, warps, iters, start_roles);
+    const GraphKey key{masked, warm_level, warps, iters, start_roles, P.limitRange};
+    int rc = engine_run(b->eng, P, pr, key, f->use_graph, &b->err);
     if (rc) return rc;
     for (vfs_solver *s : b->members) {
-        s->roles = roles;
+        s->roles = pr.final_roles;
         s->raw_i1_fresh = false;
+        s->solved = true;
         HIPCHKB(b, hipEventRecord(s->done, b->eng.stream));
     }
     return VFS_OK;
+}
+
+int vfs_batch_solve(vfs_batch *b, int masked) { return batch_solve(b, masked, -1, 0, 0); }
+
+int vfs_batch_solve_warm(vfs_batch *b, int masked, int start_level, int warps, int iters)
+{
+    if (start_level < 0 || warps < 0 || iters < 0) return failb(b, VFS_ERR_ARG, "bad warm schedule");
+    return batch_solve(b, masked, start_level, warps, iters);
 }
 
 int vfs_batch_synchronize(vfs_batch *b)
@@ -1039,6 +1149,444 @@ int vfs_batch_time_solves(vfs_batch *b, int masked, int n, float *ms_total)
     (void)hipEventDestroy(e1);
     if (rc) return rc;
     *ms_total = ms;
+    return VFS_OK;
+}
+
+}  // extern "C"
+
+// ------------------------------------------------------------------ serving
+//
+// A continuous-batching server for many camera streams on one device.  Each stream owns a
+// resident solver (pyramids, duals, previous disparity), materialized when one of its
+// frames is admitted and the HBM budget allows, and evicted (least recently used, idle
+// only) when another stream needs the memory.  A frame is admitted as soon as its
+// stream is idle, an entry slot is free and its stream is resident; it then runs its own
+// program (cold: the full schedule; warm: from the stream's previous result).  Every
+// scheduling step launches ONE k_tick carrying the next op of every active frame, so
+// frames that arrived at different times share launches while each result depends only on
+// its own ops.  Everything is ordered on the server's stream; completion is observed
+// through events.
+
+namespace {
+
+struct SrvStream {
+    int width = 0, height = 0, nLevels = 0, nWarpIters = 0, nSolverIters = 0;
+    int warm_level = 1, warm_warps = 2, warm_iters = 20;
+    std::vector<float> mask, tv, cv;  // host copies for (re)materialization
+    vfs_solver *solver = nullptr;     // resident state, or null
+    int slot = -1;                    // descriptor slot while resident
+    size_t bytes = 0;                 // device footprint while resident
+    bool busy = false;                // a frame of this stream is admitted and unfinished
+    bool warm_ok = false;             // the resident state holds a previous result
+    long long last_use = 0;
+};
+
+struct SrvRequest {
+    long long tag;
+    int stream;
+    float *out = nullptr;  // caller buffer (width*height floats), may be null
+    float *pinned = nullptr;  // images in (2*w*h floats), then the result (w*h)
+    double t_submit = 0, t_done = 0;
+    int warm = 0;
+    Program prog;
+    size_t pc = 0;
+    hipEvent_t done = nullptr;
+};
+
+double now_ms()
+{
+    timespec ts;
+    clock_gettime(CLOCK_MONOTONIC, &ts);
+    return ts.tv_sec * 1e3 + ts.tv_nsec * 1e-6;
+}
+
+}  // namespace
+
+struct vfs_server {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    Params P{};
+    size_t budget = 0, used = 0;
+    int max_active = vfs::kMaxEntries;
+    int max_slots = 0;
+    vfs::PairDesc *d_desc = nullptr;
+    std::vector<int> free_slots;
+    std::vector<SrvStream> streams;
+    std::vector<SrvRequest *> queue, active, finishing;
+    std::vector<SrvRequest *> done;
+    std::vector<hipEvent_t> tick_events;  // in flight, oldest first
+    long long clock = 0;
+    long long ticks = 0, launched_entries = 0;
+    std::string err;
+};
+
+namespace {
+
+int fails(vfs_server *v, int code, const char *fmt, ...) __attribute__((format(printf, 3, 4)));
+int fails(vfs_server *v, int code, const char *fmt, ...)
+{
+    va_list ap;
+    va_start(ap, fmt);
+    int r = fail_str(v ? &v->err : nullptr, code, fmt, ap);
+    va_end(ap);
+    return r;
+}
+
+#define HIPCHKS(v, expr)                                                                                     \
+    do {                                                                                                     \
+        hipError_t e_ = (expr);                                                                              \
+        if (e_ != hipSuccess) return fails((v), VFS_ERR_DEVICE, "%s: %s", #expr, hipGetErrorString(e_));     \
+    } while (0)
+
+// Device bytes a resident stream needs (what vfs_initialize allocates).
+size_t stream_bytes(const SrvStream &st, float fScale)
+{
+    size_t total = 0;
+    int w = st.width, h = st.height;
+    const size_t s0 = (size_t)align_up(w, 32);
+    for (int l = 0; l < st.nLevels; l++) {
+        const size_t s = (size_t)align_up(w, 32);
+        total += s * h * sizeof(float) * 5;  // I0, I1, mask, tv (2)
+        h = (int)((float)h / fScale);
+        w = (int)((float)w / fScale);
+    }
+    const size_t N = s0 * st.height;
+    const size_t plane = (N + 63) & ~(size_t)63;
+    total += N * sizeof(float) * 8;  // inputs, cv, X, depth
+    total += plane * sizeof(float) * vfs::PL_COUNT;
+    return total;
+}
+
+int materialize(vfs_server *v, SrvStream &st)
+{
+    vfs_solver *s = nullptr;
+    int rc = vfs_create(&s);
+    if (rc) return fails(v, rc, "stream solver: %s", vfs_last_error(nullptr));
+    s->storage = v->P.storage;
+    rc = vfs_initialize(s, st.width, st.height, v->P.beta, v->P.gamma, v->P.alpha0, v->P.alpha1,
+                        v->P.timestep_lambda, v->P.lambda, st.nLevels, v->P.fScale, st.nWarpIters, st.nSolverIters);
+    if (!rc) rc = vfs_set_limit_range(s, v->P.limitRange);
+    if (!rc) rc = vfs_copy_mask(s, st.mask.data(), st.width);
+    if (!rc) rc = vfs_load_vector_fields(s, st.tv.data(), 2 * st.width, st.cv.data(), 2 * st.width);
+    if (rc) {
+        fails(v, rc, "stream materialization: %s", s->err.c_str());
+        vfs_destroy(s);
+        return rc;
+    }
+    s->has_images = true;
+    st.solver = s;
+    st.slot = v->free_slots.back();
+    v->free_slots.pop_back();
+    st.warm_ok = false;
+    v->used += st.bytes;
+    const vfs::PairDesc d = s->desc();
+    HIPCHKS(v, hipMemcpyAsync(v->d_desc + st.slot, &d, sizeof d, hipMemcpyHostToDevice, v->stream));
+    HIPCHKS(v, hipStreamSynchronize(v->stream));
+    return VFS_OK;
+}
+
+void evict(vfs_server *v, SrvStream &st)
+{
+    if (!st.solver) return;
+    vfs_destroy(st.solver);
+    st.solver = nullptr;
+    v->free_slots.push_back(st.slot);
+    st.slot = -1;
+    st.warm_ok = false;
+    v->used -= st.bytes;
+}
+
+// Makes room for `need` bytes (and a descriptor slot) by evicting idle resident streams:
+// streams with no queued frame first, least recently used first.
+bool make_room(vfs_server *v, size_t need, int keep)
+{
+    while (v->used + need > v->budget || v->free_slots.empty()) {
+        int victim = -1;
+        bool victim_queued = true;
+        for (int i = 0; i < (int)v->streams.size(); i++) {
+            const SrvStream &c = v->streams[i];
+            if (i == keep || !c.solver || c.busy) continue;
+            bool queued = false;
+            for (const SrvRequest *r : v->queue) queued |= r->stream == i;
+            if (victim < 0 || (victim_queued && !queued) ||
+                (queued == victim_queued && c.last_use < v->streams[victim].last_use)) {
+                victim = i;
+                victim_queued = queued;
+            }
+        }
+        if (victim < 0) return false;
+        evict(v, v->streams[victim]);
+    }
+    return true;
+}
+
+int retire_finished(vfs_server *v)
+{
+    for (size_t i = 0; i < v->finishing.size();) {
+        SrvRequest *r = v->finishing[i];
+        hipError_t q = hipEventQuery(r->done);
+        if (q == hipErrorNotReady) {
+            i++;
+            continue;
+        }
+        if (q != hipSuccess) return fails(v, VFS_ERR_DEVICE, "request event: %s", hipGetErrorString(q));
+        SrvStream &st = v->streams[r->stream];
+        if (r->out) memcpy(r->out, r->pinned, sizeof(float) * st.width * st.height);
+        r->t_done = now_ms();
+        st.busy = false;
+        st.warm_ok = true;
+        v->done.push_back(r);
+        v->finishing.erase(v->finishing.begin() + i);
+    }
+    while (!v->tick_events.empty() && hipEventQuery(v->tick_events.front()) == hipSuccess) {
+        (void)hipEventDestroy(v->tick_events.front());
+        v->tick_events.erase(v->tick_events.begin());
+    }
+    return VFS_OK;
+}
+
+int admit(vfs_server *v)
+{
+    for (size_t i = 0; i < v->queue.size() && (int)v->active.size() < v->max_active;) {
+        SrvRequest *r = v->queue[i];
+        SrvStream &st = v->streams[r->stream];
+        bool earlier = false;  // frames of one stream run in submission order
+        for (size_t j = 0; j < i; j++) earlier |= v->queue[j]->stream == r->stream;
+        if (st.busy || earlier) {
+            i++;
+            continue;
+        }
+        if (!st.solver) {
+            if (!make_room(v, st.bytes, r->stream)) {
+                i++;
+                continue;
+            }
+            int rc = materialize(v, st);
+            if (rc) return rc;
+        }
+        vfs_solver *s = st.solver;
+        const size_t n = (size_t)st.width * st.height;
+        const Dims d0 = s->lv[0];
+        HIPCHKS(v, hipMemcpy2DAsync(s->in_i0, sizeof(float) * d0.s, r->pinned, sizeof(float) * st.width,
+                                    sizeof(float) * st.width, st.height, hipMemcpyHostToDevice, v->stream));
+        HIPCHKS(v, hipMemcpy2DAsync(s->in_i1, sizeof(float) * d0.s, r->pinned + n, sizeof(float) * st.width,
+                                    sizeof(float) * st.width, st.height, hipMemcpyHostToDevice, v->stream));
+        Params P = v->P;
+        P.nLevels = st.nLevels;
+        P.nWarpIters = st.nWarpIters;
+        P.nSolverIters = st.nSolverIters;
+        r->warm = st.warm_ok ? 1 : 0;
+        r->prog = st.warm_ok ? build_program(P, 1, st.warm_level, st.warm_warps, st.warm_iters, s->roles)
+                             : build_program(P, 1, -1, 0, 0, 0);
+        r->pc = 0;
+        s->roles = r->prog.final_roles;
+        st.busy = true;
+        st.last_use = ++v->clock;
+        v->active.push_back(r);
+        v->queue.erase(v->queue.begin() + i);
+    }
+    return VFS_OK;
+}
+
+int launch_one_tick(vfs_server *v)
+{
+    if (v->active.empty()) return VFS_OK;
+    vfs::TickArgs a = tick_header(v->P);
+    int acc = 0, k = 0;
+    for (SrvRequest *r : v->active) {
+        const SrvStream &st = v->streams[r->stream];
+        const ProgOp &o = r->prog.ops[r->pc];
+        const vfs::PairDesc d = st.solver->desc();
+        set_entry(a.e[k], o, st.slot, acc);
+        acc += vfs::op_tiles(o.kind, o.level, d.w, d.h, d.s);
+        k++;
+    }
+    a.n = k;
+    a.total = acc;
+    vfs::launch_tick(v->d_desc, a, v->P.storage, v->stream);
+    HIPCHKS(v, hipGetLastError());
+    v->ticks++;
+    v->launched_entries += k;
+    for (size_t i = 0; i < v->active.size();) {
+        SrvRequest *r = v->active[i];
+        if (++r->pc < r->prog.ops.size()) {
+            i++;
+            continue;
+        }
+        // program done: result (u at level 0) to the pinned buffer, then the event
+        SrvStream &st = v->streams[r->stream];
+        vfs_solver *s = st.solver;
+        const float *u = s->arena + (long long)((s->roles & vfs::RB_U) ? vfs::PL_U1 : vfs::PL_U0) * s->plane;
+        HIPCHKS(v, hipMemcpy2DAsync(r->pinned, sizeof(float) * st.width, u, sizeof(float) * s->lv[0].s,
+                                    sizeof(float) * st.width, st.height, hipMemcpyDeviceToHost, v->stream));
+        HIPCHKS(v, hipEventCreateWithFlags(&r->done, hipEventDisableTiming));
+        HIPCHKS(v, hipEventRecord(r->done, v->stream));
+        v->finishing.push_back(r);
+        v->active.erase(v->active.begin() + i);
+    }
+    hipEvent_t ev;
+    HIPCHKS(v, hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+    HIPCHKS(v, hipEventRecord(ev, v->stream));
+    v->tick_events.push_back(ev);
+    return VFS_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int vfs_server_create(vfs_server **out, const vfs_server_params *p)
+{
+    if (!out || !p) return fails(nullptr, VFS_ERR_ARG, "null argument");
+    *out = nullptr;
+    if (p->max_streams < 1 || p->max_active < 1 || p->max_active > vfs::kMaxEntries)
+        return fails(nullptr, VFS_ERR_ARG, "max_streams >= 1 and 1 <= max_active <= %d required", vfs::kMaxEntries);
+    if (p->storage < vfs::kF32 || p->storage > vfs::kBF16) return fails(nullptr, VFS_ERR_ARG, "bad storage");
+    vfs_server *v = new (std::nothrow) vfs_server();
+    if (!v) return fails(nullptr, VFS_ERR_NOMEM, "host allocation failed");
+    v->P = Params{0, 0, 0, 32, p->storage, p->beta, p->gamma, p->alpha0, p->alpha1, p->timestep_lambda, p->lambda,
+                  p->fScale, p->limitRange};
+    v->budget = p->hbm_budget_bytes;
+    v->max_active = p->max_active;
+    v->max_slots = p->max_streams;
+    hipError_t e = hipGetDevice(&v->device);
+    if (e == hipSuccess) e = hipStreamCreateWithFlags(&v->stream, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipMalloc(&v->d_desc, sizeof(vfs::PairDesc) * v->max_slots);
+    if (e != hipSuccess) {
+        fails(nullptr, VFS_ERR_DEVICE, "server setup: %s", hipGetErrorString(e));
+        if (v->d_desc) (void)hipFree(v->d_desc);
+        if (v->stream) (void)hipStreamDestroy(v->stream);
+        delete v;
+        return VFS_ERR_DEVICE;
+    }
+    for (int i = v->max_slots - 1; i >= 0; i--) v->free_slots.push_back(i);
+    *out = v;
+    return VFS_OK;
+}
+
+int vfs_server_destroy(vfs_server *v)
+{
+    if (!v) return VFS_ERR_ARG;
+    (void)hipSetDevice(v->device);
+    (void)hipStreamSynchronize(v->stream);
+    auto drop = [](std::vector<SrvRequest *> &l) {
+        for (SrvRequest *r : l) {
+            if (r->done) (void)hipEventDestroy(r->done);
+            if (r->pinned) (void)hipHostFree(r->pinned);
+            delete r;
+        }
+        l.clear();
+    };
+    drop(v->queue);
+    drop(v->active);
+    drop(v->finishing);
+    drop(v->done);
+    for (hipEvent_t e : v->tick_events) (void)hipEventDestroy(e);
+    for (SrvStream &st : v->streams) evict(v, st);
+    (void)hipFree(v->d_desc);
+    (void)hipStreamDestroy(v->stream);
+    delete v;
+    return VFS_OK;
+}
+
+const char *vfs_server_last_error(const vfs_server *v) { return v ? v->err.c_str() : g_create_error.c_str(); }
+
+int vfs_server_add_stream(vfs_server *v, const vfs_stream_params *p, int *stream_id)
+{
+    if (!v || !p || !stream_id) return VFS_ERR_ARG;
+    if (p->width < 1 || p->height < 1 || p->nLevels < 1 || p->nLevels > kMaxLevels || !p->mask || !p->tv || !p->cv)
+        return fails(v, VFS_ERR_ARG, "bad stream geometry or fields");
+    if (p->warm_level < 0 || p->warm_level >= p->nLevels)
+        return fails(v, VFS_ERR_ARG, "warm start level %d outside [0, %d)", p->warm_level, p->nLevels);
+    SrvStream st;
+    st.width = p->width;
+    st.height = p->height;
+    st.nLevels = p->nLevels;
+    st.nWarpIters = p->nWarpIters;
+    st.nSolverIters = p->nSolverIters;
+    st.warm_level = p->warm_level;
+    st.warm_warps = p->warm_warps;
+    st.warm_iters = p->warm_iters;
+    const size_t n = (size_t)p->width * p->height;
+    st.mask.assign(p->mask, p->mask + n);
+    st.tv.assign(p->tv, p->tv + 2 * n);
+    st.cv.assign(p->cv, p->cv + 2 * n);
+    st.bytes = stream_bytes(st, v->P.fScale);
+    if (st.bytes > v->budget) return fails(v, VFS_ERR_NOMEM, "stream needs %zu bytes, budget %zu", st.bytes, v->budget);
+    *stream_id = (int)v->streams.size();
+    v->streams.push_back(std::move(st));
+    return VFS_OK;
+}
+
+int vfs_server_submit(vfs_server *v, int stream_id, const float *i0, const float *i1, float *u_out, long long tag)
+{
+    if (!v || !i0 || !i1) return VFS_ERR_ARG;
+    if (stream_id < 0 || stream_id >= (int)v->streams.size()) return fails(v, VFS_ERR_ARG, "bad stream id");
+    const SrvStream &st = v->streams[stream_id];
+    const size_t n = (size_t)st.width * st.height;
+    SrvRequest *r = new (std::nothrow) SrvRequest();
+    if (!r) return fails(v, VFS_ERR_NOMEM, "host allocation failed");
+    hipError_t e = hipHostMalloc((void **)&r->pinned, sizeof(float) * 2 * n, hipHostMallocDefault);
+    if (e != hipSuccess) {
+        delete r;
+        return fails(v, VFS_ERR_NOMEM, "pinned staging: %s", hipGetErrorString(e));
+    }
+    memcpy(r->pinned, i0, sizeof(float) * n);
+    memcpy(r->pinned + n, i1, sizeof(float) * n);
+    r->tag = tag;
+    r->stream = stream_id;
+    r->out = u_out;
+    r->t_submit = now_ms();
+    v->queue.push_back(r);
+    return VFS_OK;
+}
+
+int vfs_server_step(vfs_server *v, int *active, int *queued)
+{
+    if (!v) return VFS_ERR_ARG;
+    HIPCHKS(v, hipSetDevice(v->device));
+    int rc = retire_finished(v);
+    if (!rc) rc = admit(v);
+    if (!rc && (int)v->tick_events.size() < 8) rc = launch_one_tick(v);
+    else if (!rc && !v->tick_events.empty()) {
+        // bounded run-ahead: wait for the oldest tick in flight
+        HIPCHKS(v, hipEventSynchronize(v->tick_events.front()));
+    }
+    if (active) *active = (int)(v->active.size() + v->finishing.size());
+    if (queued) *queued = (int)v->queue.size();
+    return rc;
+}
+
+int vfs_server_collect(vfs_server *v, vfs_server_result *out, int max, int *n)
+{
+    if (!v || !n || (max > 0 && !out)) return VFS_ERR_ARG;
+    int k = 0;
+    while (k < max && !v->done.empty()) {
+        SrvRequest *r = v->done.front();
+        v->done.erase(v->done.begin());
+        out[k].tag = r->tag;
+        out[k].stream = r->stream;
+        out[k].warm = r->warm;
+        out[k].latency_ms = r->t_done - r->t_submit;
+        out[k].ops = (int)r->prog.ops.size();
+        if (r->done) (void)hipEventDestroy(r->done);
+        (void)hipHostFree(r->pinned);
+        delete r;
+        k++;
+    }
+    *n = k;
+    return VFS_OK;
+}
+
+int vfs_server_stats(const vfs_server *v, vfs_server_stats_t *st)
+{
+    if (!v || !st) return VFS_ERR_ARG;
+    st->ticks = v->ticks;
+    st->entries = v->launched_entries;
+    st->hbm_used = v->used;
+    int resident = 0;
+    for (const SrvStream &s : v->streams) resident += s.solver != nullptr;
+    st->resident_streams = resident;
     return VFS_OK;
 }
 

@@ -69,27 +69,49 @@ struct IterParams {
     int write_tp;  // also store Tp (read later only where a neighbour is outside the mask)
 };
 
-// One batched launch: descriptors + per-level tile prefix table.
-struct Launch {
-    const PairDesc *desc;  // device array [npairs]
-    const int *tiles;      // device array [kMaxLevels][npairs + 1] of tile prefix sums
-    int npairs;
-    int tiles_total[kMaxLevels];
-    int max_plane_floats;  // largest arena plane of the set (for the zero fill grid)
-    hipStream_t stream;
+// Ops of a solve program.  A program is the op sequence of one solve (built on the
+// host); k_tick executes one op for each of up to kMaxEntries (pair, op) entries.
+enum OpKind : int {
+    OP_ZERO = 0,   // zero the work arena (fresh state)
+    OP_CALIB = 1,  // calibration warp of I1 into the level-0 pyramid slot
+    OP_DOWN = 2,   // I0/I1 pyramid level `level` from level-1
+    OP_CINIT = 3,  // coarsest level: warpUV = u * tv
+    OP_WINIT = 4,  // warm start: u, u_, warpUV of level-1 downscaled (x 1/fScale) to `level`
+    OP_TENSOR = 5,
+    OP_PREP = 6,   // tv2, I1w, u_last, v_ = 0
+    OP_DERIV = 7,
+    OP_ITER = 8,   // one fused TGV iteration
+    OP_END = 9,    // [median +] limit + update at the end of a warp
+    OP_UP = 10     // upscale u, u_, warpUV from `level` to level-1
 };
 
-// ---- batched kernels (vfs_kernels.hip) ----
-void launch_zero_arenas(const Launch &L, hipStream_t st);
-void launch_calibrate(const Launch &L, int storage);
-void launch_downscale_images(const Launch &L, int level, int storage);
-void launch_coarsest_init(const Launch &L, int level, int roles);
-void launch_tensor(const Launch &L, int level, int masked, int xlimit_block, float beta, float gamma, int storage);
-void launch_warp_prep(const Launch &L, int level, int roles, int storage);
-void launch_deriv(const Launch &L, int level, int storage);
-void launch_iter(const Launch &L, int level, int masked, int roles, const IterParams &ip, int storage);
-void launch_warp_end(const Launch &L, int level, int masked, int roles, float limit);
-void launch_upscale(const Launch &L, int level, int roles, float scale);
+constexpr int kMaxEntries = 64;
+
+struct TickEntry {
+    int pair;          // index into the descriptor array
+    int tile_start;    // first workgroup of this entry (prefix sum)
+    unsigned char kind, level, masked, flags;  // flags: bit 0 = first iteration, bit 1 = write Tp
+    int roles;         // RoleBit set before the op
+    float k1, k0, tau, mu;  // ITER scalars
+};
+
+struct TickArgs {
+    int n;      // entries
+    int total;  // workgroups
+    float lambda, alpha0, alpha1, beta, gamma, fscale, scale, limit;
+    int xblock;  // calcTensor launch block width (column limit of the tensor)
+    TickEntry e[kMaxEntries];
+};
+
+// Workgroups (64x8 tiles) an op of `kind` at `level` needs for a pair with dims w/h/s.
+inline int op_tiles(int kind, int level, const int *w, const int *h, const int *s)
+{
+    const int tl = kind == OP_UP ? level - 1 : level;
+    const int ww = kind == OP_ZERO ? s[0] : w[tl];
+    return ((ww + kTileW - 1) / kTileW) * ((h[tl] + kTileH - 1) / kTileH);
+}
+
+void launch_tick(const PairDesc *desc, const TickArgs &args, int storage, hipStream_t st);
 
 // ---- single-plane kernels (load time, outputs) ----
 struct Dims {

@@ -126,10 +126,10 @@ def make_pair(W: int, H: int, seed: int, radius: float | None = None, dmax: floa
 def drift_pair(base: dict, frame: int, seed: int, step: float = 0.5) -> dict:
     """Frame `frame` of a camera stream: the base disparity shifted by a smooth
     per-frame drift of at most `step` px (serving workload)."""
-    rng = np.random.default_rng(seed * 1000003 + frame)
+    phase = np.random.default_rng(seed).random(2) * 2 * np.pi  # one drift pattern per stream
+    rng = np.random.default_rng(seed * 1000003 + frame)          # fresh sensor noise per frame
     H, W = base["i0"].shape
     ys, xs = np.mgrid[0:H, 0:W].astype(np.float64)
-    phase = rng.random(2) * 2 * np.pi
     drift = step * frame * 0.5 * (np.sin(2 * np.pi * xs / W + phase[0]) * 0.5
                                   + np.cos(2 * np.pi * ys / H + phase[1]) * 0.5)
     d = np.clip(base["gt"].astype(np.float64) + drift, 0.0, None)
