@@ -38,6 +38,8 @@ def main():
     ap.add_argument("--no-graph", action="store_true", help="launch kernels one by one instead of a HIP graph")
     ap.add_argument("--quick", action="store_true", help="plain form only (no batched / reduced-storage legs)")
     ap.add_argument("--batch-steps", type=int, default=2)
+    ap.add_argument("--serve-frames", type=int, default=25, help="frames per stream in the serving leg")
+    ap.add_argument("--serve-only", action="store_true", help="run only the serving leg (development aid)")
     ap.add_argument("--iter-only", type=int, default=0,
                     help="profiling aid: run N level-0 inner iterations (after a 256 MiB reference copy) and exit")
     args = ap.parse_args()
@@ -134,8 +136,10 @@ def main():
     }
 
     if not args.quick:
-        out["batched"] = measure_batched(args, P, S, vfs_amd)
-        out["reduced_storage"] = measure_reduced(args, P, S, vfs_amd, pair_checker=not args.no_cpu)
+        out["served"] = measure_served(args, P, S, vfs_amd, pairs_per_s)
+        if not args.serve_only:
+            out["batched"] = measure_batched(args, P, S, vfs_amd)
+            out["reduced_storage"] = measure_reduced(args, P, S, vfs_amd, pair_checker=not args.no_cpu)
 
     if not args.no_cpu:
         from oracle import oracle as O
@@ -229,6 +233,123 @@ def measure_reduced(args, P, S, vfs_amd, pair_checker=True):
         b.close()
         for g in solvers:
             g.close()
+    return out
+
+
+def serving_streams(S, n_streams=32, distinct=4):
+    """32 camera streams, alternately 424x400 and 848x800 (seeds 300+s).  Each stream has
+    `distinct` synthetic frames along one smooth drift (<= 0.5 px per step); frame k of
+    the stream is distinct frame triangle(k), so consecutive frames always differ by one
+    drift step."""
+    streams = []
+    for sidx in range(n_streams):
+        W, H = (424, 400) if sidx % 2 == 0 else (848, 800)
+        base = S.make_pair(W, H, 300 + sidx, H / 2.0, W / 21.2)
+        fr = [base] + [S.drift_pair(base, k, 300 + sidx) for k in range(1, distinct)]
+        streams.append((W, H, base, fr))
+    return streams
+
+
+def tri(k, n):
+    period = 2 * (n - 1)
+    m = k % period
+    return m if m < n else period - m
+
+
+def prime_streams(srv, sids, streams):
+    """Each stream's first frame (the full cold solve) so that the timed frames are the
+    steady state of a stream: warm-started from the stream's previous result."""
+    lat = []
+    for i, sid in enumerate(sids):
+        f = streams[i][3][0]
+        srv.submit(sid, f["i0"], f["i1"], tag=-1 - i, want_result=False)
+    for r in srv.run_until_idle():
+        lat.append(r["latency_ms"])
+    return np.array(lat)
+
+
+def run_served(srv, sids, streams, n_frames, rate, seed=7):
+    """Poisson arrivals at `rate` frames/s (rate <= 0: everything submitted at t = 0);
+    each arrival is the next frame of a uniformly chosen stream (frames 1.. of the stream,
+    frame 0 was primed).  Returns latencies, their stream indices, the wall time from
+    the first arrival to the last result and the number of warm-started frames."""
+    rng = np.random.default_rng(seed)
+    n_total = n_frames * len(sids)
+    order = []
+    left = [n_frames] * len(sids)
+    for _ in range(n_total):
+        live = [i for i in range(len(sids)) if left[i] > 0]
+        i = int(rng.choice(live))
+        order.append(i)
+        left[i] -= 1
+    times = np.zeros(n_total) if rate <= 0 else np.cumsum(rng.exponential(1.0 / rate, n_total))
+    next_frame = [1] * len(sids)
+    lat, who, warm = [], [], 0
+    k = 0
+    t0 = time.perf_counter()
+    while True:
+        now = time.perf_counter() - t0
+        while k < n_total and times[k] <= now:
+            i = order[k]
+            W, H, base, fr = streams[i]
+            f = fr[tri(next_frame[i], len(fr))]
+            srv.submit(sids[i], f["i0"], f["i1"], tag=k, want_result=False)
+            next_frame[i] += 1
+            k += 1
+        act, q = srv.step()
+        for r in srv.collect():
+            lat.append(r["latency_ms"])
+            who.append(order[r["tag"]])
+            warm += r["warm"]
+        if k == n_total and act == 0 and q == 0:
+            break
+        if time.perf_counter() - t0 > 300:
+            raise RuntimeError("serving run did not drain")
+    wall = time.perf_counter() - t0
+    return np.array(lat), np.array(who), wall, warm
+
+
+def _pct(a, q):
+    return round(float(np.percentile(a, q)), 2) if len(a) else None
+
+
+def measure_served(args, P, S, vfs_amd, plain_pairs_per_s):
+    """Serving: 32 streams (424x400 / 848x800), continuous batching, FP16 storage.  Each
+    stream's first frame runs the full L5 W10 I50 solve (primed, reported apart); the
+    timed frames are warm-started (W2 I20 at levels 1 and 0).  Saturation throughput
+    (every frame queued at t = 0, so its latencies are queueing delays), then Poisson
+    arrivals at 50 % and 90 % of it."""
+    streams = serving_streams(S)
+    print("bench: serving streams ready", file=sys.stderr, flush=True)
+    out = {"workload": f"32 streams (424x400 / 848x800), {args.serve_frames} timed frames each after one cold "
+                       f"frame, warm start W2 I20 at levels 1-0 (cold frame L5 W10 I50), FP16 storage, "
+                       f"continuous batching"}
+    res = {}
+    for label, rate in (("saturation", 0.0), ("rate_50", 0.5), ("rate_90", 0.9)):
+        srv = vfs_amd.Server(P, hbm_budget_bytes=64 << 30, max_streams=32, max_active=64, storage="f16")
+        sids = [srv.add_stream(b["mask"], b["tv"], b["cv"], 5, 10, 50, 1, 2, 20) for (W, H, b, fr) in streams]
+        cold = prime_streams(srv, sids, streams)
+        r = rate if rate <= 0 else rate * res["saturation"]["frames_per_s"]
+        t_before = srv.stats()
+        lat, who, wall, warm = run_served(srv, sids, streams, args.serve_frames, r)
+        st = srv.stats()
+        small = np.array([streams[i][0] == 424 for i in who])
+        entry = {"offered_frames_per_s": round(r, 1) if r > 0 else None,
+                 "frames_per_s": round(len(lat) / wall, 1),
+                 "p50_ms": _pct(lat, 50), "p99_ms": _pct(lat, 99),
+                 "p50_ms_424x400": _pct(lat[small], 50), "p99_ms_424x400": _pct(lat[small], 99),
+                 "p50_ms_848x800": _pct(lat[~small], 50), "p99_ms_848x800": _pct(lat[~small], 99),
+                 "frames": int(len(lat)), "warm_frames": int(warm),
+                 "ops_per_launch": round((st["entries"] - t_before["entries"])
+                                         / max(st["ticks"] - t_before["ticks"], 1), 2)}
+        if label == "saturation":
+            entry["cold_first_frames"] = {"frames": int(len(cold)), "p50_ms": _pct(cold, 50),
+                                          "p99_ms": _pct(cold, 99)}
+        res[label] = entry
+        print(f"bench: served {label}: {entry}", file=sys.stderr, flush=True)
+        srv.close()
+    out.update(res)
+    out["vs_plain_per_frame_solve"] = round(res["saturation"]["frames_per_s"] / plain_pairs_per_s, 2)
     return out
 
 

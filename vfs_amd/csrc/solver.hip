@@ -23,6 +23,7 @@
 #include <time.h>
 
 #include <algorithm>
+#include <deque>
 #include <map>
 #include <new>
 #include <string>
@@ -1179,9 +1180,11 @@ struct SrvStream {
     bool busy = false;                // a frame of this stream is admitted and unfinished
     bool warm_ok = false;             // the resident state holds a previous result
     long long last_use = 0;
+    std::deque<struct SrvRequest *> queue;  // submitted, not yet admitted (in order)
 };
 
 struct SrvRequest {
+    long long seq = 0;  // submission order
     long long tag;
     int stream;
     float *out = nullptr;  // caller buffer (width*height floats), may be null
@@ -1212,7 +1215,8 @@ struct vfs_server {
     vfs::PairDesc *d_desc = nullptr;
     std::vector<int> free_slots;
     std::vector<SrvStream> streams;
-    std::vector<SrvRequest *> queue, active, finishing;
+    std::vector<SrvRequest *> active, finishing;
+    long long queued = 0, seq = 0;
     std::vector<SrvRequest *> done;
     std::vector<hipEvent_t> tick_events;  // in flight, oldest first
     long long clock = 0;
@@ -1306,8 +1310,7 @@ bool make_room(vfs_server *v, size_t need, int keep)
         for (int i = 0; i < (int)v->streams.size(); i++) {
             const SrvStream &c = v->streams[i];
             if (i == keep || !c.solver || c.busy) continue;
-            bool queued = false;
-            for (const SrvRequest *r : v->queue) queued |= r->stream == i;
+            const bool queued = !c.queue.empty();
             if (victim < 0 || (victim_queued && !queued) ||
                 (queued == victim_queued && c.last_use < v->streams[victim].last_use)) {
                 victim = i;
@@ -1345,45 +1348,65 @@ int retire_finished(vfs_server *v)
     return VFS_OK;
 }
 
+// Admits queued frames, oldest first, while active slots are free: a frame is
+// admissible when it is the oldest queued frame of an idle stream that is (or can be
+// made) resident.
 int admit(vfs_server *v)
 {
-    for (size_t i = 0; i < v->queue.size() && (int)v->active.size() < v->max_active;) {
-        SrvRequest *r = v->queue[i];
-        SrvStream &st = v->streams[r->stream];
-        bool earlier = false;  // frames of one stream run in submission order
-        for (size_t j = 0; j < i; j++) earlier |= v->queue[j]->stream == r->stream;
-        if (st.busy || earlier) {
-            i++;
-            continue;
+    while ((int)v->active.size() < v->max_active && v->queued > 0) {
+        int pick = -1;
+        for (int i = 0; i < (int)v->streams.size(); i++) {
+            const SrvStream &c = v->streams[i];
+            if (c.busy || c.queue.empty()) continue;
+            if (pick < 0 || c.queue.front()->seq < v->streams[pick].queue.front()->seq) pick = i;
         }
+        if (pick < 0) break;
+        SrvStream &st = v->streams[pick];
         if (!st.solver) {
-            if (!make_room(v, st.bytes, r->stream)) {
-                i++;
-                continue;
+            if (!make_room(v, st.bytes, pick)) {
+                // not resident and no room now: try the other streams, retry next step
+                bool other = false;
+                for (int i = 0; i < (int)v->streams.size() && !other; i++) {
+                    const SrvStream &c = v->streams[i];
+                    other = i != pick && !c.busy && !c.queue.empty() && c.solver;
+                }
+                if (!other) break;
+                // admit the oldest frame among resident idle streams instead
+                int alt = -1;
+                for (int i = 0; i < (int)v->streams.size(); i++) {
+                    const SrvStream &c = v->streams[i];
+                    if (c.busy || c.queue.empty() || !c.solver) continue;
+                    if (alt < 0 || c.queue.front()->seq < v->streams[alt].queue.front()->seq) alt = i;
+                }
+                pick = alt;
+            } else {
+                int rc = materialize(v, st);
+                if (rc) return rc;
             }
-            int rc = materialize(v, st);
-            if (rc) return rc;
         }
-        vfs_solver *s = st.solver;
-        const size_t n = (size_t)st.width * st.height;
+        SrvStream &sp = v->streams[pick];
+        SrvRequest *r = sp.queue.front();
+        sp.queue.pop_front();
+        v->queued--;
+        vfs_solver *s = sp.solver;
+        const size_t n = (size_t)sp.width * sp.height;
         const Dims d0 = s->lv[0];
-        HIPCHKS(v, hipMemcpy2DAsync(s->in_i0, sizeof(float) * d0.s, r->pinned, sizeof(float) * st.width,
-                                    sizeof(float) * st.width, st.height, hipMemcpyHostToDevice, v->stream));
-        HIPCHKS(v, hipMemcpy2DAsync(s->in_i1, sizeof(float) * d0.s, r->pinned + n, sizeof(float) * st.width,
-                                    sizeof(float) * st.width, st.height, hipMemcpyHostToDevice, v->stream));
+        HIPCHKS(v, hipMemcpy2DAsync(s->in_i0, sizeof(float) * d0.s, r->pinned, sizeof(float) * sp.width,
+                                    sizeof(float) * sp.width, sp.height, hipMemcpyHostToDevice, v->stream));
+        HIPCHKS(v, hipMemcpy2DAsync(s->in_i1, sizeof(float) * d0.s, r->pinned + n, sizeof(float) * sp.width,
+                                    sizeof(float) * sp.width, sp.height, hipMemcpyHostToDevice, v->stream));
         Params P = v->P;
-        P.nLevels = st.nLevels;
-        P.nWarpIters = st.nWarpIters;
-        P.nSolverIters = st.nSolverIters;
-        r->warm = st.warm_ok ? 1 : 0;
-        r->prog = st.warm_ok ? build_program(P, 1, st.warm_level, st.warm_warps, st.warm_iters, s->roles)
+        P.nLevels = sp.nLevels;
+        P.nWarpIters = sp.nWarpIters;
+        P.nSolverIters = sp.nSolverIters;
+        r->warm = sp.warm_ok ? 1 : 0;
+        r->prog = sp.warm_ok ? build_program(P, 1, sp.warm_level, sp.warm_warps, sp.warm_iters, s->roles)
                              : build_program(P, 1, -1, 0, 0, 0);
         r->pc = 0;
         s->roles = r->prog.final_roles;
-        st.busy = true;
-        st.last_use = ++v->clock;
+        sp.busy = true;
+        sp.last_use = ++v->clock;
         v->active.push_back(r);
-        v->queue.erase(v->queue.begin() + i);
     }
     return VFS_OK;
 }
@@ -1477,7 +1500,11 @@ int vfs_server_destroy(vfs_server *v)
         }
         l.clear();
     };
-    drop(v->queue);
+    for (SrvStream &st : v->streams) {
+        std::vector<SrvRequest *> q(st.queue.begin(), st.queue.end());
+        st.queue.clear();
+        drop(q);
+    }
     drop(v->active);
     drop(v->finishing);
     drop(v->done);
@@ -1537,7 +1564,9 @@ int vfs_server_submit(vfs_server *v, int stream_id, const float *i0, const float
     r->stream = stream_id;
     r->out = u_out;
     r->t_submit = now_ms();
-    v->queue.push_back(r);
+    r->seq = ++v->seq;
+    v->streams[stream_id].queue.push_back(r);
+    v->queued++;
     return VFS_OK;
 }
 
@@ -1553,7 +1582,7 @@ int vfs_server_step(vfs_server *v, int *active, int *queued)
         HIPCHKS(v, hipEventSynchronize(v->tick_events.front()));
     }
     if (active) *active = (int)(v->active.size() + v->finishing.size());
-    if (queued) *queued = (int)v->queue.size();
+    if (queued) *queued = (int)v->queued;
     return rc;
 }
 
