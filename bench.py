@@ -135,6 +135,14 @@ def main():
                      "copy_kernel_GBs": round(copy_gbs, 1)},
     }
 
+    # per-level launch choices of the tile-shape tuner (made at the first solve)
+    out["tile_tuning"] = {}
+    for lvl in range(wl["levels"]):
+        (tw, th), pre = g.level_tuning(lvl, True)
+        lw, lh, _ = g.level_dims(lvl)
+        out["tile_tuning"][f"L{lvl} {lw}x{lh}"] = f"{tw}x{th}" + (" +Tp prefetch" if pre else "")
+    print(f"bench: tile tuning {out['tile_tuning']}", file=sys.stderr, flush=True)
+
     if not args.quick:
         out["served"] = measure_served(args, P, S, vfs_amd, pairs_per_s)
         if not args.serve_only:
@@ -326,7 +334,7 @@ def measure_served(args, P, S, vfs_amd, plain_pairs_per_s):
                        f"continuous batching"}
     res = {}
     for label, rate in (("saturation", 0.0), ("rate_50", 0.5), ("rate_90", 0.9)):
-        srv = vfs_amd.Server(P, hbm_budget_bytes=64 << 30, max_streams=32, max_active=64, storage="f16")
+        srv = vfs_amd.Server(P, hbm_budget_bytes=64 << 30, max_streams=32, max_active=60, storage="f16")
         sids = [srv.add_stream(b["mask"], b["tv"], b["cv"], 5, 10, 50, 1, 2, 20) for (W, H, b, fr) in streams]
         cold = prime_streams(srv, sids, streams)
         r = rate if rate <= 0 else rate * res["saturation"]["frames_per_s"]

@@ -131,10 +131,23 @@ int vfs_synchronize(vfs_solver *s);
  *   "graph"   1 = replay the solve as a captured HIP graph (default), 0 = launch
  *             kernels one by one;
  *   "storage" 0 = FP32 (default), 1 = FP16, 2 = BF16 storage of the image pyramid,
- *             warped image, derivatives, tensor and duals p, q; arithmetic stays FP32.
- * No counterpart in the reference (FP32 only, stereotgv.h:47-100). */
+ *             warped image, derivatives, tensor and duals p, q; arithmetic stays FP32;
+ *   "autotune" 1 = pick each level's tile shape (and Tp prefetch) by timing the inner
+ *             iteration in every shape at the first solve of a geometry (default; the
+ *             choices are cached per process), 0 = fixed 64x8 tiles;
+ *   "shape"   -1 = as "autotune" says (default), 0..6 = force one tile shape for every
+ *             level: 64x8, 64x4, 128x4, 32x16, 32x8, 16x32, 16x16.
+ * Results do not depend on "graph", "autotune" or "shape".
+ * No counterpart in the reference (FP32 only, stereotgv.h:47-100; its block shape is a
+ * constructor argument, stereotgv.cu:32-42). */
 int vfs_set_option(vfs_solver *s, const char *key, int value);
 int vfs_get_option(const vfs_solver *s, const char *key, int *value);
+
+/* The launch choices used at `level` by the masked (masked != 0) or unmasked solve:
+ * tile shape index (as option "shape") and whether the inner iteration prefetches its
+ * Tp-plane neighbours.  Runs the tuner first when it has not run for this geometry.
+ * No counterpart in the reference. */
+int vfs_get_level_tuning(vfs_solver *s, int masked, int level, int *shape, int *preload);
 
 /* Timing with HIP events on the solver's stream: runs `n` solves (masked != 0:
  * solveStereoForwardMasked) back to back on resident inputs and returns their total
@@ -184,7 +197,7 @@ typedef struct vfs_server vfs_server;
 typedef struct {
     size_t hbm_budget_bytes;  /* device memory the resident streams may use */
     int max_streams;          /* resident streams at most */
-    int max_active;           /* frames in flight at most (1..64) */
+    int max_active;           /* frames in flight at most (1..60) */
     int storage;              /* 0 FP32, 1 FP16, 2 BF16 (vfs_set_option "storage") */
     float beta, gamma, alpha0, alpha1, timestep_lambda, lambda, fScale, limitRange;
 } vfs_server_params;

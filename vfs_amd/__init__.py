@@ -52,6 +52,7 @@ EXPORTS = [
     ("vfs_time_inner_iterations", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, _c_float_p, _c_int_p]),
     ("vfs_time_copy", ctypes.c_int, [ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int, _c_float_p]),
     ("vfs_get_option", ctypes.c_int, [ctypes.c_void_p, ctypes.c_char_p, _c_int_p]),
+    ("vfs_get_level_tuning", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, _c_int_p, _c_int_p]),
     ("vfs_batch_create", ctypes.c_int, [ctypes.POINTER(ctypes.c_void_p), ctypes.POINTER(ctypes.c_void_p), ctypes.c_int]),
     ("vfs_batch_destroy", ctypes.c_int, [ctypes.c_void_p]),
     ("vfs_batch_solve", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
@@ -271,6 +272,16 @@ class StereoTgv:
         self._check(self._L.vfs_get_option(self._h, key.encode(), ctypes.byref(v)), "get_option")
         return v.value
 
+    # tile shapes of the "shape" option, by index
+    SHAPES = ((64, 8), (64, 4), (128, 4), (32, 16), (32, 8), (16, 32), (16, 16))
+
+    def level_tuning(self, level: int, masked: bool = True):
+        """(tile shape (w, h), Tp prefetch) used at `level`; runs the tuner if needed."""
+        sh, pre = ctypes.c_int(), ctypes.c_int()
+        self._check(self._L.vfs_get_level_tuning(self._h, int(masked), level, ctypes.byref(sh), ctypes.byref(pre)),
+                    "get_level_tuning")
+        return self.SHAPES[sh.value], bool(pre.value)
+
     def time_solves(self, masked: bool = True, n: int = 1) -> float:
         ms = ctypes.c_float()
         self._check(self._L.vfs_time_solves(self._h, 1 if masked else 0, n, ctypes.byref(ms)), "time_solves")
@@ -342,7 +353,7 @@ class Batch:
 class Server:
     """Continuous-batching server for camera streams (vfs_server_* of include/vfs.h)."""
 
-    def __init__(self, params: dict, hbm_budget_bytes: int = 64 << 30, max_streams: int = 64, max_active: int = 64,
+    def __init__(self, params: dict, hbm_budget_bytes: int = 64 << 30, max_streams: int = 64, max_active: int = 60,
                  storage="f32"):
         self._L = lib()
         sp = _ServerParams()

@@ -1,10 +1,9 @@
 // gfx950 kernels of the StereoTgv TGV-L1 fisheye-stereo solver.
 //
-// Every per-pixel kernel of the solve works on a *set of frame pairs* at one pyramid
-// level (see vfs_engine.h): workgroups of 256 work-items (4 wave64) own 64x8 tiles,
-// each wave covering contiguous 64-pixel row segments, and a per-level prefix table maps
-// the flat workgroup index to (pair, tile).  Pairs of different resolutions therefore
-// share one launch.
+// Every per-pixel op of the solve runs inside one tick kernel over a *set of frame pairs*
+// (see vfs_engine.h): workgroups of 256 work-items (4 wave64) own tiles of the launch's
+// shape (64x8 by default), and a prefix table maps the flat workgroup index to
+// (pair, tile).  Pairs of different resolutions therefore share one launch.
 //
 // Texture fetches of the reference (linear filter, mirror addressing, normalized
 // coordinates) are evaluated in registers: exact bilinear weights at pixel-index
@@ -16,13 +15,25 @@
 
 #include <math.h>
 
+// Every product and sum is rounded as written (no multiply-add fusion chosen by the
+// compiler): a pixel's value is computed by the same rounding sequence in every tile
+// shape and in every inlined copy of an op (tile pixel or recomputed halo), so results
+// do not depend on the launch shape.
+#pragma clang fp contract(off)
+
 namespace vfs {
 
 namespace {
 
-constexpr int BX = 64;
-constexpr int BY = 4;
-constexpr int NT = BX * BY;
+constexpr int NT = 256;  // work-items per workgroup
+
+// Work-item layout of tile shape SH: BX x BY work-items, RPT rows each.
+template <int SH>
+struct Sh {
+    static constexpr int TW = kShape[SH].tw, TH = kShape[SH].th;
+    static constexpr int BX = TW, BY = NT / TW, RPT = TH / BY;
+    static_assert(BX * BY == NT && (RPT == 1 || RPT == 2) && BY * RPT == TH, "tile shape");
+};
 
 // ------------------------------------------------------------------ storage
 
@@ -176,48 +187,50 @@ __device__ __forceinline__ float2 ld0(const float2 *__restrict__ b, long i)
 
 struct Tile {
     const PairDesc *d;
+    float *arena;        // the pair's work arena
+    long long plane;     // floats per plane slot
+    const float *mask;   // mask plane of the tile level
     int x0, y0, w, h, s;
 };
 
-__device__ __forceinline__ float *slot(const PairDesc *d, int k) { return d->arena + (long long)k * d->plane; }
+__device__ __forceinline__ float *slot(const Tile &t, int k) { return t.arena + (long long)k * t.plane; }
 
 template <typename P>
-__device__ __forceinline__ P *slotp(const PairDesc *d, int k)
+__device__ __forceinline__ P *slotp(const Tile &t, int k)
 {
-    return reinterpret_cast<P *>(d->arena + (long long)k * d->plane);
+    return reinterpret_cast<P *>(t.arena + (long long)k * t.plane);
 }
 
-// Iterates the calling work-item over its pixels of the tile (2 rows each).
-#define FOR_TILE_PIXELS(t, x, y)                                                      \
+// Iterates the calling work-item over its pixels of the tile (S::RPT rows).
+#define FOR_TILE_PIXELS(S, t, x, y)                                                   \
     for (int y = (t).y0 + (int)threadIdx.y, x = (t).x0 + (int)threadIdx.x;            \
-         y < (t).y0 + kTileH && y < (t).h && x < (t).w; y += BY)
-
-inline dim3 block2d() { return dim3(BX, BY); }
+         y < (t).y0 + S::TH && y < (t).h && x < (t).w; y += S::BY)
 
 // ------------------------------------------------------------------ solve ops
 //
-// Each op works on one 64x8 tile of one pair; k_tick (below) dispatches them.
+// Each op works on one tile (shape S) of one pair; k_tick (below) dispatches them.
 
 // Fresh state: zero every work plane of the pair at the tile's level-0 positions
 // (the tile grid spans the stride, so row padding is cleared too).
+template <class S>
 __device__ __forceinline__ void op_zero(const Tile &t)
 {
-    FOR_TILE_PIXELS(t, x, y)
+    FOR_TILE_PIXELS(S, t, x, y)
     {
         const long pos = (long)y * t.s + x;
 #pragma unroll 5
-        for (int k = 0; k < PL_COUNT; k++) slot(t.d, k)[pos] = 0.f;
+        for (int k = 0; k < PL_COUNT; k++) slot(t, k)[pos] = 0.f;
     }
 }
 
 // Calibration warp of I1 (stereotgv.cu:392-393, warpImage.cu:9-23) into the level-0
 // pyramid slot; with 16-bit storage also narrows I0 into its pyramid slot.
-template <typename T>
+template <class S, typename T>
 __device__ __forceinline__ void op_calib(const Tile &t)
 {
     const float *i1 = t.d->in_i1;
     T *o1 = (T *)t.d->I1[0];
-    FOR_TILE_PIXELS(t, x, y)
+    FOR_TILE_PIXELS(S, t, x, y)
     {
         const long pos = (long)y * t.s + x;
         const float2 c = t.d->cv[pos];
@@ -228,34 +241,35 @@ __device__ __forceinline__ void op_calib(const Tile &t)
 
 // downscale.cu:10-30 for I0 and I1 (stereotgv.cu:400-405): mean of 4 bilinear taps at
 // +-1/4 coarse pixel around the coarse pixel centre, fetched from the finer level.
-template <typename T>
+template <class S, typename T>
 __device__ __forceinline__ void op_down(const Tile &t, int level)
 {
-    const int W = t.d->w[level - 1], H = t.d->h[level - 1], S = t.d->s[level - 1];
+    const int W = t.d->w[level - 1], H = t.d->h[level - 1], SF = t.d->s[level - 1];
     const T *a = (const T *)t.d->I0[level - 1];
     const T *b = (const T *)t.d->I1[level - 1];
     T *oa = (T *)t.d->I0[level];
     T *ob = (T *)t.d->I1[level];
     const float rx = (float)W / (float)t.w, ry = (float)H / (float)t.h;
     const float ox = 0.25f * rx, oy = 0.25f * ry;
-    FOR_TILE_PIXELS(t, x, y)
+    FOR_TILE_PIXELS(S, t, x, y)
     {
         const float cx = ((float)x + 0.5f) * rx - 0.5f, cy = ((float)y + 0.5f) * ry - 0.5f;
         const long pos = (long)y * t.s + x;
-        stT(oa, pos, 0.25f * (bil(a, W, H, S, cx - ox, cy) + bil(a, W, H, S, cx + ox, cy) +
-                              bil(a, W, H, S, cx, cy - oy) + bil(a, W, H, S, cx, cy + oy)));
-        stT(ob, pos, 0.25f * (bil(b, W, H, S, cx - ox, cy) + bil(b, W, H, S, cx + ox, cy) +
-                              bil(b, W, H, S, cx, cy - oy) + bil(b, W, H, S, cx, cy + oy)));
+        stT(oa, pos, 0.25f * (bil(a, W, H, SF, cx - ox, cy) + bil(a, W, H, SF, cx + ox, cy) +
+                              bil(a, W, H, SF, cx, cy - oy) + bil(a, W, H, SF, cx, cy + oy)));
+        stT(ob, pos, 0.25f * (bil(b, W, H, SF, cx - ox, cy) + bil(b, W, H, SF, cx + ox, cy) +
+                              bil(b, W, H, SF, cx, cy - oy) + bil(b, W, H, SF, cx, cy + oy)));
     }
 }
 
 // Coarsest-level init warpUV = u * tv (stereotgv.cu:419-421, warpImage.cu:136-150).
+template <class S>
 __device__ __forceinline__ void op_cinit(const Tile &t, int level, int roles)
 {
-    const float *u = slot(t.d, (roles & RB_U) ? PL_U1 : PL_U0);
-    float2 *uv = slotp<float2>(t.d, (roles & RB_UV) ? PL_UV1 : PL_UV0);
+    const float *u = slot(t, (roles & RB_U) ? PL_U1 : PL_U0);
+    float2 *uv = slotp<float2>(t, (roles & RB_UV) ? PL_UV1 : PL_UV0);
     const float2 *tv = t.d->tv[level];
-    FOR_TILE_PIXELS(t, x, y)
+    FOR_TILE_PIXELS(S, t, x, y)
     {
         const long pos = (long)y * t.s + x;
         const float us = u[pos];
@@ -267,27 +281,28 @@ __device__ __forceinline__ void op_cinit(const Tile &t, int level, int roles)
 // Warm start: u, u_ and warpUV of the finer level downscaled to `level` and multiplied
 // by `scale` (= 1/fScale) into the other slot of each pair: the 4-tap Downscale with a
 // scale factor of downscale.cu:148-237.
+template <class S>
 __device__ __forceinline__ void op_winit(const Tile &t, int level, int roles, float scale)
 {
-    const int W = t.d->w[level - 1], H = t.d->h[level - 1], S = t.d->s[level - 1];
-    const float *u = slot(t.d, (roles & RB_U) ? PL_U1 : PL_U0);
-    float *ou = slot(t.d, (roles & RB_U) ? PL_U0 : PL_U1);
-    const float *u_ = slot(t.d, (roles & RB_UB) ? PL_UB1 : PL_UB0);
-    float *ou_ = slot(t.d, (roles & RB_UB) ? PL_UB0 : PL_UB1);
-    const float2 *uv = slotp<float2>(t.d, (roles & RB_UV) ? PL_UV1 : PL_UV0);
-    float2 *ouv = slotp<float2>(t.d, (roles & RB_UV) ? PL_UV0 : PL_UV1);
+    const int W = t.d->w[level - 1], H = t.d->h[level - 1], SF = t.d->s[level - 1];
+    const float *u = slot(t, (roles & RB_U) ? PL_U1 : PL_U0);
+    float *ou = slot(t, (roles & RB_U) ? PL_U0 : PL_U1);
+    const float *u_ = slot(t, (roles & RB_UB) ? PL_UB1 : PL_UB0);
+    float *ou_ = slot(t, (roles & RB_UB) ? PL_UB0 : PL_UB1);
+    const float2 *uv = slotp<float2>(t, (roles & RB_UV) ? PL_UV1 : PL_UV0);
+    float2 *ouv = slotp<float2>(t, (roles & RB_UV) ? PL_UV0 : PL_UV1);
     const float rx = (float)W / (float)t.w, ry = (float)H / (float)t.h;
     const float ox = 0.25f * rx, oy = 0.25f * ry;
-    FOR_TILE_PIXELS(t, x, y)
+    FOR_TILE_PIXELS(S, t, x, y)
     {
         const float cx = ((float)x + 0.5f) * rx - 0.5f, cy = ((float)y + 0.5f) * ry - 0.5f;
         const long pos = (long)y * t.s + x;
-        ou[pos] = scale * 0.25f * (bil(u, W, H, S, cx - ox, cy) + bil(u, W, H, S, cx + ox, cy) +
-                                   bil(u, W, H, S, cx, cy - oy) + bil(u, W, H, S, cx, cy + oy));
-        ou_[pos] = scale * 0.25f * (bil(u_, W, H, S, cx - ox, cy) + bil(u_, W, H, S, cx + ox, cy) +
-                                    bil(u_, W, H, S, cx, cy - oy) + bil(u_, W, H, S, cx, cy + oy));
-        const float2 v0 = bil2(uv, W, H, S, cx - ox, cy), v1 = bil2(uv, W, H, S, cx + ox, cy);
-        const float2 v2 = bil2(uv, W, H, S, cx, cy - oy), v3 = bil2(uv, W, H, S, cx, cy + oy);
+        ou[pos] = scale * 0.25f * (bil(u, W, H, SF, cx - ox, cy) + bil(u, W, H, SF, cx + ox, cy) +
+                                   bil(u, W, H, SF, cx, cy - oy) + bil(u, W, H, SF, cx, cy + oy));
+        ou_[pos] = scale * 0.25f * (bil(u_, W, H, SF, cx - ox, cy) + bil(u_, W, H, SF, cx + ox, cy) +
+                                    bil(u_, W, H, SF, cx, cy - oy) + bil(u_, W, H, SF, cx, cy + oy));
+        const float2 v0 = bil2(uv, W, H, SF, cx - ox, cy), v1 = bil2(uv, W, H, SF, cx + ox, cy);
+        const float2 v2 = bil2(uv, W, H, SF, cx, cy - oy), v3 = bil2(uv, W, H, SF, cx, cy + oy);
         ouv[pos] = make_float2(scale * 0.25f * (v0.x + v1.x + v2.x + v3.x), scale * 0.25f * (v0.y + v1.y + v2.y + v3.y));
     }
 }
@@ -324,16 +339,16 @@ __device__ __forceinline__ float smoothed(const T *__restrict__ I, const float *
 // reference computes it, 0 where the reference leaves its per-level memset (mask == 0,
 // or columns x >= xlimit: calcTensor.cu:81,227 size the launch by the HEIGHT).  The
 // preconditioners eta (preconditioning.cu) are recomputed from (a, b, c) where needed.
-template <typename T, bool MASKED>
+template <class S, typename T, bool MASKED>
 __device__ __forceinline__ void op_tensor(const Tile &t, int level, int xblock, float beta, float gamma)
 {
     const T *I0 = (const T *)t.d->I0[level];
-    const float *mask = t.d->mask[level];
-    T *at = (T *)slot(t.d, PL_A);
-    T *bt = (T *)slot(t.d, PL_B);
-    T *ct = (T *)slot(t.d, PL_C);
+    const float *mask = t.mask;
+    T *at = (T *)slot(t, PL_A);
+    T *bt = (T *)slot(t, PL_B);
+    T *ct = (T *)slot(t, PL_C);
     const int xlimit = xblock * ((t.h + xblock - 1) / xblock);
-    FOR_TILE_PIXELS(t, x, y)
+    FOR_TILE_PIXELS(S, t, x, y)
     {
         const long pos = (long)y * t.s + x;
         float a = 0.f, b = 0.f, c = 0.f;
@@ -366,18 +381,18 @@ __device__ __forceinline__ void op_tensor(const Tile &t, int level, int xblock, 
 // Per-warp preparation (stereotgv.cu:431-447): tv2 = bilinear(tv, x + warpUV)
 // (warpImage.cu:96-111), I1w = bilinear(I1, x + warpUV) (warpImage.cu:9-23),
 // u_last = u (copyClone.cu:3-12), v_ = 0 (Clone of the zeroed v).
-template <typename T>
+template <class S, typename T>
 __device__ __forceinline__ void op_prep(const Tile &t, int level, int roles)
 {
-    const float2 *uv = slotp<float2>(t.d, (roles & RB_UV) ? PL_UV1 : PL_UV0);
-    const float *u = slot(t.d, (roles & RB_U) ? PL_U1 : PL_U0);
-    float2 *v_ = slotp<float2>(t.d, (roles & RB_V) ? PL_V1 : PL_V0);
+    const float2 *uv = slotp<float2>(t, (roles & RB_UV) ? PL_UV1 : PL_UV0);
+    const float *u = slot(t, (roles & RB_U) ? PL_U1 : PL_U0);
+    float2 *v_ = slotp<float2>(t, (roles & RB_V) ? PL_V1 : PL_V0);
     const float2 *tv = t.d->tv[level];
     const T *I1 = (const T *)t.d->I1[level];
-    float2 *tv2 = slotp<float2>(t.d, PL_TV2);
-    T *i1w = (T *)slot(t.d, PL_I1W);
-    float *ulast = slot(t.d, PL_ULAST);
-    FOR_TILE_PIXELS(t, x, y)
+    float2 *tv2 = slotp<float2>(t, PL_TV2);
+    T *i1w = (T *)slot(t, PL_I1W);
+    float *ulast = slot(t, PL_ULAST);
+    FOR_TILE_PIXELS(S, t, x, y)
     {
         const long pos = (long)y * t.s + x;
         const float2 d = uv[pos];
@@ -391,15 +406,15 @@ __device__ __forceinline__ void op_prep(const Tile &t, int level, int roles)
 
 // computeDerivatives.cu:88-129: 5-point derivative along t = tv/|tv| of I0 and I1w,
 // averaged; Iz = I1w - I0 at the texel centre.
-template <typename T>
+template <class S, typename T>
 __device__ __forceinline__ void op_deriv(const Tile &t, int level)
 {
     const T *I0 = (const T *)t.d->I0[level];
-    const T *I1w = (const T *)slot(t.d, PL_I1W);
+    const T *I1w = (const T *)slot(t, PL_I1W);
     const float2 *tv = t.d->tv[level];
-    T *Iu = (T *)slot(t.d, PL_IU);
-    T *Iz = (T *)slot(t.d, PL_IZ);
-    FOR_TILE_PIXELS(t, x, y)
+    T *Iu = (T *)slot(t, PL_IU);
+    T *Iz = (T *)slot(t, PL_IZ);
+    FOR_TILE_PIXELS(S, t, x, y)
     {
         const long pos = (long)y * t.s + x;
         const float2 v = tv[pos];
@@ -443,8 +458,6 @@ __device__ __forceinline__ float thresh_core(float divp, float tau, float etau, 
     return upos + du;
 }
 
-constexpr int EXW = kTileW + 1, EXH = kTileH + 1;
-constexpr int HALO = EXW + kTileH;  // top row (incl. corner) + left column positions
 
 // Operands of the dual update at one pixel, gathered with in-buffer (clamped) addresses
 // and no data-dependent branch, so that all loads of a work-item issue back to back.
@@ -551,45 +564,49 @@ __device__ __forceinline__ void dual_step(const DualIn &r, int x, int y, int w, 
 // (updateDualVariables.cu:122-237 / :4-101), Tp (thresholdingL1.cu:234-244), L1
 // thresholding (thresholdingL1.cu:128-216 / :3-86) and the primal step with
 // over-relaxation (updatePrimalVariables.cu:138-245 / :3-113).
-// Phase A computes the new p, q and Tp of the 64x8 tile and of its left column and top
-// row (the threshold and primal of a pixel read Tp and q of its left and upper
-// neighbours) into LDS: each work-item takes the two tile pixels it finishes in phase B,
-// and 73 work-items also one halo position.  Phase B finishes thresholding and the primal
-// step.  Old duals are read from one buffer and new ones written to the other, so every
-// workgroup recomputes its halo from the previous iteration's values (the reference's
-// Jacobi order).  Every load is issued before any data-dependent branch.
-template <bool MASKED, typename T>
-__device__ __forceinline__ void op_iter(const Tile &t, int level, int roles, const IterParams &ip, float2 (*sP)[EXW],
-                                        float4 (*sQ)[EXW], float2 (*sT)[EXW])
+// Phase A computes the new p, q and Tp of the tile and of its left column and top row
+// (the threshold and primal of a pixel read Tp and q of its left and upper neighbours)
+// into LDS: each work-item takes the S::RPT tile pixels it finishes in phase B, and the
+// first TW + TH + 1 work-items also one halo position.  Phase B finishes thresholding and
+// the primal step.  Old duals are read from one buffer and new ones written to the other,
+// so every workgroup recomputes its halo from the previous iteration's values (the
+// reference's Jacobi order) and the result does not depend on the tile shape.  Every
+// load is issued before any data-dependent branch.
+template <class S, bool MASKED, typename T>
+__device__ __forceinline__ void op_iter(const Tile &t, int roles, const IterParams &ip,
+                                        float2 (*sP)[S::TW + 1], float4 (*sQ)[S::TW + 1], float2 (*sT)[S::TW + 1])
 {
+    constexpr int EXW = S::TW + 1;
+    constexpr int HALO = EXW + S::TH;  // top row (incl. corner) + left column positions
+    static_assert(HALO <= NT, "halo positions per workgroup");
+    constexpr int BX = S::BX, BY = S::BY, RPT = S::RPT;
     const int w = t.w, h = t.h, s = t.s, x0 = t.x0, y0 = t.y0;
-    const PairDesc *d = t.d;
-    const float *__restrict__ mask = d->mask[level];
-    const float *__restrict__ u = slot(d, (roles & RB_U) ? PL_U1 : PL_U0);
-    float *__restrict__ us = slot(d, (roles & RB_U) ? PL_U0 : PL_U1);
-    const float *__restrict__ u_ = slot(d, (roles & RB_UB) ? PL_UB1 : PL_UB0);
-    float *__restrict__ u_s = slot(d, (roles & RB_UB) ? PL_UB0 : PL_UB1);
-    const float2 *__restrict__ v_ = slotp<float2>(d, (roles & RB_V) ? PL_V1 : PL_V0);
-    float2 *__restrict__ v_s = slotp<float2>(d, (roles & RB_V) ? PL_V0 : PL_V1);
-    const V2<T> *__restrict__ p_in = slotp<V2<T>>(d, (roles & RB_PQ) ? PL_P1 : PL_P0);
-    V2<T> *__restrict__ p_out = slotp<V2<T>>(d, (roles & RB_PQ) ? PL_P0 : PL_P1);
-    const V4<T> *__restrict__ q_in = slotp<V4<T>>(d, (roles & RB_PQ) ? PL_Q1 : PL_Q0);
-    V4<T> *__restrict__ q_out = slotp<V4<T>>(d, (roles & RB_PQ) ? PL_Q0 : PL_Q1);
-    float2 *__restrict__ Tp = slotp<float2>(d, PL_TP);
-    const T *__restrict__ at = (const T *)slot(d, PL_A);
-    const T *__restrict__ bt = (const T *)slot(d, PL_B);
-    const T *__restrict__ ct = (const T *)slot(d, PL_C);
-    const T *__restrict__ Iu = (const T *)slot(d, PL_IU);
-    const T *__restrict__ Iz = (const T *)slot(d, PL_IZ);
+    const float *__restrict__ mask = t.mask;
+    const float *__restrict__ u = slot(t, (roles & RB_U) ? PL_U1 : PL_U0);
+    float *__restrict__ us = slot(t, (roles & RB_U) ? PL_U0 : PL_U1);
+    const float *__restrict__ u_ = slot(t, (roles & RB_UB) ? PL_UB1 : PL_UB0);
+    float *__restrict__ u_s = slot(t, (roles & RB_UB) ? PL_UB0 : PL_UB1);
+    const float2 *__restrict__ v_ = slotp<float2>(t, (roles & RB_V) ? PL_V1 : PL_V0);
+    float2 *__restrict__ v_s = slotp<float2>(t, (roles & RB_V) ? PL_V0 : PL_V1);
+    const V2<T> *__restrict__ p_in = slotp<V2<T>>(t, (roles & RB_PQ) ? PL_P1 : PL_P0);
+    V2<T> *__restrict__ p_out = slotp<V2<T>>(t, (roles & RB_PQ) ? PL_P0 : PL_P1);
+    const V4<T> *__restrict__ q_in = slotp<V4<T>>(t, (roles & RB_PQ) ? PL_Q1 : PL_Q0);
+    V4<T> *__restrict__ q_out = slotp<V4<T>>(t, (roles & RB_PQ) ? PL_Q0 : PL_Q1);
+    float2 *__restrict__ Tp = slotp<float2>(t, PL_TP);
+    const T *__restrict__ at = (const T *)slot(t, PL_A);
+    const T *__restrict__ bt = (const T *)slot(t, PL_B);
+    const T *__restrict__ ct = (const T *)slot(t, PL_C);
+    const T *__restrict__ Iu = (const T *)slot(t, PL_IU);
+    const T *__restrict__ Iz = (const T *)slot(t, PL_IZ);
     const int tid = threadIdx.y * BX + threadIdx.x;
     const int x = x0 + threadIdx.x;
 
-    // ---- loads: two tile pixels (rows threadIdx.y and threadIdx.y + 4) + one halo position
-    DualIn in[2];
-    float uu[2], iu[2], iz[2];
-    bool hasL[2], hasU[2];
+    // ---- loads: the tile pixels (rows threadIdx.y + BY * r) + one halo position
+    DualIn in[RPT];
+    float uu[RPT], iu[RPT], iz[RPT], tpl[RPT], tpu[RPT];
+    bool hasL[RPT], hasU[RPT];
 #pragma unroll
-    for (int r = 0; r < 2; r++) {
+    for (int r = 0; r < RPT; r++) {
         const int y = y0 + threadIdx.y + BY * r;
         dual_load<MASKED, T>(in[r], x, y, w, h, s, mask, u_, v_, at, bt, ct, p_in, q_in, ip.first);
         const long pos = in[r].pos;
@@ -603,6 +620,14 @@ __device__ __forceinline__ void op_iter(const Tile &t, int level, int roles, con
         } else {
             hasL[r] = xc >= 1;
             hasU[r] = yc >= 1;
+        }
+        // Tp-plane values of the left / upper neighbour: read only where that neighbour
+        // is outside the mask or the image; fetched here on latency-bound levels
+        tpl[r] = 0.f;
+        tpu[r] = 0.f;
+        if (ip.preload_tp) {
+            tpl[r] = ld0(Tp, pos - 1).x;
+            tpu[r] = ld0(Tp, pos - s).y;
         }
     }
     DualIn hin;
@@ -623,7 +648,7 @@ __device__ __forceinline__ void op_iter(const Tile &t, int level, int roles, con
 
     // ---- phase A: new duals + Tp into LDS (and the tile's to global memory)
 #pragma unroll
-    for (int r = 0; r < 2; r++) {
+    for (int r = 0; r < RPT; r++) {
         const int y = y0 + threadIdx.y + BY * r;
         if (!in[r].act) continue;
         float2 pn, tpn;
@@ -650,7 +675,7 @@ __device__ __forceinline__ void op_iter(const Tile &t, int level, int roles, con
     // ---- phase B: thresholding + primal step on the tile
     const float aa = ip.alpha1 * ip.alpha1;
 #pragma unroll
-    for (int r = 0; r < 2; r++) {
+    for (int r = 0; r < RPT; r++) {
         const DualIn &q = in[r];
         const int y = y0 + threadIdx.y + BY * r;
         if (!q.ok) continue;
@@ -668,8 +693,8 @@ __device__ __forceinline__ void op_iter(const Tile &t, int level, int roles, con
         const float2 tp0 = sT[ey][ex];
         // a neighbour outside the mask (or the image) is read from the Tp plane, which
         // holds what earlier levels left there, as the reference does
-        const float tpLx = hL ? sT[ey][ex - 1].x : ld0(Tp, pos - 1).x;
-        const float tpUy = hU ? sT[ey - 1][ex].y : ld0(Tp, pos - s).y;
+        const float tpLx = hL ? sT[ey][ex - 1].x : (ip.preload_tp ? tpl[r] : ld0(Tp, pos - 1).x);
+        const float tpUy = hU ? sT[ey - 1][ex].y : (ip.preload_tp ? tpu[r] : ld0(Tp, pos - s).y);
         float dx, dy;
         if (MASKED) {
             if (hL && hR) dx = tp0.x - tpLx;
@@ -779,16 +804,16 @@ __device__ __forceinline__ float median5_at(const float *__restrict__ u, int x, 
 // (medianFilter.cu:264-319) into the other u slot,] du = min(u - u_last, limit)
 // (scalar.cu:129-141, limitRange.cu:3-24), u = u_last + du (scalar.cu:77-89),
 // u_ = u (copyClone.cu:3-12), warpUV += du * tv2 (warpImage.cu:136-150, scalar.cu:102-115).
-template <bool MASKED>
+template <class S, bool MASKED>
 __device__ __forceinline__ void op_end(const Tile &t, int roles, float limit)
 {
-    const float *u = slot(t.d, (roles & RB_U) ? PL_U1 : PL_U0);
-    float *out = MASKED ? slot(t.d, (roles & RB_U) ? PL_U0 : PL_U1) : (float *)u;
-    float *u_ = slot(t.d, (roles & RB_UB) ? PL_UB1 : PL_UB0);
-    float2 *uv = slotp<float2>(t.d, (roles & RB_UV) ? PL_UV1 : PL_UV0);
-    const float *ulast = slot(t.d, PL_ULAST);
-    const float2 *tv2 = slotp<float2>(t.d, PL_TV2);
-    FOR_TILE_PIXELS(t, x, y)
+    const float *u = slot(t, (roles & RB_U) ? PL_U1 : PL_U0);
+    float *out = MASKED ? slot(t, (roles & RB_U) ? PL_U0 : PL_U1) : (float *)u;
+    float *u_ = slot(t, (roles & RB_UB) ? PL_UB1 : PL_UB0);
+    float2 *uv = slotp<float2>(t, (roles & RB_UV) ? PL_UV1 : PL_UV0);
+    const float *ulast = slot(t, PL_ULAST);
+    const float2 *tv2 = slotp<float2>(t, PL_TV2);
+    FOR_TILE_PIXELS(S, t, x, y)
     {
         const long pos = (long)y * t.s + x;
         const float uval = MASKED ? median5_at(u, x, y, t.w, t.h, t.s) : u[pos];
@@ -806,17 +831,18 @@ __device__ __forceinline__ void op_end(const Tile &t, int roles, float limit)
 
 // upscale.cu:7-21, 46-61 for u, u_ and warpUV (stereotgv.cu:497-510): bilinear at the
 // finer level's pixel centres, times fScale; writes the other slot of each pair.
+template <class S>
 __device__ __forceinline__ void op_up(const Tile &t, int level, int roles, float scale)
 {  // the tile belongs to the finer level level - 1
     const int w = t.d->w[level], h = t.d->h[level], s = t.d->s[level];
-    const float *u = slot(t.d, (roles & RB_U) ? PL_U1 : PL_U0);
-    float *ou = slot(t.d, (roles & RB_U) ? PL_U0 : PL_U1);
-    const float *u_ = slot(t.d, (roles & RB_UB) ? PL_UB1 : PL_UB0);
-    float *ou_ = slot(t.d, (roles & RB_UB) ? PL_UB0 : PL_UB1);
-    const float2 *uv = slotp<float2>(t.d, (roles & RB_UV) ? PL_UV1 : PL_UV0);
-    float2 *ouv = slotp<float2>(t.d, (roles & RB_UV) ? PL_UV0 : PL_UV1);
+    const float *u = slot(t, (roles & RB_U) ? PL_U1 : PL_U0);
+    float *ou = slot(t, (roles & RB_U) ? PL_U0 : PL_U1);
+    const float *u_ = slot(t, (roles & RB_UB) ? PL_UB1 : PL_UB0);
+    float *ou_ = slot(t, (roles & RB_UB) ? PL_UB0 : PL_UB1);
+    const float2 *uv = slotp<float2>(t, (roles & RB_UV) ? PL_UV1 : PL_UV0);
+    float2 *ouv = slotp<float2>(t, (roles & RB_UV) ? PL_UV0 : PL_UV1);
     const float rx = (float)w / (float)t.w, ry = (float)h / (float)t.h;
-    FOR_TILE_PIXELS(t, x, y)
+    FOR_TILE_PIXELS(S, t, x, y)
     {
         const float px = ((float)x + 0.5f) * rx - 0.5f, py = ((float)y + 0.5f) * ry - 0.5f;
         const long pos = (long)y * t.s + x;
@@ -835,60 +861,66 @@ __device__ __forceinline__ void op_up(const Tile &t, int level, int roles, float
 // pair; the serving loop passes each active request's next op, so requests at different
 // points of their schedules share launches.  Each pair's result depends only on its own
 // entries, never on what runs beside it.
-template <typename T>
+template <typename T, int SH>
 __global__ __launch_bounds__(NT) void k_tick(const PairDesc *__restrict__ D, const TickArgs A)
 {
-    __shared__ float2 sP[EXH][EXW];
-    __shared__ float4 sQ[EXH][EXW];
-    __shared__ float2 sT[EXH][EXW];
+    using S = Sh<SH>;
+    __shared__ float2 sP[S::TH + 1][S::TW + 1];
+    __shared__ float4 sQ[S::TH + 1][S::TW + 1];
+    __shared__ float2 sT[S::TH + 1][S::TW + 1];
     // Workgroups are dealt to the 8 XCDs round-robin; renumber them so that each XCD
     // owns one contiguous run of tiles (whole bands of rows), keeping the halo rows that
     // neighbouring tiles share in the same L2.
     const int nb = gridDim.x, i = blockIdx.x, xcd = i & 7, q = nb >> 3, r = nb & 7;
     const int b = (xcd < r ? xcd * (q + 1) : r * (q + 1) + (xcd - r) * q) + (i >> 3);
-    int lo = 0, hi = A.n - 1;
-    while (lo < hi) {
-        const int mid = (lo + hi + 1) >> 1;
-        if (A.e[mid].tile_start <= b) lo = mid;
-        else hi = mid - 1;
+    // Entry of this workgroup: the last one whose first tile is <= b.  Each lane of the
+    // wave reads one entry's tile_start (kMaxEntries <= 64), so the lookup costs one load.
+    int lo = 0;
+    if (A.n > 1) {
+        const int lane = __lane_id();
+        const int ts = lane < A.n ? A.e[lane].tile_start : 0x7fffffff;
+        lo = __popcll(__ballot(ts <= b)) - 1;
+        lo = __builtin_amdgcn_readfirstlane(lo);
     }
     const TickEntry &E = A.e[lo];
     const int kind = E.kind, level = E.level;
-    const int tl = kind == OP_UP ? level - 1 : level;
     Tile t;
     t.d = D + E.pair;
-    t.w = kind == OP_ZERO ? t.d->s[0] : t.d->w[tl];
-    t.h = t.d->h[tl];
-    t.s = t.d->s[tl];
+    t.arena = E.arena;
+    t.plane = E.plane;
+    t.mask = E.mask;
+    t.w = E.w;
+    t.h = E.h;
+    t.s = E.s;
     const int local = b - E.tile_start;
-    const int ntx = (t.w + kTileW - 1) / kTileW;
-    t.x0 = (local % ntx) * kTileW;
-    t.y0 = (local / ntx) * kTileH;
+    const int ntx = (t.w + S::TW - 1) / S::TW;
+    t.x0 = (local % ntx) * S::TW;
+    t.y0 = (local / ntx) * S::TH;
     switch (kind) {
     case OP_ZERO:
-        op_zero(t);
+        op_zero<S>(t);
         break;
     case OP_CALIB:
-        op_calib<T>(t);
+        op_calib<S, T>(t);
         break;
     case OP_DOWN:
-        op_down<T>(t, level);
+        op_down<S, T>(t, level);
         break;
     case OP_CINIT:
-        op_cinit(t, level, E.roles);
+        op_cinit<S>(t, level, E.roles);
         break;
     case OP_WINIT:
-        op_winit(t, level, E.roles, A.scale);
+        op_winit<S>(t, level, E.roles, A.scale);
         break;
     case OP_TENSOR:
-        if (E.masked) op_tensor<T, true>(t, level, A.xblock, A.beta, A.gamma);
-        else op_tensor<T, false>(t, level, A.xblock, A.beta, A.gamma);
+        if (E.masked) op_tensor<S, T, true>(t, level, A.xblock, A.beta, A.gamma);
+        else op_tensor<S, T, false>(t, level, A.xblock, A.beta, A.gamma);
         break;
     case OP_PREP:
-        op_prep<T>(t, level, E.roles);
+        op_prep<S, T>(t, level, E.roles);
         break;
     case OP_DERIV:
-        op_deriv<T>(t, level);
+        op_deriv<S, T>(t, level);
         break;
     case OP_ITER: {
         IterParams ip;
@@ -899,18 +931,19 @@ __global__ __launch_bounds__(NT) void k_tick(const PairDesc *__restrict__ D, con
         ip.lambda = A.lambda;
         ip.alpha0 = A.alpha0;
         ip.alpha1 = A.alpha1;
-        ip.first = E.flags & 1;
-        ip.write_tp = (E.flags >> 1) & 1;
-        if (E.masked) op_iter<true, T>(t, level, E.roles, ip, sP, sQ, sT);
-        else op_iter<false, T>(t, level, E.roles, ip, sP, sQ, sT);
+        ip.first = E.flags & EF_FIRST;
+        ip.write_tp = (E.flags & EF_WRITE_TP) != 0;
+        ip.preload_tp = (E.flags & EF_PRELOAD_TP) != 0;
+        if (E.masked) op_iter<S, true, T>(t, E.roles, ip, sP, sQ, sT);
+        else op_iter<S, false, T>(t, E.roles, ip, sP, sQ, sT);
         break;
     }
     case OP_END:
-        if (E.masked) op_end<true>(t, E.roles, A.limit);
-        else op_end<false>(t, E.roles, A.limit);
+        if (E.masked) op_end<S, true>(t, E.roles, A.limit);
+        else op_end<S, false>(t, E.roles, A.limit);
         break;
     case OP_UP:
-        op_up(t, level, E.roles, A.fscale);
+        op_up<S>(t, level, E.roles, A.fscale);
         break;
     default:
         break;
@@ -919,13 +952,17 @@ __global__ __launch_bounds__(NT) void k_tick(const PairDesc *__restrict__ D, con
 
 // ------------------------------------------------------------------ single-plane kernels
 
+// 2-D launch layout of the single-plane kernels: 64 x 4 work-items per workgroup.
+constexpr int PBX = 64, PBY = 4;
+
 __device__ __forceinline__ bool pixel(int &x, int &y, int w, int h)
 {
-    x = blockIdx.x * BX + threadIdx.x;
-    y = blockIdx.y * BY + threadIdx.y;
+    x = blockIdx.x * PBX + threadIdx.x;
+    y = blockIdx.y * PBY + threadIdx.y;
     return x < w && y < h;
 }
-inline dim3 grid_for(int w, int h) { return dim3((w + BX - 1) / BX, (h + BY - 1) / BY); }
+inline dim3 grid_for(int w, int h) { return dim3((w + PBX - 1) / PBX, (h + PBY - 1) / PBY); }
+inline dim3 block2d() { return dim3(PBX, PBY); }
 
 __global__ void k_fill_zero(float4 *__restrict__ p, size_t n4)
 {
@@ -1052,10 +1089,31 @@ __global__ void k_widen(const T *__restrict__ src, int nch, float *__restrict__ 
         }                                                       \
     } while (0)
 
-void launch_tick(const PairDesc *desc, const TickArgs &args, int storage, hipStream_t st)
+template <typename T, int SH>
+void tick_shape(const PairDesc *desc, const TickArgs &args, hipStream_t st)
+{
+    k_tick<T, SH><<<dim3((unsigned)args.total), dim3(Sh<SH>::BX, Sh<SH>::BY), 0, st>>>(desc, args);
+}
+
+template <typename T>
+void tick_storage(const PairDesc *desc, const TickArgs &args, int shape, hipStream_t st)
+{
+    switch (shape) {
+    case 1: tick_shape<T, 1>(desc, args, st); break;
+    case 2: tick_shape<T, 2>(desc, args, st); break;
+    case 3: tick_shape<T, 3>(desc, args, st); break;
+    case 4: tick_shape<T, 4>(desc, args, st); break;
+    case 5: tick_shape<T, 5>(desc, args, st); break;
+    case 6: tick_shape<T, 6>(desc, args, st); break;
+    default: tick_shape<T, 0>(desc, args, st); break;
+    }
+}
+static_assert(kShapes == 7, "tick_storage dispatches every shape");
+
+void launch_tick(const PairDesc *desc, const TickArgs &args, int storage, int shape, hipStream_t st)
 {
     if (args.n < 1 || args.total < 1) return;
-    STORAGE_DISPATCH(storage, (k_tick<T><<<dim3((unsigned)args.total), block2d(), 0, st>>>(desc, args)));
+    STORAGE_DISPATCH(storage, tick_storage<T>(desc, args, shape, st));
 }
 
 void launch_fill_zero(void *ptr, size_t bytes, hipStream_t st)

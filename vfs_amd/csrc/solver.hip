@@ -25,6 +25,7 @@
 #include <algorithm>
 #include <deque>
 #include <map>
+#include <mutex>
 #include <new>
 #include <string>
 #include <vector>
@@ -44,8 +45,10 @@ thread_local std::string g_create_error;
 struct GraphKey {
     int masked, warm_level, warps, iters, start_roles;
     float limit;
+    unsigned long long tuning;  // Tuning::sig() of the program's launch choices
     bool operator<(const GraphKey &o) const
     {
+        if (tuning != o.tuning) return tuning < o.tuning;
         if (masked != o.masked) return masked < o.masked;
         if (warm_level != o.warm_level) return warm_level < o.warm_level;
         if (warps != o.warps) return warps < o.warps;
@@ -73,11 +76,28 @@ struct Params {
     float beta, gamma, alpha0, alpha1, timestep_lambda, lambda, fScale, limitRange;
 };
 
-// One op of a solve program (see vfs::OpKind), with the scalars it needs.
+// One op of a solve program (see vfs::OpKind), with the scalars it needs and the tile
+// shape of its launch.
 struct ProgOp {
-    unsigned char kind, level, masked, flags;
+    unsigned char kind, level, masked, flags, shape;
     int roles;
     float k1, k0, tau, mu;
+};
+
+// Launch choices for one pyramid level: tile shape of every op at the level and whether
+// the inner iteration fetches its Tp-plane neighbours up front (vfs::EF_PRELOAD_TP).
+struct LevelTune {
+    unsigned char shape = 0, preload = 0;
+};
+
+struct Tuning {
+    LevelTune lv[kMaxLevels];
+    unsigned long long sig() const
+    {
+        unsigned long long k = 0;
+        for (int l = 0; l < kMaxLevels; l++) k |= (unsigned long long)(lv[l].shape | (lv[l].preload << 3)) << (4 * l);
+        return k;
+    }
 };
 
 struct Program {
@@ -100,6 +120,10 @@ struct vfs_solver {
     float limitRange = 1.0f;
     int storage = vfs::kF32;
     bool use_graph = true;
+    int force_shape = -1;   // option "shape": -1 = per-level tuner
+    bool autotune = true;   // option "autotune"
+    Tuning tuning[2];       // [masked] launch choices per level
+    bool tuned[2] = {false, false};
     Dims lv[kMaxLevels] = {};
 
     // persistent planes
@@ -272,7 +296,7 @@ void push_level(Program &pr, const Params &P, int masked, int l, int warps, int 
             o.kind = vfs::OP_ITER;
             o.level = (unsigned char)l;
             o.masked = (unsigned char)masked;
-            o.flags = (unsigned char)((it == 0 ? 1 : 0) | (it == iters - 1 ? 2 : 0));
+            o.flags = (unsigned char)((it == 0 ? vfs::EF_FIRST : 0) | (it == iters - 1 ? vfs::EF_WRITE_TP : 0));
             o.roles = roles;
             o.k1 = P.alpha1 * sigma / 3.0f;  // eta_p = 3 (stereotgv.cu:416)
             o.k0 = P.alpha0 * sigma / 2.0f;  // eta_q = 2 (stereotgv.cu:417)
@@ -288,12 +312,23 @@ void push_level(Program &pr, const Params &P, int masked, int l, int warps, int 
     }
 }
 
+// Sets every op's tile shape, and the Tp preload of the inner iterations, per level.
+void apply_tuning(Program &pr, const Tuning &tn)
+{
+    for (ProgOp &o : pr.ops) {
+        o.shape = tn.lv[vfs::op_tile_level(o.kind, o.level)].shape;
+        o.flags &= (unsigned char)~vfs::EF_PRELOAD_TP;
+        if (o.kind == vfs::OP_ITER && tn.lv[o.level].preload) o.flags |= vfs::EF_PRELOAD_TP;
+    }
+}
+
 // The solve program.  Cold (warm_level < 0): stereotgv.cu:389-515 (masked) or :256-387,
 // from zeroed buffers, all levels with the configured warps and iterations.
 // Warm (warm_level >= 0): the previous solve's u, u_ and warpUV are downscaled (x 1/fScale
 // per level) to warm_level and the solve runs from there to level 0 with `warps` and
 // `iters`; nothing is zeroed.
-Program build_program(const Params &P, int masked, int warm_level, int warps, int iters, int start_roles)
+Program build_program(const Params &P, int masked, int warm_level, int warps, int iters, int start_roles,
+                      const Tuning &tn)
 {
     Program pr;
     int roles = 0;
@@ -326,6 +361,7 @@ Program build_program(const Params &P, int masked, int warm_level, int warps, in
         }
     }
     pr.final_roles = roles;
+    apply_tuning(pr, tn);
     return pr;
 }
 
@@ -345,8 +381,15 @@ vfs::TickArgs tick_header(const Params &P)
     return a;
 }
 
-void set_entry(vfs::TickEntry &e, const ProgOp &o, int pair, int tile_start)
+void set_entry(vfs::TickEntry &e, const ProgOp &o, int pair, const vfs::PairDesc &d, int tile_start)
 {
+    const int tl = vfs::op_tile_level(o.kind, o.level);
+    e.arena = d.arena;
+    e.mask = d.mask[tl];
+    e.plane = d.plane;
+    e.w = (unsigned short)(o.kind == vfs::OP_ZERO ? d.s[0] : d.w[tl]);
+    e.h = (unsigned short)d.h[tl];
+    e.s = d.s[tl];
     e.pair = pair;
     e.tile_start = tile_start;
     e.kind = o.kind;
@@ -372,12 +415,12 @@ void issue_program(const Engine &e, const Params &P, const Program &pr)
             int acc = 0;
             for (int k = 0; k < cnt; k++) {
                 const vfs::PairDesc &d = e.h_desc[p0 + k];
-                set_entry(a.e[k], o, p0 + k, acc);
-                acc += vfs::op_tiles(o.kind, o.level, d.w, d.h, d.s);
+                set_entry(a.e[k], o, p0 + k, d, acc);
+                acc += vfs::op_tiles(o.kind, o.level, d.w, d.h, d.s, o.shape);
             }
             a.n = cnt;
             a.total = acc;
-            vfs::launch_tick(e.d_desc, a, P.storage, e.stream);
+            vfs::launch_tick(e.d_desc, a, P.storage, o.shape, e.stream);
         }
     }
 }
@@ -488,6 +531,137 @@ int ensure_engine(vfs_solver *s)
     return VFS_OK;
 }
 
+// ---- per-level launch tuner
+//
+// The inner iteration (the dominant op) of each level is timed in every tile shape, with
+// and without the Tp preload, on a zeroed scratch arena (the pair's own state is not
+// touched), and the fastest choice is kept.  Every op of the level then launches in that
+// shape.  Results are shared by all solvers of the process on the same device with the
+// same storage, solve form and level geometry.  Results do not depend on the choice: the
+// kernels compute every pixel identically in every shape.
+struct TuneKey {
+    int device, storage, masked, w, h, s;
+    bool operator<(const TuneKey &o) const
+    {
+        const int a[6] = {device, storage, masked, w, h, s}, b[6] = {o.device, o.storage, o.masked, o.w, o.h, o.s};
+        return std::lexicographical_compare(a, a + 6, b, b + 6);
+    }
+};
+
+std::mutex g_tune_mu;
+std::map<TuneKey, LevelTune> g_tune;
+
+// Without tuning: 64x8 tiles; the Tp preload below level 0, where launches are latency-bound.
+LevelTune default_tune(int level)
+{
+    LevelTune t;
+    t.shape = 0;
+    t.preload = level > 0 ? 1 : 0;
+    return t;
+}
+
+int measure_levels(vfs_solver *s, int masked, const std::vector<int> &levels, Tuning &tn)
+{
+    constexpr int kWarm = 3, kReps = 20, kRounds = 2;
+    const size_t bytes = sizeof(float) * (size_t)s->plane * vfs::PL_COUNT;
+    float *scratch = nullptr;
+    HIPCHK(s, hipMalloc(&scratch, bytes));
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    hipError_t r = hipEventCreate(&e0);
+    if (r == hipSuccess) r = hipEventCreate(&e1);
+    vfs::launch_fill_zero(scratch, bytes, s->stream);
+    vfs::PairDesc d = s->desc();
+    d.arena = scratch;
+    const Params P = s->params();
+    std::vector<float> best(levels.size() * vfs::kShapes * 2, 1e30f);
+    for (int round = 0; round < kRounds && r == hipSuccess; round++) {
+        for (size_t li = 0; li < levels.size() && r == hipSuccess; li++) {
+            for (int shape = 0; shape < vfs::kShapes && r == hipSuccess; shape++) {
+                for (int pre = 0; pre < 2 && r == hipSuccess; pre++) {
+                    ProgOp o{};
+                    o.kind = vfs::OP_ITER;
+                    o.level = (unsigned char)levels[li];
+                    o.masked = (unsigned char)masked;
+                    o.flags = (unsigned char)(pre ? vfs::EF_PRELOAD_TP : 0);
+                    o.shape = (unsigned char)shape;
+                    o.k1 = P.alpha1 / 3.0f;
+                    o.k0 = P.alpha0 / 2.0f;
+                    o.tau = 1.0f;
+                    o.mu = 0.8f;
+                    vfs::TickArgs a = tick_header(P);
+                    set_entry(a.e[0], o, 0, d, 0);
+                    a.n = 1;
+                    a.total = vfs::op_tiles(o.kind, o.level, d.w, d.h, d.s, shape);
+                    for (int k = 0; k < kWarm; k++) vfs::launch_tick(s->eng.d_desc, a, P.storage, shape, s->stream);
+                    r = hipEventRecord(e0, s->stream);
+                    for (int k = 0; k < kReps; k++) vfs::launch_tick(s->eng.d_desc, a, P.storage, shape, s->stream);
+                    if (r == hipSuccess) r = hipEventRecord(e1, s->stream);
+                    if (r == hipSuccess) r = hipEventSynchronize(e1);
+                    float ms = 0;
+                    if (r == hipSuccess) r = hipEventElapsedTime(&ms, e0, e1);
+                    if (r == hipSuccess) r = hipGetLastError();
+                    float &b = best[(li * vfs::kShapes + shape) * 2 + pre];
+                    b = std::min(b, ms);
+                }
+            }
+        }
+    }
+    if (r == hipSuccess) r = hipStreamSynchronize(s->stream);
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    (void)hipFree(scratch);
+    if (r != hipSuccess) return fail(s, VFS_ERR_DEVICE, "launch tuner: %s", hipGetErrorString(r));
+    for (size_t li = 0; li < levels.size(); li++) {
+        const int l = levels[li];
+        const LevelTune def = default_tune(l);
+        LevelTune pick = def;
+        float pick_ms = best[(li * vfs::kShapes + def.shape) * 2 + def.preload];
+        for (int shape = 0; shape < vfs::kShapes; shape++)
+            for (int pre = 0; pre < 2; pre++) {
+                const float ms = best[(li * vfs::kShapes + shape) * 2 + pre];
+                if (ms < 0.98f * pick_ms) {  // leave the default only for a clear gain
+                    pick.shape = (unsigned char)shape;
+                    pick.preload = (unsigned char)pre;
+                    pick_ms = ms;
+                }
+            }
+        tn.lv[l] = pick;
+    }
+    return VFS_OK;
+}
+
+// Fills s->tuning[masked] (once per initialize / option change).
+int tune_solver(vfs_solver *s, int masked)
+{
+    const int m = masked ? 1 : 0;
+    if (s->tuned[m]) return VFS_OK;
+    Tuning &tn = s->tuning[m];
+    for (int l = 0; l < kMaxLevels; l++) tn.lv[l] = default_tune(l);
+    if (s->force_shape >= 0) {
+        for (int l = 0; l < s->nLevels; l++) tn.lv[l].shape = (unsigned char)s->force_shape;
+    } else if (s->autotune) {
+        std::vector<int> todo;
+        auto key = [&](int l) { return TuneKey{s->device, s->storage, m, s->lv[l].w, s->lv[l].h, s->lv[l].s}; };
+        {
+            std::lock_guard<std::mutex> g(g_tune_mu);
+            for (int l = 0; l < s->nLevels; l++) {
+                auto it = g_tune.find(key(l));
+                if (it != g_tune.end()) tn.lv[l] = it->second;
+                else todo.push_back(l);
+            }
+        }
+        if (!todo.empty()) {
+            int rc = ensure_engine(s);
+            if (!rc) rc = measure_levels(s, m, todo, tn);
+            if (rc) return rc;
+            std::lock_guard<std::mutex> g(g_tune_mu);
+            for (int l : todo) g_tune[key(l)] = tn.lv[l];
+        }
+    }
+    s->tuned[m] = true;
+    return VFS_OK;
+}
+
 // warm_level < 0: cold solve; otherwise a warm start at warm_level with warps/iters.
 int solve_common(vfs_solver *s, int masked, int warm_level = -1, int warps = 0, int iters = 0)
 {
@@ -501,9 +675,11 @@ int solve_common(vfs_solver *s, int masked, int warm_level = -1, int warps = 0, 
     if ((rc = ensure_engine(s))) return rc;
     HIPCHK(s, refresh_raw_i1(s, s->stream));
     const Params P = s->params();
+    if ((rc = tune_solver(s, masked))) return rc;
+    const Tuning &tn = s->tuning[masked ? 1 : 0];
     const int start_roles = warm_level < 0 ? 0 : s->roles;
-    const Program pr = build_program(P, masked, warm_level, warps, iters, start_roles);
-    const GraphKey key{masked, warm_level, warps, iters, start_roles, P.limitRange};
+    const Program pr = build_program(P, masked, warm_level, warps, iters, start_roles, tn);
+    const GraphKey key{masked, warm_level, warps, iters, start_roles, P.limitRange, tn.sig()};
     rc = engine_run(s->eng, P, pr, key, s->use_graph, &s->err);
     if (rc) return rc;
     s->roles = pr.final_roles;
@@ -636,7 +812,8 @@ int vfs_initialize(vfs_solver *s, int width, int height, float beta, float gamma
 {
     if (!s) return VFS_ERR_ARG;
     s->err.clear();
-    if (width < 1 || height < 1) return fail(s, VFS_ERR_ARG, "bad size %dx%d", width, height);
+    if (width < 1 || height < 1 || width > 32767 || height > 32767)
+        return fail(s, VFS_ERR_ARG, "bad size %dx%d (1 .. 32767 per side)", width, height);
     if (nLevels < 1 || nLevels > kMaxLevels)
         return fail(s, VFS_ERR_ARG, "nLevels %d outside [1,%d]", nLevels, kMaxLevels);
     if (nWarpIters < 0 || nSolverIters < 0) return fail(s, VFS_ERR_ARG, "negative iteration count");
@@ -657,6 +834,7 @@ int vfs_initialize(vfs_solver *s, int width, int height, float beta, float gamma
     s->nLevels = nLevels;
     s->nWarpIters = nWarpIters;
     s->nSolverIters = nSolverIters;
+    s->tuned[0] = s->tuned[1] = false;
     // pyramid dims (stereotgv.cu:74-95)
     int nh = height, nw = width;
     for (int l = 0; l < nLevels; l++) {
@@ -917,7 +1095,19 @@ int vfs_set_option(vfs_solver *s, const char *key, int value)
         if (value != s->storage) {
             s->storage = value;
             s->desc_dirty = true;
+            s->tuned[0] = s->tuned[1] = false;
         }
+        return VFS_OK;
+    }
+    if (!strcmp(key, "shape")) {
+        if (value < -1 || value >= vfs::kShapes) return fail(s, VFS_ERR_ARG, "shape must be -1 .. %d", vfs::kShapes - 1);
+        s->force_shape = value;
+        s->tuned[0] = s->tuned[1] = false;
+        return VFS_OK;
+    }
+    if (!strcmp(key, "autotune")) {
+        s->autotune = value != 0;
+        s->tuned[0] = s->tuned[1] = false;
         return VFS_OK;
     }
     return fail(s, VFS_ERR_ARG, "unknown option '%s'", key);
@@ -928,7 +1118,22 @@ int vfs_get_option(const vfs_solver *s, const char *key, int *value)
     if (!s || !key || !value) return VFS_ERR_ARG;
     if (!strcmp(key, "graph")) *value = s->use_graph ? 1 : 0;
     else if (!strcmp(key, "storage")) *value = s->storage;
+    else if (!strcmp(key, "shape")) *value = s->force_shape;
+    else if (!strcmp(key, "autotune")) *value = s->autotune ? 1 : 0;
     else return VFS_ERR_ARG;
+    return VFS_OK;
+}
+
+int vfs_get_level_tuning(vfs_solver *s, int masked, int level, int *shape, int *preload)
+{
+    if (!s || !shape || !preload) return VFS_ERR_ARG;
+    if (!s->initialized || level < 0 || level >= s->nLevels) return fail(s, VFS_ERR_ARG, "bad level");
+    int rc = set_device(s);
+    if (!rc) rc = tune_solver(s, masked);
+    if (rc) return rc;
+    const LevelTune t = s->tuning[masked ? 1 : 0].lv[level];
+    *shape = t.shape;
+    *preload = t.preload;
     return VFS_OK;
 }
 
@@ -958,6 +1163,7 @@ int vfs_time_inner_iterations(vfs_solver *s, int level, int reps, float *ms_tota
     int rc = set_device(s);
     if (rc) return rc;
     if ((rc = ensure_engine(s))) return rc;
+    if ((rc = tune_solver(s, 1))) return rc;
     if ((rc = wait_done(s))) return rc;
     const Params P = s->params();
     Program pr;
@@ -973,6 +1179,7 @@ int vfs_time_inner_iterations(vfs_solver *s, int level, int reps, float *ms_tota
             for (const ProgOp &o : w.ops)
                 if (o.kind == vfs::OP_ITER && (int)pr.ops.size() < reps) pr.ops.push_back(o);
         }
+        apply_tuning(pr, s->tuning[1]);
     }
     hipEvent_t e0, e1;
     if ((rc = time_events(s, s->stream, &e0, &e1))) return rc;
@@ -1102,9 +1309,17 @@ static int batch_solve(vfs_batch *b, int masked, int warm_level, int warps, int 
         HIPCHKB(b, hipStreamWaitEvent(b->eng.stream, s->done, 0));
         HIPCHKB(b, refresh_raw_i1(s, b->eng.stream));
     }
+    // launch choices of the largest member (every member is tuned; the shared results
+    // make this cheap)
+    const vfs_solver *big = f;
+    for (vfs_solver *s : b->members) {
+        if (tune_solver(s, masked)) return failb(b, VFS_ERR_DEVICE, "member tuning: %s", s->err.c_str());
+        if ((long long)s->lv[0].w * s->lv[0].h > (long long)big->lv[0].w * big->lv[0].h) big = s;
+    }
+    const Tuning &tn = big->tuning[masked ? 1 : 0];
     const int start_roles = warm_level < 0 ? 0 : f->roles;
-    const Program pr = build_program(P, masked, warm_level, warps, iters, start_roles);
-    const GraphKey key{masked, warm_level, warps, iters, start_roles, P.limitRange};
+    const Program pr = build_program(P, masked, warm_level, warps, iters, start_roles, tn);
+    const GraphKey key{masked, warm_level, warps, iters, start_roles, P.limitRange, tn.sig()};
     int rc = engine_run(b->eng, P, pr, key, f->use_graph, &b->err);
     if (rc) return rc;
     for (vfs_solver *s : b->members) {
@@ -1278,6 +1493,11 @@ int materialize(vfs_server *v, SrvStream &st)
         return rc;
     }
     s->has_images = true;
+    if ((rc = tune_solver(s, 1))) {
+        fails(v, rc, "stream tuning: %s", s->err.c_str());
+        vfs_destroy(s);
+        return rc;
+    }
     st.solver = s;
     st.slot = v->free_slots.back();
     v->free_slots.pop_back();
@@ -1400,8 +1620,9 @@ int admit(vfs_server *v)
         P.nWarpIters = sp.nWarpIters;
         P.nSolverIters = sp.nSolverIters;
         r->warm = sp.warm_ok ? 1 : 0;
-        r->prog = sp.warm_ok ? build_program(P, 1, sp.warm_level, sp.warm_warps, sp.warm_iters, s->roles)
-                             : build_program(P, 1, -1, 0, 0, 0);
+        const Tuning &tn = s->tuning[1];
+        r->prog = sp.warm_ok ? build_program(P, 1, sp.warm_level, sp.warm_warps, sp.warm_iters, s->roles, tn)
+                             : build_program(P, 1, -1, 0, 0, 0, tn);
         r->pc = 0;
         s->roles = r->prog.final_roles;
         sp.busy = true;
@@ -1415,18 +1636,29 @@ int launch_one_tick(vfs_server *v)
 {
     if (v->active.empty()) return VFS_OK;
     vfs::TickArgs a = tick_header(v->P);
+    // one tile shape per launch: the one chosen for the entry with the most pixels
+    int shape = 0;
+    long long most = -1;
+    for (SrvRequest *r : v->active) {
+        const ProgOp &o = r->prog.ops[r->pc];
+        const Dims dl = v->streams[r->stream].solver->lv[vfs::op_tile_level(o.kind, o.level)];
+        if ((long long)dl.w * dl.h > most) {
+            most = (long long)dl.w * dl.h;
+            shape = o.shape;
+        }
+    }
     int acc = 0, k = 0;
     for (SrvRequest *r : v->active) {
         const SrvStream &st = v->streams[r->stream];
         const ProgOp &o = r->prog.ops[r->pc];
         const vfs::PairDesc d = st.solver->desc();
-        set_entry(a.e[k], o, st.slot, acc);
-        acc += vfs::op_tiles(o.kind, o.level, d.w, d.h, d.s);
+        set_entry(a.e[k], o, st.slot, d, acc);
+        acc += vfs::op_tiles(o.kind, o.level, d.w, d.h, d.s, shape);
         k++;
     }
     a.n = k;
     a.total = acc;
-    vfs::launch_tick(v->d_desc, a, v->P.storage, v->stream);
+    vfs::launch_tick(v->d_desc, a, v->P.storage, shape, v->stream);
     HIPCHKS(v, hipGetLastError());
     v->ticks++;
     v->launched_entries += k;

@@ -1,11 +1,11 @@
-// Internal interface between the host solver (vfs_solver.hip) and the gfx950 kernels
-// (vfs_kernels.hip) of the StereoTgv solve.
+// Internal interface between the host solver (solver.hip) and the gfx950 kernels
+// (kernels.hip) of the StereoTgv solve.
 //
-// A launch covers a *set of frame pairs* at one pyramid level: every pair has a
-// descriptor (PairDesc, in device memory) holding its level geometry, its pyramid planes
-// and the base of its work arena.  Workgroups own 64x8 tiles; a per-level prefix table
-// of tile counts maps the flat workgroup index to (pair, tile), so pairs of different
-// resolutions share one launch.  A single solve is the same code with one pair.
+// A launch covers a *set of frame pairs*: every pair has a descriptor (PairDesc, in
+// device memory) holding its level geometry, its pyramid planes and the base of its work
+// arena.  Workgroups own rectangular tiles of one of kShapes shapes (chosen per launch); a
+// prefix table of tile counts maps the flat workgroup index to (pair, tile), so pairs of
+// different resolutions share one launch.  A single solve is the same code with one pair.
 #pragma once
 #include <hip/hip_bf16.h>
 #include <hip/hip_fp16.h>
@@ -15,8 +15,15 @@
 namespace vfs {
 
 constexpr int kMaxLevels = 16;
-constexpr int kTileW = 64;  // tile width = one wave64 per row segment
-constexpr int kTileH = 8;   // tile height; a 256-work-item workgroup covers it in 2 row passes
+
+// Tile shapes of the tick kernel.  A workgroup is 256 work-items laid out tw x (256 / tw);
+// each work-item covers th / (256 / tw) rows (1 or 2) of the tw x th tile.  Shape 0 is the
+// default; the others are candidates of the per-level tile-shape tuner.
+constexpr int kShapes = 7;
+struct ShapeDim {
+    int tw, th;
+};
+constexpr ShapeDim kShape[kShapes] = {{64, 8}, {64, 4}, {128, 4}, {32, 16}, {32, 8}, {16, 32}, {16, 16}};
 
 // Storage precision of the image pyramid, warped image, derivatives, tensor and duals.
 // Arithmetic is FP32 in every mode.
@@ -65,8 +72,9 @@ struct IterParams {
     float lambda;
     float alpha0;
     float alpha1;
-    int first;     // p and q were just zeroed (stereotgv.cu:431-432): read them as 0
-    int write_tp;  // also store Tp (read later only where a neighbour is outside the mask)
+    int first;       // p and q were just zeroed (stereotgv.cu:431-432): read them as 0
+    int write_tp;    // also store Tp (read later only where a neighbour is outside the mask)
+    int preload_tp;  // fetch the Tp-plane neighbours up front (latency-bound levels)
 };
 
 // Ops of a solve program.  A program is the op sequence of one solve (built on the
@@ -85,15 +93,31 @@ enum OpKind : int {
     OP_UP = 10     // upscale u, u_, warpUV from `level` to level-1
 };
 
-constexpr int kMaxEntries = 64;
+// Entries per launch: 60 entries of 64 bytes keep the kernel arguments under 4 KiB.
+constexpr int kMaxEntries = 60;
 
-struct TickEntry {
-    int pair;          // index into the descriptor array
-    int tile_start;    // first workgroup of this entry (prefix sum)
-    unsigned char kind, level, masked, flags;  // flags: bit 0 = first iteration, bit 1 = write Tp
-    int roles;         // RoleBit set before the op
-    float k1, k0, tau, mu;  // ITER scalars
+// Flags of a TickEntry.
+enum EntryFlag : int {
+    EF_FIRST = 1,       // ITER: p and q were just zeroed
+    EF_WRITE_TP = 2,    // ITER: also store Tp
+    EF_PRELOAD_TP = 4   // ITER: fetch the Tp-plane neighbours with the other operands
 };
+
+// One (pair, op) of a launch.  The geometry and arena of the op's tile level travel in
+// the kernel arguments, so the ITER op needs no descriptor load before its data loads.
+struct TickEntry {
+    float *arena;              // the pair's work arena
+    const float *mask;         // mask plane of the tile level
+    long long plane;           // floats per plane slot
+    int pair;                  // index into the descriptor array
+    int tile_start;            // first workgroup of this entry (prefix sum)
+    unsigned short w, h;       // tile level dims (OP_ZERO: level-0 stride x height)
+    int s;                     // tile level row stride
+    unsigned char kind, level, masked, flags;
+    int roles;                 // RoleBit set before the op
+    float k1, k0, tau, mu;     // ITER scalars
+};
+static_assert(sizeof(TickEntry) == 64, "TickEntry layout");
 
 struct TickArgs {
     int n;      // entries
@@ -103,15 +127,19 @@ struct TickArgs {
     TickEntry e[kMaxEntries];
 };
 
-// Workgroups (64x8 tiles) an op of `kind` at `level` needs for a pair with dims w/h/s.
-inline int op_tiles(int kind, int level, const int *w, const int *h, const int *s)
+// Level whose pixels an op's tiles cover (OP_UP writes the finer level).
+inline int op_tile_level(int kind, int level) { return kind == OP_UP ? level - 1 : level; }
+
+// Workgroups an op of `kind` at `level` needs for a pair with dims w/h/s in tile `shape`.
+inline int op_tiles(int kind, int level, const int *w, const int *h, const int *s, int shape)
 {
-    const int tl = kind == OP_UP ? level - 1 : level;
+    const int tl = op_tile_level(kind, level);
     const int ww = kind == OP_ZERO ? s[0] : w[tl];
-    return ((ww + kTileW - 1) / kTileW) * ((h[tl] + kTileH - 1) / kTileH);
+    const ShapeDim d = kShape[shape];
+    return ((ww + d.tw - 1) / d.tw) * ((h[tl] + d.th - 1) / d.th);
 }
 
-void launch_tick(const PairDesc *desc, const TickArgs &args, int storage, hipStream_t st);
+void launch_tick(const PairDesc *desc, const TickArgs &args, int storage, int shape, hipStream_t st);
 
 // ---- single-plane kernels (load time, outputs) ----
 struct Dims {
