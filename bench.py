@@ -25,6 +25,21 @@ HBM_PEAK_GBS = 8000.0
 # Algorithmic bytes per pixel and inner iteration (FP32 storage): read u, u_, v_(2),
 # p(2), q(4), a, b, c, Iu, Iz and write u, u_, v_(2), p(2), q(4) = 25 floats.
 B_ITER_FP32 = 100.0
+# HBM bytes per launch of the level-0 inner iteration measured with rocprofv3 --pmc
+# (FETCH_SIZE and WRITE_SIZE, scaled by a copy kernel of known size); the summary file
+# holds the command and the raw counters.
+TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "pmc_traffic_tgv_848x800_iteration_L0.md")
+
+
+def profiled_traffic():
+    """`traffic_bytes_per_launch = N` from the committed PMC summary, or None."""
+    import re
+    try:
+        with open(TRAFFIC_PROFILE) as f:
+            m = re.search(r"traffic_bytes_per_launch\s*=\s*([0-9]+)", f.read())
+        return int(m.group(1)) if m else None
+    except OSError:
+        return None
 
 
 def main():
@@ -129,7 +144,7 @@ def main():
                        "p99": round(float(np.percentile(lat, 99)), 4),
                        "mean": round(float(lat.mean()), 4)},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": profiled_traffic(),
                      "kernel": f"inner TGV iteration at level 0 ({w0}x{h0}), {launches} launch(es)",
                      "us_per_iter": round(t_iter * 1e6, 3), "bytes_per_iter": int(bytes_iter),
                      "copy_kernel_GBs": round(copy_gbs, 1)},
@@ -150,21 +165,61 @@ def main():
             out["reduced_storage"] = measure_reduced(args, P, S, vfs_amd, pair_checker=not args.no_cpu)
 
     if not args.no_cpu:
-        from oracle import oracle as O
-        threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
-        t0 = time.perf_counter()
-        o = O.solve_pair(pair, wl["levels"], wl["warps"], wl["iters"], P, precision="f64")
-        cpu_s = time.perf_counter() - t0
-        u_ref = o.copy1DDisparityToHost()
+        base, u_ref = cpu_baseline(wl, pair, P, S)
         m = pair["mask"] > 0
         err = np.abs(u_gpu - u_ref)[m]
         out["mean_disp_err_px"] = float(f"{err.mean():.3e}")
         out["max_disp_err_px"] = float(f"{err.max():.3e}")
-        out["cpu_baseline"] = {"value": round(1.0 / cpu_s, 4), "unit": "frame-pairs/s", "cores": threads,
-                               "kind": "port",
-                               "sample": f"one full {W}x{H} L{wl['levels']} W{wl['warps']} I{wl['iters']} masked "
-                                         f"solve of the same pair by the FP64 C oracle (OpenMP), {cpu_s:.2f} s"}
+        out["cpu_baseline"] = base
     print(json.dumps(out), flush=True)
+
+
+def _oracle_solve_times(pair, wl, P, threads, runs, warm):
+    """Median wall time of `runs` masked solves (after `warm`) by the FP64 oracle with
+    `threads` OpenMP threads; inputs resident (copied in before each timed solve, so
+    every solve is a first solve of fresh images).  Returns (seconds, oracle)."""
+    from oracle import oracle as O
+    O.set_threads(threads)
+    H, W = pair["i0"].shape
+    o = O.Oracle(W, H, P["beta"], P["gamma"], P["alpha0"], P["alpha1"], P["timestep_lambda"], P["lam"],
+                 wl["levels"], P["fScale"], wl["warps"], wl["iters"], "f64")
+    o.set_limit_range(P["limitRange"])
+    o.copyMaskToDevice(pair["mask"])
+    o.loadVectorFields(pair["tv"], pair["cv"])
+    t = []
+    for _ in range(warm + runs):
+        o.copyImagesToDevice(pair["i0"], pair["i1"])
+        t0 = time.perf_counter()
+        o.solveStereoForwardMasked()
+        t.append(time.perf_counter() - t0)
+    return float(np.median(t[warm:])), o
+
+
+def cpu_baseline(wl, pair, P, S):
+    """The FP64 oracle (OpenMP C restatement of the reference kernels) on the host:
+    the measured workload with every host thread (median of 3 after 1 warm-up) and
+    with 1 thread (one solve), plus the 424x400 workload both ways (median of 3)."""
+    from oracle import oracle as O
+    threads = O.get_threads()
+    t_all, o = _oracle_solve_times(pair, wl, P, threads, 3, 1)
+    u_ref = o.copy1DDisparityToHost()
+    t_one, _ = _oracle_solve_times(pair, wl, P, 1, 1, 0)
+    w1 = S.WORKLOADS["tgv_424x400"]
+    p1 = S.make_pair(w1["W"], w1["H"], w1["seed"], w1["radius"], w1["dmax"])
+    c1_all, _ = _oracle_solve_times(p1, w1, P, threads, 3, 1)
+    c1_one, _ = _oracle_solve_times(p1, w1, P, 1, 3, 1)
+    O.set_threads(threads)
+    H, W = pair["i0"].shape
+    base = {"value": round(1.0 / t_all, 4), "unit": "frame-pairs/s", "cores": threads, "kind": "port",
+            "sample": f"full {W}x{H} L{wl['levels']} W{wl['warps']} I{wl['iters']} masked solves of the measured pair "
+                      f"by the FP64 C oracle, inputs resident, {threads} OpenMP threads: median of 3 after 1 warm-up "
+                      f"({t_all:.2f} s per solve)",
+            "single_thread": {"value": round(1.0 / t_one, 4), "cores": 1, "sample": f"one solve, {t_one:.2f} s"},
+            "tgv_424x400": {"threads": threads, "frame_pairs_per_s": round(1.0 / c1_all, 3),
+                            "frame_pairs_per_s_1_thread": round(1.0 / c1_one, 3),
+                            "sample": "424x400 L4 W5 I20 masked solves, median of 3 after 1 warm-up"}}
+    print(f"bench: cpu baseline {base}", file=sys.stderr, flush=True)
+    return base, u_ref
 
 
 def _solver(vfs_amd, P, pair, levels, warps, iters, storage="f32", graph=True):
@@ -205,6 +260,8 @@ def measure_batched(args, P, S, vfs_amd):
     print("bench: batched 64 mixed pairs ready", file=sys.stderr, flush=True)
     b = vfs_amd.Batch(solvers)
     res = _time_batch(args, b, 64)
+    res["tile_tuning"] = {f"L{l}": "%dx%d" % b.level_tuning(l)[0] + (" +Tp prefetch" if b.level_tuning(l)[1] else "")
+                          for l in range(5)}
     mpx = sum(p["i0"].size for p in pairs) / 1e6
     res["Mpx_per_s"] = round(res["pairs_per_s"] * mpx / 64, 2)
     res["workload"] = "64 pairs, sizes 424x400 / 640x480 / 848x800 in turn, L5 W10 I50, FP32, one batch"
@@ -228,6 +285,7 @@ def measure_reduced(args, P, S, vfs_amd, pair_checker=True):
         solvers = [_solver(vfs_amd, P, p, 5, 10, 50, storage=st) for p in pairs]
         b = vfs_amd.Batch(solvers)
         res = _time_batch(args, b, 16)
+        res["tile_tuning_L0"] = "%dx%d" % b.level_tuning(0)[0]
         # bytes per pixel-iteration: FP32 100; 16-bit storage of p, q, a, b, c, Iu, Iz: 66
         res["bytes_per_px_iter"] = 100 if st == "f32" else 66
         if ref is not None:
@@ -334,7 +392,7 @@ def measure_served(args, P, S, vfs_amd, plain_pairs_per_s):
                        f"continuous batching"}
     res = {}
     for label, rate in (("saturation", 0.0), ("rate_50", 0.5), ("rate_90", 0.9)):
-        srv = vfs_amd.Server(P, hbm_budget_bytes=64 << 30, max_streams=32, max_active=60, storage="f16")
+        srv = vfs_amd.Server(P, hbm_budget_bytes=64 << 30, max_streams=32, max_active=64, storage="f16")
         sids = [srv.add_stream(b["mask"], b["tv"], b["cv"], 5, 10, 50, 1, 2, 20) for (W, H, b, fr) in streams]
         cold = prime_streams(srv, sids, streams)
         r = rate if rate <= 0 else rate * res["saturation"]["frames_per_s"]

@@ -178,6 +178,9 @@ int vfs_batch_solve(vfs_batch *b, int masked);
 /* vfs_solve_warm for every member (all members solved last by the same program). */
 int vfs_batch_solve_warm(vfs_batch *b, int masked, int start_level, int warps, int iters);
 int vfs_batch_synchronize(vfs_batch *b);
+/* Launch choices of the batch's launches at `level` (as vfs_get_level_tuning); the batch
+ * times them on scratch copies of its largest members at its first solve. */
+int vfs_batch_get_level_tuning(vfs_batch *b, int masked, int level, int *shape, int *preload);
 /* `n` batch solves back to back, HIP events around them; total ms. */
 int vfs_batch_time_solves(vfs_batch *b, int masked, int n, float *ms_total);
 const char *vfs_batch_last_error(const vfs_batch *b);
@@ -197,7 +200,7 @@ typedef struct vfs_server vfs_server;
 typedef struct {
     size_t hbm_budget_bytes;  /* device memory the resident streams may use */
     int max_streams;          /* resident streams at most */
-    int max_active;           /* frames in flight at most (1..60) */
+    int max_active;           /* frames in flight at most (1..64) */
     int storage;              /* 0 FP32, 1 FP16, 2 BF16 (vfs_set_option "storage") */
     float beta, gamma, alpha0, alpha1, timestep_lambda, lambda, fScale, limitRange;
 } vfs_server_params;

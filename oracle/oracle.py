@@ -47,6 +47,16 @@ def lib() -> ctypes.CDLL:
     return _LIB
 
 
+def set_threads(n: int) -> None:
+    """OpenMP threads of the oracle's loops."""
+    lib().oracle_set_threads(int(n))
+
+
+def get_threads() -> int:
+    lib().oracle_get_threads.restype = ctypes.c_int
+    return int(lib().oracle_get_threads())
+
+
 def _p(a: np.ndarray, ct=ctypes.c_double):
     return a.ctypes.data_as(ctypes.POINTER(ct))
 

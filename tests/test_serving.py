@@ -95,7 +95,7 @@ def test_warm_solve_matches_oracle():
         assert err.mean() <= 5e-4, (k, err.mean())
 
 
-def make_server(budget=16 << 30, max_active=60):
+def make_server(budget=16 << 30, max_active=64):
     from vfs_amd import Server
     return Server(P, hbm_budget_bytes=budget, max_streams=16, max_active=max_active)
 

@@ -58,6 +58,7 @@ EXPORTS = [
     ("vfs_batch_solve", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
     ("vfs_batch_solve_warm", ctypes.c_int, [ctypes.c_void_p] + [ctypes.c_int] * 4),
     ("vfs_batch_synchronize", ctypes.c_int, [ctypes.c_void_p]),
+    ("vfs_batch_get_level_tuning", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, _c_int_p, _c_int_p]),
     ("vfs_batch_time_solves", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, _c_float_p]),
     ("vfs_batch_last_error", ctypes.c_char_p, [ctypes.c_void_p]),
     ("vfs_server_create", ctypes.c_int, [ctypes.POINTER(ctypes.c_void_p), ctypes.c_void_p]),
@@ -333,6 +334,13 @@ class Batch:
     def synchronize(self):
         return self._check(self._L.vfs_batch_synchronize(self._h), "batch_synchronize")
 
+    def level_tuning(self, level: int, masked: bool = True):
+        """(tile shape (w, h), Tp prefetch) of the batch's launches at `level` (after a solve)."""
+        sh, pre = ctypes.c_int(), ctypes.c_int()
+        self._check(self._L.vfs_batch_get_level_tuning(self._h, int(masked), level, ctypes.byref(sh),
+                                                        ctypes.byref(pre)), "batch_get_level_tuning")
+        return StereoTgv.SHAPES[sh.value], bool(pre.value)
+
     def time_solves(self, masked: bool = True, n: int = 1) -> float:
         ms = ctypes.c_float()
         self._check(self._L.vfs_batch_time_solves(self._h, 1 if masked else 0, n, ctypes.byref(ms)), "batch_time")
@@ -353,7 +361,7 @@ class Batch:
 class Server:
     """Continuous-batching server for camera streams (vfs_server_* of include/vfs.h)."""
 
-    def __init__(self, params: dict, hbm_budget_bytes: int = 64 << 30, max_streams: int = 64, max_active: int = 60,
+    def __init__(self, params: dict, hbm_budget_bytes: int = 64 << 30, max_streams: int = 64, max_active: int = 64,
                  storage="f32"):
         self._L = lib()
         sp = _ServerParams()

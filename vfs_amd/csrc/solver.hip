@@ -179,6 +179,8 @@ struct vfs_batch {
     Engine eng;
     std::vector<vfs_solver *> members;
     std::string err;
+    Tuning tuning[2];  // [masked] launch choices of the batch's launches
+    bool tuned[2] = {false, false};
 };
 
 namespace {
@@ -386,17 +388,17 @@ void set_entry(vfs::TickEntry &e, const ProgOp &o, int pair, const vfs::PairDesc
     const int tl = vfs::op_tile_level(o.kind, o.level);
     e.arena = d.arena;
     e.mask = d.mask[tl];
-    e.plane = d.plane;
+    e.plane = (int)d.plane;  // < 2^31 (checked at initialize)
     e.w = (unsigned short)(o.kind == vfs::OP_ZERO ? d.s[0] : d.w[tl]);
     e.h = (unsigned short)d.h[tl];
-    e.s = d.s[tl];
-    e.pair = pair;
+    e.s = (unsigned short)d.s[tl];
+    e.pair = (unsigned short)pair;
     e.tile_start = tile_start;
     e.kind = o.kind;
     e.level = o.level;
     e.masked = o.masked;
     e.flags = o.flags;
-    e.roles = o.roles;
+    e.roles = (unsigned char)o.roles;
     e.k1 = o.k1;
     e.k0 = o.k0;
     e.tau = o.tau;
@@ -560,20 +562,18 @@ LevelTune default_tune(int level)
     return t;
 }
 
-int measure_levels(vfs_solver *s, int masked, const std::vector<int> &levels, Tuning &tn)
+// Times the inner iteration of every level in `levels` in each (tile shape, Tp preload)
+// on the launch set `descs` (their arenas point at scratch memory; `d_desc` is any valid
+// descriptor array of at least descs.size() entries) and stores the fastest choices in tn.
+hipError_t time_launch_choices(hipStream_t st, const vfs::PairDesc *d_desc, const std::vector<vfs::PairDesc> &descs,
+                               const Params &P, int masked, const std::vector<int> &levels, Tuning &tn)
 {
     constexpr int kWarm = 3, kReps = 20, kRounds = 2;
-    const size_t bytes = sizeof(float) * (size_t)s->plane * vfs::PL_COUNT;
-    float *scratch = nullptr;
-    HIPCHK(s, hipMalloc(&scratch, bytes));
     hipEvent_t e0 = nullptr, e1 = nullptr;
     hipError_t r = hipEventCreate(&e0);
     if (r == hipSuccess) r = hipEventCreate(&e1);
-    vfs::launch_fill_zero(scratch, bytes, s->stream);
-    vfs::PairDesc d = s->desc();
-    d.arena = scratch;
-    const Params P = s->params();
     std::vector<float> best(levels.size() * vfs::kShapes * 2, 1e30f);
+    const int n = std::min((int)descs.size(), vfs::kMaxEntries);
     for (int round = 0; round < kRounds && r == hipSuccess; round++) {
         for (size_t li = 0; li < levels.size() && r == hipSuccess; li++) {
             for (int shape = 0; shape < vfs::kShapes && r == hipSuccess; shape++) {
@@ -589,13 +589,18 @@ int measure_levels(vfs_solver *s, int masked, const std::vector<int> &levels, Tu
                     o.tau = 1.0f;
                     o.mu = 0.8f;
                     vfs::TickArgs a = tick_header(P);
-                    set_entry(a.e[0], o, 0, d, 0);
-                    a.n = 1;
-                    a.total = vfs::op_tiles(o.kind, o.level, d.w, d.h, d.s, shape);
-                    for (int k = 0; k < kWarm; k++) vfs::launch_tick(s->eng.d_desc, a, P.storage, shape, s->stream);
-                    r = hipEventRecord(e0, s->stream);
-                    for (int k = 0; k < kReps; k++) vfs::launch_tick(s->eng.d_desc, a, P.storage, shape, s->stream);
-                    if (r == hipSuccess) r = hipEventRecord(e1, s->stream);
+                    int acc = 0;
+                    for (int k = 0; k < n; k++) {
+                        const vfs::PairDesc &d = descs[k];
+                        set_entry(a.e[k], o, k, d, acc);
+                        acc += vfs::op_tiles(o.kind, o.level, d.w, d.h, d.s, shape);
+                    }
+                    a.n = n;
+                    a.total = acc;
+                    for (int k = 0; k < kWarm; k++) vfs::launch_tick(d_desc, a, P.storage, shape, st);
+                    r = hipEventRecord(e0, st);
+                    for (int k = 0; k < kReps; k++) vfs::launch_tick(d_desc, a, P.storage, shape, st);
+                    if (r == hipSuccess) r = hipEventRecord(e1, st);
                     if (r == hipSuccess) r = hipEventSynchronize(e1);
                     float ms = 0;
                     if (r == hipSuccess) r = hipEventElapsedTime(&ms, e0, e1);
@@ -606,11 +611,10 @@ int measure_levels(vfs_solver *s, int masked, const std::vector<int> &levels, Tu
             }
         }
     }
-    if (r == hipSuccess) r = hipStreamSynchronize(s->stream);
+    if (r == hipSuccess) r = hipStreamSynchronize(st);
     if (e0) (void)hipEventDestroy(e0);
     if (e1) (void)hipEventDestroy(e1);
-    (void)hipFree(scratch);
-    if (r != hipSuccess) return fail(s, VFS_ERR_DEVICE, "launch tuner: %s", hipGetErrorString(r));
+    if (r != hipSuccess) return r;
     for (size_t li = 0; li < levels.size(); li++) {
         const int l = levels[li];
         const LevelTune def = default_tune(l);
@@ -627,6 +631,21 @@ int measure_levels(vfs_solver *s, int masked, const std::vector<int> &levels, Tu
             }
         tn.lv[l] = pick;
     }
+    return hipSuccess;
+}
+
+int measure_levels(vfs_solver *s, int masked, const std::vector<int> &levels, Tuning &tn)
+{
+    const size_t bytes = sizeof(float) * (size_t)s->plane * vfs::PL_COUNT;
+    float *scratch = nullptr;
+    HIPCHK(s, hipMalloc(&scratch, bytes));
+    vfs::launch_fill_zero(scratch, bytes, s->stream);
+    std::vector<vfs::PairDesc> d{s->desc()};
+    d[0].arena = scratch;
+    const hipError_t r = time_launch_choices(s->stream, s->eng.d_desc, d, s->params(), masked, levels, tn);
+    (void)hipStreamSynchronize(s->stream);
+    (void)hipFree(scratch);
+    if (r != hipSuccess) return fail(s, VFS_ERR_DEVICE, "launch tuner: %s", hipGetErrorString(r));
     return VFS_OK;
 }
 
@@ -844,6 +863,9 @@ int vfs_initialize(vfs_solver *s, int width, int height, float beta, float gamma
         nw = (int)((float)nw / fScale);
     }
     const size_t N = (size_t)s->stride * height;
+    // launch entries carry strides as 16-bit and plane sizes as 31-bit integers
+    if (s->stride > 65535 || N + 63 > (size_t)0x7fffffff)
+        return fail(s, VFS_ERR_ARG, "row stride %d x height %d too large", s->stride, height);
     for (int l = 0; l < nLevels; l++) {
         const size_t n = (size_t)s->lv[l].s * s->lv[l].h;
         if ((rc = dev_alloc(s, &s->I0p[l], n * sizeof(float)))) return rc;
@@ -1275,6 +1297,48 @@ int vfs_batch_destroy(vfs_batch *b)
 
 const char *vfs_batch_last_error(const vfs_batch *b) { return b ? b->err.c_str() : g_create_error.c_str(); }
 
+// Launch choices of a batch, timed on scratch copies of its (up to) 8 largest members: a
+// batched launch covers many pairs, so its fastest tile shape can differ from a single
+// pair's.  The first member's "shape" / "autotune" options apply.
+static int tune_batch(vfs_batch *b, int masked)
+{
+    const int m = masked ? 1 : 0;
+    if (b->tuned[m]) return VFS_OK;
+    const vfs_solver *f = b->members[0];
+    Tuning &tn = b->tuning[m];
+    for (int l = 0; l < kMaxLevels; l++) tn.lv[l] = default_tune(l);
+    if (f->force_shape >= 0) {
+        for (int l = 0; l < f->nLevels; l++) tn.lv[l].shape = (unsigned char)f->force_shape;
+    } else if (f->autotune) {
+        std::vector<const vfs_solver *> order(b->members.begin(), b->members.end());
+        std::stable_sort(order.begin(), order.end(), [](const vfs_solver *x, const vfs_solver *y) {
+            return (long long)x->lv[0].w * x->lv[0].h > (long long)y->lv[0].w * y->lv[0].h;
+        });
+        const int n = std::min((int)order.size(), 8);
+        std::vector<vfs::PairDesc> descs;
+        std::vector<float *> scratch;
+        hipError_t r = hipSuccess;
+        for (int i = 0; i < n && r == hipSuccess; i++) {
+            const size_t bytes = sizeof(float) * (size_t)order[i]->plane * vfs::PL_COUNT;
+            float *p = nullptr;
+            r = hipMalloc(&p, bytes);
+            if (r != hipSuccess) break;
+            scratch.push_back(p);
+            vfs::launch_fill_zero(p, bytes, b->eng.stream);
+            descs.push_back(order[i]->desc());
+            descs.back().arena = p;
+        }
+        std::vector<int> levels;
+        for (int l = 0; l < f->nLevels; l++) levels.push_back(l);
+        if (r == hipSuccess) r = time_launch_choices(b->eng.stream, b->eng.d_desc, descs, f->params(), m, levels, tn);
+        (void)hipStreamSynchronize(b->eng.stream);
+        for (float *p : scratch) (void)hipFree(p);
+        if (r != hipSuccess) return failb(b, VFS_ERR_DEVICE, "batch launch tuner: %s", hipGetErrorString(r));
+    }
+    b->tuned[m] = true;
+    return VFS_OK;
+}
+
 static int batch_solve(vfs_batch *b, int masked, int warm_level, int warps, int iters)
 {
     if (!b) return VFS_ERR_ARG;
@@ -1291,7 +1355,8 @@ static int batch_solve(vfs_batch *b, int masked, int warm_level, int warps, int 
         if (warm_level >= 0 && (!s->solved || s->roles != f->roles))
             return failb(b, VFS_ERR_STATE, "member %zu: warm start needs every member solved with the same program",
                          i);
-        if (s->desc_dirty) rebuild = true;
+        // the batch keeps its own copy of every member's descriptor: re-upload (and
+        // re-tune) only when one of them changed
         if (!rebuild) {
             const vfs::PairDesc d = s->desc();
             if (memcmp(&b->eng.h_desc[i], &d, sizeof d) != 0) rebuild = true;
@@ -1303,20 +1368,16 @@ static int batch_solve(vfs_batch *b, int masked, int warm_level, int warps, int 
         for (vfs_solver *s : b->members) descs.push_back(s->desc());
         hipError_t r = engine_setup(b->eng, descs);
         if (r != hipSuccess) return failb(b, VFS_ERR_DEVICE, "engine setup: %s", hipGetErrorString(r));
+        b->tuned[0] = b->tuned[1] = false;
     }
     // order after each member's pending work, then refresh re-used raw inputs
     for (vfs_solver *s : b->members) {
         HIPCHKB(b, hipStreamWaitEvent(b->eng.stream, s->done, 0));
         HIPCHKB(b, refresh_raw_i1(s, b->eng.stream));
     }
-    // launch choices of the largest member (every member is tuned; the shared results
-    // make this cheap)
-    const vfs_solver *big = f;
-    for (vfs_solver *s : b->members) {
-        if (tune_solver(s, masked)) return failb(b, VFS_ERR_DEVICE, "member tuning: %s", s->err.c_str());
-        if ((long long)s->lv[0].w * s->lv[0].h > (long long)big->lv[0].w * big->lv[0].h) big = s;
-    }
-    const Tuning &tn = big->tuning[masked ? 1 : 0];
+    int rc0 = tune_batch(b, masked);
+    if (rc0) return rc0;
+    const Tuning &tn = b->tuning[masked ? 1 : 0];
     const int start_roles = warm_level < 0 ? 0 : f->roles;
     const Program pr = build_program(P, masked, warm_level, warps, iters, start_roles, tn);
     const GraphKey key{masked, warm_level, warps, iters, start_roles, P.limitRange, tn.sig()};
@@ -1337,6 +1398,17 @@ int vfs_batch_solve_warm(vfs_batch *b, int masked, int start_level, int warps, i
 {
     if (start_level < 0 || warps < 0 || iters < 0) return failb(b, VFS_ERR_ARG, "bad warm schedule");
     return batch_solve(b, masked, start_level, warps, iters);
+}
+
+int vfs_batch_get_level_tuning(vfs_batch *b, int masked, int level, int *shape, int *preload)
+{
+    if (!b || !shape || !preload) return VFS_ERR_ARG;
+    if (level < 0 || level >= b->members[0]->nLevels) return failb(b, VFS_ERR_ARG, "bad level");
+    if (!b->tuned[masked ? 1 : 0]) return failb(b, VFS_ERR_STATE, "the batch has not been solved in this form yet");
+    const LevelTune t = b->tuning[masked ? 1 : 0].lv[level];
+    *shape = t.shape;
+    *preload = t.preload;
+    return VFS_OK;
 }
 
 int vfs_batch_synchronize(vfs_batch *b)

@@ -93,8 +93,9 @@ enum OpKind : int {
     OP_UP = 10     // upscale u, u_, warpUV from `level` to level-1
 };
 
-// Entries per launch: 60 entries of 64 bytes keep the kernel arguments under 4 KiB.
-constexpr int kMaxEntries = 60;
+// Entries per launch: one per lane of a wave64 (the kernel's entry lookup), and 64
+// entries of 56 bytes keep the kernel arguments under 4 KiB.
+constexpr int kMaxEntries = 64;
 
 // Flags of a TickEntry.
 enum EntryFlag : int {
@@ -108,16 +109,15 @@ enum EntryFlag : int {
 struct TickEntry {
     float *arena;              // the pair's work arena
     const float *mask;         // mask plane of the tile level
-    long long plane;           // floats per plane slot
-    int pair;                  // index into the descriptor array
+    int plane;                 // floats per plane slot
     int tile_start;            // first workgroup of this entry (prefix sum)
-    unsigned short w, h;       // tile level dims (OP_ZERO: level-0 stride x height)
-    int s;                     // tile level row stride
+    unsigned short pair;       // index into the descriptor array
+    unsigned short w, h, s;    // tile level dims and row stride (OP_ZERO: level-0 stride x height)
     unsigned char kind, level, masked, flags;
-    int roles;                 // RoleBit set before the op
+    unsigned char roles, pad0, pad1, pad2;  // roles: RoleBit set before the op
     float k1, k0, tau, mu;     // ITER scalars
 };
-static_assert(sizeof(TickEntry) == 64, "TickEntry layout");
+static_assert(sizeof(TickEntry) == 56, "TickEntry layout");
 
 struct TickArgs {
     int n;      // entries

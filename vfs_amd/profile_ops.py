@@ -1,0 +1,75 @@
+"""Per-op breakdown of a solve from a rocprofv3 kernel trace.
+
+Every op of a solve program is one `k_tick` dispatch when the solve is launched without
+a graph (option "graph" = 0), in program order.  This module rebuilds the cold masked
+program's op list (the same sequence the library issues: zero, calibration warp,
+pyramid, then per level the tensor and, per warp, prep / derivatives / inner iterations
+/ end, then the upscale) and attributes each dispatch of one solve to its op and level.
+
+    python -m vfs_amd.profile_ops TRACE.csv --levels 5 --warps 10 --iters 50 --skip-tail 100
+
+`--skip-tail` drops trailing dispatches that follow the solves (e.g. timing launches);
+the solve analysed is the last complete one before them.
+"""
+from __future__ import annotations
+
+import argparse
+import collections
+import csv
+
+
+def program_ops(levels: int, warps: int, iters: int) -> list[tuple[str, int]]:
+    ops = [("ZERO", 0), ("CALIB", 0)] + [("DOWN", l) for l in range(1, levels)] + [("CINIT", levels - 1)]
+    for l in range(levels - 1, -1, -1):
+        ops.append(("TENSOR", l))
+        for _ in range(warps):
+            ops += [("PREP", l), ("DERIV", l)] + [("ITER", l)] * iters + [("END", l)]
+        if l > 0:
+            ops.append(("UP", l))
+    return ops
+
+
+def summarize(trace_csv: str, levels: int, warps: int, iters: int, skip_tail: int = 0) -> str:
+    rows = [r for r in csv.DictReader(open(trace_csv)) if r["Kernel_Name"].startswith("k_tick")]
+    rows.sort(key=lambda r: int(r["Start_Timestamp"]))
+    seq = program_ops(levels, warps, iters)
+    n = len(seq)
+    end = len(rows) - skip_tail
+    if end < n:
+        raise ValueError(f"trace holds {end} tick dispatches, one solve needs {n}")
+    solve = rows[end - n:end]
+    agg = collections.OrderedDict()
+    for (kind, lvl), r in zip(seq, solve):
+        d = (int(r["End_Timestamp"]) - int(r["Start_Timestamp"])) * 1e-3
+        wgs = (int(r["Grid_Size_X"]) // max(int(r["Workgroup_Size_X"]), 1)) * \
+              (int(r["Grid_Size_Y"]) // max(int(r["Workgroup_Size_Y"]), 1))
+        a = agg.setdefault((kind, lvl), [0, 0.0, wgs])
+        a[0] += 1
+        a[1] += d
+    total = sum(v[1] for v in agg.values())
+    span = (int(solve[-1]["End_Timestamp"]) - int(solve[0]["Start_Timestamp"])) * 1e-3
+    out = [f"one solve: {n} dispatches, kernel time {total:.0f} us, first start to last end {span:.0f} us", "",
+           "| op | level | dispatches | workgroups | total us | avg us | share |",
+           "|---|---|---|---|---|---|---|"]
+    for (kind, lvl), (c, s, g) in sorted(agg.items(), key=lambda kv: -kv[1][1]):
+        out.append(f"| {kind} | {lvl} | {c} | {g} | {s:.1f} | {s / c:.2f} | {100 * s / total:.1f}% |")
+    by_level = collections.defaultdict(float)
+    for (kind, lvl), (c, s, g) in agg.items():
+        by_level[lvl] += s
+    out += ["", "per level: " + ", ".join(f"L{l} {v:.0f} us" for l, v in sorted(by_level.items()))]
+    return "\n".join(out)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("trace")
+    ap.add_argument("--levels", type=int, default=5)
+    ap.add_argument("--warps", type=int, default=10)
+    ap.add_argument("--iters", type=int, default=50)
+    ap.add_argument("--skip-tail", type=int, default=0)
+    a = ap.parse_args()
+    print(summarize(a.trace, a.levels, a.warps, a.iters, a.skip_tail))
+
+
+if __name__ == "__main__":
+    main()
