@@ -55,6 +55,8 @@ def main():
     ap.add_argument("--batch-steps", type=int, default=2)
     ap.add_argument("--serve-frames", type=int, default=25, help="frames per stream in the serving leg")
     ap.add_argument("--serve-only", action="store_true", help="run only the serving leg (development aid)")
+    ap.add_argument("--shape", type=int, default=-1,
+                    help="force one tile shape (0..6) instead of the per-level tuner (profiling aid)")
     ap.add_argument("--iter-only", type=int, default=0,
                     help="profiling aid: run N level-0 inner iterations (after a 256 MiB reference copy) and exit")
     args = ap.parse_args()
@@ -80,6 +82,8 @@ def main():
     g.limitRange = P["limitRange"]
     if args.no_graph:
         g.set_option("graph", 0)
+    if args.shape >= 0:
+        g.set_option("shape", args.shape)
     g.copyMaskToDevice(pair["mask"])
     g.loadVectorFields(pair["tv"], pair["cv"])
     g.copyImagesToDevice(pair["i0"], pair["i1"])
