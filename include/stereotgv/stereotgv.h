@@ -111,6 +111,11 @@ public:
     int copy1DDisparityToHost(float *u) { return vfs_copy_disparity_1d(h_, u, width); }
     // stereotgv.h:122 -- width*height*2 floats
     int copyDisparityToHost(float *flow) { return vfs_copy_flow(h_, flow, 2 * width); }
+    // stereotgv.h:123 -- width*height*3 floats, (B, G, R) in [0, 1]
+    int copyDisparityVisToHost(float *bgr, float flowScale)
+    {
+        return vfs_copy_disparity_vis(h_, bgr, 3 * width, flowScale);
+    }
     // stereotgv.h:124
     int copyWarpedImageToHost(float *out) { return vfs_copy_warped_image(h_, out, width); }
 
@@ -174,6 +179,13 @@ public:
         wCropped = flow_;
         return rc;
     }
+    int copyDisparityVisToHost(cv::Mat &wCropped, float flowScale)
+    {
+        vis_.create(height, width, CV_32FC3);
+        int rc = copyDisparityVisToHost(vis_.ptr<float>(), flowScale);
+        wCropped = vis_;
+        return rc;
+    }
     int copyWarpedImageToHost(cv::Mat &wCropped)
     {
         cv::Mat out(height, width, CV_32F);
@@ -183,7 +195,7 @@ public:
     }
 
 private:
-    cv::Mat depth_, X_, disp_, flow_;
+    cv::Mat depth_, X_, disp_, flow_, vis_;
 #endif
 
 private:

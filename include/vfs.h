@@ -106,6 +106,12 @@ int vfs_copy_stereo(vfs_solver *s, float *depth, int depth_pitch, float *X, int 
                     float focaly, float cx, float cy, float d1, float d2, float d3, float d4, float t1,
                     float t2, float t3);
 
+/* StereoTgv::copyDisparityVisToHost(cv::Mat&, float flowScale) (stereotgv.h:123,
+ * stereotgv.cu:551-559): FlowToHSV colour coding of the flow warpUV / flowScale
+ * (flowToHsv.cu:3-121), 3 floats per pixel in (B, G, R) order, values in [0, 1].
+ * Pitch in floats (>= 3 * width). */
+int vfs_copy_disparity_vis(vfs_solver *s, float *out, int pitch, float flowScale);
+
 /* StereoTgv::copyWarpedImageToHost (stereotgv.h:124, stereotgv.cu:561-568): the last
  * warped I1 (valid at level 0 after a solve). */
 int vfs_copy_warped_image(vfs_solver *s, float *out, int pitch);

@@ -338,6 +338,22 @@ def k_primal_masked(mask, u_, v_, p, q, a, b, c, tau, etav1, etav2, alpha0, alph
     return v, u_s, v_s
 
 
+def k_flow_hsv(uv, scale):
+    """FlowToHSV colour coding of a (h, w, 2) flow: (h, w, 3) in (B, G, R) order."""
+    uv = _d(uv)
+    h, w = uv.shape[:2]
+    out = np.zeros((h, w, 3))
+    _k("oracle_k_flow_hsv")(_p(uv), ctypes.c_double(scale), _p(out), w, h, w)
+    return out
+
+
+def wheel_entry(k):
+    """Entry k (0..54) of the oracle's colour wheel as (R, G, B) in 0..255."""
+    out = np.zeros(3)
+    _k("oracle_wheel")(ctypes.c_int(k), _p(out))
+    return out
+
+
 def k_convert_kb(uv, focalx, focaly, cx, cy, d1, d2, d3, d4, t1, t2, t3):
     uv = _d(uv)
     h, w = uv.shape[:2]

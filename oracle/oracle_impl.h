@@ -628,6 +628,71 @@ void FN(oracle_k_convert_kb)(const R *uv, R focalx, R focaly, R cx, R cy,
         }
 }
 
+/* flowToHsv.cu:3-111 (ComputeColorKernel): flow colour coding.  (du, dv) = uv / scale;
+ * angle a = atan2(-dv, -du) / 3.14159 picks a position fk = (a + 1) / 2 * 54 + 1 on the
+ * 55-entry Middlebury colour wheel (segments red-yellow 15, yellow-green 6, green-cyan
+ * 4, cyan-blue 11, blue-magenta 13, magenta-red 6; entry i of a rising segment of length
+ * n is floor(255 i / n)), interpolated between floor(fk) and floor(fk) + 1 (wrapping
+ * 56 -> 1); the reference indexes its 55-entry table with these 1-based positions, so
+ * position 55 (angle within 2.7e-6 rad of +pi) reads past its table and is taken as
+ * entry 0 here.  Magnitude <= 1 blends towards white (1 - r (1 - c)), larger ones are
+ * darkened (0.75 c).  Output per pixel: (B, G, R) in [0, 1], the CV_32FC3 layout of the
+ * reference's host Mat. */
+static void FN(wheel_rgb)(int k, R *r, R *g, R *b)
+{
+    static const int seg[6] = {15, 6, 4, 11, 13, 6};
+    k = ((k % 55) + 55) % 55;
+    int start = 0, si = 0;
+    while (k >= start + seg[si]) start += seg[si++];
+    const int i = k - start, n = seg[si];
+    const int up = (255 * i) / n, down = 255 - (255 * i) / n;
+    int rr = 0, gg = 0, bb = 0;
+    switch (si) {
+    case 0: rr = 255; gg = up; break;            /* red -> yellow */
+    case 1: rr = down; gg = 255; break;          /* yellow -> green */
+    case 2: gg = 255; bb = up; break;            /* green -> cyan */
+    case 3: gg = down; bb = 255; break;          /* cyan -> blue */
+    case 4: rr = up; bb = 255; break;            /* blue -> magenta */
+    default: rr = 255; bb = down; break;         /* magenta -> red */
+    }
+    *r = (R)rr; *g = (R)gg; *b = (R)bb;
+}
+
+void FN(oracle_wheel)(int k, R *rgb) { FN(wheel_rgb)(k, &rgb[0], &rgb[1], &rgb[2]); }
+
+void FN(oracle_k_flow_hsv)(const R *uv, R scale, R *bgr, int w, int h, int s)
+{
+#pragma omp parallel for schedule(static)
+    for (int y = 0; y < h; y++)
+        for (int x = 0; x < w; x++) {
+            size_t pos = (size_t)y * s + x;
+            R du = uv[2 * pos] / scale, dv = uv[2 * pos + 1] / scale;
+            R rad = sqrt(du * du + dv * dv);
+            R a = atan2(-dv, -du) / (R)3.14159;
+            R fk = (a + 1) / 2 * (R)54 + 1;
+            int k0 = (int)floor(fk);
+            int k1 = k0 + 1;
+            if (k1 == 56) k1 = 1;
+            R f = fk - (R)k0;
+            R r0, g0, b0, r1, g1, b1;
+            FN(wheel_rgb)(k0, &r0, &g0, &b0);
+            FN(wheel_rgb)(k1, &r1, &g1, &b1);
+            R cr = (1 - f) * (r0 / 255) + f * (r1 / 255);
+            R cg = (1 - f) * (g0 / 255) + f * (g1 / 255);
+            R cb = (1 - f) * (b0 / 255) + f * (b1 / 255);
+            if (rad <= 1) {
+                cr = 1 - rad * (1 - cr);
+                cg = 1 - rad * (1 - cg);
+                cb = 1 - rad * (1 - cb);
+            } else {
+                cr = cr * (R)0.75;
+                cg = cg * (R)0.75;
+                cb = cb * (R)0.75;
+            }
+            bgr[3 * pos] = cb; bgr[3 * pos + 1] = cg; bgr[3 * pos + 2] = cr;
+        }
+}
+
 /* computeDerivatives.cu:88-129: 5-point derivative along the normalised level
  * trajectory t = tv/|tv| (|tv| = 0 gives NaN, as the reference), averaged over
  * I0 and the warped I1; Iz = I1w - I0 at the texel centre. */

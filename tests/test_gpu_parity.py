@@ -236,6 +236,42 @@ def _workload_parity(name, tol_mean):
     return g, o, pair
 
 
+@pytest.mark.parametrize("masked", [True, False])
+def test_parity_non_power_of_two_pyramid(masked):
+    """fScale 1.2 (test_vfs/main.cpp:136-137 uses 11 levels at 1.2): non-exact down- and
+    upscaling at every level, and the upscale multiplies by fScale rather than the true
+    size ratio (upscale.cu:7-21, stereotgv.cu:497-510)."""
+    pair = small_pair(212, 200, seed=61)
+    L, fs = 6, 1.2
+    H, W = pair["i0"].shape
+    g = StereoTgv()
+    g.initialize(W, H, P["beta"], P["gamma"], P["alpha0"], P["alpha1"], P["timestep_lambda"], P["lam"], L, fs, 3, 10)
+    g.limitRange = P["limitRange"]
+    g.copyMaskToDevice(pair["mask"])
+    g.loadVectorFields(pair["tv"], pair["cv"])
+    g.copyImagesToDevice(pair["i0"], pair["i1"])
+    o = O.Oracle(W, H, P["beta"], P["gamma"], P["alpha0"], P["alpha1"], P["timestep_lambda"], P["lam"], L, fs, 3, 10)
+    o.set_limit_range(P["limitRange"])
+    o.copyMaskToDevice(pair["mask"])
+    o.loadVectorFields(pair["tv"], pair["cv"])
+    o.copyImagesToDevice(pair["i0"], pair["i1"])
+    if masked:
+        g.solveStereoForwardMasked()
+        o.solveStereoForwardMasked()
+    else:
+        g.solveStereoForward()
+        o.solveStereoForward()
+    dims = [g.level_dims(l) for l in range(L)]
+    assert dims == [o.level_dims(l) for l in range(L)]
+    assert dims[1][:2] == (176, 166) and dims[5][:2] == (83, 79)  # (int)((float)n / 1.2) per level
+    for lvl in range(L):
+        assert_close(f"I0@{lvl}", g.read_plane("I0", lvl), o.read_plane("I0", lvl), 1e-5, 1e-6)
+        assert np.array_equal(g.read_plane("mask", lvl), o.read_plane("mask", lvl)), f"mask@{lvl}"
+    m = pair["mask"] > 0 if masked else np.ones(pair["mask"].shape, bool)
+    err = np.abs(g.copy1DDisparityToHost() - o.copy1DDisparityToHost())[m]
+    assert err.mean() <= 5e-4, err.mean()
+
+
 def test_parity_424x400():
     _workload_parity("tgv_424x400", 0.005)
 

@@ -43,6 +43,7 @@ EXPORTS = [
     ("vfs_copy_stereo", ctypes.c_int, [ctypes.c_void_p, _c_float_p, ctypes.c_int, _c_float_p, ctypes.c_int]
      + [ctypes.c_float] * 11),
     ("vfs_copy_warped_image", ctypes.c_int, [ctypes.c_void_p, _c_float_p, ctypes.c_int]),
+    ("vfs_copy_disparity_vis", ctypes.c_int, [ctypes.c_void_p, _c_float_p, ctypes.c_int, ctypes.c_float]),
     ("vfs_last_error", ctypes.c_char_p, [ctypes.c_void_p]),
     ("vfs_get_level_dims", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, _c_int_p, _c_int_p, _c_int_p]),
     ("vfs_read_plane", ctypes.c_int, [ctypes.c_void_p, ctypes.c_char_p, ctypes.c_int, _c_float_p]),
@@ -237,6 +238,13 @@ class StereoTgv:
     def copyWarpedImageToHost(self) -> np.ndarray:
         out = np.empty((self.height, self.width), np.float32)
         self._check(self._L.vfs_copy_warped_image(self._h, _fptr(out), self.width), "copyWarpedImageToHost")
+        return out
+
+    def copyDisparityVisToHost(self, flowScale: float) -> np.ndarray:
+        """FlowToHSV colour coding of the flow, (h, w, 3) float32 in (B, G, R) order."""
+        out = np.empty((self.height, self.width, 3), np.float32)
+        self._check(self._L.vfs_copy_disparity_vis(self._h, _fptr(out), 3 * self.width, float(flowScale)),
+                    "copyDisparityVisToHost")
         return out
 
     def copyStereoToHost(self, focalx, focaly, cx, cy, d1, d2, d3, d4, t1, t2, t3):

@@ -1060,6 +1060,70 @@ __global__ void k_convert_kb(const float2 *__restrict__ uv, double fx, double fy
     depth[pos] = (float)sqrt(X0 * X0 + X1 * X1 + X2 * X2);
 }
 
+// flowToHsv.cu:3-111 (ComputeColorKernel): colour coding of the flow.  The 55-entry
+// Middlebury colour wheel is built from its segment lengths (red-yellow 15, yellow-green
+// 6, green-cyan 4, cyan-blue 11, blue-magenta 13, magenta-red 6; entry i of a segment of
+// length n ramps by floor(255 i / n)).  Positions are 1-based as in the reference, whose
+// 55-entry table is read one past its end at position 55 (angle within 2.7e-6 rad of
+// +pi): that position is taken as entry 0.  Output (B, G, R) per pixel.
+__device__ __forceinline__ void wheel_rgb(int k, float &r, float &g, float &b)
+{
+    k = k % 55;
+    int si = 0, start = 0, n = 15;
+    if (k >= 15) { si = 1; start = 15; n = 6; }
+    if (k >= 21) { si = 2; start = 21; n = 4; }
+    if (k >= 25) { si = 3; start = 25; n = 11; }
+    if (k >= 36) { si = 4; start = 36; n = 13; }
+    if (k >= 49) { si = 5; start = 49; n = 6; }
+    const int up = (255 * (k - start)) / n, down = 255 - up;
+    int rr = 0, gg = 0, bb = 0;
+    switch (si) {
+    case 0: rr = 255; gg = up; break;
+    case 1: rr = down; gg = 255; break;
+    case 2: gg = 255; bb = up; break;
+    case 3: gg = down; bb = 255; break;
+    case 4: rr = up; bb = 255; break;
+    default: rr = 255; bb = down; break;
+    }
+    r = (float)rr;
+    g = (float)gg;
+    b = (float)bb;
+}
+
+__global__ void k_flow_hsv(const float2 *__restrict__ uv, int w, int h, int s, float *__restrict__ bgr, float scale)
+{
+    int x, y;
+    if (!pixel(x, y, w, h)) return;
+    const long pos = (long)y * s + x;
+    const float2 f = uv[pos];
+    const float du = f.x / scale, dv = f.y / scale;
+    const float rad = sqrtf(du * du + dv * dv);
+    const float a = atan2f(-dv, -du) / 3.14159f;
+    const float fk = (a + 1.f) / 2.f * 54.f + 1.f;
+    const int k0 = (int)floorf(fk);
+    int k1 = k0 + 1;
+    if (k1 == 56) k1 = 1;
+    const float fr = fk - (float)k0;
+    float r0, g0, b0, r1, g1, b1;
+    wheel_rgb(k0, r0, g0, b0);
+    wheel_rgb(k1, r1, g1, b1);
+    float cr = (1.f - fr) * (r0 / 255.f) + fr * (r1 / 255.f);
+    float cg = (1.f - fr) * (g0 / 255.f) + fr * (g1 / 255.f);
+    float cb = (1.f - fr) * (b0 / 255.f) + fr * (b1 / 255.f);
+    if (rad <= 1.f) {
+        cr = 1.f - rad * (1.f - cr);
+        cg = 1.f - rad * (1.f - cg);
+        cb = 1.f - rad * (1.f - cb);
+    } else {
+        cr *= 0.75f;
+        cg *= 0.75f;
+        cb *= 0.75f;
+    }
+    bgr[3 * pos] = cb;
+    bgr[3 * pos + 1] = cg;
+    bgr[3 * pos + 2] = cr;
+}
+
 template <typename T>
 __global__ void k_widen(const T *__restrict__ src, int nch, float *__restrict__ dst, int w, int h, int s)
 {
@@ -1159,6 +1223,11 @@ void launch_convert_kb(const float2 *uv, double focalx, double focaly, double cx
 {
     k_convert_kb<<<grid_for(d.w, d.h), block2d(), 0, st>>>(uv, focalx, focaly, cx, cy, d1, d2, d3, d4, t1, t2, t3, X,
                                                           depth, d.w, d.h, d.s);
+}
+
+void launch_flow_hsv(const float2 *uv, Dims d, float *bgr, float scale, hipStream_t st)
+{
+    k_flow_hsv<<<grid_for(d.w, d.h), block2d(), 0, st>>>(uv, d.w, d.h, d.s, bgr, scale);
 }
 
 void launch_widen(const void *src, int storage, int nch, float *dst, Dims d, hipStream_t st)

@@ -1047,6 +1047,20 @@ int vfs_copy_stereo(vfs_solver *s, float *depth, int depth_pitch, float *X, int 
     return VFS_OK;
 }
 
+int vfs_copy_disparity_vis(vfs_solver *s, float *out, int pitch, float flowScale)
+{
+    if (!s) return VFS_ERR_ARG;
+    if (!s->initialized) return fail(s, VFS_ERR_STATE, "copy before initialize");
+    if (!out || pitch < 3 * s->width) return fail(s, VFS_ERR_ARG, "bad output pointer or pitch");
+    int rc = set_device(s);
+    if (rc) return rc;
+    if ((rc = wait_done(s))) return rc;
+    const float2 *uv = (const float2 *)slot_ptr(s, (s->roles & vfs::RB_UV) ? vfs::PL_UV1 : vfs::PL_UV0);
+    vfs::launch_flow_hsv(uv, s->lv[0], s->outX, flowScale, s->stream);  // outX: 3-float staging plane
+    if ((rc = check_launch(s))) return rc;
+    return copy_out(s, s->outX, 3, 0, out, pitch);
+}
+
 int vfs_get_level_dims(const vfs_solver *s, int level, int *w, int *h, int *stride)
 {
     if (!s || !w || !h || !stride) return VFS_ERR_ARG;

@@ -154,6 +154,8 @@ void launch_convert_kb(const float2 *uv, double focalx, double focaly, double cx
                        double d2, double d3, double d4, double t1, double t2, double t3, float *X,
                        float *depth, Dims d, hipStream_t st);
 void launch_copy(const void *src, void *dst, size_t bytes, hipStream_t st);
+// FlowToHSV colour coding of a flow plane into 3 floats (B, G, R) per pixel.
+void launch_flow_hsv(const float2 *uv, Dims d, float *bgr, float scale, hipStream_t st);
 // Copies a storage-typed plane into a float plane (inspection).
 void launch_widen(const void *src, int storage, int nch, float *dst, Dims d, hipStream_t st);
 
