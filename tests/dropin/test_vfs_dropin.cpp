@@ -1,15 +1,20 @@
 // Drop-in consumer of the StereoTgv class surface: replays the call sequence of the
 // reference demo test_vfs/main.cpp:104-231 (test_t265_data) through
-// include/stereotgv/stereotgv.h, without OpenCV.  Files replace the OpenCV I/O:
-//   mask / images: binary PGM (P5, 8-bit); vector fields: Middlebury .flo
-//   ("PIEH", int32 width, height, interleaved float32 x, y), as cv::readOpticalFlow.
+// include/stereotgv/stereotgv.h, without OpenCV; include/stereotgv/vfs_io.h replaces the
+// OpenCV I/O: mask and images as 8-bit grayscale PNG (cv::imread IMREAD_GRAYSCALE) or
+// binary PGM; vector fields as Middlebury .flo (cv::readOpticalFlow); equalizeHist and
+// the 2x resize.
 // Outputs written to <outdir>: disparity.flo (copyDisparityToHost), u.raw
-// (copy1DDisparityToHost), depth.raw and X.raw (copyStereoToHost), float32.
+// (copy1DDisparityToHost), depth.raw and X.raw (copyStereoToHost), vis.raw
+// (copyDisparityVisToHost, flow scale 40 as main.cpp:221), float32.
 //
 // usage: test_vfs_dropin <dir> <outdir> [nLevel fScale nWarpIters nSolverIters limitRange
-//                                        equalize(0|1)]
-//   <dir> holds i0.pgm i1.pgm mask.pgm tv.flo cv.flo
+//                                        equalize(0|1) resize(0|1) camera(0|1)]
+//   <dir> holds i0, i1 and mask as .png or .pgm, tv.flo and cv.flo.  resize = 1 halves
+//   the (equalized) frames as main.cpp:196-199; camera 0 = the T265 intrinsics of
+//   main.cpp:116-128, 1 = those of the wang run (main.cpp:235-247).
 #include <stereotgv/stereotgv.h>
+#include <stereotgv/vfs_io.h>
 
 #include <stdint.h>
 #include <stdio.h>
@@ -21,50 +26,18 @@
 
 namespace {
 
-bool read_pgm(const std::string &path, int &w, int &h, std::vector<uint8_t> &data)
+bool exists(const std::string &path)
 {
     FILE *f = fopen(path.c_str(), "rb");
-    if (!f) return false;
-    char magic[3] = {0};
-    int maxv = 0;
-    if (fscanf(f, "%2s %d %d %d", magic, &w, &h, &maxv) != 4 || strcmp(magic, "P5") || maxv != 255) {
-        fclose(f);
-        return false;
-    }
-    fgetc(f);  // single whitespace after the header
-    data.resize((size_t)w * h);
-    const bool ok = fread(data.data(), 1, data.size(), f) == data.size();
-    fclose(f);
-    return ok;
+    if (f) fclose(f);
+    return f != nullptr;
 }
 
-bool read_flo(const std::string &path, int &w, int &h, std::vector<float> &data)
+// <name>.png (8-bit grayscale, cv::imread(IMREAD_GRAYSCALE)) or <name>.pgm
+bool read_gray(const std::string &stem, int &w, int &h, std::vector<uint8_t> &data)
 {
-    FILE *f = fopen(path.c_str(), "rb");
-    if (!f) return false;
-    float tag = 0;
-    int32_t ww = 0, hh = 0;
-    bool ok = fread(&tag, 4, 1, f) == 1 && tag == 202021.25f && fread(&ww, 4, 1, f) == 1 && fread(&hh, 4, 1, f) == 1;
-    if (ok) {
-        w = ww;
-        h = hh;
-        data.resize((size_t)w * h * 2);
-        ok = fread(data.data(), sizeof(float), data.size(), f) == data.size();
-    }
-    fclose(f);
-    return ok;
-}
-
-bool write_flo(const std::string &path, int w, int h, const std::vector<float> &data)
-{
-    FILE *f = fopen(path.c_str(), "wb");
-    if (!f) return false;
-    const float tag = 202021.25f;
-    const int32_t ww = w, hh = h;
-    bool ok = fwrite(&tag, 4, 1, f) == 1 && fwrite(&ww, 4, 1, f) == 1 && fwrite(&hh, 4, 1, f) == 1 &&
-              fwrite(data.data(), sizeof(float), data.size(), f) == data.size();
-    fclose(f);
-    return ok;
+    if (exists(stem + ".png")) return vfs_io::read_png_u8(stem + ".png", w, h, data);
+    return vfs_io::read_pgm(stem + ".pgm", w, h, data);
 }
 
 bool write_raw(const std::string &path, const std::vector<float> &data)
@@ -74,30 +47,6 @@ bool write_raw(const std::string &path, const std::vector<float> &data)
     const bool ok = fwrite(data.data(), sizeof(float), data.size(), f) == data.size();
     fclose(f);
     return ok;
-}
-
-// Histogram equalisation of an 8-bit image (the preprocessing of main.cpp:196-197):
-// lut[v] = round(255 * (cdf(v) - cdf_min) / (N - cdf_min)).
-void equalize(std::vector<uint8_t> &img)
-{
-    size_t hist[256] = {0};
-    for (uint8_t v : img) hist[v]++;
-    size_t cdf = 0, cdf_min = 0;
-    for (int v = 0; v < 256; v++)
-        if (hist[v]) {
-            cdf_min = hist[v];
-            break;
-        }
-    const size_t n = img.size();
-    uint8_t lut[256];
-    for (int v = 0; v < 256; v++) {
-        cdf += hist[v];
-        const double num = (double)(cdf > cdf_min ? cdf - cdf_min : 0);
-        const double den = (double)(n > cdf_min ? n - cdf_min : 1);
-        int o = (int)(255.0 * num / den + 0.5);
-        lut[v] = (uint8_t)(o > 255 ? 255 : o);
-    }
-    for (uint8_t &v : img) v = lut[v];
 }
 
 int fail(const char *what, StereoTgv *s)
@@ -111,7 +60,8 @@ int fail(const char *what, StereoTgv *s)
 int main(int argc, char **argv)
 {
     if (argc < 3) {
-        fprintf(stderr, "usage: %s <dir> <outdir> [nLevel fScale nWarpIters nSolverIters limitRange equalize]\n",
+        fprintf(stderr,
+                "usage: %s <dir> <outdir> [nLevel fScale nWarpIters nSolverIters limitRange equalize resize camera]\n",
                 argv[0]);
         return 2;
     }
@@ -122,17 +72,29 @@ int main(int argc, char **argv)
     int nSolverIters = argc > 6 ? atoi(argv[6]) : 50;
     float limit = argc > 7 ? (float)atof(argv[7]) : 1.0f;
     bool equ = argc > 8 ? atoi(argv[8]) != 0 : false;
+    bool half = argc > 9 ? atoi(argv[9]) != 0 : false;
+    int camera = argc > 10 ? atoi(argv[10]) : 0;
 
     std::vector<uint8_t> im1, im2, mask8;
     int w = 0, h = 0, w2, h2, wm, hm;
-    if (!read_pgm(dir + "/i0.pgm", w, h, im1) || !read_pgm(dir + "/i1.pgm", w2, h2, im2) || w2 != w || h2 != h) {
-        fprintf(stderr, "cannot read i0.pgm / i1.pgm\n");
+    if (!read_gray(dir + "/i0", w, h, im1) || !read_gray(dir + "/i1", w2, h2, im2) || w2 != w || h2 != h) {
+        fprintf(stderr, "cannot read i0 / i1\n");
         return 2;
     }
-    // Camera parameters (main.cpp:116-128)
+    // Camera parameters (main.cpp:116-128; the wang run: main.cpp:235-247)
     float focalx = 285.722f, focaly = 286.759f, cx = 420.135f, cy = 403.394f;
     float d0 = -0.00659769f, d1 = 0.0473251f, d2 = -0.0458264f, d3 = 0.00897725f;
     float tx = -0.0641854f, ty = -0.000218299f, tz = 0.000111253f;
+    if (camera == 1) {
+        focalx = 286.518f, focaly = 286.602f, cx = 423.142f, cy = 393.321f;
+        d0 = -0.00727452f, d1 = 0.05200624f, d2 = -0.05105740f, d3 = 0.01108238f;
+        tx = -0.0639505f, ty = 0.000184912f, tz = -0.0000766807f;
+    }
+    // Solve size: the frames, halved when resize = 1 (main.cpp:148-150)
+    if (half) {
+        w /= 2;
+        h /= 2;
+    }
     float stereoScaling = 848.0f / (float)w;
 
     // VFS parameters (main.cpp:130-146)
@@ -147,9 +109,10 @@ int main(int argc, char **argv)
     // Load fisheye mask and vector fields (main.cpp:155-192)
     std::vector<float> tv, cv;
     int wt, ht, wc, hc;
-    if (!read_pgm(dir + "/mask.pgm", wm, hm, mask8) || !read_flo(dir + "/tv.flo", wt, ht, tv) ||
-        !read_flo(dir + "/cv.flo", wc, hc, cv) || wm != w || hm != h || wt != w || ht != h || wc != w || hc != h) {
-        fprintf(stderr, "cannot read mask.pgm / tv.flo / cv.flo at %dx%d\n", w, h);
+    if (!read_gray(dir + "/mask", wm, hm, mask8) || !vfs_io::read_flo(dir + "/tv.flo", wt, ht, tv) ||
+        !vfs_io::read_flo(dir + "/cv.flo", wc, hc, cv) || wm != w || hm != h || wt != w || ht != h || wc != w ||
+        hc != h) {
+        fprintf(stderr, "cannot read mask / tv.flo / cv.flo at %dx%d\n", w, h);
         return 2;
     }
     std::vector<float> mask(mask8.size());
@@ -159,8 +122,12 @@ int main(int argc, char **argv)
 
     // Solve stereo depth (main.cpp:194-211)
     if (equ) {
-        equalize(im1);
-        equalize(im2);
+        vfs_io::equalize_hist(im1.data(), im1.size());
+        vfs_io::equalize_hist(im2.data(), im2.size());
+    }
+    if (half) {
+        im1 = vfs_io::resize_half(im1.data(), 2 * w, 2 * h);
+        im2 = vfs_io::resize_half(im2.data(), 2 * w, 2 * h);
     }
     if (stereotgv->copyImagesToDevice(im1.data(), im2.data())) return fail("copyImagesToDevice", stereotgv);
     if (stereotgv->solveStereoForwardMasked()) return fail("solveStereoForwardMasked", stereotgv);
@@ -172,8 +139,11 @@ int main(int argc, char **argv)
     std::vector<float> disparity((size_t)w * h * 2), u((size_t)w * h);
     if (stereotgv->copyDisparityToHost(disparity.data())) return fail("copyDisparityToHost", stereotgv);
     if (stereotgv->copy1DDisparityToHost(u.data())) return fail("copy1DDisparityToHost", stereotgv);
-    if (!write_flo(out + "/disparity.flo", w, h, disparity) || !write_raw(out + "/u.raw", u) ||
-        !write_raw(out + "/depth.raw", depth) || !write_raw(out + "/X.raw", X)) {
+    // main.cpp:220-221
+    std::vector<float> vis((size_t)w * h * 3);
+    if (stereotgv->copyDisparityVisToHost(vis.data(), 40.0f)) return fail("copyDisparityVisToHost", stereotgv);
+    if (!vfs_io::write_flo(out + "/disparity.flo", w, h, disparity) || !write_raw(out + "/u.raw", u) ||
+        !write_raw(out + "/depth.raw", depth) || !write_raw(out + "/X.raw", X) || !write_raw(out + "/vis.raw", vis)) {
         fprintf(stderr, "cannot write outputs to %s\n", out.c_str());
         return 2;
     }

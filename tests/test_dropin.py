@@ -92,6 +92,8 @@ def test_dropin_matches_python_mirror_and_oracle(tmp_path):
     dg, _ = g.copyStereoToHost(f32(285.722) / s, f32(286.759) / s, f32(420.135) / s, f32(403.394) / s, -0.00659769,
                                0.0473251, -0.0458264, 0.00897725, -0.0641854, -0.000218299, 0.000111253)
     assert np.array_equal(depth, dg, equal_nan=True)
+    vis = np.fromfile(out / "vis.raw", np.float32).reshape(H, W, 3)
+    assert np.array_equal(vis, g.copyDisparityVisToHost(40.0))
 
     o = O.Oracle(W, H, P["beta"], P["gamma"], P["alpha0"], P["alpha1"], P["timestep_lambda"], P["lam"], L, 2.0, WI,
                  IT)
@@ -103,3 +105,38 @@ def test_dropin_matches_python_mirror_and_oracle(tmp_path):
     m = mask > 0
     err = np.abs(u - o.copy1DDisparityToHost())[m]
     assert err.mean() <= 5e-4, err.mean()
+
+
+@pytest.mark.gpu
+def test_dropin_on_the_reference_fixtures(tmp_path):
+    """The consumer on the reference's own wang data (PNG frames and mask, .flo fields;
+    equalize at 848x800 then halve, L5 W20 I10, limitRange 0.1, wang intrinsics: the run
+    that wrote test_vfs/depth16.png): the C++ I/O and preprocessing give the same
+    disparity as the Python mirror of them, and the depth meets the fixture pin."""
+    from test_fixtures import D, compare, inputs, INTRINSICS
+    from vfs_amd import StereoTgv
+    from vfs_amd import imageio as io
+
+    for src, dst in (("wang1.png", "i0.png"), ("wang2.png", "i1.png"), ("maskHalf.png", "mask.png"),
+                     ("translationVectorWangHalf.flo", "tv.flo"), ("calibrationVectorWangHalf.flo", "cv.flo")):
+        os.symlink(os.path.join(D, src), tmp_path / dst)
+    out = tmp_path / "out"
+    out.mkdir()
+    print(f"dropin: running {BIN} on the wang fixtures", flush=True)
+    r = subprocess.run([BIN, str(tmp_path), str(out), "5", "2.0", "20", "10", "0.1", "1", "1", "1"],
+                       stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout
+    W, H = 424, 400
+    u = np.fromfile(out / "u.raw", np.float32).reshape(H, W)
+    depth = np.fromfile(out / "depth.raw", np.float32).reshape(H, W)
+    i0, i1, mask, tv, cv = inputs()
+    P = S.SOLVER_PARAMS
+    g = StereoTgv()
+    g.initialize(W, H, P["beta"], P["gamma"], P["alpha0"], P["alpha1"], P["timestep_lambda"], P["lam"], 5, 2.0, 20, 10)
+    g.limitRange = 0.1
+    g.copyMaskToDevice(mask)
+    g.loadVectorFields(tv, cv)
+    g.copyImagesToDevice(i0, i1)
+    g.solveStereoForwardMasked()
+    assert np.array_equal(u, g.copy1DDisparityToHost())
+    compare("dropin depth16", io.to_u16(depth, 256), io.read_png_gray(os.path.join(D, "depth16.png")), mask > 0)

@@ -43,6 +43,17 @@ def build_lib(verbose=False) -> str:
     return out
 
 
+def build_dllhook(verbose=False) -> str:
+    """libvfs_dllhook.so: the reference's DLL entry points (Create / Init / SolveStereo)
+    over libvfs (host C++ only)."""
+    out = os.path.join(PKG, "libvfs_dllhook.so")
+    tmp = out + ".tmp"
+    _run(["g++", "-O2", "-std=c++17", "-Wall", "-fPIC", "-shared", "-I", os.path.join(ROOT, "include"),
+          os.path.join(CSRC, "dllhook.cpp"), "-o", tmp, "-L", PKG, "-lvfs", "-lz", "-Wl,-rpath,$ORIGIN"], verbose)
+    os.replace(tmp, out)
+    return out
+
+
 def build_dropin(verbose=False) -> str | None:
     """The C++ consumer that replays test_vfs/main.cpp through include/stereotgv/stereotgv.h."""
     src = os.path.join(ROOT, "tests", "dropin", "test_vfs_dropin.cpp")
@@ -50,12 +61,13 @@ def build_dropin(verbose=False) -> str | None:
         return None
     out = os.path.join(ROOT, "tests", "dropin", "test_vfs_dropin")
     _run(["g++", "-O2", "-std=c++17", "-Wall", "-I", os.path.join(ROOT, "include"), src, "-o", out,
-          "-L", PKG, "-lvfs", "-Wl,-rpath,$ORIGIN/../../vfs_amd"], verbose)
+          "-L", PKG, "-lvfs", "-lz", "-Wl,-rpath,$ORIGIN/../../vfs_amd"], verbose)
     return out
 
 
 def build_all(verbose=False):
     build_lib(verbose)
+    build_dllhook(verbose)
     build_dropin(verbose)
 
 
