@@ -437,11 +437,18 @@ __device__ __forceinline__ void op_deriv(const Tile &t, int level)
     }
 }
 
+// The inner iteration divides and takes square roots several times per pixel; it uses
+// the hardware reciprocal and square root (about 1 ulp) instead of the correctly rounded
+// expansions, which are several times longer.  NaN and infinity propagate as in IEEE
+// division.
+__device__ __forceinline__ float fast_div(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
+__device__ __forceinline__ float fast_sqrt(float a) { return __builtin_amdgcn_sqrtf(a); }
+
 // L1 proximal step (thresholdingL1.cu:180-215), linearised at the current u.
 __device__ __forceinline__ float thresh_core(float divp, float tau, float etau, float u_pos, float upos, float Iu,
                                              float Iz, float lambda)
 {
-    const float te = (etau == 0.f) ? tau : tau / etau;
+    const float te = (etau == 0.f) ? tau : fast_div(tau, etau);
     const float uhat = u_pos + te * divp;
     const float dun = uhat - upos;
     const float rho = Iu * dun + Iz;
@@ -449,7 +456,7 @@ __device__ __forceinline__ float thresh_core(float divp, float tau, float etau, 
     const float lower = -lambda * te * (Iu * Iu);
     float du = 0.f;
     if ((rho <= upper) && (rho >= lower)) {
-        du = (Iu == 0.f) ? dun : dun - rho / Iu;
+        du = (Iu == 0.f) ? dun : dun - fast_div(rho, Iu);
     } else if (rho < lower) {
         du = dun + lambda * te * Iu;
     } else if (rho > upper) {
@@ -533,9 +540,10 @@ __device__ __forceinline__ void dual_step(const DualIn &r, int x, int y, int w, 
         const float dty = r.c * ux + r.b * uy;
         const float px = r.po.x + ip.k1 * dtx;
         const float py = r.po.y + ip.k1 * dty;
-        float rp = sqrtf(px * px + py * py);
+        float rp = fast_sqrt(px * px + py * py);
         if (rp < 1.0f) rp = 1.0f;
-        pn = make_float2(px / rp, py / rp);
+        const float irp = __builtin_amdgcn_rcpf(rp);
+        pn = make_float2(px * irp, py * irp);
         float g0, g1, g2, g3;
         if (r.hasR) {
             g0 = r.vR.x - r.v0.x;
@@ -553,9 +561,10 @@ __device__ __forceinline__ void dual_step(const DualIn &r, int x, int y, int w, 
         }
         const float q0 = r.qo.x + ip.k0 * g0, q1 = r.qo.y + ip.k0 * g1, q2 = r.qo.z + ip.k0 * g2,
                     q3 = r.qo.w + ip.k0 * g3;
-        float rq = sqrtf(q0 * q0 + q1 * q1 + q2 * q2 + q3 * q3);
+        float rq = fast_sqrt(q0 * q0 + q1 * q1 + q2 * q2 + q3 * q3);
         if (rq < 1.0f) rq = 1.0f;
-        qn = make_float4(q0 / rq, q1 / rq, q2 / rq, q3 / rq);
+        const float irq = __builtin_amdgcn_rcpf(rq);
+        qn = make_float4(q0 * irq, q1 * irq, q2 * irq, q3 * irq);
     }
     tpn = make_float2(r.a * pn.x + r.c * pn.y, r.c * pn.x + r.b * pn.y);
 }
@@ -746,10 +755,10 @@ __device__ __forceinline__ void op_iter(const Tile &t, int roles, const IterPara
         const float2 vo = q.v0;
         // e1, e2 >= 4 alpha0^2 > 0 for alpha0 != 0; the reference's zero branches
         // (updatePrimalVariables.cu:228-239, the second writes .x) are kept for alpha0 == 0.
-        float vx = vo.x + ((e1 != 0.f) ? ip.tau / e1 : ip.tau) * (ip.alpha1 * tx + ip.alpha0 * dqx);
+        float vx = vo.x + ((e1 != 0.f) ? fast_div(ip.tau, e1) : ip.tau) * (ip.alpha1 * tx + ip.alpha0 * dqx);
         float vy;
         if (e2 != 0.f) {
-            vy = vo.y + (ip.tau / e2) * (ip.alpha1 * tyy + ip.alpha0 * dqy);
+            vy = vo.y + fast_div(ip.tau, e2) * (ip.alpha1 * tyy + ip.alpha0 * dqy);
         } else {
             vx = vo.x + ip.tau * (ip.alpha1 * tx + ip.alpha0 * dqx);
             vy = 0.f;
