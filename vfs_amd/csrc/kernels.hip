@@ -266,8 +266,8 @@ __device__ __forceinline__ void op_down(const Tile &t, int level)
 template <class S>
 __device__ __forceinline__ void op_cinit(const Tile &t, int level, int roles)
 {
-    const float *u = slot(t, (roles & RB_U) ? PL_U1 : PL_U0);
-    float2 *uv = slotp<float2>(t, (roles & RB_UV) ? PL_UV1 : PL_UV0);
+    const float *u = slot(t, slot_u(rcur(roles, G_U)));
+    float2 *uv = slotp<float2>(t, slot_uv(roles, true));
     const float2 *tv = t.d->tv[level];
     FOR_TILE_PIXELS(S, t, x, y)
     {
@@ -285,12 +285,12 @@ template <class S>
 __device__ __forceinline__ void op_winit(const Tile &t, int level, int roles, float scale)
 {
     const int W = t.d->w[level - 1], H = t.d->h[level - 1], SF = t.d->s[level - 1];
-    const float *u = slot(t, (roles & RB_U) ? PL_U1 : PL_U0);
-    float *ou = slot(t, (roles & RB_U) ? PL_U0 : PL_U1);
-    const float *u_ = slot(t, (roles & RB_UB) ? PL_UB1 : PL_UB0);
-    float *ou_ = slot(t, (roles & RB_UB) ? PL_UB0 : PL_UB1);
-    const float2 *uv = slotp<float2>(t, (roles & RB_UV) ? PL_UV1 : PL_UV0);
-    float2 *ouv = slotp<float2>(t, (roles & RB_UV) ? PL_UV0 : PL_UV1);
+    const float *u = slot(t, slot_u(rcur(roles, G_U)));
+    float *ou = slot(t, slot_u(roth(roles, G_U)));
+    const float *u_ = slot(t, slot_ub(rcur(roles, G_UB)));
+    float *ou_ = slot(t, slot_ub(roth(roles, G_UB)));
+    const float2 *uv = slotp<float2>(t, slot_uv(roles, true));
+    float2 *ouv = slotp<float2>(t, slot_uv(roles, false));
     const float rx = (float)W / (float)t.w, ry = (float)H / (float)t.h;
     const float ox = 0.25f * rx, oy = 0.25f * ry;
     FOR_TILE_PIXELS(S, t, x, y)
@@ -384,9 +384,9 @@ __device__ __forceinline__ void op_tensor(const Tile &t, int level, int xblock, 
 template <class S, typename T>
 __device__ __forceinline__ void op_prep(const Tile &t, int level, int roles)
 {
-    const float2 *uv = slotp<float2>(t, (roles & RB_UV) ? PL_UV1 : PL_UV0);
-    const float *u = slot(t, (roles & RB_U) ? PL_U1 : PL_U0);
-    float2 *v_ = slotp<float2>(t, (roles & RB_V) ? PL_V1 : PL_V0);
+    const float2 *uv = slotp<float2>(t, slot_uv(roles, true));
+    const float *u = slot(t, slot_u(rcur(roles, G_U)));
+    float2 *v_ = slotp<float2>(t, slot_v(rcur(roles, G_V)));
     const float2 *tv = t.d->tv[level];
     const T *I1 = (const T *)t.d->I1[level];
     float2 *tv2 = slotp<float2>(t, PL_TV2);
@@ -569,6 +569,82 @@ __device__ __forceinline__ void dual_step(const DualIn &r, int x, int y, int w, 
     tpn = make_float2(r.a * pn.x + r.c * pn.y, r.c * pn.x + r.b * pn.y);
 }
 
+// Thresholding and primal step at one pixel (thresholdingL1.cu:176-216 / :60-86,
+// updatePrimalVariables.cu:165-245 / :30-113) from the new duals: divp is the divergence
+// of Tp, qc4 / pp the pixel's new q / p, ql / qu the new q of its left / upper
+// neighbours (read only where they are inside the mask).  Returns u, u_ and v_ of the
+// next iteration.
+__device__ __forceinline__ void primal_step(const IterParams &ip, float a, float b, float c, float u_pos, float uu,
+                                            float iu, float iz, float2 vo, bool hR, bool hD, bool hL, bool hU,
+                                            float divp, float4 qc4, float4 ql, float4 qu, float2 pp, float &un,
+                                            float &u_s, float2 &v_s)
+{
+    const float aa = ip.alpha1 * ip.alpha1;
+    const float etau = (a * a + b * b + 2 * c * c + (a + c) * (a + c) + (b + c) * (b + c)) * aa;
+    un = thresh_core(divp, ip.tau, etau, u_pos, uu, iu, iz, ip.lambda);
+    float qcx = 0.f, qcw = 0.f, qcy = 0.f, qcz = 0.f;
+    if (hR) {
+        qcx = qc4.x;
+        qcw = qc4.w;
+    }
+    if (hD) {
+        qcy = qc4.y;
+        qcz = qc4.z;
+    }
+    float qwx = 0.f, qww = 0.f, qny = 0.f, qnz = 0.f;
+    if (hL) {
+        qwx = ql.x;
+        qww = ql.w;
+    }
+    if (hU) {
+        qny = qu.y;
+        qnz = qu.z;
+    }
+    const float dqx = (qcx - qwx) + (qcz - qnz);
+    const float dqy = (qcw - qww) + (qcy - qny);
+    const float tx = a * pp.x + c * pp.y;
+    const float tyy = c * pp.x + b * pp.y;
+    const float e1 = aa * (b * b + c * c) + 4 * ip.alpha0 * ip.alpha0;
+    const float e2 = aa * (a * a + c * c) + 4 * ip.alpha0 * ip.alpha0;
+    // e1, e2 >= 4 alpha0^2 > 0 for alpha0 != 0; the reference's zero branches
+    // (updatePrimalVariables.cu:228-239, the second writes .x) are kept for alpha0 == 0.
+    float vx = vo.x + ((e1 != 0.f) ? fast_div(ip.tau, e1) : ip.tau) * (ip.alpha1 * tx + ip.alpha0 * dqx);
+    float vy;
+    if (e2 != 0.f) {
+        vy = vo.y + fast_div(ip.tau, e2) * (ip.alpha1 * tyy + ip.alpha0 * dqy);
+    } else {
+        vx = vo.x + ip.tau * (ip.alpha1 * tx + ip.alpha0 * dqx);
+        vy = 0.f;
+    }
+    u_s = un + ip.mu * (un - u_pos);
+    v_s = make_float2(vx + ip.mu * (vx - vo.x), vy + ip.mu * (vy - vo.y));
+}
+
+// Divergence of Tp at a pixel (thresholdingL1.cu:157-175 masked / :29-45 unmasked): tp0
+// the pixel's new Tp, tpLx / tpUy the left / upper neighbour's.
+template <bool MASKED>
+__device__ __forceinline__ float div_tp(int x, int y, int w, int h, bool hR, bool hD, bool hL, bool hU, float2 tp0,
+                                        float tpLx, float tpUy)
+{
+    float dx, dy;
+    if (MASKED) {
+        if (hL && hR) dx = tp0.x - tpLx;
+        else if (!hR) dx = -tpLx;
+        else dx = tp0.x;
+        if (hU && hD) dy = tp0.y - tpUy;
+        else if (!hD) dy = -tpUy;
+        else dy = tp0.y;
+    } else {
+        if (hL) dx = tp0.x - tpLx;
+        else if (x == w - 1) dx = -tpLx;
+        else dx = tp0.x;
+        if (hU) dy = tp0.y - tpUy;
+        else if (y == h - 1) dy = -tpUy;
+        else dy = tp0.y;
+    }
+    return dx + dy;
+}
+
 // One TGV-L1 inner iteration in one launch: dual ascent + projection
 // (updateDualVariables.cu:122-237 / :4-101), Tp (thresholdingL1.cu:234-244), L1
 // thresholding (thresholdingL1.cu:128-216 / :3-86) and the primal step with
@@ -591,16 +667,16 @@ __device__ __forceinline__ void op_iter(const Tile &t, int roles, const IterPara
     constexpr int BX = S::BX, BY = S::BY, RPT = S::RPT;
     const int w = t.w, h = t.h, s = t.s, x0 = t.x0, y0 = t.y0;
     const float *__restrict__ mask = t.mask;
-    const float *__restrict__ u = slot(t, (roles & RB_U) ? PL_U1 : PL_U0);
-    float *__restrict__ us = slot(t, (roles & RB_U) ? PL_U0 : PL_U1);
-    const float *__restrict__ u_ = slot(t, (roles & RB_UB) ? PL_UB1 : PL_UB0);
-    float *__restrict__ u_s = slot(t, (roles & RB_UB) ? PL_UB0 : PL_UB1);
-    const float2 *__restrict__ v_ = slotp<float2>(t, (roles & RB_V) ? PL_V1 : PL_V0);
-    float2 *__restrict__ v_s = slotp<float2>(t, (roles & RB_V) ? PL_V0 : PL_V1);
-    const V2<T> *__restrict__ p_in = slotp<V2<T>>(t, (roles & RB_PQ) ? PL_P1 : PL_P0);
-    V2<T> *__restrict__ p_out = slotp<V2<T>>(t, (roles & RB_PQ) ? PL_P0 : PL_P1);
-    const V4<T> *__restrict__ q_in = slotp<V4<T>>(t, (roles & RB_PQ) ? PL_Q1 : PL_Q0);
-    V4<T> *__restrict__ q_out = slotp<V4<T>>(t, (roles & RB_PQ) ? PL_Q0 : PL_Q1);
+    const float *__restrict__ u = slot(t, slot_u(rcur(roles, G_U)));
+    float *__restrict__ us = slot(t, slot_u(roth(roles, G_U)));
+    const float *__restrict__ u_ = slot(t, slot_ub(rcur(roles, G_UB)));
+    float *__restrict__ u_s = slot(t, slot_ub(roth(roles, G_UB)));
+    const float2 *__restrict__ v_ = slotp<float2>(t, slot_v(rcur(roles, G_V)));
+    float2 *__restrict__ v_s = slotp<float2>(t, slot_v(roth(roles, G_V)));
+    const V2<T> *__restrict__ p_in = slotp<V2<T>>(t, slot_p(rcur(roles, G_PQ)));
+    V2<T> *__restrict__ p_out = slotp<V2<T>>(t, slot_p(roth(roles, G_PQ)));
+    const V4<T> *__restrict__ q_in = slotp<V4<T>>(t, slot_q(rcur(roles, G_PQ)));
+    V4<T> *__restrict__ q_out = slotp<V4<T>>(t, slot_q(roth(roles, G_PQ)));
     float2 *__restrict__ Tp = slotp<float2>(t, PL_TP);
     const T *__restrict__ at = (const T *)slot(t, PL_A);
     const T *__restrict__ bt = (const T *)slot(t, PL_B);
@@ -682,7 +758,6 @@ __device__ __forceinline__ void op_iter(const Tile &t, int roles, const IterPara
     __syncthreads();
 
     // ---- phase B: thresholding + primal step on the tile
-    const float aa = ip.alpha1 * ip.alpha1;
 #pragma unroll
     for (int r = 0; r < RPT; r++) {
         const DualIn &q = in[r];
@@ -704,67 +779,16 @@ __device__ __forceinline__ void op_iter(const Tile &t, int roles, const IterPara
         // holds what earlier levels left there, as the reference does
         const float tpLx = hL ? sT[ey][ex - 1].x : (ip.preload_tp ? tpl[r] : ld0(Tp, pos - 1).x);
         const float tpUy = hU ? sT[ey - 1][ex].y : (ip.preload_tp ? tpu[r] : ld0(Tp, pos - s).y);
-        float dx, dy;
-        if (MASKED) {
-            if (hL && hR) dx = tp0.x - tpLx;
-            else if (!hR) dx = -tpLx;
-            else dx = tp0.x;
-            if (hU && hD) dy = tp0.y - tpUy;
-            else if (!hD) dy = -tpUy;
-            else dy = tp0.y;
-        } else {
-            if (hL) dx = tp0.x - tpLx;
-            else if (x == w - 1) dx = -tpLx;
-            else dx = tp0.x;
-            if (hU) dy = tp0.y - tpUy;
-            else if (y == h - 1) dy = -tpUy;
-            else dy = tp0.y;
-        }
-        const float etau = (a * a + b * b + 2 * c * c + (a + c) * (a + c) + (b + c) * (b + c)) * aa;
-        const float u_pos = q.u0;
-        const float un = thresh_core(dx + dy, ip.tau, etau, u_pos, uu[r], iu[r], iz[r], ip.lambda);
+        const float divp = div_tp<MASKED>(x, y, w, h, hR, hD, hL, hU, tp0, tpLx, tpUy);
+        const float4 ql = hL ? sQ[ey][ex - 1] : make_float4(0.f, 0.f, 0.f, 0.f);
+        const float4 qu = hU ? sQ[ey - 1][ex] : make_float4(0.f, 0.f, 0.f, 0.f);
+        float un, u_s_v;
+        float2 v_s_v;
+        primal_step(ip, a, b, c, q.u0, uu[r], iu[r], iz[r], q.v0, hR, hD, hL, hU, divp, sQ[ey][ex], ql, qu,
+                    sP[ey][ex], un, u_s_v, v_s_v);
         us[pos] = un;
-        const float4 qc4 = sQ[ey][ex];
-        float qcx = 0.f, qcw = 0.f, qcy = 0.f, qcz = 0.f;
-        if (hR) {
-            qcx = qc4.x;
-            qcw = qc4.w;
-        }
-        if (hD) {
-            qcy = qc4.y;
-            qcz = qc4.z;
-        }
-        float qwx = 0.f, qww = 0.f, qny = 0.f, qnz = 0.f;
-        if (hL) {
-            const float4 ql = sQ[ey][ex - 1];
-            qwx = ql.x;
-            qww = ql.w;
-        }
-        if (hU) {
-            const float4 qu = sQ[ey - 1][ex];
-            qny = qu.y;
-            qnz = qu.z;
-        }
-        const float dqx = (qcx - qwx) + (qcz - qnz);
-        const float dqy = (qcw - qww) + (qcy - qny);
-        const float2 pp = sP[ey][ex];
-        const float tx = a * pp.x + c * pp.y;
-        const float tyy = c * pp.x + b * pp.y;
-        const float e1 = aa * (b * b + c * c) + 4 * ip.alpha0 * ip.alpha0;
-        const float e2 = aa * (a * a + c * c) + 4 * ip.alpha0 * ip.alpha0;
-        const float2 vo = q.v0;
-        // e1, e2 >= 4 alpha0^2 > 0 for alpha0 != 0; the reference's zero branches
-        // (updatePrimalVariables.cu:228-239, the second writes .x) are kept for alpha0 == 0.
-        float vx = vo.x + ((e1 != 0.f) ? fast_div(ip.tau, e1) : ip.tau) * (ip.alpha1 * tx + ip.alpha0 * dqx);
-        float vy;
-        if (e2 != 0.f) {
-            vy = vo.y + fast_div(ip.tau, e2) * (ip.alpha1 * tyy + ip.alpha0 * dqy);
-        } else {
-            vx = vo.x + ip.tau * (ip.alpha1 * tx + ip.alpha0 * dqx);
-            vy = 0.f;
-        }
-        u_s[pos] = un + ip.mu * (un - u_pos);
-        v_s[pos] = make_float2(vx + ip.mu * (vx - vo.x), vy + ip.mu * (vy - vo.y));
+        u_s[pos] = u_s_v;
+        v_s[pos] = v_s_v;
     }
 }
 
@@ -816,10 +840,10 @@ __device__ __forceinline__ float median5_at(const float *__restrict__ u, int x, 
 template <class S, bool MASKED>
 __device__ __forceinline__ void op_end(const Tile &t, int roles, float limit)
 {
-    const float *u = slot(t, (roles & RB_U) ? PL_U1 : PL_U0);
-    float *out = MASKED ? slot(t, (roles & RB_U) ? PL_U0 : PL_U1) : (float *)u;
-    float *u_ = slot(t, (roles & RB_UB) ? PL_UB1 : PL_UB0);
-    float2 *uv = slotp<float2>(t, (roles & RB_UV) ? PL_UV1 : PL_UV0);
+    const float *u = slot(t, slot_u(rcur(roles, G_U)));
+    float *out = MASKED ? slot(t, slot_u(roth(roles, G_U))) : (float *)u;
+    float *u_ = slot(t, slot_ub(rcur(roles, G_UB)));
+    float2 *uv = slotp<float2>(t, slot_uv(roles, true));
     const float *ulast = slot(t, PL_ULAST);
     const float2 *tv2 = slotp<float2>(t, PL_TV2);
     FOR_TILE_PIXELS(S, t, x, y)
@@ -844,12 +868,12 @@ template <class S>
 __device__ __forceinline__ void op_up(const Tile &t, int level, int roles, float scale)
 {  // the tile belongs to the finer level level - 1
     const int w = t.d->w[level], h = t.d->h[level], s = t.d->s[level];
-    const float *u = slot(t, (roles & RB_U) ? PL_U1 : PL_U0);
-    float *ou = slot(t, (roles & RB_U) ? PL_U0 : PL_U1);
-    const float *u_ = slot(t, (roles & RB_UB) ? PL_UB1 : PL_UB0);
-    float *ou_ = slot(t, (roles & RB_UB) ? PL_UB0 : PL_UB1);
-    const float2 *uv = slotp<float2>(t, (roles & RB_UV) ? PL_UV1 : PL_UV0);
-    float2 *ouv = slotp<float2>(t, (roles & RB_UV) ? PL_UV0 : PL_UV1);
+    const float *u = slot(t, slot_u(rcur(roles, G_U)));
+    float *ou = slot(t, slot_u(roth(roles, G_U)));
+    const float *u_ = slot(t, slot_ub(rcur(roles, G_UB)));
+    float *ou_ = slot(t, slot_ub(roth(roles, G_UB)));
+    const float2 *uv = slotp<float2>(t, slot_uv(roles, true));
+    float2 *ouv = slotp<float2>(t, slot_uv(roles, false));
     const float rx = (float)w / (float)t.w, ry = (float)h / (float)t.h;
     FOR_TILE_PIXELS(S, t, x, y)
     {
@@ -933,8 +957,8 @@ __global__ __launch_bounds__(NT) void k_tick(const PairDesc *__restrict__ D, con
         break;
     case OP_ITER: {
         IterParams ip;
-        ip.k1 = E.k1;
-        ip.k0 = E.k0;
+        ip.k1 = A.alpha1 * E.sigma / 3.0f;  // eta_p = 3 (stereotgv.cu:416)
+        ip.k0 = A.alpha0 * E.sigma / 2.0f;  // eta_q = 2 (stereotgv.cu:417)
         ip.tau = E.tau;
         ip.mu = E.mu;
         ip.lambda = A.lambda;

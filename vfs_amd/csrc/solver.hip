@@ -81,7 +81,7 @@ struct Params {
 struct ProgOp {
     unsigned char kind, level, masked, flags, shape;
     int roles;
-    float k1, k0, tau, mu;
+    float sigma, tau, mu, mu_next;
 };
 
 // Launch choices for one pyramid level: tile shape of every op at the level and whether
@@ -140,7 +140,7 @@ struct vfs_solver {
     // work arena
     float *arena = nullptr;
     long long plane = 0;
-    int roles = 0;  // ping-pong roles after the last solve
+    int roles = vfs::kRolesInit;  // buffer roles after the last solve
     bool solved = false;  // a solve has run since initialize
 
     Engine eng;  // single-pair launch set
@@ -300,17 +300,16 @@ void push_level(Program &pr, const Params &P, int masked, int l, int warps, int 
             o.masked = (unsigned char)masked;
             o.flags = (unsigned char)((it == 0 ? vfs::EF_FIRST : 0) | (it == iters - 1 ? vfs::EF_WRITE_TP : 0));
             o.roles = roles;
-            o.k1 = P.alpha1 * sigma / 3.0f;  // eta_p = 3 (stereotgv.cu:416)
-            o.k0 = P.alpha0 * sigma / 2.0f;  // eta_q = 2 (stereotgv.cu:417)
+            o.sigma = sigma;  // the kernels form alpha1 sigma / 3 and alpha0 sigma / 2
             o.tau = tau;
             o.mu = mu;
             pr.ops.push_back(o);
-            roles ^= vfs::RB_U | vfs::RB_UB | vfs::RB_V | vfs::RB_PQ;
+            roles = vfs::rswap_iter(roles);
             sigma = sigma / mu;
             tau = tau * mu;
         }
         push_op(pr, vfs::OP_END, l, masked, roles);
-        if (masked) roles ^= vfs::RB_U;
+        if (masked) roles = vfs::rswap(roles, vfs::G_U);
     }
 }
 
@@ -333,7 +332,7 @@ Program build_program(const Params &P, int masked, int warm_level, int warps, in
                       const Tuning &tn)
 {
     Program pr;
-    int roles = 0;
+    int roles = vfs::kRolesInit;
     if (warm_level < 0) {
         push_op(pr, vfs::OP_ZERO, 0, masked, 0);
         push_op(pr, vfs::OP_CALIB, 0, masked, 0);
@@ -343,7 +342,7 @@ Program build_program(const Params &P, int masked, int warm_level, int warps, in
             push_level(pr, P, masked, l, P.nWarpIters, P.nSolverIters, roles);
             if (l > 0) {
                 push_op(pr, vfs::OP_UP, l, masked, roles);
-                roles ^= vfs::RB_U | vfs::RB_UB | vfs::RB_UV;
+                roles = vfs::rswap(vfs::rswap(roles, vfs::G_U), vfs::G_UB) ^ vfs::RB_UV;
             }
         }
     } else {
@@ -352,13 +351,13 @@ Program build_program(const Params &P, int masked, int warm_level, int warps, in
         for (int l = 1; l <= warm_level; l++) push_op(pr, vfs::OP_DOWN, l, masked, roles);
         for (int l = 1; l <= warm_level; l++) {
             push_op(pr, vfs::OP_WINIT, l, masked, roles);
-            roles ^= vfs::RB_U | vfs::RB_UB | vfs::RB_UV;
+            roles = vfs::rswap(vfs::rswap(roles, vfs::G_U), vfs::G_UB) ^ vfs::RB_UV;
         }
         for (int l = warm_level; l >= 0; l--) {
             push_level(pr, P, masked, l, warps, iters, roles);
             if (l > 0) {
                 push_op(pr, vfs::OP_UP, l, masked, roles);
-                roles ^= vfs::RB_U | vfs::RB_UB | vfs::RB_UV;
+                roles = vfs::rswap(vfs::rswap(roles, vfs::G_U), vfs::G_UB) ^ vfs::RB_UV;
             }
         }
     }
@@ -398,11 +397,11 @@ void set_entry(vfs::TickEntry &e, const ProgOp &o, int pair, const vfs::PairDesc
     e.level = o.level;
     e.masked = o.masked;
     e.flags = o.flags;
-    e.roles = (unsigned char)o.roles;
-    e.k1 = o.k1;
-    e.k0 = o.k0;
+    e.roles = (unsigned int)o.roles;
+    e.sigma = o.sigma;
     e.tau = o.tau;
     e.mu = o.mu;
+    e.mu_next = o.mu_next;
 }
 
 // Issues every op of `pr` for every pair of the launch set: one k_tick launch per op per
@@ -584,10 +583,10 @@ hipError_t time_launch_choices(hipStream_t st, const vfs::PairDesc *d_desc, cons
                     o.masked = (unsigned char)masked;
                     o.flags = (unsigned char)(pre ? vfs::EF_PRELOAD_TP : 0);
                     o.shape = (unsigned char)shape;
-                    o.k1 = P.alpha1 / 3.0f;
-                    o.k0 = P.alpha0 / 2.0f;
+                    o.sigma = 1.0f;
                     o.tau = 1.0f;
                     o.mu = 0.8f;
+                    o.mu_next = 0.8f;
                     vfs::TickArgs a = tick_header(P);
                     int acc = 0;
                     for (int k = 0; k < n; k++) {
@@ -882,7 +881,7 @@ int vfs_initialize(vfs_solver *s, int width, int height, float beta, float gamma
     // work arena: PL_COUNT slots of >= N floats, each 256 B aligned
     s->plane = (long long)((N + 63) & ~(size_t)63);
     if ((rc = dev_alloc(s, (void **)&s->arena, sizeof(float) * (size_t)vfs::PL_COUNT * s->plane))) return rc;
-    s->roles = 0;
+    s->roles = vfs::kRolesInit;
     s->solved = false;
     s->desc_dirty = true;
     HIPCHK(s, hipStreamSynchronize(s->stream));
@@ -1003,7 +1002,7 @@ int vfs_copy_disparity_1d(vfs_solver *s, float *out, int pitch)
     if (!out || pitch < s->width) return fail(s, VFS_ERR_ARG, "bad output pointer or pitch");
     int rc = set_device(s);
     if (rc) return rc;
-    return copy_out(s, slot_ptr(s, (s->roles & vfs::RB_U) ? vfs::PL_U1 : vfs::PL_U0), 1, 0, out, pitch);
+    return copy_out(s, slot_ptr(s, vfs::slot_u(vfs::rcur(s->roles, vfs::G_U))), 1, 0, out, pitch);
 }
 
 int vfs_copy_flow(vfs_solver *s, float *out, int pitch)
@@ -1013,7 +1012,7 @@ int vfs_copy_flow(vfs_solver *s, float *out, int pitch)
     if (!out || pitch < 2 * s->width) return fail(s, VFS_ERR_ARG, "bad output pointer or pitch");
     int rc = set_device(s);
     if (rc) return rc;
-    return copy_out(s, slot_ptr(s, (s->roles & vfs::RB_UV) ? vfs::PL_UV1 : vfs::PL_UV0), 2, 0, out, pitch);
+    return copy_out(s, slot_ptr(s, vfs::slot_uv(s->roles, true)), 2, 0, out, pitch);
 }
 
 int vfs_copy_warped_image(vfs_solver *s, float *out, int pitch)
@@ -1037,7 +1036,7 @@ int vfs_copy_stereo(vfs_solver *s, float *depth, int depth_pitch, float *X, int 
     if (rc) return rc;
     if ((rc = wait_done(s))) return rc;
     const Dims d0 = s->lv[0];
-    const float2 *uv = (const float2 *)slot_ptr(s, (s->roles & vfs::RB_UV) ? vfs::PL_UV1 : vfs::PL_UV0);
+    const float2 *uv = (const float2 *)slot_ptr(s, vfs::slot_uv(s->roles, true));
     vfs::launch_convert_kb(uv, focalx, focaly, cx, cy, d1, d2, d3, d4, t1, t2, t3, s->outX, s->outDepth, d0,
                            s->stream);
     if ((rc = check_launch(s))) return rc;
@@ -1055,7 +1054,7 @@ int vfs_copy_disparity_vis(vfs_solver *s, float *out, int pitch, float flowScale
     int rc = set_device(s);
     if (rc) return rc;
     if ((rc = wait_done(s))) return rc;
-    const float2 *uv = (const float2 *)slot_ptr(s, (s->roles & vfs::RB_UV) ? vfs::PL_UV1 : vfs::PL_UV0);
+    const float2 *uv = (const float2 *)slot_ptr(s, vfs::slot_uv(s->roles, true));
     vfs::launch_flow_hsv(uv, s->lv[0], s->outX, flowScale, s->stream);  // outX: 3-float staging plane
     if ((rc = check_launch(s))) return rc;
     return copy_out(s, s->outX, 3, 0, out, pitch);
@@ -1082,8 +1081,8 @@ int vfs_read_plane(vfs_solver *s, const char *name, int level, float *out)
     int nch = 1;
     bool typed = false;  // stored in the storage type
     using namespace vfs;
-    if (n == "u") src = slot_ptr(s, (r & RB_U) ? PL_U1 : PL_U0);
-    else if (n == "u_") src = slot_ptr(s, (r & RB_UB) ? PL_UB1 : PL_UB0);
+    if (n == "u") src = slot_ptr(s, slot_u(rcur(r, G_U)));
+    else if (n == "u_") src = slot_ptr(s, slot_ub(rcur(r, G_UB)));
     else if (n == "u_last") src = slot_ptr(s, PL_ULAST);
     else if (n == "a") { src = slot_ptr(s, PL_A); typed = true; }
     else if (n == "b") { src = slot_ptr(s, PL_B); typed = true; }
@@ -1094,13 +1093,13 @@ int vfs_read_plane(vfs_solver *s, const char *name, int level, float *out)
     else if (n == "I0") { src = s->desc().I0[level]; typed = true; }
     else if (n == "I1") { src = s->I1p[level]; typed = true; }
     else if (n == "mask") src = s->mask[level];
-    else if (n == "warpUV") { src = slot_ptr(s, (r & RB_UV) ? PL_UV1 : PL_UV0); nch = 2; }
+    else if (n == "warpUV") { src = slot_ptr(s, slot_uv(r, true)); nch = 2; }
     else if (n == "tv2") { src = slot_ptr(s, PL_TV2); nch = 2; }
     else if (n == "tv") { src = s->tv[level]; nch = 2; }
-    else if (n == "p") { src = slot_ptr(s, (r & RB_PQ) ? PL_P1 : PL_P0); nch = 2; typed = true; }
-    else if (n == "v_") { src = slot_ptr(s, (r & RB_V) ? PL_V1 : PL_V0); nch = 2; }
+    else if (n == "p") { src = slot_ptr(s, slot_p(rcur(r, G_PQ))); nch = 2; typed = true; }
+    else if (n == "v_") { src = slot_ptr(s, slot_v(rcur(r, G_V))); nch = 2; }
     else if (n == "Tp") { src = slot_ptr(s, PL_TP); nch = 2; }
-    else if (n == "q") { src = slot_ptr(s, (r & RB_PQ) ? PL_Q1 : PL_Q0); nch = 4; typed = true; }
+    else if (n == "q") { src = slot_ptr(s, slot_q(rcur(r, G_PQ))); nch = 4; typed = true; }
     if (!src) return fail(s, VFS_ERR_ARG, "unknown plane '%s'", name);
     int rc = set_device(s);
     if (rc) return rc;
@@ -1203,7 +1202,7 @@ int vfs_time_inner_iterations(vfs_solver *s, int level, int reps, float *ms_tota
     if ((rc = wait_done(s))) return rc;
     const Params P = s->params();
     Program pr;
-    int roles = 0;
+    int roles = vfs::kRolesInit;
     // `reps` iterations of the 50-iteration schedule, restarted every 50
     {
         Params Q = P;
@@ -1757,7 +1756,7 @@ int launch_one_tick(vfs_server *v)
         // program done: result (u at level 0) to the pinned buffer, then the event
         SrvStream &st = v->streams[r->stream];
         vfs_solver *s = st.solver;
-        const float *u = s->arena + (long long)((s->roles & vfs::RB_U) ? vfs::PL_U1 : vfs::PL_U0) * s->plane;
+        const float *u = s->arena + (long long)vfs::slot_u(vfs::rcur(s->roles, vfs::G_U)) * s->plane;
         HIPCHKS(v, hipMemcpy2DAsync(r->pinned, sizeof(float) * st.width, u, sizeof(float) * s->lv[0].s,
                                     sizeof(float) * st.width, st.height, hipMemcpyDeviceToHost, v->stream));
         HIPCHKS(v, hipEventCreateWithFlags(&r->done, hipEventDisableTiming));

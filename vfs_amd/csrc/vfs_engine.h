@@ -30,22 +30,61 @@ constexpr ShapeDim kShape[kShapes] = {{64, 8}, {64, 4}, {128, 4}, {32, 16}, {32,
 enum Storage : int { kF32 = 0, kF16 = 1, kBF16 = 2 };
 
 // Work planes of one pair: slot k starts at arena + k * plane (floats).  2- and 4-channel
-// planes take 2 / 4 consecutive slots.  Planes marked (st) hold the storage type.
+// planes take 2 / 4 consecutive slots.  Planes marked (st) hold the storage type.  The
+// state planes u, u_, v_ and the duals (p, q) form ping-pong groups of three buffers each:
+// the current state, the other state (written by the next iteration) and a spare that
+// the two-iteration op writes; warpUV has two buffers.
 enum Plane : int {
-    PL_U0 = 0, PL_U1 = 1,    // u / us ping-pong
-    PL_UB0 = 2, PL_UB1 = 3,  // u_ / u_s ping-pong
-    PL_ULAST = 4,
-    PL_I1W = 5,              // warped I1 (st)
-    PL_IU = 6, PL_IZ = 7,    // derivatives (st)
-    PL_A = 8, PL_B = 9, PL_C = 10,  // tensor (st)
-    PL_V0 = 11, PL_V1 = 13,  // v_ / v_s ping-pong (float2)
-    PL_P0 = 15, PL_P1 = 17,  // p double buffer (2 x st)
-    PL_TP = 19,              // Tp (float2)
-    PL_TV2 = 21,             // tv2 (float2)
-    PL_UV0 = 23, PL_UV1 = 25,  // warpUV ping-pong (float2)
-    PL_Q0 = 27, PL_Q1 = 31,  // q double buffer (4 x st)
-    PL_COUNT = 35
+    PL_U = 0,                // u: 3 buffers
+    PL_UB = 3,               // u_: 3 buffers
+    PL_ULAST = 6,
+    PL_I1W = 7,              // warped I1 (st)
+    PL_IU = 8, PL_IZ = 9,    // derivatives (st)
+    PL_A = 10, PL_B = 11, PL_C = 12,  // tensor (st)
+    PL_V = 13,               // v_: 3 buffers (float2)
+    PL_P = 19,               // p: 3 buffers (2 x st)
+    PL_TP = 25,              // Tp (float2)
+    PL_TV2 = 27,             // tv2 (float2)
+    PL_UV0 = 29, PL_UV1 = 31,  // warpUV ping-pong (float2)
+    PL_Q = 33,               // q: 3 buffers (4 x st)
+    PL_COUNT = 45
 };
+
+// Ping-pong groups and their roles.  `roles` holds, for group g, the buffer index (0..2)
+// of the current state in bits 4g..4g+1 and of the other state in bits 4g+2..4g+3; the
+// remaining index is the spare.  Bit 16 selects the current warpUV buffer.
+enum Group : int { G_U = 0, G_UB = 1, G_V = 2, G_PQ = 3 };
+constexpr int RB_UV = 1 << 16;
+constexpr int kRolesInit = 0x4444;  // every group: current 0, other 1, spare 2
+
+__host__ __device__ inline int rcur(int roles, int g) { return (roles >> (4 * g)) & 3; }
+__host__ __device__ inline int roth(int roles, int g) { return (roles >> (4 * g + 2)) & 3; }
+__host__ __device__ inline int rspare(int roles, int g) { return 3 - rcur(roles, g) - roth(roles, g); }
+__host__ __device__ inline int rset(int roles, int g, int cur, int oth)
+{
+    return (roles & ~(15 << (4 * g))) | ((cur | (oth << 2)) << (4 * g));
+}
+// One iteration: the other state becomes current.
+__host__ __device__ inline int rswap(int roles, int g) { return rset(roles, g, roth(roles, g), rcur(roles, g)); }
+// Two iterations in one op: the spare (written last) becomes current, the other state
+// stays, the old current becomes the spare.
+__host__ __device__ inline int rrotate(int roles, int g) { return rset(roles, g, rspare(roles, g), roth(roles, g)); }
+// The four groups an inner iteration advances.
+__host__ __device__ inline int rswap_iter(int roles)
+{
+    return rswap(rswap(rswap(rswap(roles, G_U), G_UB), G_V), G_PQ);
+}
+__host__ __device__ inline int rrotate_iter(int roles)
+{
+    return rrotate(rrotate(rrotate(rrotate(roles, G_U), G_UB), G_V), G_PQ);
+}
+// Slot of buffer k of the u, u_, v_, p and q planes.
+__host__ __device__ inline int slot_u(int k) { return PL_U + k; }
+__host__ __device__ inline int slot_ub(int k) { return PL_UB + k; }
+__host__ __device__ inline int slot_v(int k) { return PL_V + 2 * k; }
+__host__ __device__ inline int slot_p(int k) { return PL_P + 2 * k; }
+__host__ __device__ inline int slot_q(int k) { return PL_Q + 4 * k; }
+__host__ __device__ inline int slot_uv(int roles, bool current) { return ((roles & RB_UV) != 0) == current ? PL_UV1 : PL_UV0; }
 
 struct PairDesc {
     float *arena;
@@ -60,10 +99,7 @@ struct PairDesc {
     const float2 *tv[kMaxLevels];
 };
 
-// Which member of each ping-pong pair is current (bit set = slot 1).
-enum RoleBit : int { RB_U = 1, RB_UB = 2, RB_V = 4, RB_PQ = 8, RB_UV = 16 };
-
-// Scalars of one inner iteration (host-computed schedule, stereotgv.cu:449-477).
+// Scalars of one inner iteration (schedule of stereotgv.cu:449-477).
 struct IterParams {
     float k1;      // alpha1 * sigma / eta_p
     float k0;      // alpha0 * sigma / eta_q
@@ -90,7 +126,8 @@ enum OpKind : int {
     OP_DERIV = 7,
     OP_ITER = 8,   // one fused TGV iteration
     OP_END = 9,    // [median +] limit + update at the end of a warp
-    OP_UP = 10     // upscale u, u_, warpUV from `level` to level-1
+    OP_UP = 10,    // upscale u, u_, warpUV from `level` to level-1
+    OP_ITER2 = 11  // two fused TGV iterations (recomputed 4-pixel halo)
 };
 
 // Entries per launch: one per lane of a wave64 (the kernel's entry lookup), and 64
@@ -99,9 +136,10 @@ constexpr int kMaxEntries = 64;
 
 // Flags of a TickEntry.
 enum EntryFlag : int {
-    EF_FIRST = 1,       // ITER: p and q were just zeroed
-    EF_WRITE_TP = 2,    // ITER: also store Tp
-    EF_PRELOAD_TP = 4   // ITER: fetch the Tp-plane neighbours with the other operands
+    EF_FIRST = 1,       // ITER / ITER2: p and q were just zeroed
+    EF_WRITE_TP = 2,    // ITER / ITER2: the (first) iteration also stores Tp
+    EF_PRELOAD_TP = 4,  // ITER: fetch the Tp-plane neighbours with the other operands
+    EF_WRITE_TP2 = 8    // ITER2: the second iteration stores Tp
 };
 
 // One (pair, op) of a launch.  The geometry and arena of the op's tile level travel in
@@ -114,8 +152,9 @@ struct TickEntry {
     unsigned short pair;       // index into the descriptor array
     unsigned short w, h, s;    // tile level dims and row stride (OP_ZERO: level-0 stride x height)
     unsigned char kind, level, masked, flags;
-    unsigned char roles, pad0, pad1, pad2;  // roles: RoleBit set before the op
-    float k1, k0, tau, mu;     // ITER scalars
+    unsigned int roles;        // buffer roles before the op (see Group)
+    float sigma, tau, mu;      // ITER / ITER2: step sizes of the (first) iteration
+    float mu_next;             // ITER2: mu of the second iteration
 };
 static_assert(sizeof(TickEntry) == 56, "TickEntry layout");
 
