@@ -95,7 +95,7 @@ def main():
         g.time_copy(256 << 20, 3)
         ms, nl = g.time_inner_iterations(0, args.iter_only)
         print(f"iter-only: {args.iter_only} level-0 iterations, {ms / args.iter_only * 1e3:.2f} us each "
-              f"({nl} launches per iteration)", flush=True)
+              f"({nl} iteration(s) per launch)", flush=True)
         return
 
     for _ in range(args.warmup):
@@ -127,7 +127,7 @@ def main():
 
     # roofline of the dominant work: the inner TGV iteration at level 0
     w0, h0, _ = g.level_dims(0)
-    ms_it, launches = g.time_inner_iterations(0, args.iter_reps)
+    ms_it, per_launch = g.time_inner_iterations(0, args.iter_reps)
     t_iter = ms_it / args.iter_reps * 1e-3
     bytes_iter = B_ITER_FP32 * w0 * h0
     achieved = bytes_iter / t_iter / 1e9
@@ -158,7 +158,7 @@ def main():
                        "mean": round(float(lat.mean()), 4)},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": profiled_traffic(),
-                     "kernel": f"inner TGV iteration at level 0 ({w0}x{h0}), {launches} launch(es)",
+                     "kernel": f"inner TGV iteration at level 0 ({w0}x{h0}), {per_launch} iteration(s) per launch",
                      "us_per_iter": round(t_iter * 1e6, 3), "bytes_per_iter": int(bytes_iter),
                      "copy_kernel_GBs": round(copy_gbs, 1)},
     }
@@ -166,9 +166,10 @@ def main():
     # per-level launch choices of the tile-shape tuner (made at the first solve)
     out["tile_tuning"] = {}
     for lvl in range(wl["levels"]):
-        (tw, th), pre = g.level_tuning(lvl, True)
+        (tw, th), pre, pair = g.level_tuning(lvl, True, with_pair=True)
         lw, lh, _ = g.level_dims(lvl)
-        out["tile_tuning"][f"L{lvl} {lw}x{lh}"] = f"{tw}x{th}" + (" +Tp prefetch" if pre else "")
+        out["tile_tuning"][f"L{lvl} {lw}x{lh}"] = (f"{tw}x{th}" + (" +Tp prefetch" if pre else "")
+                                                   + (", 2 iterations per launch" if pair else ""))
     print(f"bench: tile tuning {out['tile_tuning']}", file=sys.stderr, flush=True)
 
     if not args.quick:
@@ -273,8 +274,11 @@ def measure_batched(args, P, S, vfs_amd):
     print("bench: batched 64 mixed pairs ready", file=sys.stderr, flush=True)
     b = vfs_amd.Batch(solvers)
     res = _time_batch(args, b, 64)
-    res["tile_tuning"] = {f"L{l}": "%dx%d" % b.level_tuning(l)[0] + (" +Tp prefetch" if b.level_tuning(l)[1] else "")
-                          for l in range(5)}
+    res["tile_tuning"] = {}
+    for l in range(5):
+        (tw, th), pre, pair = b.level_tuning(l, with_pair=True)
+        res["tile_tuning"][f"L{l}"] = (f"{tw}x{th}" + (" +Tp prefetch" if pre else "")
+                                       + (", 2 iterations per launch" if pair else ""))
     mpx = sum(p["i0"].size for p in pairs) / 1e6
     res["Mpx_per_s"] = round(res["pairs_per_s"] * mpx / 64, 2)
     res["workload"] = "64 pairs, sizes 424x400 / 640x480 / 848x800 in turn, L5 W10 I50, FP32, one batch"

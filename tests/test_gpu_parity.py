@@ -186,6 +186,53 @@ def test_tile_shapes_give_identical_results(masked):
                 assert (tw, th) == StereoTgv.SHAPES[shape]
 
 
+@pytest.mark.parametrize("masked", [True, False])
+@pytest.mark.parametrize("shape", [4, 1, 6])
+def test_two_iterations_per_launch_give_identical_results(masked, shape):
+    """Two inner iterations per launch (recomputed halo, spare buffers, rotated roles)
+    reproduce one iteration per launch bit for bit, including odd iteration counts and
+    the buffers that later ops read outside the mask."""
+    pair = small_pair(200, 120, seed=37)
+    ref = None
+    for fuse in (0, 1):
+        g = make_gpu(pair, 3, 2, 9)
+        g.set_option("shape", shape)
+        g.set_option("pair", fuse)
+        if masked:
+            g.solveStereoForwardMasked()
+        else:
+            g.solveStereoForward()
+        planes = {n: g.read_plane(n, 0) for n in ("u", "u_", "v_", "warpUV")}
+        planes["flow"] = g.copyDisparityToHost()
+        for lvl in range(3):
+            (tw, th), pre, paired = g.level_tuning(lvl, masked, with_pair=True)
+            assert paired == bool(fuse)
+        if ref is None:
+            ref = planes
+            continue
+        for n, v in planes.items():
+            assert np.array_equal(v, ref[n], equal_nan=True), (n, float(np.nanmax(np.abs(v - ref[n]))))
+
+
+def test_two_iterations_skip_levels_with_mask_in_column_0():
+    """A mask reaching column 0 keeps one iteration per launch at those levels."""
+    pair = small_pair(160, 120, seed=39)
+    pair["mask"][:, :8] = 1.0
+    g = make_gpu(pair, 3, 2, 8)
+    g.set_option("pair", 1)
+    g.solveStereoForwardMasked()
+    assert not any(g.level_tuning(l, True, with_pair=True)[2] for l in range(3))
+    u1 = g.copy1DDisparityToHost()
+    g2 = make_gpu(pair, 3, 2, 8)
+    g2.set_option("pair", 0)
+    g2.solveStereoForwardMasked()
+    assert np.array_equal(u1, g2.copy1DDisparityToHost())
+    # the unmasked form has no such restriction
+    g.set_option("pair", 1)
+    g.solveStereoForward()
+    assert all(g.level_tuning(l, False, with_pair=True)[2] for l in range(3))
+
+
 def test_resolve_without_new_images_recalibrates():
     """A second solve without copyImagesToDevice warps the calibrated I1 again
     (the pI1[0] / d_i1calibrated swap of stereotgv.cu:392-393)."""
@@ -322,6 +369,14 @@ def test_batch_equals_alone_mixed_sizes():
     b.solve(True)
     for k, s in enumerate(solvers):
         assert np.array_equal(s.copy1DDisparityToHost(), alone[k]), k
+    # a batch running two iterations per launch gives the same results
+    solvers2 = [make_gpu(p, 3, 3, 10) for p in pairs]
+    solvers2[0].set_option("pair", 1)
+    b2 = Batch(solvers2)
+    b2.solve(True)
+    assert all(b2.level_tuning(l, with_pair=True)[2] for l in range(3))
+    for k, s in enumerate(solvers2):
+        assert np.array_equal(s.copy1DDisparityToHost(), alone[k]), ("pair", k)
     # a second batch solve on freshly copied images is identical again
     for s, p in zip(solvers, pairs):
         s.copyImagesToDevice(p["i0"], p["i1"])

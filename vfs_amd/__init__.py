@@ -53,13 +53,15 @@ EXPORTS = [
     ("vfs_time_inner_iterations", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, _c_float_p, _c_int_p]),
     ("vfs_time_copy", ctypes.c_int, [ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int, _c_float_p]),
     ("vfs_get_option", ctypes.c_int, [ctypes.c_void_p, ctypes.c_char_p, _c_int_p]),
-    ("vfs_get_level_tuning", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, _c_int_p, _c_int_p]),
+    ("vfs_get_level_tuning", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, _c_int_p, _c_int_p,
+                                            _c_int_p]),
     ("vfs_batch_create", ctypes.c_int, [ctypes.POINTER(ctypes.c_void_p), ctypes.POINTER(ctypes.c_void_p), ctypes.c_int]),
     ("vfs_batch_destroy", ctypes.c_int, [ctypes.c_void_p]),
     ("vfs_batch_solve", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
     ("vfs_batch_solve_warm", ctypes.c_int, [ctypes.c_void_p] + [ctypes.c_int] * 4),
     ("vfs_batch_synchronize", ctypes.c_int, [ctypes.c_void_p]),
-    ("vfs_batch_get_level_tuning", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, _c_int_p, _c_int_p]),
+    ("vfs_batch_get_level_tuning", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, _c_int_p, _c_int_p,
+                                                  _c_int_p]),
     ("vfs_batch_time_solves", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, _c_float_p]),
     ("vfs_batch_last_error", ctypes.c_char_p, [ctypes.c_void_p]),
     ("vfs_server_create", ctypes.c_int, [ctypes.POINTER(ctypes.c_void_p), ctypes.c_void_p]),
@@ -284,11 +286,14 @@ class StereoTgv:
     # tile shapes of the "shape" option, by index
     SHAPES = ((64, 8), (64, 4), (128, 4), (32, 16), (32, 8), (16, 32), (16, 16))
 
-    def level_tuning(self, level: int, masked: bool = True):
-        """(tile shape (w, h), Tp prefetch) used at `level`; runs the tuner if needed."""
-        sh, pre = ctypes.c_int(), ctypes.c_int()
-        self._check(self._L.vfs_get_level_tuning(self._h, int(masked), level, ctypes.byref(sh), ctypes.byref(pre)),
-                    "get_level_tuning")
+    def level_tuning(self, level: int, masked: bool = True, with_pair: bool = False):
+        """(tile shape (w, h), Tp prefetch[, two iterations per launch]) used at `level`;
+        runs the tuner if needed."""
+        sh, pre, pair = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+        self._check(self._L.vfs_get_level_tuning(self._h, int(masked), level, ctypes.byref(sh), ctypes.byref(pre),
+                                                 ctypes.byref(pair)), "get_level_tuning")
+        if with_pair:
+            return self.SHAPES[sh.value], bool(pre.value), bool(pair.value)
         return self.SHAPES[sh.value], bool(pre.value)
 
     def time_solves(self, masked: bool = True, n: int = 1) -> float:
@@ -342,11 +347,14 @@ class Batch:
     def synchronize(self):
         return self._check(self._L.vfs_batch_synchronize(self._h), "batch_synchronize")
 
-    def level_tuning(self, level: int, masked: bool = True):
-        """(tile shape (w, h), Tp prefetch) of the batch's launches at `level` (after a solve)."""
-        sh, pre = ctypes.c_int(), ctypes.c_int()
+    def level_tuning(self, level: int, masked: bool = True, with_pair: bool = False):
+        """(tile shape (w, h), Tp prefetch[, two iterations per launch]) of the batch's
+        launches at `level` (after a solve)."""
+        sh, pre, pair = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
         self._check(self._L.vfs_batch_get_level_tuning(self._h, int(masked), level, ctypes.byref(sh),
-                                                        ctypes.byref(pre)), "batch_get_level_tuning")
+                                                        ctypes.byref(pre), ctypes.byref(pair)), "batch_get_level_tuning")
+        if with_pair:
+            return StereoTgv.SHAPES[sh.value], bool(pre.value), bool(pair.value)
         return StereoTgv.SHAPES[sh.value], bool(pre.value)
 
     def time_solves(self, masked: bool = True, n: int = 1) -> float:

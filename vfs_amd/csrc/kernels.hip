@@ -792,6 +792,296 @@ __device__ __forceinline__ void op_iter(const Tile &t, int roles, const IterPara
     }
 }
 
+// ------------------------------------------------------------------ two iterations per op
+//
+// Temporal blocking of the inner iteration (OP_ITER2): one workgroup advances its tile by
+// two iterations.  It stages the current state over the tile and a border (4 pixels to
+// the left and top, 2 to the right and bottom) in LDS, computes the first iteration's
+// duals and primal step over the shrinking regions the second iteration reads, then the
+// second iteration on the tile.  Every pixel goes through the same dual_step /
+// primal_step code as in op_iter, so the result equals two single iterations bit for
+// bit.  Buffers: the first iteration's state goes to the `other` buffers (as a single
+// iteration's would), the second's to the spare buffers together with, outside the mask,
+// the current buffers' values (never written by an iteration); the host then rotates the
+// roles (spare -> current).  Other workgroups of the launch read the current buffers and,
+// outside the mask only, the other buffers, so no write of this op is read by it.
+// The host uses this op only where no mask pixel lies in column 0: there the reference's
+// flat-index left neighbour (pos - 1) is the previous row's last pixel.
+// Work-items of a two-iteration workgroup: its staged regions hold 1.5 - 2 times the
+// tile's pixels, so it takes four times the work-items of a tile.
+constexpr int NT2 = 1024;
+
+template <class S>
+struct Blk2 {
+    static constexpr int TW = S::TW, TH = S::TH;
+    // origin offset (both axes) and size of each region, relative to the tile
+    static constexpr int RO = -4, RW = TW + 6, RH = TH + 6;  // state loads (u_, v_, mask)
+    static constexpr int DO = -3, DW = TW + 4, DH = TH + 4;  // first iteration's duals
+    static constexpr int PO = -2, PW = TW + 3, PH = TH + 3;  // first iteration's primal step
+    static constexpr int EO = -1, EW = TW + 1, EH = TH + 1;  // second iteration's duals
+    static constexpr int NR = RW * RH, ND = DW * DH, NP = PW * PH, NE = EW * EH;
+};
+
+template <class S>
+struct Lds2 {
+    using B = Blk2<S>;
+    float ua[B::NR];      // u_ (current buffers)
+    float2 va[B::NR];     // v_
+    float mk[B::NR];      // mask
+    float2 p[B::ND];      // p: loaded, then each iteration's new duals in place
+    float4 q[B::ND];
+    float2 tp[B::ND];     // new Tp
+    float2 tpl[B::ND];    // Tp plane
+    float ta[B::ND], tb[B::ND], tc[B::ND];  // tensor
+    float uc[B::NP];      // u (current buffer)
+    float iu[B::NP], iz[B::NP];
+    float ub[B::NP];      // u_ (other buffer), read outside the mask by the second iteration
+    float2 vb[B::NP];     // v_ (other buffer)
+    float u1[B::NP], ub1[B::NP];  // u, u_ after the first iteration
+    float2 v1[B::NP];     // v_ after the first iteration
+};
+
+template <class S, bool MASKED, typename T>
+__device__ __forceinline__ void op_iter2(const Tile &t, int roles, const IterParams &ip0, const IterParams &ip1,
+                                         Lds2<S> &L)
+{
+    using B = Blk2<S>;
+    const int w = t.w, h = t.h, s = t.s, x0 = t.x0, y0 = t.y0;
+    const int tid = threadIdx.y * S::BX + threadIdx.x;  // NT2 work-items
+    const float *__restrict__ mask = t.mask;
+    const float *__restrict__ u_cur = slot(t, slot_u(rcur(roles, G_U)));
+    float *__restrict__ u_oth = slot(t, slot_u(roth(roles, G_U)));
+    float *__restrict__ u_spr = slot(t, slot_u(rspare(roles, G_U)));
+    const float *__restrict__ ub_cur = slot(t, slot_ub(rcur(roles, G_UB)));
+    float *__restrict__ ub_oth = slot(t, slot_ub(roth(roles, G_UB)));
+    float *__restrict__ ub_spr = slot(t, slot_ub(rspare(roles, G_UB)));
+    const float2 *__restrict__ v_cur = slotp<float2>(t, slot_v(rcur(roles, G_V)));
+    float2 *__restrict__ v_oth = slotp<float2>(t, slot_v(roth(roles, G_V)));
+    float2 *__restrict__ v_spr = slotp<float2>(t, slot_v(rspare(roles, G_V)));
+    const V2<T> *__restrict__ p_cur = slotp<V2<T>>(t, slot_p(rcur(roles, G_PQ)));
+    V2<T> *__restrict__ p_oth = slotp<V2<T>>(t, slot_p(roth(roles, G_PQ)));
+    V2<T> *__restrict__ p_spr = slotp<V2<T>>(t, slot_p(rspare(roles, G_PQ)));
+    const V4<T> *__restrict__ q_cur = slotp<V4<T>>(t, slot_q(rcur(roles, G_PQ)));
+    V4<T> *__restrict__ q_oth = slotp<V4<T>>(t, slot_q(roth(roles, G_PQ)));
+    V4<T> *__restrict__ q_spr = slotp<V4<T>>(t, slot_q(rspare(roles, G_PQ)));
+    float2 *__restrict__ Tp = slotp<float2>(t, PL_TP);
+    const T *__restrict__ at = (const T *)slot(t, PL_A);
+    const T *__restrict__ bt = (const T *)slot(t, PL_B);
+    const T *__restrict__ ct = (const T *)slot(t, PL_C);
+    const T *__restrict__ Iu = (const T *)slot(t, PL_IU);
+    const T *__restrict__ Iz = (const T *)slot(t, PL_IZ);
+
+    // LDS index of image position (x, y) in each region
+    auto iR = [&](int x, int y) { return (y - y0 - B::RO) * B::RW + (x - x0 - B::RO); };
+    auto iD = [&](int x, int y) { return (y - y0 - B::DO) * B::DW + (x - x0 - B::DO); };
+    auto iP = [&](int x, int y) { return (y - y0 - B::PO) * B::PW + (x - x0 - B::PO); };
+    auto gpos = [&](int x, int y) {
+        const int xc = min(max(x, 0), w - 1), yc = min(max(y, 0), h - 1);
+        return (long)yc * s + xc;
+    };
+    auto in_tile = [&](int x, int y) { return x >= x0 && x < x0 + S::TW && y >= y0 && y < y0 + S::TH; };
+
+    // ---- stage the current state (clamped positions outside the image)
+#pragma unroll
+    for (int k = 0; k < (B::NR + NT2 - 1) / NT2; k++) {
+        const int i = tid + k * NT2;
+        if (i < B::NR) {
+            const long pos = gpos(x0 + B::RO + i % B::RW, y0 + B::RO + i / B::RW);
+            L.ua[i] = ub_cur[pos];
+            L.va[i] = v_cur[pos];
+            L.mk[i] = MASKED ? mask[pos] : 1.f;
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < (B::ND + NT2 - 1) / NT2; k++) {
+        const int i = tid + k * NT2;
+        if (i < B::ND) {
+            const long pos = gpos(x0 + B::DO + i % B::DW, y0 + B::DO + i / B::DW);
+            L.p[i] = ip0.first ? make_float2(0.f, 0.f) : ld2T(p_cur, pos);
+            L.q[i] = ip0.first ? make_float4(0.f, 0.f, 0.f, 0.f) : ld4T(q_cur, pos);
+            L.ta[i] = ldT(at, pos);
+            L.tb[i] = ldT(bt, pos);
+            L.tc[i] = ldT(ct, pos);
+            L.tpl[i] = Tp[pos];
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < (B::NP + NT2 - 1) / NT2; k++) {
+        const int i = tid + k * NT2;
+        if (i < B::NP) {
+            const long pos = gpos(x0 + B::PO + i % B::PW, y0 + B::PO + i / B::PW);
+            L.uc[i] = u_cur[pos];
+            L.iu[i] = ldT(Iu, pos);
+            L.iz[i] = ldT(Iz, pos);
+            L.ub[i] = ub_oth[pos];
+            L.vb[i] = v_oth[pos];
+        }
+    }
+    __syncthreads();
+
+    auto act = [&](int x, int y) { return !MASKED || L.mk[iR(x, y)] != 0.f; };
+    // u_ and v_ of the state entering iteration j (in-image positions): iteration 1 reads
+    // the first iteration's result inside the mask and the other buffers' values outside
+    auto U_ = [&](int j, int x, int y) {
+        return j == 0 ? L.ua[iR(x, y)] : (act(x, y) ? L.ub1[iP(x, y)] : L.ub[iP(x, y)]);
+    };
+    auto V_ = [&](int j, int x, int y) {
+        return j == 0 ? L.va[iR(x, y)] : (act(x, y) ? L.v1[iP(x, y)] : L.vb[iP(x, y)]);
+    };
+
+#pragma unroll
+    for (int j = 0; j < 2; j++) {
+        const IterParams &ip = j == 0 ? ip0 : ip1;
+        // ---- duals over the region the primal step reads (first: D, second: E)
+        const int dO = j == 0 ? B::DO : B::EO, dW = j == 0 ? B::DW : B::EW, dN = j == 0 ? B::ND : B::NE;
+        for (int i = tid; i < dN; i += NT2) {
+            const int x = x0 + dO + i % dW, y = y0 + dO + i / dW;
+            if (x < 0 || y < 0 || x >= w || y >= h || !act(x, y)) continue;
+            const int xr = x + 1 < w ? x + 1 : x, yd = y + 1 < h ? y + 1 : y;
+            DualIn r;
+            r.ok = true;
+            r.act = true;
+            r.hasR = (x + 1 < w) && act(xr, y);
+            r.hasD = (y + 1 < h) && act(x, yd);
+            const int d = iD(x, y);
+            r.a = L.ta[d];
+            r.b = L.tb[d];
+            r.c = L.tc[d];
+            r.u0 = U_(j, x, y);
+            r.uR = U_(j, xr, y);
+            r.uD = U_(j, x, yd);
+            r.uL = x >= 1 ? U_(j, x - 1, y) : 0.f;  // column 0 inside the mask: not used here
+            r.uU = y >= 1 ? U_(j, x, y - 1) : 0.f;
+            r.v0 = V_(j, x, y);
+            r.vR = V_(j, xr, y);
+            r.vD = V_(j, x, yd);
+            r.vL = x >= 1 ? V_(j, x - 1, y) : make_float2(0.f, 0.f);
+            r.vU = y >= 1 ? V_(j, x, y - 1) : make_float2(0.f, 0.f);
+            r.po = L.p[d];
+            r.qo = L.q[d];
+            float2 pn, tpn;
+            float4 qn;
+            dual_step<MASKED>(r, x, y, w, h, ip, pn, qn, tpn);
+            L.p[d] = pn;
+            L.q[d] = qn;
+            L.tp[d] = tpn;
+            if (in_tile(x, y)) {
+                const long pos = (long)y * s + x;
+                st2T(j == 0 ? p_oth : p_spr, pos, pn);
+                st4T(j == 0 ? q_oth : q_spr, pos, qn);
+                if (ip.write_tp) Tp[pos] = tpn;
+            }
+        }
+        __syncthreads();
+
+        // ---- thresholding + primal step (first: region P, second: the tile)
+        const int pO = j == 0 ? B::PO : 0, pW = j == 0 ? B::PW : S::TW, pN = j == 0 ? B::NP : S::TW * S::TH;
+        for (int i = tid; i < pN; i += NT2) {
+            const int x = x0 + pO + i % pW, y = y0 + pO + i / pW;
+            if (x < 0 || y < 0 || x >= w || y >= h) continue;
+            const long pos = (long)y * s + x;
+            const bool mine = in_tile(x, y);
+            float un, u_s, v_sx, v_sy;
+            if (!MASKED && outside_circle(x, y, w, h)) {
+                un = 0.f;
+                u_s = 0.f;
+                v_sx = v_sy = 0.f;
+            } else if (!act(x, y)) {
+                // outside the mask: the spare buffers take the current buffers' values
+                if (j == 1 && mine) {
+                    u_spr[pos] = L.uc[iP(x, y)];
+                    ub_spr[pos] = L.ua[iR(x, y)];
+                    v_spr[pos] = L.va[iR(x, y)];
+                }
+                continue;
+            } else {
+                const int xr = x + 1 < w ? x + 1 : x, yd = y + 1 < h ? y + 1 : y;
+                const bool hR = (x + 1 < w) && act(xr, y), hD = (y + 1 < h) && act(x, yd);
+                const bool hL = x >= 1 && act(x - 1, y), hU = y >= 1 && act(x, y - 1);
+                const int d = iD(x, y), pp = iP(x, y);
+                // a neighbour outside the mask (or the image) is read from the Tp plane
+                const float tpLx = hL ? L.tp[iD(x - 1, y)].x : (x >= 1 ? L.tpl[iD(x - 1, y)].x : 0.f);
+                const float tpUy = hU ? L.tp[iD(x, y - 1)].y : (y >= 1 ? L.tpl[iD(x, y - 1)].y : 0.f);
+                const float divp = div_tp<MASKED>(x, y, w, h, hR, hD, hL, hU, L.tp[d], tpLx, tpUy);
+                const float4 ql = hL ? L.q[iD(x - 1, y)] : make_float4(0.f, 0.f, 0.f, 0.f);
+                const float4 qu = hU ? L.q[iD(x, y - 1)] : make_float4(0.f, 0.f, 0.f, 0.f);
+                const float uu = j == 0 ? L.uc[pp] : L.u1[pp];
+                float2 vs;
+                primal_step(ip, L.ta[d], L.tb[d], L.tc[d], U_(j, x, y), uu, L.iu[pp], L.iz[pp], V_(j, x, y), hR,
+                            hD, hL, hU, divp, L.q[d], ql, qu, L.p[d], un, u_s, vs);
+                v_sx = vs.x;
+                v_sy = vs.y;
+            }
+            if (j == 0) {
+                const int pp = iP(x, y);
+                L.u1[pp] = un;
+                L.ub1[pp] = u_s;
+                L.v1[pp] = make_float2(v_sx, v_sy);
+                if (mine) {
+                    u_oth[pos] = un;
+                    ub_oth[pos] = u_s;
+                    v_oth[pos] = make_float2(v_sx, v_sy);
+                }
+            } else {
+                u_spr[pos] = un;
+                ub_spr[pos] = u_s;
+                v_spr[pos] = make_float2(v_sx, v_sy);
+            }
+        }
+        __syncthreads();
+    }
+}
+
+template <typename T, int SH>
+__global__ __launch_bounds__(NT2) void k_iter2(const PairDesc *__restrict__ D, const TickArgs A)
+{
+    using S = Sh<SH>;
+    __shared__ Lds2<S> L;
+    const int nb = gridDim.x, i = blockIdx.x, xcd = i & 7, q = nb >> 3, r = nb & 7;
+    const int b = (xcd < r ? xcd * (q + 1) : r * (q + 1) + (xcd - r) * q) + (i >> 3);
+    int lo = 0;
+    if (A.n > 1) {
+        const int lane = __lane_id();
+        const int ts = lane < A.n ? A.e[lane].tile_start : 0x7fffffff;
+        lo = __popcll(__ballot(ts <= b)) - 1;
+        lo = __builtin_amdgcn_readfirstlane(lo);
+    }
+    const TickEntry &E = A.e[lo];
+    Tile t;
+    t.d = D + E.pair;
+    t.arena = E.arena;
+    t.plane = E.plane;
+    t.mask = E.mask;
+    t.w = E.w;
+    t.h = E.h;
+    t.s = E.s;
+    const int local = b - E.tile_start;
+    const int ntx = (t.w + S::TW - 1) / S::TW;
+    t.x0 = (local % ntx) * S::TW;
+    t.y0 = (local / ntx) * S::TH;
+    // the second iteration's step sizes follow the schedule (stereotgv.cu:475-476)
+    IterParams ip0, ip1;
+    const float sigma1 = E.sigma / E.mu, tau1 = E.tau * E.mu;
+    ip0.k1 = A.alpha1 * E.sigma / 3.0f;
+    ip0.k0 = A.alpha0 * E.sigma / 2.0f;
+    ip0.tau = E.tau;
+    ip0.mu = E.mu;
+    ip1.k1 = A.alpha1 * sigma1 / 3.0f;
+    ip1.k0 = A.alpha0 * sigma1 / 2.0f;
+    ip1.tau = tau1;
+    ip1.mu = E.mu_next;
+    ip0.lambda = ip1.lambda = A.lambda;
+    ip0.alpha0 = ip1.alpha0 = A.alpha0;
+    ip0.alpha1 = ip1.alpha1 = A.alpha1;
+    ip0.first = E.flags & EF_FIRST;
+    ip1.first = 0;
+    ip0.write_tp = (E.flags & EF_WRITE_TP) != 0;
+    ip1.write_tp = (E.flags & EF_WRITE_TP2) != 0;
+    ip0.preload_tp = ip1.preload_tp = 0;
+    if (E.masked) op_iter2<S, true, T>(t, E.roles, ip0, ip1, L);
+    else op_iter2<S, false, T>(t, E.roles, ip0, ip1, L);
+}
+
 // 13th smallest of 25 by the partial selection sort of medianFilter.cu:307-316.
 __device__ __forceinline__ float median25(float m[25])
 {
@@ -1207,9 +1497,27 @@ void tick_storage(const PairDesc *desc, const TickArgs &args, int shape, hipStre
 }
 static_assert(kShapes == 7, "tick_storage dispatches every shape");
 
+// Two-iteration launches exist for the one-row shapes (64x4, 32x8, 16x16), whose staged
+// regions fit the workgroup's LDS.
+template <typename T>
+void iter2_storage(const PairDesc *desc, const TickArgs &args, int shape, hipStream_t st)
+{
+    const dim3 grid((unsigned)args.total);
+    switch (shape) {
+    case 1: k_iter2<T, 1><<<grid, dim3(Sh<1>::BX, NT2 / Sh<1>::BX), 0, st>>>(desc, args); break;
+    case 4: k_iter2<T, 4><<<grid, dim3(Sh<4>::BX, NT2 / Sh<4>::BX), 0, st>>>(desc, args); break;
+    case 6: k_iter2<T, 6><<<grid, dim3(Sh<6>::BX, NT2 / Sh<6>::BX), 0, st>>>(desc, args); break;
+    default: break;  // the host never asks for another shape
+    }
+}
+
 void launch_tick(const PairDesc *desc, const TickArgs &args, int storage, int shape, hipStream_t st)
 {
     if (args.n < 1 || args.total < 1) return;
+    if (args.e[0].kind == OP_ITER2) {
+        STORAGE_DISPATCH(storage, iter2_storage<T>(desc, args, shape, st));
+        return;
+    }
     STORAGE_DISPATCH(storage, tick_storage<T>(desc, args, shape, st));
 }
 

@@ -84,10 +84,11 @@ struct ProgOp {
     float sigma, tau, mu, mu_next;
 };
 
-// Launch choices for one pyramid level: tile shape of every op at the level and whether
-// the inner iteration fetches its Tp-plane neighbours up front (vfs::EF_PRELOAD_TP).
+// Launch choices for one pyramid level: tile shape of every op at the level, whether the
+// inner iteration fetches its Tp-plane neighbours up front (vfs::EF_PRELOAD_TP) and
+// whether two iterations share one launch (vfs::OP_ITER2).
 struct LevelTune {
-    unsigned char shape = 0, preload = 0;
+    unsigned char shape = 0, preload = 0, pair = 0;
 };
 
 struct Tuning {
@@ -95,10 +96,20 @@ struct Tuning {
     unsigned long long sig() const
     {
         unsigned long long k = 0;
-        for (int l = 0; l < kMaxLevels; l++) k |= (unsigned long long)(lv[l].shape | (lv[l].preload << 3)) << (4 * l);
+        for (int l = 0; l < 12; l++)
+            k |= (unsigned long long)(lv[l].shape | (lv[l].preload << 3) | (lv[l].pair << 4)) << (5 * l);
         return k;
     }
+    Tuning without_pairs() const
+    {
+        Tuning t = *this;
+        for (LevelTune &v : t.lv) v.pair = 0;
+        return t;
+    }
 };
+
+// Tile shapes with a two-iteration kernel (their staged regions fit a workgroup's LDS).
+bool pair_shape(int shape) { return shape == 1 || shape == 4 || shape == 6; }
 
 struct Program {
     std::vector<ProgOp> ops;
@@ -122,6 +133,8 @@ struct vfs_solver {
     bool use_graph = true;
     int force_shape = -1;   // option "shape": -1 = per-level tuner
     bool autotune = true;   // option "autotune"
+    int force_pair = -1;    // option "pair": -1 = tuner, 0 = never, 1 = wherever allowed
+    bool mask_col0[kMaxLevels] = {};  // a mask pixel lies in column 0 of the level
     Tuning tuning[2];       // [masked] launch choices per level
     bool tuned[2] = {false, false};
     Dims lv[kMaxLevels] = {};
@@ -282,7 +295,7 @@ void push_op(Program &pr, int kind, int level, int masked, int roles)
 
 // One level of the solve from the tensor to the end of the last warp
 // (stereotgv.cu:409-494, masked; :275-364 unmasked).
-void push_level(Program &pr, const Params &P, int masked, int l, int warps, int iters, int &roles)
+void push_level(Program &pr, const Params &P, int masked, int l, int warps, int iters, int &roles, bool pairs)
 {
     push_op(pr, vfs::OP_TENSOR, l, masked, roles);
     for (int wi = 0; wi < warps; wi++) {
@@ -290,23 +303,38 @@ void push_level(Program &pr, const Params &P, int masked, int l, int warps, int 
         push_op(pr, vfs::OP_DERIV, l, masked, roles);
         float tau = 1.0f;
         float sigma = 1.0f / tau;
-        for (int it = 0; it < iters; it++) {
-            float mu;
-            if (sigma < 1000.0f) mu = 1.0f / sqrtf(1 + 0.7f * tau * P.timestep_lambda);
-            else mu = 1;
+        auto mu_of = [&](float tau_, float sigma_) {
+            return sigma_ < 1000.0f ? 1.0f / sqrtf(1 + 0.7f * tau_ * P.timestep_lambda) : 1.0f;
+        };
+        for (int it = 0; it < iters;) {
+            const float mu = mu_of(tau, sigma);
             ProgOp o{};
-            o.kind = vfs::OP_ITER;
             o.level = (unsigned char)l;
             o.masked = (unsigned char)masked;
-            o.flags = (unsigned char)((it == 0 ? vfs::EF_FIRST : 0) | (it == iters - 1 ? vfs::EF_WRITE_TP : 0));
             o.roles = roles;
             o.sigma = sigma;  // the kernels form alpha1 sigma / 3 and alpha0 sigma / 2
             o.tau = tau;
             o.mu = mu;
-            pr.ops.push_back(o);
-            roles = vfs::rswap_iter(roles);
-            sigma = sigma / mu;
-            tau = tau * mu;
+            if (pairs && it + 1 < iters) {
+                // iterations it and it + 1 in one op; the kernel continues the schedule
+                const float sigma1 = sigma / mu, tau1 = tau * mu;
+                o.kind = vfs::OP_ITER2;
+                o.mu_next = mu_of(tau1, sigma1);
+                o.flags = (unsigned char)((it == 0 ? vfs::EF_FIRST : 0) | (it + 1 == iters - 1 ? vfs::EF_WRITE_TP2 : 0));
+                pr.ops.push_back(o);
+                roles = vfs::rrotate_iter(roles);
+                sigma = sigma1 / o.mu_next;
+                tau = tau1 * o.mu_next;
+                it += 2;
+            } else {
+                o.kind = vfs::OP_ITER;
+                o.flags = (unsigned char)((it == 0 ? vfs::EF_FIRST : 0) | (it == iters - 1 ? vfs::EF_WRITE_TP : 0));
+                pr.ops.push_back(o);
+                roles = vfs::rswap_iter(roles);
+                sigma = sigma / mu;
+                tau = tau * mu;
+                it += 1;
+            }
         }
         push_op(pr, vfs::OP_END, l, masked, roles);
         if (masked) roles = vfs::rswap(roles, vfs::G_U);
@@ -320,6 +348,8 @@ void apply_tuning(Program &pr, const Tuning &tn)
         o.shape = tn.lv[vfs::op_tile_level(o.kind, o.level)].shape;
         o.flags &= (unsigned char)~vfs::EF_PRELOAD_TP;
         if (o.kind == vfs::OP_ITER && tn.lv[o.level].preload) o.flags |= vfs::EF_PRELOAD_TP;
+        // a level's launches take one shape; two-iteration ops exist for some shapes only
+        if (o.kind == vfs::OP_ITER2 && !pair_shape(o.shape)) o.shape = 4;
     }
 }
 
@@ -339,7 +369,7 @@ Program build_program(const Params &P, int masked, int warm_level, int warps, in
         for (int l = 1; l < P.nLevels; l++) push_op(pr, vfs::OP_DOWN, l, masked, 0);
         for (int l = P.nLevels - 1; l >= 0; l--) {
             if (l == P.nLevels - 1) push_op(pr, vfs::OP_CINIT, l, masked, roles);
-            push_level(pr, P, masked, l, P.nWarpIters, P.nSolverIters, roles);
+            push_level(pr, P, masked, l, P.nWarpIters, P.nSolverIters, roles, tn.lv[l].pair != 0);
             if (l > 0) {
                 push_op(pr, vfs::OP_UP, l, masked, roles);
                 roles = vfs::rswap(vfs::rswap(roles, vfs::G_U), vfs::G_UB) ^ vfs::RB_UV;
@@ -354,7 +384,7 @@ Program build_program(const Params &P, int masked, int warm_level, int warps, in
             roles = vfs::rswap(vfs::rswap(roles, vfs::G_U), vfs::G_UB) ^ vfs::RB_UV;
         }
         for (int l = warm_level; l >= 0; l--) {
-            push_level(pr, P, masked, l, warps, iters, roles);
+            push_level(pr, P, masked, l, warps, iters, roles, tn.lv[l].pair != 0);
             if (l > 0) {
                 push_op(pr, vfs::OP_UP, l, masked, roles);
                 roles = vfs::rswap(vfs::rswap(roles, vfs::G_U), vfs::G_UB) ^ vfs::RB_UV;
@@ -541,11 +571,12 @@ int ensure_engine(vfs_solver *s)
 // same storage, solve form and level geometry.  Results do not depend on the choice: the
 // kernels compute every pixel identically in every shape.
 struct TuneKey {
-    int device, storage, masked, w, h, s;
+    int device, storage, masked, w, h, s, pair_ok;
     bool operator<(const TuneKey &o) const
     {
-        const int a[6] = {device, storage, masked, w, h, s}, b[6] = {o.device, o.storage, o.masked, o.w, o.h, o.s};
-        return std::lexicographical_compare(a, a + 6, b, b + 6);
+        const int a[7] = {device, storage, masked, w, h, s, pair_ok};
+        const int b[7] = {o.device, o.storage, o.masked, o.w, o.h, o.s, o.pair_ok};
+        return std::lexicographical_compare(a, a + 7, b, b + 7);
     }
 };
 
@@ -565,48 +596,54 @@ LevelTune default_tune(int level)
 // on the launch set `descs` (their arenas point at scratch memory; `d_desc` is any valid
 // descriptor array of at least descs.size() entries) and stores the fastest choices in tn.
 hipError_t time_launch_choices(hipStream_t st, const vfs::PairDesc *d_desc, const std::vector<vfs::PairDesc> &descs,
-                               const Params &P, int masked, const std::vector<int> &levels, Tuning &tn)
+                               const Params &P, int masked, const std::vector<int> &levels,
+                               const std::vector<bool> &allow_pair, Tuning &tn)
 {
     constexpr int kWarm = 3, kReps = 20, kRounds = 2;
+    // candidates: (shape, preload) of one iteration per launch, then two-iteration
+    // launches for the shapes that have them
+    constexpr int kSingle = vfs::kShapes * 2, kCand = kSingle + vfs::kShapes;
     hipEvent_t e0 = nullptr, e1 = nullptr;
     hipError_t r = hipEventCreate(&e0);
     if (r == hipSuccess) r = hipEventCreate(&e1);
-    std::vector<float> best(levels.size() * vfs::kShapes * 2, 1e30f);
+    std::vector<float> best(levels.size() * kCand, 1e30f);  // ms per iteration
     const int n = std::min((int)descs.size(), vfs::kMaxEntries);
     for (int round = 0; round < kRounds && r == hipSuccess; round++) {
         for (size_t li = 0; li < levels.size() && r == hipSuccess; li++) {
-            for (int shape = 0; shape < vfs::kShapes && r == hipSuccess; shape++) {
-                for (int pre = 0; pre < 2 && r == hipSuccess; pre++) {
-                    ProgOp o{};
-                    o.kind = vfs::OP_ITER;
-                    o.level = (unsigned char)levels[li];
-                    o.masked = (unsigned char)masked;
-                    o.flags = (unsigned char)(pre ? vfs::EF_PRELOAD_TP : 0);
-                    o.shape = (unsigned char)shape;
-                    o.sigma = 1.0f;
-                    o.tau = 1.0f;
-                    o.mu = 0.8f;
-                    o.mu_next = 0.8f;
-                    vfs::TickArgs a = tick_header(P);
-                    int acc = 0;
-                    for (int k = 0; k < n; k++) {
-                        const vfs::PairDesc &d = descs[k];
-                        set_entry(a.e[k], o, k, d, acc);
-                        acc += vfs::op_tiles(o.kind, o.level, d.w, d.h, d.s, shape);
-                    }
-                    a.n = n;
-                    a.total = acc;
-                    for (int k = 0; k < kWarm; k++) vfs::launch_tick(d_desc, a, P.storage, shape, st);
-                    r = hipEventRecord(e0, st);
-                    for (int k = 0; k < kReps; k++) vfs::launch_tick(d_desc, a, P.storage, shape, st);
-                    if (r == hipSuccess) r = hipEventRecord(e1, st);
-                    if (r == hipSuccess) r = hipEventSynchronize(e1);
-                    float ms = 0;
-                    if (r == hipSuccess) r = hipEventElapsedTime(&ms, e0, e1);
-                    if (r == hipSuccess) r = hipGetLastError();
-                    float &b = best[(li * vfs::kShapes + shape) * 2 + pre];
-                    b = std::min(b, ms);
+            for (int c = 0; c < kCand && r == hipSuccess; c++) {
+                const bool pair = c >= kSingle;
+                const int shape = pair ? c - kSingle : c / 2, pre = pair ? 0 : c % 2;
+                if (pair && (!pair_shape(shape) || !allow_pair[li])) continue;
+                ProgOp o{};
+                o.kind = pair ? vfs::OP_ITER2 : vfs::OP_ITER;
+                o.level = (unsigned char)levels[li];
+                o.masked = (unsigned char)masked;
+                o.flags = (unsigned char)(pre ? vfs::EF_PRELOAD_TP : 0);
+                o.shape = (unsigned char)shape;
+                o.roles = vfs::kRolesInit;
+                o.sigma = 1.0f;
+                o.tau = 1.0f;
+                o.mu = 0.8f;
+                o.mu_next = 0.8f;
+                vfs::TickArgs a = tick_header(P);
+                int acc = 0;
+                for (int k = 0; k < n; k++) {
+                    const vfs::PairDesc &d = descs[k];
+                    set_entry(a.e[k], o, k, d, acc);
+                    acc += vfs::op_tiles(vfs::OP_ITER, o.level, d.w, d.h, d.s, shape);
                 }
+                a.n = n;
+                a.total = acc;
+                for (int k = 0; k < kWarm; k++) vfs::launch_tick(d_desc, a, P.storage, shape, st);
+                r = hipEventRecord(e0, st);
+                for (int k = 0; k < kReps; k++) vfs::launch_tick(d_desc, a, P.storage, shape, st);
+                if (r == hipSuccess) r = hipEventRecord(e1, st);
+                if (r == hipSuccess) r = hipEventSynchronize(e1);
+                float ms = 0;
+                if (r == hipSuccess) r = hipEventElapsedTime(&ms, e0, e1);
+                if (r == hipSuccess) r = hipGetLastError();
+                float &b = best[li * kCand + c];
+                b = std::min(b, pair ? ms / 2 : ms);
             }
         }
     }
@@ -618,19 +655,29 @@ hipError_t time_launch_choices(hipStream_t st, const vfs::PairDesc *d_desc, cons
         const int l = levels[li];
         const LevelTune def = default_tune(l);
         LevelTune pick = def;
-        float pick_ms = best[(li * vfs::kShapes + def.shape) * 2 + def.preload];
-        for (int shape = 0; shape < vfs::kShapes; shape++)
-            for (int pre = 0; pre < 2; pre++) {
-                const float ms = best[(li * vfs::kShapes + shape) * 2 + pre];
-                if (ms < 0.98f * pick_ms) {  // leave the default only for a clear gain
-                    pick.shape = (unsigned char)shape;
-                    pick.preload = (unsigned char)pre;
-                    pick_ms = ms;
-                }
+        float pick_ms = best[li * kCand + def.shape * 2 + def.preload];
+        for (int c = 0; c < kCand; c++) {
+            const float ms = best[li * kCand + c];
+            if (ms < 0.98f * pick_ms) {  // leave the default only for a clear gain
+                const bool pair = c >= kSingle;
+                pick.shape = (unsigned char)(pair ? c - kSingle : c / 2);
+                pick.preload = (unsigned char)(pair ? 0 : c % 2);
+                pick.pair = (unsigned char)pair;
+                pick_ms = ms;
             }
+        }
         tn.lv[l] = pick;
     }
     return hipSuccess;
+}
+
+// Levels where two iterations may share a launch: at least 2 columns and, in the masked
+// form, no mask pixel in column 0.
+std::vector<bool> pair_allowed(const vfs_solver *s, int masked, const std::vector<int> &levels)
+{
+    std::vector<bool> ok;
+    for (int l : levels) ok.push_back(s->lv[l].w >= 2 && !(masked && s->mask_col0[l]));
+    return ok;
 }
 
 int measure_levels(vfs_solver *s, int masked, const std::vector<int> &levels, Tuning &tn)
@@ -641,7 +688,8 @@ int measure_levels(vfs_solver *s, int masked, const std::vector<int> &levels, Tu
     vfs::launch_fill_zero(scratch, bytes, s->stream);
     std::vector<vfs::PairDesc> d{s->desc()};
     d[0].arena = scratch;
-    const hipError_t r = time_launch_choices(s->stream, s->eng.d_desc, d, s->params(), masked, levels, tn);
+    const hipError_t r = time_launch_choices(s->stream, s->eng.d_desc, d, s->params(), masked, levels,
+                                             pair_allowed(s, masked, levels), tn);
     (void)hipStreamSynchronize(s->stream);
     (void)hipFree(scratch);
     if (r != hipSuccess) return fail(s, VFS_ERR_DEVICE, "launch tuner: %s", hipGetErrorString(r));
@@ -659,7 +707,10 @@ int tune_solver(vfs_solver *s, int masked)
         for (int l = 0; l < s->nLevels; l++) tn.lv[l].shape = (unsigned char)s->force_shape;
     } else if (s->autotune) {
         std::vector<int> todo;
-        auto key = [&](int l) { return TuneKey{s->device, s->storage, m, s->lv[l].w, s->lv[l].h, s->lv[l].s}; };
+        auto key = [&](int l) {
+            const int ok = pair_allowed(s, m, std::vector<int>{l})[0] ? 1 : 0;
+            return TuneKey{s->device, s->storage, m, s->lv[l].w, s->lv[l].h, s->lv[l].s, ok};
+        };
         {
             std::lock_guard<std::mutex> g(g_tune_mu);
             for (int l = 0; l < s->nLevels; l++) {
@@ -674,6 +725,16 @@ int tune_solver(vfs_solver *s, int masked)
             if (rc) return rc;
             std::lock_guard<std::mutex> g(g_tune_mu);
             for (int l : todo) g_tune[key(l)] = tn.lv[l];
+        }
+    }
+    // option "pair": 0 = one iteration per launch everywhere, 1 = two wherever allowed
+    if (s->force_pair >= 0) {
+        std::vector<int> all;
+        for (int l = 0; l < s->nLevels; l++) all.push_back(l);
+        const std::vector<bool> ok = pair_allowed(s, m, all);
+        for (int l = 0; l < s->nLevels; l++) {
+            tn.lv[l].pair = (unsigned char)(s->force_pair == 1 && ok[l]);
+            if (tn.lv[l].pair && !pair_shape(tn.lv[l].shape)) tn.lv[l].shape = 4;
         }
     }
     s->tuned[m] = true;
@@ -917,8 +978,17 @@ int vfs_copy_mask(vfs_solver *s, const float *m, int pitch)
     for (int l = 1; l < s->nLevels; l++)
         vfs::launch_downscale_nn(s->mask[l - 1], s->lv[l - 1], s->mask[l], s->lv[l], s->stream);
     if ((rc = check_launch(s))) return rc;
+    // column 0 of every mask level (two-iteration launches need it empty, see op_iter2)
+    for (int l = 0; l < s->nLevels; l++) {
+        std::vector<float> col(s->lv[l].h);
+        HIPCHK(s, hipMemcpy2DAsync(col.data(), sizeof(float), s->mask[l], sizeof(float) * (size_t)s->lv[l].s,
+                                   sizeof(float), s->lv[l].h, hipMemcpyDeviceToHost, s->stream));
+        HIPCHK(s, hipStreamSynchronize(s->stream));
+        s->mask_col0[l] = std::any_of(col.begin(), col.end(), [](float v) { return v != 0.f; });
+    }
     HIPCHK(s, hipStreamSynchronize(s->stream));
     s->has_mask = true;
+    s->tuned[1] = false;
     return VFS_OK;
 }
 
@@ -1145,6 +1215,12 @@ int vfs_set_option(vfs_solver *s, const char *key, int value)
         s->tuned[0] = s->tuned[1] = false;
         return VFS_OK;
     }
+    if (!strcmp(key, "pair")) {
+        if (value < -1 || value > 1) return fail(s, VFS_ERR_ARG, "pair must be -1, 0 or 1");
+        s->force_pair = value;
+        s->tuned[0] = s->tuned[1] = false;
+        return VFS_OK;
+    }
     return fail(s, VFS_ERR_ARG, "unknown option '%s'", key);
 }
 
@@ -1155,11 +1231,12 @@ int vfs_get_option(const vfs_solver *s, const char *key, int *value)
     else if (!strcmp(key, "storage")) *value = s->storage;
     else if (!strcmp(key, "shape")) *value = s->force_shape;
     else if (!strcmp(key, "autotune")) *value = s->autotune ? 1 : 0;
+    else if (!strcmp(key, "pair")) *value = s->force_pair;
     else return VFS_ERR_ARG;
     return VFS_OK;
 }
 
-int vfs_get_level_tuning(vfs_solver *s, int masked, int level, int *shape, int *preload)
+int vfs_get_level_tuning(vfs_solver *s, int masked, int level, int *shape, int *preload, int *pair)
 {
     if (!s || !shape || !preload) return VFS_ERR_ARG;
     if (!s->initialized || level < 0 || level >= s->nLevels) return fail(s, VFS_ERR_ARG, "bad level");
@@ -1169,6 +1246,7 @@ int vfs_get_level_tuning(vfs_solver *s, int masked, int level, int *shape, int *
     const LevelTune t = s->tuning[masked ? 1 : 0].lv[level];
     *shape = t.shape;
     *preload = t.preload;
+    if (pair) *pair = t.pair;
     return VFS_OK;
 }
 
@@ -1191,8 +1269,9 @@ int vfs_time_solves(vfs_solver *s, int masked, int n, float *ms_total)
     return VFS_OK;
 }
 
-int vfs_time_inner_iterations(vfs_solver *s, int level, int reps, float *ms_total, int *launches_per_iter)
+int vfs_time_inner_iterations(vfs_solver *s, int level, int reps, float *ms_total, int *iters_per_launch)
 {
+    const int reps0 = reps;
     if (!s || !ms_total || reps < 1) return VFS_ERR_ARG;
     if (!s->initialized || level < 0 || level >= s->nLevels) return fail(s, VFS_ERR_ARG, "bad level");
     int rc = set_device(s);
@@ -1203,16 +1282,24 @@ int vfs_time_inner_iterations(vfs_solver *s, int level, int reps, float *ms_tota
     const Params P = s->params();
     Program pr;
     int roles = vfs::kRolesInit;
+    const bool pairs = s->tuning[1].lv[level].pair != 0;
+    if (pairs) reps += reps & 1;
     // `reps` iterations of the 50-iteration schedule, restarted every 50
     {
         Params Q = P;
         Q.nSolverIters = 50;
         Program w;
-        while ((int)pr.ops.size() < reps) {
+        int done = 0;
+        while (done < reps) {
             w.ops.clear();
-            push_level(w, Q, 1, level, 1, 50, roles);
-            for (const ProgOp &o : w.ops)
-                if (o.kind == vfs::OP_ITER && (int)pr.ops.size() < reps) pr.ops.push_back(o);
+            push_level(w, Q, 1, level, 1, 50, roles, pairs);
+            for (const ProgOp &o : w.ops) {
+                if (done >= reps) break;
+                if (o.kind == vfs::OP_ITER || o.kind == vfs::OP_ITER2) {
+                    pr.ops.push_back(o);
+                    done += o.kind == vfs::OP_ITER2 ? 2 : 1;
+                }
+            }
         }
         apply_tuning(pr, s->tuning[1]);
     }
@@ -1227,8 +1314,8 @@ int vfs_time_inner_iterations(vfs_solver *s, int level, int reps, float *ms_tota
     (void)hipEventDestroy(e1);
     if ((rc = check_launch(s))) return rc;
     HIPCHK(s, hipEventRecord(s->done, s->stream));
-    *ms_total = ms;
-    if (launches_per_iter) *launches_per_iter = 1;
+    *ms_total = ms * (float)reps0 / (float)reps;
+    if (iters_per_launch) *iters_per_launch = pairs ? 2 : 1;
     return VFS_OK;
 }
 
@@ -1343,10 +1430,30 @@ static int tune_batch(vfs_batch *b, int masked)
         }
         std::vector<int> levels;
         for (int l = 0; l < f->nLevels; l++) levels.push_back(l);
-        if (r == hipSuccess) r = time_launch_choices(b->eng.stream, b->eng.d_desc, descs, f->params(), m, levels, tn);
+        // two-iteration launches only where every member allows them
+        std::vector<bool> ok(levels.size(), true);
+        for (const vfs_solver *s : b->members) {
+            const std::vector<bool> o = pair_allowed(s, m, levels);
+            for (size_t i = 0; i < ok.size(); i++) ok[i] = ok[i] && o[i];
+        }
+        if (r == hipSuccess)
+            r = time_launch_choices(b->eng.stream, b->eng.d_desc, descs, f->params(), m, levels, ok, tn);
         (void)hipStreamSynchronize(b->eng.stream);
         for (float *p : scratch) (void)hipFree(p);
         if (r != hipSuccess) return failb(b, VFS_ERR_DEVICE, "batch launch tuner: %s", hipGetErrorString(r));
+    }
+    if (f->force_pair >= 0) {
+        std::vector<int> all;
+        for (int l = 0; l < f->nLevels; l++) all.push_back(l);
+        std::vector<bool> ok(all.size(), true);
+        for (const vfs_solver *s : b->members) {
+            const std::vector<bool> o = pair_allowed(s, m, all);
+            for (size_t i = 0; i < ok.size(); i++) ok[i] = ok[i] && o[i];
+        }
+        for (int l = 0; l < f->nLevels; l++) {
+            tn.lv[l].pair = (unsigned char)(f->force_pair == 1 && ok[l]);
+            if (tn.lv[l].pair && !pair_shape(tn.lv[l].shape)) tn.lv[l].shape = 4;
+        }
     }
     b->tuned[m] = true;
     return VFS_OK;
@@ -1413,7 +1520,7 @@ int vfs_batch_solve_warm(vfs_batch *b, int masked, int start_level, int warps, i
     return batch_solve(b, masked, start_level, warps, iters);
 }
 
-int vfs_batch_get_level_tuning(vfs_batch *b, int masked, int level, int *shape, int *preload)
+int vfs_batch_get_level_tuning(vfs_batch *b, int masked, int level, int *shape, int *preload, int *pair)
 {
     if (!b || !shape || !preload) return VFS_ERR_ARG;
     if (level < 0 || level >= b->members[0]->nLevels) return failb(b, VFS_ERR_ARG, "bad level");
@@ -1421,6 +1528,7 @@ int vfs_batch_get_level_tuning(vfs_batch *b, int masked, int level, int *shape, 
     const LevelTune t = b->tuning[masked ? 1 : 0].lv[level];
     *shape = t.shape;
     *preload = t.preload;
+    if (pair) *pair = t.pair;
     return VFS_OK;
 }
 
@@ -1705,7 +1813,8 @@ int admit(vfs_server *v)
         P.nWarpIters = sp.nWarpIters;
         P.nSolverIters = sp.nSolverIters;
         r->warm = sp.warm_ok ? 1 : 0;
-        const Tuning &tn = s->tuning[1];
+        // serving launches mix ops of different frames: one iteration per launch
+        const Tuning tn = s->tuning[1].without_pairs();
         r->prog = sp.warm_ok ? build_program(P, 1, sp.warm_level, sp.warm_warps, sp.warm_iters, s->roles, tn)
                              : build_program(P, 1, -1, 0, 0, 0, tn);
         r->pc = 0;
