@@ -158,6 +158,9 @@ int vfs_get_option(const vfs_solver *s, const char *key, int *value);
  * Runs the tuner first when it has not run for this geometry.
  * No counterpart in the reference. */
 int vfs_get_level_tuning(vfs_solver *s, int masked, int level, int *shape, int *preload, int *pair);
+/* Sets the launch choices of one level (reproducible profiling, expert use); they hold
+ * until the next initialize or option change.  No counterpart in the reference. */
+int vfs_set_level_tuning(vfs_solver *s, int masked, int level, int shape, int preload, int pair);
 
 /* Timing with HIP events on the solver's stream: runs `n` solves (masked != 0:
  * solveStereoForwardMasked) back to back on resident inputs and returns their total

@@ -55,6 +55,7 @@ EXPORTS = [
     ("vfs_get_option", ctypes.c_int, [ctypes.c_void_p, ctypes.c_char_p, _c_int_p]),
     ("vfs_get_level_tuning", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, _c_int_p, _c_int_p,
                                             _c_int_p]),
+    ("vfs_set_level_tuning", ctypes.c_int, [ctypes.c_void_p] + [ctypes.c_int] * 5),
     ("vfs_batch_create", ctypes.c_int, [ctypes.POINTER(ctypes.c_void_p), ctypes.POINTER(ctypes.c_void_p), ctypes.c_int]),
     ("vfs_batch_destroy", ctypes.c_int, [ctypes.c_void_p]),
     ("vfs_batch_solve", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
@@ -295,6 +296,12 @@ class StereoTgv:
         if with_pair:
             return self.SHAPES[sh.value], bool(pre.value), bool(pair.value)
         return self.SHAPES[sh.value], bool(pre.value)
+
+    def set_level_tuning(self, level: int, shape: int, preload: bool = False, pair: bool = False,
+                         masked: bool = True):
+        """Fix the launch choices of one level (shape index as option "shape")."""
+        return self._check(self._L.vfs_set_level_tuning(self._h, int(masked), level, shape, int(preload), int(pair)),
+                           "set_level_tuning")
 
     def time_solves(self, masked: bool = True, n: int = 1) -> float:
         ms = ctypes.c_float()

@@ -1250,6 +1250,25 @@ int vfs_get_level_tuning(vfs_solver *s, int masked, int level, int *shape, int *
     return VFS_OK;
 }
 
+int vfs_set_level_tuning(vfs_solver *s, int masked, int level, int shape, int preload, int pair)
+{
+    if (!s) return VFS_ERR_ARG;
+    if (!s->initialized || level < 0 || level >= s->nLevels) return fail(s, VFS_ERR_ARG, "bad level");
+    if (shape < 0 || shape >= vfs::kShapes || (pair && !pair_shape(shape)))
+        return fail(s, VFS_ERR_ARG, "bad shape %d (two iterations per launch need 64x4, 32x8 or 16x16)", shape);
+    const int m = masked ? 1 : 0;
+    int rc = set_device(s);
+    if (!rc && !s->tuned[m]) rc = tune_solver(s, m);  // the other levels keep their choices
+    if (rc) return rc;
+    if (pair && !pair_allowed(s, m, std::vector<int>{level})[0])
+        return fail(s, VFS_ERR_ARG, "two iterations per launch are not allowed at level %d", level);
+    LevelTune &t = s->tuning[m].lv[level];
+    t.shape = (unsigned char)shape;
+    t.preload = (unsigned char)(preload != 0);
+    t.pair = (unsigned char)(pair != 0);
+    return VFS_OK;
+}
+
 int vfs_time_solves(vfs_solver *s, int masked, int n, float *ms_total)
 {
     if (!s || !ms_total || n < 1) return VFS_ERR_ARG;
