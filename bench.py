@@ -58,8 +58,8 @@ def main():
     ap.add_argument("--shape", type=int, default=-1,
                     help="force one tile shape (0..6) instead of the per-level tuner (profiling aid)")
     ap.add_argument("--tuning", default="",
-                    help="per-level launch choices 'shape[p][x2],...' from level 0 (profiling aid): shape index 0..6, "
-                         "p = Tp prefetch, x2 = two iterations per launch")
+                    help="per-level launch choices 'shape[p][xK],...' from level 0 (profiling aid): shape index 0..6, "
+                         "p = Tp prefetch, x2 / x4 = two / four inner iterations per launch")
     ap.add_argument("--iter-only", type=int, default=0,
                     help="profiling aid: run N level-0 inner iterations (after a 256 MiB reference copy) and exit")
     args = ap.parse_args()
@@ -91,7 +91,8 @@ def main():
     g.loadVectorFields(pair["tv"], pair["cv"])
     g.copyImagesToDevice(pair["i0"], pair["i1"])
     for lvl, spec in enumerate(c for c in args.tuning.split(",") if c):
-        g.set_level_tuning(lvl, int(spec[0]), preload="p" in spec, pair="x2" in spec)
+        g.set_level_tuning(lvl, int(spec[0]), preload="p" in spec,
+                           per_launch=int(spec.split("x")[1]) if "x" in spec else 1)
 
     if args.iter_only:
         # A plain device copy of known size (256 MiB read + 256 MiB written per dispatch)
@@ -171,10 +172,10 @@ def main():
     # per-level launch choices of the tile-shape tuner (made at the first solve)
     out["tile_tuning"] = {}
     for lvl in range(wl["levels"]):
-        (tw, th), pre, paired = g.level_tuning(lvl, True, with_pair=True)
+        (tw, th), pre, k = g.level_tuning(lvl, True, with_per_launch=True)
         lw, lh, _ = g.level_dims(lvl)
         out["tile_tuning"][f"L{lvl} {lw}x{lh}"] = (f"{tw}x{th}" + (" +Tp prefetch" if pre else "")
-                                                   + (", 2 iterations per launch" if paired else ""))
+                                                   + (f", {k} iterations per launch" if k > 1 else ""))
     print(f"bench: tile tuning {out['tile_tuning']}", file=sys.stderr, flush=True)
 
     if not args.quick:
@@ -281,9 +282,9 @@ def measure_batched(args, P, S, vfs_amd):
     res = _time_batch(args, b, 64)
     res["tile_tuning"] = {}
     for l in range(5):
-        (tw, th), pre, paired = b.level_tuning(l, with_pair=True)
+        (tw, th), pre, k = b.level_tuning(l, with_per_launch=True)
         res["tile_tuning"][f"L{l}"] = (f"{tw}x{th}" + (" +Tp prefetch" if pre else "")
-                                       + (", 2 iterations per launch" if paired else ""))
+                                       + (f", {k} iterations per launch" if k > 1 else ""))
     mpx = sum(p["i0"].size for p in pairs) / 1e6
     res["Mpx_per_s"] = round(res["pairs_per_s"] * mpx / 64, 2)
     res["workload"] = "64 pairs, sizes 424x400 / 640x480 / 848x800 in turn, L5 W10 I50, FP32, one batch"

@@ -138,15 +138,17 @@ int vfs_synchronize(vfs_solver *s);
  *             kernels one by one;
  *   "storage" 0 = FP32 (default), 1 = FP16, 2 = BF16 storage of the image pyramid,
  *             warped image, derivatives, tensor and duals p, q; arithmetic stays FP32;
- *   "autotune" 1 = pick each level's tile shape (and Tp prefetch) by timing the inner
- *             iteration in every shape at the first solve of a geometry (default; the
- *             choices are cached per process), 0 = fixed 64x8 tiles;
+ *   "autotune" 1 = pick each level's tile shape, Tp prefetch and iterations per launch
+ *             by timing the inner iteration in every choice at the first solve of a
+ *             geometry (default; the choices are cached per process), 0 = fixed 64x8
+ *             tiles, one iteration per launch;
  *   "shape"   -1 = as "autotune" says (default), 0..6 = force one tile shape for every
  *             level: 64x8, 64x4, 128x4, 32x16, 32x8, 16x32, 16x16;
- *   "pair"    -1 = as "autotune" says (default), 0 = one inner iteration per launch,
- *             1 = two per launch wherever allowed (a recomputed 4-pixel halo; levels
- *             with a mask pixel in column 0 keep one).
- * Results do not depend on "graph", "autotune", "shape" or "pair".
+ *   "per_launch" -1 = as "autotune" says (default), 1, 2 or 4 = that many inner
+ *             iterations per launch wherever allowed (the workgroup recomputes a halo
+ *             of 3 pixels per iteration; levels with a mask pixel in column 0 run one);
+ *   "pair"    the same as "per_launch" -1 / 1 / 2 written as -1 / 0 / 1.
+ * Results do not depend on "graph", "autotune", "shape", "per_launch" or "pair".
  * No counterpart in the reference (FP32 only, stereotgv.h:47-100; its block shape is a
  * constructor argument, stereotgv.cu:32-42). */
 int vfs_set_option(vfs_solver *s, const char *key, int value);
@@ -154,13 +156,13 @@ int vfs_get_option(const vfs_solver *s, const char *key, int *value);
 
 /* The launch choices used at `level` by the masked (masked != 0) or unmasked solve:
  * tile shape index (as option "shape"), whether the inner iteration prefetches its
- * Tp-plane neighbours, and whether two iterations share one launch (`pair` may be NULL).
- * Runs the tuner first when it has not run for this geometry.
+ * Tp-plane neighbours, and the inner iterations per launch, 1, 2 or 4 (`per_launch` may
+ * be NULL).  Runs the tuner first when it has not run for this geometry.
  * No counterpart in the reference. */
-int vfs_get_level_tuning(vfs_solver *s, int masked, int level, int *shape, int *preload, int *pair);
+int vfs_get_level_tuning(vfs_solver *s, int masked, int level, int *shape, int *preload, int *per_launch);
 /* Sets the launch choices of one level (reproducible profiling, expert use); they hold
  * until the next initialize or option change.  No counterpart in the reference. */
-int vfs_set_level_tuning(vfs_solver *s, int masked, int level, int shape, int preload, int pair);
+int vfs_set_level_tuning(vfs_solver *s, int masked, int level, int shape, int preload, int per_launch);
 
 /* Timing with HIP events on the solver's stream: runs `n` solves (masked != 0:
  * solveStereoForwardMasked) back to back on resident inputs and returns their total
@@ -168,9 +170,10 @@ int vfs_set_level_tuning(vfs_solver *s, int masked, int level, int shape, int pr
 int vfs_time_solves(vfs_solver *s, int masked, int n, float *ms_total);
 
 /* Timing of the inner TGV iteration at `level` (the dominant work): `reps` iterations
- * (rounded up to even when the level runs two iterations per launch) launched back to
- * back on the current solver state, HIP events around them.  Returns the time of `reps`
- * iterations in ms and, in the last argument, the iterations per launch (1 or 2).
+ * (rounded up to whole launches when the level runs several iterations per launch)
+ * launched back to back on the current solver state, HIP events around them.  Returns
+ * the time of `reps` iterations in ms and, in the last argument, the iterations per
+ * launch (1, 2 or 4).
  * The solver state is left modified; the next solve starts fresh as always. */
 int vfs_time_inner_iterations(vfs_solver *s, int level, int reps, float *ms_total, int *iters_per_launch);
 
@@ -194,7 +197,7 @@ int vfs_batch_solve_warm(vfs_batch *b, int masked, int start_level, int warps, i
 int vfs_batch_synchronize(vfs_batch *b);
 /* Launch choices of the batch's launches at `level` (as vfs_get_level_tuning); the batch
  * times them on scratch copies of its largest members at its first solve. */
-int vfs_batch_get_level_tuning(vfs_batch *b, int masked, int level, int *shape, int *preload, int *pair);
+int vfs_batch_get_level_tuning(vfs_batch *b, int masked, int level, int *shape, int *preload, int *per_launch);
 /* `n` batch solves back to back, HIP events around them; total ms. */
 int vfs_batch_time_solves(vfs_batch *b, int masked, int n, float *ms_total);
 const char *vfs_batch_last_error(const vfs_batch *b);

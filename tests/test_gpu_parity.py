@@ -187,17 +187,19 @@ def test_tile_shapes_give_identical_results(masked):
 
 
 @pytest.mark.parametrize("masked", [True, False])
-@pytest.mark.parametrize("shape", [4, 1, 6])
-def test_two_iterations_per_launch_give_identical_results(masked, shape):
-    """Two inner iterations per launch (recomputed halo, spare buffers, rotated roles)
-    reproduce one iteration per launch bit for bit, including odd iteration counts and
-    the buffers that later ops read outside the mask."""
+@pytest.mark.parametrize("shape,storage", [(4, "f32"), (1, "f32"), (6, "f32"), (3, "f32"), (4, "f16"), (6, "bf16")])
+def test_several_iterations_per_launch_give_identical_results(masked, shape, storage):
+    """Two and four inner iterations per launch (recomputed halo, spare buffers, rotated
+    roles; duals rounded to the storage type between iterations) reproduce one iteration
+    per launch bit for bit, including iteration counts that leave a remainder
+    (11 = 4 + 4 + 2 + 1) and the buffers that later ops read outside the mask."""
     pair = small_pair(200, 120, seed=37)
     ref = None
-    for fuse in (0, 1):
-        g = make_gpu(pair, 3, 2, 9)
+    for per_launch in (1, 2, 4):
+        g = make_gpu(pair, 3, 2, 11, storage=storage)
         g.set_option("shape", shape)
-        g.set_option("pair", fuse)
+        g.set_option("per_launch", per_launch)
+        print(f"per_launch {per_launch} shape {shape} masked {masked}", flush=True)
         if masked:
             g.solveStereoForwardMasked()
         else:
@@ -205,32 +207,55 @@ def test_two_iterations_per_launch_give_identical_results(masked, shape):
         planes = {n: g.read_plane(n, 0) for n in ("u", "u_", "v_", "warpUV")}
         planes["flow"] = g.copyDisparityToHost()
         for lvl in range(3):
-            (tw, th), pre, paired = g.level_tuning(lvl, masked, with_pair=True)
-            assert paired == bool(fuse)
+            (tw, th), pre, k = g.level_tuning(lvl, masked, with_per_launch=True)
+            assert k == per_launch
         if ref is None:
             ref = planes
             continue
         for n, v in planes.items():
-            assert np.array_equal(v, ref[n], equal_nan=True), (n, float(np.nanmax(np.abs(v - ref[n]))))
+            assert np.array_equal(v, ref[n], equal_nan=True), (per_launch, n, float(np.nanmax(np.abs(v - ref[n]))))
 
 
-def test_two_iterations_skip_levels_with_mask_in_column_0():
+@pytest.mark.parametrize("masked", [True, False])
+def test_small_tiles_several_iterations_give_identical_results(masked):
+    """The 8x8, 4x4 and 32x32 tiles of the multi-iteration launches (set per level)
+    reproduce one iteration per launch bit for bit."""
+    pair = small_pair(200, 120, seed=41)
+    g = make_gpu(pair, 3, 2, 11)
+    g.set_option("per_launch", 1)
+    (g.solveStereoForwardMasked if masked else g.solveStereoForward)()
+    ref = {n: g.read_plane(n, 0) for n in ("u", "u_", "v_", "warpUV")}
+    for shape in (7, 8, 9):
+        for per_launch in (2, 4):
+            g = make_gpu(pair, 3, 2, 11)
+            print(f"small tiles: shape {shape} per_launch {per_launch} masked {masked}", flush=True)
+            for lvl in range(3):
+                g.set_level_tuning(lvl, shape, per_launch=per_launch, masked=masked)
+            (g.solveStereoForwardMasked if masked else g.solveStereoForward)()
+            for lvl in range(3):
+                assert g.level_tuning(lvl, masked, with_per_launch=True)[::2] == (StereoTgv.SHAPES[shape], per_launch)
+            for n, v in ref.items():
+                got = g.read_plane(n, 0)
+                assert np.array_equal(got, v, equal_nan=True), (shape, per_launch, n)
+
+
+def test_several_iterations_skip_levels_with_mask_in_column_0():
     """A mask reaching column 0 keeps one iteration per launch at those levels."""
     pair = small_pair(160, 120, seed=39)
     pair["mask"][:, :8] = 1.0
     g = make_gpu(pair, 3, 2, 8)
-    g.set_option("pair", 1)
+    g.set_option("per_launch", 4)
     g.solveStereoForwardMasked()
-    assert not any(g.level_tuning(l, True, with_pair=True)[2] for l in range(3))
+    assert all(g.level_tuning(l, True, with_per_launch=True)[2] == 1 for l in range(3))
     u1 = g.copy1DDisparityToHost()
     g2 = make_gpu(pair, 3, 2, 8)
-    g2.set_option("pair", 0)
+    g2.set_option("per_launch", 1)
     g2.solveStereoForwardMasked()
     assert np.array_equal(u1, g2.copy1DDisparityToHost())
     # the unmasked form has no such restriction
     g.set_option("pair", 1)
     g.solveStereoForward()
-    assert all(g.level_tuning(l, False, with_pair=True)[2] for l in range(3))
+    assert all(g.level_tuning(l, False, with_per_launch=True)[2] == 2 for l in range(3))
 
 
 def test_resolve_without_new_images_recalibrates():
@@ -369,14 +394,15 @@ def test_batch_equals_alone_mixed_sizes():
     b.solve(True)
     for k, s in enumerate(solvers):
         assert np.array_equal(s.copy1DDisparityToHost(), alone[k]), k
-    # a batch running two iterations per launch gives the same results
-    solvers2 = [make_gpu(p, 3, 3, 10) for p in pairs]
-    solvers2[0].set_option("pair", 1)
-    b2 = Batch(solvers2)
-    b2.solve(True)
-    assert all(b2.level_tuning(l, with_pair=True)[2] for l in range(3))
-    for k, s in enumerate(solvers2):
-        assert np.array_equal(s.copy1DDisparityToHost(), alone[k]), ("pair", k)
+    # batches running two and four iterations per launch give the same results
+    for per_launch in (2, 4):
+        solvers2 = [make_gpu(p, 3, 3, 10) for p in pairs]
+        solvers2[0].set_option("per_launch", per_launch)
+        b2 = Batch(solvers2)
+        b2.solve(True)
+        assert all(b2.level_tuning(l, with_per_launch=True)[2] == per_launch for l in range(3))
+        for k, s in enumerate(solvers2):
+            assert np.array_equal(s.copy1DDisparityToHost(), alone[k]), (per_launch, k)
     # a second batch solve on freshly copied images is identical again
     for s, p in zip(solvers, pairs):
         s.copyImagesToDevice(p["i0"], p["i1"])

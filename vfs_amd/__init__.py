@@ -284,24 +284,26 @@ class StereoTgv:
         self._check(self._L.vfs_get_option(self._h, key.encode(), ctypes.byref(v)), "get_option")
         return v.value
 
-    # tile shapes of the "shape" option, by index
-    SHAPES = ((64, 8), (64, 4), (128, 4), (32, 16), (32, 8), (16, 32), (16, 16))
+    # tile shapes by index: 0 .. 6 are those of the "shape" option; 8x8, 4x4 and 32x32
+    # exist for the multi-iteration launches only
+    SHAPES = ((64, 8), (64, 4), (128, 4), (32, 16), (32, 8), (16, 32), (16, 16), (8, 8), (4, 4), (32, 32))
 
-    def level_tuning(self, level: int, masked: bool = True, with_pair: bool = False):
-        """(tile shape (w, h), Tp prefetch[, two iterations per launch]) used at `level`;
-        runs the tuner if needed."""
-        sh, pre, pair = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+    def level_tuning(self, level: int, masked: bool = True, with_per_launch: bool = False):
+        """(tile shape (w, h), Tp prefetch[, inner iterations per launch]) used at
+        `level`; runs the tuner if needed."""
+        sh, pre, k = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
         self._check(self._L.vfs_get_level_tuning(self._h, int(masked), level, ctypes.byref(sh), ctypes.byref(pre),
-                                                 ctypes.byref(pair)), "get_level_tuning")
-        if with_pair:
-            return self.SHAPES[sh.value], bool(pre.value), bool(pair.value)
+                                                 ctypes.byref(k)), "get_level_tuning")
+        if with_per_launch:
+            return self.SHAPES[sh.value], bool(pre.value), k.value
         return self.SHAPES[sh.value], bool(pre.value)
 
-    def set_level_tuning(self, level: int, shape: int, preload: bool = False, pair: bool = False,
+    def set_level_tuning(self, level: int, shape: int, preload: bool = False, per_launch: int = 1,
                          masked: bool = True):
-        """Fix the launch choices of one level (shape index as option "shape")."""
-        return self._check(self._L.vfs_set_level_tuning(self._h, int(masked), level, shape, int(preload), int(pair)),
-                           "set_level_tuning")
+        """Fix the launch choices of one level (shape index as option "shape"; 1, 2 or 4
+        inner iterations per launch)."""
+        return self._check(self._L.vfs_set_level_tuning(self._h, int(masked), level, shape, int(preload),
+                                                        int(per_launch)), "set_level_tuning")
 
     def time_solves(self, masked: bool = True, n: int = 1) -> float:
         ms = ctypes.c_float()
@@ -354,14 +356,14 @@ class Batch:
     def synchronize(self):
         return self._check(self._L.vfs_batch_synchronize(self._h), "batch_synchronize")
 
-    def level_tuning(self, level: int, masked: bool = True, with_pair: bool = False):
-        """(tile shape (w, h), Tp prefetch[, two iterations per launch]) of the batch's
+    def level_tuning(self, level: int, masked: bool = True, with_per_launch: bool = False):
+        """(tile shape (w, h), Tp prefetch[, inner iterations per launch]) of the batch's
         launches at `level` (after a solve)."""
-        sh, pre, pair = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+        sh, pre, k = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
         self._check(self._L.vfs_batch_get_level_tuning(self._h, int(masked), level, ctypes.byref(sh),
-                                                        ctypes.byref(pre), ctypes.byref(pair)), "batch_get_level_tuning")
-        if with_pair:
-            return StereoTgv.SHAPES[sh.value], bool(pre.value), bool(pair.value)
+                                                        ctypes.byref(pre), ctypes.byref(k)), "batch_get_level_tuning")
+        if with_per_launch:
+            return StereoTgv.SHAPES[sh.value], bool(pre.value), k.value
         return StereoTgv.SHAPES[sh.value], bool(pre.value)
 
     def time_solves(self, masked: bool = True, n: int = 1) -> float:

@@ -792,62 +792,89 @@ __device__ __forceinline__ void op_iter(const Tile &t, int roles, const IterPara
     }
 }
 
-// ------------------------------------------------------------------ two iterations per op
+// ------------------------------------------------------------------ K iterations per op
 //
-// Temporal blocking of the inner iteration (OP_ITER2): one workgroup advances its tile by
-// two iterations.  It stages the current state over the tile and a border (4 pixels to
-// the left and top, 2 to the right and bottom) in LDS, computes the first iteration's
-// duals and primal step over the shrinking regions the second iteration reads, then the
-// second iteration on the tile.  Every pixel goes through the same dual_step /
-// primal_step code as in op_iter, so the result equals two single iterations bit for
-// bit.  Buffers: the first iteration's state goes to the `other` buffers (as a single
-// iteration's would), the second's to the spare buffers together with, outside the mask,
-// the current buffers' values (never written by an iteration); the host then rotates the
-// roles (spare -> current).  Other workgroups of the launch read the current buffers and,
-// outside the mask only, the other buffers, so no write of this op is read by it.
-// The host uses this op only where no mask pixel lies in column 0: there the reference's
-// flat-index left neighbour (pos - 1) is the previous row's last pixel.
-// Work-items of a two-iteration workgroup: its staged regions hold 1.5 - 2 times the
-// tile's pixels, so it takes four times the work-items of a tile.
-constexpr int NT2 = 1024;
-
-template <class S>
-struct Blk2 {
-    static constexpr int TW = S::TW, TH = S::TH;
-    // origin offset (both axes) and size of each region, relative to the tile
-    static constexpr int RO = -4, RW = TW + 6, RH = TH + 6;  // state loads (u_, v_, mask)
-    static constexpr int DO = -3, DW = TW + 4, DH = TH + 4;  // first iteration's duals
-    static constexpr int PO = -2, PW = TW + 3, PH = TH + 3;  // first iteration's primal step
-    static constexpr int EO = -1, EW = TW + 1, EH = TH + 1;  // second iteration's duals
-    static constexpr int NR = RW * RH, ND = DW * DH, NP = PW * PH, NE = EW * EH;
+// Temporal blocking of the inner iteration (OP_ITER2, OP_ITER4): one workgroup advances
+// its tile by K iterations.  Iteration j (r = K - 1 - j iterations after it) computes the
+// duals over the tile grown by 2r + 1 pixels to the left / top and r to the right /
+// bottom, and the primal step over the tile grown by 2r / r: the primal step of a pixel
+// reads the new q and Tp of its left and upper neighbours, the dual step the state of all
+// four neighbours.  The workgroup stages the state over the tile grown by 2K / K pixels
+// (every global load issued before the first LDS store): each work-item keeps what only
+// its own positions read in registers, and what neighbours read in LDS; the K iterations
+// then run in place.  Every pixel goes through the same dual_step / primal_step code as
+// in op_iter, and p and q are rounded to the storage type between iterations as the
+// planes round them, so the result equals K single iterations bit for bit.
+// Buffers (K even): iteration K - 2 stores u, u_ and v_ to the `other` buffers (as a
+// single iteration's would), iteration K - 1 its whole state to the spare buffers
+// together with, outside the mask, the current buffers' values (never written by an
+// iteration); the host then
+// rotates the roles (spare -> current).  Out of the mask, iteration j reads the current
+// buffers' values for even j and the other buffers' values for odd j, as the reference's
+// two ping-pong buffers give them.  Other workgroups of the launch read the current
+// buffers and, outside the mask only, the other buffers, so no write of this op is read
+// by it.  The spare buffers also take the current buffers' values in the row padding
+// (columns w .. s - 1, written by the tiles of the last tile column), so that the new
+// current buffers equal the old ones wherever no iteration writes: coarser levels'
+// values there are read again at finer levels.  The host uses this op only where no
+// mask pixel lies in column 0: there the reference's flat-index left neighbour (pos - 1)
+// is the previous row's last pixel.
+// The step sizes of iteration j come from the pair's schedule table at index it0 + j.
+// Work-items: one per staged position (rounded up to whole waves, at most 1024), so that
+// every phase takes at most one pixel per work-item on all but the largest tiles.
+template <int SH, int K>
+struct BlkK {
+    static constexpr int TW = kShape[SH].tw, TH = kShape[SH].th;
+    static constexpr int RO = -2 * K, RW = TW + 3 * K, RH = TH + 3 * K;  // staged region
+    static constexpr int NR = RW * RH;
+    static constexpr int NT = NR >= 1024 ? 1024 : (NR + 63) / 64 * 64;  // work-items
+    static constexpr int NK = (NR + NT - 1) / NT;  // staged positions per work-item
+    static_assert(multi_iter_shape(SH), "multi-iteration tile shape");
 };
 
-template <class S>
-struct Lds2 {
-    using B = Blk2<S>;
-    float ua[B::NR];      // u_ (current buffers)
-    float2 va[B::NR];     // v_
-    float mk[B::NR];      // mask
-    float2 p[B::ND];      // p: loaded, then each iteration's new duals in place
-    float4 q[B::ND];
-    float2 tp[B::ND];     // new Tp
-    float2 tpl[B::ND];    // Tp plane
-    float ta[B::ND], tb[B::ND], tc[B::ND];  // tensor
-    float uc[B::NP];      // u (current buffer)
-    float iu[B::NP], iz[B::NP];
-    float ub[B::NP];      // u_ (other buffer), read outside the mask by the second iteration
-    float2 vb[B::NP];     // v_ (other buffer)
-    float u1[B::NP], ub1[B::NP];  // u, u_ after the first iteration
-    float2 v1[B::NP];     // v_ after the first iteration
+// Values that neighbours read: u_ and v_ (inside the mask the iteration's state; outside
+// it the value of the buffer the reference reads at that step), q, Tp (inside the mask
+// the iteration's new Tp, outside it the Tp plane) and the mask.
+template <int SH, int K>
+struct LdsK {
+    using B = BlkK<SH, K>;
+    float ub[B::NR];
+    float2 v[B::NR];
+    float4 q[B::NR];
+    float2 tp[B::NR];
+    unsigned char mk[B::NR];
 };
 
-template <class S, bool MASKED, typename T>
-__device__ __forceinline__ void op_iter2(const Tile &t, int roles, const IterParams &ip0, const IterParams &ip1,
-                                         Lds2<S> &L)
+// A work-item's own staged position: everything only that position reads.
+struct OwnK {
+    float a, b, c, iu, iz;
+    float u, ub;        // u and u_ (current buffers, then each iteration's)
+    float2 v;           // v_
+    float2 p;           // p and q: the last dual step's values
+    float4 q;
+    float ubo;          // outside the mask: u_ and v_ of the other buffers
+    float2 vo;
+};
+
+// Rounds a dual value to the storage type: the planes hold p and q in it, and every
+// iteration's dual step reads them from there (its primal step uses the unrounded values).
+template <typename T>
+__device__ __forceinline__ float rT(float v)
 {
-    using B = Blk2<S>;
+    return Cv<T>::f(Cv<T>::t(v));
+}
+
+// The step sizes of iteration j: sched[j] = (sigma, tau, mu); `ip0` holds the
+// schedule-independent fields and the flags of the op.
+template <int SH, int K, bool MASKED, typename T>
+__device__ __forceinline__ void op_iterk(const Tile &t, int roles, const float4 *__restrict__ sched,
+                                         const IterParams &ip0, LdsK<SH, K> &L)
+{
+    static_assert(K % 2 == 0, "K-iteration ops rotate the roles like two iterations");
+    using B = BlkK<SH, K>;
+    constexpr int NT = B::NT, NK = B::NK;
     const int w = t.w, h = t.h, s = t.s, x0 = t.x0, y0 = t.y0;
-    const int tid = threadIdx.y * S::BX + threadIdx.x;  // NT2 work-items
+    const int tid = threadIdx.x;
     const float *__restrict__ mask = t.mask;
     const float *__restrict__ u_cur = slot(t, slot_u(rcur(roles, G_U)));
     float *__restrict__ u_oth = slot(t, slot_u(roth(roles, G_U)));
@@ -859,10 +886,8 @@ __device__ __forceinline__ void op_iter2(const Tile &t, int roles, const IterPar
     float2 *__restrict__ v_oth = slotp<float2>(t, slot_v(roth(roles, G_V)));
     float2 *__restrict__ v_spr = slotp<float2>(t, slot_v(rspare(roles, G_V)));
     const V2<T> *__restrict__ p_cur = slotp<V2<T>>(t, slot_p(rcur(roles, G_PQ)));
-    V2<T> *__restrict__ p_oth = slotp<V2<T>>(t, slot_p(roth(roles, G_PQ)));
     V2<T> *__restrict__ p_spr = slotp<V2<T>>(t, slot_p(rspare(roles, G_PQ)));
     const V4<T> *__restrict__ q_cur = slotp<V4<T>>(t, slot_q(rcur(roles, G_PQ)));
-    V4<T> *__restrict__ q_oth = slotp<V4<T>>(t, slot_q(roth(roles, G_PQ)));
     V4<T> *__restrict__ q_spr = slotp<V4<T>>(t, slot_q(rspare(roles, G_PQ)));
     float2 *__restrict__ Tp = slotp<float2>(t, PL_TP);
     const T *__restrict__ at = (const T *)slot(t, PL_A);
@@ -870,173 +895,206 @@ __device__ __forceinline__ void op_iter2(const Tile &t, int roles, const IterPar
     const T *__restrict__ ct = (const T *)slot(t, PL_C);
     const T *__restrict__ Iu = (const T *)slot(t, PL_IU);
     const T *__restrict__ Iz = (const T *)slot(t, PL_IZ);
+    // The duals of the `other` buffers are never read (an op reads the current ones and
+    // each warp starts from zero): the op stores p and q after its last iteration only.
 
-    // LDS index of image position (x, y) in each region
-    auto iR = [&](int x, int y) { return (y - y0 - B::RO) * B::RW + (x - x0 - B::RO); };
-    auto iD = [&](int x, int y) { return (y - y0 - B::DO) * B::DW + (x - x0 - B::DO); };
-    auto iP = [&](int x, int y) { return (y - y0 - B::PO) * B::PW + (x - x0 - B::PO); };
-    auto gpos = [&](int x, int y) {
-        const int xc = min(max(x, 0), w - 1), yc = min(max(y, 0), h - 1);
-        return (long)yc * s + xc;
-    };
-    auto in_tile = [&](int x, int y) { return x >= x0 && x < x0 + S::TW && y >= y0 && y < y0 + S::TH; };
-
-    // ---- stage the current state (clamped positions outside the image)
+    // ---- stage: own values into registers, shared ones into LDS (clamped positions
+    // outside the image); every load is issued before the first LDS store
+    OwnK o[NK];
+    int ox[NK], oy[NK];
+    float omk[NK];
+    float2 otp[NK];
 #pragma unroll
-    for (int k = 0; k < (B::NR + NT2 - 1) / NT2; k++) {
-        const int i = tid + k * NT2;
+    for (int k = 0; k < NK; k++) {
+        const int i = tid + k * NT;
+        ox[k] = x0 + B::RO + i % B::RW;
+        oy[k] = y0 + B::RO + i / B::RW;
         if (i < B::NR) {
-            const long pos = gpos(x0 + B::RO + i % B::RW, y0 + B::RO + i / B::RW);
-            L.ua[i] = ub_cur[pos];
-            L.va[i] = v_cur[pos];
-            L.mk[i] = MASKED ? mask[pos] : 1.f;
+            const int xc = min(max(ox[k], 0), w - 1), yc = min(max(oy[k], 0), h - 1);
+            const long pos = (long)yc * s + xc;
+            omk[k] = MASKED ? mask[pos] : 1.f;
+            o[k].ub = ub_cur[pos];
+            o[k].v = v_cur[pos];
+            o[k].u = u_cur[pos];
+            o[k].a = ldT(at, pos);
+            o[k].b = ldT(bt, pos);
+            o[k].c = ldT(ct, pos);
+            o[k].iu = ldT(Iu, pos);
+            o[k].iz = ldT(Iz, pos);
+            if (ip0.first) {
+                o[k].p = make_float2(0.f, 0.f);
+                o[k].q = make_float4(0.f, 0.f, 0.f, 0.f);
+            } else {
+                o[k].p = ld2T(p_cur, pos);
+                o[k].q = ld4T(q_cur, pos);
+            }
+        }
+    }
+    // outside the mask: the other buffers' u_, v_ and the Tp plane (read by neighbours)
+#pragma unroll
+    for (int k = 0; k < NK; k++) {
+        const int i = tid + k * NT;
+        o[k].ubo = 0.f;
+        o[k].vo = make_float2(0.f, 0.f);
+        otp[k] = make_float2(0.f, 0.f);
+        if (MASKED && i < B::NR && omk[k] == 0.f) {
+            const int xc = min(max(ox[k], 0), w - 1), yc = min(max(oy[k], 0), h - 1);
+            const long pos = (long)yc * s + xc;
+            o[k].ubo = ub_oth[pos];
+            o[k].vo = v_oth[pos];
+            otp[k] = Tp[pos];
         }
     }
 #pragma unroll
-    for (int k = 0; k < (B::ND + NT2 - 1) / NT2; k++) {
-        const int i = tid + k * NT2;
-        if (i < B::ND) {
-            const long pos = gpos(x0 + B::DO + i % B::DW, y0 + B::DO + i / B::DW);
-            L.p[i] = ip0.first ? make_float2(0.f, 0.f) : ld2T(p_cur, pos);
-            L.q[i] = ip0.first ? make_float4(0.f, 0.f, 0.f, 0.f) : ld4T(q_cur, pos);
-            L.ta[i] = ldT(at, pos);
-            L.tb[i] = ldT(bt, pos);
-            L.tc[i] = ldT(ct, pos);
-            L.tpl[i] = Tp[pos];
+    for (int k = 0; k < NK; k++) {
+        const int i = tid + k * NT;
+        if (i < B::NR) {
+            L.ub[i] = o[k].ub;
+            L.v[i] = o[k].v;
+            L.q[i] = o[k].q;
+            L.tp[i] = otp[k];
+            L.mk[i] = omk[k] != 0.f ? 1 : 0;
         }
     }
-#pragma unroll
-    for (int k = 0; k < (B::NP + NT2 - 1) / NT2; k++) {
-        const int i = tid + k * NT2;
-        if (i < B::NP) {
-            const long pos = gpos(x0 + B::PO + i % B::PW, y0 + B::PO + i / B::PW);
-            L.uc[i] = u_cur[pos];
-            L.iu[i] = ldT(Iu, pos);
-            L.iz[i] = ldT(Iz, pos);
-            L.ub[i] = ub_oth[pos];
-            L.vb[i] = v_oth[pos];
+    // row padding of the last tile column: the spare buffers take the current values
+    if (x0 + B::TW >= w && s > w) {
+        const int pw = s - w, rows = min(B::TH, h - y0);
+        for (int i = tid; i < pw * rows; i += NT) {
+            const long pos = (long)(y0 + i / pw) * s + w + i % pw;
+            u_spr[pos] = u_cur[pos];
+            ub_spr[pos] = ub_cur[pos];
+            v_spr[pos] = v_cur[pos];
         }
     }
     __syncthreads();
 
-    auto act = [&](int x, int y) { return !MASKED || L.mk[iR(x, y)] != 0.f; };
-    // u_ and v_ of the state entering iteration j (in-image positions): iteration 1 reads
-    // the first iteration's result inside the mask and the other buffers' values outside
-    auto U_ = [&](int j, int x, int y) {
-        return j == 0 ? L.ua[iR(x, y)] : (act(x, y) ? L.ub1[iP(x, y)] : L.ub[iP(x, y)]);
-    };
-    auto V_ = [&](int j, int x, int y) {
-        return j == 0 ? L.va[iR(x, y)] : (act(x, y) ? L.v1[iP(x, y)] : L.vb[iP(x, y)]);
-    };
+    auto act = [&](int k) { return !MASKED || L.mk[k] != 0; };
+    auto in_tile = [&](int x, int y) { return x >= x0 && x < x0 + B::TW && y >= y0 && y < y0 + B::TH; };
 
+#pragma unroll 1
+    for (int j = 0; j < K; j++) {
+        IterParams ip = ip0;
+        const float4 sc = sched[j];
+        ip.k1 = ip0.alpha1 * sc.x / 3.0f;
+        ip.k0 = ip0.alpha0 * sc.x / 2.0f;
+        ip.tau = sc.y;
+        ip.mu = sc.z;
+        ip.first = j == 0 && ip0.first;
+        ip.write_tp = j == K - 1 && ip0.write_tp;
+        const int r = K - 1 - j;
+
+        // ---- duals over the tile grown by 2r + 1 (left / top) and r (right / bottom)
 #pragma unroll
-    for (int j = 0; j < 2; j++) {
-        const IterParams &ip = j == 0 ? ip0 : ip1;
-        // ---- duals over the region the primal step reads (first: D, second: E)
-        const int dO = j == 0 ? B::DO : B::EO, dW = j == 0 ? B::DW : B::EW, dN = j == 0 ? B::ND : B::NE;
-        for (int i = tid; i < dN; i += NT2) {
-            const int x = x0 + dO + i % dW, y = y0 + dO + i / dW;
-            if (x < 0 || y < 0 || x >= w || y >= h || !act(x, y)) continue;
-            const int xr = x + 1 < w ? x + 1 : x, yd = y + 1 < h ? y + 1 : y;
-            DualIn r;
-            r.ok = true;
-            r.act = true;
-            r.hasR = (x + 1 < w) && act(xr, y);
-            r.hasD = (y + 1 < h) && act(x, yd);
-            const int d = iD(x, y);
-            r.a = L.ta[d];
-            r.b = L.tb[d];
-            r.c = L.tc[d];
-            r.u0 = U_(j, x, y);
-            r.uR = U_(j, xr, y);
-            r.uD = U_(j, x, yd);
-            r.uL = x >= 1 ? U_(j, x - 1, y) : 0.f;  // column 0 inside the mask: not used here
-            r.uU = y >= 1 ? U_(j, x, y - 1) : 0.f;
-            r.v0 = V_(j, x, y);
-            r.vR = V_(j, xr, y);
-            r.vD = V_(j, x, yd);
-            r.vL = x >= 1 ? V_(j, x - 1, y) : make_float2(0.f, 0.f);
-            r.vU = y >= 1 ? V_(j, x, y - 1) : make_float2(0.f, 0.f);
-            r.po = L.p[d];
-            r.qo = L.q[d];
+        for (int k = 0; k < NK; k++) {
+            const int i = tid + k * NT, x = ox[k], y = oy[k];
+            if (i >= B::NR || x < 0 || y < 0 || x >= w || y >= h) continue;
+            if (x < x0 - 2 * r - 1 || y < y0 - 2 * r - 1 || x > x0 + B::TW - 1 + r || y > y0 + B::TH - 1 + r) continue;
+            if (!act(i)) continue;
+            const int kR = x + 1 < w ? i + 1 : i, kD = y + 1 < h ? i + B::RW : i;
+            DualIn rr;
+            rr.ok = true;
+            rr.act = true;
+            rr.hasR = (x + 1 < w) && act(kR);
+            rr.hasD = (y + 1 < h) && act(kD);
+            rr.a = o[k].a;
+            rr.b = o[k].b;
+            rr.c = o[k].c;
+            rr.u0 = o[k].ub;
+            rr.uR = L.ub[kR];
+            rr.uD = L.ub[kD];
+            rr.uL = x >= 1 ? L.ub[i - 1] : 0.f;  // column 0 inside the mask: not used here
+            rr.uU = y >= 1 ? L.ub[i - B::RW] : 0.f;
+            rr.v0 = o[k].v;
+            rr.vR = L.v[kR];
+            rr.vD = L.v[kD];
+            rr.vL = x >= 1 ? L.v[i - 1] : make_float2(0.f, 0.f);
+            rr.vU = y >= 1 ? L.v[i - B::RW] : make_float2(0.f, 0.f);
+            // the previous iteration's duals as the planes hold them
+            rr.po = make_float2(rT<T>(o[k].p.x), rT<T>(o[k].p.y));
+            rr.qo = make_float4(rT<T>(o[k].q.x), rT<T>(o[k].q.y), rT<T>(o[k].q.z), rT<T>(o[k].q.w));
             float2 pn, tpn;
             float4 qn;
-            dual_step<MASKED>(r, x, y, w, h, ip, pn, qn, tpn);
-            L.p[d] = pn;
-            L.q[d] = qn;
-            L.tp[d] = tpn;
-            if (in_tile(x, y)) {
+            dual_step<MASKED>(rr, x, y, w, h, ip, pn, qn, tpn);
+            o[k].p = pn;  // the primal step of this iteration reads them unrounded
+            o[k].q = qn;
+            L.q[i] = qn;
+            L.tp[i] = tpn;
+            if (r == 0 && in_tile(x, y)) {
                 const long pos = (long)y * s + x;
-                st2T(j == 0 ? p_oth : p_spr, pos, pn);
-                st4T(j == 0 ? q_oth : q_spr, pos, qn);
+                st2T(p_spr, pos, pn);
+                st4T(q_spr, pos, qn);
                 if (ip.write_tp) Tp[pos] = tpn;
             }
         }
         __syncthreads();
 
-        // ---- thresholding + primal step (first: region P, second: the tile)
-        const int pO = j == 0 ? B::PO : 0, pW = j == 0 ? B::PW : S::TW, pN = j == 0 ? B::NP : S::TW * S::TH;
-        for (int i = tid; i < pN; i += NT2) {
-            const int x = x0 + pO + i % pW, y = y0 + pO + i / pW;
-            if (x < 0 || y < 0 || x >= w || y >= h) continue;
+        // ---- thresholding + primal step over the tile grown by 2r (left / top) and r;
+        // outside the mask, the stored u_ and v_ switch to the buffer the next
+        // iteration reads there
+#pragma unroll
+        for (int k = 0; k < NK; k++) {
+            const int i = tid + k * NT, x = ox[k], y = oy[k];
+            if (i >= B::NR || x < 0 || y < 0 || x >= w || y >= h) continue;
             const long pos = (long)y * s + x;
-            const bool mine = in_tile(x, y);
-            float un, u_s, v_sx, v_sy;
+            if (MASKED && !act(i)) {
+                if (r == 0) {
+                    // outside the mask: the spare buffers take the current buffers' values
+                    if (in_tile(x, y)) {
+                        u_spr[pos] = o[k].u;
+                        ub_spr[pos] = o[k].ub;
+                        v_spr[pos] = o[k].v;
+                    }
+                } else {
+                    L.ub[i] = (j & 1) ? o[k].ub : o[k].ubo;
+                    L.v[i] = (j & 1) ? o[k].v : o[k].vo;
+                }
+                continue;
+            }
+            if (x < x0 - 2 * r || y < y0 - 2 * r || x > x0 + B::TW - 1 + r || y > y0 + B::TH - 1 + r) continue;
+            float un, u_s;
+            float2 vs;
             if (!MASKED && outside_circle(x, y, w, h)) {
                 un = 0.f;
                 u_s = 0.f;
-                v_sx = v_sy = 0.f;
-            } else if (!act(x, y)) {
-                // outside the mask: the spare buffers take the current buffers' values
-                if (j == 1 && mine) {
-                    u_spr[pos] = L.uc[iP(x, y)];
-                    ub_spr[pos] = L.ua[iR(x, y)];
-                    v_spr[pos] = L.va[iR(x, y)];
-                }
-                continue;
+                vs = make_float2(0.f, 0.f);
             } else {
-                const int xr = x + 1 < w ? x + 1 : x, yd = y + 1 < h ? y + 1 : y;
-                const bool hR = (x + 1 < w) && act(xr, y), hD = (y + 1 < h) && act(x, yd);
-                const bool hL = x >= 1 && act(x - 1, y), hU = y >= 1 && act(x, y - 1);
-                const int d = iD(x, y), pp = iP(x, y);
-                // a neighbour outside the mask (or the image) is read from the Tp plane
-                const float tpLx = hL ? L.tp[iD(x - 1, y)].x : (x >= 1 ? L.tpl[iD(x - 1, y)].x : 0.f);
-                const float tpUy = hU ? L.tp[iD(x, y - 1)].y : (y >= 1 ? L.tpl[iD(x, y - 1)].y : 0.f);
-                const float divp = div_tp<MASKED>(x, y, w, h, hR, hD, hL, hU, L.tp[d], tpLx, tpUy);
-                const float4 ql = hL ? L.q[iD(x - 1, y)] : make_float4(0.f, 0.f, 0.f, 0.f);
-                const float4 qu = hU ? L.q[iD(x, y - 1)] : make_float4(0.f, 0.f, 0.f, 0.f);
-                const float uu = j == 0 ? L.uc[pp] : L.u1[pp];
-                float2 vs;
-                primal_step(ip, L.ta[d], L.tb[d], L.tc[d], U_(j, x, y), uu, L.iu[pp], L.iz[pp], V_(j, x, y), hR,
-                            hD, hL, hU, divp, L.q[d], ql, qu, L.p[d], un, u_s, vs);
-                v_sx = vs.x;
-                v_sy = vs.y;
+                const int kR = x + 1 < w ? i + 1 : i, kD = y + 1 < h ? i + B::RW : i;
+                const bool hR = (x + 1 < w) && act(kR), hD = (y + 1 < h) && act(kD);
+                const bool hL = x >= 1 && act(i - 1), hU = y >= 1 && act(i - B::RW);
+                // the left / upper neighbour's Tp: this iteration's inside the mask,
+                // the Tp plane's outside it
+                const float tpLx = x >= 1 ? L.tp[i - 1].x : 0.f;
+                const float tpUy = y >= 1 ? L.tp[i - B::RW].y : 0.f;
+                const float divp = div_tp<MASKED>(x, y, w, h, hR, hD, hL, hU, L.tp[i], tpLx, tpUy);
+                const float4 ql = hL ? L.q[i - 1] : make_float4(0.f, 0.f, 0.f, 0.f);
+                const float4 qu = hU ? L.q[i - B::RW] : make_float4(0.f, 0.f, 0.f, 0.f);
+                primal_step(ip, o[k].a, o[k].b, o[k].c, o[k].ub, o[k].u, o[k].iu, o[k].iz, o[k].v, hR, hD, hL, hU,
+                            divp, o[k].q, ql, qu, o[k].p, un, u_s, vs);
             }
-            if (j == 0) {
-                const int pp = iP(x, y);
-                L.u1[pp] = un;
-                L.ub1[pp] = u_s;
-                L.v1[pp] = make_float2(v_sx, v_sy);
-                if (mine) {
-                    u_oth[pos] = un;
-                    ub_oth[pos] = u_s;
-                    v_oth[pos] = make_float2(v_sx, v_sy);
-                }
-            } else {
+            o[k].u = un;
+            o[k].ub = u_s;
+            o[k].v = vs;
+            L.ub[i] = u_s;
+            L.v[i] = vs;
+            if (r == 1 && in_tile(x, y)) {
+                u_oth[pos] = un;
+                ub_oth[pos] = u_s;
+                v_oth[pos] = vs;
+            } else if (r == 0) {
                 u_spr[pos] = un;
                 ub_spr[pos] = u_s;
-                v_spr[pos] = make_float2(v_sx, v_sy);
+                v_spr[pos] = vs;
             }
         }
         __syncthreads();
     }
 }
 
-template <typename T, int SH>
-__global__ __launch_bounds__(NT2) void k_iter2(const PairDesc *__restrict__ D, const TickArgs A)
+template <typename T, int SH, int K>
+__global__ __launch_bounds__((BlkK<SH, K>::NT)) void k_iterk(const PairDesc *__restrict__ D, const TickArgs A)
 {
-    using S = Sh<SH>;
-    __shared__ Lds2<S> L;
+    using B = BlkK<SH, K>;
+    __shared__ LdsK<SH, K> L;
     const int nb = gridDim.x, i = blockIdx.x, xcd = i & 7, q = nb >> 3, r = nb & 7;
     const int b = (xcd < r ? xcd * (q + 1) : r * (q + 1) + (xcd - r) * q) + (i >> 3);
     int lo = 0;
@@ -1056,30 +1114,21 @@ __global__ __launch_bounds__(NT2) void k_iter2(const PairDesc *__restrict__ D, c
     t.h = E.h;
     t.s = E.s;
     const int local = b - E.tile_start;
-    const int ntx = (t.w + S::TW - 1) / S::TW;
-    t.x0 = (local % ntx) * S::TW;
-    t.y0 = (local / ntx) * S::TH;
-    // the second iteration's step sizes follow the schedule (stereotgv.cu:475-476)
-    IterParams ip0, ip1;
-    const float sigma1 = E.sigma / E.mu, tau1 = E.tau * E.mu;
-    ip0.k1 = A.alpha1 * E.sigma / 3.0f;
-    ip0.k0 = A.alpha0 * E.sigma / 2.0f;
-    ip0.tau = E.tau;
-    ip0.mu = E.mu;
-    ip1.k1 = A.alpha1 * sigma1 / 3.0f;
-    ip1.k0 = A.alpha0 * sigma1 / 2.0f;
-    ip1.tau = tau1;
-    ip1.mu = E.mu_next;
-    ip0.lambda = ip1.lambda = A.lambda;
-    ip0.alpha0 = ip1.alpha0 = A.alpha0;
-    ip0.alpha1 = ip1.alpha1 = A.alpha1;
-    ip0.first = E.flags & EF_FIRST;
-    ip1.first = 0;
-    ip0.write_tp = (E.flags & EF_WRITE_TP) != 0;
-    ip1.write_tp = (E.flags & EF_WRITE_TP2) != 0;
-    ip0.preload_tp = ip1.preload_tp = 0;
-    if (E.masked) op_iter2<S, true, T>(t, E.roles, ip0, ip1, L);
-    else op_iter2<S, false, T>(t, E.roles, ip0, ip1, L);
+    const int ntx = (t.w + B::TW - 1) / B::TW;
+    t.x0 = (local % ntx) * B::TW;
+    t.y0 = (local / ntx) * B::TH;
+    // step sizes of iterations it0 .. it0 + K - 1 of the warp (stereotgv.cu:449-477)
+    const float4 *sched = t.d->sched + E.it0;
+    IterParams ip0;
+    ip0.k1 = ip0.k0 = ip0.tau = ip0.mu = 0.f;  // set per iteration
+    ip0.lambda = A.lambda;
+    ip0.alpha0 = A.alpha0;
+    ip0.alpha1 = A.alpha1;
+    ip0.first = (E.flags & EF_FIRST) != 0;
+    ip0.write_tp = (E.flags & EF_WRITE_TP_LAST) != 0;  // the op's last iteration
+    ip0.preload_tp = 0;
+    if (E.masked) op_iterk<SH, K, true, T>(t, E.roles, sched, ip0, L);
+    else op_iterk<SH, K, false, T>(t, E.roles, sched, ip0, L);
 }
 
 // 13th smallest of 25 by the partial selection sort of medianFilter.cu:307-316.
@@ -1495,18 +1544,27 @@ void tick_storage(const PairDesc *desc, const TickArgs &args, int shape, hipStre
     default: tick_shape<T, 0>(desc, args, st); break;
     }
 }
-static_assert(kShapes == 7, "tick_storage dispatches every shape");
+static_assert(kTickShapes == 7, "tick_storage dispatches every tick shape");
 
-// Two-iteration launches exist for the one-row shapes (64x4, 32x8, 16x16), whose staged
-// regions fit the workgroup's LDS.
-template <typename T>
-void iter2_storage(const PairDesc *desc, const TickArgs &args, int shape, hipStream_t st)
+// K-iteration launches exist for the shapes of multi_iter_shape() (64x4, 32x16, 32x8,
+// 16x16, 8x8, 4x4, 32x32), whose staged regions fit the workgroup's LDS.
+template <typename T, int SH, int K>
+void iterk_shape(const PairDesc *desc, const TickArgs &args, hipStream_t st)
 {
-    const dim3 grid((unsigned)args.total);
+    k_iterk<T, SH, K><<<dim3((unsigned)args.total), dim3(BlkK<SH, K>::NT), 0, st>>>(desc, args);
+}
+
+template <typename T, int K>
+void iterk_storage(const PairDesc *desc, const TickArgs &args, int shape, hipStream_t st)
+{
     switch (shape) {
-    case 1: k_iter2<T, 1><<<grid, dim3(Sh<1>::BX, NT2 / Sh<1>::BX), 0, st>>>(desc, args); break;
-    case 4: k_iter2<T, 4><<<grid, dim3(Sh<4>::BX, NT2 / Sh<4>::BX), 0, st>>>(desc, args); break;
-    case 6: k_iter2<T, 6><<<grid, dim3(Sh<6>::BX, NT2 / Sh<6>::BX), 0, st>>>(desc, args); break;
+    case 1: iterk_shape<T, 1, K>(desc, args, st); break;
+    case 3: iterk_shape<T, 3, K>(desc, args, st); break;
+    case 4: iterk_shape<T, 4, K>(desc, args, st); break;
+    case 6: iterk_shape<T, 6, K>(desc, args, st); break;
+    case 7: iterk_shape<T, 7, K>(desc, args, st); break;
+    case 8: iterk_shape<T, 8, K>(desc, args, st); break;
+    case 9: iterk_shape<T, 9, K>(desc, args, st); break;
     default: break;  // the host never asks for another shape
     }
 }
@@ -1515,7 +1573,11 @@ void launch_tick(const PairDesc *desc, const TickArgs &args, int storage, int sh
 {
     if (args.n < 1 || args.total < 1) return;
     if (args.e[0].kind == OP_ITER2) {
-        STORAGE_DISPATCH(storage, iter2_storage<T>(desc, args, shape, st));
+        STORAGE_DISPATCH(storage, (iterk_storage<T, 2>(desc, args, shape, st)));
+        return;
+    }
+    if (args.e[0].kind == OP_ITER4) {
+        STORAGE_DISPATCH(storage, (iterk_storage<T, 4>(desc, args, shape, st)));
         return;
     }
     STORAGE_DISPATCH(storage, tick_storage<T>(desc, args, shape, st));

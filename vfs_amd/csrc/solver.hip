@@ -23,6 +23,7 @@
 #include <time.h>
 
 #include <algorithm>
+#include <array>
 #include <deque>
 #include <map>
 #include <mutex>
@@ -45,7 +46,7 @@ thread_local std::string g_create_error;
 struct GraphKey {
     int masked, warm_level, warps, iters, start_roles;
     float limit;
-    unsigned long long tuning;  // Tuning::sig() of the program's launch choices
+    std::array<unsigned short, kMaxLevels> tuning;  // Tuning::sig() of the program's launch choices
     bool operator<(const GraphKey &o) const
     {
         if (tuning != o.tuning) return tuning < o.tuning;
@@ -81,35 +82,58 @@ struct Params {
 struct ProgOp {
     unsigned char kind, level, masked, flags, shape;
     int roles;
-    float sigma, tau, mu, mu_next;
+    float sigma, tau, mu;  // OP_ITER
+    int it0;               // OP_ITER2 / OP_ITER4: first iteration's index in the warp
 };
 
-// Launch choices for one pyramid level: tile shape of every op at the level, whether the
-// inner iteration fetches its Tp-plane neighbours up front (vfs::EF_PRELOAD_TP) and
-// whether two iterations share one launch (vfs::OP_ITER2).
+// Launch choices for one pyramid level: tile shape of the level's one-iteration launches
+// (every op but the multi-iteration ones), whether the inner iteration fetches its
+// Tp-plane neighbours up front (vfs::EF_PRELOAD_TP), how many inner iterations share one
+// launch (1, 2 = vfs::OP_ITER2 or 4 = vfs::OP_ITER4) and the tile shape of those
+// multi-iteration launches.
 struct LevelTune {
-    unsigned char shape = 0, preload = 0, pair = 0;
+    unsigned char shape = 0, preload = 0, per_launch = 1, mshape = 4;
+    // tile shape of the inner iterations
+    int iter_shape() const { return per_launch > 1 ? mshape : shape; }
 };
 
 struct Tuning {
     LevelTune lv[kMaxLevels];
-    unsigned long long sig() const
+    std::array<unsigned short, kMaxLevels> sig() const
     {
-        unsigned long long k = 0;
-        for (int l = 0; l < 12; l++)
-            k |= (unsigned long long)(lv[l].shape | (lv[l].preload << 3) | (lv[l].pair << 4)) << (5 * l);
+        std::array<unsigned short, kMaxLevels> k{};
+        for (int l = 0; l < kMaxLevels; l++) {
+            const int code = lv[l].per_launch == 4 ? 2 : lv[l].per_launch == 2 ? 1 : 0;
+            k[l] = (unsigned short)(lv[l].shape | (lv[l].preload << 3) | (code << 4) | (lv[l].mshape << 6));
+        }
         return k;
     }
-    Tuning without_pairs() const
+    Tuning single_iterations() const
     {
         Tuning t = *this;
-        for (LevelTune &v : t.lv) v.pair = 0;
+        for (LevelTune &v : t.lv) v.per_launch = 1;
         return t;
     }
 };
 
-// Tile shapes with a two-iteration kernel (their staged regions fit a workgroup's LDS).
-bool pair_shape(int shape) { return shape == 1 || shape == 4 || shape == 6; }
+bool multi_shape(int shape) { return shape >= 0 && shape < vfs::kShapes && vfs::multi_iter_shape(shape); }
+bool valid_per_launch(int k) { return k == 1 || k == 2 || k == 4; }
+
+// Step sizes (sigma, tau, mu) of the first n inner iterations of a warp
+// (stereotgv.cu:449-477): tau = 1, sigma = 1 / tau at the start of every warp;
+// mu = 1 / sqrt(1 + 0.7 tau timeStepLambda) while sigma < 1000 (else 1), then
+// sigma /= mu and tau *= mu.
+void iter_schedule(float timestep_lambda, int n, float4 *out)
+{
+    float tau = 1.0f;
+    float sigma = 1.0f / tau;
+    for (int it = 0; it < n; it++) {
+        const float mu = sigma < 1000.0f ? 1.0f / sqrtf(1 + 0.7f * tau * timestep_lambda) : 1.0f;
+        out[it] = make_float4(sigma, tau, mu, 0.0f);
+        sigma = sigma / mu;
+        tau = tau * mu;
+    }
+}
 
 struct Program {
     std::vector<ProgOp> ops;
@@ -133,7 +157,7 @@ struct vfs_solver {
     bool use_graph = true;
     int force_shape = -1;   // option "shape": -1 = per-level tuner
     bool autotune = true;   // option "autotune"
-    int force_pair = -1;    // option "pair": -1 = tuner, 0 = never, 1 = wherever allowed
+    int force_per_launch = -1;  // option "per_launch": -1 = tuner, else 1, 2 or 4 wherever allowed
     bool mask_col0[kMaxLevels] = {};  // a mask pixel lies in column 0 of the level
     Tuning tuning[2];       // [masked] launch choices per level
     bool tuned[2] = {false, false};
@@ -184,6 +208,7 @@ struct vfs_solver {
             d.tv[l] = tv[l];
         }
         d.I0[0] = storage == vfs::kF32 ? (void *)in_i0 : (void *)i0p0;
+        iter_schedule(timestep_lambda, vfs::kMaxSched, d.sched);
         return d;
     }
 };
@@ -295,46 +320,39 @@ void push_op(Program &pr, int kind, int level, int masked, int roles)
 
 // One level of the solve from the tensor to the end of the last warp
 // (stereotgv.cu:409-494, masked; :275-364 unmasked).
-void push_level(Program &pr, const Params &P, int masked, int l, int warps, int iters, int &roles, bool pairs)
+void push_level(Program &pr, const Params &P, int masked, int l, int warps, int iters, int &roles, int per_launch)
 {
+    std::vector<float4> sched((size_t)std::max(iters, 1));
+    iter_schedule(P.timestep_lambda, iters, sched.data());
     push_op(pr, vfs::OP_TENSOR, l, masked, roles);
     for (int wi = 0; wi < warps; wi++) {
         push_op(pr, vfs::OP_PREP, l, masked, roles);
         push_op(pr, vfs::OP_DERIV, l, masked, roles);
-        float tau = 1.0f;
-        float sigma = 1.0f / tau;
-        auto mu_of = [&](float tau_, float sigma_) {
-            return sigma_ < 1000.0f ? 1.0f / sqrtf(1 + 0.7f * tau_ * P.timestep_lambda) : 1.0f;
-        };
         for (int it = 0; it < iters;) {
-            const float mu = mu_of(tau, sigma);
             ProgOp o{};
             o.level = (unsigned char)l;
             o.masked = (unsigned char)masked;
             o.roles = roles;
-            o.sigma = sigma;  // the kernels form alpha1 sigma / 3 and alpha0 sigma / 2
-            o.tau = tau;
-            o.mu = mu;
-            if (pairs && it + 1 < iters) {
-                // iterations it and it + 1 in one op; the kernel continues the schedule
-                const float sigma1 = sigma / mu, tau1 = tau * mu;
-                o.kind = vfs::OP_ITER2;
-                o.mu_next = mu_of(tau1, sigma1);
-                o.flags = (unsigned char)((it == 0 ? vfs::EF_FIRST : 0) | (it + 1 == iters - 1 ? vfs::EF_WRITE_TP2 : 0));
-                pr.ops.push_back(o);
+            // K iterations in one op while they fit the iteration count and the
+            // descriptors' schedule table; the rest one or two at a time
+            int k = 1;
+            for (int c : {4, 2})
+                if (k == 1 && c <= per_launch && it + c <= iters && it + c <= vfs::kMaxSched) k = c;
+            if (k > 1) {
+                o.kind = (unsigned char)(k == 4 ? vfs::OP_ITER4 : vfs::OP_ITER2);
+                o.it0 = it;
+                o.flags = (unsigned char)((it == 0 ? vfs::EF_FIRST : 0) | (it + k == iters ? vfs::EF_WRITE_TP_LAST : 0));
                 roles = vfs::rrotate_iter(roles);
-                sigma = sigma1 / o.mu_next;
-                tau = tau1 * o.mu_next;
-                it += 2;
             } else {
                 o.kind = vfs::OP_ITER;
+                o.sigma = sched[it].x;  // the kernels form alpha1 sigma / 3 and alpha0 sigma / 2
+                o.tau = sched[it].y;
+                o.mu = sched[it].z;
                 o.flags = (unsigned char)((it == 0 ? vfs::EF_FIRST : 0) | (it == iters - 1 ? vfs::EF_WRITE_TP : 0));
-                pr.ops.push_back(o);
                 roles = vfs::rswap_iter(roles);
-                sigma = sigma / mu;
-                tau = tau * mu;
-                it += 1;
             }
+            pr.ops.push_back(o);
+            it += k;
         }
         push_op(pr, vfs::OP_END, l, masked, roles);
         if (masked) roles = vfs::rswap(roles, vfs::G_U);
@@ -345,11 +363,13 @@ void push_level(Program &pr, const Params &P, int masked, int l, int warps, int 
 void apply_tuning(Program &pr, const Tuning &tn)
 {
     for (ProgOp &o : pr.ops) {
-        o.shape = tn.lv[vfs::op_tile_level(o.kind, o.level)].shape;
+        const LevelTune &lt = tn.lv[vfs::op_tile_level(o.kind, o.level)];
+        const bool multi = o.kind == vfs::OP_ITER2 || o.kind == vfs::OP_ITER4;
+        o.shape = multi ? lt.mshape : lt.shape;
         o.flags &= (unsigned char)~vfs::EF_PRELOAD_TP;
-        if (o.kind == vfs::OP_ITER && tn.lv[o.level].preload) o.flags |= vfs::EF_PRELOAD_TP;
-        // a level's launches take one shape; two-iteration ops exist for some shapes only
-        if (o.kind == vfs::OP_ITER2 && !pair_shape(o.shape)) o.shape = 4;
+        if (o.kind == vfs::OP_ITER && lt.preload) o.flags |= vfs::EF_PRELOAD_TP;
+        // multi-iteration ops exist for some shapes only
+        if (multi && !multi_shape(o.shape)) o.shape = 4;
     }
 }
 
@@ -369,7 +389,7 @@ Program build_program(const Params &P, int masked, int warm_level, int warps, in
         for (int l = 1; l < P.nLevels; l++) push_op(pr, vfs::OP_DOWN, l, masked, 0);
         for (int l = P.nLevels - 1; l >= 0; l--) {
             if (l == P.nLevels - 1) push_op(pr, vfs::OP_CINIT, l, masked, roles);
-            push_level(pr, P, masked, l, P.nWarpIters, P.nSolverIters, roles, tn.lv[l].pair != 0);
+            push_level(pr, P, masked, l, P.nWarpIters, P.nSolverIters, roles, tn.lv[l].per_launch);
             if (l > 0) {
                 push_op(pr, vfs::OP_UP, l, masked, roles);
                 roles = vfs::rswap(vfs::rswap(roles, vfs::G_U), vfs::G_UB) ^ vfs::RB_UV;
@@ -384,7 +404,7 @@ Program build_program(const Params &P, int masked, int warm_level, int warps, in
             roles = vfs::rswap(vfs::rswap(roles, vfs::G_U), vfs::G_UB) ^ vfs::RB_UV;
         }
         for (int l = warm_level; l >= 0; l--) {
-            push_level(pr, P, masked, l, warps, iters, roles, tn.lv[l].pair != 0);
+            push_level(pr, P, masked, l, warps, iters, roles, tn.lv[l].per_launch);
             if (l > 0) {
                 push_op(pr, vfs::OP_UP, l, masked, roles);
                 roles = vfs::rswap(vfs::rswap(roles, vfs::G_U), vfs::G_UB) ^ vfs::RB_UV;
@@ -431,7 +451,7 @@ void set_entry(vfs::TickEntry &e, const ProgOp &o, int pair, const vfs::PairDesc
     e.sigma = o.sigma;
     e.tau = o.tau;
     e.mu = o.mu;
-    e.mu_next = o.mu_next;
+    e.it0 = o.it0;
 }
 
 // Issues every op of `pr` for every pair of the launch set: one k_tick launch per op per
@@ -571,11 +591,11 @@ int ensure_engine(vfs_solver *s)
 // same storage, solve form and level geometry.  Results do not depend on the choice: the
 // kernels compute every pixel identically in every shape.
 struct TuneKey {
-    int device, storage, masked, w, h, s, pair_ok;
+    int device, storage, masked, w, h, s, multi_ok;
     bool operator<(const TuneKey &o) const
     {
-        const int a[7] = {device, storage, masked, w, h, s, pair_ok};
-        const int b[7] = {o.device, o.storage, o.masked, o.w, o.h, o.s, o.pair_ok};
+        const int a[7] = {device, storage, masked, w, h, s, multi_ok};
+        const int b[7] = {o.device, o.storage, o.masked, o.w, o.h, o.s, o.multi_ok};
         return std::lexicographical_compare(a, a + 7, b, b + 7);
     }
 };
@@ -597,12 +617,23 @@ LevelTune default_tune(int level)
 // descriptor array of at least descs.size() entries) and stores the fastest choices in tn.
 hipError_t time_launch_choices(hipStream_t st, const vfs::PairDesc *d_desc, const std::vector<vfs::PairDesc> &descs,
                                const Params &P, int masked, const std::vector<int> &levels,
-                               const std::vector<bool> &allow_pair, Tuning &tn)
+                               const std::vector<bool> &allow_multi, Tuning &tn)
 {
     constexpr int kWarm = 3, kReps = 20, kRounds = 2;
-    // candidates: (shape, preload) of one iteration per launch, then two-iteration
-    // launches for the shapes that have them
-    constexpr int kSingle = vfs::kShapes * 2, kCand = kSingle + vfs::kShapes;
+    // candidates: (shape, preload) of one iteration per launch, then two- and
+    // four-iteration launches for the shapes that have them
+    constexpr int kSingle = vfs::kTickShapes * 2, kCand = kSingle + 2 * vfs::kShapes;
+    auto cand = [&](int c, int &shape, int &pre, int &per_launch) {
+        if (c < kSingle) {
+            shape = c / 2;
+            pre = c % 2;
+            per_launch = 1;
+        } else {
+            shape = (c - kSingle) % vfs::kShapes;
+            pre = 0;
+            per_launch = c - kSingle < vfs::kShapes ? 2 : 4;
+        }
+    };
     hipEvent_t e0 = nullptr, e1 = nullptr;
     hipError_t r = hipEventCreate(&e0);
     if (r == hipSuccess) r = hipEventCreate(&e1);
@@ -611,11 +642,12 @@ hipError_t time_launch_choices(hipStream_t st, const vfs::PairDesc *d_desc, cons
     for (int round = 0; round < kRounds && r == hipSuccess; round++) {
         for (size_t li = 0; li < levels.size() && r == hipSuccess; li++) {
             for (int c = 0; c < kCand && r == hipSuccess; c++) {
-                const bool pair = c >= kSingle;
-                const int shape = pair ? c - kSingle : c / 2, pre = pair ? 0 : c % 2;
-                if (pair && (!pair_shape(shape) || !allow_pair[li])) continue;
+                int shape, pre, per_launch;
+                cand(c, shape, pre, per_launch);
+                if (per_launch > 1 && (!multi_shape(shape) || !allow_multi[li]))
+                    continue;
                 ProgOp o{};
-                o.kind = pair ? vfs::OP_ITER2 : vfs::OP_ITER;
+                o.kind = (unsigned char)(per_launch == 4 ? vfs::OP_ITER4 : per_launch == 2 ? vfs::OP_ITER2 : vfs::OP_ITER);
                 o.level = (unsigned char)levels[li];
                 o.masked = (unsigned char)masked;
                 o.flags = (unsigned char)(pre ? vfs::EF_PRELOAD_TP : 0);
@@ -624,7 +656,7 @@ hipError_t time_launch_choices(hipStream_t st, const vfs::PairDesc *d_desc, cons
                 o.sigma = 1.0f;
                 o.tau = 1.0f;
                 o.mu = 0.8f;
-                o.mu_next = 0.8f;
+                o.it0 = 0;
                 vfs::TickArgs a = tick_header(P);
                 int acc = 0;
                 for (int k = 0; k < n; k++) {
@@ -643,7 +675,7 @@ hipError_t time_launch_choices(hipStream_t st, const vfs::PairDesc *d_desc, cons
                 if (r == hipSuccess) r = hipEventElapsedTime(&ms, e0, e1);
                 if (r == hipSuccess) r = hipGetLastError();
                 float &b = best[li * kCand + c];
-                b = std::min(b, pair ? ms / 2 : ms);
+                b = std::min(b, ms / (float)per_launch);
             }
         }
     }
@@ -655,14 +687,25 @@ hipError_t time_launch_choices(hipStream_t st, const vfs::PairDesc *d_desc, cons
         const int l = levels[li];
         const LevelTune def = default_tune(l);
         LevelTune pick = def;
+        // the one-iteration launch (also the shape of the level's other ops), then a
+        // multi-iteration launch where it is faster still; each leaves the current
+        // choice only for a clear gain
         float pick_ms = best[li * kCand + def.shape * 2 + def.preload];
-        for (int c = 0; c < kCand; c++) {
+        for (int c = 0; c < kSingle; c++) {
             const float ms = best[li * kCand + c];
-            if (ms < 0.98f * pick_ms) {  // leave the default only for a clear gain
-                const bool pair = c >= kSingle;
-                pick.shape = (unsigned char)(pair ? c - kSingle : c / 2);
-                pick.preload = (unsigned char)(pair ? 0 : c % 2);
-                pick.pair = (unsigned char)pair;
+            if (ms < 0.98f * pick_ms) {
+                pick.shape = (unsigned char)(c / 2);
+                pick.preload = (unsigned char)(c % 2);
+                pick_ms = ms;
+            }
+        }
+        for (int c = kSingle; c < kCand; c++) {
+            const float ms = best[li * kCand + c];
+            if (ms < 0.98f * pick_ms) {
+                int shape, pre, per_launch;
+                cand(c, shape, pre, per_launch);
+                pick.mshape = (unsigned char)shape;
+                pick.per_launch = (unsigned char)per_launch;
                 pick_ms = ms;
             }
         }
@@ -671,9 +714,9 @@ hipError_t time_launch_choices(hipStream_t st, const vfs::PairDesc *d_desc, cons
     return hipSuccess;
 }
 
-// Levels where two iterations may share a launch: at least 2 columns and, in the masked
-// form, no mask pixel in column 0.
-std::vector<bool> pair_allowed(const vfs_solver *s, int masked, const std::vector<int> &levels)
+// Levels where several iterations may share a launch: at least 2 columns and, in the
+// masked form, no mask pixel in column 0.
+std::vector<bool> multi_allowed(const vfs_solver *s, int masked, const std::vector<int> &levels)
 {
     std::vector<bool> ok;
     for (int l : levels) ok.push_back(s->lv[l].w >= 2 && !(masked && s->mask_col0[l]));
@@ -689,7 +732,7 @@ int measure_levels(vfs_solver *s, int masked, const std::vector<int> &levels, Tu
     std::vector<vfs::PairDesc> d{s->desc()};
     d[0].arena = scratch;
     const hipError_t r = time_launch_choices(s->stream, s->eng.d_desc, d, s->params(), masked, levels,
-                                             pair_allowed(s, masked, levels), tn);
+                                             multi_allowed(s, masked, levels), tn);
     (void)hipStreamSynchronize(s->stream);
     (void)hipFree(scratch);
     if (r != hipSuccess) return fail(s, VFS_ERR_DEVICE, "launch tuner: %s", hipGetErrorString(r));
@@ -704,11 +747,14 @@ int tune_solver(vfs_solver *s, int masked)
     Tuning &tn = s->tuning[m];
     for (int l = 0; l < kMaxLevels; l++) tn.lv[l] = default_tune(l);
     if (s->force_shape >= 0) {
-        for (int l = 0; l < s->nLevels; l++) tn.lv[l].shape = (unsigned char)s->force_shape;
+        for (int l = 0; l < s->nLevels; l++) {
+            tn.lv[l].shape = (unsigned char)s->force_shape;
+            tn.lv[l].mshape = (unsigned char)(multi_shape(s->force_shape) ? s->force_shape : 4);
+        }
     } else if (s->autotune) {
         std::vector<int> todo;
         auto key = [&](int l) {
-            const int ok = pair_allowed(s, m, std::vector<int>{l})[0] ? 1 : 0;
+            const int ok = multi_allowed(s, m, std::vector<int>{l})[0] ? 1 : 0;
             return TuneKey{s->device, s->storage, m, s->lv[l].w, s->lv[l].h, s->lv[l].s, ok};
         };
         {
@@ -727,14 +773,14 @@ int tune_solver(vfs_solver *s, int masked)
             for (int l : todo) g_tune[key(l)] = tn.lv[l];
         }
     }
-    // option "pair": 0 = one iteration per launch everywhere, 1 = two wherever allowed
-    if (s->force_pair >= 0) {
+    // option "per_launch": iterations per launch wherever allowed (1 everywhere else)
+    if (s->force_per_launch >= 0) {
         std::vector<int> all;
         for (int l = 0; l < s->nLevels; l++) all.push_back(l);
-        const std::vector<bool> ok = pair_allowed(s, m, all);
+        const std::vector<bool> ok = multi_allowed(s, m, all);
         for (int l = 0; l < s->nLevels; l++) {
-            tn.lv[l].pair = (unsigned char)(s->force_pair == 1 && ok[l]);
-            if (tn.lv[l].pair && !pair_shape(tn.lv[l].shape)) tn.lv[l].shape = 4;
+            tn.lv[l].per_launch = (unsigned char)(ok[l] ? s->force_per_launch : 1);
+            if (!multi_shape(tn.lv[l].mshape)) tn.lv[l].mshape = 4;
         }
     }
     s->tuned[m] = true;
@@ -978,7 +1024,7 @@ int vfs_copy_mask(vfs_solver *s, const float *m, int pitch)
     for (int l = 1; l < s->nLevels; l++)
         vfs::launch_downscale_nn(s->mask[l - 1], s->lv[l - 1], s->mask[l], s->lv[l], s->stream);
     if ((rc = check_launch(s))) return rc;
-    // column 0 of every mask level (two-iteration launches need it empty, see op_iter2)
+    // column 0 of every mask level (multi-iteration launches need it empty, see op_iterk)
     for (int l = 0; l < s->nLevels; l++) {
         std::vector<float> col(s->lv[l].h);
         HIPCHK(s, hipMemcpy2DAsync(col.data(), sizeof(float), s->mask[l], sizeof(float) * (size_t)s->lv[l].s,
@@ -1205,7 +1251,8 @@ int vfs_set_option(vfs_solver *s, const char *key, int value)
         return VFS_OK;
     }
     if (!strcmp(key, "shape")) {
-        if (value < -1 || value >= vfs::kShapes) return fail(s, VFS_ERR_ARG, "shape must be -1 .. %d", vfs::kShapes - 1);
+        if (value < -1 || value >= vfs::kTickShapes)
+            return fail(s, VFS_ERR_ARG, "shape must be -1 .. %d", vfs::kTickShapes - 1);
         s->force_shape = value;
         s->tuned[0] = s->tuned[1] = false;
         return VFS_OK;
@@ -1215,9 +1262,14 @@ int vfs_set_option(vfs_solver *s, const char *key, int value)
         s->tuned[0] = s->tuned[1] = false;
         return VFS_OK;
     }
-    if (!strcmp(key, "pair")) {
-        if (value < -1 || value > 1) return fail(s, VFS_ERR_ARG, "pair must be -1, 0 or 1");
-        s->force_pair = value;
+    if (!strcmp(key, "per_launch") || !strcmp(key, "pair")) {
+        // "pair": -1 = tuner, 0 = one iteration per launch, 1 = two
+        if (!strcmp(key, "pair")) {
+            if (value < -1 || value > 1) return fail(s, VFS_ERR_ARG, "pair must be -1, 0 or 1");
+            value = value < 0 ? -1 : value + 1;
+        }
+        if (value != -1 && !valid_per_launch(value)) return fail(s, VFS_ERR_ARG, "per_launch must be -1, 1, 2 or 4");
+        s->force_per_launch = value;
         s->tuned[0] = s->tuned[1] = false;
         return VFS_OK;
     }
@@ -1231,12 +1283,13 @@ int vfs_get_option(const vfs_solver *s, const char *key, int *value)
     else if (!strcmp(key, "storage")) *value = s->storage;
     else if (!strcmp(key, "shape")) *value = s->force_shape;
     else if (!strcmp(key, "autotune")) *value = s->autotune ? 1 : 0;
-    else if (!strcmp(key, "pair")) *value = s->force_pair;
+    else if (!strcmp(key, "per_launch")) *value = s->force_per_launch;
+    else if (!strcmp(key, "pair")) *value = s->force_per_launch < 0 ? -1 : s->force_per_launch > 1 ? 1 : 0;
     else return VFS_ERR_ARG;
     return VFS_OK;
 }
 
-int vfs_get_level_tuning(vfs_solver *s, int masked, int level, int *shape, int *preload, int *pair)
+int vfs_get_level_tuning(vfs_solver *s, int masked, int level, int *shape, int *preload, int *per_launch)
 {
     if (!s || !shape || !preload) return VFS_ERR_ARG;
     if (!s->initialized || level < 0 || level >= s->nLevels) return fail(s, VFS_ERR_ARG, "bad level");
@@ -1244,28 +1297,33 @@ int vfs_get_level_tuning(vfs_solver *s, int masked, int level, int *shape, int *
     if (!rc) rc = tune_solver(s, masked);
     if (rc) return rc;
     const LevelTune t = s->tuning[masked ? 1 : 0].lv[level];
-    *shape = t.shape;
+    *shape = t.iter_shape();
     *preload = t.preload;
-    if (pair) *pair = t.pair;
+    if (per_launch) *per_launch = t.per_launch;
     return VFS_OK;
 }
 
-int vfs_set_level_tuning(vfs_solver *s, int masked, int level, int shape, int preload, int pair)
+int vfs_set_level_tuning(vfs_solver *s, int masked, int level, int shape, int preload, int per_launch)
 {
     if (!s) return VFS_ERR_ARG;
     if (!s->initialized || level < 0 || level >= s->nLevels) return fail(s, VFS_ERR_ARG, "bad level");
-    if (shape < 0 || shape >= vfs::kShapes || (pair && !pair_shape(shape)))
-        return fail(s, VFS_ERR_ARG, "bad shape %d (two iterations per launch need 64x4, 32x8 or 16x16)", shape);
+    if (!valid_per_launch(per_launch)) return fail(s, VFS_ERR_ARG, "per_launch must be 1, 2 or 4");
+    if (per_launch > 1 ? !multi_shape(shape) : (shape < 0 || shape >= vfs::kTickShapes))
+        return fail(s, VFS_ERR_ARG, "bad shape %d for %d iteration(s) per launch", shape, per_launch);
     const int m = masked ? 1 : 0;
     int rc = set_device(s);
     if (!rc && !s->tuned[m]) rc = tune_solver(s, m);  // the other levels keep their choices
     if (rc) return rc;
-    if (pair && !pair_allowed(s, m, std::vector<int>{level})[0])
-        return fail(s, VFS_ERR_ARG, "two iterations per launch are not allowed at level %d", level);
+    if (per_launch > 1 && !multi_allowed(s, m, std::vector<int>{level})[0])
+        return fail(s, VFS_ERR_ARG, "several iterations per launch are not allowed at level %d", level);
     LevelTune &t = s->tuning[m].lv[level];
-    t.shape = (unsigned char)shape;
-    t.preload = (unsigned char)(preload != 0);
-    t.pair = (unsigned char)(pair != 0);
+    if (per_launch > 1) {
+        t.mshape = (unsigned char)shape;
+    } else {
+        t.shape = (unsigned char)shape;
+        t.preload = (unsigned char)(preload != 0);
+    }
+    t.per_launch = (unsigned char)per_launch;
     return VFS_OK;
 }
 
@@ -1301,25 +1359,27 @@ int vfs_time_inner_iterations(vfs_solver *s, int level, int reps, float *ms_tota
     const Params P = s->params();
     Program pr;
     int roles = vfs::kRolesInit;
-    const bool pairs = s->tuning[1].lv[level].pair != 0;
-    if (pairs) reps += reps & 1;
-    // `reps` iterations of the 50-iteration schedule, restarted every 50
+    const int per_launch = s->tuning[1].lv[level].per_launch;
+    // at least `reps` iterations of the 48-iteration schedule, restarted every 48, in
+    // whole ops
     {
         Params Q = P;
-        Q.nSolverIters = 50;
+        Q.nSolverIters = 48;
         Program w;
         int done = 0;
         while (done < reps) {
             w.ops.clear();
-            push_level(w, Q, 1, level, 1, 50, roles, pairs);
+            push_level(w, Q, 1, level, 1, 48, roles, per_launch);
             for (const ProgOp &o : w.ops) {
                 if (done >= reps) break;
-                if (o.kind == vfs::OP_ITER || o.kind == vfs::OP_ITER2) {
+                const int k = vfs::op_iterations(o.kind);
+                if (k > 0) {
                     pr.ops.push_back(o);
-                    done += o.kind == vfs::OP_ITER2 ? 2 : 1;
+                    done += k;
                 }
             }
         }
+        reps = done;
         apply_tuning(pr, s->tuning[1]);
     }
     hipEvent_t e0, e1;
@@ -1334,7 +1394,7 @@ int vfs_time_inner_iterations(vfs_solver *s, int level, int reps, float *ms_tota
     if ((rc = check_launch(s))) return rc;
     HIPCHK(s, hipEventRecord(s->done, s->stream));
     *ms_total = ms * (float)reps0 / (float)reps;
-    if (iters_per_launch) *iters_per_launch = pairs ? 2 : 1;
+    if (iters_per_launch) *iters_per_launch = per_launch;
     return VFS_OK;
 }
 
@@ -1427,7 +1487,10 @@ static int tune_batch(vfs_batch *b, int masked)
     Tuning &tn = b->tuning[m];
     for (int l = 0; l < kMaxLevels; l++) tn.lv[l] = default_tune(l);
     if (f->force_shape >= 0) {
-        for (int l = 0; l < f->nLevels; l++) tn.lv[l].shape = (unsigned char)f->force_shape;
+        for (int l = 0; l < f->nLevels; l++) {
+            tn.lv[l].shape = (unsigned char)f->force_shape;
+            tn.lv[l].mshape = (unsigned char)(multi_shape(f->force_shape) ? f->force_shape : 4);
+        }
     } else if (f->autotune) {
         std::vector<const vfs_solver *> order(b->members.begin(), b->members.end());
         std::stable_sort(order.begin(), order.end(), [](const vfs_solver *x, const vfs_solver *y) {
@@ -1449,10 +1512,10 @@ static int tune_batch(vfs_batch *b, int masked)
         }
         std::vector<int> levels;
         for (int l = 0; l < f->nLevels; l++) levels.push_back(l);
-        // two-iteration launches only where every member allows them
+        // multi-iteration launches only where every member allows them
         std::vector<bool> ok(levels.size(), true);
         for (const vfs_solver *s : b->members) {
-            const std::vector<bool> o = pair_allowed(s, m, levels);
+            const std::vector<bool> o = multi_allowed(s, m, levels);
             for (size_t i = 0; i < ok.size(); i++) ok[i] = ok[i] && o[i];
         }
         if (r == hipSuccess)
@@ -1461,17 +1524,17 @@ static int tune_batch(vfs_batch *b, int masked)
         for (float *p : scratch) (void)hipFree(p);
         if (r != hipSuccess) return failb(b, VFS_ERR_DEVICE, "batch launch tuner: %s", hipGetErrorString(r));
     }
-    if (f->force_pair >= 0) {
+    if (f->force_per_launch >= 0) {
         std::vector<int> all;
         for (int l = 0; l < f->nLevels; l++) all.push_back(l);
         std::vector<bool> ok(all.size(), true);
         for (const vfs_solver *s : b->members) {
-            const std::vector<bool> o = pair_allowed(s, m, all);
+            const std::vector<bool> o = multi_allowed(s, m, all);
             for (size_t i = 0; i < ok.size(); i++) ok[i] = ok[i] && o[i];
         }
         for (int l = 0; l < f->nLevels; l++) {
-            tn.lv[l].pair = (unsigned char)(f->force_pair == 1 && ok[l]);
-            if (tn.lv[l].pair && !pair_shape(tn.lv[l].shape)) tn.lv[l].shape = 4;
+            tn.lv[l].per_launch = (unsigned char)(ok[l] ? f->force_per_launch : 1);
+            if (!multi_shape(tn.lv[l].mshape)) tn.lv[l].mshape = 4;
         }
     }
     b->tuned[m] = true;
@@ -1539,15 +1602,15 @@ int vfs_batch_solve_warm(vfs_batch *b, int masked, int start_level, int warps, i
     return batch_solve(b, masked, start_level, warps, iters);
 }
 
-int vfs_batch_get_level_tuning(vfs_batch *b, int masked, int level, int *shape, int *preload, int *pair)
+int vfs_batch_get_level_tuning(vfs_batch *b, int masked, int level, int *shape, int *preload, int *per_launch)
 {
     if (!b || !shape || !preload) return VFS_ERR_ARG;
     if (level < 0 || level >= b->members[0]->nLevels) return failb(b, VFS_ERR_ARG, "bad level");
     if (!b->tuned[masked ? 1 : 0]) return failb(b, VFS_ERR_STATE, "the batch has not been solved in this form yet");
     const LevelTune t = b->tuning[masked ? 1 : 0].lv[level];
-    *shape = t.shape;
+    *shape = t.iter_shape();
     *preload = t.preload;
-    if (pair) *pair = t.pair;
+    if (per_launch) *per_launch = t.per_launch;
     return VFS_OK;
 }
 
@@ -1833,7 +1896,7 @@ int admit(vfs_server *v)
         P.nSolverIters = sp.nSolverIters;
         r->warm = sp.warm_ok ? 1 : 0;
         // serving launches mix ops of different frames: one iteration per launch
-        const Tuning tn = s->tuning[1].without_pairs();
+        const Tuning tn = s->tuning[1].single_iterations();
         r->prog = sp.warm_ok ? build_program(P, 1, sp.warm_level, sp.warm_warps, sp.warm_iters, s->roles, tn)
                              : build_program(P, 1, -1, 0, 0, 0, tn);
         r->pc = 0;

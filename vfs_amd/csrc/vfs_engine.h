@@ -15,15 +15,26 @@
 namespace vfs {
 
 constexpr int kMaxLevels = 16;
+// Inner-iteration steps of a warp held in the schedule table of a pair descriptor; the
+// multi-iteration ops run only within it.
+constexpr int kMaxSched = 128;
 
-// Tile shapes of the tick kernel.  A workgroup is 256 work-items laid out tw x (256 / tw);
-// each work-item covers th / (256 / tw) rows (1 or 2) of the tw x th tile.  Shape 0 is the
-// default; the others are candidates of the per-level tile-shape tuner.
-constexpr int kShapes = 7;
+// Tile shapes.  Shapes 0 .. kTickShapes - 1 are those of the tick kernel: a workgroup is
+// 256 work-items laid out tw x (256 / tw); each work-item covers th / (256 / tw) rows (1
+// or 2) of the tw x th tile.  Shape 0 is the default; the others are candidates of the
+// per-level tile-shape tuner.  The multi-iteration ops (OP_ITER2, OP_ITER4) take the
+// shapes of multi_iter_shape(), including the small tiles 8x8 and 4x4 for the coarsest,
+// latency-bound levels and 32x32 for the largest.
+constexpr int kShapes = 10, kTickShapes = 7;
 struct ShapeDim {
     int tw, th;
 };
-constexpr ShapeDim kShape[kShapes] = {{64, 8}, {64, 4}, {128, 4}, {32, 16}, {32, 8}, {16, 32}, {16, 16}};
+constexpr ShapeDim kShape[kShapes] = {{64, 8}, {64, 4}, {128, 4}, {32, 16}, {32, 8}, {16, 32}, {16, 16},
+                                      {8, 8}, {4, 4}, {32, 32}};
+__host__ __device__ constexpr bool multi_iter_shape(int shape)
+{
+    return shape == 1 || shape == 3 || shape == 4 || shape == 6 || shape == 7 || shape == 8 || shape == 9;
+}
 
 // Storage precision of the image pyramid, warped image, derivatives, tensor and duals.
 // Arithmetic is FP32 in every mode.
@@ -33,7 +44,7 @@ enum Storage : int { kF32 = 0, kF16 = 1, kBF16 = 2 };
 // planes take 2 / 4 consecutive slots.  Planes marked (st) hold the storage type.  The
 // state planes u, u_, v_ and the duals (p, q) form ping-pong groups of three buffers each:
 // the current state, the other state (written by the next iteration) and a spare that
-// the two-iteration op writes; warpUV has two buffers.
+// the multi-iteration ops write; warpUV has two buffers.
 enum Plane : int {
     PL_U = 0,                // u: 3 buffers
     PL_UB = 3,               // u_: 3 buffers
@@ -97,6 +108,9 @@ struct PairDesc {
     void *I1[kMaxLevels];
     const float *mask[kMaxLevels];
     const float2 *tv[kMaxLevels];
+    // step sizes (sigma, tau, mu, 0) of inner iteration k of every warp
+    // (stereotgv.cu:449-477); the multi-iteration ops read them
+    float4 sched[kMaxSched];
 };
 
 // Scalars of one inner iteration (schedule of stereotgv.cu:449-477).
@@ -127,8 +141,12 @@ enum OpKind : int {
     OP_ITER = 8,   // one fused TGV iteration
     OP_END = 9,    // [median +] limit + update at the end of a warp
     OP_UP = 10,    // upscale u, u_, warpUV from `level` to level-1
-    OP_ITER2 = 11  // two fused TGV iterations (recomputed 4-pixel halo)
+    OP_ITER2 = 11,  // two fused TGV iterations (recomputed halo)
+    OP_ITER4 = 12   // four fused TGV iterations (recomputed halo)
 };
+
+// Iterations of an inner-iteration op.
+inline int op_iterations(int kind) { return kind == OP_ITER4 ? 4 : kind == OP_ITER2 ? 2 : kind == OP_ITER ? 1 : 0; }
 
 // Entries per launch: one per lane of a wave64 (the kernel's entry lookup), and 64
 // entries of 56 bytes keep the kernel arguments under 4 KiB.
@@ -136,10 +154,10 @@ constexpr int kMaxEntries = 64;
 
 // Flags of a TickEntry.
 enum EntryFlag : int {
-    EF_FIRST = 1,       // ITER / ITER2: p and q were just zeroed
-    EF_WRITE_TP = 2,    // ITER / ITER2: the (first) iteration also stores Tp
-    EF_PRELOAD_TP = 4,  // ITER: fetch the Tp-plane neighbours with the other operands
-    EF_WRITE_TP2 = 8    // ITER2: the second iteration stores Tp
+    EF_FIRST = 1,          // inner-iteration ops: p and q were just zeroed
+    EF_WRITE_TP = 2,       // ITER: the iteration also stores Tp
+    EF_PRELOAD_TP = 4,     // ITER: fetch the Tp-plane neighbours with the other operands
+    EF_WRITE_TP_LAST = 8   // ITER2 / ITER4: the op's last iteration stores Tp
 };
 
 // One (pair, op) of a launch.  The geometry and arena of the op's tile level travel in
@@ -153,8 +171,8 @@ struct TickEntry {
     unsigned short w, h, s;    // tile level dims and row stride (OP_ZERO: level-0 stride x height)
     unsigned char kind, level, masked, flags;
     unsigned int roles;        // buffer roles before the op (see Group)
-    float sigma, tau, mu;      // ITER / ITER2: step sizes of the (first) iteration
-    float mu_next;             // ITER2: mu of the second iteration
+    float sigma, tau, mu;      // ITER: step sizes of the iteration
+    int it0;                   // ITER2 / ITER4: index of the first iteration in the warp
 };
 static_assert(sizeof(TickEntry) == 56, "TickEntry layout");
 

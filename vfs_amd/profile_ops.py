@@ -5,8 +5,11 @@ a graph (option "graph" = 0), in program order.  This module rebuilds the cold m
 program's op list (the same sequence the library issues: zero, calibration warp,
 pyramid, then per level the tensor and, per warp, prep / derivatives / inner iterations
 / end, then the upscale) and attributes each dispatch of one solve to its op and level.
+Levels that run two inner iterations per launch (`--pairs`, as the tuner reports them)
+issue `k_iter2` dispatches, ITER2, one per two iterations (plus one ITER when the
+iteration count is odd).
 
-    python -m vfs_amd.profile_ops TRACE.csv --levels 5 --warps 10 --iters 50 --skip-tail 100
+    python -m vfs_amd.profile_ops TRACE.csv --levels 5 --warps 10 --iters 50 --pairs 2,3,4 --skip-tail 100
 
 `--skip-tail` drops trailing dispatches that follow the solves (e.g. timing launches);
 the solve analysed is the last complete one before them.
@@ -18,26 +21,30 @@ import collections
 import csv
 
 
-def program_ops(levels: int, warps: int, iters: int) -> list[tuple[str, int]]:
+def program_ops(levels: int, warps: int, iters: int, pairs=()) -> list[tuple[str, int]]:
     ops = [("ZERO", 0), ("CALIB", 0)] + [("DOWN", l) for l in range(1, levels)] + [("CINIT", levels - 1)]
     for l in range(levels - 1, -1, -1):
         ops.append(("TENSOR", l))
+        inner = [("ITER2", l)] * (iters // 2) + [("ITER", l)] * (iters % 2) if l in pairs else [("ITER", l)] * iters
         for _ in range(warps):
-            ops += [("PREP", l), ("DERIV", l)] + [("ITER", l)] * iters + [("END", l)]
+            ops += [("PREP", l), ("DERIV", l)] + inner + [("END", l)]
         if l > 0:
             ops.append(("UP", l))
     return ops
 
 
-def summarize(trace_csv: str, levels: int, warps: int, iters: int, skip_tail: int = 0) -> str:
-    rows = [r for r in csv.DictReader(open(trace_csv)) if r["Kernel_Name"].startswith("k_tick")]
+def summarize(trace_csv: str, levels: int, warps: int, iters: int, skip_tail: int = 0, pairs=()) -> str:
+    rows = [r for r in csv.DictReader(open(trace_csv)) if r["Kernel_Name"].startswith(("k_tick", "k_iter2"))]
     rows.sort(key=lambda r: int(r["Start_Timestamp"]))
-    seq = program_ops(levels, warps, iters)
+    seq = program_ops(levels, warps, iters, pairs)
     n = len(seq)
     end = len(rows) - skip_tail
     if end < n:
         raise ValueError(f"trace holds {end} tick dispatches, one solve needs {n}")
     solve = rows[end - n:end]
+    for (kind, lvl), r in zip(seq, solve):
+        if (kind == "ITER2") != r["Kernel_Name"].startswith("k_iter2"):
+            raise ValueError(f"dispatch {r['Kernel_Name']} does not match op {kind} at level {lvl}: check --pairs")
     agg = collections.OrderedDict()
     for (kind, lvl), r in zip(seq, solve):
         d = (int(r["End_Timestamp"]) - int(r["Start_Timestamp"])) * 1e-3
@@ -67,8 +74,10 @@ def main():
     ap.add_argument("--warps", type=int, default=10)
     ap.add_argument("--iters", type=int, default=50)
     ap.add_argument("--skip-tail", type=int, default=0)
+    ap.add_argument("--pairs", default="", help="comma-separated levels with two iterations per launch")
     a = ap.parse_args()
-    print(summarize(a.trace, a.levels, a.warps, a.iters, a.skip_tail))
+    pairs = {int(x) for x in a.pairs.split(",") if x}
+    print(summarize(a.trace, a.levels, a.warps, a.iters, a.skip_tail, pairs))
 
 
 if __name__ == "__main__":
