@@ -58,8 +58,8 @@ def main():
     ap.add_argument("--shape", type=int, default=-1,
                     help="force one tile shape (0..6) instead of the per-level tuner (profiling aid)")
     ap.add_argument("--tuning", default="",
-                    help="per-level launch choices 'shape[p][xK],...' from level 0 (profiling aid): shape index 0..6, "
-                         "p = Tp prefetch, x2 / x4 = two / four inner iterations per launch")
+                    help="per-level launch choices 'shape[p][xK[h]],...' from level 0 (profiling aid): shape index, "
+                         "p = Tp prefetch, x2 / x4 = two / four inner iterations per launch, h = half workgroups")
     ap.add_argument("--iter-only", type=int, default=0,
                     help="profiling aid: run N level-0 inner iterations (after a 256 MiB reference copy) and exit")
     args = ap.parse_args()
@@ -91,8 +91,10 @@ def main():
     g.loadVectorFields(pair["tv"], pair["cv"])
     g.copyImagesToDevice(pair["i0"], pair["i1"])
     for lvl, spec in enumerate(c for c in args.tuning.split(",") if c):
-        g.set_level_tuning(lvl, int(spec[0]), preload="p" in spec,
-                           per_launch=int(spec.split("x")[1]) if "x" in spec else 1)
+        sh = int(spec.split("x")[0].rstrip("p"))
+        k = int(spec.split("x")[1].rstrip("h")) if "x" in spec else 1
+        g.set_level_tuning(lvl, sh, preload=spec.split("x")[0].endswith("p"), per_launch=k,
+                           half_workgroup=spec.endswith("h"))
 
     if args.iter_only:
         # A plain device copy of known size (256 MiB read + 256 MiB written per dispatch)

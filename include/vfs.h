@@ -161,7 +161,11 @@ int vfs_get_option(const vfs_solver *s, const char *key, int *value);
  * No counterpart in the reference. */
 int vfs_get_level_tuning(vfs_solver *s, int masked, int level, int *shape, int *preload, int *per_launch);
 /* Sets the launch choices of one level (reproducible profiling, expert use); they hold
- * until the next initialize or option change.  No counterpart in the reference. */
+ * until the next initialize or option change.  `per_launch` (1, 2 or 4) may carry
+ * VFS_HALF_WORKGROUP: multi-iteration workgroups of 512 work-items, two positions each,
+ * where the tile's staged region holds 513 .. 1024 positions.
+ * No counterpart in the reference. */
+#define VFS_HALF_WORKGROUP 0x100
 int vfs_set_level_tuning(vfs_solver *s, int masked, int level, int shape, int preload, int per_launch);
 
 /* Timing with HIP events on the solver's stream: runs `n` solves (masked != 0:

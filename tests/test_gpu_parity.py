@@ -218,25 +218,25 @@ def test_several_iterations_per_launch_give_identical_results(masked, shape, sto
 
 @pytest.mark.parametrize("masked", [True, False])
 def test_small_tiles_several_iterations_give_identical_results(masked):
-    """The 8x8, 4x4 and 32x32 tiles of the multi-iteration launches (set per level)
-    reproduce one iteration per launch bit for bit."""
+    """The 8x8, 4x4, 32x32 and 20x20 tiles of the multi-iteration launches and the half
+    workgroups (set per level) reproduce one iteration per launch bit for bit."""
     pair = small_pair(200, 120, seed=41)
     g = make_gpu(pair, 3, 2, 11)
     g.set_option("per_launch", 1)
     (g.solveStereoForwardMasked if masked else g.solveStereoForward)()
     ref = {n: g.read_plane(n, 0) for n in ("u", "u_", "v_", "warpUV")}
-    for shape in (7, 8, 9):
-        for per_launch in (2, 4):
-            g = make_gpu(pair, 3, 2, 11)
-            print(f"small tiles: shape {shape} per_launch {per_launch} masked {masked}", flush=True)
-            for lvl in range(3):
-                g.set_level_tuning(lvl, shape, per_launch=per_launch, masked=masked)
-            (g.solveStereoForwardMasked if masked else g.solveStereoForward)()
-            for lvl in range(3):
-                assert g.level_tuning(lvl, masked, with_per_launch=True)[::2] == (StereoTgv.SHAPES[shape], per_launch)
-            for n, v in ref.items():
-                got = g.read_plane(n, 0)
-                assert np.array_equal(got, v, equal_nan=True), (shape, per_launch, n)
+    cases = [(sh, k, False) for sh in (7, 8, 9, 10) for k in (2, 4)] + [(10, 4, True), (6, 4, True), (4, 2, True)]
+    for shape, per_launch, half in cases:
+        g = make_gpu(pair, 3, 2, 11)
+        print(f"tiles: shape {shape} per_launch {per_launch} half {half} masked {masked}", flush=True)
+        for lvl in range(3):
+            g.set_level_tuning(lvl, shape, per_launch=per_launch, masked=masked, half_workgroup=half)
+        (g.solveStereoForwardMasked if masked else g.solveStereoForward)()
+        for lvl in range(3):
+            assert g.level_tuning(lvl, masked, with_per_launch=True)[::2] == (StereoTgv.SHAPES[shape], per_launch)
+        for n, v in ref.items():
+            got = g.read_plane(n, 0)
+            assert np.array_equal(got, v, equal_nan=True), (shape, per_launch, half, n)
 
 
 def test_several_iterations_skip_levels_with_mask_in_column_0():

@@ -284,9 +284,10 @@ class StereoTgv:
         self._check(self._L.vfs_get_option(self._h, key.encode(), ctypes.byref(v)), "get_option")
         return v.value
 
-    # tile shapes by index: 0 .. 6 are those of the "shape" option; 8x8, 4x4 and 32x32
-    # exist for the multi-iteration launches only
-    SHAPES = ((64, 8), (64, 4), (128, 4), (32, 16), (32, 8), (16, 32), (16, 16), (8, 8), (4, 4), (32, 32))
+    # tile shapes by index: 0 .. 6 are those of the "shape" option; 8x8, 4x4, 32x32 and
+    # 20x20 exist for the multi-iteration launches only
+    SHAPES = ((64, 8), (64, 4), (128, 4), (32, 16), (32, 8), (16, 32), (16, 16), (8, 8), (4, 4), (32, 32),
+              (20, 20))
 
     def level_tuning(self, level: int, masked: bool = True, with_per_launch: bool = False):
         """(tile shape (w, h), Tp prefetch[, inner iterations per launch]) used at
@@ -298,12 +299,15 @@ class StereoTgv:
             return self.SHAPES[sh.value], bool(pre.value), k.value
         return self.SHAPES[sh.value], bool(pre.value)
 
+    HALF_WORKGROUP = 0x100
+
     def set_level_tuning(self, level: int, shape: int, preload: bool = False, per_launch: int = 1,
-                         masked: bool = True):
+                         masked: bool = True, half_workgroup: bool = False):
         """Fix the launch choices of one level (shape index as option "shape"; 1, 2 or 4
-        inner iterations per launch)."""
-        return self._check(self._L.vfs_set_level_tuning(self._h, int(masked), level, shape, int(preload),
-                                                        int(per_launch)), "set_level_tuning")
+        inner iterations per launch, optionally in half workgroups)."""
+        k = int(per_launch) | (self.HALF_WORKGROUP if half_workgroup else 0)
+        return self._check(self._L.vfs_set_level_tuning(self._h, int(masked), level, shape, int(preload), k),
+                           "set_level_tuning")
 
     def time_solves(self, masked: bool = True, n: int = 1) -> float:
         ms = ctypes.c_float()

@@ -15,6 +15,8 @@
 
 #include <math.h>
 
+#include <type_traits>
+
 // Every product and sum is rounded as written (no multiply-add fusion chosen by the
 // compiler): a pixel's value is computed by the same rounding sequence in every tile
 // shape and in every inlined copy of an op (tile pixel or recomputed halo), so results
@@ -820,14 +822,15 @@ __device__ __forceinline__ void op_iter(const Tile &t, int roles, const IterPara
 // mask pixel lies in column 0: there the reference's flat-index left neighbour (pos - 1)
 // is the previous row's last pixel.
 // The step sizes of iteration j come from the pair's schedule table at index it0 + j.
-// Work-items: one per staged position (rounded up to whole waves, at most 1024), so that
-// every phase takes at most one pixel per work-item on all but the largest tiles.
-template <int SH, int K>
+// Work-items: see BlkK::NT.
+template <int SH, int K, bool HALF>
 struct BlkK {
     static constexpr int TW = kShape[SH].tw, TH = kShape[SH].th;
     static constexpr int RO = -2 * K, RW = TW + 3 * K, RH = TH + 3 * K;  // staged region
     static constexpr int NR = RW * RH;
-    static constexpr int NT = NR >= 1024 ? 1024 : (NR + 63) / 64 * 64;  // work-items
+    // work-items (half_workgroup_ok)
+    static constexpr int NT = HALF ? 512 : NR >= 1024 ? 1024 : (NR + 63) / 64 * 64;
+    static_assert(!HALF || half_workgroup_ok(SH, K), "half workgroup");
     static constexpr int NK = (NR + NT - 1) / NT;  // staged positions per work-item
     static_assert(multi_iter_shape(SH), "multi-iteration tile shape");
 };
@@ -837,7 +840,7 @@ struct BlkK {
 // the iteration's new Tp, outside it the Tp plane) and the mask.
 template <int SH, int K>
 struct LdsK {
-    using B = BlkK<SH, K>;
+    using B = BlkK<SH, K, false>;
     float ub[B::NR];
     float2 v[B::NR];
     float4 q[B::NR];
@@ -850,8 +853,7 @@ struct OwnK {
     float a, b, c, iu, iz;
     float u, ub;        // u and u_ (current buffers, then each iteration's)
     float2 v;           // v_
-    float2 p;           // p and q: the last dual step's values
-    float4 q;
+    float2 p;           // p: the last dual step's value (q: in LDS)
     float ubo;          // outside the mask: u_ and v_ of the other buffers
     float2 vo;
 };
@@ -866,12 +868,12 @@ __device__ __forceinline__ float rT(float v)
 
 // The step sizes of iteration j: sched[j] = (sigma, tau, mu); `ip0` holds the
 // schedule-independent fields and the flags of the op.
-template <int SH, int K, bool MASKED, typename T>
+template <int SH, int K, bool HALF, bool MASKED, typename T>
 __device__ __forceinline__ void op_iterk(const Tile &t, int roles, const float4 *__restrict__ sched,
                                          const IterParams &ip0, LdsK<SH, K> &L)
 {
     static_assert(K % 2 == 0, "K-iteration ops rotate the roles like two iterations");
-    using B = BlkK<SH, K>;
+    using B = BlkK<SH, K, HALF>;
     constexpr int NT = B::NT, NK = B::NK;
     const int w = t.w, h = t.h, s = t.s, x0 = t.x0, y0 = t.y0;
     const int tid = threadIdx.x;
@@ -904,6 +906,7 @@ __device__ __forceinline__ void op_iterk(const Tile &t, int roles, const float4 
     int ox[NK], oy[NK];
     float omk[NK];
     float2 otp[NK];
+    float4 oq[NK];
 #pragma unroll
     for (int k = 0; k < NK; k++) {
         const int i = tid + k * NT;
@@ -923,10 +926,10 @@ __device__ __forceinline__ void op_iterk(const Tile &t, int roles, const float4 
             o[k].iz = ldT(Iz, pos);
             if (ip0.first) {
                 o[k].p = make_float2(0.f, 0.f);
-                o[k].q = make_float4(0.f, 0.f, 0.f, 0.f);
+                oq[k] = make_float4(0.f, 0.f, 0.f, 0.f);
             } else {
                 o[k].p = ld2T(p_cur, pos);
-                o[k].q = ld4T(q_cur, pos);
+                oq[k] = ld4T(q_cur, pos);
             }
         }
     }
@@ -951,7 +954,7 @@ __device__ __forceinline__ void op_iterk(const Tile &t, int roles, const float4 
         if (i < B::NR) {
             L.ub[i] = o[k].ub;
             L.v[i] = o[k].v;
-            L.q[i] = o[k].q;
+            L.q[i] = oq[k];
             L.tp[i] = otp[k];
             L.mk[i] = omk[k] != 0.f ? 1 : 0;
         }
@@ -966,134 +969,158 @@ __device__ __forceinline__ void op_iterk(const Tile &t, int roles, const float4 
             v_spr[pos] = v_cur[pos];
         }
     }
-    __syncthreads();
+    // A workgroup whose staged region lies inside the image and (masked form) inside the
+    // mask, or (unmasked form) inside the circle, takes every neighbour branch the same
+    // way: it runs the iterations without the boundary tests (same arithmetic).
+    bool inner = true;
+#pragma unroll
+    for (int k = 0; k < NK; k++) {
+        const int i = tid + k * NT, x = ox[k], y = oy[k];
+        if (i < B::NR)
+            inner = inner && x >= 0 && y >= 0 && x < w && y < h && omk[k] != 0.f &&
+                    (MASKED || !outside_circle(x, y, w, h));
+    }
+    const bool full = __syncthreads_and(inner) != 0;
 
     auto act = [&](int k) { return !MASKED || L.mk[k] != 0; };
     auto in_tile = [&](int x, int y) { return x >= x0 && x < x0 + B::TW && y >= y0 && y < y0 + B::TH; };
 
+    auto iterate = [&](auto full_c) {
+        constexpr bool FULL = decltype(full_c)::value;
 #pragma unroll 1
-    for (int j = 0; j < K; j++) {
-        IterParams ip = ip0;
-        const float4 sc = sched[j];
-        ip.k1 = ip0.alpha1 * sc.x / 3.0f;
-        ip.k0 = ip0.alpha0 * sc.x / 2.0f;
-        ip.tau = sc.y;
-        ip.mu = sc.z;
-        ip.first = j == 0 && ip0.first;
-        ip.write_tp = j == K - 1 && ip0.write_tp;
-        const int r = K - 1 - j;
+        for (int j = 0; j < K; j++) {
+            IterParams ip = ip0;
+            const float4 sc = sched[j];
+            ip.k1 = ip0.alpha1 * sc.x / 3.0f;
+            ip.k0 = ip0.alpha0 * sc.x / 2.0f;
+            ip.tau = sc.y;
+            ip.mu = sc.z;
+            ip.first = j == 0 && ip0.first;
+            ip.write_tp = j == K - 1 && ip0.write_tp;
+            const int r = K - 1 - j;
 
-        // ---- duals over the tile grown by 2r + 1 (left / top) and r (right / bottom)
+            // ---- duals over the tile grown by 2r + 1 (left / top) and r (right / bottom)
 #pragma unroll
-        for (int k = 0; k < NK; k++) {
-            const int i = tid + k * NT, x = ox[k], y = oy[k];
-            if (i >= B::NR || x < 0 || y < 0 || x >= w || y >= h) continue;
-            if (x < x0 - 2 * r - 1 || y < y0 - 2 * r - 1 || x > x0 + B::TW - 1 + r || y > y0 + B::TH - 1 + r) continue;
-            if (!act(i)) continue;
-            const int kR = x + 1 < w ? i + 1 : i, kD = y + 1 < h ? i + B::RW : i;
-            DualIn rr;
-            rr.ok = true;
-            rr.act = true;
-            rr.hasR = (x + 1 < w) && act(kR);
-            rr.hasD = (y + 1 < h) && act(kD);
-            rr.a = o[k].a;
-            rr.b = o[k].b;
-            rr.c = o[k].c;
-            rr.u0 = o[k].ub;
-            rr.uR = L.ub[kR];
-            rr.uD = L.ub[kD];
-            rr.uL = x >= 1 ? L.ub[i - 1] : 0.f;  // column 0 inside the mask: not used here
-            rr.uU = y >= 1 ? L.ub[i - B::RW] : 0.f;
-            rr.v0 = o[k].v;
-            rr.vR = L.v[kR];
-            rr.vD = L.v[kD];
-            rr.vL = x >= 1 ? L.v[i - 1] : make_float2(0.f, 0.f);
-            rr.vU = y >= 1 ? L.v[i - B::RW] : make_float2(0.f, 0.f);
-            // the previous iteration's duals as the planes hold them
-            rr.po = make_float2(rT<T>(o[k].p.x), rT<T>(o[k].p.y));
-            rr.qo = make_float4(rT<T>(o[k].q.x), rT<T>(o[k].q.y), rT<T>(o[k].q.z), rT<T>(o[k].q.w));
-            float2 pn, tpn;
-            float4 qn;
-            dual_step<MASKED>(rr, x, y, w, h, ip, pn, qn, tpn);
-            o[k].p = pn;  // the primal step of this iteration reads them unrounded
-            o[k].q = qn;
-            L.q[i] = qn;
-            L.tp[i] = tpn;
-            if (r == 0 && in_tile(x, y)) {
-                const long pos = (long)y * s + x;
-                st2T(p_spr, pos, pn);
-                st4T(q_spr, pos, qn);
-                if (ip.write_tp) Tp[pos] = tpn;
-            }
-        }
-        __syncthreads();
-
-        // ---- thresholding + primal step over the tile grown by 2r (left / top) and r;
-        // outside the mask, the stored u_ and v_ switch to the buffer the next
-        // iteration reads there
-#pragma unroll
-        for (int k = 0; k < NK; k++) {
-            const int i = tid + k * NT, x = ox[k], y = oy[k];
-            if (i >= B::NR || x < 0 || y < 0 || x >= w || y >= h) continue;
-            const long pos = (long)y * s + x;
-            if (MASKED && !act(i)) {
-                if (r == 0) {
-                    // outside the mask: the spare buffers take the current buffers' values
-                    if (in_tile(x, y)) {
-                        u_spr[pos] = o[k].u;
-                        ub_spr[pos] = o[k].ub;
-                        v_spr[pos] = o[k].v;
-                    }
-                } else {
-                    L.ub[i] = (j & 1) ? o[k].ub : o[k].ubo;
-                    L.v[i] = (j & 1) ? o[k].v : o[k].vo;
+            for (int k = 0; k < NK; k++) {
+                const int i = tid + k * NT, x = ox[k], y = oy[k];
+                if (i >= B::NR) continue;
+                if (!FULL && (x < 0 || y < 0 || x >= w || y >= h)) continue;
+                if (x < x0 - 2 * r - 1 || y < y0 - 2 * r - 1 || x > x0 + B::TW - 1 + r || y > y0 + B::TH - 1 + r)
+                    continue;
+                if (!FULL && !act(i)) continue;
+                const int kR = (FULL || x + 1 < w) ? i + 1 : i, kD = (FULL || y + 1 < h) ? i + B::RW : i;
+                DualIn rr;
+                rr.ok = true;
+                rr.act = true;
+                rr.hasR = FULL || ((x + 1 < w) && act(kR));
+                rr.hasD = FULL || ((y + 1 < h) && act(kD));
+                rr.a = o[k].a;
+                rr.b = o[k].b;
+                rr.c = o[k].c;
+                rr.u0 = o[k].ub;
+                rr.uR = L.ub[kR];
+                rr.uD = L.ub[kD];
+                // backward differences (read where the right / lower neighbour is outside
+                // the mask; column 0 inside the mask is not run here)
+                rr.uL = (!FULL && x >= 1) ? L.ub[i - 1] : 0.f;
+                rr.uU = (!FULL && y >= 1) ? L.ub[i - B::RW] : 0.f;
+                rr.v0 = o[k].v;
+                rr.vR = L.v[kR];
+                rr.vD = L.v[kD];
+                rr.vL = (!FULL && x >= 1) ? L.v[i - 1] : make_float2(0.f, 0.f);
+                rr.vU = (!FULL && y >= 1) ? L.v[i - B::RW] : make_float2(0.f, 0.f);
+                // the previous iteration's duals as the planes hold them
+                rr.po = make_float2(rT<T>(o[k].p.x), rT<T>(o[k].p.y));
+                const float4 qo = L.q[i];
+                rr.qo = make_float4(rT<T>(qo.x), rT<T>(qo.y), rT<T>(qo.z), rT<T>(qo.w));
+                float2 pn, tpn;
+                float4 qn;
+                dual_step<MASKED>(rr, x, y, w, h, ip, pn, qn, tpn);
+                o[k].p = pn;  // the primal step of this iteration reads them unrounded
+                L.q[i] = qn;
+                L.tp[i] = tpn;
+                if (r == 0 && in_tile(x, y)) {
+                    const long pos = (long)y * s + x;
+                    st2T(p_spr, pos, pn);
+                    st4T(q_spr, pos, qn);
+                    if (ip.write_tp) Tp[pos] = tpn;
                 }
-                continue;
             }
-            if (x < x0 - 2 * r || y < y0 - 2 * r || x > x0 + B::TW - 1 + r || y > y0 + B::TH - 1 + r) continue;
-            float un, u_s;
-            float2 vs;
-            if (!MASKED && outside_circle(x, y, w, h)) {
-                un = 0.f;
-                u_s = 0.f;
-                vs = make_float2(0.f, 0.f);
-            } else {
-                const int kR = x + 1 < w ? i + 1 : i, kD = y + 1 < h ? i + B::RW : i;
-                const bool hR = (x + 1 < w) && act(kR), hD = (y + 1 < h) && act(kD);
-                const bool hL = x >= 1 && act(i - 1), hU = y >= 1 && act(i - B::RW);
-                // the left / upper neighbour's Tp: this iteration's inside the mask,
-                // the Tp plane's outside it
-                const float tpLx = x >= 1 ? L.tp[i - 1].x : 0.f;
-                const float tpUy = y >= 1 ? L.tp[i - B::RW].y : 0.f;
-                const float divp = div_tp<MASKED>(x, y, w, h, hR, hD, hL, hU, L.tp[i], tpLx, tpUy);
-                const float4 ql = hL ? L.q[i - 1] : make_float4(0.f, 0.f, 0.f, 0.f);
-                const float4 qu = hU ? L.q[i - B::RW] : make_float4(0.f, 0.f, 0.f, 0.f);
-                primal_step(ip, o[k].a, o[k].b, o[k].c, o[k].ub, o[k].u, o[k].iu, o[k].iz, o[k].v, hR, hD, hL, hU,
-                            divp, o[k].q, ql, qu, o[k].p, un, u_s, vs);
+            __syncthreads();
+
+            // ---- thresholding + primal step over the tile grown by 2r (left / top) and
+            // r; outside the mask, the stored u_ and v_ switch to the buffer the next
+            // iteration reads there
+#pragma unroll
+            for (int k = 0; k < NK; k++) {
+                const int i = tid + k * NT, x = ox[k], y = oy[k];
+                if (i >= B::NR) continue;
+                if (!FULL && (x < 0 || y < 0 || x >= w || y >= h)) continue;
+                const long pos = (long)y * s + x;
+                if (!FULL && MASKED && !act(i)) {
+                    if (r == 0) {
+                        // outside the mask: the spare buffers take the current buffers' values
+                        if (in_tile(x, y)) {
+                            u_spr[pos] = o[k].u;
+                            ub_spr[pos] = o[k].ub;
+                            v_spr[pos] = o[k].v;
+                        }
+                    } else {
+                        L.ub[i] = (j & 1) ? o[k].ub : o[k].ubo;
+                        L.v[i] = (j & 1) ? o[k].v : o[k].vo;
+                    }
+                    continue;
+                }
+                if (x < x0 - 2 * r || y < y0 - 2 * r || x > x0 + B::TW - 1 + r || y > y0 + B::TH - 1 + r) continue;
+                float un, u_s;
+                float2 vs;
+                if (!FULL && !MASKED && outside_circle(x, y, w, h)) {
+                    un = 0.f;
+                    u_s = 0.f;
+                    vs = make_float2(0.f, 0.f);
+                } else {
+                    const int kR = (FULL || x + 1 < w) ? i + 1 : i, kD = (FULL || y + 1 < h) ? i + B::RW : i;
+                    const bool hR = FULL || ((x + 1 < w) && act(kR)), hD = FULL || ((y + 1 < h) && act(kD));
+                    const bool hL = FULL || (x >= 1 && act(i - 1)), hU = FULL || (y >= 1 && act(i - B::RW));
+                    // the left / upper neighbour's Tp: this iteration's inside the mask,
+                    // the Tp plane's outside it
+                    const float tpLx = (FULL || x >= 1) ? L.tp[i - 1].x : 0.f;
+                    const float tpUy = (FULL || y >= 1) ? L.tp[i - B::RW].y : 0.f;
+                    const float divp = div_tp<MASKED>(x, y, w, h, hR, hD, hL, hU, L.tp[i], tpLx, tpUy);
+                    const float4 ql = hL ? L.q[i - 1] : make_float4(0.f, 0.f, 0.f, 0.f);
+                    const float4 qu = hU ? L.q[i - B::RW] : make_float4(0.f, 0.f, 0.f, 0.f);
+                    primal_step(ip, o[k].a, o[k].b, o[k].c, o[k].ub, o[k].u, o[k].iu, o[k].iz, o[k].v, hR, hD, hL,
+                                hU, divp, L.q[i], ql, qu, o[k].p, un, u_s, vs);
+                }
+                o[k].u = un;
+                o[k].ub = u_s;
+                o[k].v = vs;
+                L.ub[i] = u_s;
+                L.v[i] = vs;
+                if (r == 1 && in_tile(x, y)) {
+                    u_oth[pos] = un;
+                    ub_oth[pos] = u_s;
+                    v_oth[pos] = vs;
+                } else if (r == 0) {
+                    u_spr[pos] = un;
+                    ub_spr[pos] = u_s;
+                    v_spr[pos] = vs;
+                }
             }
-            o[k].u = un;
-            o[k].ub = u_s;
-            o[k].v = vs;
-            L.ub[i] = u_s;
-            L.v[i] = vs;
-            if (r == 1 && in_tile(x, y)) {
-                u_oth[pos] = un;
-                ub_oth[pos] = u_s;
-                v_oth[pos] = vs;
-            } else if (r == 0) {
-                u_spr[pos] = un;
-                ub_spr[pos] = u_s;
-                v_spr[pos] = vs;
-            }
+            __syncthreads();
         }
-        __syncthreads();
-    }
+    };
+    if (full) iterate(std::true_type{});
+    else iterate(std::false_type{});
 }
 
-template <typename T, int SH, int K>
-__global__ __launch_bounds__((BlkK<SH, K>::NT)) void k_iterk(const PairDesc *__restrict__ D, const TickArgs A)
+// Half workgroups (512 work-items, two positions each) are held to 128 registers, so that
+// two of them share a compute unit.
+template <typename T, int SH, int K, bool HALF>
+__global__ __launch_bounds__((BlkK<SH, K, HALF>::NT)) __attribute__((amdgpu_waves_per_eu(HALF ? 4 : 1)))
+void k_iterk(const PairDesc *__restrict__ D, const TickArgs A)
 {
-    using B = BlkK<SH, K>;
+    using B = BlkK<SH, K, HALF>;
     __shared__ LdsK<SH, K> L;
     const int nb = gridDim.x, i = blockIdx.x, xcd = i & 7, q = nb >> 3, r = nb & 7;
     const int b = (xcd < r ? xcd * (q + 1) : r * (q + 1) + (xcd - r) * q) + (i >> 3);
@@ -1127,8 +1154,8 @@ __global__ __launch_bounds__((BlkK<SH, K>::NT)) void k_iterk(const PairDesc *__r
     ip0.first = (E.flags & EF_FIRST) != 0;
     ip0.write_tp = (E.flags & EF_WRITE_TP_LAST) != 0;  // the op's last iteration
     ip0.preload_tp = 0;
-    if (E.masked) op_iterk<SH, K, true, T>(t, E.roles, sched, ip0, L);
-    else op_iterk<SH, K, false, T>(t, E.roles, sched, ip0, L);
+    if (E.masked) op_iterk<SH, K, HALF, true, T>(t, E.roles, sched, ip0, L);
+    else op_iterk<SH, K, HALF, false, T>(t, E.roles, sched, ip0, L);
 }
 
 // 13th smallest of 25 by the partial selection sort of medianFilter.cu:307-316.
@@ -1547,24 +1574,33 @@ void tick_storage(const PairDesc *desc, const TickArgs &args, int shape, hipStre
 static_assert(kTickShapes == 7, "tick_storage dispatches every tick shape");
 
 // K-iteration launches exist for the shapes of multi_iter_shape() (64x4, 32x16, 32x8,
-// 16x16, 8x8, 4x4, 32x32), whose staged regions fit the workgroup's LDS.
+// 16x16, 8x8, 4x4, 32x32, 20x20), whose staged regions fit the workgroup's LDS.
 template <typename T, int SH, int K>
-void iterk_shape(const PairDesc *desc, const TickArgs &args, hipStream_t st)
+void iterk_shape(const PairDesc *desc, const TickArgs &args, bool half, hipStream_t st)
 {
-    k_iterk<T, SH, K><<<dim3((unsigned)args.total), dim3(BlkK<SH, K>::NT), 0, st>>>(desc, args);
+    const dim3 grid((unsigned)args.total);
+    if constexpr (half_workgroup_ok(SH, K)) {
+        if (half) {
+            k_iterk<T, SH, K, true><<<grid, dim3(BlkK<SH, K, true>::NT), 0, st>>>(desc, args);
+            return;
+        }
+    }
+    k_iterk<T, SH, K, false><<<grid, dim3(BlkK<SH, K, false>::NT), 0, st>>>(desc, args);
 }
 
 template <typename T, int K>
-void iterk_storage(const PairDesc *desc, const TickArgs &args, int shape, hipStream_t st)
+void iterk_storage(const PairDesc *desc, const TickArgs &args, int launch_shape, hipStream_t st)
 {
-    switch (shape) {
-    case 1: iterk_shape<T, 1, K>(desc, args, st); break;
-    case 3: iterk_shape<T, 3, K>(desc, args, st); break;
-    case 4: iterk_shape<T, 4, K>(desc, args, st); break;
-    case 6: iterk_shape<T, 6, K>(desc, args, st); break;
-    case 7: iterk_shape<T, 7, K>(desc, args, st); break;
-    case 8: iterk_shape<T, 8, K>(desc, args, st); break;
-    case 9: iterk_shape<T, 9, K>(desc, args, st); break;
+    const bool half = (launch_shape & kHalfWorkgroup) != 0;
+    switch (shape_index(launch_shape)) {
+    case 1: iterk_shape<T, 1, K>(desc, args, half, st); break;
+    case 3: iterk_shape<T, 3, K>(desc, args, half, st); break;
+    case 4: iterk_shape<T, 4, K>(desc, args, half, st); break;
+    case 6: iterk_shape<T, 6, K>(desc, args, half, st); break;
+    case 7: iterk_shape<T, 7, K>(desc, args, half, st); break;
+    case 8: iterk_shape<T, 8, K>(desc, args, half, st); break;
+    case 9: iterk_shape<T, 9, K>(desc, args, half, st); break;
+    case 10: iterk_shape<T, 10, K>(desc, args, half, st); break;
     default: break;  // the host never asks for another shape
     }
 }

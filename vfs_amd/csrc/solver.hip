@@ -93,6 +93,7 @@ struct ProgOp {
 // multi-iteration launches.
 struct LevelTune {
     unsigned char shape = 0, preload = 0, per_launch = 1, mshape = 4;
+    unsigned char mhalf = 0;  // multi-iteration launches in half workgroups (vfs::kHalfWorkgroup)
     // tile shape of the inner iterations
     int iter_shape() const { return per_launch > 1 ? mshape : shape; }
 };
@@ -104,7 +105,8 @@ struct Tuning {
         std::array<unsigned short, kMaxLevels> k{};
         for (int l = 0; l < kMaxLevels; l++) {
             const int code = lv[l].per_launch == 4 ? 2 : lv[l].per_launch == 2 ? 1 : 0;
-            k[l] = (unsigned short)(lv[l].shape | (lv[l].preload << 3) | (code << 4) | (lv[l].mshape << 6));
+            k[l] = (unsigned short)(lv[l].shape | (lv[l].preload << 3) | (code << 4) | (lv[l].mshape << 6) |
+                                    (lv[l].mhalf << 10));
         }
         return k;
     }
@@ -370,6 +372,8 @@ void apply_tuning(Program &pr, const Tuning &tn)
         if (o.kind == vfs::OP_ITER && lt.preload) o.flags |= vfs::EF_PRELOAD_TP;
         // multi-iteration ops exist for some shapes only
         if (multi && !multi_shape(o.shape)) o.shape = 4;
+        if (multi && lt.mhalf && vfs::half_workgroup_ok(o.shape, vfs::op_iterations(o.kind)))
+            o.shape = (unsigned char)(o.shape | vfs::kHalfWorkgroup);
     }
 }
 
@@ -621,17 +625,20 @@ hipError_t time_launch_choices(hipStream_t st, const vfs::PairDesc *d_desc, cons
 {
     constexpr int kWarm = 3, kReps = 20, kRounds = 2;
     // candidates: (shape, preload) of one iteration per launch, then two- and
-    // four-iteration launches for the shapes that have them
-    constexpr int kSingle = vfs::kTickShapes * 2, kCand = kSingle + 2 * vfs::kShapes;
-    auto cand = [&](int c, int &shape, int &pre, int &per_launch) {
+    // four-iteration launches for the shapes that have them, in full or half workgroups
+    constexpr int kSingle = vfs::kTickShapes * 2, kCand = kSingle + 2 * vfs::kShapes * 2;
+    auto cand = [&](int c, int &shape, int &pre, int &per_launch, int &half) {
         if (c < kSingle) {
             shape = c / 2;
             pre = c % 2;
             per_launch = 1;
+            half = 0;
         } else {
-            shape = (c - kSingle) % vfs::kShapes;
+            const int m = c - kSingle;
+            per_launch = m < vfs::kShapes * 2 ? 2 : 4;
+            shape = (m % (vfs::kShapes * 2)) / 2;
+            half = m % 2;
             pre = 0;
-            per_launch = c - kSingle < vfs::kShapes ? 2 : 4;
         }
     };
     hipEvent_t e0 = nullptr, e1 = nullptr;
@@ -642,16 +649,17 @@ hipError_t time_launch_choices(hipStream_t st, const vfs::PairDesc *d_desc, cons
     for (int round = 0; round < kRounds && r == hipSuccess; round++) {
         for (size_t li = 0; li < levels.size() && r == hipSuccess; li++) {
             for (int c = 0; c < kCand && r == hipSuccess; c++) {
-                int shape, pre, per_launch;
-                cand(c, shape, pre, per_launch);
-                if (per_launch > 1 && (!multi_shape(shape) || !allow_multi[li]))
+                int shape, pre, per_launch, half;
+                cand(c, shape, pre, per_launch, half);
+                if (per_launch > 1 && (!multi_shape(shape) || !allow_multi[li] ||
+                                       (half && !vfs::half_workgroup_ok(shape, per_launch))))
                     continue;
                 ProgOp o{};
                 o.kind = (unsigned char)(per_launch == 4 ? vfs::OP_ITER4 : per_launch == 2 ? vfs::OP_ITER2 : vfs::OP_ITER);
                 o.level = (unsigned char)levels[li];
                 o.masked = (unsigned char)masked;
                 o.flags = (unsigned char)(pre ? vfs::EF_PRELOAD_TP : 0);
-                o.shape = (unsigned char)shape;
+                o.shape = (unsigned char)(shape | (half ? vfs::kHalfWorkgroup : 0));
                 o.roles = vfs::kRolesInit;
                 o.sigma = 1.0f;
                 o.tau = 1.0f;
@@ -666,9 +674,9 @@ hipError_t time_launch_choices(hipStream_t st, const vfs::PairDesc *d_desc, cons
                 }
                 a.n = n;
                 a.total = acc;
-                for (int k = 0; k < kWarm; k++) vfs::launch_tick(d_desc, a, P.storage, shape, st);
+                for (int k = 0; k < kWarm; k++) vfs::launch_tick(d_desc, a, P.storage, o.shape, st);
                 r = hipEventRecord(e0, st);
-                for (int k = 0; k < kReps; k++) vfs::launch_tick(d_desc, a, P.storage, shape, st);
+                for (int k = 0; k < kReps; k++) vfs::launch_tick(d_desc, a, P.storage, o.shape, st);
                 if (r == hipSuccess) r = hipEventRecord(e1, st);
                 if (r == hipSuccess) r = hipEventSynchronize(e1);
                 float ms = 0;
@@ -702,10 +710,11 @@ hipError_t time_launch_choices(hipStream_t st, const vfs::PairDesc *d_desc, cons
         for (int c = kSingle; c < kCand; c++) {
             const float ms = best[li * kCand + c];
             if (ms < 0.98f * pick_ms) {
-                int shape, pre, per_launch;
-                cand(c, shape, pre, per_launch);
+                int shape, pre, per_launch, half;
+                cand(c, shape, pre, per_launch, half);
                 pick.mshape = (unsigned char)shape;
                 pick.per_launch = (unsigned char)per_launch;
+                pick.mhalf = (unsigned char)half;
                 pick_ms = ms;
             }
         }
@@ -1307,7 +1316,12 @@ int vfs_set_level_tuning(vfs_solver *s, int masked, int level, int shape, int pr
 {
     if (!s) return VFS_ERR_ARG;
     if (!s->initialized || level < 0 || level >= s->nLevels) return fail(s, VFS_ERR_ARG, "bad level");
-    if (!valid_per_launch(per_launch)) return fail(s, VFS_ERR_ARG, "per_launch must be 1, 2 or 4");
+    const bool half = (per_launch & VFS_HALF_WORKGROUP) != 0;
+    per_launch &= ~VFS_HALF_WORKGROUP;
+    if (!valid_per_launch(per_launch) || (half && (per_launch == 1 || !multi_shape(shape) ||
+                                                   !vfs::half_workgroup_ok(shape, per_launch))))
+        return fail(s, VFS_ERR_ARG, "per_launch must be 1, 2 or 4 (half workgroups where the staged region "
+                                    "holds 513 .. 1024 positions)");
     if (per_launch > 1 ? !multi_shape(shape) : (shape < 0 || shape >= vfs::kTickShapes))
         return fail(s, VFS_ERR_ARG, "bad shape %d for %d iteration(s) per launch", shape, per_launch);
     const int m = masked ? 1 : 0;
@@ -1319,6 +1333,7 @@ int vfs_set_level_tuning(vfs_solver *s, int masked, int level, int shape, int pr
     LevelTune &t = s->tuning[m].lv[level];
     if (per_launch > 1) {
         t.mshape = (unsigned char)shape;
+        t.mhalf = (unsigned char)half;
     } else {
         t.shape = (unsigned char)shape;
         t.preload = (unsigned char)(preload != 0);

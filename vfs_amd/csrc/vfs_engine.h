@@ -24,17 +24,33 @@ constexpr int kMaxSched = 128;
 // or 2) of the tw x th tile.  Shape 0 is the default; the others are candidates of the
 // per-level tile-shape tuner.  The multi-iteration ops (OP_ITER2, OP_ITER4) take the
 // shapes of multi_iter_shape(), including the small tiles 8x8 and 4x4 for the coarsest,
-// latency-bound levels and 32x32 for the largest.
-constexpr int kShapes = 10, kTickShapes = 7;
+// latency-bound levels and 32x32 and 20x20 for the largest.
+constexpr int kShapes = 11, kTickShapes = 7;
 struct ShapeDim {
     int tw, th;
 };
 constexpr ShapeDim kShape[kShapes] = {{64, 8}, {64, 4}, {128, 4}, {32, 16}, {32, 8}, {16, 32}, {16, 16},
-                                      {8, 8}, {4, 4}, {32, 32}};
+                                      {8, 8}, {4, 4}, {32, 32}, {20, 20}};
 __host__ __device__ constexpr bool multi_iter_shape(int shape)
 {
-    return shape == 1 || shape == 3 || shape == 4 || shape == 6 || shape == 7 || shape == 8 || shape == 9;
+    return shape == 1 || shape == 3 || (shape >= 4 && shape != 5);
 }
+// Positions a K-iteration workgroup stages: the tile grown by 2K pixels to the left / top
+// and K to the right / bottom.
+__host__ __device__ constexpr int iterk_region(int shape, int K)
+{
+    return (kShape[shape].tw + 3 * K) * (kShape[shape].th + 3 * K);
+}
+// A K-iteration launch takes one work-item per staged position up to 512 positions and
+// 1024 above; regions of 513 .. 1024 positions may instead take 512 work-items with two
+// positions each (more workgroups per compute unit).  Launch shapes carry that choice as
+// kHalfWorkgroup added to the tile shape index.
+constexpr int kHalfWorkgroup = 16;
+__host__ __device__ constexpr bool half_workgroup_ok(int shape, int K)
+{
+    return iterk_region(shape, K) > 512 && iterk_region(shape, K) <= 1024;
+}
+__host__ __device__ constexpr int shape_index(int launch_shape) { return launch_shape & (kHalfWorkgroup - 1); }
 
 // Storage precision of the image pyramid, warped image, derivatives, tensor and duals.
 // Arithmetic is FP32 in every mode.
@@ -192,7 +208,7 @@ inline int op_tiles(int kind, int level, const int *w, const int *h, const int *
 {
     const int tl = op_tile_level(kind, level);
     const int ww = kind == OP_ZERO ? s[0] : w[tl];
-    const ShapeDim d = kShape[shape];
+    const ShapeDim d = kShape[shape_index(shape)];
     return ((ww + d.tw - 1) / d.tw) * ((h[tl] + d.th - 1) / d.th);
 }
 
