@@ -3,8 +3,8 @@
 A step is one masked solve (solveStereoForwardMasked) of one synthetic 848x800 pair
 (5 levels, 10 warps, 50 iterations; inputs resident in HBM).  Prints one JSON line:
 frame-pairs/s, p50/p99 per-solve latency, mean disparity error against the FP64 oracle,
-the roofline of the dominant kernel work (one inner TGV iteration at level 0) and the
-CPU baseline (the FP64 oracle on the host cores, one solve).
+the roofline of the dominant kernel (the inner TGV iterations at level 0, several per
+launch as the tuner chose) and the CPU baseline (the FP64 oracle on the host cores).
 
     python bench.py [--steps K] [--warmup W] [--workload tgv_848x800] [--no-cpu]
 """
@@ -22,13 +22,15 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
-# Algorithmic bytes per pixel and inner iteration (FP32 storage): read u, u_, v_(2),
-# p(2), q(4), a, b, c, Iu, Iz and write u, u_, v_(2), p(2), q(4) = 25 floats.
+# Algorithmic bytes per pixel and inner iteration (FP32 storage): what one iteration
+# moves when each iteration is its own pass: read u, u_, v_(2), p(2), q(4), a, b, c, Iu,
+# Iz and write u, u_, v_(2), p(2), q(4) = 25 floats.  A launch of K iterations is
+# credited K times that.
 B_ITER_FP32 = 100.0
-# HBM bytes per launch of the level-0 inner iteration measured with rocprofv3 --pmc
-# (FETCH_SIZE and WRITE_SIZE, scaled by a copy kernel of known size); the summary file
-# holds the command and the raw counters.
-TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "pmc_traffic_tgv_848x800_iteration_L0.md")
+# HBM bytes per launch of the level-0 inner-iteration kernel (32x32 tiles, four
+# iterations per launch) measured with rocprofv3 --pmc (FETCH_SIZE and WRITE_SIZE, scaled
+# by a copy kernel of known size); the summary file holds the command and the counters.
+TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "pmc_traffic_k_iterk_32x32x4_tgv_848x800_L0.md")
 
 
 def profiled_traffic():
@@ -58,8 +60,11 @@ def main():
     ap.add_argument("--shape", type=int, default=-1,
                     help="force one tile shape (0..6) instead of the per-level tuner (profiling aid)")
     ap.add_argument("--tuning", default="",
-                    help="per-level launch choices 'shape[p][xK[h]],...' from level 0 (profiling aid): shape index, "
-                         "p = Tp prefetch, x2 / x4 = two / four inner iterations per launch, h = half workgroups")
+                    help="per-level launch choices from level 0, comma-separated (profiling aid): 'S[p]' one "
+                         "iteration per launch in tile shape S (p = Tp prefetch), 'MxK[h]' K = 2 or 4 iterations per "
+                         "launch in tile shape M (h = half workgroups), or 'S[p]/MxK[h]' for both")
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="no launch tuner (fixed defaults; with --tuning for reproducible profiles)")
     ap.add_argument("--iter-only", type=int, default=0,
                     help="profiling aid: run N level-0 inner iterations (after a 256 MiB reference copy) and exit")
     args = ap.parse_args()
@@ -90,11 +95,15 @@ def main():
     g.copyMaskToDevice(pair["mask"])
     g.loadVectorFields(pair["tv"], pair["cv"])
     g.copyImagesToDevice(pair["i0"], pair["i1"])
+    if args.no_autotune:
+        g.set_option("autotune", 0)
     for lvl, spec in enumerate(c for c in args.tuning.split(",") if c):
-        sh = int(spec.split("x")[0].rstrip("p"))
-        k = int(spec.split("x")[1].rstrip("h")) if "x" in spec else 1
-        g.set_level_tuning(lvl, sh, preload=spec.split("x")[0].endswith("p"), per_launch=k,
-                           half_workgroup=spec.endswith("h"))
+        for part in spec.split("/"):
+            if "x" in part:
+                m, k = part.split("x")
+                g.set_level_tuning(lvl, int(m), per_launch=int(k.rstrip("h")), half_workgroup=k.endswith("h"))
+            else:
+                g.set_level_tuning(lvl, int(part.rstrip("p")), preload=part.endswith("p"))
 
     if args.iter_only:
         # A plain device copy of known size (256 MiB read + 256 MiB written per dispatch)
@@ -133,12 +142,15 @@ def main():
         g.copy1DDisparityToHost()
     pcie_pairs_per_s = args.steps / (time.perf_counter() - t0)
 
-    # roofline of the dominant work: the inner TGV iteration at level 0
+    # roofline of the dominant kernel: the inner TGV iterations at level 0, timed with HIP
+    # events on the solver's stream (launches back to back, as the tuner chose them)
     w0, h0, _ = g.level_dims(0)
     ms_it, per_launch = g.time_inner_iterations(0, args.iter_reps)
     t_iter = ms_it / args.iter_reps * 1e-3
-    bytes_iter = B_ITER_FP32 * w0 * h0
-    achieved = bytes_iter / t_iter / 1e9
+    t_launch = t_iter * per_launch
+    bytes_launch = B_ITER_FP32 * w0 * h0 * per_launch
+    achieved = bytes_launch / t_launch / 1e9
+    (tw0, th0), _, _ = g.level_tuning(0, True, with_per_launch=True)
 
     # achievable-bandwidth reference: a plain copy kernel, 1 GiB moved per launch
     copy_ms = g.time_copy(512 << 20, 10) / 10
@@ -166,8 +178,10 @@ def main():
                        "mean": round(float(lat.mean()), 4)},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": profiled_traffic(),
-                     "kernel": f"inner TGV iteration at level 0 ({w0}x{h0}), {per_launch} iteration(s) per launch",
-                     "us_per_iter": round(t_iter * 1e6, 3), "bytes_per_iter": int(bytes_iter),
+                     "kernel": f"inner TGV iterations at level 0 ({w0}x{h0}): {per_launch} per launch, "
+                               f"{tw0}x{th0} tiles",
+                     "us_per_launch": round(t_launch * 1e6, 3), "us_per_iter": round(t_iter * 1e6, 3),
+                     "algorithmic_bytes_per_launch": int(bytes_launch),
                      "copy_kernel_GBs": round(copy_gbs, 1)},
     }
 

@@ -95,6 +95,28 @@ def test_warm_solve_matches_oracle():
         assert err.mean() <= 5e-4, (k, err.mean())
 
 
+@pytest.mark.gpu
+def test_warm_solves_do_not_depend_on_iterations_per_launch():
+    """Cold and warm solves (start levels 1 and 2) give the same disparity whether the
+    inner iterations run one or four per launch."""
+    base, fr = frames(200, 120, 13, 4)
+    out = {}
+    for per_launch in (1, 4):
+        g = gpu_solver(base, 3, 3, 10)
+        g.set_option("per_launch", per_launch)
+        res = []
+        for k, f in enumerate(fr):
+            g.copyImagesToDevice(f["i0"], f["i1"])
+            if k == 0:
+                g.solveStereoForwardMasked()
+            else:
+                g.solve_warm(1 + k % 2, 2, 6)
+            res.append(g.copy1DDisparityToHost())
+        out[per_launch] = res
+    for k, (a, b) in enumerate(zip(out[1], out[4])):
+        assert np.array_equal(a, b), k
+
+
 def make_server(budget=16 << 30, max_active=64):
     from vfs_amd import Server
     return Server(P, hbm_budget_bytes=budget, max_streams=16, max_active=max_active)

@@ -408,7 +408,11 @@ Program build_program(const Params &P, int masked, int warm_level, int warps, in
             roles = vfs::rswap(vfs::rswap(roles, vfs::G_U), vfs::G_UB) ^ vfs::RB_UV;
         }
         for (int l = warm_level; l >= 0; l--) {
-            push_level(pr, P, masked, l, warps, iters, roles, tn.lv[l].per_launch);
+            // Above level 0 a warm program runs one iteration per launch: a multi-iteration
+            // launch leaves the spare buffer (holding what the previous solve left there)
+            // current beyond the level's extent, which coarse levels of a cold solve never
+            // reach but a warm start's finer levels read.
+            push_level(pr, P, masked, l, warps, iters, roles, l > 0 ? 1 : tn.lv[l].per_launch);
             if (l > 0) {
                 push_op(pr, vfs::OP_UP, l, masked, roles);
                 roles = vfs::rswap(vfs::rswap(roles, vfs::G_U), vfs::G_UB) ^ vfs::RB_UV;

@@ -5,11 +5,11 @@ a graph (option "graph" = 0), in program order.  This module rebuilds the cold m
 program's op list (the same sequence the library issues: zero, calibration warp,
 pyramid, then per level the tensor and, per warp, prep / derivatives / inner iterations
 / end, then the upscale) and attributes each dispatch of one solve to its op and level.
-Levels that run two inner iterations per launch (`--pairs`, as the tuner reports them)
-issue `k_iter2` dispatches, ITER2, one per two iterations (plus one ITER when the
-iteration count is odd).
+Levels that run several inner iterations per launch (`--per-launch`, one count per level
+from level 0, as the tuner reports them) issue `k_iterk` dispatches: ITER4 / ITER2 while
+they fit the iteration count, the rest one at a time.
 
-    python -m vfs_amd.profile_ops TRACE.csv --levels 5 --warps 10 --iters 50 --pairs 2,3,4 --skip-tail 100
+    python -m vfs_amd.profile_ops TRACE.csv --levels 5 --warps 10 --iters 50 --per-launch 4,4,4,4,4 --skip-tail 100
 
 `--skip-tail` drops trailing dispatches that follow the solves (e.g. timing launches);
 the solve analysed is the last complete one before them.
@@ -21,11 +21,22 @@ import collections
 import csv
 
 
-def program_ops(levels: int, warps: int, iters: int, pairs=()) -> list[tuple[str, int]]:
+def inner_ops(iters: int, per_launch: int) -> list[str]:
+    """Inner-iteration ops of one warp (the library's greedy split)."""
+    out, it = [], 0
+    while it < iters:
+        k = next((c for c in (4, 2) if c <= per_launch and it + c <= iters), 1)
+        out.append({4: "ITER4", 2: "ITER2", 1: "ITER"}[k])
+        it += k
+    return out
+
+
+def program_ops(levels: int, warps: int, iters: int, per_launch=()) -> list[tuple[str, int]]:
     ops = [("ZERO", 0), ("CALIB", 0)] + [("DOWN", l) for l in range(1, levels)] + [("CINIT", levels - 1)]
     for l in range(levels - 1, -1, -1):
         ops.append(("TENSOR", l))
-        inner = [("ITER2", l)] * (iters // 2) + [("ITER", l)] * (iters % 2) if l in pairs else [("ITER", l)] * iters
+        k = per_launch[l] if l < len(per_launch) else 1
+        inner = [(name, l) for name in inner_ops(iters, k)]
         for _ in range(warps):
             ops += [("PREP", l), ("DERIV", l)] + inner + [("END", l)]
         if l > 0:
@@ -33,18 +44,19 @@ def program_ops(levels: int, warps: int, iters: int, pairs=()) -> list[tuple[str
     return ops
 
 
-def summarize(trace_csv: str, levels: int, warps: int, iters: int, skip_tail: int = 0, pairs=()) -> str:
-    rows = [r for r in csv.DictReader(open(trace_csv)) if r["Kernel_Name"].startswith(("k_tick", "k_iter2"))]
+def summarize(trace_csv: str, levels: int, warps: int, iters: int, skip_tail: int = 0, per_launch=()) -> str:
+    rows = [r for r in csv.DictReader(open(trace_csv)) if r["Kernel_Name"].startswith(("k_tick", "k_iterk"))]
     rows.sort(key=lambda r: int(r["Start_Timestamp"]))
-    seq = program_ops(levels, warps, iters, pairs)
+    seq = program_ops(levels, warps, iters, per_launch)
     n = len(seq)
     end = len(rows) - skip_tail
     if end < n:
         raise ValueError(f"trace holds {end} tick dispatches, one solve needs {n}")
     solve = rows[end - n:end]
     for (kind, lvl), r in zip(seq, solve):
-        if (kind == "ITER2") != r["Kernel_Name"].startswith("k_iter2"):
-            raise ValueError(f"dispatch {r['Kernel_Name']} does not match op {kind} at level {lvl}: check --pairs")
+        if (kind in ("ITER2", "ITER4")) != r["Kernel_Name"].startswith("k_iterk"):
+            raise ValueError(f"dispatch {r['Kernel_Name']} does not match op {kind} at level {lvl}: "
+                             "check --per-launch")
     agg = collections.OrderedDict()
     for (kind, lvl), r in zip(seq, solve):
         d = (int(r["End_Timestamp"]) - int(r["Start_Timestamp"])) * 1e-3
@@ -64,6 +76,12 @@ def summarize(trace_csv: str, levels: int, warps: int, iters: int, skip_tail: in
     for (kind, lvl), (c, s, g) in agg.items():
         by_level[lvl] += s
     out += ["", "per level: " + ", ".join(f"L{l} {v:.0f} us" for l, v in sorted(by_level.items()))]
+    if skip_tail:
+        tail = rows[end:]
+        d = [(int(r["End_Timestamp"]) - int(r["Start_Timestamp"])) * 1e-3 for r in tail]
+        names = sorted({r["Kernel_Name"] for r in tail})
+        out += ["", f"the {len(tail)} dispatches after the solves ({', '.join(names)}): "
+                    f"average {sum(d) / len(d):.2f} us, min {min(d):.2f}, max {max(d):.2f}"]
     return "\n".join(out)
 
 
@@ -74,10 +92,10 @@ def main():
     ap.add_argument("--warps", type=int, default=10)
     ap.add_argument("--iters", type=int, default=50)
     ap.add_argument("--skip-tail", type=int, default=0)
-    ap.add_argument("--pairs", default="", help="comma-separated levels with two iterations per launch")
+    ap.add_argument("--per-launch", default="", help="inner iterations per launch of each level, from level 0")
     a = ap.parse_args()
-    pairs = {int(x) for x in a.pairs.split(",") if x}
-    print(summarize(a.trace, a.levels, a.warps, a.iters, a.skip_tail, pairs))
+    per_launch = [int(x) for x in a.per_launch.split(",") if x]
+    print(summarize(a.trace, a.levels, a.warps, a.iters, a.skip_tail, per_launch))
 
 
 if __name__ == "__main__":
