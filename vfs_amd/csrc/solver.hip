@@ -110,12 +110,6 @@ struct Tuning {
         }
         return k;
     }
-    Tuning single_iterations() const
-    {
-        Tuning t = *this;
-        for (LevelTune &v : t.lv) v.per_launch = 1;
-        return t;
-    }
 };
 
 bool multi_shape(int shape) { return shape >= 0 && shape < vfs::kShapes && vfs::multi_iter_shape(shape); }
@@ -1702,6 +1696,7 @@ struct SrvRequest {
     int warm = 0;
     Program prog;
     size_t pc = 0;
+    int waited = 0;  // steps since this frame's last op was launched
     hipEvent_t done = nullptr;
 };
 
@@ -1722,18 +1717,58 @@ struct vfs_server {
     int max_active = vfs::kMaxEntries;
     int max_slots = 0;
     vfs::PairDesc *d_desc = nullptr;
+    std::vector<vfs::PairDesc> h_desc;  // host copy of every slot's descriptor
     std::vector<int> free_slots;
     std::vector<SrvStream> streams;
     std::vector<SrvRequest *> active, finishing;
     long long queued = 0, seq = 0;
     std::vector<SrvRequest *> done;
     std::vector<hipEvent_t> tick_events;  // in flight, oldest first
+    // recycled host resources: pinned frame buffers (by size in floats) and events, so
+    // that a frame costs no page locking and no event creation once the server is warm
+    std::vector<std::pair<size_t, float *>> pinned_pool;
+    std::vector<hipEvent_t> event_pool;
     long long clock = 0;
     long long ticks = 0, launched_entries = 0;
     std::string err;
 };
 
 namespace {
+
+float *take_pinned(vfs_server *v, size_t floats)
+{
+    for (size_t i = 0; i < v->pinned_pool.size(); i++)
+        if (v->pinned_pool[i].first == floats) {
+            float *p = v->pinned_pool[i].second;
+            v->pinned_pool.erase(v->pinned_pool.begin() + i);
+            return p;
+        }
+    float *p = nullptr;
+    return hipHostMalloc((void **)&p, sizeof(float) * floats, hipHostMallocDefault) == hipSuccess ? p : nullptr;
+}
+
+// Keeps at most 256 buffers for reuse.
+void give_pinned(vfs_server *v, float *p, size_t floats)
+{
+    if (!p) return;
+    if (v->pinned_pool.size() < 256) v->pinned_pool.emplace_back(floats, p);
+    else (void)hipHostFree(p);
+}
+
+hipError_t take_event(vfs_server *v, hipEvent_t *e)
+{
+    if (!v->event_pool.empty()) {
+        *e = v->event_pool.back();
+        v->event_pool.pop_back();
+        return hipSuccess;
+    }
+    return hipEventCreateWithFlags(e, hipEventDisableTiming);
+}
+
+void give_event(vfs_server *v, hipEvent_t e)
+{
+    if (e) v->event_pool.push_back(e);
+}
 
 int fails(vfs_server *v, int code, const char *fmt, ...) __attribute__((format(printf, 3, 4)));
 int fails(vfs_server *v, int code, const char *fmt, ...)
@@ -1797,8 +1832,9 @@ int materialize(vfs_server *v, SrvStream &st)
     v->free_slots.pop_back();
     st.warm_ok = false;
     v->used += st.bytes;
-    const vfs::PairDesc d = s->desc();
-    HIPCHKS(v, hipMemcpyAsync(v->d_desc + st.slot, &d, sizeof d, hipMemcpyHostToDevice, v->stream));
+    v->h_desc[st.slot] = s->desc();
+    HIPCHKS(v, hipMemcpyAsync(v->d_desc + st.slot, &v->h_desc[st.slot], sizeof(vfs::PairDesc),
+                              hipMemcpyHostToDevice, v->stream));
     HIPCHKS(v, hipStreamSynchronize(v->stream));
     return VFS_OK;
 }
@@ -1856,7 +1892,7 @@ int retire_finished(vfs_server *v)
         v->finishing.erase(v->finishing.begin() + i);
     }
     while (!v->tick_events.empty() && hipEventQuery(v->tick_events.front()) == hipSuccess) {
-        (void)hipEventDestroy(v->tick_events.front());
+        give_event(v, v->tick_events.front());
         v->tick_events.erase(v->tick_events.begin());
     }
     return VFS_OK;
@@ -1914,8 +1950,7 @@ int admit(vfs_server *v)
         P.nWarpIters = sp.nWarpIters;
         P.nSolverIters = sp.nSolverIters;
         r->warm = sp.warm_ok ? 1 : 0;
-        // serving launches mix ops of different frames: one iteration per launch
-        const Tuning tn = s->tuning[1].single_iterations();
+        const Tuning &tn = s->tuning[1];
         r->prog = sp.warm_ok ? build_program(P, 1, sp.warm_level, sp.warm_warps, sp.warm_iters, s->roles, tn)
                              : build_program(P, 1, -1, 0, 0, 0, tn);
         r->pc = 0;
@@ -1927,14 +1962,15 @@ int admit(vfs_server *v)
     return VFS_OK;
 }
 
-int launch_one_tick(vfs_server *v)
+// One launch of the next op of every frame in `grp` (ops of one kernel: tick ops, or
+// multi-iteration ops of one iteration count).
+int launch_group(vfs_server *v, const std::vector<SrvRequest *> &grp)
 {
-    if (v->active.empty()) return VFS_OK;
     vfs::TickArgs a = tick_header(v->P);
     // one tile shape per launch: the one chosen for the entry with the most pixels
     int shape = 0;
     long long most = -1;
-    for (SrvRequest *r : v->active) {
+    for (SrvRequest *r : grp) {
         const ProgOp &o = r->prog.ops[r->pc];
         const Dims dl = v->streams[r->stream].solver->lv[vfs::op_tile_level(o.kind, o.level)];
         if ((long long)dl.w * dl.h > most) {
@@ -1943,10 +1979,10 @@ int launch_one_tick(vfs_server *v)
         }
     }
     int acc = 0, k = 0;
-    for (SrvRequest *r : v->active) {
+    for (SrvRequest *r : grp) {
         const SrvStream &st = v->streams[r->stream];
         const ProgOp &o = r->prog.ops[r->pc];
-        const vfs::PairDesc d = st.solver->desc();
+        const vfs::PairDesc &d = v->h_desc[st.slot];
         set_entry(a.e[k], o, st.slot, d, acc);
         acc += vfs::op_tiles(o.kind, o.level, d.w, d.h, d.s, shape);
         k++;
@@ -1957,9 +1993,39 @@ int launch_one_tick(vfs_server *v)
     HIPCHKS(v, hipGetLastError());
     v->ticks++;
     v->launched_entries += k;
+    return VFS_OK;
+}
+
+// One step: one launch advances by one op every active frame whose next op runs on the
+// step's kernel: the tick kernel, or the multi-iteration kernel of one iteration count.
+// The step takes the kernel with the most such frames, unless a frame has waited
+// kMaxWait steps, whose kernel then goes first; the other frames keep their op.
+constexpr int kMaxWait = 4;
+
+int launch_one_tick(vfs_server *v)
+{
+    if (v->active.empty()) return VFS_OK;
+    std::vector<SrvRequest *> groups[3];  // tick ops, two-iteration ops, four-iteration ops
+    int starving = -1;
+    for (SrvRequest *r : v->active) {
+        const int kind = r->prog.ops[r->pc].kind;
+        const int g = kind == vfs::OP_ITER4 ? 2 : kind == vfs::OP_ITER2 ? 1 : 0;
+        groups[g].push_back(r);
+        if (r->waited >= kMaxWait && starving < 0) starving = g;
+    }
+    int pick = starving;
+    if (pick < 0) {
+        pick = 0;
+        for (int g = 1; g < 3; g++)
+            if (groups[g].size() > groups[pick].size()) pick = g;
+    }
+    const int rc = launch_group(v, groups[pick]);
+    if (rc) return rc;
+    for (int g = 0; g < 3; g++)
+        for (SrvRequest *r : groups[g]) r->waited = g == pick ? 0 : r->waited + 1;
     for (size_t i = 0; i < v->active.size();) {
         SrvRequest *r = v->active[i];
-        if (++r->pc < r->prog.ops.size()) {
+        if (r->waited > 0 || ++r->pc < r->prog.ops.size()) {
             i++;
             continue;
         }
@@ -1969,13 +2035,13 @@ int launch_one_tick(vfs_server *v)
         const float *u = s->arena + (long long)vfs::slot_u(vfs::rcur(s->roles, vfs::G_U)) * s->plane;
         HIPCHKS(v, hipMemcpy2DAsync(r->pinned, sizeof(float) * st.width, u, sizeof(float) * s->lv[0].s,
                                     sizeof(float) * st.width, st.height, hipMemcpyDeviceToHost, v->stream));
-        HIPCHKS(v, hipEventCreateWithFlags(&r->done, hipEventDisableTiming));
+        HIPCHKS(v, take_event(v, &r->done));
         HIPCHKS(v, hipEventRecord(r->done, v->stream));
         v->finishing.push_back(r);
         v->active.erase(v->active.begin() + i);
     }
     hipEvent_t ev;
-    HIPCHKS(v, hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+    HIPCHKS(v, take_event(v, &ev));
     HIPCHKS(v, hipEventRecord(ev, v->stream));
     v->tick_events.push_back(ev);
     return VFS_OK;
@@ -1999,6 +2065,7 @@ int vfs_server_create(vfs_server **out, const vfs_server_params *p)
     v->budget = p->hbm_budget_bytes;
     v->max_active = p->max_active;
     v->max_slots = p->max_streams;
+    v->h_desc.resize((size_t)v->max_slots);
     hipError_t e = hipGetDevice(&v->device);
     if (e == hipSuccess) e = hipStreamCreateWithFlags(&v->stream, hipStreamNonBlocking);
     if (e == hipSuccess) e = hipMalloc(&v->d_desc, sizeof(vfs::PairDesc) * v->max_slots);
@@ -2036,6 +2103,8 @@ int vfs_server_destroy(vfs_server *v)
     drop(v->finishing);
     drop(v->done);
     for (hipEvent_t e : v->tick_events) (void)hipEventDestroy(e);
+    for (hipEvent_t e : v->event_pool) (void)hipEventDestroy(e);
+    for (auto &b : v->pinned_pool) (void)hipHostFree(b.second);
     for (SrvStream &st : v->streams) evict(v, st);
     (void)hipFree(v->d_desc);
     (void)hipStreamDestroy(v->stream);
@@ -2080,10 +2149,10 @@ int vfs_server_submit(vfs_server *v, int stream_id, const float *i0, const float
     const size_t n = (size_t)st.width * st.height;
     SrvRequest *r = new (std::nothrow) SrvRequest();
     if (!r) return fails(v, VFS_ERR_NOMEM, "host allocation failed");
-    hipError_t e = hipHostMalloc((void **)&r->pinned, sizeof(float) * 2 * n, hipHostMallocDefault);
-    if (e != hipSuccess) {
+    r->pinned = take_pinned(v, 2 * n);
+    if (!r->pinned) {
         delete r;
-        return fails(v, VFS_ERR_NOMEM, "pinned staging: %s", hipGetErrorString(e));
+        return fails(v, VFS_ERR_NOMEM, "pinned staging of %zu bytes failed", sizeof(float) * 2 * n);
     }
     memcpy(r->pinned, i0, sizeof(float) * n);
     memcpy(r->pinned + n, i1, sizeof(float) * n);
@@ -2125,8 +2194,9 @@ int vfs_server_collect(vfs_server *v, vfs_server_result *out, int max, int *n)
         out[k].warm = r->warm;
         out[k].latency_ms = r->t_done - r->t_submit;
         out[k].ops = (int)r->prog.ops.size();
-        if (r->done) (void)hipEventDestroy(r->done);
-        (void)hipHostFree(r->pinned);
+        give_event(v, r->done);
+        const SrvStream &st = v->streams[r->stream];
+        give_pinned(v, r->pinned, 2 * (size_t)st.width * st.height);
         delete r;
         k++;
     }
