@@ -984,6 +984,24 @@ __device__ __forceinline__ void op_iterk(const Tile &t, int roles, const float4 
 
     auto act = [&](int k) { return !MASKED || L.mk[k] != 0; };
     auto in_tile = [&](int x, int y) { return x >= x0 && x < x0 + B::TW && y >= y0 && y < y0 + B::TH; };
+    // Per own position, once per launch (the mask does not change): in the image, in the
+    // mask, and which of the four neighbours are in the image and the mask.
+    enum : unsigned { F_IN = 1, F_ACT = 2, F_R = 4, F_D = 8, F_L = 16, F_U = 32 };
+    unsigned fl[NK];
+#pragma unroll
+    for (int k = 0; k < NK; k++) {
+        const int i = tid + k * NT, x = ox[k], y = oy[k];
+        fl[k] = 0;
+        if (i < B::NR && x >= 0 && y >= 0 && x < w && y < h) {
+            // neighbours outside the staged region are never asked for
+            const int col = i % B::RW, row = i / B::RW;
+            fl[k] = F_IN | (act(i) ? F_ACT : 0u);
+            if (x + 1 < w && col + 1 < B::RW && act(i + 1)) fl[k] |= F_R;
+            if (y + 1 < h && row + 1 < B::RH && act(i + B::RW)) fl[k] |= F_D;
+            if (x >= 1 && col >= 1 && act(i - 1)) fl[k] |= F_L;
+            if (y >= 1 && row >= 1 && act(i - B::RW)) fl[k] |= F_U;
+        }
+    }
 
     auto iterate = [&](auto full_c) {
         constexpr bool FULL = decltype(full_c)::value;
@@ -1004,16 +1022,15 @@ __device__ __forceinline__ void op_iterk(const Tile &t, int roles, const float4 
             for (int k = 0; k < NK; k++) {
                 const int i = tid + k * NT, x = ox[k], y = oy[k];
                 if (i >= B::NR) continue;
-                if (!FULL && (x < 0 || y < 0 || x >= w || y >= h)) continue;
+                if (!FULL && !(fl[k] & F_ACT && fl[k] & F_IN)) continue;
                 if (x < x0 - 2 * r - 1 || y < y0 - 2 * r - 1 || x > x0 + B::TW - 1 + r || y > y0 + B::TH - 1 + r)
                     continue;
-                if (!FULL && !act(i)) continue;
                 const int kR = (FULL || x + 1 < w) ? i + 1 : i, kD = (FULL || y + 1 < h) ? i + B::RW : i;
                 DualIn rr;
                 rr.ok = true;
                 rr.act = true;
-                rr.hasR = FULL || ((x + 1 < w) && act(kR));
-                rr.hasD = FULL || ((y + 1 < h) && act(kD));
+                rr.hasR = FULL || (fl[k] & F_R) != 0;
+                rr.hasD = FULL || (fl[k] & F_D) != 0;
                 rr.a = o[k].a;
                 rr.b = o[k].b;
                 rr.c = o[k].c;
@@ -1055,9 +1072,9 @@ __device__ __forceinline__ void op_iterk(const Tile &t, int roles, const float4 
             for (int k = 0; k < NK; k++) {
                 const int i = tid + k * NT, x = ox[k], y = oy[k];
                 if (i >= B::NR) continue;
-                if (!FULL && (x < 0 || y < 0 || x >= w || y >= h)) continue;
+                if (!FULL && !(fl[k] & F_IN)) continue;
                 const long pos = (long)y * s + x;
-                if (!FULL && MASKED && !act(i)) {
+                if (!FULL && MASKED && !(fl[k] & F_ACT)) {
                     if (r == 0) {
                         // outside the mask: the spare buffers take the current buffers' values
                         if (in_tile(x, y)) {
@@ -1079,9 +1096,8 @@ __device__ __forceinline__ void op_iterk(const Tile &t, int roles, const float4 
                     u_s = 0.f;
                     vs = make_float2(0.f, 0.f);
                 } else {
-                    const int kR = (FULL || x + 1 < w) ? i + 1 : i, kD = (FULL || y + 1 < h) ? i + B::RW : i;
-                    const bool hR = FULL || ((x + 1 < w) && act(kR)), hD = FULL || ((y + 1 < h) && act(kD));
-                    const bool hL = FULL || (x >= 1 && act(i - 1)), hU = FULL || (y >= 1 && act(i - B::RW));
+                    const bool hR = FULL || (fl[k] & F_R) != 0, hD = FULL || (fl[k] & F_D) != 0;
+                    const bool hL = FULL || (fl[k] & F_L) != 0, hU = FULL || (fl[k] & F_U) != 0;
                     // the left / upper neighbour's Tp: this iteration's inside the mask,
                     // the Tp plane's outside it
                     const float tpLx = (FULL || x >= 1) ? L.tp[i - 1].x : 0.f;
