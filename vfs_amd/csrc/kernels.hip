@@ -1244,6 +1244,47 @@ __device__ __forceinline__ void op_end(const Tile &t, int roles, float limit)
     }
 }
 
+// The end of one warp and the preparation of the next at the same level in one op: each
+// pixel's end-of-warp values (op_end) are all the next warp's preparation (op_prep) reads
+// at that pixel, so the two run per pixel without a launch between them.  Roles as for
+// op_end; the preparation's u is the end's result.
+template <class S, bool MASKED, typename T>
+__device__ __forceinline__ void op_end_prep(const Tile &t, int level, int roles, float limit)
+{
+    const float *u = slot(t, slot_u(rcur(roles, G_U)));
+    float *out = MASKED ? slot(t, slot_u(roth(roles, G_U))) : (float *)u;
+    float *u_ = slot(t, slot_ub(rcur(roles, G_UB)));
+    float2 *uv = slotp<float2>(t, slot_uv(roles, true));
+    float *ulast = slot(t, PL_ULAST);
+    float2 *tv2 = slotp<float2>(t, PL_TV2);
+    float2 *v_ = slotp<float2>(t, slot_v(rcur(roles, G_V)));
+    const float2 *tv = t.d->tv[level];
+    const T *I1 = (const T *)t.d->I1[level];
+    T *i1w = (T *)slot(t, PL_I1W);
+    FOR_TILE_PIXELS(S, t, x, y)
+    {
+        const long pos = (long)y * t.s + x;
+        // op_end
+        const float uval = MASKED ? median5_at(u, x, y, t.w, t.h, t.s) : u[pos];
+        const float ul = ulast[pos];
+        float du = uval - ul;
+        du = (du > limit) ? limit : du;
+        const float un = ul + du;
+        out[pos] = un;
+        u_[pos] = un;
+        const float2 tt = tv2[pos];
+        const float2 o = uv[pos];
+        const float2 d = make_float2(o.x + du * tt.x, o.y + du * tt.y);
+        uv[pos] = d;
+        // op_prep of the next warp
+        const float px = (float)x + d.x, py = (float)y + d.y;
+        tv2[pos] = bil2(tv, t.w, t.h, t.s, px, py);
+        stT(i1w, pos, bil(I1, t.w, t.h, t.s, px, py));
+        ulast[pos] = un;
+        v_[pos] = make_float2(0.f, 0.f);
+    }
+}
+
 // upscale.cu:7-21, 46-61 for u, u_ and warpUV (stereotgv.cu:497-510): bilinear at the
 // finer level's pixel centres, times fScale; writes the other slot of each pair.
 template <class S>
@@ -1356,6 +1397,10 @@ __global__ __launch_bounds__(NT) void k_tick(const PairDesc *__restrict__ D, con
     case OP_END:
         if (E.masked) op_end<S, true>(t, E.roles, A.limit);
         else op_end<S, false>(t, E.roles, A.limit);
+        break;
+    case OP_ENDPREP:
+        if (E.masked) op_end_prep<S, true, T>(t, level, E.roles, A.limit);
+        else op_end_prep<S, false, T>(t, level, E.roles, A.limit);
         break;
     case OP_UP:
         op_up<S>(t, level, E.roles, A.fscale);

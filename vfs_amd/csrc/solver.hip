@@ -322,7 +322,8 @@ void push_level(Program &pr, const Params &P, int masked, int l, int warps, int 
     iter_schedule(P.timestep_lambda, iters, sched.data());
     push_op(pr, vfs::OP_TENSOR, l, masked, roles);
     for (int wi = 0; wi < warps; wi++) {
-        push_op(pr, vfs::OP_PREP, l, masked, roles);
+        // a warp's preparation runs with the previous warp's end (OP_ENDPREP)
+        if (wi == 0) push_op(pr, vfs::OP_PREP, l, masked, roles);
         push_op(pr, vfs::OP_DERIV, l, masked, roles);
         for (int it = 0; it < iters;) {
             ProgOp o{};
@@ -350,7 +351,7 @@ void push_level(Program &pr, const Params &P, int masked, int l, int warps, int 
             pr.ops.push_back(o);
             it += k;
         }
-        push_op(pr, vfs::OP_END, l, masked, roles);
+        push_op(pr, wi + 1 < warps ? vfs::OP_ENDPREP : vfs::OP_END, l, masked, roles);
         if (masked) roles = vfs::rswap(roles, vfs::G_U);
     }
 }

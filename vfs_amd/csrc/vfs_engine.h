@@ -158,7 +158,8 @@ enum OpKind : int {
     OP_END = 9,    // [median +] limit + update at the end of a warp
     OP_UP = 10,    // upscale u, u_, warpUV from `level` to level-1
     OP_ITER2 = 11,  // two fused TGV iterations (recomputed halo)
-    OP_ITER4 = 12   // four fused TGV iterations (recomputed halo)
+    OP_ITER4 = 12,  // four fused TGV iterations (recomputed halo)
+    OP_ENDPREP = 13 // END of a warp and PREP of the next one at the same level
 };
 
 // Iterations of an inner-iteration op.

@@ -4,7 +4,8 @@ Every op of a solve program is one `k_tick` dispatch when the solve is launched 
 a graph (option "graph" = 0), in program order.  This module rebuilds the cold masked
 program's op list (the same sequence the library issues: zero, calibration warp,
 pyramid, then per level the tensor and, per warp, prep / derivatives / inner iterations
-/ end, then the upscale) and attributes each dispatch of one solve to its op and level.
+/ end, each end but the level's last fused with the next warp's prep, then the upscale)
+and attributes each dispatch of one solve to its op and level.
 Levels that run several inner iterations per launch (`--per-launch`, one count per level
 from level 0, as the tuner reports them) issue `k_iterk` dispatches: ITER4 / ITER2 while
 they fit the iteration count, the rest one at a time.
@@ -37,8 +38,9 @@ def program_ops(levels: int, warps: int, iters: int, per_launch=()) -> list[tupl
         ops.append(("TENSOR", l))
         k = per_launch[l] if l < len(per_launch) else 1
         inner = [(name, l) for name in inner_ops(iters, k)]
-        for _ in range(warps):
-            ops += [("PREP", l), ("DERIV", l)] + inner + [("END", l)]
+        for wi in range(warps):
+            ops += ([("PREP", l)] if wi == 0 else []) + [("DERIV", l)] + inner
+            ops.append(("ENDPREP" if wi + 1 < warps else "END", l))
         if l > 0:
             ops.append(("UP", l))
     return ops
