@@ -94,6 +94,9 @@ struct ProgOp {
 struct LevelTune {
     unsigned char shape = 0, preload = 0, per_launch = 1, mshape = 4;
     unsigned char mhalf = 0;  // multi-iteration launches in half workgroups (vfs::kHalfWorkgroup)
+    // with four iterations per launch: tile shape and half flag of the two-iteration
+    // launches that finish an iteration count not divisible by four
+    unsigned char m2shape = 4, m2half = 0;
     // tile shape of the inner iterations
     int iter_shape() const { return per_launch > 1 ? mshape : shape; }
 };
@@ -106,7 +109,7 @@ struct Tuning {
         for (int l = 0; l < kMaxLevels; l++) {
             const int code = lv[l].per_launch == 4 ? 2 : lv[l].per_launch == 2 ? 1 : 0;
             k[l] = (unsigned short)(lv[l].shape | (lv[l].preload << 3) | (code << 4) | (lv[l].mshape << 6) |
-                                    (lv[l].mhalf << 10));
+                                    (lv[l].mhalf << 10) | (lv[l].m2shape << 11) | (lv[l].m2half << 15));
         }
         return k;
     }
@@ -362,12 +365,13 @@ void apply_tuning(Program &pr, const Tuning &tn)
     for (ProgOp &o : pr.ops) {
         const LevelTune &lt = tn.lv[vfs::op_tile_level(o.kind, o.level)];
         const bool multi = o.kind == vfs::OP_ITER2 || o.kind == vfs::OP_ITER4;
-        o.shape = multi ? lt.mshape : lt.shape;
+        const bool rest = o.kind == vfs::OP_ITER2 && lt.per_launch == 4;  // the remainder of a warp
+        o.shape = multi ? (rest ? lt.m2shape : lt.mshape) : lt.shape;
         o.flags &= (unsigned char)~vfs::EF_PRELOAD_TP;
         if (o.kind == vfs::OP_ITER && lt.preload) o.flags |= vfs::EF_PRELOAD_TP;
         // multi-iteration ops exist for some shapes only
         if (multi && !multi_shape(o.shape)) o.shape = 4;
-        if (multi && lt.mhalf && vfs::half_workgroup_ok(o.shape, vfs::op_iterations(o.kind)))
+        if (multi && (rest ? lt.m2half : lt.mhalf) && vfs::half_workgroup_ok(o.shape, vfs::op_iterations(o.kind)))
             o.shape = (unsigned char)(o.shape | vfs::kHalfWorkgroup);
     }
 }
@@ -706,11 +710,17 @@ hipError_t time_launch_choices(hipStream_t st, const vfs::PairDesc *d_desc, cons
                 pick_ms = ms;
             }
         }
+        float best2 = 1e30f;  // the fastest two-iteration launch, for remainders
         for (int c = kSingle; c < kCand; c++) {
             const float ms = best[li * kCand + c];
+            int shape, pre, per_launch, half;
+            cand(c, shape, pre, per_launch, half);
+            if (per_launch == 2 && ms < best2) {
+                best2 = ms;
+                pick.m2shape = (unsigned char)shape;
+                pick.m2half = (unsigned char)half;
+            }
             if (ms < 0.98f * pick_ms) {
-                int shape, pre, per_launch, half;
-                cand(c, shape, pre, per_launch, half);
                 pick.mshape = (unsigned char)shape;
                 pick.per_launch = (unsigned char)per_launch;
                 pick.mhalf = (unsigned char)half;
@@ -757,7 +767,7 @@ int tune_solver(vfs_solver *s, int masked)
     if (s->force_shape >= 0) {
         for (int l = 0; l < s->nLevels; l++) {
             tn.lv[l].shape = (unsigned char)s->force_shape;
-            tn.lv[l].mshape = (unsigned char)(multi_shape(s->force_shape) ? s->force_shape : 4);
+            tn.lv[l].mshape = tn.lv[l].m2shape = (unsigned char)(multi_shape(s->force_shape) ? s->force_shape : 4);
         }
     } else if (s->autotune) {
         std::vector<int> todo;
@@ -789,6 +799,7 @@ int tune_solver(vfs_solver *s, int masked)
         for (int l = 0; l < s->nLevels; l++) {
             tn.lv[l].per_launch = (unsigned char)(ok[l] ? s->force_per_launch : 1);
             if (!multi_shape(tn.lv[l].mshape)) tn.lv[l].mshape = 4;
+            if (!multi_shape(tn.lv[l].m2shape)) tn.lv[l].m2shape = 4;
         }
     }
     s->tuned[m] = true;
@@ -1331,8 +1342,8 @@ int vfs_set_level_tuning(vfs_solver *s, int masked, int level, int shape, int pr
         return fail(s, VFS_ERR_ARG, "several iterations per launch are not allowed at level %d", level);
     LevelTune &t = s->tuning[m].lv[level];
     if (per_launch > 1) {
-        t.mshape = (unsigned char)shape;
-        t.mhalf = (unsigned char)half;
+        t.mshape = t.m2shape = (unsigned char)shape;
+        t.mhalf = t.m2half = (unsigned char)half;
     } else {
         t.shape = (unsigned char)shape;
         t.preload = (unsigned char)(preload != 0);
@@ -1503,7 +1514,7 @@ static int tune_batch(vfs_batch *b, int masked)
     if (f->force_shape >= 0) {
         for (int l = 0; l < f->nLevels; l++) {
             tn.lv[l].shape = (unsigned char)f->force_shape;
-            tn.lv[l].mshape = (unsigned char)(multi_shape(f->force_shape) ? f->force_shape : 4);
+            tn.lv[l].mshape = tn.lv[l].m2shape = (unsigned char)(multi_shape(f->force_shape) ? f->force_shape : 4);
         }
     } else if (f->autotune) {
         std::vector<const vfs_solver *> order(b->members.begin(), b->members.end());
@@ -1549,6 +1560,7 @@ static int tune_batch(vfs_batch *b, int masked)
         for (int l = 0; l < f->nLevels; l++) {
             tn.lv[l].per_launch = (unsigned char)(ok[l] ? f->force_per_launch : 1);
             if (!multi_shape(tn.lv[l].mshape)) tn.lv[l].mshape = 4;
+            if (!multi_shape(tn.lv[l].m2shape)) tn.lv[l].m2shape = 4;
         }
     }
     b->tuned[m] = true;
