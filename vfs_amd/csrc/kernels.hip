@@ -1,9 +1,10 @@
 // gfx950 kernels of the StereoTgv TGV-L1 fisheye-stereo solver.
 //
-// Every per-pixel op of the solve runs inside one tick kernel over a *set of frame pairs*
-// (see vfs_engine.h): workgroups of 256 work-items (4 wave64) own tiles of the launch's
-// shape (64x8 by default), and a prefix table maps the flat workgroup index to
-// (pair, tile).  Pairs of different resolutions therefore share one launch.
+// Every per-pixel op of the solve runs inside one of two kernels over a *set of frame
+// pairs* (see vfs_engine.h): the tick kernel k_tick (one op per launch, workgroups of 256
+// work-items own tiles of the launch's shape, 64x8 by default) and k_iterk (two or four
+// inner iterations per launch over a staged, recomputed halo).  A prefix table maps the
+// flat workgroup index to (pair, tile), so pairs of different resolutions share a launch.
 //
 // Texture fetches of the reference (linear filter, mirror addressing, normalized
 // coordinates) are evaluated in registers: exact bilinear weights at pixel-index
