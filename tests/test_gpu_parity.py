@@ -239,6 +239,25 @@ def test_small_tiles_several_iterations_give_identical_results(masked):
             assert np.array_equal(got, v, equal_nan=True), (shape, per_launch, half, n)
 
 
+@pytest.mark.parametrize("masked", [True, False])
+def test_several_iterations_odd_geometry(masked):
+    """An odd-width pair (157x94, strides 160 / 96 / 64, row padding wider than the
+    small tiles): every multi-iteration tile, four per launch, equals one per launch."""
+    pair = small_pair(157, 94, seed=43)
+    g = make_gpu(pair, 3, 2, 9)
+    g.set_option("per_launch", 1)
+    (g.solveStereoForwardMasked if masked else g.solveStereoForward)()
+    ref = {n: g.read_plane(n, 0) for n in ("u", "u_", "v_", "warpUV")}
+    for shape in (1, 3, 4, 6, 7, 8, 9, 10):
+        g = make_gpu(pair, 3, 2, 9)
+        print(f"odd geometry: shape {shape} masked {masked}", flush=True)
+        for lvl in range(3):
+            g.set_level_tuning(lvl, shape, per_launch=4, masked=masked)
+        (g.solveStereoForwardMasked if masked else g.solveStereoForward)()
+        for n, v in ref.items():
+            assert np.array_equal(g.read_plane(n, 0), v, equal_nan=True), (shape, n)
+
+
 def test_several_iterations_skip_levels_with_mask_in_column_0():
     """A mask reaching column 0 keeps one iteration per launch at those levels."""
     pair = small_pair(160, 120, seed=39)
