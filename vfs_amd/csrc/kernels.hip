@@ -447,23 +447,28 @@ __device__ __forceinline__ void op_deriv(const Tile &t, int level)
 __device__ __forceinline__ float fast_div(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
 __device__ __forceinline__ float fast_sqrt(float a) { return __builtin_amdgcn_sqrtf(a); }
 
+// The inner iteration (dual_step, thresh_core, primal_step) writes every product that is
+// summed as an explicit fused multiply-add (fmaf: one rounding), as the reference's
+// compiler fuses them; explicit, so that every copy of the code rounds alike.
+
 // L1 proximal step (thresholdingL1.cu:180-215), linearised at the current u.
 __device__ __forceinline__ float thresh_core(float divp, float tau, float etau, float u_pos, float upos, float Iu,
                                              float Iz, float lambda)
 {
     const float te = (etau == 0.f) ? tau : fast_div(tau, etau);
-    const float uhat = u_pos + te * divp;
+    const float uhat = fmaf(te, divp, u_pos);
     const float dun = uhat - upos;
-    const float rho = Iu * dun + Iz;
-    const float upper = lambda * te * (Iu * Iu);
-    const float lower = -lambda * te * (Iu * Iu);
+    const float rho = fmaf(Iu, dun, Iz);
+    const float lt = lambda * te;
+    const float upper = lt * (Iu * Iu);
+    const float lower = -upper;
     float du = 0.f;
     if ((rho <= upper) && (rho >= lower)) {
-        du = (Iu == 0.f) ? dun : dun - fast_div(rho, Iu);
+        du = (Iu == 0.f) ? dun : fmaf(-rho, __builtin_amdgcn_rcpf(Iu), dun);
     } else if (rho < lower) {
-        du = dun + lambda * te * Iu;
+        du = fmaf(lt, Iu, dun);
     } else if (rho > upper) {
-        du = dun - lambda * te * Iu;
+        du = fmaf(-lt, Iu, dun);
     }
     return upos + du;
 }
@@ -539,11 +544,11 @@ __device__ __forceinline__ void dual_step(const DualIn &r, int x, int y, int w, 
     } else {
         const float ux = r.hasR ? (r.uR - r.u0 - r.v0.x) : (r.u0 - r.uL - r.v0.x);
         const float uy = r.hasD ? (r.uD - r.u0 - r.v0.y) : (r.u0 - r.uU - r.v0.y);
-        const float dtx = r.a * ux + r.c * uy;
-        const float dty = r.c * ux + r.b * uy;
-        const float px = r.po.x + ip.k1 * dtx;
-        const float py = r.po.y + ip.k1 * dty;
-        float rp = fast_sqrt(px * px + py * py);
+        const float dtx = fmaf(r.a, ux, r.c * uy);
+        const float dty = fmaf(r.c, ux, r.b * uy);
+        const float px = fmaf(ip.k1, dtx, r.po.x);
+        const float py = fmaf(ip.k1, dty, r.po.y);
+        float rp = fast_sqrt(fmaf(px, px, py * py));
         if (rp < 1.0f) rp = 1.0f;
         const float irp = __builtin_amdgcn_rcpf(rp);
         pn = make_float2(px * irp, py * irp);
@@ -562,14 +567,14 @@ __device__ __forceinline__ void dual_step(const DualIn &r, int x, int y, int w, 
             g1 = r.v0.y - r.vU.y;
             g2 = r.v0.x - r.vU.x;
         }
-        const float q0 = r.qo.x + ip.k0 * g0, q1 = r.qo.y + ip.k0 * g1, q2 = r.qo.z + ip.k0 * g2,
-                    q3 = r.qo.w + ip.k0 * g3;
-        float rq = fast_sqrt(q0 * q0 + q1 * q1 + q2 * q2 + q3 * q3);
+        const float q0 = fmaf(ip.k0, g0, r.qo.x), q1 = fmaf(ip.k0, g1, r.qo.y), q2 = fmaf(ip.k0, g2, r.qo.z),
+                    q3 = fmaf(ip.k0, g3, r.qo.w);
+        float rq = fast_sqrt(fmaf(q0, q0, fmaf(q1, q1, fmaf(q2, q2, q3 * q3))));
         if (rq < 1.0f) rq = 1.0f;
         const float irq = __builtin_amdgcn_rcpf(rq);
         qn = make_float4(q0 * irq, q1 * irq, q2 * irq, q3 * irq);
     }
-    tpn = make_float2(r.a * pn.x + r.c * pn.y, r.c * pn.x + r.b * pn.y);
+    tpn = make_float2(fmaf(r.a, pn.x, r.c * pn.y), fmaf(r.c, pn.x, r.b * pn.y));
 }
 
 // Thresholding and primal step at one pixel (thresholdingL1.cu:176-216 / :60-86,
@@ -583,7 +588,7 @@ __device__ __forceinline__ void primal_step(const IterParams &ip, float a, float
                                             float &u_s, float2 &v_s)
 {
     const float aa = ip.alpha1 * ip.alpha1;
-    const float etau = (a * a + b * b + 2 * c * c + (a + c) * (a + c) + (b + c) * (b + c)) * aa;
+    const float etau = fmaf(b + c, b + c, fmaf(a + c, a + c, fmaf(2 * c, c, fmaf(a, a, b * b)))) * aa;
     un = thresh_core(divp, ip.tau, etau, u_pos, uu, iu, iz, ip.lambda);
     float qcx = 0.f, qcw = 0.f, qcy = 0.f, qcz = 0.f;
     if (hR) {
@@ -605,22 +610,25 @@ __device__ __forceinline__ void primal_step(const IterParams &ip, float a, float
     }
     const float dqx = (qcx - qwx) + (qcz - qnz);
     const float dqy = (qcw - qww) + (qcy - qny);
-    const float tx = a * pp.x + c * pp.y;
-    const float tyy = c * pp.x + b * pp.y;
-    const float e1 = aa * (b * b + c * c) + 4 * ip.alpha0 * ip.alpha0;
-    const float e2 = aa * (a * a + c * c) + 4 * ip.alpha0 * ip.alpha0;
+    const float tx = fmaf(a, pp.x, c * pp.y);
+    const float tyy = fmaf(c, pp.x, b * pp.y);
+    const float a04 = 4 * ip.alpha0 * ip.alpha0;
+    const float e1 = fmaf(aa, fmaf(b, b, c * c), a04);
+    const float e2 = fmaf(aa, fmaf(a, a, c * c), a04);
+    const float gx = fmaf(ip.alpha1, tx, ip.alpha0 * dqx);
+    const float gy = fmaf(ip.alpha1, tyy, ip.alpha0 * dqy);
     // e1, e2 >= 4 alpha0^2 > 0 for alpha0 != 0; the reference's zero branches
     // (updatePrimalVariables.cu:228-239, the second writes .x) are kept for alpha0 == 0.
-    float vx = vo.x + ((e1 != 0.f) ? fast_div(ip.tau, e1) : ip.tau) * (ip.alpha1 * tx + ip.alpha0 * dqx);
+    float vx = fmaf((e1 != 0.f) ? fast_div(ip.tau, e1) : ip.tau, gx, vo.x);
     float vy;
     if (e2 != 0.f) {
-        vy = vo.y + fast_div(ip.tau, e2) * (ip.alpha1 * tyy + ip.alpha0 * dqy);
+        vy = fmaf(fast_div(ip.tau, e2), gy, vo.y);
     } else {
-        vx = vo.x + ip.tau * (ip.alpha1 * tx + ip.alpha0 * dqx);
+        vx = fmaf(ip.tau, gx, vo.x);
         vy = 0.f;
     }
-    u_s = un + ip.mu * (un - u_pos);
-    v_s = make_float2(vx + ip.mu * (vx - vo.x), vy + ip.mu * (vy - vo.y));
+    u_s = fmaf(ip.mu, un - u_pos, un);
+    v_s = make_float2(fmaf(ip.mu, vx - vo.x, vx), fmaf(ip.mu, vy - vo.y, vy));
 }
 
 // Divergence of Tp at a pixel (thresholdingL1.cu:157-175 masked / :29-45 unmasked): tp0
