@@ -451,11 +451,35 @@ __device__ __forceinline__ float fast_sqrt(float a) { return __builtin_amdgcn_sq
 // summed as an explicit fused multiply-add (fmaf: one rounding), as the reference's
 // compiler fuses them; explicit, so that every copy of the code rounds alike.
 
-// L1 proximal step (thresholdingL1.cu:180-215), linearised at the current u.
-__device__ __forceinline__ float thresh_core(float divp, float tau, float etau, float u_pos, float upos, float Iu,
+// Per-pixel constants of the primal step (fixed within a warp: they depend on the
+// tensor only): the reciprocals of eta_u (the threshold's step scale) and of the two
+// eta_v, each 1 where the reference divides by zero only in its zero branch.
+struct PrimalCoef {
+    float rt, r1, r2;
+    bool e2z;  // eta_v2 == 0 (alpha0 == 0 and a = c = 0): the reference's zero branch
+};
+
+__device__ __forceinline__ PrimalCoef primal_coef(const IterParams &ip, float a, float b, float c)
+{
+    const float aa = ip.alpha1 * ip.alpha1;
+    const float etau = fmaf(b + c, b + c, fmaf(a + c, a + c, fmaf(2 * c, c, fmaf(a, a, b * b)))) * aa;
+    const float a04 = 4 * ip.alpha0 * ip.alpha0;
+    const float e1 = fmaf(aa, fmaf(b, b, c * c), a04);
+    const float e2 = fmaf(aa, fmaf(a, a, c * c), a04);
+    PrimalCoef k;
+    k.rt = etau == 0.f ? 1.0f : __builtin_amdgcn_rcpf(etau);
+    k.r1 = e1 == 0.f ? 1.0f : __builtin_amdgcn_rcpf(e1);
+    k.e2z = e2 == 0.f;
+    k.r2 = k.e2z ? 1.0f : __builtin_amdgcn_rcpf(e2);
+    return k;
+}
+
+// L1 proximal step (thresholdingL1.cu:180-215), linearised at the current u; rt is
+// 1 / eta_u (1 for eta_u = 0, where the reference takes tau itself).
+__device__ __forceinline__ float thresh_core(float divp, float tau, float rt, float u_pos, float upos, float Iu,
                                              float Iz, float lambda)
 {
-    const float te = (etau == 0.f) ? tau : fast_div(tau, etau);
+    const float te = tau * rt;
     const float uhat = fmaf(te, divp, u_pos);
     const float dun = uhat - upos;
     const float rho = fmaf(Iu, dun, Iz);
@@ -582,14 +606,12 @@ __device__ __forceinline__ void dual_step(const DualIn &r, int x, int y, int w, 
 // of Tp, qc4 / pp the pixel's new q / p, ql / qu the new q of its left / upper
 // neighbours (read only where they are inside the mask).  Returns u, u_ and v_ of the
 // next iteration.
-__device__ __forceinline__ void primal_step(const IterParams &ip, float a, float b, float c, float u_pos, float uu,
-                                            float iu, float iz, float2 vo, bool hR, bool hD, bool hL, bool hU,
-                                            float divp, float4 qc4, float4 ql, float4 qu, float2 pp, float &un,
-                                            float &u_s, float2 &v_s)
+__device__ __forceinline__ void primal_core(const IterParams &ip, const PrimalCoef &k, float a, float b, float c,
+                                            float u_pos, float uu, float iu, float iz, float2 vo, bool hR, bool hD,
+                                            bool hL, bool hU, float divp, float4 qc4, float4 ql, float4 qu,
+                                            float2 pp, float &un, float &u_s, float2 &v_s)
 {
-    const float aa = ip.alpha1 * ip.alpha1;
-    const float etau = fmaf(b + c, b + c, fmaf(a + c, a + c, fmaf(2 * c, c, fmaf(a, a, b * b)))) * aa;
-    un = thresh_core(divp, ip.tau, etau, u_pos, uu, iu, iz, ip.lambda);
+    un = thresh_core(divp, ip.tau, k.rt, u_pos, uu, iu, iz, ip.lambda);
     float qcx = 0.f, qcw = 0.f, qcy = 0.f, qcz = 0.f;
     if (hR) {
         qcx = qc4.x;
@@ -612,23 +634,29 @@ __device__ __forceinline__ void primal_step(const IterParams &ip, float a, float
     const float dqy = (qcw - qww) + (qcy - qny);
     const float tx = fmaf(a, pp.x, c * pp.y);
     const float tyy = fmaf(c, pp.x, b * pp.y);
-    const float a04 = 4 * ip.alpha0 * ip.alpha0;
-    const float e1 = fmaf(aa, fmaf(b, b, c * c), a04);
-    const float e2 = fmaf(aa, fmaf(a, a, c * c), a04);
     const float gx = fmaf(ip.alpha1, tx, ip.alpha0 * dqx);
     const float gy = fmaf(ip.alpha1, tyy, ip.alpha0 * dqy);
-    // e1, e2 >= 4 alpha0^2 > 0 for alpha0 != 0; the reference's zero branches
+    // eta_v1, eta_v2 >= 4 alpha0^2 > 0 for alpha0 != 0; the reference's zero branches
     // (updatePrimalVariables.cu:228-239, the second writes .x) are kept for alpha0 == 0.
-    float vx = fmaf((e1 != 0.f) ? fast_div(ip.tau, e1) : ip.tau, gx, vo.x);
+    float vx = fmaf(ip.tau * k.r1, gx, vo.x);
     float vy;
-    if (e2 != 0.f) {
-        vy = fmaf(fast_div(ip.tau, e2), gy, vo.y);
+    if (!k.e2z) {
+        vy = fmaf(ip.tau * k.r2, gy, vo.y);
     } else {
         vx = fmaf(ip.tau, gx, vo.x);
         vy = 0.f;
     }
     u_s = fmaf(ip.mu, un - u_pos, un);
     v_s = make_float2(fmaf(ip.mu, vx - vo.x, vx), fmaf(ip.mu, vy - vo.y, vy));
+}
+
+__device__ __forceinline__ void primal_step(const IterParams &ip, float a, float b, float c, float u_pos, float uu,
+                                            float iu, float iz, float2 vo, bool hR, bool hD, bool hL, bool hU,
+                                            float divp, float4 qc4, float4 ql, float4 qu, float2 pp, float &un,
+                                            float &u_s, float2 &v_s)
+{
+    primal_core(ip, primal_coef(ip, a, b, c), a, b, c, u_pos, uu, iu, iz, vo, hR, hD, hL, hU, divp, qc4, ql, qu,
+                pp, un, u_s, v_s);
 }
 
 // Divergence of Tp at a pixel (thresholdingL1.cu:157-175 masked / :29-45 unmasked): tp0
@@ -990,6 +1018,10 @@ __device__ __forceinline__ void op_iterk(const Tile &t, int roles, const float4 
                     (MASKED || !outside_circle(x, y, w, h));
     }
     const bool full = __syncthreads_and(inner) != 0;
+    // the primal step's per-pixel constants, once for the K iterations
+    PrimalCoef kc[NK];
+#pragma unroll
+    for (int k = 0; k < NK; k++) kc[k] = primal_coef(ip0, o[k].a, o[k].b, o[k].c);
 
     auto act = [&](int k) { return !MASKED || L.mk[k] != 0; };
     auto in_tile = [&](int x, int y) { return x >= x0 && x < x0 + B::TW && y >= y0 && y < y0 + B::TH; };
@@ -1114,7 +1146,7 @@ __device__ __forceinline__ void op_iterk(const Tile &t, int roles, const float4 
                     const float divp = div_tp<MASKED>(x, y, w, h, hR, hD, hL, hU, L.tp[i], tpLx, tpUy);
                     const float4 ql = hL ? L.q[i - 1] : make_float4(0.f, 0.f, 0.f, 0.f);
                     const float4 qu = hU ? L.q[i - B::RW] : make_float4(0.f, 0.f, 0.f, 0.f);
-                    primal_step(ip, o[k].a, o[k].b, o[k].c, o[k].ub, o[k].u, o[k].iu, o[k].iz, o[k].v, hR, hD, hL,
+                    primal_core(ip, kc[k], o[k].a, o[k].b, o[k].c, o[k].ub, o[k].u, o[k].iu, o[k].iz, o[k].v, hR, hD, hL,
                                 hU, divp, L.q[i], ql, qu, o[k].p, un, u_s, vs);
                 }
                 o[k].u = un;
