@@ -440,12 +440,10 @@ __device__ __forceinline__ void op_deriv(const Tile &t, int level)
     }
 }
 
-// The inner iteration divides and takes square roots several times per pixel; it uses
-// the hardware reciprocal and square root (about 1 ulp) instead of the correctly rounded
-// expansions, which are several times longer.  NaN and infinity propagate as in IEEE
-// division.
-__device__ __forceinline__ float fast_div(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
-__device__ __forceinline__ float fast_sqrt(float a) { return __builtin_amdgcn_sqrtf(a); }
+// The inner iteration divides and normalises several times per pixel; it multiplies by
+// the hardware reciprocal and reciprocal square root (about 1 ulp) instead of the
+// correctly rounded expansions, which are several times longer.  NaN and infinity
+// propagate as in IEEE division.
 
 // The inner iteration (dual_step, thresh_core, primal_step) writes every product that is
 // summed as an explicit fused multiply-add (fmaf: one rounding), as the reference's
@@ -572,9 +570,10 @@ __device__ __forceinline__ void dual_step(const DualIn &r, int x, int y, int w, 
         const float dty = fmaf(r.c, ux, r.b * uy);
         const float px = fmaf(ip.k1, dtx, r.po.x);
         const float py = fmaf(ip.k1, dty, r.po.y);
-        float rp = fast_sqrt(fmaf(px, px, py * py));
-        if (rp < 1.0f) rp = 1.0f;
-        const float irp = __builtin_amdgcn_rcpf(rp);
+        // projection onto the unit ball: p / max(|p|, 1), with the hardware reciprocal
+        // square root
+        const float sp = fmaf(px, px, py * py);
+        const float irp = sp > 1.0f ? __builtin_amdgcn_rsqf(sp) : 1.0f;
         pn = make_float2(px * irp, py * irp);
         float g0, g1, g2, g3;
         if (r.hasR) {
@@ -593,9 +592,8 @@ __device__ __forceinline__ void dual_step(const DualIn &r, int x, int y, int w, 
         }
         const float q0 = fmaf(ip.k0, g0, r.qo.x), q1 = fmaf(ip.k0, g1, r.qo.y), q2 = fmaf(ip.k0, g2, r.qo.z),
                     q3 = fmaf(ip.k0, g3, r.qo.w);
-        float rq = fast_sqrt(fmaf(q0, q0, fmaf(q1, q1, fmaf(q2, q2, q3 * q3))));
-        if (rq < 1.0f) rq = 1.0f;
-        const float irq = __builtin_amdgcn_rcpf(rq);
+        const float sq = fmaf(q0, q0, fmaf(q1, q1, fmaf(q2, q2, q3 * q3)));
+        const float irq = sq > 1.0f ? __builtin_amdgcn_rsqf(sq) : 1.0f;
         qn = make_float4(q0 * irq, q1 * irq, q2 * irq, q3 * irq);
     }
     tpn = make_float2(fmaf(r.a, pn.x, r.c * pn.y), fmaf(r.c, pn.x, r.b * pn.y));
