@@ -454,10 +454,11 @@ __device__ __forceinline__ void op_deriv(const Tile &t, int level)
 // eta_v, each 1 where the reference divides by zero only in its zero branch.
 struct PrimalCoef {
     float rt, r1, r2;
+    float ri, iu2;  // 1 / Iu (0 for Iu = 0) and Iu^2 of the threshold
     bool e2z;  // eta_v2 == 0 (alpha0 == 0 and a = c = 0): the reference's zero branch
 };
 
-__device__ __forceinline__ PrimalCoef primal_coef(const IterParams &ip, float a, float b, float c)
+__device__ __forceinline__ PrimalCoef primal_coef(const IterParams &ip, float a, float b, float c, float iu)
 {
     const float aa = ip.alpha1 * ip.alpha1;
     const float etau = fmaf(b + c, b + c, fmaf(a + c, a + c, fmaf(2 * c, c, fmaf(a, a, b * b)))) * aa;
@@ -469,24 +470,26 @@ __device__ __forceinline__ PrimalCoef primal_coef(const IterParams &ip, float a,
     k.r1 = e1 == 0.f ? 1.0f : __builtin_amdgcn_rcpf(e1);
     k.e2z = e2 == 0.f;
     k.r2 = k.e2z ? 1.0f : __builtin_amdgcn_rcpf(e2);
+    k.ri = iu == 0.f ? 0.f : __builtin_amdgcn_rcpf(iu);
+    k.iu2 = iu * iu;
     return k;
 }
 
-// L1 proximal step (thresholdingL1.cu:180-215), linearised at the current u; rt is
+// L1 proximal step (thresholdingL1.cu:180-215), linearised at the current u; k.rt is
 // 1 / eta_u (1 for eta_u = 0, where the reference takes tau itself).
-__device__ __forceinline__ float thresh_core(float divp, float tau, float rt, float u_pos, float upos, float Iu,
-                                             float Iz, float lambda)
+__device__ __forceinline__ float thresh_core(float divp, float tau, const PrimalCoef &k, float u_pos, float upos,
+                                             float Iu, float Iz, float lambda)
 {
-    const float te = tau * rt;
+    const float te = tau * k.rt;
     const float uhat = fmaf(te, divp, u_pos);
     const float dun = uhat - upos;
     const float rho = fmaf(Iu, dun, Iz);
     const float lt = lambda * te;
-    const float upper = lt * (Iu * Iu);
+    const float upper = lt * k.iu2;
     const float lower = -upper;
     float du = 0.f;
     if ((rho <= upper) && (rho >= lower)) {
-        du = (Iu == 0.f) ? dun : fmaf(-rho, __builtin_amdgcn_rcpf(Iu), dun);
+        du = (Iu == 0.f) ? dun : fmaf(-rho, k.ri, dun);
     } else if (rho < lower) {
         du = fmaf(lt, Iu, dun);
     } else if (rho > upper) {
@@ -609,7 +612,7 @@ __device__ __forceinline__ void primal_core(const IterParams &ip, const PrimalCo
                                             bool hL, bool hU, float divp, float4 qc4, float4 ql, float4 qu,
                                             float2 pp, float &un, float &u_s, float2 &v_s)
 {
-    un = thresh_core(divp, ip.tau, k.rt, u_pos, uu, iu, iz, ip.lambda);
+    un = thresh_core(divp, ip.tau, k, u_pos, uu, iu, iz, ip.lambda);
     float qcx = 0.f, qcw = 0.f, qcy = 0.f, qcz = 0.f;
     if (hR) {
         qcx = qc4.x;
@@ -653,7 +656,7 @@ __device__ __forceinline__ void primal_step(const IterParams &ip, float a, float
                                             float divp, float4 qc4, float4 ql, float4 qu, float2 pp, float &un,
                                             float &u_s, float2 &v_s)
 {
-    primal_core(ip, primal_coef(ip, a, b, c), a, b, c, u_pos, uu, iu, iz, vo, hR, hD, hL, hU, divp, qc4, ql, qu,
+    primal_core(ip, primal_coef(ip, a, b, c, iu), a, b, c, u_pos, uu, iu, iz, vo, hR, hD, hL, hU, divp, qc4, ql, qu,
                 pp, un, u_s, v_s);
 }
 
@@ -1019,7 +1022,7 @@ __device__ __forceinline__ void op_iterk(const Tile &t, int roles, const float4 
     // the primal step's per-pixel constants, once for the K iterations
     PrimalCoef kc[NK];
 #pragma unroll
-    for (int k = 0; k < NK; k++) kc[k] = primal_coef(ip0, o[k].a, o[k].b, o[k].c);
+    for (int k = 0; k < NK; k++) kc[k] = primal_coef(ip0, o[k].a, o[k].b, o[k].c, o[k].iu);
 
     auto act = [&](int k) { return !MASKED || L.mk[k] != 0; };
     auto in_tile = [&](int x, int y) { return x >= x0 && x < x0 + B::TW && y >= y0 && y < y0 + B::TH; };
