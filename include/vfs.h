@@ -144,9 +144,10 @@ int vfs_synchronize(vfs_solver *s);
  *             tiles, one iteration per launch;
  *   "shape"   -1 = as "autotune" says (default), 0..6 = force one tile shape for every
  *             level: 64x8, 64x4, 128x4, 32x16, 32x8, 16x32, 16x16;
- *   "per_launch" -1 = as "autotune" says (default), 1, 2 or 4 = that many inner
+ *   "per_launch" -1 = as "autotune" says (default), 1, 2, 4 or 8 = that many inner
  *             iterations per launch wherever allowed (the workgroup recomputes a halo
- *             of 3 pixels per iteration; levels with a mask pixel in column 0 run one);
+ *             of 3 pixels per iteration; levels with a mask pixel in column 0 run one;
+ *             eight only in the tiles 32x8, 16x16, 20x20, 8x8 and 4x4, else four);
  *   "pair"    the same as "per_launch" -1 / 1 / 2 written as -1 / 0 / 1.
  * Results do not depend on "graph", "autotune", "shape", "per_launch" or "pair".
  * No counterpart in the reference (FP32 only, stereotgv.h:47-100; its block shape is a
@@ -156,12 +157,12 @@ int vfs_get_option(const vfs_solver *s, const char *key, int *value);
 
 /* The launch choices used at `level` by the masked (masked != 0) or unmasked solve:
  * tile shape index (as option "shape"), whether the inner iteration prefetches its
- * Tp-plane neighbours, and the inner iterations per launch, 1, 2 or 4 (`per_launch` may
+ * Tp-plane neighbours, and the inner iterations per launch, 1, 2, 4 or 8 (`per_launch` may
  * be NULL).  Runs the tuner first when it has not run for this geometry.
  * No counterpart in the reference. */
 int vfs_get_level_tuning(vfs_solver *s, int masked, int level, int *shape, int *preload, int *per_launch);
 /* Sets the launch choices of one level (reproducible profiling, expert use); they hold
- * until the next initialize or option change.  `per_launch` (1, 2 or 4) may carry
+ * until the next initialize or option change.  `per_launch` (1, 2, 4 or 8) may carry
  * VFS_HALF_WORKGROUP: multi-iteration workgroups of 512 work-items, two positions each,
  * where the tile's staged region holds 513 .. 1024 positions.
  * No counterpart in the reference. */
@@ -177,7 +178,7 @@ int vfs_time_solves(vfs_solver *s, int masked, int n, float *ms_total);
  * (rounded up to whole launches when the level runs several iterations per launch)
  * launched back to back on the current solver state, HIP events around them.  Returns
  * the time of `reps` iterations in ms and, in the last argument, the iterations per
- * launch (1, 2 or 4).
+ * launch (1, 2, 4 or 8).
  * The solver state is left modified; the next solve starts fresh as always. */
 int vfs_time_inner_iterations(vfs_solver *s, int level, int reps, float *ms_total, int *iters_per_launch);
 

@@ -189,14 +189,15 @@ def test_tile_shapes_give_identical_results(masked):
 @pytest.mark.parametrize("masked", [True, False])
 @pytest.mark.parametrize("shape,storage", [(4, "f32"), (1, "f32"), (6, "f32"), (3, "f32"), (4, "f16"), (6, "bf16")])
 def test_several_iterations_per_launch_give_identical_results(masked, shape, storage):
-    """Two and four inner iterations per launch (recomputed halo, spare buffers, rotated
-    roles; duals rounded to the storage type between iterations) reproduce one iteration
-    per launch bit for bit, including iteration counts that leave a remainder
-    (11 = 4 + 4 + 2 + 1) and the buffers that later ops read outside the mask."""
+    """Two, four and eight inner iterations per launch (recomputed halo, spare buffers,
+    rotated roles; duals rounded to the storage type between iterations) reproduce one
+    iteration per launch bit for bit, including iteration counts that leave a remainder
+    (15 = 4 + 4 + 4 + 2 + 1 = 8 + 4 + 2 + 1; tiles too large for eight take 32x8) and the
+    buffers that later ops read outside the mask."""
     pair = small_pair(200, 120, seed=37)
     ref = None
-    for per_launch in (1, 2, 4):
-        g = make_gpu(pair, 3, 2, 11, storage=storage)
+    for per_launch in (1, 2, 4, 8):
+        g = make_gpu(pair, 3, 2, 15, storage=storage)
         g.set_option("shape", shape)
         g.set_option("per_launch", per_launch)
         print(f"per_launch {per_launch} shape {shape} masked {masked}", flush=True)
@@ -218,14 +219,16 @@ def test_several_iterations_per_launch_give_identical_results(masked, shape, sto
 
 @pytest.mark.parametrize("masked", [True, False])
 def test_small_tiles_several_iterations_give_identical_results(masked):
-    """The 8x8, 4x4, 32x32 and 20x20 tiles of the multi-iteration launches and the half
-    workgroups (set per level) reproduce one iteration per launch bit for bit."""
+    """The 8x8, 4x4, 32x32 and 20x20 tiles of the multi-iteration launches, eight
+    iterations per launch and the half workgroups (set per level) reproduce one
+    iteration per launch bit for bit."""
     pair = small_pair(200, 120, seed=41)
     g = make_gpu(pair, 3, 2, 11)
     g.set_option("per_launch", 1)
     (g.solveStereoForwardMasked if masked else g.solveStereoForward)()
     ref = {n: g.read_plane(n, 0) for n in ("u", "u_", "v_", "warpUV")}
     cases = [(sh, k, False) for sh in (7, 8, 9, 10) for k in (2, 4)] + [(10, 4, True), (6, 4, True), (4, 2, True)]
+    cases += [(sh, 8, False) for sh in (4, 6, 7, 8, 10)] + [(7, 8, True), (8, 8, True)]
     for shape, per_launch, half in cases:
         g = make_gpu(pair, 3, 2, 11)
         print(f"tiles: shape {shape} per_launch {per_launch} half {half} masked {masked}", flush=True)
@@ -239,23 +242,44 @@ def test_small_tiles_several_iterations_give_identical_results(masked):
             assert np.array_equal(got, v, equal_nan=True), (shape, per_launch, half, n)
 
 
+def test_level_tuning_rejects_invalid_choices():
+    """Launch choices the kernels do not have fail with an error and change nothing:
+    eight iterations in tiles whose staged region exceeds 2048 positions (64x4, 32x16,
+    32x32), iteration counts other than 1, 2, 4, 8, half workgroups outside 513 .. 1024
+    staged positions."""
+    from vfs_amd import VfsError
+    g = make_gpu(small_pair(96, 64, seed=45), 2, 1, 4)
+    before = g.level_tuning(0, True, with_per_launch=True)
+    for shape, per_launch, half in [(9, 8, False), (1, 8, False), (3, 8, False), (4, 3, False), (4, 16, False),
+                                    (9, 4, True), (4, 1, True)]:
+        with pytest.raises(VfsError):
+            g.set_level_tuning(0, shape, per_launch=per_launch, half_workgroup=half)
+    assert g.level_tuning(0, True, with_per_launch=True) == before
+    with pytest.raises(VfsError):
+        g.set_option("per_launch", 3)
+    g.set_level_tuning(0, 10, per_launch=8)
+    assert g.level_tuning(0, True, with_per_launch=True)[::2] == (StereoTgv.SHAPES[10], 8)
+
+
 @pytest.mark.parametrize("masked", [True, False])
 def test_several_iterations_odd_geometry(masked):
     """An odd-width pair (157x94, strides 160 / 96 / 64, row padding wider than the
-    small tiles): every multi-iteration tile, four per launch, equals one per launch."""
+    small tiles): every multi-iteration tile, four (and where allowed eight) per launch,
+    equals one per launch."""
     pair = small_pair(157, 94, seed=43)
     g = make_gpu(pair, 3, 2, 9)
     g.set_option("per_launch", 1)
     (g.solveStereoForwardMasked if masked else g.solveStereoForward)()
     ref = {n: g.read_plane(n, 0) for n in ("u", "u_", "v_", "warpUV")}
-    for shape in (1, 3, 4, 6, 7, 8, 9, 10):
+    cases = [(sh, 4) for sh in (1, 3, 4, 6, 7, 8, 9, 10)] + [(sh, 8) for sh in (4, 6, 7, 8, 10)]
+    for shape, per_launch in cases:
         g = make_gpu(pair, 3, 2, 9)
-        print(f"odd geometry: shape {shape} masked {masked}", flush=True)
+        print(f"odd geometry: shape {shape} per_launch {per_launch} masked {masked}", flush=True)
         for lvl in range(3):
-            g.set_level_tuning(lvl, shape, per_launch=4, masked=masked)
+            g.set_level_tuning(lvl, shape, per_launch=per_launch, masked=masked)
         (g.solveStereoForwardMasked if masked else g.solveStereoForward)()
         for n, v in ref.items():
-            assert np.array_equal(g.read_plane(n, 0), v, equal_nan=True), (shape, n)
+            assert np.array_equal(g.read_plane(n, 0), v, equal_nan=True), (shape, per_launch, n)
 
 
 def test_several_iterations_skip_levels_with_mask_in_column_0():
@@ -413,8 +437,8 @@ def test_batch_equals_alone_mixed_sizes():
     b.solve(True)
     for k, s in enumerate(solvers):
         assert np.array_equal(s.copy1DDisparityToHost(), alone[k]), k
-    # batches running two and four iterations per launch give the same results
-    for per_launch in (2, 4):
+    # batches running two, four and eight iterations per launch give the same results
+    for per_launch in (2, 4, 8):
         solvers2 = [make_gpu(p, 3, 3, 10) for p in pairs]
         solvers2[0].set_option("per_launch", per_launch)
         b2 = Batch(solvers2)

@@ -98,10 +98,10 @@ def test_warm_solve_matches_oracle():
 @pytest.mark.gpu
 def test_warm_solves_do_not_depend_on_iterations_per_launch():
     """Cold and warm solves (start levels 1 and 2) give the same disparity whether the
-    inner iterations run one or four per launch."""
+    inner iterations run one, four or eight per launch."""
     base, fr = frames(200, 120, 13, 4)
     out = {}
-    for per_launch in (1, 4):
+    for per_launch in (1, 4, 8):
         g = gpu_solver(base, 3, 3, 10)
         g.set_option("per_launch", per_launch)
         res = []
@@ -113,8 +113,9 @@ def test_warm_solves_do_not_depend_on_iterations_per_launch():
                 g.solve_warm(1 + k % 2, 2, 6)
             res.append(g.copy1DDisparityToHost())
         out[per_launch] = res
-    for k, (a, b) in enumerate(zip(out[1], out[4])):
-        assert np.array_equal(a, b), k
+    for k in range(len(fr)):
+        assert np.array_equal(out[1][k], out[4][k]), k
+        assert np.array_equal(out[1][k], out[8][k]), k
 
 
 def make_server(budget=16 << 30, max_active=64):

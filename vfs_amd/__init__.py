@@ -303,7 +303,7 @@ class StereoTgv:
 
     def set_level_tuning(self, level: int, shape: int, preload: bool = False, per_launch: int = 1,
                          masked: bool = True, half_workgroup: bool = False):
-        """Fix the launch choices of one level (shape index as option "shape"; 1, 2 or 4
+        """Fix the launch choices of one level (shape index as option "shape"; 1, 2, 4 or 8
         inner iterations per launch, optionally in half workgroups)."""
         k = int(per_launch) | (self.HALF_WORKGROUP if half_workgroup else 0)
         return self._check(self._L.vfs_set_level_tuning(self._h, int(masked), level, shape, int(preload), k),

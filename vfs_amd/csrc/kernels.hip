@@ -834,7 +834,7 @@ __device__ __forceinline__ void op_iter(const Tile &t, int roles, const IterPara
 
 // ------------------------------------------------------------------ K iterations per op
 //
-// Temporal blocking of the inner iteration (OP_ITER2, OP_ITER4): one workgroup advances
+// Temporal blocking of the inner iteration (OP_ITER2, OP_ITER4, OP_ITER8): one workgroup advances
 // its tile by K iterations.  Iteration j (r = K - 1 - j iterations after it) computes the
 // duals over the tile grown by 2r + 1 pixels to the left / top and r to the right /
 // bottom, and the primal step over the tile grown by 2r / r: the primal step of a pixel
@@ -1681,14 +1681,16 @@ static_assert(kTickShapes == 7, "tick_storage dispatches every tick shape");
 template <typename T, int SH, int K>
 void iterk_shape(const PairDesc *desc, const TickArgs &args, bool half, hipStream_t st)
 {
-    const dim3 grid((unsigned)args.total);
-    if constexpr (half_workgroup_ok(SH, K)) {
-        if (half) {
-            k_iterk<T, SH, K, true><<<grid, dim3(BlkK<SH, K, true>::NT), 0, st>>>(desc, args);
-            return;
+    if constexpr (iterk_ok(SH, K)) {  // the host asks for no other
+        const dim3 grid((unsigned)args.total);
+        if constexpr (half_workgroup_ok(SH, K)) {
+            if (half) {
+                k_iterk<T, SH, K, true><<<grid, dim3(BlkK<SH, K, true>::NT), 0, st>>>(desc, args);
+                return;
+            }
         }
+        k_iterk<T, SH, K, false><<<grid, dim3(BlkK<SH, K, false>::NT), 0, st>>>(desc, args);
     }
-    k_iterk<T, SH, K, false><<<grid, dim3(BlkK<SH, K, false>::NT), 0, st>>>(desc, args);
 }
 
 template <typename T, int K>
@@ -1717,6 +1719,10 @@ void launch_tick(const PairDesc *desc, const TickArgs &args, int storage, int sh
     }
     if (args.e[0].kind == OP_ITER4) {
         STORAGE_DISPATCH(storage, (iterk_storage<T, 4>(desc, args, shape, st)));
+        return;
+    }
+    if (args.e[0].kind == OP_ITER8) {
+        STORAGE_DISPATCH(storage, (iterk_storage<T, 8>(desc, args, shape, st)));
         return;
     }
     STORAGE_DISPATCH(storage, tick_storage<T>(desc, args, shape, st));

@@ -83,19 +83,19 @@ struct ProgOp {
     unsigned char kind, level, masked, flags, shape;
     int roles;
     float sigma, tau, mu;  // OP_ITER
-    int it0;               // OP_ITER2 / OP_ITER4: first iteration's index in the warp
+    int it0;               // OP_ITER2 / OP_ITER4 / OP_ITER8: first iteration's index in the warp
 };
 
 // Launch choices for one pyramid level: tile shape of the level's one-iteration launches
 // (every op but the multi-iteration ones), whether the inner iteration fetches its
 // Tp-plane neighbours up front (vfs::EF_PRELOAD_TP), how many inner iterations share one
-// launch (1, 2 = vfs::OP_ITER2 or 4 = vfs::OP_ITER4) and the tile shape of those
-// multi-iteration launches.
+// launch (1, 2 = vfs::OP_ITER2, 4 = vfs::OP_ITER4 or 8 = vfs::OP_ITER8) and the tile
+// shape of those multi-iteration launches.
 struct LevelTune {
     unsigned char shape = 0, preload = 0, per_launch = 1, mshape = 4;
     unsigned char mhalf = 0;  // multi-iteration launches in half workgroups (vfs::kHalfWorkgroup)
-    // with four iterations per launch: tile shape and half flag of the two-iteration
-    // launches that finish an iteration count not divisible by four
+    // with four or eight iterations per launch: tile shape and half flag of the
+    // two-iteration launches that finish an iteration count not divisible by four
     unsigned char m2shape = 4, m2half = 0;
     // tile shape of the inner iterations
     int iter_shape() const { return per_launch > 1 ? mshape : shape; }
@@ -107,7 +107,7 @@ struct Tuning {
     {
         std::array<unsigned short, kMaxLevels> k{};
         for (int l = 0; l < kMaxLevels; l++) {
-            const int code = lv[l].per_launch == 4 ? 2 : lv[l].per_launch == 2 ? 1 : 0;
+            const int pl = lv[l].per_launch, code = pl == 8 ? 3 : pl == 4 ? 2 : pl == 2 ? 1 : 0;
             k[l] = (unsigned short)(lv[l].shape | (lv[l].preload << 3) | (code << 4) | (lv[l].mshape << 6) |
                                     (lv[l].mhalf << 10) | (lv[l].m2shape << 11) | (lv[l].m2half << 15));
         }
@@ -116,7 +116,12 @@ struct Tuning {
 };
 
 bool multi_shape(int shape) { return shape >= 0 && shape < vfs::kShapes && vfs::multi_iter_shape(shape); }
-bool valid_per_launch(int k) { return k == 1 || k == 2 || k == 4; }
+bool valid_per_launch(int k) { return k == 1 || k == 2 || k == 4 || k == 8; }
+// The op of `k` iterations per launch.
+unsigned char iter_kind(int k)
+{
+    return (unsigned char)(k == 8 ? vfs::OP_ITER8 : k == 4 ? vfs::OP_ITER4 : k == 2 ? vfs::OP_ITER2 : vfs::OP_ITER);
+}
 
 // Step sizes (sigma, tau, mu) of the first n inner iterations of a warp
 // (stereotgv.cu:449-477): tau = 1, sigma = 1 / tau at the start of every warp;
@@ -156,7 +161,7 @@ struct vfs_solver {
     bool use_graph = true;
     int force_shape = -1;   // option "shape": -1 = per-level tuner
     bool autotune = true;   // option "autotune"
-    int force_per_launch = -1;  // option "per_launch": -1 = tuner, else 1, 2 or 4 wherever allowed
+    int force_per_launch = -1;  // option "per_launch": -1 = tuner, else 1, 2, 4 or 8 wherever allowed
     bool mask_col0[kMaxLevels] = {};  // a mask pixel lies in column 0 of the level
     Tuning tuning[2];       // [masked] launch choices per level
     bool tuned[2] = {false, false};
@@ -336,10 +341,10 @@ void push_level(Program &pr, const Params &P, int masked, int l, int warps, int 
             // K iterations in one op while they fit the iteration count and the
             // descriptors' schedule table; the rest one or two at a time
             int k = 1;
-            for (int c : {4, 2})
+            for (int c : {8, 4, 2})
                 if (k == 1 && c <= per_launch && it + c <= iters && it + c <= vfs::kMaxSched) k = c;
             if (k > 1) {
-                o.kind = (unsigned char)(k == 4 ? vfs::OP_ITER4 : vfs::OP_ITER2);
+                o.kind = iter_kind(k);
                 o.it0 = it;
                 o.flags = (unsigned char)((it == 0 ? vfs::EF_FIRST : 0) | (it + k == iters ? vfs::EF_WRITE_TP_LAST : 0));
                 roles = vfs::rrotate_iter(roles);
@@ -364,13 +369,13 @@ void apply_tuning(Program &pr, const Tuning &tn)
 {
     for (ProgOp &o : pr.ops) {
         const LevelTune &lt = tn.lv[vfs::op_tile_level(o.kind, o.level)];
-        const bool multi = o.kind == vfs::OP_ITER2 || o.kind == vfs::OP_ITER4;
-        const bool rest = o.kind == vfs::OP_ITER2 && lt.per_launch == 4;  // the remainder of a warp
+        const bool multi = vfs::op_multi(o.kind);
+        const bool rest = o.kind == vfs::OP_ITER2 && lt.per_launch >= 4;  // the remainder of a warp
         o.shape = multi ? (rest ? lt.m2shape : lt.mshape) : lt.shape;
         o.flags &= (unsigned char)~vfs::EF_PRELOAD_TP;
         if (o.kind == vfs::OP_ITER && lt.preload) o.flags |= vfs::EF_PRELOAD_TP;
         // multi-iteration ops exist for some shapes only
-        if (multi && !multi_shape(o.shape)) o.shape = 4;
+        if (multi && !vfs::iterk_ok(o.shape, vfs::op_iterations(o.kind))) o.shape = 4;
         if (multi && (rest ? lt.m2half : lt.mhalf) && vfs::half_workgroup_ok(o.shape, vfs::op_iterations(o.kind)))
             o.shape = (unsigned char)(o.shape | vfs::kHalfWorkgroup);
     }
@@ -627,9 +632,9 @@ hipError_t time_launch_choices(hipStream_t st, const vfs::PairDesc *d_desc, cons
                                const std::vector<bool> &allow_multi, Tuning &tn)
 {
     constexpr int kWarm = 3, kReps = 20, kRounds = 2;
-    // candidates: (shape, preload) of one iteration per launch, then two- and
-    // four-iteration launches for the shapes that have them, in full or half workgroups
-    constexpr int kSingle = vfs::kTickShapes * 2, kCand = kSingle + 2 * vfs::kShapes * 2;
+    // candidates: (shape, preload) of one iteration per launch, then two-, four- and
+    // eight-iteration launches for the shapes that have them, in full or half workgroups
+    constexpr int kSingle = vfs::kTickShapes * 2, kCand = kSingle + 3 * vfs::kShapes * 2;
     auto cand = [&](int c, int &shape, int &pre, int &per_launch, int &half) {
         if (c < kSingle) {
             shape = c / 2;
@@ -638,7 +643,7 @@ hipError_t time_launch_choices(hipStream_t st, const vfs::PairDesc *d_desc, cons
             half = 0;
         } else {
             const int m = c - kSingle;
-            per_launch = m < vfs::kShapes * 2 ? 2 : 4;
+            per_launch = 2 << (m / (vfs::kShapes * 2));
             shape = (m % (vfs::kShapes * 2)) / 2;
             half = m % 2;
             pre = 0;
@@ -654,11 +659,11 @@ hipError_t time_launch_choices(hipStream_t st, const vfs::PairDesc *d_desc, cons
             for (int c = 0; c < kCand && r == hipSuccess; c++) {
                 int shape, pre, per_launch, half;
                 cand(c, shape, pre, per_launch, half);
-                if (per_launch > 1 && (!multi_shape(shape) || !allow_multi[li] ||
+                if (per_launch > 1 && (!vfs::iterk_ok(shape, per_launch) || !allow_multi[li] ||
                                        (half && !vfs::half_workgroup_ok(shape, per_launch))))
                     continue;
                 ProgOp o{};
-                o.kind = (unsigned char)(per_launch == 4 ? vfs::OP_ITER4 : per_launch == 2 ? vfs::OP_ITER2 : vfs::OP_ITER);
+                o.kind = iter_kind(per_launch);
                 o.level = (unsigned char)levels[li];
                 o.masked = (unsigned char)masked;
                 o.flags = (unsigned char)(pre ? vfs::EF_PRELOAD_TP : 0);
@@ -1287,7 +1292,8 @@ int vfs_set_option(vfs_solver *s, const char *key, int value)
             if (value < -1 || value > 1) return fail(s, VFS_ERR_ARG, "pair must be -1, 0 or 1");
             value = value < 0 ? -1 : value + 1;
         }
-        if (value != -1 && !valid_per_launch(value)) return fail(s, VFS_ERR_ARG, "per_launch must be -1, 1, 2 or 4");
+        if (value != -1 && !valid_per_launch(value))
+            return fail(s, VFS_ERR_ARG, "per_launch must be -1, 1, 2, 4 or 8");
         s->force_per_launch = value;
         s->tuned[0] = s->tuned[1] = false;
         return VFS_OK;
@@ -1330,9 +1336,10 @@ int vfs_set_level_tuning(vfs_solver *s, int masked, int level, int shape, int pr
     per_launch &= ~VFS_HALF_WORKGROUP;
     if (!valid_per_launch(per_launch) || (half && (per_launch == 1 || !multi_shape(shape) ||
                                                    !vfs::half_workgroup_ok(shape, per_launch))))
-        return fail(s, VFS_ERR_ARG, "per_launch must be 1, 2 or 4 (half workgroups where the staged region "
+        return fail(s, VFS_ERR_ARG, "per_launch must be 1, 2, 4 or 8 (half workgroups where the staged region "
                                     "holds 513 .. 1024 positions)");
-    if (per_launch > 1 ? !multi_shape(shape) : (shape < 0 || shape >= vfs::kTickShapes))
+    if (per_launch > 1 ? !(multi_shape(shape) && vfs::iterk_ok(shape, per_launch))
+                       : (shape < 0 || shape >= vfs::kTickShapes))
         return fail(s, VFS_ERR_ARG, "bad shape %d for %d iteration(s) per launch", shape, per_launch);
     const int m = masked ? 1 : 0;
     int rc = set_device(s);
@@ -2018,23 +2025,24 @@ constexpr int kMaxWait = 4;
 int launch_one_tick(vfs_server *v)
 {
     if (v->active.empty()) return VFS_OK;
-    std::vector<SrvRequest *> groups[3];  // tick ops, two-iteration ops, four-iteration ops
+    constexpr int kGroups = 4;
+    std::vector<SrvRequest *> groups[kGroups];  // tick ops, then ops of 2, 4 and 8 iterations
     int starving = -1;
     for (SrvRequest *r : v->active) {
         const int kind = r->prog.ops[r->pc].kind;
-        const int g = kind == vfs::OP_ITER4 ? 2 : kind == vfs::OP_ITER2 ? 1 : 0;
+        const int g = kind == vfs::OP_ITER8 ? 3 : kind == vfs::OP_ITER4 ? 2 : kind == vfs::OP_ITER2 ? 1 : 0;
         groups[g].push_back(r);
         if (r->waited >= kMaxWait && starving < 0) starving = g;
     }
     int pick = starving;
     if (pick < 0) {
         pick = 0;
-        for (int g = 1; g < 3; g++)
+        for (int g = 1; g < kGroups; g++)
             if (groups[g].size() > groups[pick].size()) pick = g;
     }
     const int rc = launch_group(v, groups[pick]);
     if (rc) return rc;
-    for (int g = 0; g < 3; g++)
+    for (int g = 0; g < kGroups; g++)
         for (SrvRequest *r : groups[g]) r->waited = g == pick ? 0 : r->waited + 1;
     for (size_t i = 0; i < v->active.size();) {
         SrvRequest *r = v->active[i];

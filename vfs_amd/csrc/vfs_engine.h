@@ -22,7 +22,7 @@ constexpr int kMaxSched = 128;
 // Tile shapes.  Shapes 0 .. kTickShapes - 1 are those of the tick kernel: a workgroup is
 // 256 work-items laid out tw x (256 / tw); each work-item covers th / (256 / tw) rows (1
 // or 2) of the tw x th tile.  Shape 0 is the default; the others are candidates of the
-// per-level tile-shape tuner.  The multi-iteration ops (OP_ITER2, OP_ITER4) take the
+// per-level tile-shape tuner.  The multi-iteration ops (OP_ITER2, OP_ITER4, OP_ITER8) take the
 // shapes of multi_iter_shape(), including the small tiles 8x8 and 4x4 for the coarsest,
 // latency-bound levels and 32x32 and 20x20 for the largest.
 constexpr int kShapes = 11, kTickShapes = 7;
@@ -51,6 +51,12 @@ __host__ __device__ constexpr bool half_workgroup_ok(int shape, int K)
     return iterk_region(shape, K) > 512 && iterk_region(shape, K) <= 1024;
 }
 __host__ __device__ constexpr int shape_index(int launch_shape) { return launch_shape & (kHalfWorkgroup - 1); }
+// Tile shapes of K-iteration launches: every multi-iteration shape for K = 2 and 4; for
+// K = 8 those whose staged region holds at most 2048 positions.
+__host__ __device__ constexpr bool iterk_ok(int shape, int K)
+{
+    return multi_iter_shape(shape) && (K <= 4 || iterk_region(shape, K) <= 2048);
+}
 
 // Storage precision of the image pyramid, warped image, derivatives, tensor and duals.
 // Arithmetic is FP32 in every mode.
@@ -159,11 +165,16 @@ enum OpKind : int {
     OP_UP = 10,    // upscale u, u_, warpUV from `level` to level-1
     OP_ITER2 = 11,  // two fused TGV iterations (recomputed halo)
     OP_ITER4 = 12,  // four fused TGV iterations (recomputed halo)
-    OP_ENDPREP = 13 // END of a warp and PREP of the next one at the same level
+    OP_ENDPREP = 13, // END of a warp and PREP of the next one at the same level
+    OP_ITER8 = 14   // eight fused TGV iterations (recomputed halo)
 };
 
 // Iterations of an inner-iteration op.
-inline int op_iterations(int kind) { return kind == OP_ITER4 ? 4 : kind == OP_ITER2 ? 2 : kind == OP_ITER ? 1 : 0; }
+inline int op_iterations(int kind)
+{
+    return kind == OP_ITER8 ? 8 : kind == OP_ITER4 ? 4 : kind == OP_ITER2 ? 2 : kind == OP_ITER ? 1 : 0;
+}
+inline bool op_multi(int kind) { return kind == OP_ITER2 || kind == OP_ITER4 || kind == OP_ITER8; }
 
 // Entries per launch: one per lane of a wave64 (the kernel's entry lookup), and 64
 // entries of 56 bytes keep the kernel arguments under 4 KiB.
@@ -174,7 +185,7 @@ enum EntryFlag : int {
     EF_FIRST = 1,          // inner-iteration ops: p and q were just zeroed
     EF_WRITE_TP = 2,       // ITER: the iteration also stores Tp
     EF_PRELOAD_TP = 4,     // ITER: fetch the Tp-plane neighbours with the other operands
-    EF_WRITE_TP_LAST = 8   // ITER2 / ITER4: the op's last iteration stores Tp
+    EF_WRITE_TP_LAST = 8   // ITER2 / ITER4 / ITER8: the op's last iteration stores Tp
 };
 
 // One (pair, op) of a launch.  The geometry and arena of the op's tile level travel in
@@ -189,7 +200,7 @@ struct TickEntry {
     unsigned char kind, level, masked, flags;
     unsigned int roles;        // buffer roles before the op (see Group)
     float sigma, tau, mu;      // ITER: step sizes of the iteration
-    int it0;                   // ITER2 / ITER4: index of the first iteration in the warp
+    int it0;                   // ITER2 / ITER4 / ITER8: index of the first iteration in the warp
 };
 static_assert(sizeof(TickEntry) == 56, "TickEntry layout");
 

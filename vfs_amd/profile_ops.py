@@ -7,7 +7,7 @@ pyramid, then per level the tensor and, per warp, prep / derivatives / inner ite
 / end, each end but the level's last fused with the next warp's prep, then the upscale)
 and attributes each dispatch of one solve to its op and level.
 Levels that run several inner iterations per launch (`--per-launch`, one count per level
-from level 0, as the tuner reports them) issue `k_iterk` dispatches: ITER4 / ITER2 while
+from level 0, as the tuner reports them) issue `k_iterk` dispatches: ITER8 / ITER4 / ITER2 while
 they fit the iteration count, the rest one at a time.
 
     python -m vfs_amd.profile_ops TRACE.csv --levels 5 --warps 10 --iters 50 --per-launch 4,4,4,4,4 --skip-tail 100
@@ -26,8 +26,8 @@ def inner_ops(iters: int, per_launch: int) -> list[str]:
     """Inner-iteration ops of one warp (the library's greedy split)."""
     out, it = [], 0
     while it < iters:
-        k = next((c for c in (4, 2) if c <= per_launch and it + c <= iters), 1)
-        out.append({4: "ITER4", 2: "ITER2", 1: "ITER"}[k])
+        k = next((c for c in (8, 4, 2) if c <= per_launch and it + c <= iters), 1)
+        out.append({8: "ITER8", 4: "ITER4", 2: "ITER2", 1: "ITER"}[k])
         it += k
     return out
 
@@ -56,7 +56,7 @@ def summarize(trace_csv: str, levels: int, warps: int, iters: int, skip_tail: in
         raise ValueError(f"trace holds {end} tick dispatches, one solve needs {n}")
     solve = rows[end - n:end]
     for (kind, lvl), r in zip(seq, solve):
-        if (kind in ("ITER2", "ITER4")) != r["Kernel_Name"].startswith("k_iterk"):
+        if (kind in ("ITER2", "ITER4", "ITER8")) != r["Kernel_Name"].startswith("k_iterk"):
             raise ValueError(f"dispatch {r['Kernel_Name']} does not match op {kind} at level {lvl}: "
                              "check --per-launch")
     agg = collections.OrderedDict()
