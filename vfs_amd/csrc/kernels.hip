@@ -1706,6 +1706,7 @@ void iterk_storage(const PairDesc *desc, const TickArgs &args, int launch_shape,
     case 8: iterk_shape<T, 8, K>(desc, args, half, st); break;
     case 9: iterk_shape<T, 9, K>(desc, args, half, st); break;
     case 10: iterk_shape<T, 10, K>(desc, args, half, st); break;
+    case 11: iterk_shape<T, 11, K>(desc, args, half, st); break;
     default: break;  // the host never asks for another shape
     }
 }

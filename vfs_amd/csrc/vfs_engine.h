@@ -24,13 +24,14 @@ constexpr int kMaxSched = 128;
 // or 2) of the tw x th tile.  Shape 0 is the default; the others are candidates of the
 // per-level tile-shape tuner.  The multi-iteration ops (OP_ITER2, OP_ITER4, OP_ITER8) take the
 // shapes of multi_iter_shape(), including the small tiles 8x8 and 4x4 for the coarsest,
-// latency-bound levels and 32x32 and 20x20 for the largest.
-constexpr int kShapes = 11, kTickShapes = 7;
+// latency-bound levels and 32x32, 20x20 and 36x25 for the largest (36x25 splits an
+// 848x800 image into 768 tiles: three workgroups for each of 256 compute units).
+constexpr int kShapes = 12, kTickShapes = 7;
 struct ShapeDim {
     int tw, th;
 };
 constexpr ShapeDim kShape[kShapes] = {{64, 8}, {64, 4}, {128, 4}, {32, 16}, {32, 8}, {16, 32}, {16, 16},
-                                      {8, 8}, {4, 4}, {32, 32}, {20, 20}};
+                                      {8, 8}, {4, 4}, {32, 32}, {20, 20}, {36, 25}};
 __host__ __device__ constexpr bool multi_iter_shape(int shape)
 {
     return shape == 1 || shape == 3 || (shape >= 4 && shape != 5);
