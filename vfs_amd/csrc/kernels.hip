@@ -904,7 +904,7 @@ __device__ __forceinline__ float rT(float v)
     return Cv<T>::f(Cv<T>::t(v));
 }
 
-// The step sizes of iteration j: sched[j] = (sigma, tau, mu); `ip0` holds the
+// The step sizes of iteration j: sched[j] = (sigma, tau, mu, alpha1 sigma / 3); `ip0` holds the
 // schedule-independent fields and the flags of the op.
 template <int SH, int K, bool HALF, bool MASKED, typename T>
 __device__ __forceinline__ void op_iterk(const Tile &t, int roles, const float4 *__restrict__ sched,
@@ -1051,7 +1051,7 @@ __device__ __forceinline__ void op_iterk(const Tile &t, int roles, const float4 
         for (int j = 0; j < K; j++) {
             IterParams ip = ip0;
             const float4 sc = sched[j];
-            ip.k1 = ip0.alpha1 * sc.x / 3.0f;
+            ip.k1 = sc.w;  // alpha1 sigma / 3, as op_iter forms it
             ip.k0 = ip0.alpha0 * sc.x / 2.0f;
             ip.tau = sc.y;
             ip.mu = sc.z;

@@ -213,6 +213,9 @@ struct vfs_solver {
         }
         d.I0[0] = storage == vfs::kF32 ? (void *)in_i0 : (void *)i0p0;
         iter_schedule(timestep_lambda, vfs::kMaxSched, d.sched);
+        // k1 = alpha1 sigma / 3 of each iteration, rounded as the one-iteration kernel
+        // forms it (float product, correctly rounded division)
+        for (int i = 0; i < vfs::kMaxSched; i++) d.sched[i].w = alpha1 * d.sched[i].x / 3.0f;
         return d;
     }
 };

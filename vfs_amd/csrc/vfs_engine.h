@@ -131,8 +131,9 @@ struct PairDesc {
     void *I1[kMaxLevels];
     const float *mask[kMaxLevels];
     const float2 *tv[kMaxLevels];
-    // step sizes (sigma, tau, mu, 0) of inner iteration k of every warp
-    // (stereotgv.cu:449-477); the multi-iteration ops read them
+    // step sizes (sigma, tau, mu) of inner iteration k of every warp
+    // (stereotgv.cu:449-477) and k1 = alpha1 sigma / 3 (float product, correctly rounded
+    // division); the multi-iteration ops read them
     float4 sched[kMaxSched];
 };
 
