@@ -706,9 +706,9 @@ hipError_t time_launch_choices(hipStream_t st, const vfs::PairDesc *d_desc, cons
         const int l = levels[li];
         const LevelTune def = default_tune(l);
         LevelTune pick = def;
-        // the one-iteration launch (also the shape of the level's other ops), then a
-        // multi-iteration launch where it is faster still; each leaves the current
-        // choice only for a clear gain
+        // the one-iteration launch (also the shape of the level's other ops), leaving the
+        // default only for a clear gain; then the fastest multi-iteration launch where it
+        // is clearly faster still
         float pick_ms = best[li * kCand + def.shape * 2 + def.preload];
         for (int c = 0; c < kSingle; c++) {
             const float ms = best[li * kCand + c];
@@ -719,6 +719,7 @@ hipError_t time_launch_choices(hipStream_t st, const vfs::PairDesc *d_desc, cons
             }
         }
         float best2 = 1e30f;  // the fastest two-iteration launch, for remainders
+        int bestm = -1;       // the fastest multi-iteration launch
         for (int c = kSingle; c < kCand; c++) {
             const float ms = best[li * kCand + c];
             int shape, pre, per_launch, half;
@@ -728,12 +729,14 @@ hipError_t time_launch_choices(hipStream_t st, const vfs::PairDesc *d_desc, cons
                 pick.m2shape = (unsigned char)shape;
                 pick.m2half = (unsigned char)half;
             }
-            if (ms < 0.98f * pick_ms) {
-                pick.mshape = (unsigned char)shape;
-                pick.per_launch = (unsigned char)per_launch;
-                pick.mhalf = (unsigned char)half;
-                pick_ms = ms;
-            }
+            if (bestm < 0 || ms < best[li * kCand + bestm]) bestm = c;
+        }
+        if (bestm >= 0 && best[li * kCand + bestm] < 0.98f * pick_ms) {
+            int shape, pre, per_launch, half;
+            cand(bestm, shape, pre, per_launch, half);
+            pick.mshape = (unsigned char)shape;
+            pick.per_launch = (unsigned char)per_launch;
+            pick.mhalf = (unsigned char)half;
         }
         tn.lv[l] = pick;
     }
