@@ -27,17 +27,18 @@ HBM_PEAK_GBS = 8000.0
 # Iz and write u, u_, v_(2), p(2), q(4) = 25 floats.  A launch of K iterations is
 # credited K times that.
 B_ITER_FP32 = 100.0
-# HBM bytes per launch of the level-0 inner-iteration kernel (32x32 tiles, four
-# iterations per launch) measured with rocprofv3 --pmc (FETCH_SIZE and WRITE_SIZE, scaled
-# by a copy kernel of known size); the summary file holds the command and the counters.
-TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "pmc_traffic_k_iterk_32x32x4_tgv_848x800_L0.md")
+# HBM bytes per launch of the level-0 inner-iteration kernel measured with rocprofv3
+# --pmc (FETCH_SIZE and WRITE_SIZE, scaled by a copy kernel of known size), one summary
+# per launch choice (tile shape, iterations per launch); each holds its command and counters.
+TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "pmc_traffic_k_iterk_{tw}x{th}x{k}_{workload}_L0.md")
 
 
-def profiled_traffic():
-    """`traffic_bytes_per_launch = N` from the committed PMC summary, or None."""
+def profiled_traffic(tw, th, k, workload):
+    """`traffic_bytes_per_launch = N` from the committed PMC summary of this launch choice,
+    or None when there is none."""
     import re
     try:
-        with open(TRAFFIC_PROFILE) as f:
+        with open(TRAFFIC_PROFILE.format(tw=tw, th=th, k=k, workload=workload)) as f:
             m = re.search(r"traffic_bytes_per_launch\s*=\s*([0-9]+)", f.read())
         return int(m.group(1)) if m else None
     except OSError:
@@ -177,7 +178,7 @@ def main():
                        "p99": round(float(np.percentile(lat, 99)), 4),
                        "mean": round(float(lat.mean()), 4)},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": profiled_traffic(),
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": profiled_traffic(tw0, th0, per_launch, args.workload),
                      "kernel": f"inner TGV iterations at level 0 ({w0}x{h0}): {per_launch} per launch, "
                                f"{tw0}x{th0} tiles",
                      "us_per_launch": round(t_launch * 1e6, 3), "us_per_iter": round(t_iter * 1e6, 3),
