@@ -1,0 +1,23 @@
+"""The dispatch model of vfs_amd.profile_ops (used to attribute rocprofv3 kernel traces to
+the ops of a solve) against the library's greedy split of a warp's inner iterations."""
+from vfs_amd.profile_ops import inner_ops, program_ops
+
+
+def test_inner_ops_greedy_split():
+    assert inner_ops(50, 1) == ["ITER"] * 50
+    assert inner_ops(50, 2) == ["ITER2"] * 25
+    assert inner_ops(50, 4) == ["ITER4"] * 12 + ["ITER2"]
+    assert inner_ops(50, 8) == ["ITER8"] * 6 + ["ITER2"]
+    assert inner_ops(15, 8) == ["ITER8", "ITER4", "ITER2", "ITER"]
+    assert inner_ops(11, 4) == ["ITER4", "ITER4", "ITER2", "ITER"]
+    assert inner_ops(1, 8) == ["ITER"]
+
+
+def test_program_ops_count_matches_traced_solve():
+    # a traced 848x800 solve (L5 W10 I50; four iterations per launch at levels 0-2,
+    # eight at levels 3-4) issued 651 dispatches: 530 multi-iteration, 121 one-op
+    ops = program_ops(5, 10, 50, (4, 4, 4, 8, 8))
+    assert len(ops) == 651
+    multi = [o for o in ops if o[0] in ("ITER2", "ITER4", "ITER8")]
+    assert len(multi) == 530
+    assert len(program_ops(5, 10, 50, (4, 4, 4, 4, 4))) == 771
