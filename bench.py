@@ -185,6 +185,16 @@ def main():
                      "algorithmic_bytes_per_launch": int(bytes_launch),
                      "copy_kernel_GBs": round(copy_gbs, 1)},
     }
+    # the whole solve against the same roofline: algorithmic bytes of one pair (valid
+    # pixels, one pass per kernel, no credit for reuse), B_pair = sum_l N_l (W (I B_iter
+    # + B_warp) + B_level) + 16 N_0 + 40 (sum_l N_l - N_0), B_warp = 104, B_level = 44
+    n_l = [g.level_dims(l)[0] * g.level_dims(l)[1] for l in range(wl["levels"])]
+    b_pair = (sum(n_l) * (wl["warps"] * (wl["iters"] * B_ITER_FP32 + 104.0) + 44.0) + 16.0 * n_l[0]
+              + 40.0 * (sum(n_l) - n_l[0]))
+    out["solve_roofline"] = {"algorithmic_bytes_per_pair": int(b_pair),
+                             "achieved": round(b_pair * pairs_per_s / 1e9, 1), "unit": "GB/s",
+                             "frac": round(b_pair * pairs_per_s / 1e9 / HBM_PEAK_GBS, 4),
+                             "ceiling_pairs_per_s": round(HBM_PEAK_GBS * 1e9 / b_pair, 1)}
 
     # per-level launch choices of the tile-shape tuner (made at the first solve)
     out["tile_tuning"] = {}
