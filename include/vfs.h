@@ -147,7 +147,8 @@ int vfs_synchronize(vfs_solver *s);
  *   "per_launch" -1 = as "autotune" says (default), 1, 2, 4 or 8 = that many inner
  *             iterations per launch wherever allowed (the workgroup recomputes a halo
  *             of 3 pixels per iteration; levels with a mask pixel in column 0 run one;
- *             eight only in the tiles 32x8, 16x16, 20x20, 8x8 and 4x4, else four);
+ *             eight only in the tiles 32x8, 16x16, 20x20, 14x13, 8x8, 7x7 and 4x4, else
+ *             four);
  *   "pair"    the same as "per_launch" -1 / 1 / 2 written as -1 / 0 / 1.
  * Results do not depend on "graph", "autotune", "shape", "per_launch" or "pair".
  * No counterpart in the reference (FP32 only, stereotgv.h:47-100; its block shape is a

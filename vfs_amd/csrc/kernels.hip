@@ -1707,6 +1707,9 @@ void iterk_storage(const PairDesc *desc, const TickArgs &args, int launch_shape,
     case 9: iterk_shape<T, 9, K>(desc, args, half, st); break;
     case 10: iterk_shape<T, 10, K>(desc, args, half, st); break;
     case 11: iterk_shape<T, 11, K>(desc, args, half, st); break;
+    case 12: iterk_shape<T, 12, K>(desc, args, half, st); break;
+    case 13: iterk_shape<T, 13, K>(desc, args, half, st); break;
+    case 14: iterk_shape<T, 14, K>(desc, args, half, st); break;
     default: break;  // the host never asks for another shape
     }
 }

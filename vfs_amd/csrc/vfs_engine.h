@@ -24,14 +24,16 @@ constexpr int kMaxSched = 128;
 // or 2) of the tw x th tile.  Shape 0 is the default; the others are candidates of the
 // per-level tile-shape tuner.  The multi-iteration ops (OP_ITER2, OP_ITER4, OP_ITER8) take the
 // shapes of multi_iter_shape(), including the small tiles 8x8 and 4x4 for the coarsest,
-// latency-bound levels and 32x32, 20x20 and 36x25 for the largest (36x25 splits an
-// 848x800 image into 768 tiles: three workgroups for each of 256 compute units).
-constexpr int kShapes = 12, kTickShapes = 7;
+// latency-bound levels, 32x32, 20x20, 36x25 and 27x25 for the largest, and 14x13 and 7x7
+// between them (36x25 splits an 848x800 image into 768 tiles, three workgroups for each
+// of 256 compute units; 27x25, 14x13 and 7x7 split 424x400, 212x200 and 106x100 into
+// 256, 256 and 240: one each).
+constexpr int kShapes = 15, kTickShapes = 7;
 struct ShapeDim {
     int tw, th;
 };
 constexpr ShapeDim kShape[kShapes] = {{64, 8}, {64, 4}, {128, 4}, {32, 16}, {32, 8}, {16, 32}, {16, 16},
-                                      {8, 8}, {4, 4}, {32, 32}, {20, 20}, {36, 25}};
+                                      {8, 8}, {4, 4}, {32, 32}, {20, 20}, {36, 25}, {27, 25}, {14, 13}, {7, 7}};
 __host__ __device__ constexpr bool multi_iter_shape(int shape)
 {
     return shape == 1 || shape == 3 || (shape >= 4 && shape != 5);
