@@ -54,6 +54,7 @@ __host__ __device__ constexpr bool half_workgroup_ok(int shape, int K)
     return iterk_region(shape, K) > 512 && iterk_region(shape, K) <= 1024;
 }
 __host__ __device__ constexpr int shape_index(int launch_shape) { return launch_shape & (kHalfWorkgroup - 1); }
+static_assert(kShapes <= kHalfWorkgroup, "shape indices fit below the half-workgroup flag");
 // Tile shapes of K-iteration launches: every multi-iteration shape for K = 2 and 4; for
 // K = 8 those whose staged region holds at most 2048 positions.
 __host__ __device__ constexpr bool iterk_ok(int shape, int K)
