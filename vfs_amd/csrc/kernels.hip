@@ -1027,14 +1027,15 @@ __device__ __forceinline__ void op_iterk(const Tile &t, int roles, const float4 
     auto act = [&](int k) { return !MASKED || L.mk[k] != 0; };
     auto in_tile = [&](int x, int y) { return x >= x0 && x < x0 + B::TW && y >= y0 && y < y0 + B::TH; };
     // Per own position, once per launch (the mask does not change): in the image, in the
-    // mask, and which of the four neighbours are in the image and the mask.
+    // mask, and which of the four neighbours are in the image and the mask (read by the
+    // path with boundary tests only).
     enum : unsigned { F_IN = 1, F_ACT = 2, F_R = 4, F_D = 8, F_L = 16, F_U = 32 };
     unsigned fl[NK];
 #pragma unroll
     for (int k = 0; k < NK; k++) {
         const int i = tid + k * NT, x = ox[k], y = oy[k];
         fl[k] = 0;
-        if (i < B::NR && x >= 0 && y >= 0 && x < w && y < h) {
+        if (!full && i < B::NR && x >= 0 && y >= 0 && x < w && y < h) {
             // neighbours outside the staged region are never asked for
             const int col = i % B::RW, row = i / B::RW;
             fl[k] = F_IN | (act(i) ? F_ACT : 0u);
