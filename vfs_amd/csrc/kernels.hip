@@ -1715,22 +1715,31 @@ void iterk_storage(const PairDesc *desc, const TickArgs &args, int launch_shape,
     }
 }
 
-void launch_tick(const PairDesc *desc, const TickArgs &args, int storage, int shape, hipStream_t st)
+hipError_t launch_tick(const PairDesc *desc, const TickArgs &args, int storage, int shape, hipStream_t st)
 {
-    if (args.n < 1 || args.total < 1) return;
+    if (args.n < 1 || args.total < 1) return hipSuccess;
+    if (op_multi(args.e[0].kind)) {
+        const int K = op_iterations(args.e[0].kind), sh = shape_index(shape);
+        if (shape < 0 || sh >= kShapes || !iterk_ok(sh, K) ||
+            ((shape & kHalfWorkgroup) && !half_workgroup_ok(sh, K)))
+            return hipErrorInvalidValue;
+    } else if (shape < 0 || shape >= kTickShapes) {
+        return hipErrorInvalidValue;
+    }
     if (args.e[0].kind == OP_ITER2) {
         STORAGE_DISPATCH(storage, (iterk_storage<T, 2>(desc, args, shape, st)));
-        return;
+        return hipSuccess;
     }
     if (args.e[0].kind == OP_ITER4) {
         STORAGE_DISPATCH(storage, (iterk_storage<T, 4>(desc, args, shape, st)));
-        return;
+        return hipSuccess;
     }
     if (args.e[0].kind == OP_ITER8) {
         STORAGE_DISPATCH(storage, (iterk_storage<T, 8>(desc, args, shape, st)));
-        return;
+        return hipSuccess;
     }
     STORAGE_DISPATCH(storage, tick_storage<T>(desc, args, shape, st));
+    return hipSuccess;
 }
 
 void launch_fill_zero(void *ptr, size_t bytes, hipStream_t st)

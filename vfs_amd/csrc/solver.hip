@@ -470,8 +470,8 @@ void set_entry(vfs::TickEntry &e, const ProgOp &o, int pair, const vfs::PairDesc
 }
 
 // Issues every op of `pr` for every pair of the launch set: one k_tick launch per op per
-// group of up to kMaxEntries pairs.
-void issue_program(const Engine &e, const Params &P, const Program &pr)
+// group of up to kMaxEntries pairs.  Stops at the first launch that has no kernel.
+hipError_t issue_program(const Engine &e, const Params &P, const Program &pr)
 {
     vfs::TickArgs a = tick_header(P);
     const int n = (int)e.h_desc.size();
@@ -486,9 +486,11 @@ void issue_program(const Engine &e, const Params &P, const Program &pr)
             }
             a.n = cnt;
             a.total = acc;
-            vfs::launch_tick(e.d_desc, a, P.storage, o.shape, e.stream);
+            const hipError_t r = vfs::launch_tick(e.d_desc, a, P.storage, o.shape, e.stream);
+            if (r != hipSuccess) return r;
         }
     }
+    return hipSuccess;
 }
 
 // Runs `pr` on the engine's stream: replays (capturing first if needed) the graph cached
@@ -502,17 +504,21 @@ int engine_run(Engine &e, const Params &P, const Program &pr, const GraphKey &ke
         return VFS_ERR_DEVICE;
     };
     if (!use_graph) {
-        issue_program(e, P, pr);
-        hipError_t r = hipGetLastError();
+        hipError_t r = issue_program(e, P, pr);
+        if (r == hipSuccess) r = hipGetLastError();
         return r == hipSuccess ? VFS_OK : ferr("kernel launch", r);
     }
     auto it = e.graphs.find(key);
     if (it == e.graphs.end()) {
         hipError_t r = hipStreamBeginCapture(e.stream, hipStreamCaptureModeThreadLocal);
         if (r != hipSuccess) return ferr("hipStreamBeginCapture", r);
-        issue_program(e, P, pr);
+        const hipError_t ri = issue_program(e, P, pr);
         hipGraph_t g = nullptr;
         r = hipStreamEndCapture(e.stream, &g);
+        if (ri != hipSuccess) {
+            if (g) (void)hipGraphDestroy(g);
+            return ferr("kernel launch", ri);
+        }
         if (r != hipSuccess) return ferr("hipStreamEndCapture", r);
         GraphEntry ge;
         r = hipGraphInstantiate(&ge.exec, g, nullptr, nullptr, 0);
@@ -685,9 +691,10 @@ hipError_t time_launch_choices(hipStream_t st, const vfs::PairDesc *d_desc, cons
                 }
                 a.n = n;
                 a.total = acc;
-                for (int k = 0; k < kWarm; k++) vfs::launch_tick(d_desc, a, P.storage, o.shape, st);
-                r = hipEventRecord(e0, st);
-                for (int k = 0; k < kReps; k++) vfs::launch_tick(d_desc, a, P.storage, o.shape, st);
+                auto launch = [&]() { return vfs::launch_tick(d_desc, a, P.storage, o.shape, st); };
+                for (int k = 0; k < kWarm && r == hipSuccess; k++) r = launch();
+                if (r == hipSuccess) r = hipEventRecord(e0, st);
+                for (int k = 0; k < kReps && r == hipSuccess; k++) r = launch();
                 if (r == hipSuccess) r = hipEventRecord(e1, st);
                 if (r == hipSuccess) r = hipEventSynchronize(e1);
                 float ms = 0;
@@ -1422,7 +1429,12 @@ int vfs_time_inner_iterations(vfs_solver *s, int level, int reps, float *ms_tota
     }
     hipEvent_t e0, e1;
     if ((rc = time_events(s, s->stream, &e0, &e1))) return rc;
-    issue_program(s->eng, P, pr);
+    const hipError_t ri = issue_program(s->eng, P, pr);
+    if (ri != hipSuccess) {
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        return fail(s, VFS_ERR_DEVICE, "kernel launch: %s", hipGetErrorString(ri));
+    }
     HIPCHK(s, hipEventRecord(e1, s->stream));
     HIPCHK(s, hipEventSynchronize(e1));
     float ms = 0;
@@ -2015,7 +2027,7 @@ int launch_group(vfs_server *v, const std::vector<SrvRequest *> &grp)
     }
     a.n = k;
     a.total = acc;
-    vfs::launch_tick(v->d_desc, a, v->P.storage, shape, v->stream);
+    HIPCHKS(v, vfs::launch_tick(v->d_desc, a, v->P.storage, shape, v->stream));
     HIPCHKS(v, hipGetLastError());
     v->ticks++;
     v->launched_entries += k;

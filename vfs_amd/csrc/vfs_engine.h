@@ -229,7 +229,9 @@ inline int op_tiles(int kind, int level, const int *w, const int *h, const int *
     return ((ww + d.tw - 1) / d.tw) * ((h[tl] + d.th - 1) / d.th);
 }
 
-void launch_tick(const PairDesc *desc, const TickArgs &args, int storage, int shape, hipStream_t st);
+// Launches one op for the entries of `args` (all of one kind) in tile shape `shape`;
+// hipErrorInvalidValue (nothing launched) when no kernel exists for that kind and shape.
+hipError_t launch_tick(const PairDesc *desc, const TickArgs &args, int storage, int shape, hipStream_t st);
 
 // ---- single-plane kernels (load time, outputs) ----
 struct Dims {
