@@ -126,7 +126,9 @@ int vfs_get_level_dims(const vfs_solver *s, int level, int *w, int *h, int *stri
 /* Inspection of an internal plane at `level` dims into a compact host array of
  * w*h*nch floats (the reference exposes these as public device pointers,
  * stereotgv.h:47-100).  Names: "u", "u_", "u_last", "a", "b", "c", "Iu", "Iz",
- * "i1warp", "I0", "I1", "mask", "warpUV", "tv2", "tv", "p", "q", "v_", "Tp".
+ * "i1warp", "I0", "I1", "mask", "warpUV", "tv2", "tv", "p", "q", "v_", "Tp"; and
+ * "u.other", "u_.other", "v_.other": the ping-pong partner of u, u_, v_ (the
+ * reference's second buffer of each pair, whose stale values its mask-edge reads see).
  * Returns the channel count (> 0) or a negative status. */
 int vfs_read_plane(vfs_solver *s, const char *name, int level, float *out);
 

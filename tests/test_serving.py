@@ -118,6 +118,35 @@ def test_warm_solves_do_not_depend_on_iterations_per_launch():
         assert np.array_equal(out[1][k], out[8][k]), k
 
 
+@pytest.mark.gpu
+@pytest.mark.parametrize("iters", [8, 9])
+def test_warm_solves_leave_the_same_buffers_whatever_the_iterations_per_launch(iters):
+    """After every cold and warm solve, both buffers of each ping-pong pair (u, u_, v_: the
+    current one and its partner, whose stale values the next solve's mask-edge reads see)
+    hold the same values at every level-0 pixel whether the inner iterations ran one,
+    four or eight per launch; also with an odd iteration count, which ends each warp with
+    a one-iteration launch."""
+    base, fr = frames(200, 120, 17, 5)
+    names = ("u", "u_", "v_", "u.other", "u_.other", "v_.other")
+    out = {}
+    for per_launch in (1, 4, 8):
+        g = gpu_solver(base, 3, 3, iters + 1)
+        g.set_option("per_launch", per_launch)
+        res = []
+        for k, f in enumerate(fr):
+            g.copyImagesToDevice(f["i0"], f["i1"])
+            if k == 0:
+                g.solveStereoForwardMasked()
+            else:
+                g.solve_warm(1 + k % 2, 2, iters)
+            res.append({n: g.read_plane(n, 0) for n in names})
+        out[per_launch] = res
+    for k in range(len(fr)):
+        for n in names:
+            for pl in (4, 8):
+                assert np.array_equal(out[1][k][n], out[pl][k][n], equal_nan=True), (k, n, pl)
+
+
 def make_server(budget=16 << 30, max_active=64):
     from vfs_amd import Server
     return Server(P, hbm_budget_bytes=budget, max_streams=16, max_active=max_active)

@@ -226,6 +226,33 @@ __device__ __forceinline__ void op_zero(const Tile &t)
     }
 }
 
+// Before the multi-iteration launches of a warp at a coarse level of a warm solve: the
+// spare buffers of u, u_ and v_ take the current buffers' values beyond the level's flat
+// extent (stride x height), where level-0 data of the previous solve lies and every
+// buffer differs.  A multi-iteration launch makes the spare buffers current there
+// without writing them; with this copy the current buffers keep their values, as the
+// reference's buffers do beyond the level's extent.  The tile covers the level-0 planes.
+template <class S>
+__device__ __forceinline__ void op_sync(const Tile &t, int level, int roles)
+{
+    const long ext = (long)t.d->s[level] * t.d->h[level];
+    const float *u = slot(t, slot_u(rcur(roles, G_U)));
+    float *us = slot(t, slot_u(rspare(roles, G_U)));
+    const float *u_ = slot(t, slot_ub(rcur(roles, G_UB)));
+    float *u_s = slot(t, slot_ub(rspare(roles, G_UB)));
+    const float2 *v = slotp<float2>(t, slot_v(rcur(roles, G_V)));
+    float2 *vs = slotp<float2>(t, slot_v(rspare(roles, G_V)));
+    FOR_TILE_PIXELS(S, t, x, y)
+    {
+        const long pos = (long)y * t.s + x;
+        if (pos >= ext) {
+            us[pos] = u[pos];
+            u_s[pos] = u_[pos];
+            vs[pos] = v[pos];
+        }
+    }
+}
+
 // Calibration warp of I1 (stereotgv.cu:392-393, warpImage.cu:9-23) into the level-0
 // pyramid slot; with 16-bit storage also narrows I0 into its pyramid slot.
 template <class S, typename T>
@@ -1447,6 +1474,9 @@ __global__ __launch_bounds__(NT) void k_tick(const PairDesc *__restrict__ D, con
         break;
     case OP_UP:
         op_up<S>(t, level, E.roles, A.fscale);
+        break;
+    case OP_SYNC:
+        op_sync<S>(t, level, E.roles);
         break;
     default:
         break;

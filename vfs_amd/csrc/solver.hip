@@ -326,8 +326,10 @@ void push_op(Program &pr, int kind, int level, int masked, int roles)
 }
 
 // One level of the solve from the tensor to the end of the last warp
-// (stereotgv.cu:409-494, masked; :275-364 unmasked).
-void push_level(Program &pr, const Params &P, int masked, int l, int warps, int iters, int &roles, int per_launch)
+// (stereotgv.cu:409-494, masked; :275-364 unmasked).  With `sync_spare`, each warp's
+// first multi-iteration op is preceded by OP_SYNC (warm solves above level 0).
+void push_level(Program &pr, const Params &P, int masked, int l, int warps, int iters, int &roles, int per_launch,
+                bool sync_spare = false)
 {
     std::vector<float4> sched((size_t)std::max(iters, 1));
     iter_schedule(P.timestep_lambda, iters, sched.data());
@@ -347,6 +349,7 @@ void push_level(Program &pr, const Params &P, int masked, int l, int warps, int 
             for (int c : {8, 4, 2})
                 if (k == 1 && c <= per_launch && it + c <= iters && it + c <= vfs::kMaxSched) k = c;
             if (k > 1) {
+                if (sync_spare && it == 0) push_op(pr, vfs::OP_SYNC, l, masked, roles);
                 o.kind = iter_kind(k);
                 o.it0 = it;
                 o.flags = (unsigned char)((it == 0 ? vfs::EF_FIRST : 0) | (it + k == iters ? vfs::EF_WRITE_TP_LAST : 0));
@@ -415,11 +418,11 @@ Program build_program(const Params &P, int masked, int warm_level, int warps, in
             roles = vfs::rswap(vfs::rswap(roles, vfs::G_U), vfs::G_UB) ^ vfs::RB_UV;
         }
         for (int l = warm_level; l >= 0; l--) {
-            // Above level 0 a warm program runs one iteration per launch: a multi-iteration
-            // launch leaves the spare buffer (holding what the previous solve left there)
-            // current beyond the level's extent, which coarse levels of a cold solve never
-            // reach but a warm start's finer levels read.
-            push_level(pr, P, masked, l, warps, iters, roles, l > 0 ? 1 : tn.lv[l].per_launch);
+            // Above level 0 the spare buffers first take the current values beyond the
+            // level's extent (OP_SYNC): a multi-iteration launch makes the spare buffers
+            // current there, where a warm solve's buffers still hold the previous solve's
+            // level-0 data (a cold solve's are all zero).
+            push_level(pr, P, masked, l, warps, iters, roles, tn.lv[l].per_launch, l > 0);
             if (l > 0) {
                 push_op(pr, vfs::OP_UP, l, masked, roles);
                 roles = vfs::rswap(vfs::rswap(roles, vfs::G_U), vfs::G_UB) ^ vfs::RB_UV;
@@ -453,7 +456,7 @@ void set_entry(vfs::TickEntry &e, const ProgOp &o, int pair, const vfs::PairDesc
     e.arena = d.arena;
     e.mask = d.mask[tl];
     e.plane = (int)d.plane;  // < 2^31 (checked at initialize)
-    e.w = (unsigned short)(o.kind == vfs::OP_ZERO ? d.s[0] : d.w[tl]);
+    e.w = (unsigned short)(vfs::op_flat(o.kind) ? d.s[0] : d.w[tl]);
     e.h = (unsigned short)d.h[tl];
     e.s = (unsigned short)d.s[tl];
     e.pair = (unsigned short)pair;
@@ -1252,6 +1255,9 @@ int vfs_read_plane(vfs_solver *s, const char *name, int level, float *out)
     else if (n == "p") { src = slot_ptr(s, slot_p(rcur(r, G_PQ))); nch = 2; typed = true; }
     else if (n == "v_") { src = slot_ptr(s, slot_v(rcur(r, G_V))); nch = 2; }
     else if (n == "Tp") { src = slot_ptr(s, PL_TP); nch = 2; }
+    else if (n == "u.other") src = slot_ptr(s, slot_u(roth(r, G_U)));
+    else if (n == "u_.other") src = slot_ptr(s, slot_ub(roth(r, G_UB)));
+    else if (n == "v_.other") { src = slot_ptr(s, slot_v(roth(r, G_V))); nch = 2; }
     else if (n == "q") { src = slot_ptr(s, slot_q(rcur(r, G_PQ))); nch = 4; typed = true; }
     if (!src) return fail(s, VFS_ERR_ARG, "unknown plane '%s'", name);
     int rc = set_device(s);

@@ -171,7 +171,8 @@ enum OpKind : int {
     OP_ITER2 = 11,  // two fused TGV iterations (recomputed halo)
     OP_ITER4 = 12,  // four fused TGV iterations (recomputed halo)
     OP_ENDPREP = 13, // END of a warp and PREP of the next one at the same level
-    OP_ITER8 = 14   // eight fused TGV iterations (recomputed halo)
+    OP_ITER8 = 14,  // eight fused TGV iterations (recomputed halo)
+    OP_SYNC = 15    // spare buffers of u, u_, v_ take the current ones beyond the level's extent
 };
 
 // Iterations of an inner-iteration op.
@@ -218,13 +219,15 @@ struct TickArgs {
 };
 
 // Level whose pixels an op's tiles cover (OP_UP writes the finer level).
-inline int op_tile_level(int kind, int level) { return kind == OP_UP ? level - 1 : level; }
+inline int op_tile_level(int kind, int level) { return kind == OP_UP ? level - 1 : kind == OP_SYNC ? 0 : level; }
+// Ops whose tiles cover the level-0 planes' whole flat extent, row padding included.
+inline bool op_flat(int kind) { return kind == OP_ZERO || kind == OP_SYNC; }
 
 // Workgroups an op of `kind` at `level` needs for a pair with dims w/h/s in tile `shape`.
 inline int op_tiles(int kind, int level, const int *w, const int *h, const int *s, int shape)
 {
     const int tl = op_tile_level(kind, level);
-    const int ww = kind == OP_ZERO ? s[0] : w[tl];
+    const int ww = op_flat(kind) ? s[0] : w[tl];
     const ShapeDim d = kShape[shape_index(shape)];
     return ((ww + d.tw - 1) / d.tw) * ((h[tl] + d.th - 1) / d.th);
 }
