@@ -68,3 +68,44 @@ def test_repeated_create_use_close_releases_device_memory():
         _cycle(pairs)
     lost = base - _free_bytes()
     assert lost < (32 << 20), f"{lost / 2**20:.1f} MiB not released after 4 cycles"
+
+
+def test_misuse_is_reported_and_leaves_the_solver_usable():
+    """Calls out of order or with bad arguments fail with an error and a message (the
+    reference exits the process instead); the same solver then solves normally and
+    matches a fresh one bit for bit."""
+    from vfs_amd import VfsError
+    pair = S.make_pair(160, 120, 63, radius=60, dmax=160 / 21.2)
+    g = StereoTgv()
+    with pytest.raises(VfsError, match="before initialize"):
+        g.copyMaskToDevice(pair["mask"])
+    H, W = pair["i0"].shape
+    g.initialize(W, H, P["beta"], P["gamma"], P["alpha0"], P["alpha1"], P["timestep_lambda"], P["lam"], 3,
+                 P["fScale"], 2, 8)
+    g.limitRange = P["limitRange"]
+    with pytest.raises(VfsError, match="loadVectorFields"):
+        g.solveStereoForwardMasked()
+    g.loadVectorFields(pair["tv"], pair["cv"])
+    with pytest.raises(VfsError, match="copyMaskToDevice"):
+        g.solveStereoForwardMasked()
+    g.copyMaskToDevice(pair["mask"])
+    with pytest.raises(VfsError, match="copyImagesToDevice"):
+        g.solveStereoForwardMasked()
+    g.copyImagesToDevice(pair["i0"], pair["i1"])
+    with pytest.raises(VfsError, match="warm"):
+        g.solve_warm(1, 2, 4)
+    with pytest.raises(VfsError):
+        g.read_plane("no such plane", 0)
+    with pytest.raises(VfsError):
+        g.read_plane("u", 3)
+    with pytest.raises(VfsError):
+        g.set_option("no such option", 1)
+    with pytest.raises(VfsError):
+        g.set_option("per_launch", 3)
+    g.solveStereoForwardMasked()
+    with pytest.raises(VfsError):
+        g.solve_warm(3, 2, 4)  # start level >= nLevels
+    u = g.copy1DDisparityToHost()
+    fresh = _solver(pair)
+    fresh.solveStereoForwardMasked()
+    assert np.array_equal(u, fresh.copy1DDisparityToHost())

@@ -326,9 +326,10 @@ class StereoTgv:
         self._check(self._L.vfs_time_copy(self._h, nbytes, reps, ctypes.byref(ms)), "time_copy")
         return ms.value
 
-    @staticmethod
-    def _shape(a: np.ndarray, shape):
-        if a.shape != tuple(shape):
+    def _shape(self, a: np.ndarray, shape):
+        # before initialize the library itself reports the call as out of order (and
+        # reads nothing)
+        if self.width and a.shape != tuple(shape):
             raise VfsError(f"expected array of shape {tuple(shape)}, got {a.shape}")
 
 
