@@ -354,6 +354,34 @@ def _workload_parity(name, tol_mean):
 
 
 @pytest.mark.parametrize("masked", [True, False])
+def test_tensor_column_limit_wide_image(masked):
+    """A 160x64 image: the reference's tensor grid covers columns x < 32 ceil(h / 32) = 64
+    only (calcTensor.cu launch), so columns 64..159 keep a = b = c = 0 from the memset,
+    inside the mask too, and their threshold takes the eta_u = 0 branch (tau itself).
+    The tensor planes match the oracle exactly there, one and two iterations match plane
+    by plane, and a full solve meets the small-solve bar."""
+    pair = S.make_pair(160, 64, 71, radius=32, dmax=160 / 21.2)
+    for iters in (1, 2):
+        g = run_gpu(pair, 1, 1, iters, masked=masked)
+        o = run_oracle(pair, 1, 1, iters, masked=masked)
+        for name in ("a", "b", "c"):
+            got, ref = g.read_plane(name, 0), o.read_plane(name, 0)
+            assert not got[:, 64:].any() and not ref[:, 64:].any(), name
+            assert_close(name, got, ref, 1e-4, 1e-5)
+        m = pair["mask"] > 0 if masked else np.ones(pair["mask"].shape, bool)
+        for name in ("p", "q"):
+            assert_close(name, g.read_plane(name, 0)[m], o.read_plane(name, 0)[m], 1e-4, 1e-5)
+        for name in ("u", "u_", "v_"):
+            got, ref = g.read_plane(name, 0), o.read_plane(name, 0)
+            assert_close(name, got, ref, 1e-3, 2e-4)
+    g = run_gpu(pair, 3, 3, 10, masked=masked)
+    o = run_oracle(pair, 3, 3, 10, masked=masked)
+    m = pair["mask"] > 0
+    du = np.abs(g.copy1DDisparityToHost() - o.copy1DDisparityToHost())[m]
+    assert du.mean() <= 5e-4, du.mean()
+
+
+@pytest.mark.parametrize("masked", [True, False])
 def test_parity_non_power_of_two_pyramid(masked):
     """fScale 1.2 (test_vfs/main.cpp:136-137 uses 11 levels at 1.2): non-exact down- and
     upscaling at every level, and the upscale multiplies by fScale rather than the true
