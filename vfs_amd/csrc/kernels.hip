@@ -1075,7 +1075,9 @@ __device__ __forceinline__ void op_iterk(const Tile &t, int roles, const float4 
 
     auto iterate = [&](auto full_c) {
         constexpr bool FULL = decltype(full_c)::value;
-#pragma unroll 1
+        // unrolled: each iteration's regions and stores are known at compile time (fewer
+        // registers and tests than a loop over j)
+#pragma unroll
         for (int j = 0; j < K; j++) {
             IterParams ip = ip0;
             const float4 sc = sched[j];
