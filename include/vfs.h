@@ -160,7 +160,7 @@ int vfs_get_option(const vfs_solver *s, const char *key, int *value);
 
 /* The launch choices used at `level` by the masked (masked != 0) or unmasked solve:
  * tile shape index of the inner iterations (0..6 as option "shape"; with several
- * iterations per launch also 7..14 = 8x8, 4x4, 32x32, 20x20, 36x25, 27x25, 14x13, 7x7,
+ * iterations per launch also 7..15 = 8x8, 4x4, 32x32, 20x20, 36x25, 27x25, 14x13, 7x7, 43x34,
  * and 1, 3, 4, 6 of the former), whether the inner iteration prefetches its Tp-plane
  * neighbours, and the inner iterations per launch, 1, 2, 4 or 8 (`per_launch` may be
  * NULL).  Runs the tuner first when it has not run for this geometry.
@@ -168,8 +168,8 @@ int vfs_get_option(const vfs_solver *s, const char *key, int *value);
 int vfs_get_level_tuning(vfs_solver *s, int masked, int level, int *shape, int *preload, int *per_launch);
 /* Sets the launch choices of one level (reproducible profiling, expert use); they hold
  * until the next initialize or option change.  `per_launch` (1, 2, 4 or 8) may carry
- * VFS_HALF_WORKGROUP: multi-iteration workgroups of 512 work-items, two positions each,
- * where the tile's staged region holds 513 .. 1024 positions.
+ * VFS_HALF_WORKGROUP: multi-iteration workgroups of 512 work-items, two or three
+ * positions each, where the tile's staged region holds 513 .. 1536 positions.
  * No counterpart in the reference. */
 #define VFS_HALF_WORKGROUP 0x100
 int vfs_set_level_tuning(vfs_solver *s, int masked, int level, int shape, int preload, int per_launch);

@@ -1202,7 +1202,7 @@ __device__ __forceinline__ void op_iterk(const Tile &t, int roles, const float4 
     else iterate(std::false_type{});
 }
 
-// Half workgroups (512 work-items, two positions each) are held to 128 registers, so that
+// Half workgroups (512 work-items, two or three positions each) are held to 128 registers, so that
 // two of them share a compute unit.
 template <typename T, int SH, int K, bool HALF>
 __global__ __launch_bounds__((BlkK<SH, K, HALF>::NT)) __attribute__((amdgpu_waves_per_eu(HALF ? 4 : 1)))
@@ -1743,6 +1743,7 @@ void iterk_storage(const PairDesc *desc, const TickArgs &args, int launch_shape,
     case 12: iterk_shape<T, 12, K>(desc, args, half, st); break;
     case 13: iterk_shape<T, 13, K>(desc, args, half, st); break;
     case 14: iterk_shape<T, 14, K>(desc, args, half, st); break;
+    case 15: iterk_shape<T, 15, K>(desc, args, half, st); break;
     default: break;  // the host never asks for another shape
     }
 }

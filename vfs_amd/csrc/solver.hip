@@ -1356,7 +1356,7 @@ int vfs_set_level_tuning(vfs_solver *s, int masked, int level, int shape, int pr
     if (!valid_per_launch(per_launch) || (half && (per_launch == 1 || !multi_shape(shape) ||
                                                    !vfs::half_workgroup_ok(shape, per_launch))))
         return fail(s, VFS_ERR_ARG, "per_launch must be 1, 2, 4 or 8 (half workgroups where the staged region "
-                                    "holds 513 .. 1024 positions)");
+                                    "holds 513 .. 1536 positions)");
     if (per_launch > 1 ? !(multi_shape(shape) && vfs::iterk_ok(shape, per_launch))
                        : (shape < 0 || shape >= vfs::kTickShapes))
         return fail(s, VFS_ERR_ARG, "bad shape %d for %d iteration(s) per launch", shape, per_launch);

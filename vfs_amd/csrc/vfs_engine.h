@@ -24,16 +24,16 @@ constexpr int kMaxSched = 128;
 // or 2) of the tw x th tile.  Shape 0 is the default; the others are candidates of the
 // per-level tile-shape tuner.  The multi-iteration ops (OP_ITER2, OP_ITER4, OP_ITER8) take the
 // shapes of multi_iter_shape(), including the small tiles 8x8 and 4x4 for the coarsest,
-// latency-bound levels, 32x32, 20x20, 36x25 and 27x25 for the largest, and 14x13 and 7x7
-// between them (36x25 splits an 848x800 image into 768 tiles, three workgroups for each
-// of 256 compute units; 27x25, 14x13 and 7x7 split 424x400, 212x200 and 106x100 into
-// 256, 256 and 240: one each).
-constexpr int kShapes = 15, kTickShapes = 7;
+// latency-bound levels, 32x32, 20x20, 36x25, 43x34 and 27x25 for the largest, and 14x13
+// and 7x7 between them (43x34 and 36x25 split an 848x800 image into 480 and 768 tiles, two
+// and three workgroups for each of 256 compute units; 27x25, 14x13 and 7x7 split
+// 424x400, 212x200 and 106x100 into 256, 256 and 240: one each).
+constexpr int kShapes = 16, kTickShapes = 7;
 struct ShapeDim {
     int tw, th;
 };
 constexpr ShapeDim kShape[kShapes] = {{64, 8}, {64, 4}, {128, 4}, {32, 16}, {32, 8}, {16, 32}, {16, 16},
-                                      {8, 8}, {4, 4}, {32, 32}, {20, 20}, {36, 25}, {27, 25}, {14, 13}, {7, 7}};
+                                      {8, 8}, {4, 4}, {32, 32}, {20, 20}, {36, 25}, {27, 25}, {14, 13}, {7, 7}, {43, 34}};
 __host__ __device__ constexpr bool multi_iter_shape(int shape)
 {
     return shape == 1 || shape == 3 || (shape >= 4 && shape != 5);
@@ -45,13 +45,13 @@ __host__ __device__ constexpr int iterk_region(int shape, int K)
     return (kShape[shape].tw + 3 * K) * (kShape[shape].th + 3 * K);
 }
 // A K-iteration launch takes one work-item per staged position up to 512 positions and
-// 1024 above; regions of 513 .. 1024 positions may instead take 512 work-items with two
-// positions each (more workgroups per compute unit).  Launch shapes carry that choice as
-// kHalfWorkgroup added to the tile shape index.
+// 1024 above (two or three positions each); regions of 513 .. 1536 positions may instead
+// take 512 work-items with two or three positions each (more workgroups per compute
+// unit).  Launch shapes carry that choice as kHalfWorkgroup added to the tile shape index.
 constexpr int kHalfWorkgroup = 16;
 __host__ __device__ constexpr bool half_workgroup_ok(int shape, int K)
 {
-    return iterk_region(shape, K) > 512 && iterk_region(shape, K) <= 1024;
+    return iterk_region(shape, K) > 512 && iterk_region(shape, K) <= 1536;
 }
 __host__ __device__ constexpr int shape_index(int launch_shape) { return launch_shape & (kHalfWorkgroup - 1); }
 static_assert(kShapes <= kHalfWorkgroup, "shape indices fit below the half-workgroup flag");
