@@ -160,8 +160,8 @@ int vfs_get_option(const vfs_solver *s, const char *key, int *value);
 
 /* The launch choices used at `level` by the masked (masked != 0) or unmasked solve:
  * tile shape index of the inner iterations (0..6 as option "shape"; with several
- * iterations per launch also 7..15 = 8x8, 4x4, 32x32, 20x20, 36x25, 27x25, 14x13, 7x7, 43x34,
- * and 1, 3, 4, 6 of the former), whether the inner iteration prefetches its Tp-plane
+ * iterations per launch also 7..17 = 8x8, 4x4, 32x32, 20x20, 36x25, 27x25, 14x13, 7x7,
+ * 43x34, 34x40, 53x25, and 1, 3, 4, 6 of the former), whether the inner iteration prefetches its Tp-plane
  * neighbours, and the inner iterations per launch, 1, 2, 4 or 8 (`per_launch` may be
  * NULL).  Runs the tuner first when it has not run for this geometry.
  * No counterpart in the reference. */

@@ -219,7 +219,7 @@ def test_several_iterations_per_launch_give_identical_results(masked, shape, sto
 
 @pytest.mark.parametrize("masked", [True, False])
 def test_small_tiles_several_iterations_give_identical_results(masked):
-    """The 8x8, 4x4, 32x32, 20x20, 36x25, 27x25, 14x13, 7x7 and 43x34 tiles of the
+    """The 8x8, 4x4, 32x32, 20x20, 36x25, 27x25, 14x13, 7x7, 43x34, 34x40 and 53x25 tiles of the
     multi-iteration launches, eight iterations per launch and the half workgroups (two
     or three positions per work-item; set per level) reproduce one iteration per launch
     bit for bit."""
@@ -228,7 +228,7 @@ def test_small_tiles_several_iterations_give_identical_results(masked):
     g.set_option("per_launch", 1)
     (g.solveStereoForwardMasked if masked else g.solveStereoForward)()
     ref = {n: g.read_plane(n, 0) for n in ("u", "u_", "v_", "warpUV")}
-    cases = [(sh, k, False) for sh in (7, 8, 9, 10, 11, 12, 13, 14, 15) for k in (2, 4)]
+    cases = [(sh, k, False) for sh in (7, 8, 9, 10, 11, 12, 13, 14, 15, 16, 17) for k in (2, 4)]
     cases += [(10, 4, True), (6, 4, True), (4, 2, True), (13, 4, True)]
     cases += [(9, 2, True), (11, 2, True), (1, 4, True), (3, 4, True), (12, 4, True), (13, 8, True)]
     cases += [(sh, 8, False) for sh in (4, 6, 7, 8, 10, 13, 14)] + [(7, 8, True), (8, 8, True), (14, 8, True)]
@@ -248,14 +248,14 @@ def test_small_tiles_several_iterations_give_identical_results(masked):
 def test_level_tuning_rejects_invalid_choices():
     """Launch choices the kernels do not have fail with an error and change nothing:
     eight iterations in tiles whose staged region exceeds 2048 positions (64x4, 32x16,
-    32x32, 36x25, 27x25, 43x34), iteration counts other than 1, 2, 4, 8, half workgroups
+    32x32, 36x25, 27x25, 43x34, 34x40, 53x25), iteration counts other than 1, 2, 4, 8, half workgroups
     outside 513 .. 1536 staged positions, shape indices past the last."""
     from vfs_amd import VfsError
     g = make_gpu(small_pair(96, 64, seed=45), 2, 1, 4)
     before = g.level_tuning(0, True, with_per_launch=True)
     for shape, per_launch, half in [(9, 8, False), (1, 8, False), (3, 8, False), (11, 8, False), (12, 8, False),
-                                    (15, 8, False), (4, 3, False), (4, 16, False), (9, 4, True), (15, 4, True),
-                                    (4, 1, True), (16, 4, False)]:
+                                    (15, 8, False), (16, 8, False), (17, 8, False), (4, 3, False), (4, 16, False),
+                                    (9, 4, True), (15, 4, True), (17, 2, True), (4, 1, True), (18, 4, False)]:
         with pytest.raises(VfsError):
             g.set_level_tuning(0, shape, per_launch=per_launch, half_workgroup=half)
     assert g.level_tuning(0, True, with_per_launch=True) == before
@@ -275,7 +275,7 @@ def test_several_iterations_odd_geometry(masked):
     g.set_option("per_launch", 1)
     (g.solveStereoForwardMasked if masked else g.solveStereoForward)()
     ref = {n: g.read_plane(n, 0) for n in ("u", "u_", "v_", "warpUV")}
-    cases = [(sh, 4) for sh in (1, 3, 4, 6, 7, 8, 9, 10, 11, 12, 13, 14, 15)]
+    cases = [(sh, 4) for sh in (1, 3, 4, 6, 7, 8, 9, 10, 11, 12, 13, 14, 15, 16, 17)]
     cases += [(sh, 8) for sh in (4, 6, 7, 8, 10, 13, 14)]
     for shape, per_launch in cases:
         g = make_gpu(pair, 3, 2, 9)

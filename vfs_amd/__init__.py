@@ -285,9 +285,10 @@ class StereoTgv:
         return v.value
 
     # tile shapes by index: 0 .. 6 are those of the "shape" option; 8x8, 4x4, 32x32,
-    # 20x20, 36x25, 27x25, 14x13, 7x7 and 43x34 exist for the multi-iteration launches only
+    # 20x20, 36x25, 27x25, 14x13, 7x7, 43x34, 34x40 and 53x25 exist for the multi-iteration
+    # launches only
     SHAPES = ((64, 8), (64, 4), (128, 4), (32, 16), (32, 8), (16, 32), (16, 16), (8, 8), (4, 4), (32, 32),
-              (20, 20), (36, 25), (27, 25), (14, 13), (7, 7), (43, 34))
+              (20, 20), (36, 25), (27, 25), (14, 13), (7, 7), (43, 34), (34, 40), (53, 25))
 
     def level_tuning(self, level: int, masked: bool = True, with_per_launch: bool = False):
         """(tile shape (w, h), Tp prefetch[, inner iterations per launch]) used at

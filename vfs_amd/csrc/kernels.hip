@@ -1744,6 +1744,8 @@ void iterk_storage(const PairDesc *desc, const TickArgs &args, int launch_shape,
     case 13: iterk_shape<T, 13, K>(desc, args, half, st); break;
     case 14: iterk_shape<T, 14, K>(desc, args, half, st); break;
     case 15: iterk_shape<T, 15, K>(desc, args, half, st); break;
+    case 16: iterk_shape<T, 16, K>(desc, args, half, st); break;
+    case 17: iterk_shape<T, 17, K>(desc, args, half, st); break;
     default: break;  // the host never asks for another shape
     }
 }

@@ -46,7 +46,7 @@ thread_local std::string g_create_error;
 struct GraphKey {
     int masked, warm_level, warps, iters, start_roles;
     float limit;
-    std::array<unsigned short, kMaxLevels> tuning;  // Tuning::sig() of the program's launch choices
+    std::array<unsigned int, kMaxLevels> tuning;  // Tuning::sig() of the program's launch choices
     bool operator<(const GraphKey &o) const
     {
         if (tuning != o.tuning) return tuning < o.tuning;
@@ -103,13 +103,16 @@ struct LevelTune {
 
 struct Tuning {
     LevelTune lv[kMaxLevels];
-    std::array<unsigned short, kMaxLevels> sig() const
+    std::array<unsigned int, kMaxLevels> sig() const
     {
-        std::array<unsigned short, kMaxLevels> k{};
+        std::array<unsigned int, kMaxLevels> k{};
         for (int l = 0; l < kMaxLevels; l++) {
             const int pl = lv[l].per_launch, code = pl == 8 ? 3 : pl == 4 ? 2 : pl == 2 ? 1 : 0;
-            k[l] = (unsigned short)(lv[l].shape | (lv[l].preload << 3) | (code << 4) | (lv[l].mshape << 6) |
-                                    (lv[l].mhalf << 10) | (lv[l].m2shape << 11) | (lv[l].m2half << 15));
+            // tick shape 3 bits, preload 1, iterations-per-launch code 2, multi-iteration
+            // shapes 5 bits each (kShapes <= 32) with their half flags
+            k[l] = (unsigned int)lv[l].shape | (unsigned int)lv[l].preload << 3 | (unsigned int)code << 4 |
+                   (unsigned int)lv[l].mshape << 6 | (unsigned int)lv[l].mhalf << 11 |
+                   (unsigned int)lv[l].m2shape << 12 | (unsigned int)lv[l].m2half << 17;
         }
         return k;
     }

@@ -24,16 +24,17 @@ constexpr int kMaxSched = 128;
 // or 2) of the tw x th tile.  Shape 0 is the default; the others are candidates of the
 // per-level tile-shape tuner.  The multi-iteration ops (OP_ITER2, OP_ITER4, OP_ITER8) take the
 // shapes of multi_iter_shape(), including the small tiles 8x8 and 4x4 for the coarsest,
-// latency-bound levels, 32x32, 20x20, 36x25, 43x34 and 27x25 for the largest, and 14x13
-// and 7x7 between them (43x34 and 36x25 split an 848x800 image into 480 and 768 tiles, two
-// and three workgroups for each of 256 compute units; 27x25, 14x13 and 7x7 split
-// 424x400, 212x200 and 106x100 into 256, 256 and 240: one each).
-constexpr int kShapes = 16, kTickShapes = 7;
+// latency-bound levels, 32x32, 20x20, 36x25, 43x34, 34x40, 53x25 and 27x25 for the
+// largest, and 14x13 and 7x7 between them (53x25, 34x40, 43x34 and 36x25 split an 848x800
+// image into 512, 500, 480 and 768 tiles, two or three workgroups for each of 256 compute
+// units, 53x25 with the smallest staged region of the two-round ones; 27x25, 14x13 and
+// 7x7 split 424x400, 212x200 and 106x100 into 256, 256 and 240: one each).
+constexpr int kShapes = 18, kTickShapes = 7;
 struct ShapeDim {
     int tw, th;
 };
 constexpr ShapeDim kShape[kShapes] = {{64, 8}, {64, 4}, {128, 4}, {32, 16}, {32, 8}, {16, 32}, {16, 16},
-                                      {8, 8}, {4, 4}, {32, 32}, {20, 20}, {36, 25}, {27, 25}, {14, 13}, {7, 7}, {43, 34}};
+                                      {8, 8}, {4, 4}, {32, 32}, {20, 20}, {36, 25}, {27, 25}, {14, 13}, {7, 7}, {43, 34}, {34, 40}, {53, 25}};
 __host__ __device__ constexpr bool multi_iter_shape(int shape)
 {
     return shape == 1 || shape == 3 || (shape >= 4 && shape != 5);
@@ -48,7 +49,7 @@ __host__ __device__ constexpr int iterk_region(int shape, int K)
 // 1024 above (two or three positions each); regions of 513 .. 1536 positions may instead
 // take 512 work-items with two or three positions each (more workgroups per compute
 // unit).  Launch shapes carry that choice as kHalfWorkgroup added to the tile shape index.
-constexpr int kHalfWorkgroup = 16;
+constexpr int kHalfWorkgroup = 32;
 __host__ __device__ constexpr bool half_workgroup_ok(int shape, int K)
 {
     return iterk_region(shape, K) > 512 && iterk_region(shape, K) <= 1536;
