@@ -646,7 +646,10 @@ hipError_t time_launch_choices(hipStream_t st, const vfs::PairDesc *d_desc, cons
                                const Params &P, int masked, const std::vector<int> &levels,
                                const std::vector<bool> &allow_multi, Tuning &tn)
 {
-    constexpr int kWarm = 3, kReps = 20, kRounds = 2;
+    // a short timing of every candidate, then a longer one of those within 10 % of the
+    // fastest of their kind (one-iteration, multi-iteration, two-iteration launches)
+    constexpr int kWarm = 3, kRounds = 2, kReps[kRounds] = {6, 20};
+    constexpr float kRefine = 1.10f;
     // candidates: (shape, preload) of one iteration per launch, then two-, four- and
     // eight-iteration launches for the shapes that have them, in full or half workgroups
     constexpr int kSingle = vfs::kTickShapes * 2, kCand = kSingle + 3 * vfs::kShapes * 2;
@@ -671,12 +674,24 @@ hipError_t time_launch_choices(hipStream_t st, const vfs::PairDesc *d_desc, cons
     const int n = std::min((int)descs.size(), vfs::kMaxEntries);
     for (int round = 0; round < kRounds && r == hipSuccess; round++) {
         for (size_t li = 0; li < levels.size() && r == hipSuccess; li++) {
+            float fast[3] = {1e30f, 1e30f, 1e30f};  // one-iteration, multi-iteration, two-iteration
+            for (int c = 0; round > 0 && c < kCand; c++) {
+                const float b = best[li * kCand + c];
+                fast[c < kSingle ? 0 : 1] = std::min(fast[c < kSingle ? 0 : 1], b);
+                if (c >= kSingle && c < kSingle + vfs::kShapes * 2) fast[2] = std::min(fast[2], b);
+            }
             for (int c = 0; c < kCand && r == hipSuccess; c++) {
                 int shape, pre, per_launch, half;
                 cand(c, shape, pre, per_launch, half);
                 if (per_launch > 1 && (!vfs::iterk_ok(shape, per_launch) || !allow_multi[li] ||
                                        (half && !vfs::half_workgroup_ok(shape, per_launch))))
                     continue;
+                if (round > 0) {
+                    const float b = best[li * kCand + c];
+                    const bool near = b < kRefine * fast[c < kSingle ? 0 : 1] ||
+                                      (per_launch == 2 && b < kRefine * fast[2]);
+                    if (!near) continue;
+                }
                 ProgOp o{};
                 o.kind = iter_kind(per_launch);
                 o.level = (unsigned char)levels[li];
@@ -700,14 +715,14 @@ hipError_t time_launch_choices(hipStream_t st, const vfs::PairDesc *d_desc, cons
                 auto launch = [&]() { return vfs::launch_tick(d_desc, a, P.storage, o.shape, st); };
                 for (int k = 0; k < kWarm && r == hipSuccess; k++) r = launch();
                 if (r == hipSuccess) r = hipEventRecord(e0, st);
-                for (int k = 0; k < kReps && r == hipSuccess; k++) r = launch();
+                for (int k = 0; k < kReps[round] && r == hipSuccess; k++) r = launch();
                 if (r == hipSuccess) r = hipEventRecord(e1, st);
                 if (r == hipSuccess) r = hipEventSynchronize(e1);
                 float ms = 0;
                 if (r == hipSuccess) r = hipEventElapsedTime(&ms, e0, e1);
                 if (r == hipSuccess) r = hipGetLastError();
-                float &b = best[li * kCand + c];
-                b = std::min(b, ms / (float)per_launch);
+                // ms per iteration; the longer timing replaces the short one
+                best[li * kCand + c] = ms / (float)(kReps[round] * per_launch);
             }
         }
     }
