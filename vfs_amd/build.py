@@ -20,7 +20,13 @@ HIPCC = os.environ.get("HIPCC", shutil.which("hipcc") or "/opt/rocm/bin/hipcc")
 HIP_FLAGS = ["--offload-arch=" + ARCH, "-O3", "-std=c++17", "-fPIC", "-Wall", "-Wno-unused-result",
              "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
 
+# Translation units of libvfs.so: the launchers and the host solver, plus the op kernels
+# instantiated once per storage type (0 float, 1 __half, 2 __hip_bfloat16) and kernel
+# family (1 = tick kernel; 2, 4, 8 = K-iteration kernel) from tgv_inst.hip, compiled in
+# parallel into objects under vfs_amd/_build/.
 LIB_SOURCES = ["kernels.hip", "solver.hip"]
+INST_UNITS = [(st, k) for st in (0, 1, 2) for k in (1, 2, 4, 8)]
+BUILD_DIR = os.path.join(PKG, "_build")
 
 
 def _run(cmd, verbose):
@@ -34,11 +40,29 @@ def _run(cmd, verbose):
         print(out)
 
 
+def _jobs():
+    return max(1, min(int(os.environ.get("MAX_JOBS", os.cpu_count() or 1)), 16))
+
+
 def build_lib(verbose=False) -> str:
+    from concurrent.futures import ThreadPoolExecutor
+    os.makedirs(BUILD_DIR, exist_ok=True)
+    compiles = []
+    for src in LIB_SOURCES:
+        obj = os.path.join(BUILD_DIR, src.replace(".hip", ".o"))
+        compiles.append([HIPCC] + HIP_FLAGS + ["-c", os.path.join(CSRC, src), "-o", obj])
+    for st, k in INST_UNITS:
+        obj = os.path.join(BUILD_DIR, f"tgv_inst_s{st}_k{k}.o")
+        compiles.append([HIPCC] + HIP_FLAGS + [f"-DVFS_INST_STORAGE={st}", f"-DVFS_INST_K={k}", "-c",
+                                              os.path.join(CSRC, "tgv_inst.hip"), "-o", obj])
+    # the slowest units (multi-iteration kernels) first
+    compiles.sort(key=lambda c: 0 if any(f"-DVFS_INST_K={k}" in c for k in (2, 4, 8)) else 1)
+    with ThreadPoolExecutor(_jobs()) as ex:
+        list(ex.map(lambda c: _run(c, verbose), compiles))
     out = os.path.join(PKG, "libvfs.so")
     tmp = out + ".tmp"
-    srcs = [os.path.join(CSRC, s) for s in LIB_SOURCES]
-    _run([HIPCC] + HIP_FLAGS + ["-shared", "-o", tmp] + srcs, verbose)
+    objs = [c[c.index("-o") + 1] for c in compiles]
+    _run([HIPCC, "--offload-arch=" + ARCH, "-shared", "-fPIC", "-o", tmp] + objs, verbose)
     os.replace(tmp, out)
     return out
 

@@ -1,1489 +1,29 @@
-// gfx950 kernels of the StereoTgv TGV-L1 fisheye-stereo solver.
-//
-// Every per-pixel op of the solve runs inside one of two kernels over a *set of frame
-// pairs* (see vfs_engine.h): the tick kernel k_tick (one op per launch, workgroups of 256
-// work-items own tiles of the launch's shape, 64x8 by default) and k_iterk (two or four
-// inner iterations per launch over a staged, recomputed halo).  A prefix table maps the
-// flat workgroup index to (pair, tile), so pairs of different resolutions share a launch.
-//
-// Texture fetches of the reference (linear filter, mirror addressing, normalized
-// coordinates) are evaluated in registers: exact bilinear weights at pixel-index
-// position p = x_norm * dim - 0.5 over the symmetric extension of the image.
-//
-// Storage type T (float, __half, __hip_bfloat16) applies to the image pyramid, the
-// warped image, the derivatives, the tensor and the duals p, q; all arithmetic is FP32.
-#include "vfs_engine.h"
-
-#include <math.h>
-
-#include <type_traits>
-
-// Every product and sum is rounded as written (no multiply-add fusion chosen by the
-// compiler): a pixel's value is computed by the same rounding sequence in every tile
-// shape and in every inlined copy of an op (tile pixel or recomputed halo), so results
-// do not depend on the launch shape.
-#pragma clang fp contract(off)
+// Host launchers of libvfs's kernels: launch_tick (the op kernels, instantiated per storage
+// type and kernel family in tgv_inst.hip) and the single-plane kernels used at load time
+// and for the outputs.
+#include "tgv_device.h"
 
 namespace vfs {
 
+// Op launches, one per (storage type, kernel family): tgv_inst.hip built with
+// VFS_INST_STORAGE (0 float, 1 __half, 2 __hip_bfloat16) and VFS_INST_K (1: k_tick, else
+// k_iterk with that many iterations).
+#define VFS_INST_DECL(sfx) \
+    void launch_ops_##sfx(const PairDesc *desc, const TickArgs &args, int shape, hipStream_t st);
+VFS_INST_DECL(f32_k1)
+VFS_INST_DECL(f32_k2)
+VFS_INST_DECL(f32_k4)
+VFS_INST_DECL(f32_k8)
+VFS_INST_DECL(f16_k1)
+VFS_INST_DECL(f16_k2)
+VFS_INST_DECL(f16_k4)
+VFS_INST_DECL(f16_k8)
+VFS_INST_DECL(bf16_k1)
+VFS_INST_DECL(bf16_k2)
+VFS_INST_DECL(bf16_k4)
+VFS_INST_DECL(bf16_k8)
+
 namespace {
-
-constexpr int NT = 256;  // work-items per workgroup
-
-// Work-item layout of tile shape SH: BX x BY work-items, RPT rows each.
-template <int SH>
-struct Sh {
-    static constexpr int TW = kShape[SH].tw, TH = kShape[SH].th;
-    static constexpr int BX = TW, BY = NT / TW, RPT = TH / BY;
-    static_assert(BX * BY == NT && (RPT == 1 || RPT == 2) && BY * RPT == TH, "tile shape");
-};
-
-// ------------------------------------------------------------------ storage
-
-template <typename T>
-struct Cv;
-template <>
-struct Cv<float> {
-    static __device__ __forceinline__ float f(float v) { return v; }
-    static __device__ __forceinline__ float t(float v) { return v; }
-};
-template <>
-struct Cv<__half> {
-    static __device__ __forceinline__ float f(__half v) { return __half2float(v); }
-    static __device__ __forceinline__ __half t(float v) { return __float2half(v); }
-};
-template <>
-struct Cv<__hip_bfloat16> {
-    static __device__ __forceinline__ float f(__hip_bfloat16 v) { return __bfloat162float(v); }
-    static __device__ __forceinline__ __hip_bfloat16 t(float v) { return __float2bfloat16(v); }
-};
-
-template <typename T>
-struct alignas(2 * sizeof(T)) V2 {
-    T x, y;
-};
-template <typename T>
-struct alignas(4 * sizeof(T)) V4 {
-    T x, y, z, w;
-};
-
-template <typename T>
-__device__ __forceinline__ float ldT(const T *__restrict__ p, long i)
-{
-    return Cv<T>::f(p[i]);
-}
-template <typename T>
-__device__ __forceinline__ void stT(T *__restrict__ p, long i, float v)
-{
-    p[i] = Cv<T>::t(v);
-}
-template <typename T>
-__device__ __forceinline__ float2 ld2T(const V2<T> *__restrict__ p, long i)
-{
-    const V2<T> v = p[i];
-    return make_float2(Cv<T>::f(v.x), Cv<T>::f(v.y));
-}
-template <typename T>
-__device__ __forceinline__ void st2T(V2<T> *__restrict__ p, long i, float2 v)
-{
-    V2<T> o;
-    o.x = Cv<T>::t(v.x);
-    o.y = Cv<T>::t(v.y);
-    p[i] = o;
-}
-template <typename T>
-__device__ __forceinline__ float4 ld4T(const V4<T> *__restrict__ p, long i)
-{
-    const V4<T> v = p[i];
-    return make_float4(Cv<T>::f(v.x), Cv<T>::f(v.y), Cv<T>::f(v.z), Cv<T>::f(v.w));
-}
-template <typename T>
-__device__ __forceinline__ void st4T(V4<T> *__restrict__ p, long i, float4 v)
-{
-    V4<T> o;
-    o.x = Cv<T>::t(v.x);
-    o.y = Cv<T>::t(v.y);
-    o.z = Cv<T>::t(v.z);
-    o.w = Cv<T>::t(v.w);
-    p[i] = o;
-}
-
-// ------------------------------------------------------------------ sampling
-
-__device__ __forceinline__ int mir(int k, int n)
-{
-    if ((unsigned)k < (unsigned)n) return k;
-    if (k < 0) k = -1 - k;
-    if (k >= n) {
-        const int P = 2 * n;
-        k %= P;
-        if (k >= n) k = P - 1 - k;
-    }
-    return k;
-}
-
-__device__ __forceinline__ float clampc(float v) { return fminf(fmaxf(v, -1e8f), 1e8f); }
-
-// Bilinear fetch with mirror addressing at pixel-index position (px, py).
-template <typename T>
-__device__ __forceinline__ float bil(const T *__restrict__ img, int w, int h, int s, float px, float py)
-{
-    if (__builtin_isnan(px) || __builtin_isnan(py)) return __builtin_nanf("");
-    px = clampc(px);
-    py = clampc(py);
-    const float fx = floorf(px), fy = floorf(py);
-    const float ax = px - fx, ay = py - fy;
-    const int i = (int)fx, j = (int)fy;
-    const int i0 = mir(i, w), i1 = mir(i + 1, w);
-    const int j0 = mir(j, h), j1 = mir(j + 1, h);
-    const T *r0 = img + (size_t)j0 * s;
-    const T *r1 = img + (size_t)j1 * s;
-    const float top = (1.f - ax) * Cv<T>::f(r0[i0]) + ax * Cv<T>::f(r0[i1]);
-    const float bot = (1.f - ax) * Cv<T>::f(r1[i0]) + ax * Cv<T>::f(r1[i1]);
-    return (1.f - ay) * top + ay * bot;
-}
-
-__device__ __forceinline__ float2 bil2(const float2 *__restrict__ img, int w, int h, int s, float px, float py)
-{
-    if (__builtin_isnan(px) || __builtin_isnan(py)) {
-        const float n = __builtin_nanf("");
-        return make_float2(n, n);
-    }
-    px = clampc(px);
-    py = clampc(py);
-    const float fx = floorf(px), fy = floorf(py);
-    const float ax = px - fx, ay = py - fy;
-    const int i = (int)fx, j = (int)fy;
-    const int i0 = mir(i, w), i1 = mir(i + 1, w);
-    const int j0 = mir(j, h), j1 = mir(j + 1, h);
-    const float2 *r0 = img + (size_t)j0 * s;
-    const float2 *r1 = img + (size_t)j1 * s;
-    const float2 a = r0[i0], b = r0[i1], c = r1[i0], d = r1[i1];
-    float2 o;
-    o.x = (1.f - ay) * ((1.f - ax) * a.x + ax * b.x) + ay * ((1.f - ax) * c.x + ax * d.x);
-    o.y = (1.f - ay) * ((1.f - ax) * a.y + ax * b.y) + ay * ((1.f - ax) * c.y + ax * d.y);
-    return o;
-}
-
-template <typename T>
-__device__ __forceinline__ float texel(const T *__restrict__ img, int w, int h, int s, int x, int y)
-{
-    return Cv<T>::f(img[(size_t)mir(y, h) * s + mir(x, w)]);
-}
-
-// Circle gate of the unmasked solve (updateDualVariables.cu:13-22): true when
-// radius >= width / 2.2 around (w/2, h/2).  Evaluated in double on the float bound.
-__device__ __forceinline__ bool outside_circle(int x, int y, int w, int h)
-{
-    const float desired = (float)w / 2.20f;
-    const double hw = (double)((float)w / 2.0f), hh = (double)((float)h / 2.0f);
-    const double dx = (double)x - hw, dy = (double)y - hh;
-    return dx * dx + dy * dy >= (double)desired * (double)desired;
-}
-
-__device__ __forceinline__ float2 ld0(const float2 *__restrict__ b, long i)
-{
-    return i < 0 ? make_float2(0.f, 0.f) : b[i];
-}
-
-// ------------------------------------------------------------------ tiling
-
-struct Tile {
-    const PairDesc *d;
-    float *arena;        // the pair's work arena
-    long long plane;     // floats per plane slot
-    const float *mask;   // mask plane of the tile level
-    int x0, y0, w, h, s;
-};
-
-__device__ __forceinline__ float *slot(const Tile &t, int k) { return t.arena + (long long)k * t.plane; }
-
-template <typename P>
-__device__ __forceinline__ P *slotp(const Tile &t, int k)
-{
-    return reinterpret_cast<P *>(t.arena + (long long)k * t.plane);
-}
-
-// Iterates the calling work-item over its pixels of the tile (S::RPT rows).
-#define FOR_TILE_PIXELS(S, t, x, y)                                                   \
-    for (int y = (t).y0 + (int)threadIdx.y, x = (t).x0 + (int)threadIdx.x;            \
-         y < (t).y0 + S::TH && y < (t).h && x < (t).w; y += S::BY)
-
-// ------------------------------------------------------------------ solve ops
-//
-// Each op works on one tile (shape S) of one pair; k_tick (below) dispatches them.
-
-// Fresh state: zero every work plane of the pair at the tile's level-0 positions
-// (the tile grid spans the stride, so row padding is cleared too).
-template <class S>
-__device__ __forceinline__ void op_zero(const Tile &t)
-{
-    FOR_TILE_PIXELS(S, t, x, y)
-    {
-        const long pos = (long)y * t.s + x;
-#pragma unroll 5
-        for (int k = 0; k < PL_COUNT; k++) slot(t, k)[pos] = 0.f;
-    }
-}
-
-// Before the multi-iteration launches of a warp at a coarse level of a warm solve: the
-// spare buffers of u, u_ and v_ take the current buffers' values beyond the level's flat
-// extent (stride x height), where level-0 data of the previous solve lies and every
-// buffer differs.  A multi-iteration launch makes the spare buffers current there
-// without writing them; with this copy the current buffers keep their values, as the
-// reference's buffers do beyond the level's extent.  The tile covers the level-0 planes.
-template <class S>
-__device__ __forceinline__ void op_sync(const Tile &t, int level, int roles)
-{
-    const long ext = (long)t.d->s[level] * t.d->h[level];
-    const float *u = slot(t, slot_u(rcur(roles, G_U)));
-    float *us = slot(t, slot_u(rspare(roles, G_U)));
-    const float *u_ = slot(t, slot_ub(rcur(roles, G_UB)));
-    float *u_s = slot(t, slot_ub(rspare(roles, G_UB)));
-    const float2 *v = slotp<float2>(t, slot_v(rcur(roles, G_V)));
-    float2 *vs = slotp<float2>(t, slot_v(rspare(roles, G_V)));
-    FOR_TILE_PIXELS(S, t, x, y)
-    {
-        const long pos = (long)y * t.s + x;
-        if (pos >= ext) {
-            us[pos] = u[pos];
-            u_s[pos] = u_[pos];
-            vs[pos] = v[pos];
-        }
-    }
-}
-
-// Calibration warp of I1 (stereotgv.cu:392-393, warpImage.cu:9-23) into the level-0
-// pyramid slot; with 16-bit storage also narrows I0 into its pyramid slot.
-template <class S, typename T>
-__device__ __forceinline__ void op_calib(const Tile &t)
-{
-    const float *i1 = t.d->in_i1;
-    T *o1 = (T *)t.d->I1[0];
-    FOR_TILE_PIXELS(S, t, x, y)
-    {
-        const long pos = (long)y * t.s + x;
-        const float2 c = t.d->cv[pos];
-        stT(o1, pos, bil(i1, t.w, t.h, t.s, (float)x + c.x, (float)y + c.y));
-        if (sizeof(T) != sizeof(float)) stT((T *)t.d->I0[0], pos, t.d->in_i0[pos]);
-    }
-}
-
-// downscale.cu:10-30 for I0 and I1 (stereotgv.cu:400-405): mean of 4 bilinear taps at
-// +-1/4 coarse pixel around the coarse pixel centre, fetched from the finer level.
-template <class S, typename T>
-__device__ __forceinline__ void op_down(const Tile &t, int level)
-{
-    const int W = t.d->w[level - 1], H = t.d->h[level - 1], SF = t.d->s[level - 1];
-    const T *a = (const T *)t.d->I0[level - 1];
-    const T *b = (const T *)t.d->I1[level - 1];
-    T *oa = (T *)t.d->I0[level];
-    T *ob = (T *)t.d->I1[level];
-    const float rx = (float)W / (float)t.w, ry = (float)H / (float)t.h;
-    const float ox = 0.25f * rx, oy = 0.25f * ry;
-    FOR_TILE_PIXELS(S, t, x, y)
-    {
-        const float cx = ((float)x + 0.5f) * rx - 0.5f, cy = ((float)y + 0.5f) * ry - 0.5f;
-        const long pos = (long)y * t.s + x;
-        stT(oa, pos, 0.25f * (bil(a, W, H, SF, cx - ox, cy) + bil(a, W, H, SF, cx + ox, cy) +
-                              bil(a, W, H, SF, cx, cy - oy) + bil(a, W, H, SF, cx, cy + oy)));
-        stT(ob, pos, 0.25f * (bil(b, W, H, SF, cx - ox, cy) + bil(b, W, H, SF, cx + ox, cy) +
-                              bil(b, W, H, SF, cx, cy - oy) + bil(b, W, H, SF, cx, cy + oy)));
-    }
-}
-
-// Coarsest-level init warpUV = u * tv (stereotgv.cu:419-421, warpImage.cu:136-150).
-template <class S>
-__device__ __forceinline__ void op_cinit(const Tile &t, int level, int roles)
-{
-    const float *u = slot(t, slot_u(rcur(roles, G_U)));
-    float2 *uv = slotp<float2>(t, slot_uv(roles, true));
-    const float2 *tv = t.d->tv[level];
-    FOR_TILE_PIXELS(S, t, x, y)
-    {
-        const long pos = (long)y * t.s + x;
-        const float us = u[pos];
-        const float2 v = tv[pos];
-        uv[pos] = make_float2(us * v.x, us * v.y);
-    }
-}
-
-// Warm start: u, u_ and warpUV of the finer level downscaled to `level` and multiplied
-// by `scale` (= 1/fScale) into the other slot of each pair: the 4-tap Downscale with a
-// scale factor of downscale.cu:148-237.
-template <class S>
-__device__ __forceinline__ void op_winit(const Tile &t, int level, int roles, float scale)
-{
-    const int W = t.d->w[level - 1], H = t.d->h[level - 1], SF = t.d->s[level - 1];
-    const float *u = slot(t, slot_u(rcur(roles, G_U)));
-    float *ou = slot(t, slot_u(roth(roles, G_U)));
-    const float *u_ = slot(t, slot_ub(rcur(roles, G_UB)));
-    float *ou_ = slot(t, slot_ub(roth(roles, G_UB)));
-    const float2 *uv = slotp<float2>(t, slot_uv(roles, true));
-    float2 *ouv = slotp<float2>(t, slot_uv(roles, false));
-    const float rx = (float)W / (float)t.w, ry = (float)H / (float)t.h;
-    const float ox = 0.25f * rx, oy = 0.25f * ry;
-    FOR_TILE_PIXELS(S, t, x, y)
-    {
-        const float cx = ((float)x + 0.5f) * rx - 0.5f, cy = ((float)y + 0.5f) * ry - 0.5f;
-        const long pos = (long)y * t.s + x;
-        ou[pos] = scale * 0.25f * (bil(u, W, H, SF, cx - ox, cy) + bil(u, W, H, SF, cx + ox, cy) +
-                                   bil(u, W, H, SF, cx, cy - oy) + bil(u, W, H, SF, cx, cy + oy));
-        ou_[pos] = scale * 0.25f * (bil(u_, W, H, SF, cx - ox, cy) + bil(u_, W, H, SF, cx + ox, cy) +
-                                    bil(u_, W, H, SF, cx, cy - oy) + bil(u_, W, H, SF, cx, cy + oy));
-        const float2 v0 = bil2(uv, W, H, SF, cx - ox, cy), v1 = bil2(uv, W, H, SF, cx + ox, cy);
-        const float2 v2 = bil2(uv, W, H, SF, cx, cy - oy), v3 = bil2(uv, W, H, SF, cx, cy + oy);
-        ouv[pos] = make_float2(scale * 0.25f * (v0.x + v1.x + v2.x + v3.x), scale * 0.25f * (v0.y + v1.y + v2.y + v3.y));
-    }
-}
-
-// 3x3 binomial of the level image at (x, y), mirror addressing (calcTensor.cu:245-273).
-template <typename T>
-__device__ __forceinline__ float gauss3(const T *__restrict__ I, int w, int h, int s, int x, int y)
-{
-    float r = 0.25f * texel(I, w, h, s, x, y);
-    r += 0.0625f * texel(I, w, h, s, x - 1, y - 1);
-    r += 0.0625f * texel(I, w, h, s, x - 1, y + 1);
-    r += 0.0625f * texel(I, w, h, s, x + 1, y - 1);
-    r += 0.0625f * texel(I, w, h, s, x + 1, y + 1);
-    r += 0.125f * texel(I, w, h, s, x - 1, y);
-    r += 0.125f * texel(I, w, h, s, x + 1, y);
-    r += 0.125f * texel(I, w, h, s, x, y - 1);
-    r += 0.125f * texel(I, w, h, s, x, y + 1);
-    return r;
-}
-
-// The smoothed image as the tensor kernel samples it: the masked blur writes only
-// mask != 0 pixels of a zeroed plane, and the texture fetch mirrors at the level edge.
-template <typename T, bool MASKED>
-__device__ __forceinline__ float smoothed(const T *__restrict__ I, const float *__restrict__ mask, int w, int h,
-                                          int s, int x, int y)
-{
-    const int mx = mir(x, w), my = mir(y, h);
-    if (MASKED && mask[(size_t)my * s + mx] == 0.f) return 0.f;
-    return gauss3(I, w, h, s, mx, my);
-}
-
-// Fused GaussianMasked + CalcTensorMasked (calcTensor.cu:154-273) or the unmasked pair
-// (calcTensor.cu:7-128).  Writes a, b, c for every in-image pixel: the tensor where the
-// reference computes it, 0 where the reference leaves its per-level memset (mask == 0,
-// or columns x >= xlimit: calcTensor.cu:81,227 size the launch by the HEIGHT).  The
-// preconditioners eta (preconditioning.cu) are recomputed from (a, b, c) where needed.
-template <class S, typename T, bool MASKED>
-__device__ __forceinline__ void op_tensor(const Tile &t, int level, int xblock, float beta, float gamma)
-{
-    const T *I0 = (const T *)t.d->I0[level];
-    const float *mask = t.mask;
-    T *at = (T *)slot(t, PL_A);
-    T *bt = (T *)slot(t, PL_B);
-    T *ct = (T *)slot(t, PL_C);
-    const int xlimit = xblock * ((t.h + xblock - 1) / xblock);
-    FOR_TILE_PIXELS(S, t, x, y)
-    {
-        const long pos = (long)y * t.s + x;
-        float a = 0.f, b = 0.f, c = 0.f;
-        const bool active = MASKED ? (mask[pos] != 0.f) : true;
-        if (active && x < xlimit) {
-            const float s11 = smoothed<T, MASKED>(I0, mask, t.w, t.h, t.s, x + 1, y + 1);
-            const float s01 = smoothed<T, MASKED>(I0, mask, t.w, t.h, t.s, x, y + 1);
-            const float s10 = smoothed<T, MASKED>(I0, mask, t.w, t.h, t.s, x + 1, y);
-            const float gx = s11 - s01;
-            const float gy = s11 - s10;
-            const float absg = sqrtf(gx * gx + gy * gy);
-            float nx = gx / absg, ny = gy / absg;
-            if (absg < 1e-8f) {
-                nx = 1.f;
-                ny = 0.f;
-            }
-            const float tx = ny, ty = -nx;
-            float wt = expf(-beta * powf(absg, gamma));
-            if (!(wt > 1e-8f)) wt = 1e-8f;
-            a = wt * nx * nx + tx * tx;
-            c = wt * nx * ny + tx * ty;
-            b = wt * ny * ny + ty * ty;
-        }
-        stT(at, pos, a);
-        stT(bt, pos, b);
-        stT(ct, pos, c);
-    }
-}
-
-// Per-warp preparation (stereotgv.cu:431-447): tv2 = bilinear(tv, x + warpUV)
-// (warpImage.cu:96-111), I1w = bilinear(I1, x + warpUV) (warpImage.cu:9-23),
-// u_last = u (copyClone.cu:3-12), v_ = 0 (Clone of the zeroed v).
-template <class S, typename T>
-__device__ __forceinline__ void op_prep(const Tile &t, int level, int roles)
-{
-    const float2 *uv = slotp<float2>(t, slot_uv(roles, true));
-    const float *u = slot(t, slot_u(rcur(roles, G_U)));
-    float2 *v_ = slotp<float2>(t, slot_v(rcur(roles, G_V)));
-    const float2 *tv = t.d->tv[level];
-    const T *I1 = (const T *)t.d->I1[level];
-    float2 *tv2 = slotp<float2>(t, PL_TV2);
-    T *i1w = (T *)slot(t, PL_I1W);
-    float *ulast = slot(t, PL_ULAST);
-    FOR_TILE_PIXELS(S, t, x, y)
-    {
-        const long pos = (long)y * t.s + x;
-        const float2 d = uv[pos];
-        const float px = (float)x + d.x, py = (float)y + d.y;
-        tv2[pos] = bil2(tv, t.w, t.h, t.s, px, py);
-        stT(i1w, pos, bil(I1, t.w, t.h, t.s, px, py));
-        ulast[pos] = u[pos];
-        v_[pos] = make_float2(0.f, 0.f);
-    }
-}
-
-// computeDerivatives.cu:88-129: 5-point derivative along t = tv/|tv| of I0 and I1w,
-// averaged; Iz = I1w - I0 at the texel centre.
-template <class S, typename T>
-__device__ __forceinline__ void op_deriv(const Tile &t, int level)
-{
-    const T *I0 = (const T *)t.d->I0[level];
-    const T *I1w = (const T *)slot(t, PL_I1W);
-    const float2 *tv = t.d->tv[level];
-    T *Iu = (T *)slot(t, PL_IU);
-    T *Iz = (T *)slot(t, PL_IZ);
-    FOR_TILE_PIXELS(S, t, x, y)
-    {
-        const long pos = (long)y * t.s + x;
-        const float2 v = tv[pos];
-        const float r = sqrtf(v.x * v.x + v.y * v.y);
-        const float dx = v.x / r, dy = v.y / r;
-        const float X = (float)x, Y = (float)y;
-        const int w = t.w, h = t.h, s = t.s;
-        float t0 = bil(I0, w, h, s, X - 2.f * dx, Y - 2.f * dy);
-        t0 -= bil(I0, w, h, s, X - dx, Y - dy) * 8.f;
-        t0 += bil(I0, w, h, s, X + dx, Y + dy) * 8.f;
-        t0 -= bil(I0, w, h, s, X + 2.f * dx, Y + 2.f * dy);
-        t0 /= 12.f;
-        float t1 = bil(I1w, w, h, s, X - 2.f * dx, Y - 2.f * dy);
-        t1 -= bil(I1w, w, h, s, X - dx, Y - dy) * 8.f;
-        t1 += bil(I1w, w, h, s, X + dx, Y + dy) * 8.f;
-        t1 -= bil(I1w, w, h, s, X + 2.f * dx, Y + 2.f * dy);
-        t1 /= 12.f;
-        stT(Iu, pos, (t0 + t1) * 0.5f);
-        stT(Iz, pos, ldT(I1w, pos) - ldT(I0, pos));
-    }
-}
-
-// The inner iteration divides and normalises several times per pixel; it multiplies by
-// the hardware reciprocal and reciprocal square root (about 1 ulp) instead of the
-// correctly rounded expansions, which are several times longer.  NaN and infinity
-// propagate as in IEEE division.
-
-// The inner iteration (dual_step, thresh_core, primal_step) writes every product that is
-// summed as an explicit fused multiply-add (fmaf: one rounding), as the reference's
-// compiler fuses them; explicit, so that every copy of the code rounds alike.
-
-// Per-pixel constants of the primal step (fixed within a warp: they depend on the
-// tensor only): the reciprocals of eta_u (the threshold's step scale) and of the two
-// eta_v, each 1 where the reference divides by zero only in its zero branch.
-struct PrimalCoef {
-    float rt, r1, r2;
-    float ri, iu2;  // 1 / Iu (0 for Iu = 0) and Iu^2 of the threshold
-    bool e2z;  // eta_v2 == 0 (alpha0 == 0 and a = c = 0): the reference's zero branch
-};
-
-__device__ __forceinline__ PrimalCoef primal_coef(const IterParams &ip, float a, float b, float c, float iu)
-{
-    const float aa = ip.alpha1 * ip.alpha1;
-    const float etau = fmaf(b + c, b + c, fmaf(a + c, a + c, fmaf(2 * c, c, fmaf(a, a, b * b)))) * aa;
-    const float a04 = 4 * ip.alpha0 * ip.alpha0;
-    const float e1 = fmaf(aa, fmaf(b, b, c * c), a04);
-    const float e2 = fmaf(aa, fmaf(a, a, c * c), a04);
-    PrimalCoef k;
-    k.rt = etau == 0.f ? 1.0f : __builtin_amdgcn_rcpf(etau);
-    k.r1 = e1 == 0.f ? 1.0f : __builtin_amdgcn_rcpf(e1);
-    k.e2z = e2 == 0.f;
-    k.r2 = k.e2z ? 1.0f : __builtin_amdgcn_rcpf(e2);
-    k.ri = iu == 0.f ? 0.f : __builtin_amdgcn_rcpf(iu);
-    k.iu2 = iu * iu;
-    return k;
-}
-
-// L1 proximal step (thresholdingL1.cu:180-215), linearised at the current u; k.rt is
-// 1 / eta_u (1 for eta_u = 0, where the reference takes tau itself).
-__device__ __forceinline__ float thresh_core(float divp, float tau, const PrimalCoef &k, float u_pos, float upos,
-                                             float Iu, float Iz, float lambda)
-{
-    const float te = tau * k.rt;
-    const float uhat = fmaf(te, divp, u_pos);
-    const float dun = uhat - upos;
-    const float rho = fmaf(Iu, dun, Iz);
-    const float lt = lambda * te;
-    const float upper = lt * k.iu2;
-    const float lower = -upper;
-    float du = 0.f;
-    if ((rho <= upper) && (rho >= lower)) {
-        du = (Iu == 0.f) ? dun : fmaf(-rho, k.ri, dun);
-    } else if (rho < lower) {
-        du = fmaf(lt, Iu, dun);
-    } else if (rho > upper) {
-        du = fmaf(-lt, Iu, dun);
-    }
-    return upos + du;
-}
-
-
-// Operands of the dual update at one pixel, gathered with in-buffer (clamped) addresses
-// and no data-dependent branch, so that all loads of a work-item issue back to back.
-struct DualIn {
-    long pos;
-    bool ok;  // inside the level image
-    bool act; // processed: in the mask (masked form) / in the image (unmasked form)
-    bool hasR, hasD;
-    float a, b, c, u0, uR, uD, uL, uU;
-    float2 v0, vR, vD, vL, vU, po;
-    float4 qo;
-};
-
-template <bool MASKED, typename T>
-__device__ __forceinline__ void dual_load(DualIn &r, int x, int y, int w, int h, int s, const float *__restrict__ mask,
-                                          const float *__restrict__ u_, const float2 *__restrict__ v_,
-                                          const T *__restrict__ at, const T *__restrict__ bt, const T *__restrict__ ct,
-                                          const V2<T> *__restrict__ p_in, const V4<T> *__restrict__ q_in, bool first)
-{
-    r.ok = x >= 0 && y >= 0 && x < w && y < h;
-    const int xc = min(max(x, 0), w - 1), yc = min(max(y, 0), h - 1);
-    const long pos = (long)yc * s + xc;
-    r.pos = pos;
-    const long pR = (xc + 1 < w) ? pos + 1 : pos;
-    const long pD = (yc + 1 < h) ? pos + s : pos;
-    const long pL = pos >= 1 ? pos - 1 : pos;
-    const long pU = pos >= s ? pos - s : pos;
-    if (MASKED) {
-        const float m0 = mask[pos], mR = mask[pR], mD = mask[pD];
-        r.act = r.ok && m0 != 0.f;
-        r.hasR = (xc + 1 < w) && mR != 0.f;
-        r.hasD = (yc + 1 < h) && mD != 0.f;
-    } else {
-        r.act = r.ok;
-        r.hasR = xc + 1 < w;
-        r.hasD = yc + 1 < h;
-    }
-    r.a = ldT(at, pos);
-    r.b = ldT(bt, pos);
-    r.c = ldT(ct, pos);
-    r.u0 = u_[pos];
-    r.uR = u_[pR];
-    r.uD = u_[pD];
-    r.uL = pos >= 1 ? u_[pL] : 0.f;   // flat index -1 reads 0
-    r.uU = pos >= s ? u_[pU] : 0.f;
-    r.v0 = v_[pos];
-    r.vR = v_[pR];
-    r.vD = v_[pD];
-    r.vL = pos >= 1 ? v_[pL] : make_float2(0.f, 0.f);
-    r.vU = pos >= s ? v_[pU] : make_float2(0.f, 0.f);
-    if (first) {
-        r.po = make_float2(0.f, 0.f);
-        r.qo = make_float4(0.f, 0.f, 0.f, 0.f);
-    } else {
-        r.po = ld2T(p_in, pos);
-        r.qo = ld4T(q_in, pos);
-    }
-}
-
-// updateDualVariables.cu:152-236 (masked) / :28-98 (unmasked): ascent on p and q with
-// unit-ball projection, then Tp = T p (thresholdingL1.cu:242-243).
-template <bool MASKED>
-__device__ __forceinline__ void dual_step(const DualIn &r, int x, int y, int w, int h, const IterParams &ip,
-                                          float2 &pn, float4 &qn, float2 &tpn)
-{
-    if (!MASKED && outside_circle(x, y, w, h)) {
-        pn = make_float2(0.f, 0.f);
-        qn = make_float4(0.f, 0.f, 0.f, 0.f);
-    } else {
-        const float ux = r.hasR ? (r.uR - r.u0 - r.v0.x) : (r.u0 - r.uL - r.v0.x);
-        const float uy = r.hasD ? (r.uD - r.u0 - r.v0.y) : (r.u0 - r.uU - r.v0.y);
-        const float dtx = fmaf(r.a, ux, r.c * uy);
-        const float dty = fmaf(r.c, ux, r.b * uy);
-        const float px = fmaf(ip.k1, dtx, r.po.x);
-        const float py = fmaf(ip.k1, dty, r.po.y);
-        // projection onto the unit ball: p / max(|p|, 1), with the hardware reciprocal
-        // square root
-        const float sp = fmaf(px, px, py * py);
-        const float irp = sp > 1.0f ? __builtin_amdgcn_rsqf(sp) : 1.0f;
-        pn = make_float2(px * irp, py * irp);
-        float g0, g1, g2, g3;
-        if (r.hasR) {
-            g0 = r.vR.x - r.v0.x;
-            g3 = r.vR.y - r.v0.y;
-        } else {
-            g0 = r.v0.x - r.vL.x;
-            g3 = r.v0.y - r.vL.y;
-        }
-        if (r.hasD) {
-            g1 = r.vD.y - r.v0.y;
-            g2 = r.vD.x - r.v0.x;
-        } else {
-            g1 = r.v0.y - r.vU.y;
-            g2 = r.v0.x - r.vU.x;
-        }
-        const float q0 = fmaf(ip.k0, g0, r.qo.x), q1 = fmaf(ip.k0, g1, r.qo.y), q2 = fmaf(ip.k0, g2, r.qo.z),
-                    q3 = fmaf(ip.k0, g3, r.qo.w);
-        const float sq = fmaf(q0, q0, fmaf(q1, q1, fmaf(q2, q2, q3 * q3)));
-        const float irq = sq > 1.0f ? __builtin_amdgcn_rsqf(sq) : 1.0f;
-        qn = make_float4(q0 * irq, q1 * irq, q2 * irq, q3 * irq);
-    }
-    tpn = make_float2(fmaf(r.a, pn.x, r.c * pn.y), fmaf(r.c, pn.x, r.b * pn.y));
-}
-
-// Thresholding and primal step at one pixel (thresholdingL1.cu:176-216 / :60-86,
-// updatePrimalVariables.cu:165-245 / :30-113) from the new duals: divp is the divergence
-// of Tp, qc4 / pp the pixel's new q / p, ql / qu the new q of its left / upper
-// neighbours (read only where they are inside the mask).  Returns u, u_ and v_ of the
-// next iteration.
-__device__ __forceinline__ void primal_core(const IterParams &ip, const PrimalCoef &k, float a, float b, float c,
-                                            float u_pos, float uu, float iu, float iz, float2 vo, bool hR, bool hD,
-                                            bool hL, bool hU, float divp, float4 qc4, float4 ql, float4 qu,
-                                            float2 pp, float &un, float &u_s, float2 &v_s)
-{
-    un = thresh_core(divp, ip.tau, k, u_pos, uu, iu, iz, ip.lambda);
-    float qcx = 0.f, qcw = 0.f, qcy = 0.f, qcz = 0.f;
-    if (hR) {
-        qcx = qc4.x;
-        qcw = qc4.w;
-    }
-    if (hD) {
-        qcy = qc4.y;
-        qcz = qc4.z;
-    }
-    float qwx = 0.f, qww = 0.f, qny = 0.f, qnz = 0.f;
-    if (hL) {
-        qwx = ql.x;
-        qww = ql.w;
-    }
-    if (hU) {
-        qny = qu.y;
-        qnz = qu.z;
-    }
-    const float dqx = (qcx - qwx) + (qcz - qnz);
-    const float dqy = (qcw - qww) + (qcy - qny);
-    const float tx = fmaf(a, pp.x, c * pp.y);
-    const float tyy = fmaf(c, pp.x, b * pp.y);
-    const float gx = fmaf(ip.alpha1, tx, ip.alpha0 * dqx);
-    const float gy = fmaf(ip.alpha1, tyy, ip.alpha0 * dqy);
-    // eta_v1, eta_v2 >= 4 alpha0^2 > 0 for alpha0 != 0; the reference's zero branches
-    // (updatePrimalVariables.cu:228-239, the second writes .x) are kept for alpha0 == 0.
-    float vx = fmaf(ip.tau * k.r1, gx, vo.x);
-    float vy;
-    if (!k.e2z) {
-        vy = fmaf(ip.tau * k.r2, gy, vo.y);
-    } else {
-        vx = fmaf(ip.tau, gx, vo.x);
-        vy = 0.f;
-    }
-    u_s = fmaf(ip.mu, un - u_pos, un);
-    v_s = make_float2(fmaf(ip.mu, vx - vo.x, vx), fmaf(ip.mu, vy - vo.y, vy));
-}
-
-__device__ __forceinline__ void primal_step(const IterParams &ip, float a, float b, float c, float u_pos, float uu,
-                                            float iu, float iz, float2 vo, bool hR, bool hD, bool hL, bool hU,
-                                            float divp, float4 qc4, float4 ql, float4 qu, float2 pp, float &un,
-                                            float &u_s, float2 &v_s)
-{
-    primal_core(ip, primal_coef(ip, a, b, c, iu), a, b, c, u_pos, uu, iu, iz, vo, hR, hD, hL, hU, divp, qc4, ql, qu,
-                pp, un, u_s, v_s);
-}
-
-// Divergence of Tp at a pixel (thresholdingL1.cu:157-175 masked / :29-45 unmasked): tp0
-// the pixel's new Tp, tpLx / tpUy the left / upper neighbour's.
-template <bool MASKED>
-__device__ __forceinline__ float div_tp(int x, int y, int w, int h, bool hR, bool hD, bool hL, bool hU, float2 tp0,
-                                        float tpLx, float tpUy)
-{
-    float dx, dy;
-    if (MASKED) {
-        if (hL && hR) dx = tp0.x - tpLx;
-        else if (!hR) dx = -tpLx;
-        else dx = tp0.x;
-        if (hU && hD) dy = tp0.y - tpUy;
-        else if (!hD) dy = -tpUy;
-        else dy = tp0.y;
-    } else {
-        if (hL) dx = tp0.x - tpLx;
-        else if (x == w - 1) dx = -tpLx;
-        else dx = tp0.x;
-        if (hU) dy = tp0.y - tpUy;
-        else if (y == h - 1) dy = -tpUy;
-        else dy = tp0.y;
-    }
-    return dx + dy;
-}
-
-// One TGV-L1 inner iteration in one launch: dual ascent + projection
-// (updateDualVariables.cu:122-237 / :4-101), Tp (thresholdingL1.cu:234-244), L1
-// thresholding (thresholdingL1.cu:128-216 / :3-86) and the primal step with
-// over-relaxation (updatePrimalVariables.cu:138-245 / :3-113).
-// Phase A computes the new p, q and Tp of the tile and of its left column and top row
-// (the threshold and primal of a pixel read Tp and q of its left and upper neighbours)
-// into LDS: each work-item takes the S::RPT tile pixels it finishes in phase B, and the
-// first TW + TH + 1 work-items also one halo position.  Phase B finishes thresholding and
-// the primal step.  Old duals are read from one buffer and new ones written to the other,
-// so every workgroup recomputes its halo from the previous iteration's values (the
-// reference's Jacobi order) and the result does not depend on the tile shape.  Every
-// load is issued before any data-dependent branch.
-template <class S, bool MASKED, typename T>
-__device__ __forceinline__ void op_iter(const Tile &t, int roles, const IterParams &ip,
-                                        float2 (*sP)[S::TW + 1], float4 (*sQ)[S::TW + 1], float2 (*sT)[S::TW + 1])
-{
-    constexpr int EXW = S::TW + 1;
-    constexpr int HALO = EXW + S::TH;  // top row (incl. corner) + left column positions
-    static_assert(HALO <= NT, "halo positions per workgroup");
-    constexpr int BX = S::BX, BY = S::BY, RPT = S::RPT;
-    const int w = t.w, h = t.h, s = t.s, x0 = t.x0, y0 = t.y0;
-    const float *__restrict__ mask = t.mask;
-    const float *__restrict__ u = slot(t, slot_u(rcur(roles, G_U)));
-    float *__restrict__ us = slot(t, slot_u(roth(roles, G_U)));
-    const float *__restrict__ u_ = slot(t, slot_ub(rcur(roles, G_UB)));
-    float *__restrict__ u_s = slot(t, slot_ub(roth(roles, G_UB)));
-    const float2 *__restrict__ v_ = slotp<float2>(t, slot_v(rcur(roles, G_V)));
-    float2 *__restrict__ v_s = slotp<float2>(t, slot_v(roth(roles, G_V)));
-    const V2<T> *__restrict__ p_in = slotp<V2<T>>(t, slot_p(rcur(roles, G_PQ)));
-    V2<T> *__restrict__ p_out = slotp<V2<T>>(t, slot_p(roth(roles, G_PQ)));
-    const V4<T> *__restrict__ q_in = slotp<V4<T>>(t, slot_q(rcur(roles, G_PQ)));
-    V4<T> *__restrict__ q_out = slotp<V4<T>>(t, slot_q(roth(roles, G_PQ)));
-    float2 *__restrict__ Tp = slotp<float2>(t, PL_TP);
-    const T *__restrict__ at = (const T *)slot(t, PL_A);
-    const T *__restrict__ bt = (const T *)slot(t, PL_B);
-    const T *__restrict__ ct = (const T *)slot(t, PL_C);
-    const T *__restrict__ Iu = (const T *)slot(t, PL_IU);
-    const T *__restrict__ Iz = (const T *)slot(t, PL_IZ);
-    const int tid = threadIdx.y * BX + threadIdx.x;
-    const int x = x0 + threadIdx.x;
-
-    // ---- loads: the tile pixels (rows threadIdx.y + BY * r) + one halo position
-    DualIn in[RPT];
-    float uu[RPT], iu[RPT], iz[RPT], tpl[RPT], tpu[RPT];
-    bool hasL[RPT], hasU[RPT];
-#pragma unroll
-    for (int r = 0; r < RPT; r++) {
-        const int y = y0 + threadIdx.y + BY * r;
-        dual_load<MASKED, T>(in[r], x, y, w, h, s, mask, u_, v_, at, bt, ct, p_in, q_in, ip.first);
-        const long pos = in[r].pos;
-        uu[r] = u[pos];
-        iu[r] = ldT(Iu, pos);
-        iz[r] = ldT(Iz, pos);
-        const int xc = min(x, w - 1), yc = min(y, h - 1);
-        if (MASKED) {
-            hasL[r] = xc >= 1 && mask[xc >= 1 ? pos - 1 : pos] != 0.f;
-            hasU[r] = yc >= 1 && mask[yc >= 1 ? pos - s : pos] != 0.f;
-        } else {
-            hasL[r] = xc >= 1;
-            hasU[r] = yc >= 1;
-        }
-        // Tp-plane values of the left / upper neighbour: read only where that neighbour
-        // is outside the mask or the image; fetched here on latency-bound levels
-        tpl[r] = 0.f;
-        tpu[r] = 0.f;
-        if (ip.preload_tp) {
-            tpl[r] = ld0(Tp, pos - 1).x;
-            tpu[r] = ld0(Tp, pos - s).y;
-        }
-    }
-    DualIn hin;
-    int hx = 0, hy = 0, hex = 0, hey = 0;
-    const bool has_halo = tid < HALO;
-    if (has_halo) {
-        if (tid < EXW) {
-            hex = tid;
-            hey = 0;
-        } else {
-            hex = 0;
-            hey = tid - EXW + 1;
-        }
-        hx = x0 - 1 + hex;
-        hy = y0 - 1 + hey;
-        dual_load<MASKED, T>(hin, hx, hy, w, h, s, mask, u_, v_, at, bt, ct, p_in, q_in, ip.first);
-    }
-
-    // ---- phase A: new duals + Tp into LDS (and the tile's to global memory)
-#pragma unroll
-    for (int r = 0; r < RPT; r++) {
-        const int y = y0 + threadIdx.y + BY * r;
-        if (!in[r].act) continue;
-        float2 pn, tpn;
-        float4 qn;
-        dual_step<MASKED>(in[r], x, y, w, h, ip, pn, qn, tpn);
-        const int ex = threadIdx.x + 1, ey = threadIdx.y + BY * r + 1;
-        sP[ey][ex] = pn;
-        sQ[ey][ex] = qn;
-        sT[ey][ex] = tpn;
-        st2T(p_out, in[r].pos, pn);
-        st4T(q_out, in[r].pos, qn);
-        if (ip.write_tp) Tp[in[r].pos] = tpn;
-    }
-    if (has_halo && hin.act) {
-        float2 pn, tpn;
-        float4 qn;
-        dual_step<MASKED>(hin, hx, hy, w, h, ip, pn, qn, tpn);
-        sP[hey][hex] = pn;
-        sQ[hey][hex] = qn;
-        sT[hey][hex] = tpn;
-    }
-    __syncthreads();
-
-    // ---- phase B: thresholding + primal step on the tile
-#pragma unroll
-    for (int r = 0; r < RPT; r++) {
-        const DualIn &q = in[r];
-        const int y = y0 + threadIdx.y + BY * r;
-        if (!q.ok) continue;
-        const long pos = q.pos;
-        if (!MASKED && outside_circle(x, y, w, h)) {
-            us[pos] = 0.f;
-            u_s[pos] = 0.f;
-            v_s[pos] = make_float2(0.f, 0.f);
-            continue;
-        }
-        if (!q.act) continue;
-        const bool hR = q.hasR, hD = q.hasD, hL = hasL[r], hU = hasU[r];
-        const int ex = threadIdx.x + 1, ey = threadIdx.y + BY * r + 1;
-        const float a = q.a, b = q.b, c = q.c;
-        const float2 tp0 = sT[ey][ex];
-        // a neighbour outside the mask (or the image) is read from the Tp plane, which
-        // holds what earlier levels left there, as the reference does
-        const float tpLx = hL ? sT[ey][ex - 1].x : (ip.preload_tp ? tpl[r] : ld0(Tp, pos - 1).x);
-        const float tpUy = hU ? sT[ey - 1][ex].y : (ip.preload_tp ? tpu[r] : ld0(Tp, pos - s).y);
-        const float divp = div_tp<MASKED>(x, y, w, h, hR, hD, hL, hU, tp0, tpLx, tpUy);
-        const float4 ql = hL ? sQ[ey][ex - 1] : make_float4(0.f, 0.f, 0.f, 0.f);
-        const float4 qu = hU ? sQ[ey - 1][ex] : make_float4(0.f, 0.f, 0.f, 0.f);
-        float un, u_s_v;
-        float2 v_s_v;
-        primal_step(ip, a, b, c, q.u0, uu[r], iu[r], iz[r], q.v0, hR, hD, hL, hU, divp, sQ[ey][ex], ql, qu,
-                    sP[ey][ex], un, u_s_v, v_s_v);
-        us[pos] = un;
-        u_s[pos] = u_s_v;
-        v_s[pos] = v_s_v;
-    }
-}
-
-// ------------------------------------------------------------------ K iterations per op
-//
-// Temporal blocking of the inner iteration (OP_ITER2, OP_ITER4, OP_ITER8): one workgroup advances
-// its tile by K iterations.  Iteration j (r = K - 1 - j iterations after it) computes the
-// duals over the tile grown by 2r + 1 pixels to the left / top and r to the right /
-// bottom, and the primal step over the tile grown by 2r / r: the primal step of a pixel
-// reads the new q and Tp of its left and upper neighbours, the dual step the state of all
-// four neighbours.  The workgroup stages the state over the tile grown by 2K / K pixels
-// (every global load issued before the first LDS store): each work-item keeps what only
-// its own positions read in registers, and what neighbours read in LDS; the K iterations
-// then run in place.  Every pixel goes through the same dual_step / primal_step code as
-// in op_iter, and p and q are rounded to the storage type between iterations as the
-// planes round them, so the result equals K single iterations bit for bit.
-// Buffers (K even): iteration K - 2 stores u, u_ and v_ to the `other` buffers (as a
-// single iteration's would), iteration K - 1 its whole state to the spare buffers
-// together with, outside the mask, the current buffers' values (never written by an
-// iteration); the host then
-// rotates the roles (spare -> current).  Out of the mask, iteration j reads the current
-// buffers' values for even j and the other buffers' values for odd j, as the reference's
-// two ping-pong buffers give them.  Other workgroups of the launch read the current
-// buffers and, outside the mask only, the other buffers, so no write of this op is read
-// by it.  The spare buffers also take the current buffers' values in the row padding
-// (columns w .. s - 1, written by the tiles of the last tile column), so that the new
-// current buffers equal the old ones wherever no iteration writes: coarser levels'
-// values there are read again at finer levels.  The host uses this op only where no
-// mask pixel lies in column 0: there the reference's flat-index left neighbour (pos - 1)
-// is the previous row's last pixel.
-// The step sizes of iteration j come from the pair's schedule table at index it0 + j.
-// Work-items: see BlkK::NT.
-template <int SH, int K, bool HALF>
-struct BlkK {
-    static constexpr int TW = kShape[SH].tw, TH = kShape[SH].th;
-    static constexpr int RO = -2 * K, RW = TW + 3 * K, RH = TH + 3 * K;  // staged region
-    static constexpr int NR = RW * RH;
-    // work-items (half_workgroup_ok)
-    static constexpr int NT = HALF ? 512 : NR >= 1024 ? 1024 : (NR + 63) / 64 * 64;
-    static_assert(!HALF || half_workgroup_ok(SH, K), "half workgroup");
-    static constexpr int NK = (NR + NT - 1) / NT;  // staged positions per work-item
-    static_assert(multi_iter_shape(SH), "multi-iteration tile shape");
-};
-
-// Values that neighbours read: u_ and v_ (inside the mask the iteration's state; outside
-// it the value of the buffer the reference reads at that step), q, Tp (inside the mask
-// the iteration's new Tp, outside it the Tp plane) and the mask.
-template <int SH, int K>
-struct LdsK {
-    using B = BlkK<SH, K, false>;
-    float ub[B::NR];
-    float2 v[B::NR];
-    float4 q[B::NR];
-    float2 tp[B::NR];
-    unsigned char mk[B::NR];
-};
-
-// A work-item's own staged position: everything only that position reads.
-struct OwnK {
-    float a, b, c, iu, iz;
-    float u, ub;        // u and u_ (current buffers, then each iteration's)
-    float2 v;           // v_
-    float2 p;           // p: the last dual step's value (q: in LDS)
-    float ubo;          // outside the mask: u_ and v_ of the other buffers
-    float2 vo;
-};
-
-// Rounds a dual value to the storage type: the planes hold p and q in it, and every
-// iteration's dual step reads them from there (its primal step uses the unrounded values).
-template <typename T>
-__device__ __forceinline__ float rT(float v)
-{
-    return Cv<T>::f(Cv<T>::t(v));
-}
-
-// The step sizes of iteration j: sched[j] = (sigma, tau, mu, alpha1 sigma / 3); `ip0` holds the
-// schedule-independent fields and the flags of the op.
-template <int SH, int K, bool HALF, bool MASKED, typename T>
-__device__ __forceinline__ void op_iterk(const Tile &t, int roles, const float4 *__restrict__ sched,
-                                         const IterParams &ip0, LdsK<SH, K> &L)
-{
-    static_assert(K % 2 == 0, "K-iteration ops rotate the roles like two iterations");
-    using B = BlkK<SH, K, HALF>;
-    constexpr int NT = B::NT, NK = B::NK;
-    const int w = t.w, h = t.h, s = t.s, x0 = t.x0, y0 = t.y0;
-    const int tid = threadIdx.x;
-    const float *__restrict__ mask = t.mask;
-    const float *__restrict__ u_cur = slot(t, slot_u(rcur(roles, G_U)));
-    float *__restrict__ u_oth = slot(t, slot_u(roth(roles, G_U)));
-    float *__restrict__ u_spr = slot(t, slot_u(rspare(roles, G_U)));
-    const float *__restrict__ ub_cur = slot(t, slot_ub(rcur(roles, G_UB)));
-    float *__restrict__ ub_oth = slot(t, slot_ub(roth(roles, G_UB)));
-    float *__restrict__ ub_spr = slot(t, slot_ub(rspare(roles, G_UB)));
-    const float2 *__restrict__ v_cur = slotp<float2>(t, slot_v(rcur(roles, G_V)));
-    float2 *__restrict__ v_oth = slotp<float2>(t, slot_v(roth(roles, G_V)));
-    float2 *__restrict__ v_spr = slotp<float2>(t, slot_v(rspare(roles, G_V)));
-    const V2<T> *__restrict__ p_cur = slotp<V2<T>>(t, slot_p(rcur(roles, G_PQ)));
-    V2<T> *__restrict__ p_spr = slotp<V2<T>>(t, slot_p(rspare(roles, G_PQ)));
-    const V4<T> *__restrict__ q_cur = slotp<V4<T>>(t, slot_q(rcur(roles, G_PQ)));
-    V4<T> *__restrict__ q_spr = slotp<V4<T>>(t, slot_q(rspare(roles, G_PQ)));
-    float2 *__restrict__ Tp = slotp<float2>(t, PL_TP);
-    const T *__restrict__ at = (const T *)slot(t, PL_A);
-    const T *__restrict__ bt = (const T *)slot(t, PL_B);
-    const T *__restrict__ ct = (const T *)slot(t, PL_C);
-    const T *__restrict__ Iu = (const T *)slot(t, PL_IU);
-    const T *__restrict__ Iz = (const T *)slot(t, PL_IZ);
-    // The duals of the `other` buffers are never read (an op reads the current ones and
-    // each warp starts from zero): the op stores p and q after its last iteration only.
-
-    // ---- stage: own values into registers, shared ones into LDS (clamped positions
-    // outside the image); every load is issued before the first LDS store
-    OwnK o[NK];
-    int ox[NK], oy[NK];
-    float omk[NK];
-    float2 otp[NK];
-    float4 oq[NK];
-#pragma unroll
-    for (int k = 0; k < NK; k++) {
-        const int i = tid + k * NT;
-        ox[k] = x0 + B::RO + i % B::RW;
-        oy[k] = y0 + B::RO + i / B::RW;
-        if (i < B::NR) {
-            const int xc = min(max(ox[k], 0), w - 1), yc = min(max(oy[k], 0), h - 1);
-            const long pos = (long)yc * s + xc;
-            omk[k] = MASKED ? mask[pos] : 1.f;
-            o[k].ub = ub_cur[pos];
-            o[k].v = v_cur[pos];
-            o[k].u = u_cur[pos];
-            o[k].a = ldT(at, pos);
-            o[k].b = ldT(bt, pos);
-            o[k].c = ldT(ct, pos);
-            o[k].iu = ldT(Iu, pos);
-            o[k].iz = ldT(Iz, pos);
-            if (ip0.first) {
-                o[k].p = make_float2(0.f, 0.f);
-                oq[k] = make_float4(0.f, 0.f, 0.f, 0.f);
-            } else {
-                o[k].p = ld2T(p_cur, pos);
-                oq[k] = ld4T(q_cur, pos);
-            }
-        }
-    }
-    // outside the mask: the other buffers' u_, v_ and the Tp plane (read by neighbours)
-#pragma unroll
-    for (int k = 0; k < NK; k++) {
-        const int i = tid + k * NT;
-        o[k].ubo = 0.f;
-        o[k].vo = make_float2(0.f, 0.f);
-        otp[k] = make_float2(0.f, 0.f);
-        if (MASKED && i < B::NR && omk[k] == 0.f) {
-            const int xc = min(max(ox[k], 0), w - 1), yc = min(max(oy[k], 0), h - 1);
-            const long pos = (long)yc * s + xc;
-            o[k].ubo = ub_oth[pos];
-            o[k].vo = v_oth[pos];
-            otp[k] = Tp[pos];
-        }
-    }
-#pragma unroll
-    for (int k = 0; k < NK; k++) {
-        const int i = tid + k * NT;
-        if (i < B::NR) {
-            L.ub[i] = o[k].ub;
-            L.v[i] = o[k].v;
-            L.q[i] = oq[k];
-            L.tp[i] = otp[k];
-            L.mk[i] = omk[k] != 0.f ? 1 : 0;
-        }
-    }
-    // row padding of the last tile column: the spare buffers take the current values
-    if (x0 + B::TW >= w && s > w) {
-        const int pw = s - w, rows = min(B::TH, h - y0);
-        for (int i = tid; i < pw * rows; i += NT) {
-            const long pos = (long)(y0 + i / pw) * s + w + i % pw;
-            u_spr[pos] = u_cur[pos];
-            ub_spr[pos] = ub_cur[pos];
-            v_spr[pos] = v_cur[pos];
-        }
-    }
-    // A workgroup whose staged region lies inside the image and (masked form) inside the
-    // mask, or (unmasked form) inside the circle, takes every neighbour branch the same
-    // way: it runs the iterations without the boundary tests (same arithmetic).
-    bool inner = true;
-#pragma unroll
-    for (int k = 0; k < NK; k++) {
-        const int i = tid + k * NT, x = ox[k], y = oy[k];
-        if (i < B::NR)
-            inner = inner && x >= 0 && y >= 0 && x < w && y < h && omk[k] != 0.f &&
-                    (MASKED || !outside_circle(x, y, w, h));
-    }
-    const bool full = __syncthreads_and(inner) != 0;
-    // the primal step's per-pixel constants, once for the K iterations
-    PrimalCoef kc[NK];
-#pragma unroll
-    for (int k = 0; k < NK; k++) kc[k] = primal_coef(ip0, o[k].a, o[k].b, o[k].c, o[k].iu);
-
-    auto act = [&](int k) { return !MASKED || L.mk[k] != 0; };
-    auto in_tile = [&](int x, int y) { return x >= x0 && x < x0 + B::TW && y >= y0 && y < y0 + B::TH; };
-    // Per own position, once per launch (the mask does not change): in the image, in the
-    // mask, and which of the four neighbours are in the image and the mask (read by the
-    // path with boundary tests only).
-    enum : unsigned { F_IN = 1, F_ACT = 2, F_R = 4, F_D = 8, F_L = 16, F_U = 32 };
-    unsigned fl[NK];
-#pragma unroll
-    for (int k = 0; k < NK; k++) {
-        const int i = tid + k * NT, x = ox[k], y = oy[k];
-        fl[k] = 0;
-        if (!full && i < B::NR && x >= 0 && y >= 0 && x < w && y < h) {
-            // neighbours outside the staged region are never asked for
-            const int col = i % B::RW, row = i / B::RW;
-            fl[k] = F_IN | (act(i) ? F_ACT : 0u);
-            if (x + 1 < w && col + 1 < B::RW && act(i + 1)) fl[k] |= F_R;
-            if (y + 1 < h && row + 1 < B::RH && act(i + B::RW)) fl[k] |= F_D;
-            if (x >= 1 && col >= 1 && act(i - 1)) fl[k] |= F_L;
-            if (y >= 1 && row >= 1 && act(i - B::RW)) fl[k] |= F_U;
-        }
-    }
-
-    auto iterate = [&](auto full_c) {
-        constexpr bool FULL = decltype(full_c)::value;
-        // unrolled: each iteration's regions and stores are known at compile time (fewer
-        // registers and tests than a loop over j)
-#pragma unroll
-        for (int j = 0; j < K; j++) {
-            IterParams ip = ip0;
-            const float4 sc = sched[j];
-            ip.k1 = sc.w;  // alpha1 sigma / 3, as op_iter forms it
-            ip.k0 = ip0.alpha0 * sc.x / 2.0f;
-            ip.tau = sc.y;
-            ip.mu = sc.z;
-            ip.first = j == 0 && ip0.first;
-            ip.write_tp = j == K - 1 && ip0.write_tp;
-            const int r = K - 1 - j;
-
-            // ---- duals over the tile grown by 2r + 1 (left / top) and r (right / bottom)
-#pragma unroll
-            for (int k = 0; k < NK; k++) {
-                const int i = tid + k * NT, x = ox[k], y = oy[k];
-                if (i >= B::NR) continue;
-                if (!FULL && !(fl[k] & F_ACT && fl[k] & F_IN)) continue;
-                if (x < x0 - 2 * r - 1 || y < y0 - 2 * r - 1 || x > x0 + B::TW - 1 + r || y > y0 + B::TH - 1 + r)
-                    continue;
-                const int kR = (FULL || x + 1 < w) ? i + 1 : i, kD = (FULL || y + 1 < h) ? i + B::RW : i;
-                DualIn rr;
-                rr.ok = true;
-                rr.act = true;
-                rr.hasR = FULL || (fl[k] & F_R) != 0;
-                rr.hasD = FULL || (fl[k] & F_D) != 0;
-                rr.a = o[k].a;
-                rr.b = o[k].b;
-                rr.c = o[k].c;
-                rr.u0 = o[k].ub;
-                rr.uR = L.ub[kR];
-                rr.uD = L.ub[kD];
-                // backward differences (read where the right / lower neighbour is outside
-                // the mask; column 0 inside the mask is not run here)
-                rr.uL = (!FULL && x >= 1) ? L.ub[i - 1] : 0.f;
-                rr.uU = (!FULL && y >= 1) ? L.ub[i - B::RW] : 0.f;
-                rr.v0 = o[k].v;
-                rr.vR = L.v[kR];
-                rr.vD = L.v[kD];
-                rr.vL = (!FULL && x >= 1) ? L.v[i - 1] : make_float2(0.f, 0.f);
-                rr.vU = (!FULL && y >= 1) ? L.v[i - B::RW] : make_float2(0.f, 0.f);
-                // the previous iteration's duals as the planes hold them
-                rr.po = make_float2(rT<T>(o[k].p.x), rT<T>(o[k].p.y));
-                const float4 qo = L.q[i];
-                rr.qo = make_float4(rT<T>(qo.x), rT<T>(qo.y), rT<T>(qo.z), rT<T>(qo.w));
-                float2 pn, tpn;
-                float4 qn;
-                dual_step<MASKED>(rr, x, y, w, h, ip, pn, qn, tpn);
-                o[k].p = pn;  // the primal step of this iteration reads them unrounded
-                L.q[i] = qn;
-                L.tp[i] = tpn;
-                if (r == 0 && in_tile(x, y)) {
-                    const long pos = (long)y * s + x;
-                    st2T(p_spr, pos, pn);
-                    st4T(q_spr, pos, qn);
-                    if (ip.write_tp) Tp[pos] = tpn;
-                }
-            }
-            __syncthreads();
-
-            // ---- thresholding + primal step over the tile grown by 2r (left / top) and
-            // r; outside the mask, the stored u_ and v_ switch to the buffer the next
-            // iteration reads there
-#pragma unroll
-            for (int k = 0; k < NK; k++) {
-                const int i = tid + k * NT, x = ox[k], y = oy[k];
-                if (i >= B::NR) continue;
-                if (!FULL && !(fl[k] & F_IN)) continue;
-                const long pos = (long)y * s + x;
-                if (!FULL && MASKED && !(fl[k] & F_ACT)) {
-                    if (r == 0) {
-                        // outside the mask: the spare buffers take the current buffers' values
-                        if (in_tile(x, y)) {
-                            u_spr[pos] = o[k].u;
-                            ub_spr[pos] = o[k].ub;
-                            v_spr[pos] = o[k].v;
-                        }
-                    } else {
-                        L.ub[i] = (j & 1) ? o[k].ub : o[k].ubo;
-                        L.v[i] = (j & 1) ? o[k].v : o[k].vo;
-                    }
-                    continue;
-                }
-                if (x < x0 - 2 * r || y < y0 - 2 * r || x > x0 + B::TW - 1 + r || y > y0 + B::TH - 1 + r) continue;
-                float un, u_s;
-                float2 vs;
-                if (!FULL && !MASKED && outside_circle(x, y, w, h)) {
-                    un = 0.f;
-                    u_s = 0.f;
-                    vs = make_float2(0.f, 0.f);
-                } else {
-                    const bool hR = FULL || (fl[k] & F_R) != 0, hD = FULL || (fl[k] & F_D) != 0;
-                    const bool hL = FULL || (fl[k] & F_L) != 0, hU = FULL || (fl[k] & F_U) != 0;
-                    // the left / upper neighbour's Tp: this iteration's inside the mask,
-                    // the Tp plane's outside it
-                    const float tpLx = (FULL || x >= 1) ? L.tp[i - 1].x : 0.f;
-                    const float tpUy = (FULL || y >= 1) ? L.tp[i - B::RW].y : 0.f;
-                    const float divp = div_tp<MASKED>(x, y, w, h, hR, hD, hL, hU, L.tp[i], tpLx, tpUy);
-                    const float4 ql = hL ? L.q[i - 1] : make_float4(0.f, 0.f, 0.f, 0.f);
-                    const float4 qu = hU ? L.q[i - B::RW] : make_float4(0.f, 0.f, 0.f, 0.f);
-                    primal_core(ip, kc[k], o[k].a, o[k].b, o[k].c, o[k].ub, o[k].u, o[k].iu, o[k].iz, o[k].v, hR, hD, hL,
-                                hU, divp, L.q[i], ql, qu, o[k].p, un, u_s, vs);
-                }
-                o[k].u = un;
-                o[k].ub = u_s;
-                o[k].v = vs;
-                L.ub[i] = u_s;
-                L.v[i] = vs;
-                if (r == 1 && in_tile(x, y)) {
-                    u_oth[pos] = un;
-                    ub_oth[pos] = u_s;
-                    v_oth[pos] = vs;
-                } else if (r == 0) {
-                    u_spr[pos] = un;
-                    ub_spr[pos] = u_s;
-                    v_spr[pos] = vs;
-                }
-            }
-            __syncthreads();
-        }
-    };
-    if (full) iterate(std::true_type{});
-    else iterate(std::false_type{});
-}
-
-// Half workgroups (512 work-items, two or three positions each) are held to 128 registers, so that
-// two of them share a compute unit.
-template <typename T, int SH, int K, bool HALF>
-__global__ __launch_bounds__((BlkK<SH, K, HALF>::NT)) __attribute__((amdgpu_waves_per_eu(HALF ? 4 : 1)))
-void k_iterk(const PairDesc *__restrict__ D, const TickArgs A)
-{
-    using B = BlkK<SH, K, HALF>;
-    __shared__ LdsK<SH, K> L;
-    const int nb = gridDim.x, i = blockIdx.x, xcd = i & 7, q = nb >> 3, r = nb & 7;
-    const int b = (xcd < r ? xcd * (q + 1) : r * (q + 1) + (xcd - r) * q) + (i >> 3);
-    int lo = 0;
-    if (A.n > 1) {
-        const int lane = __lane_id();
-        const int ts = lane < A.n ? A.e[lane].tile_start : 0x7fffffff;
-        lo = __popcll(__ballot(ts <= b)) - 1;
-        lo = __builtin_amdgcn_readfirstlane(lo);
-    }
-    const TickEntry &E = A.e[lo];
-    Tile t;
-    t.d = D + E.pair;
-    t.arena = E.arena;
-    t.plane = E.plane;
-    t.mask = E.mask;
-    t.w = E.w;
-    t.h = E.h;
-    t.s = E.s;
-    const int local = b - E.tile_start;
-    const int ntx = (t.w + B::TW - 1) / B::TW;
-    t.x0 = (local % ntx) * B::TW;
-    t.y0 = (local / ntx) * B::TH;
-    // step sizes of iterations it0 .. it0 + K - 1 of the warp (stereotgv.cu:449-477)
-    const float4 *sched = t.d->sched + E.it0;
-    IterParams ip0;
-    ip0.k1 = ip0.k0 = ip0.tau = ip0.mu = 0.f;  // set per iteration
-    ip0.lambda = A.lambda;
-    ip0.alpha0 = A.alpha0;
-    ip0.alpha1 = A.alpha1;
-    ip0.first = (E.flags & EF_FIRST) != 0;
-    ip0.write_tp = (E.flags & EF_WRITE_TP_LAST) != 0;  // the op's last iteration
-    ip0.preload_tp = 0;
-    if (E.masked) op_iterk<SH, K, HALF, true, T>(t, E.roles, sched, ip0, L);
-    else op_iterk<SH, K, HALF, false, T>(t, E.roles, sched, ip0, L);
-}
-
-// 13th smallest of 25 by the partial selection sort of medianFilter.cu:307-316.
-__device__ __forceinline__ float median25(float m[25])
-{
-#pragma unroll
-    for (int j = 0; j < 13; j++) {
-#pragma unroll
-        for (int i = j + 1; i < 25; i++) {
-            const float a = m[j], b = m[i];
-            const bool sw = a > b;
-            m[j] = sw ? b : a;
-            m[i] = sw ? a : b;
-        }
-    }
-    return m[12];
-}
-
-// 5x5 window as medianFilter.cu:282-304: edge replicate when exactly one of row / column
-// is outside the image, 0 when both are.
-__device__ __forceinline__ float median5_at(const float *__restrict__ u, int x, int y, int w, int h, int s)
-{
-    float m[25];
-#pragma unroll
-    for (int j = 0; j < 5; j++) {
-        const int row = y + j - 2;
-        const bool rin = row >= 0 && row < h;
-#pragma unroll
-        for (int i = 0; i < 5; i++) {
-            const int col = x + i - 2;
-            const bool cin = col >= 0 && col < w;
-            float v = 0.f;
-            if (rin || cin) {
-                const int rr = rin ? row : (row < 0 ? 0 : h - 1);
-                const int cc = cin ? col : (col < 0 ? 0 : w - 1);
-                v = u[(size_t)rr * s + cc];
-            }
-            m[j * 5 + i] = v;
-        }
-    }
-    return median25(m);
-}
-
-// End of a warp (stereotgv.cu:481-493 masked, :354-363 unmasked): [median of u
-// (medianFilter.cu:264-319) into the other u slot,] du = min(u - u_last, limit)
-// (scalar.cu:129-141, limitRange.cu:3-24), u = u_last + du (scalar.cu:77-89),
-// u_ = u (copyClone.cu:3-12), warpUV += du * tv2 (warpImage.cu:136-150, scalar.cu:102-115).
-template <class S, bool MASKED>
-__device__ __forceinline__ void op_end(const Tile &t, int roles, float limit)
-{
-    const float *u = slot(t, slot_u(rcur(roles, G_U)));
-    float *out = MASKED ? slot(t, slot_u(roth(roles, G_U))) : (float *)u;
-    float *u_ = slot(t, slot_ub(rcur(roles, G_UB)));
-    float2 *uv = slotp<float2>(t, slot_uv(roles, true));
-    const float *ulast = slot(t, PL_ULAST);
-    const float2 *tv2 = slotp<float2>(t, PL_TV2);
-    FOR_TILE_PIXELS(S, t, x, y)
-    {
-        const long pos = (long)y * t.s + x;
-        const float uval = MASKED ? median5_at(u, x, y, t.w, t.h, t.s) : u[pos];
-        const float ul = ulast[pos];
-        float du = uval - ul;
-        du = (du > limit) ? limit : du;
-        const float un = ul + du;
-        out[pos] = un;
-        u_[pos] = un;
-        const float2 tt = tv2[pos];
-        const float2 o = uv[pos];
-        uv[pos] = make_float2(o.x + du * tt.x, o.y + du * tt.y);
-    }
-}
-
-// The end of one warp and the preparation of the next at the same level in one op: each
-// pixel's end-of-warp values (op_end) are all the next warp's preparation (op_prep) reads
-// at that pixel, so the two run per pixel without a launch between them.  Roles as for
-// op_end; the preparation's u is the end's result.
-template <class S, bool MASKED, typename T>
-__device__ __forceinline__ void op_end_prep(const Tile &t, int level, int roles, float limit)
-{
-    const float *u = slot(t, slot_u(rcur(roles, G_U)));
-    float *out = MASKED ? slot(t, slot_u(roth(roles, G_U))) : (float *)u;
-    float *u_ = slot(t, slot_ub(rcur(roles, G_UB)));
-    float2 *uv = slotp<float2>(t, slot_uv(roles, true));
-    float *ulast = slot(t, PL_ULAST);
-    float2 *tv2 = slotp<float2>(t, PL_TV2);
-    float2 *v_ = slotp<float2>(t, slot_v(rcur(roles, G_V)));
-    const float2 *tv = t.d->tv[level];
-    const T *I1 = (const T *)t.d->I1[level];
-    T *i1w = (T *)slot(t, PL_I1W);
-    FOR_TILE_PIXELS(S, t, x, y)
-    {
-        const long pos = (long)y * t.s + x;
-        // op_end
-        const float uval = MASKED ? median5_at(u, x, y, t.w, t.h, t.s) : u[pos];
-        const float ul = ulast[pos];
-        float du = uval - ul;
-        du = (du > limit) ? limit : du;
-        const float un = ul + du;
-        out[pos] = un;
-        u_[pos] = un;
-        const float2 tt = tv2[pos];
-        const float2 o = uv[pos];
-        const float2 d = make_float2(o.x + du * tt.x, o.y + du * tt.y);
-        uv[pos] = d;
-        // op_prep of the next warp
-        const float px = (float)x + d.x, py = (float)y + d.y;
-        tv2[pos] = bil2(tv, t.w, t.h, t.s, px, py);
-        stT(i1w, pos, bil(I1, t.w, t.h, t.s, px, py));
-        ulast[pos] = un;
-        v_[pos] = make_float2(0.f, 0.f);
-    }
-}
-
-// upscale.cu:7-21, 46-61 for u, u_ and warpUV (stereotgv.cu:497-510): bilinear at the
-// finer level's pixel centres, times fScale; writes the other slot of each pair.
-template <class S>
-__device__ __forceinline__ void op_up(const Tile &t, int level, int roles, float scale)
-{  // the tile belongs to the finer level level - 1
-    const int w = t.d->w[level], h = t.d->h[level], s = t.d->s[level];
-    const float *u = slot(t, slot_u(rcur(roles, G_U)));
-    float *ou = slot(t, slot_u(roth(roles, G_U)));
-    const float *u_ = slot(t, slot_ub(rcur(roles, G_UB)));
-    float *ou_ = slot(t, slot_ub(roth(roles, G_UB)));
-    const float2 *uv = slotp<float2>(t, slot_uv(roles, true));
-    float2 *ouv = slotp<float2>(t, slot_uv(roles, false));
-    const float rx = (float)w / (float)t.w, ry = (float)h / (float)t.h;
-    FOR_TILE_PIXELS(S, t, x, y)
-    {
-        const float px = ((float)x + 0.5f) * rx - 0.5f, py = ((float)y + 0.5f) * ry - 0.5f;
-        const long pos = (long)y * t.s + x;
-        ou[pos] = bil(u, w, h, s, px, py) * scale;
-        ou_[pos] = bil(u_, w, h, s, px, py) * scale;
-        const float2 v = bil2(uv, w, h, s, px, py);
-        ouv[pos] = make_float2(v.x * scale, v.y * scale);
-    }
-}
-
-
-// ------------------------------------------------------------------ the tick kernel
-//
-// One launch executes one op for each entry of `A`: entry i owns workgroups
-// [A.e[i].tile_start, A.e[i+1].tile_start).  A batched solve passes the same op for every
-// pair; the serving loop passes each active request's next op, so requests at different
-// points of their schedules share launches.  Each pair's result depends only on its own
-// entries, never on what runs beside it.
-template <typename T, int SH>
-__global__ __launch_bounds__(NT) void k_tick(const PairDesc *__restrict__ D, const TickArgs A)
-{
-    using S = Sh<SH>;
-    __shared__ float2 sP[S::TH + 1][S::TW + 1];
-    __shared__ float4 sQ[S::TH + 1][S::TW + 1];
-    __shared__ float2 sT[S::TH + 1][S::TW + 1];
-    // Workgroups are dealt to the 8 XCDs round-robin; renumber them so that each XCD
-    // owns one contiguous run of tiles (whole bands of rows), keeping the halo rows that
-    // neighbouring tiles share in the same L2.
-    const int nb = gridDim.x, i = blockIdx.x, xcd = i & 7, q = nb >> 3, r = nb & 7;
-    const int b = (xcd < r ? xcd * (q + 1) : r * (q + 1) + (xcd - r) * q) + (i >> 3);
-    // Entry of this workgroup: the last one whose first tile is <= b.  Each lane of the
-    // wave reads one entry's tile_start (kMaxEntries <= 64), so the lookup costs one load.
-    int lo = 0;
-    if (A.n > 1) {
-        const int lane = __lane_id();
-        const int ts = lane < A.n ? A.e[lane].tile_start : 0x7fffffff;
-        lo = __popcll(__ballot(ts <= b)) - 1;
-        lo = __builtin_amdgcn_readfirstlane(lo);
-    }
-    const TickEntry &E = A.e[lo];
-    const int kind = E.kind, level = E.level;
-    Tile t;
-    t.d = D + E.pair;
-    t.arena = E.arena;
-    t.plane = E.plane;
-    t.mask = E.mask;
-    t.w = E.w;
-    t.h = E.h;
-    t.s = E.s;
-    const int local = b - E.tile_start;
-    const int ntx = (t.w + S::TW - 1) / S::TW;
-    t.x0 = (local % ntx) * S::TW;
-    t.y0 = (local / ntx) * S::TH;
-    switch (kind) {
-    case OP_ZERO:
-        op_zero<S>(t);
-        break;
-    case OP_CALIB:
-        op_calib<S, T>(t);
-        break;
-    case OP_DOWN:
-        op_down<S, T>(t, level);
-        break;
-    case OP_CINIT:
-        op_cinit<S>(t, level, E.roles);
-        break;
-    case OP_WINIT:
-        op_winit<S>(t, level, E.roles, A.scale);
-        break;
-    case OP_TENSOR:
-        if (E.masked) op_tensor<S, T, true>(t, level, A.xblock, A.beta, A.gamma);
-        else op_tensor<S, T, false>(t, level, A.xblock, A.beta, A.gamma);
-        break;
-    case OP_PREP:
-        op_prep<S, T>(t, level, E.roles);
-        break;
-    case OP_DERIV:
-        op_deriv<S, T>(t, level);
-        break;
-    case OP_ITER: {
-        IterParams ip;
-        ip.k1 = A.alpha1 * E.sigma / 3.0f;  // eta_p = 3 (stereotgv.cu:416)
-        ip.k0 = A.alpha0 * E.sigma / 2.0f;  // eta_q = 2 (stereotgv.cu:417)
-        ip.tau = E.tau;
-        ip.mu = E.mu;
-        ip.lambda = A.lambda;
-        ip.alpha0 = A.alpha0;
-        ip.alpha1 = A.alpha1;
-        ip.first = E.flags & EF_FIRST;
-        ip.write_tp = (E.flags & EF_WRITE_TP) != 0;
-        ip.preload_tp = (E.flags & EF_PRELOAD_TP) != 0;
-        if (E.masked) op_iter<S, true, T>(t, E.roles, ip, sP, sQ, sT);
-        else op_iter<S, false, T>(t, E.roles, ip, sP, sQ, sT);
-        break;
-    }
-    case OP_END:
-        if (E.masked) op_end<S, true>(t, E.roles, A.limit);
-        else op_end<S, false>(t, E.roles, A.limit);
-        break;
-    case OP_ENDPREP:
-        if (E.masked) op_end_prep<S, true, T>(t, level, E.roles, A.limit);
-        else op_end_prep<S, false, T>(t, level, E.roles, A.limit);
-        break;
-    case OP_UP:
-        op_up<S>(t, level, E.roles, A.fscale);
-        break;
-    case OP_SYNC:
-        op_sync<S>(t, level, E.roles);
-        break;
-    default:
-        break;
-    }
-}
 
 // ------------------------------------------------------------------ single-plane kernels
 
@@ -1688,68 +228,6 @@ __global__ void k_widen(const T *__restrict__ src, int nch, float *__restrict__ 
         }                                                       \
     } while (0)
 
-template <typename T, int SH>
-void tick_shape(const PairDesc *desc, const TickArgs &args, hipStream_t st)
-{
-    k_tick<T, SH><<<dim3((unsigned)args.total), dim3(Sh<SH>::BX, Sh<SH>::BY), 0, st>>>(desc, args);
-}
-
-template <typename T>
-void tick_storage(const PairDesc *desc, const TickArgs &args, int shape, hipStream_t st)
-{
-    switch (shape) {
-    case 1: tick_shape<T, 1>(desc, args, st); break;
-    case 2: tick_shape<T, 2>(desc, args, st); break;
-    case 3: tick_shape<T, 3>(desc, args, st); break;
-    case 4: tick_shape<T, 4>(desc, args, st); break;
-    case 5: tick_shape<T, 5>(desc, args, st); break;
-    case 6: tick_shape<T, 6>(desc, args, st); break;
-    default: tick_shape<T, 0>(desc, args, st); break;
-    }
-}
-static_assert(kTickShapes == 7, "tick_storage dispatches every tick shape");
-
-// K-iteration launches exist for the shapes of multi_iter_shape() (64x4, 32x16, 32x8,
-// 16x16, 8x8, 4x4, 32x32, 20x20), whose staged regions fit the workgroup's LDS.
-template <typename T, int SH, int K>
-void iterk_shape(const PairDesc *desc, const TickArgs &args, bool half, hipStream_t st)
-{
-    if constexpr (iterk_ok(SH, K)) {  // the host asks for no other
-        const dim3 grid((unsigned)args.total);
-        if constexpr (half_workgroup_ok(SH, K)) {
-            if (half) {
-                k_iterk<T, SH, K, true><<<grid, dim3(BlkK<SH, K, true>::NT), 0, st>>>(desc, args);
-                return;
-            }
-        }
-        k_iterk<T, SH, K, false><<<grid, dim3(BlkK<SH, K, false>::NT), 0, st>>>(desc, args);
-    }
-}
-
-template <typename T, int K>
-void iterk_storage(const PairDesc *desc, const TickArgs &args, int launch_shape, hipStream_t st)
-{
-    const bool half = (launch_shape & kHalfWorkgroup) != 0;
-    switch (shape_index(launch_shape)) {
-    case 1: iterk_shape<T, 1, K>(desc, args, half, st); break;
-    case 3: iterk_shape<T, 3, K>(desc, args, half, st); break;
-    case 4: iterk_shape<T, 4, K>(desc, args, half, st); break;
-    case 6: iterk_shape<T, 6, K>(desc, args, half, st); break;
-    case 7: iterk_shape<T, 7, K>(desc, args, half, st); break;
-    case 8: iterk_shape<T, 8, K>(desc, args, half, st); break;
-    case 9: iterk_shape<T, 9, K>(desc, args, half, st); break;
-    case 10: iterk_shape<T, 10, K>(desc, args, half, st); break;
-    case 11: iterk_shape<T, 11, K>(desc, args, half, st); break;
-    case 12: iterk_shape<T, 12, K>(desc, args, half, st); break;
-    case 13: iterk_shape<T, 13, K>(desc, args, half, st); break;
-    case 14: iterk_shape<T, 14, K>(desc, args, half, st); break;
-    case 15: iterk_shape<T, 15, K>(desc, args, half, st); break;
-    case 16: iterk_shape<T, 16, K>(desc, args, half, st); break;
-    case 17: iterk_shape<T, 17, K>(desc, args, half, st); break;
-    default: break;  // the host never asks for another shape
-    }
-}
-
 hipError_t launch_tick(const PairDesc *desc, const TickArgs &args, int storage, int shape, hipStream_t st)
 {
     if (args.n < 1 || args.total < 1) return hipSuccess;
@@ -1761,19 +239,14 @@ hipError_t launch_tick(const PairDesc *desc, const TickArgs &args, int storage, 
     } else if (shape < 0 || shape >= kTickShapes) {
         return hipErrorInvalidValue;
     }
-    if (args.e[0].kind == OP_ITER2) {
-        STORAGE_DISPATCH(storage, (iterk_storage<T, 2>(desc, args, shape, st)));
-        return hipSuccess;
-    }
-    if (args.e[0].kind == OP_ITER4) {
-        STORAGE_DISPATCH(storage, (iterk_storage<T, 4>(desc, args, shape, st)));
-        return hipSuccess;
-    }
-    if (args.e[0].kind == OP_ITER8) {
-        STORAGE_DISPATCH(storage, (iterk_storage<T, 8>(desc, args, shape, st)));
-        return hipSuccess;
-    }
-    STORAGE_DISPATCH(storage, tick_storage<T>(desc, args, shape, st));
+    const int K = op_iterations(args.e[0].kind);
+    const int fam = op_multi(args.e[0].kind) ? (K == 2 ? 1 : K == 4 ? 2 : 3) : 0;
+    using Fn = void (*)(const PairDesc *, const TickArgs &, int, hipStream_t);
+    static constexpr Fn kFn[3][4] = {{launch_ops_f32_k1, launch_ops_f32_k2, launch_ops_f32_k4, launch_ops_f32_k8},
+                                     {launch_ops_f16_k1, launch_ops_f16_k2, launch_ops_f16_k4, launch_ops_f16_k8},
+                                     {launch_ops_bf16_k1, launch_ops_bf16_k2, launch_ops_bf16_k4, launch_ops_bf16_k8}};
+    const int si = storage == kF16 ? 1 : storage == kBF16 ? 2 : 0;
+    kFn[si][fam](desc, args, shape, st);
     return hipSuccess;
 }
 

@@ -1,0 +1,38 @@
+// One instantiation unit of the op kernels, compiled once per storage type and kernel
+// family (the library build passes VFS_INST_STORAGE = 0 float, 1 __half, 2
+// __hip_bfloat16, and VFS_INST_K = 1 for the tick kernel, 2, 4 or 8 for the K-iteration
+// kernel), so that the kernels compile in parallel.  Defines launch_ops_<storage>_k<K>,
+// which kernels.hip's launch_tick dispatches to.
+#include "tgv_device.h"
+
+#if !defined(VFS_INST_STORAGE) || !defined(VFS_INST_K)
+#error "build with -DVFS_INST_STORAGE=<0|1|2> -DVFS_INST_K=<1|2|4|8>"
+#endif
+
+#if VFS_INST_STORAGE == 0
+#define VFS_INST_T float
+#define VFS_INST_SFX f32
+#elif VFS_INST_STORAGE == 1
+#define VFS_INST_T __half
+#define VFS_INST_SFX f16
+#else
+#define VFS_INST_T __hip_bfloat16
+#define VFS_INST_SFX bf16
+#endif
+
+#define VFS_CAT3(a, b, c) a##b##c
+#define VFS_NAME(s, k) VFS_CAT3(launch_ops_, s, _k##k)
+#define VFS_NAME_X(s, k) VFS_NAME(s, k)
+
+namespace vfs {
+
+void VFS_NAME_X(VFS_INST_SFX, VFS_INST_K)(const PairDesc *desc, const TickArgs &args, int shape, hipStream_t st)
+{
+#if VFS_INST_K == 1
+    tick_storage<VFS_INST_T>(desc, args, shape, st);
+#else
+    iterk_storage<VFS_INST_T, VFS_INST_K>(desc, args, shape, st);
+#endif
+}
+
+}  // namespace vfs
