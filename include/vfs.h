@@ -29,6 +29,8 @@ extern "C" {
 #define VFS_ERR_STATE (-2)   /* call out of order (e.g. solve before initialize) */
 #define VFS_ERR_DEVICE (-3)  /* HIP runtime / kernel failure */
 #define VFS_ERR_NOMEM (-4)   /* device or host allocation failed */
+#define VFS_ERR_NUMERIC (-5) /* the solve produced non-finite values (the outputs are still
+                                copied); see vfs_get_energy */
 
 typedef struct vfs_solver vfs_solver;
 
@@ -92,6 +94,11 @@ int vfs_solve(vfs_solver *s);
  * solve of this handle; new images are normally copied in first. */
 int vfs_solve_warm(vfs_solver *s, int masked, int start_level, int warps, int iters);
 
+/* Copy-outs of results (vfs_copy_disparity_1d, vfs_copy_flow, vfs_copy_stereo,
+ * vfs_copy_disparity_vis, vfs_copy_warped_image) and vfs_synchronize return
+ * VFS_ERR_NUMERIC, after copying, when an energy record of the last solve counted
+ * non-finite values (option "energy" >= 1; the reference returns 0 and hands NaN on). */
+
 /* StereoTgv::copy1DDisparityToHost (stereotgv.h:121, stereotgv.cu:533-540): u, width x height. */
 int vfs_copy_disparity_1d(vfs_solver *s, float *out, int pitch);
 
@@ -135,6 +142,25 @@ int vfs_read_plane(vfs_solver *s, const char *name, int level, float *out);
 /* Blocks until all work queued on the solver's stream has finished. */
 int vfs_synchronize(vfs_solver *s);
 
+/* Energy monitor of a solve (no counterpart in the reference).  With option "energy" 1
+ * (default) every level's last warp, with 2 every warp, ends with an FP64 reduction on the
+ * GPU over the processed pixels (mask != 0; unmasked: inside the circle), from u, v_ and
+ * the derivatives after the warp's last inner iteration:
+ *   energy   = sum lambda |Iu (u - u_last) + Iz| + alpha1 |T (grad u - v_)| + alpha0 |grad v_|
+ *   data     = the sum of the first term
+ *   residual = sum |u - u_prev| (the last inner iteration's change of u)
+ *   nonfinite = pixels whose terms are not finite (left out of the sums), pixels = processed
+ * T = [[a, c], [c, b]] is the diffusion tensor; the gradients are the dual step's forward
+ * differences with the backward fallback at mask / image edges (updateDualVariables.cu).
+ * The sums run in a fixed order that does not depend on launch choices or batching.
+ * vfs_get_energy waits for the last solve and returns the record of (level, warp), or
+ * VFS_ERR_STATE if that solve wrote none (warps >= 128 share slot 127). */
+typedef struct {
+    double energy, data, residual;
+    int nonfinite, pixels;
+} vfs_energy_t;
+int vfs_get_energy(vfs_solver *s, int level, int warp, vfs_energy_t *out);
+
 /* Execution options:
  *   "graph"   1 = replay the solve as a captured HIP graph (default), 0 = launch
  *             kernels one by one;
@@ -151,7 +177,9 @@ int vfs_synchronize(vfs_solver *s);
  *             of 3 pixels per iteration; levels with a mask pixel in column 0 run one;
  *             eight only in the tiles 32x8, 16x16, 20x20, 14x13, 8x8, 7x7 and 4x4, else
  *             four);
- *   "pair"    the same as "per_launch" -1 / 1 / 2 written as -1 / 0 / 1.
+ *   "pair"    the same as "per_launch" -1 / 1 / 2 written as -1 / 0 / 1;
+ *   "energy"  0 = no energy records, 1 = after the last warp of every level (default),
+ *             2 = after every warp (vfs_get_energy).
  * Results do not depend on "graph", "autotune", "shape", "per_launch" or "pair".
  * No counterpart in the reference (FP32 only, stereotgv.h:47-100; its block shape is a
  * constructor argument, stereotgv.cu:32-42). */
@@ -184,7 +212,8 @@ int vfs_time_solves(vfs_solver *s, int masked, int n, float *ms_total);
  * launched back to back on the current solver state, HIP events around them.  Returns
  * the time of `reps` iterations in ms and, in the last argument, the iterations per
  * launch (1, 2, 4 or 8).
- * The solver state is left modified; the next solve starts fresh as always. */
+ * The launches run on a copy of the solver's work buffers: the solver's state is left as
+ * the last solve left it (a following warm start continues from it). */
 int vfs_time_inner_iterations(vfs_solver *s, int level, int reps, float *ms_total, int *iters_per_launch);
 
 /* Timing of a plain device-to-device copy kernel moving `bytes` (read + write), `reps`
@@ -246,6 +275,8 @@ typedef struct {
     int warm;           /* 1 if the frame warm-started */
     int ops;            /* ops of its schedule */
     double latency_ms;  /* submit -> result in the caller's buffer */
+    int nonfinite;      /* non-finite values counted by the frame's last energy record (level 0) */
+    double energy;      /* that record's energy (vfs_energy_t) */
 } vfs_server_result;
 
 typedef struct {

@@ -150,6 +150,15 @@ class Oracle:
         self._fn("oracle_stereo")(self._h, *args, _p(depth, self.ct), _p(X, self.ct))
         return depth, X
 
+    def energy(self, level, warp):
+        """Energy record of (level, warp) of the last solve: dict(energy, data, residual,
+        nonfinite, pixels) (oracle_k_energy)."""
+        out = np.zeros(5)
+        f = self._fn("oracle_get_energy")
+        f.restype = None
+        f(self._h, level, warp, _p(out))
+        return dict(energy=out[0], data=out[1], residual=out[2], nonfinite=int(out[3]), pixels=int(out[4]))
+
     def level_dims(self, level):
         w, h, s = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
         self._fn("oracle_level_dims")(self._h, level, ctypes.byref(w), ctypes.byref(h), ctypes.byref(s))
@@ -226,6 +235,16 @@ def k_warp(img, uv):
     return out
 
 
+def k_find_tv2(uv, tv):
+    """FindWarpingVector: the 2-channel trajectory field `tv` sampled (bilinear, mirror)
+    at x + warpUV(x) for every pixel."""
+    uv, tv = _d(uv), _d(tv)
+    h, w = tv.shape[:2]
+    out = np.zeros((h, w, 2))
+    _k("oracle_k_find_tv2")(_p(uv), _p(tv), w, h, w, _p(out))
+    return out
+
+
 def k_downscale(src, w, h):
     src = _d(src)
     H, W = src.shape[:2]
@@ -288,6 +307,19 @@ def k_deriv(I0, I1w, tv):
     Iu, Iz = np.zeros((h, w)), np.zeros((h, w))
     _k("oracle_deriv")(_p(I0), _p(I1w), _p(tv), w, h, w, _p(Iu), _p(Iz))
     return Iu, Iz
+
+
+def k_energy(u, uprev, v, Iu, Iz, ulast, a, b, c, lam, alpha0, alpha1, mask=None):
+    """Energy monitor terms (oracle_k_energy) of compact (h, w) planes; v is (h, w, 2);
+    mask=None is the unmasked form (circle gate)."""
+    u, uprev, v, Iu, Iz, ulast, a, b, c = map(_d, (u, uprev, v, Iu, Iz, ulast, a, b, c))
+    h, w = u.shape
+    out = np.zeros(5)
+    m = None if mask is None else _d(mask)
+    dd = ctypes.c_double
+    _k("oracle_k_energy")(None if m is None else _p(m), _p(u), _p(uprev), _p(v), _p(Iu), _p(Iz), _p(ulast), _p(a),
+                          _p(b), _p(c), w, h, w, dd(lam), dd(alpha0), dd(alpha1), _p(out))
+    return dict(energy=out[0], data=out[1], residual=out[2], nonfinite=int(out[3]), pixels=int(out[4]))
 
 
 def k_median5(u):

@@ -305,14 +305,15 @@ void FN(oracle_k_dual_masked)(const R *mask, const R *u_, const R *v_, R alpha0,
         }
 }
 
-/* The unmasked solve's circle gate: radius >= width/2.2 around (w/2, h/2)
- * (updateDualVariables.cu:13-22).  Evaluated in double on the float radius bound. */
+/* The unmasked solve's circle gate (updateDualVariables.cu:13-22, thresholdingL1.cu:10-19,
+ * updatePrimalVariables.cu:14-23): radius = sqrtf((y - h/2)^2 + (x - w/2)^2) >= w/2.2,
+ * in float arithmetic with the correctly rounded square root, as the reference. */
 static inline int FN(outside_circle)(int x, int y, int w, int h)
 {
     float desired = (float)w / 2.20f;
-    double hw = (double)((float)w / 2.0f), hh = (double)((float)h / 2.0f);
-    double dx = (double)x - hw, dy = (double)y - hh;
-    return dx * dx + dy * dy >= (double)desired * (double)desired;
+    float hw = (float)w / 2.0f, hh = (float)h / 2.0f;
+    float dy = (float)y - hh, dx = (float)x - hw;
+    return sqrtf(dy * dy + dx * dx) >= desired;
 }
 
 /* updateDualVariables.cu:4-101 (unmasked; writes p = q = 0 outside the circle) */
@@ -721,6 +722,76 @@ void FN(oracle_deriv)(const R *I0, const R *I1w, const R *tv, int w, int h, int 
         }
 }
 
+/* Energy monitor of a warp (no reference counterpart; the GPU solver's FP64 energy and
+ * residual records): over the processed pixels (mask != 0, or inside the circle when
+ * mask == NULL), with the dual step's gradients (updateDualVariables.cu:152-212: forward
+ * differences, backward where the right / lower neighbour is outside the mask or the
+ * image, flat index < 0 reads 0):
+ *   term = lambda |Iu (u - u_last) + Iz| + alpha1 |T (grad u - v)| + alpha0 |grad v|,
+ *   T = [[a, c], [c, b]];  residual = |u - u_prev|.
+ * out = {sum of terms, sum of data terms, sum of residuals, pixels with a non-finite term
+ * or residual (left out of the sums), processed pixels}.  Evaluated in double in row
+ * order. */
+void FN(oracle_k_energy)(const R *mask, const R *u, const R *uprev, const R *v, const R *Iu, const R *Iz,
+                         const R *ulast, const R *at, const R *bt, const R *ct, int w, int h, int s,
+                         R lambda, R alpha0, R alpha1, double *out)
+{
+    double se = 0, sd = 0, sr = 0, nf = 0, np = 0;
+    for (int y = 0; y < h; y++)
+        for (int x = 0; x < w; x++) {
+            long pos = (long)y * s + x;
+            int hasR, hasD;
+            if (mask) {
+                if (mask[pos] == (R)0) continue;
+                hasR = x + 1 < w && mask[pos + 1] != (R)0;
+                hasD = y + 1 < h && mask[pos + s] != (R)0;
+            } else {
+                if (FN(outside_circle)(x, y, w, h)) continue;
+                hasR = x + 1 < w;
+                hasD = y + 1 < h;
+            }
+            double u0 = (double)u[pos], v0x = (double)v[2 * pos], v0y = (double)v[2 * pos + 1];
+            double ux, uy, g0, g1, g2, g3;
+            if (hasR) {
+                ux = ((double)u[pos + 1] - u0) - v0x;
+                g0 = (double)v[2 * (pos + 1)] - v0x;
+                g3 = (double)v[2 * (pos + 1) + 1] - v0y;
+            } else {
+                double uL = pos >= 1 ? (double)u[pos - 1] : 0.0;
+                double vLx = pos >= 1 ? (double)v[2 * (pos - 1)] : 0.0, vLy = pos >= 1 ? (double)v[2 * (pos - 1) + 1] : 0.0;
+                ux = (u0 - uL) - v0x;
+                g0 = v0x - vLx;
+                g3 = v0y - vLy;
+            }
+            if (hasD) {
+                uy = ((double)u[pos + s] - u0) - v0y;
+                g1 = (double)v[2 * (pos + s) + 1] - v0y;
+                g2 = (double)v[2 * (pos + s)] - v0x;
+            } else {
+                double uU = pos >= s ? (double)u[pos - s] : 0.0;
+                double vUx = pos >= s ? (double)v[2 * (pos - s)] : 0.0, vUy = pos >= s ? (double)v[2 * (pos - s) + 1] : 0.0;
+                uy = (u0 - uU) - v0y;
+                g1 = v0y - vUy;
+                g2 = v0x - vUx;
+            }
+            double a = (double)at[pos], b = (double)bt[pos], c = (double)ct[pos];
+            double tx = a * ux + c * uy, ty = c * ux + b * uy;
+            double rho = (double)Iu[pos] * (u0 - (double)ulast[pos]) + (double)Iz[pos];
+            double data = (double)lambda * fabs(rho);
+            double term = data + (double)alpha1 * sqrt(tx * tx + ty * ty) +
+                          (double)alpha0 * sqrt(g0 * g0 + g1 * g1 + g2 * g2 + g3 * g3);
+            double res = fabs(u0 - (double)uprev[pos]);
+            np += 1;
+            if (isfinite(term) && isfinite(res)) { se += term; sd += data; sr += res; }
+            else nf += 1;
+        }
+    out[0] = se; out[1] = sd; out[2] = sr; out[3] = nf; out[4] = np;
+}
+
+#ifndef ORACLE_MON_WARPS
+#define ORACLE_MON_WARPS 128
+#endif
+
 /* ------------------------------------------------------------ solver state */
 
 typedef struct FN(oracle_s) {
@@ -733,6 +804,7 @@ typedef struct FN(oracle_s) {
     R *i1cal, *i1warp, *Iu, *Iz, *u, *du, *us, *warpUV, *dwarpUV, *warpUVs, *tv2, *cv;
     R *a, *b, *c, *etau, *etav1, *etav2, *p, *q, *u_, *u_s, *u_last, *v, *v_, *v_s, *gradv, *Tp, *i0smooth;
     R *X, *depth;
+    double mon[ORACLE_MAXL][ORACLE_MON_WARPS][5]; /* oracle_k_energy of every (level, warp) */
     R *work[40];
     size_t worksz[40];
     int nwork;
@@ -948,6 +1020,9 @@ static void FN(solve_level)(FN(oracle_t) *o, int masked, int l, int warps, int i
             sigma = sigma / mu;
             tau = tau * mu;
         }
+        FN(oracle_k_energy)(masked ? mask : NULL, o->u, o->us, o->v_, o->Iu, o->Iz, o->u_last, o->a, o->b, o->c,
+                            w, h, s, o->lambda, o->alpha0, o->alpha1,
+                            o->mon[l][wi < ORACLE_MON_WARPS ? wi : ORACLE_MON_WARPS - 1]);
         if (masked) {
             FN(oracle_k_median5)(o->u, w, h, s, o->us);
             SWAP_PTR(o->u, o->us);
@@ -1035,6 +1110,13 @@ int FN(oracle_solve_warm)(FN(oracle_t) *o, int masked, int start_level, int warp
         if (l > 0) FN(upscale_level)(o, l);
     }
     return 0;
+}
+
+/* Energy record of (level, warp) of the last solve (oracle_k_energy). */
+void FN(oracle_get_energy)(const FN(oracle_t) *o, int level, int warp, double *out)
+{
+    if (warp >= ORACLE_MON_WARPS) warp = ORACLE_MON_WARPS - 1;
+    memcpy(out, o->mon[level][warp], 5 * sizeof(double));
 }
 
 /* Outputs (stereotgv.cu:517-568), unpadded, row pitch = width. */

@@ -124,6 +124,49 @@ def test_one_iteration_planes(masked):
 
 
 @pytest.mark.parametrize("masked", [True, False])
+def test_later_warp_planes_follow_the_flow(masked):
+    """L=1, W=3, I=5: the third warp's preparation samples the trajectory field and I1 at
+    x + warpUV with the flow of the first two warps (nonzero, fractional), so tv2, i1warp
+    and the derivatives built from it are compared where the warp is not the identity."""
+    pair = small_pair()
+    g = run_gpu(pair, 1, 3, 5, masked=masked)
+    o = run_oracle(pair, 1, 3, 5, masked=masked)
+    m = pair["mask"] > 0 if masked else np.ones(pair["mask"].shape, bool)
+    uv = o.read_plane("warpUV", 0)
+    moved = np.linalg.norm(uv, axis=-1)[m]
+    assert (moved > 0.05).mean() > 0.5, "the flow must move most pixels"
+    for name in ("tv2", "i1warp", "Iu", "Iz", "warpUV", "u", "u_last"):
+        got, ref = g.read_plane(name, 0), o.read_plane(name, 0)
+        d = np.abs(got - ref)
+        print(f"masked={masked} {name}: max |diff| {d.max():.2e}, mean {d.mean():.2e}")
+        assert_close(name, got, ref, 1e-3, 5e-4)
+        assert np.isclose(got, ref, rtol=1e-4, atol=2e-5).mean() > 0.99, name
+    # without the flow the samples would differ from the oracle's by far more
+    tv = o.read_plane("tv", 0)
+    assert np.abs(o.read_plane("tv2", 0) - tv)[m].max() > 1e-2
+
+
+def test_unmasked_circle_gate_in_float_on_the_gpu():
+    """523x392: eight pixels where sqrtf rounds onto the gate's bound (outside for the
+    reference's float test, inside for an exact one).  After one warp of two iterations
+    the GPU's duals and u are 0 exactly where the oracle's float gate says outside."""
+    pair = S.make_pair(523, 392, 8, radius=196, dmax=6.0)
+    g = run_gpu(pair, 1, 1, 2, masked=False)
+    o = run_oracle(pair, 1, 1, 2, masked=False)
+    ug, uo = g.copy1DDisparityToHost(), o.copy1DDisparityToHost()
+    assert np.array_equal(ug == 0, uo == 0)
+    f = np.float32
+    ys, xs = np.mgrid[0:392, 0:523]
+    dy, dx = ys.astype(f) - f(392) / f(2), xs.astype(f) - f(523) / f(2)
+    out = np.sqrt(dy * dy + dx * dx) >= f(523) / f(2.2)
+    assert np.array_equal(ug == 0, out)
+    for name in ("p", "q"):
+        got = g.read_plane(name, 0)
+        assert not got[out].any(), name
+        assert_close(name, got[~out], o.read_plane(name, 0)[~out], 1e-4, 1e-5)
+
+
+@pytest.mark.parametrize("masked", [True, False])
 def test_small_solve(masked):
     pair = small_pair(128, 96, seed=11)
     g = run_gpu(pair, 3, 3, 10, masked=masked)
@@ -331,6 +374,37 @@ def test_u8_input_path():
     assert np.abs(g.copy1DDisparityToHost() - o.copy1DDisparityToHost())[m].mean() <= 5e-4
 
 
+def test_u8c3_input_path():
+    """CV_8UC3 frames (imageConverter.cu:26-42): gray = (c0 + c1 + c2) / 3 / 256 per pixel,
+    then the same solve."""
+    pair = small_pair(96, 80, seed=19)
+    rng = np.random.default_rng(3)
+    base0 = (pair["i0"] * 255).astype(np.int32)
+    base1 = (pair["i1"] * 255).astype(np.int32)
+    jit = lambda b: np.clip(b[..., None] + rng.integers(-6, 7, size=b.shape + (3,)), 0, 255).astype(np.uint8)
+    pair["i0u8"], pair["i1u8"] = jit(base0), jit(base1)
+    g = run_gpu(pair, 2, 2, 5, u8=True)
+    o = run_oracle(pair, 2, 2, 5, u8=True)
+    want = pair["i0u8"].astype(np.float64).sum(-1) / 3.0 / 256.0
+    assert_close("I0@0", g.read_plane("I0", 0), want, 1e-6, 0)
+    assert_close("I0@0 vs oracle", g.read_plane("I0", 0), o.read_plane("I0", 0), 1e-6, 0)
+    m = pair["mask"] > 0
+    assert np.abs(g.copy1DDisparityToHost() - o.copy1DDisparityToHost())[m].mean() <= 5e-4
+
+
+@pytest.mark.parametrize("storage", ["f32", "f16"])
+def test_copy_warped_image(storage):
+    """copyWarpedImageToHost (stereotgv.cu:561-568): the last warp's warped I1 at level 0;
+    in 16-bit storage it is widened through a temporary float plane."""
+    pair = small_pair(96, 80, seed=27)
+    g = run_gpu(pair, 2, 2, 5, storage=storage)
+    o = run_oracle(pair, 2, 2, 5)
+    got = g.copyWarpedImageToHost()
+    assert np.array_equal(got, g.read_plane("i1warp", 0))
+    tol = 1e-3 if storage == "f32" else 4e-3
+    assert_close("warped", got, o.copyWarpedImageToHost(), tol, tol)
+
+
 def test_solve_is_repeatable():
     pair = small_pair(96, 80, seed=21)
     g = run_gpu(pair, 2, 2, 5)
@@ -353,7 +427,21 @@ def _workload_parity(name, tol_mean):
     assert np.isfinite(ug[m]).all()
     assert err.mean() <= tol_mean, err.mean()
     assert epe.mean() <= tol_mean, epe.mean()
+    assert_tail(name, err)
     return g, o, pair
+
+
+def assert_tail(name, err, p999=0.6, frac_over_1px=3e-4):
+    """The error tail against the FP64 oracle.  The FP32 oracle itself (the same algorithm
+    in float) has p99.9 0.28 px and 36 of 502,652 mask pixels over 1 px at 848x800, from
+    the median and limit thresholds that float rounding can flip; the bounds leave about
+    twice that, so a regression that corrupts a few hundred pixels fails."""
+    q = float(np.percentile(err, 99.9))
+    over = int((err > 1.0).sum())
+    print(f"{name}: p99 {np.percentile(err, 99):.3e} px, p99.9 {q:.3e} px, over 1 px {over} of {err.size}, "
+          f"max {err.max():.3f} px")
+    assert q <= p999, (name, q)
+    assert over <= frac_over_1px * err.size, (name, over, err.size)
 
 
 @pytest.mark.parametrize("masked", [True, False])
@@ -456,6 +544,7 @@ def test_reduced_storage_424x400(storage):
     err = np.abs(g.copy1DDisparityToHost() - o.copy1DDisparityToHost())[m]
     print(f"{storage} 424x400: mean |du| {err.mean():.3e} px, max {err.max():.3f} px")
     assert err.mean() <= 0.05, err.mean()
+    assert_tail(f"{storage} 424x400", err, p999=1.5, frac_over_1px=2e-3)
 
 
 def test_batch_equals_alone_mixed_sizes():

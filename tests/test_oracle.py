@@ -402,3 +402,226 @@ def test_unmasked_solve_zero_outside_circle(synth):
     outside = (xs - 48) ** 2 + (ys - 40) ** 2 >= (np.float32(96 / 2.2)) ** 2
     # outside the circle every iteration writes 0, u = u_last + min(0 - u_last, limit) = 0
     assert np.all(u[outside] == 0)
+
+
+def _float_gate(w, h):
+    """Outside-circle pixels of the unmasked kernels in float arithmetic:
+    sqrtf((y - h/2)^2 + (x - w/2)^2) >= w / 2.2 (updateDualVariables.cu:13-22), with
+    numpy's correctly rounded float32 square root."""
+    f = np.float32
+    ys, xs = np.mgrid[0:h, 0:w]
+    dy = ys.astype(f) - f(h) / f(2)
+    dx = xs.astype(f) - f(w) / f(2)
+    return np.sqrt(dy * dy + dx * dx) >= f(w) / f(2.2)
+
+
+def test_unmasked_circle_gate_rounds_in_float(synth):
+    """At 523x392 eight pixels lie where the float square root rounds up onto the bound
+    while the exact radius is below it: the reference (float sqrtf) treats them as
+    outside.  The oracle's unmasked solve writes u = 0 at exactly the float gate's
+    outside pixels, those eight included."""
+    W, H = 523, 392
+    out = _float_gate(W, H)
+    ys, xs = np.mgrid[0:H, 0:W]
+    exact = (xs - W / 2) ** 2 + (ys - H / 2) ** 2 >= float(np.float32(W) / np.float32(2.2)) ** 2
+    assert (out != exact).sum() == 8
+    pair = synth.make_pair(W, H, seed=8, radius=H / 2, dmax=6.0)
+    u = O.solve_pair(pair, 1, 1, 2, synth.SOLVER_PARAMS, masked=False).copy1DDisparityToHost()
+    assert np.array_equal(u == 0, out)
+
+
+# ------------------------------------------------------------- pins added for the warp
+# direction resample and the primal step's tensor term
+
+def _mirror(k, n):
+    """Index on the symmetric extension T[-1-i] = T[i], T[n+i] = T[n-1-i]."""
+    k = np.mod(k, 2 * n)
+    return np.where(k < n, k, 2 * n - 1 - k)
+
+
+def test_tv2_resample_follows_the_flow():
+    """FindWarpingVector (warpImage.cu:96-111): tv2(x) = tv sampled at x + warpUV(x).
+    Zero flow is the identity; a per-pixel integer flow picks the texel it points at,
+    mirrored at the edges; a half-pixel flow averages the neighbours; a constant field is
+    invariant under any flow; the two channels are sampled independently."""
+    H, W = 9, 11
+    tv = rng.normal(size=(H, W, 2))
+    assert np.array_equal(O.k_find_tv2(np.zeros((H, W, 2)), tv), tv)
+    dx = rng.integers(-4, 5, size=(H, W))
+    dy = rng.integers(-4, 5, size=(H, W))
+    uv = np.stack([dx, dy], -1).astype(np.float64)
+    ys, xs = np.mgrid[0:H, 0:W]
+    want = tv[_mirror(ys + dy, H), _mirror(xs + dx, W)]
+    assert np.allclose(O.k_find_tv2(uv, tv), want, rtol=0, atol=1e-14)
+    # channel independence: a field that is zero in channel 1 stays zero there
+    tv0 = tv.copy()
+    tv0[..., 1] = 0.0
+    got = O.k_find_tv2(uv, tv0)
+    assert not got[..., 1].any() and np.allclose(got[..., 0], want[..., 0], rtol=0, atol=1e-14)
+    # half a pixel right and a quarter down at interior pixels
+    uv = np.zeros((H, W, 2))
+    uv[..., 0] = 0.5
+    uv[..., 1] = 0.25
+    got = O.k_find_tv2(uv, tv)
+    top = 0.5 * (tv[:-1, :-1] + tv[:-1, 1:])
+    bot = 0.5 * (tv[1:, :-1] + tv[1:, 1:])
+    assert np.allclose(got[:-1, :-1], 0.75 * top + 0.25 * bot, rtol=0, atol=1e-14)
+    # the last column mirrors onto itself: (w-1) + 0.5 averages texel w-1 with its mirror
+    assert np.allclose(got[:-1, -1], 0.75 * tv[:-1, -1] + 0.25 * tv[1:, -1], rtol=0, atol=1e-14)
+    const = np.empty((H, W, 2))
+    const[..., 0], const[..., 1] = 0.3, -0.7
+    assert np.allclose(O.k_find_tv2(rng.normal(size=(H, W, 2)) * 5, const), const, rtol=0, atol=1e-14)
+
+
+def test_primal_tensor_term_along_eigenvectors():
+    """The v step's alpha1 T p term (updatePrimalVariables.cu:216-226): with q = 0 and
+    v_ = 0 the step is v = tau alpha1 (T p) / eta_v per component.  T is assembled from
+    its eigendecomposition T = w n n^T + t t^T, t perpendicular to n (the structure
+    calcTensor.cu:60-70 builds), so T n = w n and T t = t: for p = alpha n + beta t the
+    expected T p = alpha w n + beta t needs no product with a, b, c."""
+    H, W = 6, 7
+    mask = _full((H, W))
+    th = rng.uniform(0, 2 * np.pi, (H, W))
+    w = rng.uniform(0.05, 0.95, (H, W))
+    nx, ny = np.cos(th), np.sin(th)
+    tx, ty = -ny, nx
+    a = w * nx * nx + tx * tx
+    b = w * ny * ny + ty * ty
+    c = w * nx * ny + tx * ty
+    al = rng.normal(size=(H, W))
+    be = rng.normal(size=(H, W))
+    p = np.stack([al * nx + be * tx, al * ny + be * ty], -1)
+    e1 = rng.uniform(2.0, 6.0, (H, W))
+    e2 = rng.uniform(2.0, 6.0, (H, W))
+    tau, alpha0, alpha1 = 0.7, 17.0, 1.2
+    zeros = np.zeros((H, W))
+    v, _, _ = O.k_primal_masked(mask, zeros, np.zeros((H, W, 2)), p, np.zeros((H, W, 4)), a, b, c, tau, e1, e2,
+                                alpha0, alpha1, 0.5, zeros)
+    tpx = al * w * nx + be * tx
+    tpy = al * w * ny + be * ty
+    assert np.allclose(v[..., 0], tau * alpha1 * tpx / e1, rtol=1e-12, atol=1e-14)
+    assert np.allclose(v[..., 1], tau * alpha1 * tpy / e2, rtol=1e-12, atol=1e-14)
+    # pure eigenvectors: p = n is scaled by w, p = t is kept
+    vn, _, _ = O.k_primal_masked(mask, zeros, np.zeros((H, W, 2)), np.stack([nx, ny], -1), np.zeros((H, W, 4)), a, b,
+                                 c, tau, e1, e1, alpha0, alpha1, 0.5, zeros)
+    assert np.allclose(vn * e1[..., None] / (tau * alpha1), w[..., None] * np.stack([nx, ny], -1), rtol=1e-12,
+                       atol=1e-14)
+    vt, _, _ = O.k_primal_masked(mask, zeros, np.zeros((H, W, 2)), np.stack([tx, ty], -1), np.zeros((H, W, 4)), a, b,
+                                 c, tau, e2, e2, alpha0, alpha1, 0.5, zeros)
+    assert np.allclose(vt * e2[..., None] / (tau * alpha1), np.stack([tx, ty], -1), rtol=1e-12, atol=1e-14)
+
+
+# ------------------------------------------------------------- energy monitor
+
+def _energy(u, v, Iu, Iz, a, b, c, uprev=None, ulast=None, mask=None, lam=5.0, a0=17.0, a1=1.2):
+    uprev = u if uprev is None else uprev
+    ulast = u if ulast is None else ulast
+    return O.k_energy(u, uprev, v, Iu, Iz, ulast, a, b, c, lam, a0, a1, mask)
+
+
+def test_energy_data_term_alone():
+    """Constant u equal to u_last and v = 0: no gradient anywhere (forward and backward
+    differences vanish), so the energy is the data term lambda sum |Iz|."""
+    H, W = 7, 9
+    u = np.full((H, W), 0.37)
+    Iu, Iz = rng.normal(size=(H, W)), rng.normal(size=(H, W))
+    a, b, c = rng.random((H, W)), rng.random((H, W)), rng.random((H, W))
+    e = _energy(u, np.zeros((H, W, 2)), Iu, Iz, a, b, c, mask=_full((H, W)))
+    want = 5.0 * np.abs(Iz).sum()
+    assert np.isclose(e["energy"], want, rtol=1e-13) and np.isclose(e["data"], want, rtol=1e-13)
+    assert e["residual"] == 0 and e["nonfinite"] == 0 and e["pixels"] == H * W
+
+
+def test_energy_linear_u_with_matching_v_has_no_smoothness_cost():
+    """u = alpha x + beta y and v = (alpha, beta): grad u - v = 0 with the forward and the
+    backward (edge) differences alike, and grad v = 0; with Iu = 0 only lambda |Iz| is left."""
+    H, W = 8, 10
+    ys, xs = np.mgrid[0:H, 0:W].astype(float)
+    al, be = 0.7, -1.3
+    u = al * xs + be * ys
+    v = np.stack([np.full((H, W), al), np.full((H, W), be)], -1)
+    Iz = rng.normal(size=(H, W))
+    e = _energy(u, v, np.zeros((H, W)), Iz, rng.random((H, W)), rng.random((H, W)), rng.random((H, W)),
+                ulast=rng.normal(size=(H, W)), mask=_full((H, W)))
+    assert np.isclose(e["energy"], 5.0 * np.abs(Iz).sum(), rtol=1e-12)
+
+
+def test_energy_tensor_term_along_eigenvectors():
+    """u a ramp k (n . x) along a fixed unit n, v = 0, T = w n n^T + t t^T: grad u = k n
+    exactly (also at the edges), so |T (grad u - v)| = w |k| at every pixel."""
+    H, W = 6, 8
+    th = 0.6
+    nx, ny = np.cos(th), np.sin(th)
+    tx, ty = -ny, nx
+    w = rng.uniform(0.1, 0.9, (H, W))
+    a, b, c = w * nx * nx + tx * tx, w * ny * ny + ty * ty, w * nx * ny + tx * ty
+    ys, xs = np.mgrid[0:H, 0:W].astype(float)
+    k = 1.7
+    u = k * (nx * xs + ny * ys)
+    e = _energy(u, np.zeros((H, W, 2)), np.zeros((H, W)), np.zeros((H, W)), a, b, c, mask=_full((H, W)))
+    assert e["data"] == 0
+    assert np.isclose(e["energy"], 1.2 * k * w.sum(), rtol=1e-12)
+
+
+def test_energy_second_order_term_and_residual():
+    """T = 0 and v = (gamma x, 0): grad v = (gamma, 0, 0, 0) everywhere (backward at the
+    right edge), so the energy is alpha0 N |gamma|; the residual is sum |u - u_prev|."""
+    H, W = 5, 7
+    ys, xs = np.mgrid[0:H, 0:W].astype(float)
+    g = -0.45
+    v = np.stack([g * xs, np.zeros((H, W))], -1)
+    u = rng.normal(size=(H, W))
+    du = rng.normal(size=(H, W))
+    z = np.zeros((H, W))
+    e = _energy(u, v, z, z, z, z, z, uprev=u - du, ulast=u, mask=_full((H, W)))
+    assert np.isclose(e["energy"], 17.0 * H * W * abs(g), rtol=1e-12)
+    assert np.isclose(e["residual"], np.abs(du).sum(), rtol=1e-12)
+
+
+def test_energy_backward_difference_at_mask_edge_and_write_set():
+    """u = x^2, T = I, v = 0: |u_x| is the forward difference 2x + 1 inside, the backward
+    one 2x - 1 where the right neighbour is outside the mask or the image; only mask pixels
+    are counted."""
+    H, W = 4, 9
+    ys, xs = np.mgrid[0:H, 0:W].astype(float)
+    u = xs ** 2
+    mask = _full((H, W))
+    mask[:, 5] = 0.0
+    one, z = np.ones((H, W)), np.zeros((H, W))
+    e = _energy(u, np.zeros((H, W, 2)), z, z, one, one, z, mask=mask)
+    per_col = [2 * x + 1 if (x + 1 < W and x + 1 != 5) else 2 * x - 1 for x in range(W) if x != 5]
+    assert e["pixels"] == H * (W - 1)
+    assert np.isclose(e["energy"], 1.2 * H * sum(per_col), rtol=1e-12)
+
+
+def test_energy_counts_nonfinite_pixels():
+    """A NaN u at an interior pixel makes its own term and those of its left and upper
+    neighbours (whose forward differences reach it) non-finite: 3 pixels, left out of the
+    sums."""
+    H, W = 6, 6
+    u = rng.normal(size=(H, W))
+    u[3, 3] = np.nan
+    z = np.zeros((H, W))
+    one = np.ones((H, W))
+    e = _energy(u, np.zeros((H, W, 2)), z, z, one, one, z, uprev=np.nan_to_num(u), mask=_full((H, W)))
+    assert e["nonfinite"] == 3 and np.isfinite(e["energy"]) and e["pixels"] == H * W
+
+
+def test_energy_unmasked_counts_the_circle():
+    H, W = 40, 44
+    z = np.zeros((H, W))
+    e = _energy(z, np.zeros((H, W, 2)), z, z, z, z, z)
+    assert e["pixels"] == int((~_float_gate(W, H)).sum())
+
+
+def test_energy_records_of_a_solve(synth):
+    """The oracle's solve records every (level, warp): the data term is the data part of
+    the energy, every mask pixel of the level is counted, and nothing is non-finite."""
+    pair = synth.make_pair(96, 80, seed=17, radius=40, dmax=4.0)
+    o = O.solve_pair(pair, 2, 2, 6, synth.SOLVER_PARAMS)
+    for lvl in range(2):
+        npx = int((o.read_plane("mask", lvl) != 0).sum())
+        for wi in range(2):
+            r = o.energy(lvl, wi)
+            assert r["pixels"] == npx and r["nonfinite"] == 0
+            assert 0 < r["data"] < r["energy"] and r["residual"] > 0

@@ -48,6 +48,7 @@ EXPORTS = [
     ("vfs_get_level_dims", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, _c_int_p, _c_int_p, _c_int_p]),
     ("vfs_read_plane", ctypes.c_int, [ctypes.c_void_p, ctypes.c_char_p, ctypes.c_int, _c_float_p]),
     ("vfs_synchronize", ctypes.c_int, [ctypes.c_void_p]),
+    ("vfs_get_energy", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_void_p]),
     ("vfs_set_option", ctypes.c_int, [ctypes.c_void_p, ctypes.c_char_p, ctypes.c_int]),
     ("vfs_time_solves", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, _c_float_p]),
     ("vfs_time_inner_iterations", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, _c_float_p, _c_int_p]),
@@ -91,7 +92,12 @@ class _StreamParams(ctypes.Structure):
 
 class _ServerResult(ctypes.Structure):
     _fields_ = [("tag", ctypes.c_longlong), ("stream", ctypes.c_int), ("warm", ctypes.c_int), ("ops", ctypes.c_int),
-                ("latency_ms", ctypes.c_double)]
+                ("latency_ms", ctypes.c_double), ("nonfinite", ctypes.c_int), ("energy", ctypes.c_double)]
+
+
+class _Energy(ctypes.Structure):
+    _fields_ = [("energy", ctypes.c_double), ("data", ctypes.c_double), ("residual", ctypes.c_double),
+                ("nonfinite", ctypes.c_int), ("pixels", ctypes.c_int)]
 
 
 class _ServerStats(ctypes.Structure):
@@ -274,6 +280,13 @@ class StereoTgv:
     def synchronize(self):
         return self._check(self._L.vfs_synchronize(self._h), "synchronize")
 
+    def energy(self, level: int, warp: int) -> dict:
+        """FP64 energy record of (level, warp) of the last solve (vfs_get_energy):
+        dict(energy, data, residual, nonfinite, pixels)."""
+        e = _Energy()
+        self._check(self._L.vfs_get_energy(self._h, level, warp, ctypes.byref(e)), "get_energy")
+        return dict(energy=e.energy, data=e.data, residual=e.residual, nonfinite=e.nonfinite, pixels=e.pixels)
+
     def set_option(self, key: str, value):
         if key == "storage" and isinstance(value, str):
             value = STORAGE[value.lower()]
@@ -449,7 +462,7 @@ class Server:
         out = []
         for r in buf[: n.value]:
             out.append(dict(tag=r.tag, stream=r.stream, warm=r.warm, ops=r.ops, latency_ms=r.latency_ms,
-                            u=self._keep.pop(r.tag, None)))
+                            nonfinite=r.nonfinite, energy=r.energy, u=self._keep.pop(r.tag, None)))
         return out
 
     def stats(self):

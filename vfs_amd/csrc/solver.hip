@@ -44,7 +44,7 @@ thread_local std::string g_create_error;
 // Device-side launch set: pair descriptors (device copy + host copy for tile counts)
 // and the graphs captured for the programs run on it.
 struct GraphKey {
-    int masked, warm_level, warps, iters, start_roles;
+    int masked, warm_level, warps, iters, start_roles, monitor;
     float limit;
     std::array<unsigned int, kMaxLevels> tuning;  // Tuning::sig() of the program's launch choices
     bool operator<(const GraphKey &o) const
@@ -55,6 +55,7 @@ struct GraphKey {
         if (warps != o.warps) return warps < o.warps;
         if (iters != o.iters) return iters < o.iters;
         if (start_roles != o.start_roles) return start_roles < o.start_roles;
+        if (monitor != o.monitor) return monitor < o.monitor;
         return limit < o.limit;
     }
 };
@@ -75,6 +76,9 @@ struct Engine {
 struct Params {
     int nLevels, nWarpIters, nSolverIters, blockWidth, storage;
     float beta, gamma, alpha0, alpha1, timestep_lambda, lambda, fScale, limitRange;
+    // energy / residual records (OP_ENERGY): 0 none, 1 after the last warp of every level,
+    // 2 after every warp
+    int monitor = 1;
 };
 
 // One op of a solve program (see vfs::OpKind), with the scalars it needs and the tile
@@ -145,6 +149,7 @@ void iter_schedule(float timestep_lambda, int n, float4 *out)
 struct Program {
     std::vector<ProgOp> ops;
     int final_roles = 0;
+    std::vector<int> mon_slots;  // energy records the program writes (vfs::mon_slot), in order
 };
 
 }  // namespace
@@ -165,7 +170,9 @@ struct vfs_solver {
     int force_shape = -1;   // option "shape": -1 = per-level tuner
     bool autotune = true;   // option "autotune"
     int force_per_launch = -1;  // option "per_launch": -1 = tuner, else 1, 2, 4 or 8 wherever allowed
+    int monitor = 1;        // option "energy": 0, 1 or 2 (Params::monitor)
     bool mask_col0[kMaxLevels] = {};  // a mask pixel lies in column 0 of the level
+    unsigned long long mask_gen = 0;  // copyMaskToDevice calls since create (batches re-tune on change)
     Tuning tuning[2];       // [masked] launch choices per level
     bool tuned[2] = {false, false};
     Dims lv[kMaxLevels] = {};
@@ -180,6 +187,10 @@ struct vfs_solver {
     size_t staging_bytes = 0;
     float *outX = nullptr, *outDepth = nullptr;
     bool raw_i1_fresh = false;  // in_i1 holds a frame not yet calibrated
+    // energy records (kMaxLevels x kMonWarps) followed by the partial sums of one OP_ENERGY
+    vfs::EnergyPart *mon = nullptr;
+    std::vector<int> mon_slots;  // records written by the last solve of this pair
+    int numeric = 0;             // last solve: 0 not checked yet, 1 finite, -1 non-finite values
 
     // work arena
     float *arena = nullptr;
@@ -194,7 +205,7 @@ struct vfs_solver {
     Params params() const
     {
         return Params{nLevels, nWarpIters, nSolverIters, blockWidth, storage, beta, gamma, alpha0, alpha1,
-                      timestep_lambda, lambda, fScale, limitRange};
+                      timestep_lambda, lambda, fScale, limitRange, monitor};
     }
     vfs::PairDesc desc() const
     {
@@ -219,6 +230,8 @@ struct vfs_solver {
         // k1 = alpha1 sigma / 3 of each iteration, rounded as the one-iteration kernel
         // forms it (float product, correctly rounded division)
         for (int i = 0; i < vfs::kMaxSched; i++) d.sched[i].w = alpha1 * d.sched[i].x / 3.0f;
+        d.mon_rec = mon;
+        d.mon_part = mon ? mon + kMaxLevels * vfs::kMonWarps : nullptr;
         return d;
     }
 };
@@ -229,6 +242,9 @@ struct vfs_batch {
     std::string err;
     Tuning tuning[2];  // [masked] launch choices of the batch's launches
     bool tuned[2] = {false, false};
+    // members' mask generations when the launch choices were made: a new mask can forbid
+    // multi-iteration launches at a level (a mask pixel in column 0), so it re-tunes
+    std::vector<unsigned long long> mask_gen;
 };
 
 namespace {
@@ -367,6 +383,14 @@ void push_level(Program &pr, const Params &P, int masked, int l, int warps, int 
             }
             pr.ops.push_back(o);
             it += k;
+        }
+        // energy / residual record of the warp (before its end, which overwrites the other
+        // u buffer and clears v_)
+        if (P.monitor == 2 || (P.monitor == 1 && wi + 1 == warps)) {
+            push_op(pr, vfs::OP_ENERGY, l, masked, roles);
+            push_op(pr, vfs::OP_EFINAL, l, masked, roles);
+            pr.ops.back().it0 = wi;
+            pr.mon_slots.push_back(vfs::mon_slot(l, wi));
         }
         push_op(pr, wi + 1 < warps ? vfs::OP_ENDPREP : vfs::OP_END, l, masked, roles);
         if (masked) roles = vfs::rswap(roles, vfs::G_U);
@@ -554,6 +578,8 @@ void release(vfs_solver *s)
     s->staging = nullptr;
     s->staging_bytes = 0;
     s->arena = nullptr;
+    s->mon = nullptr;
+    s->mon_slots.clear();
     s->initialized = s->has_mask = s->has_fields = s->has_images = false;
     s->desc_dirty = true;
 }
@@ -862,12 +888,14 @@ int solve_common(vfs_solver *s, int masked, int warm_level = -1, int warps = 0, 
     const Tuning &tn = s->tuning[masked ? 1 : 0];
     const int start_roles = warm_level < 0 ? 0 : s->roles;
     const Program pr = build_program(P, masked, warm_level, warps, iters, start_roles, tn);
-    const GraphKey key{masked, warm_level, warps, iters, start_roles, P.limitRange, tn.sig()};
+    const GraphKey key{masked, warm_level, warps, iters, start_roles, P.monitor, P.limitRange, tn.sig()};
     rc = engine_run(s->eng, P, pr, key, s->use_graph, &s->err);
     if (rc) return rc;
     s->roles = pr.final_roles;
     s->raw_i1_fresh = false;
     s->solved = true;
+    s->mon_slots = pr.mon_slots;
+    s->numeric = 0;
     HIPCHK(s, hipEventRecord(s->done, s->stream));
     return VFS_OK;
 }
@@ -888,6 +916,42 @@ int copy_out(vfs_solver *s, const void *dev, int nch, int level, float *out, int
                                sizeof(float) * nch * (size_t)d.w, d.h, hipMemcpyDeviceToHost, s->stream));
     HIPCHK(s, hipStreamSynchronize(s->stream));
     return VFS_OK;
+}
+
+// After the last solve of the pair has finished: VFS_ERR_NUMERIC if one of its energy
+// records counted non-finite values (a zero trajectory vector inside the mask makes the
+// derivative NaN there, computeDerivatives.cu:99-104), else VFS_OK.  Reads the records
+// once per solve.
+int check_numeric(vfs_solver *s)
+{
+    if (s->mon_slots.empty()) return VFS_OK;
+    if (s->numeric == 0) {
+        const int lo = *std::min_element(s->mon_slots.begin(), s->mon_slots.end());
+        const int hi = *std::max_element(s->mon_slots.begin(), s->mon_slots.end());
+        std::vector<vfs::EnergyPart> rec((size_t)(hi - lo + 1));
+        HIPCHK(s, hipMemcpyAsync(rec.data(), s->mon + lo, sizeof(vfs::EnergyPart) * rec.size(), hipMemcpyDeviceToHost,
+                                 s->stream));
+        HIPCHK(s, hipStreamSynchronize(s->stream));
+        s->numeric = 1;
+        for (int k : s->mon_slots) {
+            const vfs::EnergyPart &r = rec[(size_t)(k - lo)];
+            if (r.nonfinite > 0) {
+                s->numeric = -1;
+                s->err = "the solve produced " + std::to_string(r.nonfinite) + " non-finite values at level " +
+                         std::to_string(k / vfs::kMonWarps) + " (warp " + std::to_string(k % vfs::kMonWarps) +
+                         "); a zero trajectory vector inside the processed region gives NaN derivatives";
+                break;
+            }
+        }
+    }
+    return s->numeric < 0 ? VFS_ERR_NUMERIC : VFS_OK;
+}
+
+// copy_out of a result plane, then the numeric check of the solve that produced it.
+int copy_result(vfs_solver *s, const void *dev, int nch, int level, float *out, int pitch_elems)
+{
+    const int rc = copy_out(s, dev, nch, level, out, pitch_elems);
+    return rc ? rc : check_numeric(s);
 }
 
 // Storage-typed plane -> float host array (through a temporary float plane).
@@ -1043,6 +1107,11 @@ int vfs_initialize(vfs_solver *s, int width, int height, float beta, float gamma
     if ((rc = dev_alloc(s, (void **)&s->cv, N * sizeof(float2)))) return rc;
     if ((rc = dev_alloc(s, (void **)&s->outX, 3 * N * sizeof(float)))) return rc;
     if ((rc = dev_alloc(s, (void **)&s->outDepth, N * sizeof(float)))) return rc;
+    // energy records and the partial sums of the largest level
+    const size_t nmon = (size_t)kMaxLevels * vfs::kMonWarps + (size_t)vfs::energy_chunks(s->stride, height);
+    if ((rc = dev_alloc(s, (void **)&s->mon, nmon * sizeof(vfs::EnergyPart)))) return rc;
+    s->mon_slots.clear();
+    s->numeric = 0;
     // work arena: PL_COUNT slots of >= N floats, each 256 B aligned
     s->plane = (long long)((N + 63) & ~(size_t)63);
     if ((rc = dev_alloc(s, (void **)&s->arena, sizeof(float) * (size_t)vfs::PL_COUNT * s->plane))) return rc;
@@ -1093,6 +1162,7 @@ int vfs_copy_mask(vfs_solver *s, const float *m, int pitch)
     HIPCHK(s, hipStreamSynchronize(s->stream));
     s->has_mask = true;
     s->tuned[1] = false;
+    s->mask_gen++;
     return VFS_OK;
 }
 
@@ -1176,7 +1246,7 @@ int vfs_copy_disparity_1d(vfs_solver *s, float *out, int pitch)
     if (!out || pitch < s->width) return fail(s, VFS_ERR_ARG, "bad output pointer or pitch");
     int rc = set_device(s);
     if (rc) return rc;
-    return copy_out(s, slot_ptr(s, vfs::slot_u(vfs::rcur(s->roles, vfs::G_U))), 1, 0, out, pitch);
+    return copy_result(s, slot_ptr(s, vfs::slot_u(vfs::rcur(s->roles, vfs::G_U))), 1, 0, out, pitch);
 }
 
 int vfs_copy_flow(vfs_solver *s, float *out, int pitch)
@@ -1186,7 +1256,7 @@ int vfs_copy_flow(vfs_solver *s, float *out, int pitch)
     if (!out || pitch < 2 * s->width) return fail(s, VFS_ERR_ARG, "bad output pointer or pitch");
     int rc = set_device(s);
     if (rc) return rc;
-    return copy_out(s, slot_ptr(s, vfs::slot_uv(s->roles, true)), 2, 0, out, pitch);
+    return copy_result(s, slot_ptr(s, vfs::slot_uv(s->roles, true)), 2, 0, out, pitch);
 }
 
 int vfs_copy_warped_image(vfs_solver *s, float *out, int pitch)
@@ -1196,7 +1266,8 @@ int vfs_copy_warped_image(vfs_solver *s, float *out, int pitch)
     if (!out || pitch < s->width) return fail(s, VFS_ERR_ARG, "bad output pointer or pitch");
     int rc = set_device(s);
     if (rc) return rc;
-    return copy_out_st(s, slot_ptr(s, vfs::PL_I1W), 1, 0, out, pitch);
+    rc = copy_out_st(s, slot_ptr(s, vfs::PL_I1W), 1, 0, out, pitch);
+    return rc ? rc : check_numeric(s);
 }
 
 int vfs_copy_stereo(vfs_solver *s, float *depth, int depth_pitch, float *X, int X_pitch, float focalx, float focaly,
@@ -1217,7 +1288,7 @@ int vfs_copy_stereo(vfs_solver *s, float *depth, int depth_pitch, float *X, int 
     if (depth && (rc = copy_out(s, s->outDepth, 1, 0, depth, depth_pitch))) return rc;
     if (X && (rc = copy_out(s, s->outX, 3, 0, X, X_pitch))) return rc;
     HIPCHK(s, hipStreamSynchronize(s->stream));
-    return VFS_OK;
+    return check_numeric(s);
 }
 
 int vfs_copy_disparity_vis(vfs_solver *s, float *out, int pitch, float flowScale)
@@ -1231,7 +1302,7 @@ int vfs_copy_disparity_vis(vfs_solver *s, float *out, int pitch, float flowScale
     const float2 *uv = (const float2 *)slot_ptr(s, vfs::slot_uv(s->roles, true));
     vfs::launch_flow_hsv(uv, s->lv[0], s->outX, flowScale, s->stream);  // outX: 3-float staging plane
     if ((rc = check_launch(s))) return rc;
-    return copy_out(s, s->outX, 3, 0, out, pitch);
+    return copy_result(s, s->outX, 3, 0, out, pitch);
 }
 
 int vfs_get_level_dims(const vfs_solver *s, int level, int *w, int *h, int *stride)
@@ -1292,6 +1363,28 @@ int vfs_synchronize(vfs_solver *s)
     if (rc) return rc;
     if ((rc = wait_done(s))) return rc;
     HIPCHK(s, hipStreamSynchronize(s->stream));
+    return check_numeric(s);
+}
+
+int vfs_get_energy(vfs_solver *s, int level, int warp, vfs_energy_t *out)
+{
+    if (!s || !out) return VFS_ERR_ARG;
+    if (!s->initialized) return fail(s, VFS_ERR_STATE, "energy before initialize");
+    if (level < 0 || level >= s->nLevels || warp < 0) return fail(s, VFS_ERR_ARG, "bad level or warp");
+    const int k = vfs::mon_slot(level, warp);
+    if (warp >= vfs::kMonWarps || std::find(s->mon_slots.begin(), s->mon_slots.end(), k) == s->mon_slots.end())
+        return fail(s, VFS_ERR_STATE, "the last solve wrote no energy record for level %d, warp %d", level, warp);
+    int rc = set_device(s);
+    if (rc) return rc;
+    if ((rc = wait_done(s))) return rc;
+    vfs::EnergyPart r;
+    HIPCHK(s, hipMemcpyAsync(&r, s->mon + k, sizeof r, hipMemcpyDeviceToHost, s->stream));
+    HIPCHK(s, hipStreamSynchronize(s->stream));
+    out->energy = r.energy;
+    out->data = r.data;
+    out->residual = r.residual;
+    out->nonfinite = (int)r.nonfinite;
+    out->pixels = (int)r.pixels;
     return VFS_OK;
 }
 
@@ -1323,6 +1416,11 @@ int vfs_set_option(vfs_solver *s, const char *key, int value)
         s->tuned[0] = s->tuned[1] = false;
         return VFS_OK;
     }
+    if (!strcmp(key, "energy")) {
+        if (value < 0 || value > 2) return fail(s, VFS_ERR_ARG, "energy must be 0, 1 or 2");
+        s->monitor = value;
+        return VFS_OK;
+    }
     if (!strcmp(key, "per_launch") || !strcmp(key, "pair")) {
         // "pair": -1 = tuner, 0 = one iteration per launch, 1 = two
         if (!strcmp(key, "pair")) {
@@ -1345,6 +1443,7 @@ int vfs_get_option(const vfs_solver *s, const char *key, int *value)
     else if (!strcmp(key, "storage")) *value = s->storage;
     else if (!strcmp(key, "shape")) *value = s->force_shape;
     else if (!strcmp(key, "autotune")) *value = s->autotune ? 1 : 0;
+    else if (!strcmp(key, "energy")) *value = s->monitor;
     else if (!strcmp(key, "per_launch")) *value = s->force_per_launch;
     else if (!strcmp(key, "pair")) *value = s->force_per_launch < 0 ? -1 : s->force_per_launch > 1 ? 1 : 0;
     else return VFS_ERR_ARG;
@@ -1451,22 +1550,33 @@ int vfs_time_inner_iterations(vfs_solver *s, int level, int reps, float *ms_tota
         reps = done;
         apply_tuning(pr, s->tuning[1]);
     }
-    hipEvent_t e0, e1;
-    if ((rc = time_events(s, s->stream, &e0, &e1))) return rc;
-    const hipError_t ri = issue_program(s->eng, P, pr);
-    if (ri != hipSuccess) {
-        (void)hipEventDestroy(e0);
-        (void)hipEventDestroy(e1);
-        return fail(s, VFS_ERR_DEVICE, "kernel launch: %s", hipGetErrorString(ri));
-    }
-    HIPCHK(s, hipEventRecord(e1, s->stream));
-    HIPCHK(s, hipEventSynchronize(e1));
+    // the launches run on a copy of the arena (same data, so the same branches), which
+    // leaves the solver's state as the last solve left it
+    const size_t bytes = sizeof(float) * (size_t)s->plane * vfs::PL_COUNT;
+    float *scratch = nullptr;
+    HIPCHK(s, hipMalloc(&scratch, bytes));
+    Engine tmp;
+    tmp.device = s->device;
+    tmp.stream = s->stream;
+    tmp.d_desc = s->eng.d_desc;
+    tmp.h_desc = s->eng.h_desc;
+    tmp.h_desc[0].arena = scratch;
+    hipError_t ri = hipMemcpyAsync(scratch, s->arena, bytes, hipMemcpyDeviceToDevice, s->stream);
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    if (ri == hipSuccess) ri = hipEventCreate(&e0);
+    if (ri == hipSuccess) ri = hipEventCreate(&e1);
+    if (ri == hipSuccess) ri = hipEventRecord(e0, s->stream);
+    if (ri == hipSuccess) ri = issue_program(tmp, P, pr);
+    if (ri == hipSuccess) ri = hipEventRecord(e1, s->stream);
+    if (ri == hipSuccess) ri = hipEventSynchronize(e1);
     float ms = 0;
-    HIPCHK(s, hipEventElapsedTime(&ms, e0, e1));
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
-    if ((rc = check_launch(s))) return rc;
-    HIPCHK(s, hipEventRecord(s->done, s->stream));
+    if (ri == hipSuccess) ri = hipEventElapsedTime(&ms, e0, e1);
+    if (ri == hipSuccess) ri = hipGetLastError();
+    (void)hipStreamSynchronize(s->stream);
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    (void)hipFree(scratch);
+    if (ri != hipSuccess) return fail(s, VFS_ERR_DEVICE, "timing launches: %s", hipGetErrorString(ri));
     *ms_total = ms * (float)reps0 / (float)reps;
     if (iters_per_launch) *iters_per_launch = per_launch;
     return VFS_OK;
@@ -1627,8 +1737,8 @@ static int batch_solve(vfs_batch *b, int masked, int warm_level, int warps, int 
         vfs_solver *s = b->members[i];
         std::string why;
         if (ready_to_solve(s, masked, &why)) return failb(b, VFS_ERR_STATE, "member %zu: %s", i, why.c_str());
-        if (s->storage != f->storage || s->limitRange != f->limitRange)
-            return failb(b, VFS_ERR_ARG, "member %zu differs in storage or limitRange", i);
+        if (s->storage != f->storage || s->limitRange != f->limitRange || s->monitor != f->monitor)
+            return failb(b, VFS_ERR_ARG, "member %zu differs in storage, limitRange or the energy option", i);
         if (warm_level >= 0 && (!s->solved || s->roles != f->roles))
             return failb(b, VFS_ERR_STATE, "member %zu: warm start needs every member solved with the same program",
                          i);
@@ -1637,6 +1747,7 @@ static int batch_solve(vfs_batch *b, int masked, int warm_level, int warps, int 
         if (!rebuild) {
             const vfs::PairDesc d = s->desc();
             if (memcmp(&b->eng.h_desc[i], &d, sizeof d) != 0) rebuild = true;
+            if (b->mask_gen.size() != b->members.size() || b->mask_gen[i] != s->mask_gen) rebuild = true;
         }
     }
     if (warm_level >= P.nLevels) return failb(b, VFS_ERR_ARG, "warm start level %d >= nLevels", warm_level);
@@ -1646,6 +1757,8 @@ static int batch_solve(vfs_batch *b, int masked, int warm_level, int warps, int 
         hipError_t r = engine_setup(b->eng, descs);
         if (r != hipSuccess) return failb(b, VFS_ERR_DEVICE, "engine setup: %s", hipGetErrorString(r));
         b->tuned[0] = b->tuned[1] = false;
+        b->mask_gen.clear();
+        for (vfs_solver *s : b->members) b->mask_gen.push_back(s->mask_gen);
     }
     // order after each member's pending work, then refresh re-used raw inputs
     for (vfs_solver *s : b->members) {
@@ -1657,13 +1770,15 @@ static int batch_solve(vfs_batch *b, int masked, int warm_level, int warps, int 
     const Tuning &tn = b->tuning[masked ? 1 : 0];
     const int start_roles = warm_level < 0 ? 0 : f->roles;
     const Program pr = build_program(P, masked, warm_level, warps, iters, start_roles, tn);
-    const GraphKey key{masked, warm_level, warps, iters, start_roles, P.limitRange, tn.sig()};
+    const GraphKey key{masked, warm_level, warps, iters, start_roles, P.monitor, P.limitRange, tn.sig()};
     int rc = engine_run(b->eng, P, pr, key, f->use_graph, &b->err);
     if (rc) return rc;
     for (vfs_solver *s : b->members) {
         s->roles = pr.final_roles;
         s->raw_i1_fresh = false;
         s->solved = true;
+        s->mon_slots = pr.mon_slots;
+        s->numeric = 0;
         HIPCHKB(b, hipEventRecord(s->done, b->eng.stream));
     }
     return VFS_OK;
@@ -1753,14 +1868,20 @@ struct SrvRequest {
     long long tag;
     int stream;
     float *out = nullptr;  // caller buffer (width*height floats), may be null
-    float *pinned = nullptr;  // images in (2*w*h floats), then the result (w*h)
+    float *pinned = nullptr;  // images in (2*w*h floats), then the result (w*h) and the last
+                              // energy record (kPinnedExtra floats at offset w*h)
     double t_submit = 0, t_done = 0;
     int warm = 0;
     Program prog;
     size_t pc = 0;
     int waited = 0;  // steps since this frame's last op was launched
     hipEvent_t done = nullptr;
+    int nonfinite = 0;    // of the frame's last energy record (level 0)
+    double energy = 0.0;
 };
+
+// Floats of a frame's pinned buffer after its 2 w h image floats: the last energy record.
+constexpr size_t kPinnedExtra = sizeof(vfs::EnergyPart) / sizeof(float);
 
 double now_ms()
 {
@@ -1947,6 +2068,12 @@ int retire_finished(vfs_server *v)
         if (q != hipSuccess) return fails(v, VFS_ERR_DEVICE, "request event: %s", hipGetErrorString(q));
         SrvStream &st = v->streams[r->stream];
         if (r->out) memcpy(r->out, r->pinned, sizeof(float) * st.width * st.height);
+        if (!r->prog.mon_slots.empty()) {
+            vfs::EnergyPart rec;
+            memcpy(&rec, r->pinned + (size_t)st.width * st.height, sizeof rec);
+            r->nonfinite = (int)rec.nonfinite;
+            r->energy = rec.energy;
+        }
         r->t_done = now_ms();
         st.busy = false;
         st.warm_ok = true;
@@ -2098,6 +2225,9 @@ int launch_one_tick(vfs_server *v)
         const float *u = s->arena + (long long)vfs::slot_u(vfs::rcur(s->roles, vfs::G_U)) * s->plane;
         HIPCHKS(v, hipMemcpy2DAsync(r->pinned, sizeof(float) * st.width, u, sizeof(float) * s->lv[0].s,
                                     sizeof(float) * st.width, st.height, hipMemcpyDeviceToHost, v->stream));
+        if (!r->prog.mon_slots.empty())
+            HIPCHKS(v, hipMemcpyAsync(r->pinned + (size_t)st.width * st.height, s->mon + r->prog.mon_slots.back(),
+                                      sizeof(vfs::EnergyPart), hipMemcpyDeviceToHost, v->stream));
         HIPCHKS(v, take_event(v, &r->done));
         HIPCHKS(v, hipEventRecord(r->done, v->stream));
         v->finishing.push_back(r);
@@ -2212,7 +2342,7 @@ int vfs_server_submit(vfs_server *v, int stream_id, const float *i0, const float
     const size_t n = (size_t)st.width * st.height;
     SrvRequest *r = new (std::nothrow) SrvRequest();
     if (!r) return fails(v, VFS_ERR_NOMEM, "host allocation failed");
-    r->pinned = take_pinned(v, 2 * n);
+    r->pinned = take_pinned(v, 2 * n + kPinnedExtra);
     if (!r->pinned) {
         delete r;
         return fails(v, VFS_ERR_NOMEM, "pinned staging of %zu bytes failed", sizeof(float) * 2 * n);
@@ -2257,9 +2387,11 @@ int vfs_server_collect(vfs_server *v, vfs_server_result *out, int max, int *n)
         out[k].warm = r->warm;
         out[k].latency_ms = r->t_done - r->t_submit;
         out[k].ops = (int)r->prog.ops.size();
+        out[k].nonfinite = r->nonfinite;
+        out[k].energy = r->energy;
         give_event(v, r->done);
         const SrvStream &st = v->streams[r->stream];
-        give_pinned(v, r->pinned, 2 * (size_t)st.width * st.height);
+        give_pinned(v, r->pinned, 2 * (size_t)st.width * st.height + kPinnedExtra);
         delete r;
         k++;
     }

@@ -173,14 +173,15 @@ __device__ __forceinline__ float texel(const T *__restrict__ img, int w, int h, 
     return Cv<T>::f(img[(size_t)mir(y, h) * s + mir(x, w)]);
 }
 
-// Circle gate of the unmasked solve (updateDualVariables.cu:13-22): true when
-// radius >= width / 2.2 around (w/2, h/2).  Evaluated in double on the float bound.
+// Circle gate of the unmasked solve (updateDualVariables.cu:13-22, thresholdingL1.cu:10-19,
+// updatePrimalVariables.cu:14-23): true when sqrtf((y - h/2)^2 + (x - w/2)^2) >= w / 2.2,
+// evaluated in float with the correctly rounded square root, as the reference does.
 __device__ __forceinline__ bool outside_circle(int x, int y, int w, int h)
 {
     const float desired = (float)w / 2.20f;
-    const double hw = (double)((float)w / 2.0f), hh = (double)((float)h / 2.0f);
-    const double dx = (double)x - hw, dy = (double)y - hh;
-    return dx * dx + dy * dy >= (double)desired * (double)desired;
+    const float hw = (float)w / 2.0f, hh = (float)h / 2.0f;
+    const float dy = (float)y - hh, dx = (float)x - hw;
+    return __fsqrt_rn(dy * dy + dx * dx) >= desired;
 }
 
 __device__ __forceinline__ float2 ld0(const float2 *__restrict__ b, long i)
@@ -1384,6 +1385,144 @@ __device__ __forceinline__ void op_up(const Tile &t, int level, int roles, float
 }
 
 
+// ------------------------------------------------------------------ energy monitor
+//
+// FP64 sums of the TGV-L1 energy and of the last iteration's change of u over the
+// processed pixels of a pair (no reference counterpart; the monitor of the solve).  Every
+// per-pixel term is evaluated in double from the stored values; the gradients are the dual
+// step's (updateDualVariables.cu:152-212: forward differences, backward where the right /
+// lower neighbour is outside the mask or the image).  The sums run in a fixed order: each
+// OP_ENERGY workgroup takes kEnergyChunk consecutive pixels of the level in row-major
+// order (4 per work-item, then a butterfly over the wave and the 4 waves in order), and
+// OP_EFINAL adds the partials in index order, so the result does not depend on the launch
+// shape or on the other pairs of a launch.
+
+__device__ __forceinline__ void add_part(EnergyPart &a, const EnergyPart &b)
+{
+    a.energy += b.energy;
+    a.data += b.data;
+    a.residual += b.residual;
+    a.nonfinite += b.nonfinite;
+    a.pixels += b.pixels;
+}
+
+// Reduces each work-item's partial over the 256 work-items of a tick workgroup; the total
+// is valid in work-item 0.
+__device__ __forceinline__ EnergyPart reduce_part(EnergyPart p, int tid)
+{
+    __shared__ EnergyPart waves[NT / 64];
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        p.energy += __shfl_xor(p.energy, o);
+        p.data += __shfl_xor(p.data, o);
+        p.residual += __shfl_xor(p.residual, o);
+        p.nonfinite += __shfl_xor(p.nonfinite, o);
+        p.pixels += __shfl_xor(p.pixels, o);
+    }
+    if ((tid & 63) == 0) waves[tid >> 6] = p;
+    __syncthreads();
+    EnergyPart r = waves[0];
+#pragma unroll
+    for (int k = 1; k < NT / 64; k++) add_part(r, waves[k]);
+    return r;
+}
+
+// One chunk of the level: roles as after the warp's last inner iteration (u and v_ the
+// current buffers, u_prev the other u buffer).
+template <class S, bool MASKED, typename T>
+__device__ __forceinline__ void op_energy(const Tile &t, int roles, int chunk, float lambda, float alpha0,
+                                          float alpha1)
+{
+    const int tid = threadIdx.y * S::BX + threadIdx.x;
+    const float *__restrict__ u = slot(t, slot_u(rcur(roles, G_U)));
+    const float *__restrict__ up = slot(t, slot_u(roth(roles, G_U)));
+    const float2 *__restrict__ v = slotp<float2>(t, slot_v(rcur(roles, G_V)));
+    const float *__restrict__ ul = slot(t, PL_ULAST);
+    const T *__restrict__ Iu = (const T *)slot(t, PL_IU);
+    const T *__restrict__ Iz = (const T *)slot(t, PL_IZ);
+    const T *__restrict__ at = (const T *)slot(t, PL_A);
+    const T *__restrict__ bt = (const T *)slot(t, PL_B);
+    const T *__restrict__ ct = (const T *)slot(t, PL_C);
+    const float *__restrict__ mask = t.mask;
+    const int w = t.w, h = t.h, s = t.s;
+    const long n = (long)w * h;
+    EnergyPart p{0.0, 0.0, 0.0, 0u, 0u};
+#pragma unroll
+    for (int k = 0; k < kEnergyChunk / NT; k++) {
+        const long i = (long)chunk * kEnergyChunk + k * NT + tid;
+        if (i >= n) break;
+        const int y = (int)(i / w), x = (int)(i - (long)y * w);
+        const long pos = (long)y * s + x;
+        bool hasR, hasD;
+        if (MASKED) {
+            if (mask[pos] == 0.f) continue;
+            hasR = x + 1 < w && mask[pos + 1] != 0.f;
+            hasD = y + 1 < h && mask[pos + s] != 0.f;
+        } else {
+            if (outside_circle(x, y, w, h)) continue;
+            hasR = x + 1 < w;
+            hasD = y + 1 < h;
+        }
+        const double u0 = u[pos];
+        const float2 v0 = v[pos];
+        double ux, uy, g0, g1, g2, g3;
+        if (hasR) {
+            const float2 vr = v[pos + 1];
+            ux = ((double)u[pos + 1] - u0) - (double)v0.x;
+            g0 = (double)vr.x - (double)v0.x;
+            g3 = (double)vr.y - (double)v0.y;
+        } else {
+            const double uL = pos >= 1 ? (double)u[pos - 1] : 0.0;
+            const float2 vl = pos >= 1 ? v[pos - 1] : make_float2(0.f, 0.f);
+            ux = (u0 - uL) - (double)v0.x;
+            g0 = (double)v0.x - (double)vl.x;
+            g3 = (double)v0.y - (double)vl.y;
+        }
+        if (hasD) {
+            const float2 vd = v[pos + s];
+            uy = ((double)u[pos + s] - u0) - (double)v0.y;
+            g1 = (double)vd.y - (double)v0.y;
+            g2 = (double)vd.x - (double)v0.x;
+        } else {
+            const double uU = pos >= s ? (double)u[pos - s] : 0.0;
+            const float2 vu = pos >= s ? v[pos - s] : make_float2(0.f, 0.f);
+            uy = (u0 - uU) - (double)v0.y;
+            g1 = (double)v0.y - (double)vu.y;
+            g2 = (double)v0.x - (double)vu.x;
+        }
+        const double a = ldT(at, pos), b = ldT(bt, pos), c = ldT(ct, pos);
+        const double tx = a * ux + c * uy, ty = c * ux + b * uy;
+        const double rho = (double)ldT(Iu, pos) * (u0 - (double)ul[pos]) + (double)ldT(Iz, pos);
+        const double data = (double)lambda * fabs(rho);
+        const double term = data + (double)alpha1 * sqrt(tx * tx + ty * ty) +
+                            (double)alpha0 * sqrt(g0 * g0 + g1 * g1 + g2 * g2 + g3 * g3);
+        const double res = fabs(u0 - (double)up[pos]);
+        p.pixels++;
+        if (__builtin_isfinite(term) && __builtin_isfinite(res)) {
+            p.energy += term;
+            p.data += data;
+            p.residual += res;
+        } else {
+            p.nonfinite++;
+        }
+    }
+    const EnergyPart r = reduce_part(p, tid);
+    if (tid == 0) t.d->mon_part[chunk] = r;
+}
+
+// The partials of the level's OP_ENERGY, added in index order into record (level, warp).
+template <class S>
+__device__ __forceinline__ void op_efinal(const Tile &t, int level, int warp)
+{
+    const int tid = threadIdx.y * S::BX + threadIdx.x;
+    const int n = energy_chunks(t.w, t.h);
+    const EnergyPart *__restrict__ part = t.d->mon_part;
+    EnergyPart p{0.0, 0.0, 0.0, 0u, 0u};
+    for (int j = tid; j < n; j += NT) add_part(p, part[j]);
+    const EnergyPart r = reduce_part(p, tid);
+    if (tid == 0) t.d->mon_rec[mon_slot(level, warp)] = r;
+}
+
 // ------------------------------------------------------------------ the tick kernel
 //
 // One launch executes one op for each entry of `A`: entry i owns workgroups
@@ -1481,6 +1620,13 @@ __global__ __launch_bounds__(NT) void k_tick(const PairDesc *__restrict__ D, con
         break;
     case OP_SYNC:
         op_sync<S>(t, level, E.roles);
+        break;
+    case OP_ENERGY:
+        if (E.masked) op_energy<S, true, T>(t, E.roles, local, A.lambda, A.alpha0, A.alpha1);
+        else op_energy<S, false, T>(t, E.roles, local, A.lambda, A.alpha0, A.alpha1);
+        break;
+    case OP_EFINAL:
+        op_efinal<S>(t, level, E.it0);
         break;
     default:
         break;

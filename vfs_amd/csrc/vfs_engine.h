@@ -124,6 +124,25 @@ __host__ __device__ inline int slot_p(int k) { return PL_P + 2 * k; }
 __host__ __device__ inline int slot_q(int k) { return PL_Q + 4 * k; }
 __host__ __device__ inline int slot_uv(int roles, bool current) { return ((roles & RB_UV) != 0) == current ? PL_UV1 : PL_UV0; }
 
+// Energy / residual monitor (OP_ENERGY, OP_EFINAL): FP64 sums over the processed pixels
+// of one pair at the end of a warp's inner iterations.  energy = data + alpha1 |T (grad u -
+// v_)| + alpha0 |grad v_|, data = lambda |Iu (u - u_last) + Iz|, residual = |u - u_prev|
+// (the last iteration's change); pixels whose terms are not finite are counted in
+// `nonfinite` and left out of the sums.  A record is written per (level, warp) slot.
+struct EnergyPart {
+    double energy, data, residual;
+    unsigned int nonfinite, pixels;
+};
+static_assert(sizeof(EnergyPart) == 32, "EnergyPart layout");
+// Pixels per OP_ENERGY workgroup (256 work-items, 4 pixels each, in row-major order of the
+// level): the partial sums, and so the result, do not depend on the launch's tile shape.
+constexpr int kEnergyChunk = 1024;
+// Warp slots per level in a pair's record array (later warps share the last slot).
+constexpr int kMonWarps = 128;
+__host__ __device__ inline int mon_slot(int level, int warp) { return level * kMonWarps + (warp < kMonWarps ? warp : kMonWarps - 1); }
+// Partial sums of an OP_ENERGY launch for a level of w x h pixels.
+__host__ __device__ inline int energy_chunks(int w, int h) { return (int)(((long long)w * h + kEnergyChunk - 1) / kEnergyChunk); }
+
 struct PairDesc {
     float *arena;
     long long plane;       // floats per plane slot (>= s0 * h0, multiple of 64)
@@ -139,6 +158,8 @@ struct PairDesc {
     // (stereotgv.cu:449-477) and k1 = alpha1 sigma / 3 (float product, correctly rounded
     // division); the multi-iteration ops read them
     float4 sched[kMaxSched];
+    EnergyPart *mon_rec;   // kMaxLevels x kMonWarps records (mon_slot)
+    EnergyPart *mon_part;  // energy_chunks(w0, h0) partial sums of the running OP_ENERGY
 };
 
 // Scalars of one inner iteration (schedule of stereotgv.cu:449-477).
@@ -173,7 +194,9 @@ enum OpKind : int {
     OP_ITER4 = 12,  // four fused TGV iterations (recomputed halo)
     OP_ENDPREP = 13, // END of a warp and PREP of the next one at the same level
     OP_ITER8 = 14,  // eight fused TGV iterations (recomputed halo)
-    OP_SYNC = 15    // spare buffers of u, u_, v_ take the current ones beyond the level's extent
+    OP_SYNC = 15,   // spare buffers of u, u_, v_ take the current ones beyond the level's extent
+    OP_ENERGY = 16, // energy / residual partial sums of the level (kEnergyChunk pixels per workgroup)
+    OP_EFINAL = 17  // one workgroup per pair: sums the partials in order into the (level, warp) record
 };
 
 // Iterations of an inner-iteration op.
@@ -207,7 +230,8 @@ struct TickEntry {
     unsigned char kind, level, masked, flags;
     unsigned int roles;        // buffer roles before the op (see Group)
     float sigma, tau, mu;      // ITER: step sizes of the iteration
-    int it0;                   // ITER2 / ITER4 / ITER8: index of the first iteration in the warp
+    int it0;                   // ITER2 / ITER4 / ITER8: index of the first iteration in the warp;
+                               // EFINAL: the warp of the record
 };
 static_assert(sizeof(TickEntry) == 56, "TickEntry layout");
 
@@ -227,6 +251,8 @@ inline bool op_flat(int kind) { return kind == OP_ZERO || kind == OP_SYNC; }
 // Workgroups an op of `kind` at `level` needs for a pair with dims w/h/s in tile `shape`.
 inline int op_tiles(int kind, int level, const int *w, const int *h, const int *s, int shape)
 {
+    if (kind == OP_ENERGY) return energy_chunks(w[level], h[level]);
+    if (kind == OP_EFINAL) return 1;
     const int tl = op_tile_level(kind, level);
     const int ww = op_flat(kind) ? s[0] : w[tl];
     const ShapeDim d = kShape[shape_index(shape)];

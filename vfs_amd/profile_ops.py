@@ -4,7 +4,8 @@ Every op of a solve program is one `k_tick` dispatch when the solve is launched 
 a graph (option "graph" = 0), in program order.  This module rebuilds the cold masked
 program's op list (the same sequence the library issues: zero, calibration warp,
 pyramid, then per level the tensor and, per warp, prep / derivatives / inner iterations
-/ end, each end but the level's last fused with the next warp's prep, then the upscale)
+/ end, each end but the level's last fused with the next warp's prep, the energy record
+(ENERGY, EFINAL) before the level's last end, then the upscale)
 and attributes each dispatch of one solve to its op and level.
 Levels that run several inner iterations per launch (`--per-launch`, one count per level
 from level 0, as the tuner reports them) issue `k_iterk` dispatches: ITER8 / ITER4 / ITER2 while
@@ -32,7 +33,9 @@ def inner_ops(iters: int, per_launch: int) -> list[str]:
     return out
 
 
-def program_ops(levels: int, warps: int, iters: int, per_launch=()) -> list[tuple[str, int]]:
+def program_ops(levels: int, warps: int, iters: int, per_launch=(), energy: int = 1) -> list[tuple[str, int]]:
+    """Ops of the cold program; `energy` as the solver option (0 none, 1 after each level's
+    last warp, 2 after every warp)."""
     ops = [("ZERO", 0), ("CALIB", 0)] + [("DOWN", l) for l in range(1, levels)] + [("CINIT", levels - 1)]
     for l in range(levels - 1, -1, -1):
         ops.append(("TENSOR", l))
@@ -40,16 +43,19 @@ def program_ops(levels: int, warps: int, iters: int, per_launch=()) -> list[tupl
         inner = [(name, l) for name in inner_ops(iters, k)]
         for wi in range(warps):
             ops += ([("PREP", l)] if wi == 0 else []) + [("DERIV", l)] + inner
+            if energy == 2 or (energy == 1 and wi + 1 == warps):
+                ops += [("ENERGY", l), ("EFINAL", l)]
             ops.append(("ENDPREP" if wi + 1 < warps else "END", l))
         if l > 0:
             ops.append(("UP", l))
     return ops
 
 
-def summarize(trace_csv: str, levels: int, warps: int, iters: int, skip_tail: int = 0, per_launch=()) -> str:
+def summarize(trace_csv: str, levels: int, warps: int, iters: int, skip_tail: int = 0, per_launch=(),
+              energy: int = 1) -> str:
     rows = [r for r in csv.DictReader(open(trace_csv)) if r["Kernel_Name"].startswith(("k_tick", "k_iterk"))]
     rows.sort(key=lambda r: int(r["Start_Timestamp"]))
-    seq = program_ops(levels, warps, iters, per_launch)
+    seq = program_ops(levels, warps, iters, per_launch, energy)
     n = len(seq)
     end = len(rows) - skip_tail
     if end < n:
@@ -95,9 +101,10 @@ def main():
     ap.add_argument("--iters", type=int, default=50)
     ap.add_argument("--skip-tail", type=int, default=0)
     ap.add_argument("--per-launch", default="", help="inner iterations per launch of each level, from level 0")
+    ap.add_argument("--energy", type=int, default=1, help="the solver's energy option (0, 1, 2)")
     a = ap.parse_args()
     per_launch = [int(x) for x in a.per_launch.split(",") if x]
-    print(summarize(a.trace, a.levels, a.warps, a.iters, a.skip_tail, per_launch))
+    print(summarize(a.trace, a.levels, a.warps, a.iters, a.skip_tail, per_launch, a.energy))
 
 
 if __name__ == "__main__":
