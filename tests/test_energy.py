@@ -55,8 +55,10 @@ def test_energy_matches_oracle_function_on_the_gpu_planes(masked, storage):
     rec = g.energy(0, 0)
     u = g.read_plane("u.other" if masked else "u", 0)
     planes = {n: g.read_plane(n, 0) for n in ("v_", "Iu", "Iz", "u_last", "a", "b", "c")}
+    # the solver's parameters are floats (stereotgv.h:107-109)
+    f = lambda v: float(np.float32(v))
     ref = O.k_energy(u, u, planes["v_"], planes["Iu"], planes["Iz"], planes["u_last"], planes["a"], planes["b"],
-                     planes["c"], P["lam"], P["alpha0"], P["alpha1"], pair["mask"] if masked else None)
+                     planes["c"], f(P["lam"]), f(P["alpha0"]), f(P["alpha1"]), pair["mask"] if masked else None)
     print(f"{storage} masked={masked}: gpu {rec}, oracle function {ref}")
     assert rec["pixels"] == ref["pixels"] > 0 and rec["nonfinite"] == 0
     # masked: the exact planes; unmasked, the warp end rewrites u as u_last + (u - u_last),

@@ -921,25 +921,28 @@ int copy_out(vfs_solver *s, const void *dev, int nch, int level, float *out, int
 // After the last solve of the pair has finished: VFS_ERR_NUMERIC if one of its energy
 // records counted non-finite values (a zero trajectory vector inside the mask makes the
 // derivative NaN there, computeDerivatives.cu:99-104), else VFS_OK.  Reads the records
-// once per solve.
+// once per solve, a few at a time into a buffer on the stack (no allocation).
 int check_numeric(vfs_solver *s)
 {
     if (s->mon_slots.empty()) return VFS_OK;
     if (s->numeric == 0) {
-        const int lo = *std::min_element(s->mon_slots.begin(), s->mon_slots.end());
-        const int hi = *std::max_element(s->mon_slots.begin(), s->mon_slots.end());
-        std::vector<vfs::EnergyPart> rec((size_t)(hi - lo + 1));
-        HIPCHK(s, hipMemcpyAsync(rec.data(), s->mon + lo, sizeof(vfs::EnergyPart) * rec.size(), hipMemcpyDeviceToHost,
-                                 s->stream));
-        HIPCHK(s, hipStreamSynchronize(s->stream));
+        constexpr size_t kBuf = 16;
+        vfs::EnergyPart rec[kBuf];
         s->numeric = 1;
-        for (int k : s->mon_slots) {
-            const vfs::EnergyPart &r = rec[(size_t)(k - lo)];
-            if (r.nonfinite > 0) {
+        for (size_t i0 = 0; i0 < s->mon_slots.size() && s->numeric > 0; i0 += kBuf) {
+            const size_t n = std::min(kBuf, s->mon_slots.size() - i0);
+            for (size_t k = 0; k < n; k++)
+                HIPCHK(s, hipMemcpyAsync(&rec[k], s->mon + s->mon_slots[i0 + k], sizeof(vfs::EnergyPart),
+                                         hipMemcpyDeviceToHost, s->stream));
+            HIPCHK(s, hipStreamSynchronize(s->stream));
+            for (size_t k = 0; k < n; k++) {
+                if (rec[k].nonfinite == 0) continue;
+                const int slot = s->mon_slots[i0 + k];
                 s->numeric = -1;
-                s->err = "the solve produced " + std::to_string(r.nonfinite) + " non-finite values at level " +
-                         std::to_string(k / vfs::kMonWarps) + " (warp " + std::to_string(k % vfs::kMonWarps) +
-                         "); a zero trajectory vector inside the processed region gives NaN derivatives";
+                fail(s, VFS_ERR_NUMERIC,
+                     "the solve produced %u non-finite values at level %d (warp %d); a zero trajectory vector "
+                     "inside the processed region gives NaN derivatives",
+                     rec[k].nonfinite, slot / vfs::kMonWarps, slot % vfs::kMonWarps);
                 break;
             }
         }

@@ -173,6 +173,24 @@ __device__ __forceinline__ float texel(const T *__restrict__ img, int w, int h, 
     return Cv<T>::f(img[(size_t)mir(y, h) * s + mir(x, w)]);
 }
 
+// Correctly rounded single-precision square root (the reference's sqrtf, IEEE-rounded):
+// the hardware square root is within one ulp of it; the rounding boundaries on either
+// side of that result (midpoints to the neighbouring floats, 25 significant bits) are
+// squared exactly in double and compared with x.
+__device__ __forceinline__ float sqrt_rn(float x)
+{
+    const float r = __builtin_sqrtf(x);
+    if (!(x > 0.f) || __builtin_isinf(x)) return r;  // 0, negative, infinite, NaN
+    const double dx = (double)x;
+    const float up = __int_as_float(__float_as_int(r) + 1);
+    const double mu = 0.5 * ((double)r + (double)up);
+    if (mu * mu < dx) return up;
+    const float dn = __int_as_float(__float_as_int(r) - 1);
+    const double md = 0.5 * ((double)dn + (double)r);
+    if (md * md > dx) return dn;
+    return r;
+}
+
 // Circle gate of the unmasked solve (updateDualVariables.cu:13-22, thresholdingL1.cu:10-19,
 // updatePrimalVariables.cu:14-23): true when sqrtf((y - h/2)^2 + (x - w/2)^2) >= w / 2.2,
 // evaluated in float with the correctly rounded square root, as the reference does.
@@ -181,7 +199,7 @@ __device__ __forceinline__ bool outside_circle(int x, int y, int w, int h)
     const float desired = (float)w / 2.20f;
     const float hw = (float)w / 2.0f, hh = (float)h / 2.0f;
     const float dy = (float)y - hh, dx = (float)x - hw;
-    return __fsqrt_rn(dy * dy + dx * dx) >= desired;
+    return sqrt_rn(dy * dy + dx * dx) >= desired;
 }
 
 __device__ __forceinline__ float2 ld0(const float2 *__restrict__ b, long i)
@@ -392,7 +410,7 @@ __device__ __forceinline__ void op_tensor(const Tile &t, int level, int xblock, 
             const float s10 = smoothed<T, MASKED>(I0, mask, t.w, t.h, t.s, x + 1, y);
             const float gx = s11 - s01;
             const float gy = s11 - s10;
-            const float absg = sqrtf(gx * gx + gy * gy);
+            const float absg = sqrt_rn(gx * gx + gy * gy);
             float nx = gx / absg, ny = gy / absg;
             if (absg < 1e-8f) {
                 nx = 1.f;
@@ -451,7 +469,7 @@ __device__ __forceinline__ void op_deriv(const Tile &t, int level)
     {
         const long pos = (long)y * t.s + x;
         const float2 v = tv[pos];
-        const float r = sqrtf(v.x * v.x + v.y * v.y);
+        const float r = sqrt_rn(v.x * v.x + v.y * v.y);
         const float dx = v.x / r, dy = v.y / r;
         const float X = (float)x, Y = (float)y;
         const int w = t.w, h = t.h, s = t.s;
@@ -588,12 +606,13 @@ __device__ __forceinline__ void dual_load(DualIn &r, int x, int y, int w, int h,
 }
 
 // updateDualVariables.cu:152-236 (masked) / :28-98 (unmasked): ascent on p and q with
-// unit-ball projection, then Tp = T p (thresholdingL1.cu:242-243).
+// unit-ball projection, then Tp = T p (thresholdingL1.cu:242-243).  `gate_out`: unmasked
+// form, the pixel lies outside the circle gate (p = q = 0 there).
 template <bool MASKED>
-__device__ __forceinline__ void dual_step(const DualIn &r, int x, int y, int w, int h, const IterParams &ip,
-                                          float2 &pn, float4 &qn, float2 &tpn)
+__device__ __forceinline__ void dual_step(const DualIn &r, bool gate_out, const IterParams &ip, float2 &pn, float4 &qn,
+                                          float2 &tpn)
 {
-    if (!MASKED && outside_circle(x, y, w, h)) {
+    if (!MASKED && gate_out) {
         pn = make_float2(0.f, 0.f);
         qn = make_float4(0.f, 0.f, 0.f, 0.f);
     } else {
@@ -759,10 +778,11 @@ __device__ __forceinline__ void op_iter(const Tile &t, int roles, const IterPara
     // ---- loads: the tile pixels (rows threadIdx.y + BY * r) + one halo position
     DualIn in[RPT];
     float uu[RPT], iu[RPT], iz[RPT], tpl[RPT], tpu[RPT];
-    bool hasL[RPT], hasU[RPT];
+    bool hasL[RPT], hasU[RPT], gout[RPT];
 #pragma unroll
     for (int r = 0; r < RPT; r++) {
         const int y = y0 + threadIdx.y + BY * r;
+        gout[r] = !MASKED && outside_circle(x, y, w, h);
         dual_load<MASKED, T>(in[r], x, y, w, h, s, mask, u_, v_, at, bt, ct, p_in, q_in, ip.first);
         const long pos = in[r].pos;
         uu[r] = u[pos];
@@ -808,7 +828,7 @@ __device__ __forceinline__ void op_iter(const Tile &t, int roles, const IterPara
         if (!in[r].act) continue;
         float2 pn, tpn;
         float4 qn;
-        dual_step<MASKED>(in[r], x, y, w, h, ip, pn, qn, tpn);
+        dual_step<MASKED>(in[r], gout[r], ip, pn, qn, tpn);
         const int ex = threadIdx.x + 1, ey = threadIdx.y + BY * r + 1;
         sP[ey][ex] = pn;
         sQ[ey][ex] = qn;
@@ -820,7 +840,7 @@ __device__ __forceinline__ void op_iter(const Tile &t, int roles, const IterPara
     if (has_halo && hin.act) {
         float2 pn, tpn;
         float4 qn;
-        dual_step<MASKED>(hin, hx, hy, w, h, ip, pn, qn, tpn);
+        dual_step<MASKED>(hin, !MASKED && outside_circle(hx, hy, w, h), ip, pn, qn, tpn);
         sP[hey][hex] = pn;
         sQ[hey][hex] = qn;
         sT[hey][hex] = tpn;
@@ -834,7 +854,7 @@ __device__ __forceinline__ void op_iter(const Tile &t, int roles, const IterPara
         const int y = y0 + threadIdx.y + BY * r;
         if (!q.ok) continue;
         const long pos = q.pos;
-        if (!MASKED && outside_circle(x, y, w, h)) {
+        if (!MASKED && gout[r]) {
             us[pos] = 0.f;
             u_s[pos] = 0.f;
             v_s[pos] = make_float2(0.f, 0.f);
@@ -1059,7 +1079,7 @@ __device__ __forceinline__ void op_iterk(const Tile &t, int roles, const float4 
     // Per own position, once per launch (the mask does not change): in the image, in the
     // mask, and which of the four neighbours are in the image and the mask (read by the
     // path with boundary tests only).
-    enum : unsigned { F_IN = 1, F_ACT = 2, F_R = 4, F_D = 8, F_L = 16, F_U = 32 };
+    enum : unsigned { F_IN = 1, F_ACT = 2, F_R = 4, F_D = 8, F_L = 16, F_U = 32, F_OUT = 64 };
     unsigned fl[NK];
 #pragma unroll
     for (int k = 0; k < NK; k++) {
@@ -1073,6 +1093,7 @@ __device__ __forceinline__ void op_iterk(const Tile &t, int roles, const float4 
             if (y + 1 < h && row + 1 < B::RH && act(i + B::RW)) fl[k] |= F_D;
             if (x >= 1 && col >= 1 && act(i - 1)) fl[k] |= F_L;
             if (y >= 1 && row >= 1 && act(i - B::RW)) fl[k] |= F_U;
+            if (!MASKED && outside_circle(x, y, w, h)) fl[k] |= F_OUT;  // the unmasked form's gate
         }
     }
 
@@ -1127,7 +1148,7 @@ __device__ __forceinline__ void op_iterk(const Tile &t, int roles, const float4 
                 rr.qo = make_float4(rT<T>(qo.x), rT<T>(qo.y), rT<T>(qo.z), rT<T>(qo.w));
                 float2 pn, tpn;
                 float4 qn;
-                dual_step<MASKED>(rr, x, y, w, h, ip, pn, qn, tpn);
+                dual_step<MASKED>(rr, !FULL && (fl[k] & F_OUT) != 0, ip, pn, qn, tpn);
                 o[k].p = pn;  // the primal step of this iteration reads them unrounded
                 L.q[i] = qn;
                 L.tp[i] = tpn;
@@ -1166,7 +1187,7 @@ __device__ __forceinline__ void op_iterk(const Tile &t, int roles, const float4 
                 if (x < x0 - 2 * r || y < y0 - 2 * r || x > x0 + B::TW - 1 + r || y > y0 + B::TH - 1 + r) continue;
                 float un, u_s;
                 float2 vs;
-                if (!FULL && !MASKED && outside_circle(x, y, w, h)) {
+                if (!FULL && !MASKED && (fl[k] & F_OUT)) {
                     un = 0.f;
                     u_s = 0.f;
                     vs = make_float2(0.f, 0.f);
