@@ -123,6 +123,15 @@ int vfs_copy_disparity_vis(vfs_solver *s, float *out, int pitch, float flowScale
  * warped I1 (valid at level 0 after a solve). */
 int vfs_copy_warped_image(vfs_solver *s, float *out, int pitch);
 
+/* No counterpart in the reference; for callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid hipMalloc): the
+ * same copy as vfs_copy_warped_image, which in FP16 / BF16 storage allocates a temporary
+ * float plane per call, with that plane staged in a static device array of 16 MiB (4 Mi
+ * floats) that the library holds on each device until the process ends; it calls no
+ * allocation routine.  A warped image of stride x height above 4 Mi floats is rejected
+ * with VFS_ERR_ARG.  Calls that overlap on one device share the array and race. */
+int vfs_copy_warped_image_noalloc(vfs_solver *s, float *out, int pitch);
+
 /* No counterpart in the reference (its checkCudaErrors exits the process): text of the
  * last failure on this handle, "" if none.  Valid until the next call on the handle. */
 const char *vfs_last_error(const vfs_solver *s);

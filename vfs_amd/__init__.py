@@ -43,6 +43,7 @@ EXPORTS = [
     ("vfs_copy_stereo", ctypes.c_int, [ctypes.c_void_p, _c_float_p, ctypes.c_int, _c_float_p, ctypes.c_int]
      + [ctypes.c_float] * 11),
     ("vfs_copy_warped_image", ctypes.c_int, [ctypes.c_void_p, _c_float_p, ctypes.c_int]),
+    ("vfs_copy_warped_image_noalloc", ctypes.c_int, [ctypes.c_void_p, _c_float_p, ctypes.c_int]),
     ("vfs_copy_disparity_vis", ctypes.c_int, [ctypes.c_void_p, _c_float_p, ctypes.c_int, ctypes.c_float]),
     ("vfs_last_error", ctypes.c_char_p, [ctypes.c_void_p]),
     ("vfs_get_level_dims", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, _c_int_p, _c_int_p, _c_int_p]),

@@ -1273,6 +1273,70 @@ int vfs_copy_warped_image(vfs_solver *s, float *out, int pitch)
     return rc ? rc : check_numeric(s);
 }
 
+}  // extern "C"
+
+// ---- the warped image without allocation
+//
+// vfs_copy_warped_image in 16-bit storage widens the plane into a temporary float plane
+// that it allocates per call.  vfs_copy_warped_image_noalloc stages it in `scratch`, a
+// file-scope device array of kNoAllocFloats floats instead (every device the library runs
+// on holds its own copy until the process ends).  In host code `scratch` names only a
+// host-side stand-in: its device address, looked up once per device with
+// hipGetSymbolAddress, is what kernels and copies take.  Nothing in it must start at zero
+// (the widen kernel writes every element the copy reads).  Every call on a device shares
+// that device's array, so calls that overlap on one device race.
+namespace {
+
+constexpr size_t kNoAllocFloats = (size_t)4 << 20;  // 16 MiB: planes up to 4 Mi floats
+static __device__ __align__(256) float scratch[kNoAllocFloats];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+float *g_scratch_dev[kMaxDevices] = {};  // device address of `scratch` per device
+
+// The device address of `scratch` on s->device (current); looked up on the first call there.
+int scratch_address(vfs_solver *s, float **out)
+{
+    if (s->device < 0 || s->device >= kMaxDevices)
+        return fail(s, VFS_ERR_DEVICE, "device %d beyond the %d devices of the staging table", s->device, kMaxDevices);
+    std::lock_guard<std::mutex> g(g_scratch_mu);
+    if (!g_scratch_dev[s->device]) {
+        void *p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p)
+            return fail(s, VFS_ERR_DEVICE, "hipGetSymbolAddress(scratch): %s", hipGetErrorString(e));
+        g_scratch_dev[s->device] = (float *)p;
+    }
+    *out = g_scratch_dev[s->device];
+    return VFS_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int vfs_copy_warped_image_noalloc(vfs_solver *s, float *out, int pitch)
+{
+    if (!s) return VFS_ERR_ARG;
+    if (!s->initialized) return fail(s, VFS_ERR_STATE, "copy before initialize");
+    if (!out || pitch < s->width) return fail(s, VFS_ERR_ARG, "bad output pointer or pitch");
+    int rc = set_device(s);
+    if (rc) return rc;
+    const void *src = slot_ptr(s, vfs::PL_I1W);
+    if (s->storage == vfs::kF32) return copy_result(s, src, 1, 0, out, pitch);  // a float plane: no staging
+    const Dims d0 = s->lv[0];
+    const size_t need = (size_t)d0.s * d0.h;
+    if (need > kNoAllocFloats)
+        return fail(s, VFS_ERR_ARG, "warped image of %zu floats (stride x height) exceeds the %zu-float staging "
+                                    "of the no-allocation copy", need, kNoAllocFloats);
+    float *stage = nullptr;
+    if ((rc = scratch_address(s, &stage))) return rc;
+    if ((rc = wait_done(s))) return rc;
+    vfs::launch_widen(src, s->storage, 1, stage, d0, s->stream);
+    if ((rc = check_launch(s))) return rc;
+    return copy_result(s, stage, 1, 0, out, pitch);
+}
+
 int vfs_copy_stereo(vfs_solver *s, float *depth, int depth_pitch, float *X, int X_pitch, float focalx, float focaly,
                     float cx, float cy, float d1, float d2, float d3, float d4, float t1, float t2, float t3)
 {
