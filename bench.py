@@ -54,10 +54,17 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU-oracle baseline leg")
     ap.add_argument("--iter-reps", type=int, default=200)
     ap.add_argument("--no-graph", action="store_true", help="launch kernels one by one instead of a HIP graph")
-    ap.add_argument("--quick", action="store_true", help="plain form only (no batched / reduced-storage legs)")
+    ap.add_argument("--quick", action="store_true", help="plain form only (no served / batched / reduced-storage legs)")
+    ap.add_argument("--legs", default="served,batched,reduced",
+                    help="legs beside the plain form, comma-separated: served, batched, reduced")
     ap.add_argument("--batch-steps", type=int, default=2)
-    ap.add_argument("--serve-frames", type=int, default=25, help="frames per stream in the serving leg")
+    ap.add_argument("--serve-frames", type=int, default=100,
+                    help="frames per stream in the serving leg (the first is the stream's cold solve)")
     ap.add_argument("--serve-only", action="store_true", help="run only the serving leg (development aid)")
+    ap.add_argument("--no-serve-check", action="store_true",
+                    help="skip the FP64 oracle check of the served frames")
+    ap.add_argument("--reduced-storage", default="f32,f16,bf16",
+                    help="storage types of the reduced-storage leg, comma-separated")
     ap.add_argument("--shape", type=int, default=-1,
                     help="force one tile shape (0..6) instead of the per-level tuner (profiling aid)")
     ap.add_argument("--tuning", default="",
@@ -205,11 +212,26 @@ def main():
                                                    + (f", {k} iterations per launch" if k > 1 else ""))
     print(f"bench: tile tuning {out['tile_tuning']}", file=sys.stderr, flush=True)
 
-    if not args.quick:
-        out["served"] = measure_served(args, P, S, vfs_amd, pairs_per_s)
-        if not args.serve_only:
-            out["batched"] = measure_batched(args, P, S, vfs_amd)
-            out["reduced_storage"] = measure_reduced(args, P, S, vfs_amd, pair_checker=not args.no_cpu)
+    # the energy / residual monitor (option "energy" 1: one FP64 record per level) against
+    # the same solve without it
+    g.set_option("energy", 0)
+    g.time_solves(True, 2)
+    off = [g.time_solves(True, 1) for _ in range(args.steps)]
+    g.set_option("energy", 1)
+    on = [g.time_solves(True, 1) for _ in range(args.steps)]
+    out["energy_monitor"] = {"ms_per_solve_on": round(float(np.median(on)), 4),
+                             "ms_per_solve_off": round(float(np.median(off)), 4),
+                             "overhead_pct": round(100.0 * (np.median(on) / np.median(off) - 1.0), 2),
+                             "records": "one per level (after its last warp), FP64 sums on the GPU"}
+    print(f"bench: energy monitor {out['energy_monitor']}", file=sys.stderr, flush=True)
+
+    legs = set() if args.quick else ({"served"} if args.serve_only else {l for l in args.legs.split(",") if l})
+    if "served" in legs:
+        out["served"] = measure_served(args, P, S, vfs_amd, pairs_per_s, W * H)
+    if "batched" in legs:
+        out["batched"] = measure_batched(args, P, S, vfs_amd)
+    if "reduced" in legs:
+        out["reduced_storage"] = measure_reduced(args, P, S, vfs_amd, pair_checker=not args.no_cpu)
 
     if not args.no_cpu:
         base, u_ref = cpu_baseline(wl, pair, P, S)
@@ -321,6 +343,22 @@ def measure_batched(args, P, S, vfs_amd):
     return res
 
 
+# HBM bytes per pixel and inner iteration of the 16-pair batch's level-0 multi-iteration
+# launches, from rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE (one summary per storage type,
+# written by vfs_amd.profile_ops --batch-traffic).
+BATCH_TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "pmc_traffic_batch16_848x800_{st}_L0.md")
+
+
+def profiled_batch_traffic(st):
+    import re
+    try:
+        with open(BATCH_TRAFFIC_PROFILE.format(st=st)) as f:
+            m = re.search(r"traffic_bytes_per_px_iter\s*=\s*([0-9.]+)", f.read())
+        return float(m.group(1)) if m else None
+    except OSError:
+        return None
+
+
 def measure_reduced(args, P, S, vfs_amd, pair_checker=True):
     """16 pairs at 848x800 (seeds 200+i) solved as one batch with FP32, FP16 and BF16
     storage; disparity error of pair 0 against the FP64 oracle per storage type."""
@@ -331,13 +369,16 @@ def measure_reduced(args, P, S, vfs_amd, pair_checker=True):
         ref = O.solve_pair(pairs[0], 5, 10, 50, P, precision="f64").copy1DDisparityToHost()
     m = pairs[0]["mask"] > 0
     out = {"workload": "16 pairs 848x800, L5 W10 I50, one batch per storage type"}
-    for st in ("f32", "f16", "bf16"):
+    for st in [x for x in args.reduced_storage.split(",") if x]:
         solvers = [_solver(vfs_amd, P, p, 5, 10, 50, storage=st) for p in pairs]
         b = vfs_amd.Batch(solvers)
         res = _time_batch(args, b, 16)
-        res["tile_tuning_L0"] = "%dx%d" % b.level_tuning(0)[0]
-        # bytes per pixel-iteration: FP32 100; 16-bit storage of p, q, a, b, c, Iu, Iz: 66
-        res["bytes_per_px_iter"] = 100 if st == "f32" else 66
+        (tw, th), _, k = b.level_tuning(0, with_per_launch=True)
+        res["tile_tuning_L0"] = f"{tw}x{th}, {k} iterations per launch"
+        # algorithmic bytes per pixel-iteration (FP32 100; 16-bit storage of p, q, a, b,
+        # c, Iu, Iz: 66) and, where profiled, the HBM bytes the counters measured
+        res["algorithmic_bytes_per_px_iter"] = 100 if st == "f32" else 66
+        res["measured_hbm_bytes_per_px_iter"] = profiled_batch_traffic(st)
         if ref is not None:
             solvers[0].copyImagesToDevice(pairs[0]["i0"], pairs[0]["i1"])
             solvers[0].solveStereoForwardMasked()
@@ -352,13 +393,13 @@ def measure_reduced(args, P, S, vfs_amd, pair_checker=True):
     return out
 
 
-def serving_streams(S, n_streams=32, distinct=4):
-    """32 camera streams, alternately 424x400 and 848x800 (seeds 300+s).  Each stream has
-    `distinct` synthetic frames along one smooth drift (<= 0.5 px per step); frame k of
-    the stream is distinct frame triangle(k), so consecutive frames always differ by one
-    drift step."""
+def serving_streams(S, n_streams=32, distinct=4, only=None):
+    """Camera streams alternately 424x400 and 848x800 (stream s: seed 300 + s).  Each
+    stream has `distinct` synthetic frames along one smooth drift (<= 0.5 px per step);
+    frame k of the stream is distinct frame triangle(k), so consecutive frames always
+    differ by one drift step.  `only`: the stream indices to build."""
     streams = []
-    for sidx in range(n_streams):
+    for sidx in range(n_streams) if only is None else only:
         W, H = (424, 400) if sidx % 2 == 0 else (848, 800)
         base = S.make_pair(W, H, 300 + sidx, H / 2.0, W / 21.2)
         fr = [base] + [S.drift_pair(base, k, 300 + sidx) for k in range(1, distinct)]
@@ -372,25 +413,29 @@ def tri(k, n):
     return m if m < n else period - m
 
 
-def prime_streams(srv, sids, streams):
+def prime_streams(srv, sids, streams, track=()):
     """Each stream's first frame (the full cold solve) so that the timed frames are the
-    steady state of a stream: warm-started from the stream's previous result."""
-    lat = []
+    steady state of a stream: warm-started from the stream's previous result.  Returns the
+    latencies and, for the streams in `track`, the frame's disparity."""
+    lat, u0 = [], {}
     for i, sid in enumerate(sids):
         f = streams[i][3][0]
-        srv.submit(sid, f["i0"], f["i1"], tag=-1 - i, want_result=False)
+        srv.submit(sid, f["i0"], f["i1"], tag=-1 - i, want_result=i in track)
     for r in srv.run_until_idle():
         lat.append(r["latency_ms"])
-    return np.array(lat)
+        if r["u"] is not None:
+            u0[-1 - r["tag"]] = r["u"]
+    return np.array(lat), u0
 
 
-def run_served(srv, sids, streams, n_frames, rate, seed=7, backlog=96):
+def run_served(srv, sids, streams, n_frames, rate, seed=7, backlog=96, track=()):
     """Poisson arrivals at `rate` frames/s, or (rate <= 0, saturation) arrivals faster
     than the server serves them: submissions interleaved with the server's steps keep
     `backlog` frames queued.  Each arrival is the next frame of a uniformly chosen stream
-    (frames 1.. of the stream, frame 0 was primed).  Returns latencies, their stream
-    indices, the wall time from the first arrival to the last result and the number of
-    warm-started frames."""
+    (frames 1 .. n_frames of the stream, frame 0 was primed).  Returns latencies, their
+    stream indices, the wall time from the first arrival to the last result, the number of
+    warm-started frames, the largest non-finite count and, for the streams in `track`, the
+    disparity of every frame by (stream, frame)."""
     rng = np.random.default_rng(seed)
     n_total = n_frames * len(sids)
     order = []
@@ -402,7 +447,9 @@ def run_served(srv, sids, streams, n_frames, rate, seed=7, backlog=96):
         left[i] -= 1
     times = np.zeros(n_total) if rate <= 0 else np.cumsum(rng.exponential(1.0 / rate, n_total))
     next_frame = [1] * len(sids)
-    lat, who, warm = [], [], 0
+    frame_of = {}
+    lat, who, warm, nonfinite = [], [], 0, 0
+    got = {}
     k = q = 0
     t0 = time.perf_counter()
     while True:
@@ -411,7 +458,8 @@ def run_served(srv, sids, streams, n_frames, rate, seed=7, backlog=96):
             i = order[k]
             W, H, base, fr = streams[i]
             f = fr[tri(next_frame[i], len(fr))]
-            srv.submit(sids[i], f["i0"], f["i1"], tag=k, want_result=False)
+            frame_of[k] = (i, next_frame[i])
+            srv.submit(sids[i], f["i0"], f["i1"], tag=k, want_result=i in track)
             next_frame[i] += 1
             k += 1
             q += 1
@@ -420,56 +468,137 @@ def run_served(srv, sids, streams, n_frames, rate, seed=7, backlog=96):
             lat.append(r["latency_ms"])
             who.append(order[r["tag"]])
             warm += r["warm"]
+            nonfinite = max(nonfinite, r["nonfinite"])
+            if r["u"] is not None:
+                got[frame_of[r["tag"]]] = r["u"]
         if k == n_total and act == 0 and q == 0:
             break
         if time.perf_counter() - t0 > 300:
             raise RuntimeError("serving run did not drain")
     wall = time.perf_counter() - t0
-    return np.array(lat), np.array(who), wall, warm
+    return np.array(lat), np.array(who), wall, warm, nonfinite, got
 
 
 def _pct(a, q):
     return round(float(np.percentile(a, q)), 2) if len(a) else None
 
 
-def measure_served(args, P, S, vfs_amd, plain_pairs_per_s):
+def _warm_frame_bytes(W, H, P, levels=(1, 0), warps=2, iters=20):
+    """Algorithmic bytes of one warm-started frame: the calibration warp and pyramid, the
+    downscaled start state, then per level warps x (iters x 100 + 104) + 44 bytes per pixel."""
+    dims, w, h = [], W, H
+    for _ in range(max(levels) + 1):
+        dims.append(w * h)
+        w, h = int(np.float32(w) / np.float32(P["fScale"])), int(np.float32(h) / np.float32(P["fScale"]))
+    return sum(dims[l] * (warps * (iters * B_ITER_FP32 + 104.0) + 44.0) for l in levels) + 16.0 * dims[0] + \
+        40.0 * sum(dims[1:])
+
+
+def check_served(P, streams, track, u0, got, n_frames, every=10):
+    """The FP64 oracle runs each tracked stream's sequence (cold frame 0, then warm W2 I20
+    from level 1, as the server) and is compared with the served frames at every
+    `every`-th frame and the last: mean and max |u_served - u_fp64| over the mask."""
+    from oracle import oracle as O
+    res = {}
+    for i in track:
+        W, H, base, fr = streams[i]
+        o = O.Oracle(W, H, P["beta"], P["gamma"], P["alpha0"], P["alpha1"], P["timestep_lambda"], P["lam"], 5,
+                     P["fScale"], 10, 50)
+        o.set_limit_range(P["limitRange"])
+        o.copyMaskToDevice(base["mask"])
+        o.loadVectorFields(base["tv"], base["cv"])
+        m = base["mask"] > 0
+        means, maxes = [], []
+        for k in range(n_frames + 1):
+            f = fr[tri(k, len(fr))]
+            o.copyImagesToDevice(f["i0"], f["i1"])
+            if k == 0:
+                o.solveStereoForwardMasked()
+            else:
+                o.solve_warm(1, 2, 20)
+            if k % every == 0 or k == n_frames:
+                u = u0[i] if k == 0 else got[(i, k)]
+                err = np.abs(u - o.copy1DDisparityToHost())[m]
+                means.append(float(err.mean()))
+                maxes.append(float(err.max()))
+        res[f"stream{i}_{W}x{H}"] = {"frames_checked": len(means), "mean_err_px_max_over_frames": float(f"{max(means):.3e}"),
+                                     "mean_err_px_avg": float(f"{np.mean(means):.3e}"),
+                                     "max_err_px": float(f"{max(maxes):.3e}")}
+    return res
+
+
+def measure_served(args, P, S, vfs_amd, plain_pairs_per_s, plain_px):
     """Serving: 32 streams (424x400 / 848x800), continuous batching, FP16 storage.  Each
     stream's first frame runs the full L5 W10 I50 solve (primed, reported apart); the
     timed frames are warm-started (W2 I20 at levels 1 and 0).  Saturation throughput
     (96 frames kept queued, so its latencies are queueing delays), then Poisson arrivals
-    at 50 % and 90 % of it."""
+    at 50 % and 90 % of it, and a saturation run of the 16 848x800 streams alone.  Streams
+    0 and 1 of the saturation run are checked frame by frame against the FP64 oracle."""
     streams = serving_streams(S)
+    n_timed = args.serve_frames - 1
     print("bench: serving streams ready", file=sys.stderr, flush=True)
-    out = {"workload": f"32 streams (424x400 / 848x800), {args.serve_frames} timed frames each after one cold "
-                       f"frame, warm start W2 I20 at levels 1-0 (cold frame L5 W10 I50), FP16 storage, "
-                       f"continuous batching"}
+    out = {"workload": f"32 streams (424x400 / 848x800), {args.serve_frames} frames each: one cold frame "
+                       f"(L5 W10 I50), then {n_timed} warm-started frames (W2 I20 at levels 1-0), FP16 storage, "
+                       f"continuous batching; frames cycle through 4 drift steps per stream"}
     res = {}
-    for label, rate in (("saturation", 0.0), ("rate_50", 0.5), ("rate_90", 0.9)):
+    track = (0, 1) if not (args.no_serve_check or args.no_cpu) else ()
+
+    def phase(label, strs, rate, track=()):
         srv = vfs_amd.Server(P, hbm_budget_bytes=64 << 30, max_streams=32, max_active=64, storage="f16")
-        sids = [srv.add_stream(b["mask"], b["tv"], b["cv"], 5, 10, 50, 1, 2, 20) for (W, H, b, fr) in streams]
-        cold = prime_streams(srv, sids, streams)
+        sids = [srv.add_stream(b["mask"], b["tv"], b["cv"], 5, 10, 50, 1, 2, 20) for (W, H, b, fr) in strs]
+        cold, u0 = prime_streams(srv, sids, strs, track)
         r = rate if rate <= 0 else rate * res["saturation"]["frames_per_s"]
         t_before = srv.stats()
-        lat, who, wall, warm = run_served(srv, sids, streams, args.serve_frames, r)
+        lat, who, wall, warm, nonfinite, got = run_served(srv, sids, strs, n_timed, r, track=track)
         st = srv.stats()
-        small = np.array([streams[i][0] == 424 for i in who])
+        small = np.array([strs[i][0] == 424 for i in who])
+        px = sum(strs[i][0] * strs[i][1] for i in who)
+        byts = sum(_warm_frame_bytes(strs[i][0], strs[i][1], P) for i in who)
         entry = {"offered_frames_per_s": round(r, 1) if r > 0 else None,
                  "frames_per_s": round(len(lat) / wall, 1),
+                 "Mpx_per_s": round(px / wall / 1e6, 1),
+                 "algorithmic_GBs": round(byts / wall / 1e9, 1),
+                 "roofline_frac": round(byts / wall / 1e9 / HBM_PEAK_GBS, 3),
                  "p50_ms": _pct(lat, 50), "p99_ms": _pct(lat, 99),
-                 "p50_ms_424x400": _pct(lat[small], 50), "p99_ms_424x400": _pct(lat[small], 99),
-                 "p50_ms_848x800": _pct(lat[~small], 50), "p99_ms_848x800": _pct(lat[~small], 99),
-                 "frames": int(len(lat)), "warm_frames": int(warm),
+                 "frames": int(len(lat)), "warm_frames": int(warm), "max_nonfinite": int(nonfinite),
                  "ops_per_launch": round((st["entries"] - t_before["entries"])
                                          / max(st["ticks"] - t_before["ticks"], 1), 2)}
-        if label == "saturation":
+        if small.any() and (~small).any():
+            entry.update({"p50_ms_424x400": _pct(lat[small], 50), "p99_ms_424x400": _pct(lat[small], 99),
+                          "p50_ms_848x800": _pct(lat[~small], 50), "p99_ms_848x800": _pct(lat[~small], 99)})
+        if rate <= 0:
             entry["cold_first_frames"] = {"frames": int(len(cold)), "p50_ms": _pct(cold, 50),
                                           "p99_ms": _pct(cold, 99)}
         res[label] = entry
         print(f"bench: served {label}: {entry}", file=sys.stderr, flush=True)
         srv.close()
+        return u0, got
+
+    u0, got = phase("saturation", streams, 0.0, track)
+    phase("rate_50", streams, 0.5)
+    phase("rate_90", streams, 0.9)
+    phase("saturation_848x800_only", [st for st in streams if st[0] == 848], 0.0)
     out.update(res)
-    out["vs_plain_per_frame_solve"] = round(res["saturation"]["frames_per_s"] / plain_pairs_per_s, 2)
+    sat, big = res["saturation"], res["saturation_848x800_only"]
+    out["vs_plain_per_frame_solve"] = round(sat["frames_per_s"] / plain_pairs_per_s, 2)
+    out["like_for_like"] = {
+        "848x800_frames_vs_plain": round(big["frames_per_s"] / plain_pairs_per_s, 2),
+        "Mpx_per_s_vs_plain": round(sat["Mpx_per_s"] / (plain_pairs_per_s * plain_px / 1e6), 2),
+        "work_per_848x800_frame_cold_over_warm": round(
+            _cold_bytes(848, 800, P) / _warm_frame_bytes(848, 800, P), 2),
+        "note": "warm frames do the W2 I20 schedule from level 1 and plain solves the full L5 W10 I50 one"}
+    if track:
+        out["fp64_check"] = check_served(P, streams, track, u0, got, n_timed)
+        print(f"bench: served FP64 check {out['fp64_check']}", file=sys.stderr, flush=True)
     return out
+
+
+def _cold_bytes(W, H, P, levels=5, warps=10, iters=50):
+    dims, w, h = [], W, H
+    for _ in range(levels):
+        dims.append(w * h)
+        w, h = int(np.float32(w) / np.float32(P["fScale"])), int(np.float32(h) / np.float32(P["fScale"]))
+    return sum(dims) * (warps * (iters * B_ITER_FP32 + 104.0) + 44.0) + 16.0 * dims[0] + 40.0 * (sum(dims) - dims[0])
 
 
 if __name__ == "__main__":

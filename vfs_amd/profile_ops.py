@@ -12,6 +12,7 @@ from level 0, as the tuner reports them) issue `k_iterk` dispatches: ITER8 / ITE
 they fit the iteration count, the rest one at a time.
 
     python -m vfs_amd.profile_ops TRACE.csv --levels 5 --warps 10 --iters 50 --per-launch 4,4,4,4,4 --skip-tail 100
+    python -m vfs_amd.profile_ops --batch-traffic FETCH.csv WRITE.csv --storage f16   # HBM bytes per px-iteration
 
 `--skip-tail` drops trailing dispatches that follow the solves (e.g. timing launches);
 the solve analysed is the last complete one before them.
@@ -93,7 +94,78 @@ def summarize(trace_csv: str, levels: int, warps: int, iters: int, skip_tail: in
     return "\n".join(out)
 
 
+def _counter_rows(path: str, counter: str):
+    """(dispatch id, kernel name, grid size, value) of one counter from a rocprofv3
+    counter_collection.csv."""
+    out = []
+    for r in csv.DictReader(open(path)):
+        name = r.get("Counter_Name", counter)
+        if name != counter:
+            continue
+        out.append((int(r["Dispatch_Id"]), r["Kernel_Name"], int(r["Grid_Size"]), float(r["Counter_Value"])))
+    return out
+
+
+def _iterk_k(name: str) -> int:
+    """K of a k_iterk<T, shape, K, half> kernel name (0 for other kernels)."""
+    if "k_iterk<" not in name:
+        return 0
+    args = name.split("k_iterk<", 1)[1].split(">", 1)[0].split(",")
+    return int(args[2])
+
+
+def batch_traffic(fetch_csv: str, write_csv: str, storage: str, pairs: int, px: int, copy_bytes: int,
+                  command: str) -> str:
+    """Summary of the HBM bytes per pixel and inner iteration of a batch's level-0
+    multi-iteration launches: the k_iterk dispatches of the largest grid with the largest
+    K, FETCH_SIZE and WRITE_SIZE per dispatch (KiB) scaled by what the same counters read
+    for the k_copy dispatches that move `copy_bytes` each way."""
+    import statistics as st
+    rows = {c: _counter_rows(p, c) for c, p in (("FETCH_SIZE", fetch_csv), ("WRITE_SIZE", write_csv))}
+    med = {}
+    scale = {}
+    sel = None
+    for c, rs in rows.items():
+        copies = [v for _, n, _, v in rs if "k_copy" in n]
+        scale[c] = copy_bytes / 1024.0 / st.median(copies)
+        iterk = [(n, g, v) for _, n, g, v in rs if _iterk_k(n) > 0]
+        kmax = max(_iterk_k(n) for n, _, _ in iterk)
+        gmax = max(g for n, g, _ in iterk if _iterk_k(n) == kmax)
+        vals = [v for n, g, v in iterk if _iterk_k(n) == kmax and g == gmax]
+        sel = (kmax, gmax, len(vals), sorted({n for n, g, _ in iterk if _iterk_k(n) == kmax and g == gmax}))
+        med[c] = (st.median(vals), min(vals), max(vals))
+    k, grid, n, names = sel
+    kib = med["FETCH_SIZE"][0] * scale["FETCH_SIZE"] + med["WRITE_SIZE"][0] * scale["WRITE_SIZE"]
+    per = kib * 1024.0 / (pairs * px * k)
+    lines = [f"# HBM traffic of the level-0 inner iterations of a {pairs}-pair batch at 848x800, {storage} storage", "",
+             "Command (two passes, one counter each; from the repository root on the GPU machine):", "",
+             "    " + command, "",
+             f"Dispatches analysed: {n} `k_iterk` launches of {k} iterations with grid {grid} work-items "
+             f"({', '.join(names)}); each covers {pairs} pairs of {px} pixels.", "",
+             "| counter | scale (k_copy, %d MiB each way) | median per dispatch (KiB) | min / max |" % (copy_bytes >> 20),
+             "|---|---|---|---|"]
+    for c in ("FETCH_SIZE", "WRITE_SIZE"):
+        m, lo, hi = med[c]
+        lines.append(f"| {c} | x {scale[c]:.3f} | {m:.1f} | {lo:.1f} / {hi:.1f} |")
+    lines += ["", f"HBM bytes per launch = {kib * 1024:.0f}; per pixel and iteration:", "",
+              f"traffic_bytes_per_px_iter = {per:.2f}", ""]
+    return "\n".join(lines)
+
+
 def main():
+    if len(__import__("sys").argv) > 1 and __import__("sys").argv[1] == "--batch-traffic":
+        bp = argparse.ArgumentParser()
+        bp.add_argument("flag")
+        bp.add_argument("fetch_csv")
+        bp.add_argument("write_csv")
+        bp.add_argument("--storage", required=True)
+        bp.add_argument("--pairs", type=int, default=16)
+        bp.add_argument("--px", type=int, default=848 * 800)
+        bp.add_argument("--copy-bytes", type=int, default=512 << 20)
+        bp.add_argument("--command", default="")
+        b = bp.parse_args()
+        print(batch_traffic(b.fetch_csv, b.write_csv, b.storage, b.pairs, b.px, b.copy_bytes, b.command))
+        return
     ap = argparse.ArgumentParser()
     ap.add_argument("trace")
     ap.add_argument("--levels", type=int, default=5)
