@@ -413,14 +413,19 @@ def tri(k, n):
     return m if m < n else period - m
 
 
-def prime_streams(srv, sids, streams, track=()):
+def pinned_frames(srv, sids, streams):
+    """Every distinct frame of every stream in page-locked host memory (as a camera
+    driver would deliver them), submitted to the server without a host copy."""
+    return [[srv.pinned_frame(sid, f["i0"], f["i1"]) for f in streams[i][3]] for i, sid in enumerate(sids)]
+
+
+def prime_streams(srv, sids, streams, frames, track=()):
     """Each stream's first frame (the full cold solve) so that the timed frames are the
     steady state of a stream: warm-started from the stream's previous result.  Returns the
     latencies and, for the streams in `track`, the frame's disparity."""
     lat, u0 = [], {}
     for i, sid in enumerate(sids):
-        f = streams[i][3][0]
-        srv.submit(sid, f["i0"], f["i1"], tag=-1 - i, want_result=i in track)
+        srv.submit_pinned(sid, frames[i][0], tag=-1 - i, want_result=i in track)
     for r in srv.run_until_idle():
         lat.append(r["latency_ms"])
         if r["u"] is not None:
@@ -428,7 +433,7 @@ def prime_streams(srv, sids, streams, track=()):
     return np.array(lat), u0
 
 
-def run_served(srv, sids, streams, n_frames, rate, seed=7, backlog=96, track=()):
+def run_served(srv, sids, streams, frames, n_frames, rate, seed=7, backlog=96, track=()):
     """Poisson arrivals at `rate` frames/s, or (rate <= 0, saturation) arrivals faster
     than the server serves them: submissions interleaved with the server's steps keep
     `backlog` frames queued.  Each arrival is the next frame of a uniformly chosen stream
@@ -456,10 +461,8 @@ def run_served(srv, sids, streams, n_frames, rate, seed=7, backlog=96, track=())
         now = time.perf_counter() - t0
         while k < n_total and (q < backlog if rate <= 0 else times[k] <= now):
             i = order[k]
-            W, H, base, fr = streams[i]
-            f = fr[tri(next_frame[i], len(fr))]
             frame_of[k] = (i, next_frame[i])
-            srv.submit(sids[i], f["i0"], f["i1"], tag=k, want_result=i in track)
+            srv.submit_pinned(sids[i], frames[i][tri(next_frame[i], len(frames[i]))], tag=k, want_result=i in track)
             next_frame[i] += 1
             k += 1
             q += 1
@@ -539,17 +542,19 @@ def measure_served(args, P, S, vfs_amd, plain_pairs_per_s, plain_px):
     print("bench: serving streams ready", file=sys.stderr, flush=True)
     out = {"workload": f"32 streams (424x400 / 848x800), {args.serve_frames} frames each: one cold frame "
                        f"(L5 W10 I50), then {n_timed} warm-started frames (W2 I20 at levels 1-0), FP16 storage, "
-                       f"continuous batching; frames cycle through 4 drift steps per stream"}
+                       f"continuous batching; frames cycle through 4 drift steps per stream and are submitted from "
+                       f"page-locked host buffers (no host copy), copied in and out on the server's copy stream"}
     res = {}
     track = (0, 1) if not (args.no_serve_check or args.no_cpu) else ()
 
     def phase(label, strs, rate, track=()):
         srv = vfs_amd.Server(P, hbm_budget_bytes=64 << 30, max_streams=32, max_active=64, storage="f16")
         sids = [srv.add_stream(b["mask"], b["tv"], b["cv"], 5, 10, 50, 1, 2, 20) for (W, H, b, fr) in strs]
-        cold, u0 = prime_streams(srv, sids, strs, track)
+        frames = pinned_frames(srv, sids, strs)
+        cold, u0 = prime_streams(srv, sids, strs, frames, track)
         r = rate if rate <= 0 else rate * res["saturation"]["frames_per_s"]
         t_before = srv.stats()
-        lat, who, wall, warm, nonfinite, got = run_served(srv, sids, strs, n_timed, r, track=track)
+        lat, who, wall, warm, nonfinite, got = run_served(srv, sids, strs, frames, n_timed, r, track=track)
         st = srv.stats()
         small = np.array([strs[i][0] == 424 for i in who])
         px = sum(strs[i][0] * strs[i][1] for i in who)

@@ -256,9 +256,9 @@ const char *vfs_batch_last_error(const vfs_batch *b);
  * allows; its first frame (and the first after an eviction) is a full solve, every later
  * frame warm-starts from the stream's previous result (vfs_solve_warm semantics).  A
  * submitted frame is admitted as soon as its stream is idle, an active slot is free and
- * the stream is resident; each vfs_server_step launches one kernel that advances every
- * admitted frame by one op of its own schedule, so frames arriving while others are on
- * the GPU join the running work.  A frame's result does not depend on what else is
+ * the stream is resident; each vfs_server_step launches one kernel that advances by one op
+ * of its own schedule every admitted frame whose next op runs on that kernel, so frames
+ * arriving while others are on the GPU join the running work.  A frame's result does not depend on what else is
  * running.  Solver parameters are per server; geometry and schedule are per stream. */
 typedef struct vfs_server vfs_server;
 
@@ -303,8 +303,16 @@ int vfs_server_add_stream(vfs_server *v, const vfs_stream_params *params, int *s
  * until the frame is collected. */
 int vfs_server_submit(vfs_server *v, int stream_id, const float *i0, const float *i1, float *u_out,
                       long long tag);
-/* One scheduling step: retire finished frames, admit queued ones, launch one tick (waits
- * only when 8 ticks are already in flight).  Reports frames in flight and queued. */
+/* Queues a frame without copying it: `frame` holds i0 then i1 (2*width*height floats) in
+ * pinned host memory (vfs_host_alloc) and must stay unchanged until the frame is
+ * collected; the server only reads it (one buffer may back several queued frames).  The
+ * images go to the device on the server's copy stream, overlapping the launches. */
+int vfs_server_submit_pinned(vfs_server *v, int stream_id, const float *frame, float *u_out, long long tag);
+/* Page-locked host memory for vfs_server_submit_pinned (and any other host buffer). */
+int vfs_host_alloc(size_t bytes, void **out);
+int vfs_host_free(void *p);
+/* One scheduling step: retire finished frames, admit queued ones, launch one kernel (waits
+ * only when 8 steps are already in flight).  Reports frames in flight and queued. */
 int vfs_server_step(vfs_server *v, int *active, int *queued);
 /* Moves up to `max` finished frames into `out`; *n receives the count. */
 int vfs_server_collect(vfs_server *v, vfs_server_result *out, int max, int *n);

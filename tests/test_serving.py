@@ -237,3 +237,33 @@ def test_server_budget_evicts_and_falls_back_to_cold():
         ref = standalone_sequence(base, fr, 3, 3, 10, flags)
         for k in range(3):
             assert np.array_equal(by[si][k]["u"], ref[k]), (si, k, flags)
+
+
+@pytest.mark.gpu
+def test_pinned_submission_equals_copying_submission():
+    """Frames submitted from page-locked buffers (no host copy; one buffer backing several
+    queued frames of two streams at once) give the results of copying submissions bit for
+    bit, and every result carries its energy record."""
+    specs = [(160, 120, 51), (96, 80, 52)]
+    streams = [frames(W, H, seed, 3) for W, H, seed in specs]
+    order = [(si, k) for k in range(5) for si in range(2)]   # frame k of each stream: distinct frame k % 3
+    ref = {}
+    srv = make_server()
+    sids = [srv.add_stream(b["mask"], b["tv"], b["cv"], 3, 3, 10, 1, 2, 5) for b, _ in streams]
+    for si, k in order:
+        f = streams[si][1][k % 3]
+        srv.submit(sids[si], f["i0"], f["i1"], tag=10 * si + k)
+    for r in srv.run_until_idle():
+        ref[r["tag"]] = r["u"]
+    srv.close()
+    srv = make_server()
+    sids = [srv.add_stream(b["mask"], b["tv"], b["cv"], 3, 3, 10, 1, 2, 5) for b, _ in streams]
+    pinned = [[srv.pinned_frame(sids[si], f["i0"], f["i1"]) for f in fr] for si, (_, fr) in enumerate(streams)]
+    for si, k in order:   # all ten queued at once: frames k and k + 3 share a buffer
+        srv.submit_pinned(sids[si], pinned[si][k % 3], tag=10 * si + k)
+    res = srv.run_until_idle()
+    assert len(res) == 10
+    for r in res:
+        assert np.array_equal(r["u"], ref[r["tag"]]), r["tag"]
+        assert r["nonfinite"] == 0 and r["energy"] > 0
+    srv.close()

@@ -72,6 +72,10 @@ EXPORTS = [
     ("vfs_server_add_stream", ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, _c_int_p]),
     ("vfs_server_submit", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, _c_float_p, _c_float_p, _c_float_p,
                                          ctypes.c_longlong]),
+    ("vfs_server_submit_pinned", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, _c_float_p, _c_float_p,
+                                                ctypes.c_longlong]),
+    ("vfs_host_alloc", ctypes.c_int, [ctypes.c_size_t, ctypes.POINTER(ctypes.c_void_p)]),
+    ("vfs_host_free", ctypes.c_int, [ctypes.c_void_p]),
     ("vfs_server_step", ctypes.c_int, [ctypes.c_void_p, _c_int_p, _c_int_p]),
     ("vfs_server_collect", ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, _c_int_p]),
     ("vfs_server_stats", ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p]),
@@ -425,6 +429,7 @@ class Server:
         self._h = h
         self._keep = {}  # result buffers by tag
         self.shapes = {}
+        self._pinned = []  # vfs_host_alloc buffers of pinned_frame()
 
     def _check(self, rc, what):
         if rc < 0:
@@ -450,6 +455,26 @@ class Server:
             self._keep[tag] = out
         self._check(self._L.vfs_server_submit(self._h, stream, _fptr(a), _fptr(b),
                                               _fptr(out) if out is not None else None, tag), "submit")
+
+    def pinned_frame(self, stream: int, i0, i1) -> np.ndarray:
+        """A frame (i0 then i1) in page-locked host memory for submit_pinned: a float32
+        array of shape (2, H, W) over a vfs_host_alloc buffer, freed with the server."""
+        H, W = self.shapes[stream]
+        p = ctypes.c_void_p()
+        self._check(self._L.vfs_host_alloc(2 * H * W * 4, ctypes.byref(p)), "host_alloc")
+        self._pinned.append(p)
+        arr = np.ctypeslib.as_array((ctypes.c_float * (2 * H * W)).from_address(p.value)).reshape(2, H, W)
+        arr[0] = i0
+        arr[1] = i1
+        return arr
+
+    def submit_pinned(self, stream: int, frame: np.ndarray, tag: int, want_result: bool = True):
+        """Queue a pinned_frame() without copying it (it must stay unchanged until collected)."""
+        out = np.empty(self.shapes[stream], np.float32) if want_result else None
+        if out is not None:
+            self._keep[tag] = out
+        self._check(self._L.vfs_server_submit_pinned(self._h, stream, _fptr(frame), _fptr(out) if out is not None
+                                                     else None, tag), "submit_pinned")
 
     def step(self):
         act, q = ctypes.c_int(), ctypes.c_int()
@@ -488,6 +513,9 @@ class Server:
         if getattr(self, "_h", None):
             self._L.vfs_server_destroy(self._h)
             self._h = None
+        for p in getattr(self, "_pinned", []):
+            self._L.vfs_host_free(p)
+        self._pinned = []
 
     def __del__(self):
         try:

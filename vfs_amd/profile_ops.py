@@ -153,9 +153,9 @@ def batch_traffic(fetch_csv: str, write_csv: str, storage: str, pairs: int, px: 
 
 
 def main():
-    if len(__import__("sys").argv) > 1 and __import__("sys").argv[1] == "--batch-traffic":
-        bp = argparse.ArgumentParser()
-        bp.add_argument("flag")
+    import sys
+    if len(sys.argv) > 1 and sys.argv[1] == "--batch-traffic":
+        bp = argparse.ArgumentParser(prog="profile_ops --batch-traffic")
         bp.add_argument("fetch_csv")
         bp.add_argument("write_csv")
         bp.add_argument("--storage", required=True)
@@ -163,7 +163,7 @@ def main():
         bp.add_argument("--px", type=int, default=848 * 800)
         bp.add_argument("--copy-bytes", type=int, default=512 << 20)
         bp.add_argument("--command", default="")
-        b = bp.parse_args()
+        b = bp.parse_args(sys.argv[2:])
         print(batch_traffic(b.fetch_csv, b.write_csv, b.storage, b.pairs, b.px, b.copy_bytes, b.command))
         return
     ap = argparse.ArgumentParser()
