@@ -65,6 +65,8 @@ def main():
                     help="skip the FP64 oracle check of the served frames")
     ap.add_argument("--reduced-storage", default="f32,f16,bf16",
                     help="storage types of the reduced-storage leg, comma-separated")
+    ap.add_argument("--batch-tuning", default="",
+                    help="per-level launch choices of the batched legs' batches, as --tuning (profiling aid)")
     ap.add_argument("--shape", type=int, default=-1,
                     help="force one tile shape (0..6) instead of the per-level tuner (profiling aid)")
     ap.add_argument("--tuning", default="",
@@ -105,13 +107,7 @@ def main():
     g.copyImagesToDevice(pair["i0"], pair["i1"])
     if args.no_autotune:
         g.set_option("autotune", 0)
-    for lvl, spec in enumerate(c for c in args.tuning.split(",") if c):
-        for part in spec.split("/"):
-            if "x" in part:
-                m, k = part.split("x")
-                g.set_level_tuning(lvl, int(m), per_launch=int(k.rstrip("h")), half_workgroup=k.endswith("h"))
-            else:
-                g.set_level_tuning(lvl, int(part.rstrip("p")), preload=part.endswith("p"))
+    apply_tuning(g, args.tuning)
 
     if args.iter_only:
         # A plain device copy of known size (256 MiB read + 256 MiB written per dispatch)
@@ -243,6 +239,17 @@ def main():
     print(json.dumps(out), flush=True)
 
 
+def apply_tuning(obj, spec):
+    """Per-level launch choices from level 0 (see --tuning) on a solver or a batch."""
+    for lvl, lspec in enumerate(c for c in spec.split(",") if c):
+        for part in lspec.split("/"):
+            if "x" in part:
+                m, k = part.split("x")
+                obj.set_level_tuning(lvl, int(m), per_launch=int(k.rstrip("h")), half_workgroup=k.endswith("h"))
+            else:
+                obj.set_level_tuning(lvl, int(part.rstrip("p")), preload=part.endswith("p"))
+
+
 def _oracle_solve_times(pair, wl, P, threads, runs, warm):
     """Median wall time of `runs` masked solves (after `warm`) by the FP64 oracle with
     `threads` OpenMP threads; inputs resident (copied in before each timed solve, so
@@ -327,7 +334,10 @@ def measure_batched(args, P, S, vfs_amd):
         solvers.append(_solver(vfs_amd, P, pair, 5, 10, 50))
         pairs.append(pair)
     print("bench: batched 64 mixed pairs ready", file=sys.stderr, flush=True)
+    if args.no_autotune:
+        solvers[0].set_option("autotune", 0)
     b = vfs_amd.Batch(solvers)
+    apply_tuning(b, args.batch_tuning)
     res = _time_batch(args, b, 64)
     res["tile_tuning"] = {}
     for l in range(5):
@@ -371,7 +381,10 @@ def measure_reduced(args, P, S, vfs_amd, pair_checker=True):
     out = {"workload": "16 pairs 848x800, L5 W10 I50, one batch per storage type"}
     for st in [x for x in args.reduced_storage.split(",") if x]:
         solvers = [_solver(vfs_amd, P, p, 5, 10, 50, storage=st) for p in pairs]
+        if args.no_autotune:
+            solvers[0].set_option("autotune", 0)
         b = vfs_amd.Batch(solvers)
+        apply_tuning(b, args.batch_tuning)
         res = _time_batch(args, b, 16)
         (tw, th), _, k = b.level_tuning(0, with_per_launch=True)
         res["tile_tuning_L0"] = f"{tw}x{th}, {k} iterations per launch"

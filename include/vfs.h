@@ -246,6 +246,10 @@ int vfs_batch_synchronize(vfs_batch *b);
 /* Launch choices of the batch's launches at `level` (as vfs_get_level_tuning); the batch
  * times them on scratch copies of its largest members at its first solve. */
 int vfs_batch_get_level_tuning(vfs_batch *b, int masked, int level, int *shape, int *preload, int *per_launch);
+/* Fixes the batch's launch choices of one level (as vfs_set_level_tuning; reproducible
+ * profiling): they replace the tuner's from the next batch solve on, with one iteration
+ * per launch where a member's mask forbids more. */
+int vfs_batch_set_level_tuning(vfs_batch *b, int masked, int level, int shape, int preload, int per_launch);
 /* `n` batch solves back to back, HIP events around them; total ms. */
 int vfs_batch_time_solves(vfs_batch *b, int masked, int n, float *ms_total);
 const char *vfs_batch_last_error(const vfs_batch *b);

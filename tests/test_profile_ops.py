@@ -15,14 +15,16 @@ def test_inner_ops_greedy_split():
 
 def test_program_ops_count_matches_traced_solve():
     # a traced 848x800 solve (L5 W10 I50; four iterations per launch at levels 0-2,
-    # eight at levels 3-4) issued 651 dispatches: 530 multi-iteration, 121 one-op
+    # eight at levels 3-4) issued 651 dispatches with a separate first preparation per
+    # level and no energy records: 530 multi-iteration, 121 one-op; the tensor op now also
+    # prepares the first warp (5 fewer)
     ops = program_ops(5, 10, 50, (4, 4, 4, 8, 8), energy=0)
-    assert len(ops) == 651
+    assert len(ops) == 651 - 5
     multi = [o for o in ops if o[0] in ("ITER2", "ITER4", "ITER8")]
     assert len(multi) == 530
-    assert len(program_ops(5, 10, 50, (4, 4, 4, 4, 4), energy=0)) == 771
-    # the energy records: two ops per level (default) or per warp
-    assert len(program_ops(5, 10, 50, (4, 4, 4, 8, 8))) == 651 + 2 * 5
-    assert len(program_ops(5, 10, 50, (4, 4, 4, 8, 8), energy=2)) == 651 + 2 * 50
-    lvl0 = [k for k, l in program_ops(2, 3, 5, (1, 1)) if l == 0]
-    assert lvl0[-3:] == ["ENERGY", "EFINAL", "END"]
+    assert len(program_ops(5, 10, 50, (4, 4, 4, 4, 4), energy=0)) == 771 - 5
+    # the energy records: one op per level (default) or per warp
+    assert len(program_ops(5, 10, 50, (4, 4, 4, 8, 8))) == 646 + 5
+    assert len(program_ops(5, 10, 50, (4, 4, 4, 8, 8), energy=2)) == 646 + 50
+    lvl0 = [k for k, l in program_ops(2, 3, 5, (1, 1)) if l == 0][2:]  # after ZERO, CALIB
+    assert lvl0[:2] == ["TENSOR", "DERIV"] and lvl0[-2:] == ["ENERGY", "END"]

@@ -65,6 +65,7 @@ EXPORTS = [
     ("vfs_batch_synchronize", ctypes.c_int, [ctypes.c_void_p]),
     ("vfs_batch_get_level_tuning", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, _c_int_p, _c_int_p,
                                                   _c_int_p]),
+    ("vfs_batch_set_level_tuning", ctypes.c_int, [ctypes.c_void_p] + [ctypes.c_int] * 5),
     ("vfs_batch_time_solves", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, _c_float_p]),
     ("vfs_batch_last_error", ctypes.c_char_p, [ctypes.c_void_p]),
     ("vfs_server_create", ctypes.c_int, [ctypes.POINTER(ctypes.c_void_p), ctypes.c_void_p]),
@@ -389,6 +390,13 @@ class Batch:
         if with_per_launch:
             return StereoTgv.SHAPES[sh.value], bool(pre.value), k.value
         return StereoTgv.SHAPES[sh.value], bool(pre.value)
+
+    def set_level_tuning(self, level: int, shape: int, preload: bool = False, per_launch: int = 1,
+                         masked: bool = True, half_workgroup: bool = False):
+        """Fix the batch's launch choices of one level (as StereoTgv.set_level_tuning)."""
+        k = int(per_launch) | (StereoTgv.HALF_WORKGROUP if half_workgroup else 0)
+        return self._check(self._L.vfs_batch_set_level_tuning(self._h, int(masked), level, shape, int(preload), k),
+                           "batch_set_level_tuning")
 
     def time_solves(self, masked: bool = True, n: int = 1) -> float:
         ms = ctypes.c_float()

@@ -245,6 +245,10 @@ struct vfs_batch {
     // members' mask generations when the launch choices were made: a new mask can forbid
     // multi-iteration launches at a level (a mask pixel in column 0), so it re-tunes
     std::vector<unsigned long long> mask_gen;
+    // launch choices fixed by vfs_batch_set_level_tuning ([masked][level]); they replace
+    // the tuner's wherever every member allows them
+    bool fixed[2][kMaxLevels] = {};
+    LevelTune fixed_lv[2][kMaxLevels];
 };
 
 namespace {
@@ -352,11 +356,13 @@ void push_level(Program &pr, const Params &P, int masked, int l, int warps, int 
 {
     std::vector<float4> sched((size_t)std::max(iters, 1));
     iter_schedule(P.timestep_lambda, iters, sched.data());
+    // the tensor and the first warp's preparation run in one op (EF_PREP); later warps'
+    // preparations run with the previous warp's end (OP_ENDPREP)
     push_op(pr, vfs::OP_TENSOR, l, masked, roles);
+    pr.ops.back().flags = vfs::EF_PREP;
     for (int wi = 0; wi < warps; wi++) {
-        // a warp's preparation runs with the previous warp's end (OP_ENDPREP)
-        if (wi == 0) push_op(pr, vfs::OP_PREP, l, masked, roles);
         push_op(pr, vfs::OP_DERIV, l, masked, roles);
+        const size_t deriv = pr.ops.size() - 1;
         for (int it = 0; it < iters;) {
             ProgOp o{};
             o.level = (unsigned char)l;
@@ -368,7 +374,9 @@ void push_level(Program &pr, const Params &P, int masked, int l, int warps, int 
             for (int c : {8, 4, 2})
                 if (k == 1 && c <= per_launch && it + c <= iters && it + c <= vfs::kMaxSched) k = c;
             if (k > 1) {
-                if (sync_spare && it == 0) push_op(pr, vfs::OP_SYNC, l, masked, roles);
+                // the spare buffers' copy beyond the level's extent runs with the
+                // derivatives (EF_SYNC), which read none of u, u_, v_
+                if (sync_spare && it == 0) pr.ops[deriv].flags |= vfs::EF_SYNC;
                 o.kind = iter_kind(k);
                 o.it0 = it;
                 o.flags = (unsigned char)((it == 0 ? vfs::EF_FIRST : 0) | (it + k == iters ? vfs::EF_WRITE_TP_LAST : 0));
@@ -386,13 +394,17 @@ void push_level(Program &pr, const Params &P, int masked, int l, int warps, int 
         }
         // energy / residual record of the warp (before its end, which overwrites the other
         // u buffer and clears v_)
-        if (P.monitor == 2 || (P.monitor == 1 && wi + 1 == warps)) {
+        // (the end's extra workgroup adds the partials into the record: EF_EFINAL)
+        const bool mon = P.monitor == 2 || (P.monitor == 1 && wi + 1 == warps);
+        if (mon) {
             push_op(pr, vfs::OP_ENERGY, l, masked, roles);
-            push_op(pr, vfs::OP_EFINAL, l, masked, roles);
-            pr.ops.back().it0 = wi;
             pr.mon_slots.push_back(vfs::mon_slot(l, wi));
         }
         push_op(pr, wi + 1 < warps ? vfs::OP_ENDPREP : vfs::OP_END, l, masked, roles);
+        if (mon) {
+            pr.ops.back().flags |= vfs::EF_EFINAL;
+            pr.ops.back().it0 = wi;
+        }
         if (masked) roles = vfs::rswap(roles, vfs::G_U);
     }
 }
@@ -512,7 +524,7 @@ hipError_t issue_program(const Engine &e, const Params &P, const Program &pr)
             for (int k = 0; k < cnt; k++) {
                 const vfs::PairDesc &d = e.h_desc[p0 + k];
                 set_entry(a.e[k], o, p0 + k, d, acc);
-                acc += vfs::op_tiles(o.kind, o.level, d.w, d.h, d.s, o.shape);
+                acc += vfs::op_tiles(o.kind, o.level, d.w, d.h, d.s, o.shape, o.flags);
             }
             a.n = cnt;
             a.total = acc;
@@ -1775,20 +1787,25 @@ static int tune_batch(vfs_batch *b, int masked)
         for (float *p : scratch) (void)hipFree(p);
         if (r != hipSuccess) return failb(b, VFS_ERR_DEVICE, "batch launch tuner: %s", hipGetErrorString(r));
     }
+    std::vector<int> all;
+    for (int l = 0; l < f->nLevels; l++) all.push_back(l);
+    std::vector<bool> ok(all.size(), true);
+    for (const vfs_solver *s : b->members) {
+        const std::vector<bool> o = multi_allowed(s, m, all);
+        for (size_t i = 0; i < ok.size(); i++) ok[i] = ok[i] && o[i];
+    }
     if (f->force_per_launch >= 0) {
-        std::vector<int> all;
-        for (int l = 0; l < f->nLevels; l++) all.push_back(l);
-        std::vector<bool> ok(all.size(), true);
-        for (const vfs_solver *s : b->members) {
-            const std::vector<bool> o = multi_allowed(s, m, all);
-            for (size_t i = 0; i < ok.size(); i++) ok[i] = ok[i] && o[i];
-        }
         for (int l = 0; l < f->nLevels; l++) {
             tn.lv[l].per_launch = (unsigned char)(ok[l] ? f->force_per_launch : 1);
             if (!multi_shape(tn.lv[l].mshape)) tn.lv[l].mshape = 4;
             if (!multi_shape(tn.lv[l].m2shape)) tn.lv[l].m2shape = 4;
         }
     }
+    for (int l = 0; l < f->nLevels; l++)
+        if (b->fixed[m][l]) {
+            tn.lv[l] = b->fixed_lv[m][l];
+            if (!ok[l]) tn.lv[l].per_launch = 1;  // a member's mask forbids several iterations per launch here
+        }
     b->tuned[m] = true;
     return VFS_OK;
 }
@@ -1868,6 +1885,35 @@ int vfs_batch_get_level_tuning(vfs_batch *b, int masked, int level, int *shape, 
     *shape = t.iter_shape();
     *preload = t.preload;
     if (per_launch) *per_launch = t.per_launch;
+    return VFS_OK;
+}
+
+int vfs_batch_set_level_tuning(vfs_batch *b, int masked, int level, int shape, int preload, int per_launch)
+{
+    if (!b) return VFS_ERR_ARG;
+    if (level < 0 || level >= b->members[0]->nLevels) return failb(b, VFS_ERR_ARG, "bad level");
+    const bool half = (per_launch & VFS_HALF_WORKGROUP) != 0;
+    per_launch &= ~VFS_HALF_WORKGROUP;
+    if (!valid_per_launch(per_launch) || (half && (per_launch == 1 || !multi_shape(shape) ||
+                                                   !vfs::half_workgroup_ok(shape, per_launch))))
+        return failb(b, VFS_ERR_ARG, "per_launch must be 1, 2, 4 or 8 (half workgroups where the staged region "
+                                     "holds 513 .. 1536 positions)");
+    if (per_launch > 1 ? !(multi_shape(shape) && vfs::iterk_ok(shape, per_launch))
+                       : (shape < 0 || shape >= vfs::kTickShapes))
+        return failb(b, VFS_ERR_ARG, "bad shape %d for %d iteration(s) per launch", shape, per_launch);
+    const int m = masked ? 1 : 0;
+    LevelTune t = b->fixed[m][level] ? b->fixed_lv[m][level] : default_tune(level);
+    if (per_launch > 1) {
+        t.mshape = t.m2shape = (unsigned char)shape;
+        t.mhalf = t.m2half = (unsigned char)half;
+    } else {
+        t.shape = (unsigned char)shape;
+        t.preload = (unsigned char)(preload != 0);
+    }
+    t.per_launch = (unsigned char)per_launch;
+    b->fixed_lv[m][level] = t;
+    b->fixed[m][level] = true;
+    b->tuned[m] = false;  // applied at the next solve
     return VFS_OK;
 }
 
@@ -2250,7 +2296,7 @@ int launch_group(vfs_server *v, const std::vector<SrvRequest *> &grp)
         const ProgOp &o = r->prog.ops[r->pc];
         const vfs::PairDesc &d = v->h_desc[st.slot];
         set_entry(a.e[k], o, st.slot, d, acc);
-        acc += vfs::op_tiles(o.kind, o.level, d.w, d.h, d.s, shape);
+        acc += vfs::op_tiles(o.kind, o.level, d.w, d.h, d.s, shape, o.flags);
         k++;
     }
     a.n = k;

@@ -274,6 +274,32 @@ __device__ __forceinline__ void op_sync(const Tile &t, int level, int roles)
     }
 }
 
+// OP_SYNC's copy for one chunk of kSyncChunk flat positions beyond the level's extent
+// (the extra workgroups of a derivatives op with EF_SYNC): chunk k covers positions
+// ext + k kSyncChunk .. of the level-0 planes.
+template <class S>
+__device__ __forceinline__ void op_sync_chunk(const Tile &t, int level, int roles, int chunk)
+{
+    const long ext = (long)t.d->s[level] * t.d->h[level];
+    const long end = (long)t.d->s[0] * t.d->h[0];
+    const float *u = slot(t, slot_u(rcur(roles, G_U)));
+    float *us = slot(t, slot_u(rspare(roles, G_U)));
+    const float *u_ = slot(t, slot_ub(rcur(roles, G_UB)));
+    float *u_s = slot(t, slot_ub(rspare(roles, G_UB)));
+    const float2 *v = slotp<float2>(t, slot_v(rcur(roles, G_V)));
+    float2 *vs = slotp<float2>(t, slot_v(rspare(roles, G_V)));
+    const int tid = threadIdx.y * S::BX + threadIdx.x;
+#pragma unroll
+    for (int k = 0; k < kSyncChunk / NT; k++) {
+        const long pos = ext + (long)chunk * kSyncChunk + k * NT + tid;
+        if (pos < end) {
+            us[pos] = u[pos];
+            u_s[pos] = u_[pos];
+            vs[pos] = v[pos];
+        }
+    }
+}
+
 // Calibration warp of I1 (stereotgv.cu:392-393, warpImage.cu:9-23) into the level-0
 // pyramid slot; with 16-bit storage also narrows I0 into its pyramid slot.
 template <class S, typename T>
@@ -1586,6 +1612,16 @@ __global__ __launch_bounds__(NT) void k_tick(const PairDesc *__restrict__ D, con
     const int ntx = (t.w + S::TW - 1) / S::TW;
     t.x0 = (local % ntx) * S::TW;
     t.y0 = (local / ntx) * S::TH;
+    // workgroups beyond the op's tiles: the energy record's final sum of a warp end
+    // (EF_EFINAL, one workgroup) or the spare buffers' copy of the derivatives' op (EF_SYNC)
+    if ((E.flags & (EF_EFINAL | EF_SYNC)) && (kind == OP_END || kind == OP_ENDPREP || kind == OP_DERIV)) {
+        const int ntiles = ntx * ((t.h + S::TH - 1) / S::TH);
+        if (local >= ntiles) {
+            if (kind == OP_DERIV) op_sync_chunk<S>(t, level, E.roles, local - ntiles);
+            else op_efinal<S>(t, level, E.it0);
+            return;
+        }
+    }
     switch (kind) {
     case OP_ZERO:
         op_zero<S>(t);
@@ -1605,6 +1641,7 @@ __global__ __launch_bounds__(NT) void k_tick(const PairDesc *__restrict__ D, con
     case OP_TENSOR:
         if (E.masked) op_tensor<S, T, true>(t, level, A.xblock, A.beta, A.gamma);
         else op_tensor<S, T, false>(t, level, A.xblock, A.beta, A.gamma);
+        if (E.flags & EF_PREP) op_prep<S, T>(t, level, E.roles);  // disjoint reads and writes
         break;
     case OP_PREP:
         op_prep<S, T>(t, level, E.roles);

@@ -215,7 +215,11 @@ enum EntryFlag : int {
     EF_FIRST = 1,          // inner-iteration ops: p and q were just zeroed
     EF_WRITE_TP = 2,       // ITER: the iteration also stores Tp
     EF_PRELOAD_TP = 4,     // ITER: fetch the Tp-plane neighbours with the other operands
-    EF_WRITE_TP_LAST = 8   // ITER2 / ITER4 / ITER8: the op's last iteration stores Tp
+    EF_WRITE_TP_LAST = 8,  // ITER2 / ITER4 / ITER8: the op's last iteration stores Tp
+    EF_EFINAL = 16,        // END / ENDPREP: one extra workgroup adds the level's energy partials
+                           // into record (level, it0), the work of OP_EFINAL
+    EF_SYNC = 32,          // DERIV: extra workgroups do OP_SYNC's copy beyond the level's extent
+    EF_PREP = 64           // TENSOR: the same workgroups also run OP_PREP at each pixel
 };
 
 // One (pair, op) of a launch.  The geometry and arena of the op's tile level travel in
@@ -248,15 +252,28 @@ inline int op_tile_level(int kind, int level) { return kind == OP_UP ? level - 1
 // Ops whose tiles cover the level-0 planes' whole flat extent, row padding included.
 inline bool op_flat(int kind) { return kind == OP_ZERO || kind == OP_SYNC; }
 
-// Workgroups an op of `kind` at `level` needs for a pair with dims w/h/s in tile `shape`.
-inline int op_tiles(int kind, int level, const int *w, const int *h, const int *s, int shape)
+// Flat positions of the level-0 planes beyond level `level`'s extent (stride x height), in
+// chunks of kSyncChunk per workgroup (EF_SYNC).
+constexpr int kSyncChunk = 1024;
+inline int sync_chunks(int level, const int *h, const int *s)
+{
+    const long long n = (long long)s[0] * h[0] - (long long)s[level] * h[level];
+    return n > 0 ? (int)((n + kSyncChunk - 1) / kSyncChunk) : 0;
+}
+
+// Workgroups an op of `kind` at `level` with `flags` needs for a pair with dims w/h/s in
+// tile `shape`.
+inline int op_tiles(int kind, int level, const int *w, const int *h, const int *s, int shape, int flags = 0)
 {
     if (kind == OP_ENERGY) return energy_chunks(w[level], h[level]);
     if (kind == OP_EFINAL) return 1;
     const int tl = op_tile_level(kind, level);
     const int ww = op_flat(kind) ? s[0] : w[tl];
     const ShapeDim d = kShape[shape_index(shape)];
-    return ((ww + d.tw - 1) / d.tw) * ((h[tl] + d.th - 1) / d.th);
+    int n = ((ww + d.tw - 1) / d.tw) * ((h[tl] + d.th - 1) / d.th);
+    if ((kind == OP_END || kind == OP_ENDPREP) && (flags & EF_EFINAL)) n += 1;
+    if (kind == OP_DERIV && (flags & EF_SYNC)) n += sync_chunks(level, h, s);
+    return n;
 }
 
 // Launches one op for the entries of `args` (all of one kind) in tile shape `shape`;

@@ -3,9 +3,10 @@
 Every op of a solve program is one `k_tick` dispatch when the solve is launched without
 a graph (option "graph" = 0), in program order.  This module rebuilds the cold masked
 program's op list (the same sequence the library issues: zero, calibration warp,
-pyramid, then per level the tensor and, per warp, prep / derivatives / inner iterations
-/ end, each end but the level's last fused with the next warp's prep, the energy record
-(ENERGY, EFINAL) before the level's last end, then the upscale)
+pyramid, then per level the tensor (with the first warp's preparation) and, per warp,
+derivatives / inner iterations / end, each end but the level's last fused with the next
+warp's prep, the energy partials (ENERGY) before the level's last end (whose extra
+workgroup adds them up), then the upscale)
 and attributes each dispatch of one solve to its op and level.
 Levels that run several inner iterations per launch (`--per-launch`, one count per level
 from level 0, as the tuner reports them) issue `k_iterk` dispatches: ITER8 / ITER4 / ITER2 while
@@ -43,9 +44,9 @@ def program_ops(levels: int, warps: int, iters: int, per_launch=(), energy: int 
         k = per_launch[l] if l < len(per_launch) else 1
         inner = [(name, l) for name in inner_ops(iters, k)]
         for wi in range(warps):
-            ops += ([("PREP", l)] if wi == 0 else []) + [("DERIV", l)] + inner
+            ops += [("DERIV", l)] + inner
             if energy == 2 or (energy == 1 and wi + 1 == warps):
-                ops += [("ENERGY", l), ("EFINAL", l)]
+                ops.append(("ENERGY", l))
             ops.append(("ENDPREP" if wi + 1 < warps else "END", l))
         if l > 0:
             ops.append(("UP", l))
