@@ -184,7 +184,8 @@ int vfs_get_energy(vfs_solver *s, int level, int warp, vfs_energy_t *out);
  *   "per_launch" -1 = as "autotune" says (default), 1, 2, 4 or 8 = that many inner
  *             iterations per launch wherever allowed (the workgroup recomputes a halo
  *             of 3 pixels per iteration; levels with a mask pixel in column 0 run one;
- *             eight only in the tiles 32x8, 16x16, 20x20, 14x13, 8x8, 7x7 and 4x4, else
+ *             eight only in the tiles 64x4, 32x16, 32x8, 16x16, 20x20, 36x25, 27x25, 14x13,
+ *             8x8, 7x7 and 4x4, else
  *             four);
  *   "pair"    the same as "per_launch" -1 / 1 / 2 written as -1 / 0 / 1;
  *   "energy"  0 = no energy records, 1 = after the last warp of every level (default),

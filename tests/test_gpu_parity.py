@@ -238,7 +238,7 @@ def test_several_iterations_per_launch_give_identical_results(masked, shape, sto
     """Two, four and eight inner iterations per launch (recomputed halo, spare buffers,
     rotated roles; duals rounded to the storage type between iterations) reproduce one
     iteration per launch bit for bit, including iteration counts that leave a remainder
-    (15 = 4 + 4 + 4 + 2 + 1 = 8 + 4 + 2 + 1; tiles too large for eight take 32x8) and the
+    (15 = 4 + 4 + 4 + 2 + 1 = 8 + 4 + 2 + 1) and the
     buffers that later ops read outside the mask."""
     pair = small_pair(200, 120, seed=37)
     ref = None
@@ -266,9 +266,9 @@ def test_several_iterations_per_launch_give_identical_results(masked, shape, sto
 @pytest.mark.parametrize("masked", [True, False])
 def test_small_tiles_several_iterations_give_identical_results(masked):
     """The 8x8, 4x4, 32x32, 20x20, 36x25, 27x25, 14x13, 7x7, 43x34, 34x40 and 53x25 tiles of the
-    multi-iteration launches, eight iterations per launch and the half workgroups (two
-    or three positions per work-item; set per level) reproduce one iteration per launch
-    bit for bit."""
+    multi-iteration launches, eight iterations per launch (in every tile whose staged
+    region holds at most 3072 positions) and the half workgroups (two or three positions
+    per work-item; set per level) reproduce one iteration per launch bit for bit."""
     pair = small_pair(200, 120, seed=41)
     g = make_gpu(pair, 3, 2, 11)
     g.set_option("per_launch", 1)
@@ -277,7 +277,8 @@ def test_small_tiles_several_iterations_give_identical_results(masked):
     cases = [(sh, k, False) for sh in (7, 8, 9, 10, 11, 12, 13, 14, 15, 16, 17) for k in (2, 4)]
     cases += [(10, 4, True), (6, 4, True), (4, 2, True), (13, 4, True)]
     cases += [(9, 2, True), (11, 2, True), (1, 4, True), (3, 4, True), (12, 4, True), (13, 8, True)]
-    cases += [(sh, 8, False) for sh in (4, 6, 7, 8, 10, 13, 14)] + [(7, 8, True), (8, 8, True), (14, 8, True)]
+    cases += [(sh, 8, False) for sh in (1, 3, 4, 6, 7, 8, 10, 11, 12, 13, 14)]
+    cases += [(7, 8, True), (8, 8, True), (14, 8, True)]
     for shape, per_launch, half in cases:
         g = make_gpu(pair, 3, 2, 11)
         print(f"tiles: shape {shape} per_launch {per_launch} half {half} masked {masked}", flush=True)
@@ -293,14 +294,14 @@ def test_small_tiles_several_iterations_give_identical_results(masked):
 
 def test_level_tuning_rejects_invalid_choices():
     """Launch choices the kernels do not have fail with an error and change nothing:
-    eight iterations in tiles whose staged region exceeds 2048 positions (64x4, 32x16,
-    32x32, 36x25, 27x25, 43x34, 34x40, 53x25), iteration counts other than 1, 2, 4, 8, half workgroups
-    outside 513 .. 1536 staged positions, shape indices past the last."""
+    eight iterations in tiles whose staged region exceeds 3072 positions (32x32, 43x34,
+    34x40, 53x25), iteration counts other than 1, 2, 4, 8, half workgroups outside
+    513 .. 1536 staged positions, shape indices past the last."""
     from vfs_amd import VfsError
     g = make_gpu(small_pair(96, 64, seed=45), 2, 1, 4)
     before = g.level_tuning(0, True, with_per_launch=True)
-    for shape, per_launch, half in [(9, 8, False), (1, 8, False), (3, 8, False), (11, 8, False), (12, 8, False),
-                                    (15, 8, False), (16, 8, False), (17, 8, False), (4, 3, False), (4, 16, False),
+    for shape, per_launch, half in [(9, 8, False), (15, 8, False), (16, 8, False), (17, 8, False), (4, 3, False),
+                                    (4, 16, False),
                                     (9, 4, True), (15, 4, True), (17, 2, True), (4, 1, True), (18, 4, False)]:
         with pytest.raises(VfsError):
             g.set_level_tuning(0, shape, per_launch=per_launch, half_workgroup=half)
@@ -322,7 +323,7 @@ def test_several_iterations_odd_geometry(masked):
     (g.solveStereoForwardMasked if masked else g.solveStereoForward)()
     ref = {n: g.read_plane(n, 0) for n in ("u", "u_", "v_", "warpUV")}
     cases = [(sh, 4) for sh in (1, 3, 4, 6, 7, 8, 9, 10, 11, 12, 13, 14, 15, 16, 17)]
-    cases += [(sh, 8) for sh in (4, 6, 7, 8, 10, 13, 14)]
+    cases += [(sh, 8) for sh in (1, 3, 4, 6, 7, 8, 10, 11, 12, 13, 14)]
     for shape, per_launch in cases:
         g = make_gpu(pair, 3, 2, 9)
         print(f"odd geometry: shape {shape} per_launch {per_launch} masked {masked}", flush=True)

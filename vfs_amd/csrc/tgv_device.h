@@ -1296,18 +1296,39 @@ void k_iterk(const PairDesc *__restrict__ D, const TickArgs A)
     else op_iterk<SH, K, HALF, false, T>(t, E.roles, sched, ip0, L);
 }
 
-// 13th smallest of 25 by the partial selection sort of medianFilter.cu:307-316.
+// 13th smallest of 25 (the value medianFilter.cu:307-316's partial selection sort leaves
+// at position 12) by a comparator network: Batcher's odd-even merge sort of 32 inputs,
+// the 7 beyond the window taken as +infinity, pruned to the 113 comparators that
+// position 12 depends on (each a min and a max; the compiler drops the unused ones).  A
+// network selects the k-th smallest of all inputs if it does so for every 0/1 input (the
+// 0-1 principle); tests/test_median_network.py checks all 2^25 of them.  With NaN in the
+// window the result may differ from the reference's selection sort (fminf / fmaxf drop
+// NaN operands).
+constexpr unsigned char kMedian25[113][2] = {
+    {0, 1}, {2, 3}, {0, 2}, {1, 3}, {1, 2}, {4, 5}, {6, 7}, {4, 6},
+    {5, 7}, {5, 6}, {0, 4}, {2, 6}, {2, 4}, {1, 5}, {3, 7}, {3, 5},
+    {1, 2}, {3, 4}, {5, 6}, {8, 9}, {10, 11}, {8, 10}, {9, 11}, {9, 10},
+    {12, 13}, {14, 15}, {12, 14}, {13, 15}, {13, 14}, {8, 12}, {10, 14}, {10, 12},
+    {9, 13}, {11, 15}, {11, 13}, {9, 10}, {11, 12}, {13, 14}, {0, 8}, {4, 12},
+    {4, 8}, {2, 10}, {6, 14}, {6, 10}, {2, 4}, {6, 8}, {10, 12}, {1, 9},
+    {5, 13}, {5, 9}, {3, 11}, {7, 15}, {7, 11}, {3, 5}, {7, 9}, {11, 13},
+    {1, 2}, {3, 4}, {5, 6}, {7, 8}, {9, 10}, {11, 12}, {13, 14}, {16, 17},
+    {18, 19}, {16, 18}, {17, 19}, {17, 18}, {20, 21}, {22, 23}, {20, 22}, {21, 23},
+    {21, 22}, {16, 20}, {18, 22}, {18, 20}, {17, 21}, {19, 23}, {19, 21}, {17, 18},
+    {19, 20}, {21, 22}, {16, 24}, {20, 24}, {18, 20}, {22, 24}, {19, 21}, {17, 18},
+    {19, 20}, {21, 22}, {23, 24}, {0, 16}, {8, 24}, {8, 16}, {4, 20}, {12, 20},
+    {12, 16}, {2, 18}, {10, 18}, {6, 22}, {6, 10}, {10, 12}, {1, 17}, {9, 17},
+    {5, 21}, {13, 21}, {13, 17}, {3, 19}, {11, 19}, {7, 23}, {7, 11}, {11, 13},
+    {11, 12},
+};
+
 __device__ __forceinline__ float median25(float m[25])
 {
 #pragma unroll
-    for (int j = 0; j < 13; j++) {
-#pragma unroll
-        for (int i = j + 1; i < 25; i++) {
-            const float a = m[j], b = m[i];
-            const bool sw = a > b;
-            m[j] = sw ? b : a;
-            m[i] = sw ? a : b;
-        }
+    for (int i = 0; i < (int)(sizeof kMedian25 / sizeof kMedian25[0]); i++) {
+        const float a = m[kMedian25[i][0]], b = m[kMedian25[i][1]];
+        m[kMedian25[i][0]] = fminf(a, b);
+        m[kMedian25[i][1]] = fmaxf(a, b);
     }
     return m[12];
 }

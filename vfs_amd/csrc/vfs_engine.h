@@ -57,10 +57,10 @@ __host__ __device__ constexpr bool half_workgroup_ok(int shape, int K)
 __host__ __device__ constexpr int shape_index(int launch_shape) { return launch_shape & (kHalfWorkgroup - 1); }
 static_assert(kShapes <= kHalfWorkgroup, "shape indices fit below the half-workgroup flag");
 // Tile shapes of K-iteration launches: every multi-iteration shape for K = 2 and 4; for
-// K = 8 those whose staged region holds at most 2048 positions.
+// K = 8 those whose staged region holds at most 3072 positions (three per work-item).
 __host__ __device__ constexpr bool iterk_ok(int shape, int K)
 {
-    return multi_iter_shape(shape) && (K <= 4 || iterk_region(shape, K) <= 2048);
+    return multi_iter_shape(shape) && (K <= 4 || iterk_region(shape, K) <= 3072);
 }
 
 // Storage precision of the image pyramid, warped image, derivatives, tensor and duals.
