@@ -67,6 +67,8 @@ def main():
                     help="storage types of the reduced-storage leg, comma-separated")
     ap.add_argument("--batch-tuning", default="",
                     help="per-level launch choices of the batched legs' batches, as --tuning (profiling aid)")
+    ap.add_argument("--batch-no-autotune", action="store_true",
+                    help="no launch tuner for the batched legs' batches (with --batch-tuning for reproducible profiles)")
     ap.add_argument("--shape", type=int, default=-1,
                     help="force one tile shape (0..6) instead of the per-level tuner (profiling aid)")
     ap.add_argument("--tuning", default="",
@@ -137,6 +139,19 @@ def main():
     u_gpu = g.copy1DDisparityToHost()
     print(f"bench: {args.steps} solves in {wall * 1e3:.1f} ms; per-solve p50 {np.percentile(lat, 50):.3f} ms",
           file=sys.stderr, flush=True)
+    # the energy / residual monitor (option "energy" 1: one FP64 record per level) against
+    # the same solve without it
+    g.set_option("energy", 0)
+    g.time_solves(True, 2)
+    off = [g.time_solves(True, 1) for _ in range(args.steps)]
+    g.set_option("energy", 1)
+    on = [g.time_solves(True, 1) for _ in range(args.steps)]
+    energy_monitor = {"ms_per_solve_on": round(float(np.median(on)), 4),
+                             "ms_per_solve_off": round(float(np.median(off)), 4),
+                             "overhead_pct": round(100.0 * (np.median(on) / np.median(off) - 1.0), 2),
+                             "records": "one per level (after its last warp), FP64 sums on the GPU"}
+    print(f"bench: energy monitor {energy_monitor}", file=sys.stderr, flush=True)
+
     # the drop-in call sequence with host buffers: copy both images in (pageable host
     # memory over PCIe), solve, copy the disparity out
     t0 = time.perf_counter()
@@ -177,6 +192,7 @@ def main():
                                f"{wl['warps']} warps, {wl['iters']} iterations, one pair per step, FP32",
                    "global_batch": 1},
         "pcie_inclusive_frame_pairs_per_s": round(pcie_pairs_per_s, 3),
+        "energy_monitor": energy_monitor,
         "latency_ms": {"p50": round(float(np.percentile(lat, 50)), 4),
                        "p99": round(float(np.percentile(lat, 99)), 4),
                        "mean": round(float(lat.mean()), 4)},
@@ -207,19 +223,6 @@ def main():
         out["tile_tuning"][f"L{lvl} {lw}x{lh}"] = (f"{tw}x{th}" + (" +Tp prefetch" if pre else "")
                                                    + (f", {k} iterations per launch" if k > 1 else ""))
     print(f"bench: tile tuning {out['tile_tuning']}", file=sys.stderr, flush=True)
-
-    # the energy / residual monitor (option "energy" 1: one FP64 record per level) against
-    # the same solve without it
-    g.set_option("energy", 0)
-    g.time_solves(True, 2)
-    off = [g.time_solves(True, 1) for _ in range(args.steps)]
-    g.set_option("energy", 1)
-    on = [g.time_solves(True, 1) for _ in range(args.steps)]
-    out["energy_monitor"] = {"ms_per_solve_on": round(float(np.median(on)), 4),
-                             "ms_per_solve_off": round(float(np.median(off)), 4),
-                             "overhead_pct": round(100.0 * (np.median(on) / np.median(off) - 1.0), 2),
-                             "records": "one per level (after its last warp), FP64 sums on the GPU"}
-    print(f"bench: energy monitor {out['energy_monitor']}", file=sys.stderr, flush=True)
 
     legs = set() if args.quick else ({"served"} if args.serve_only else {l for l in args.legs.split(",") if l})
     if "served" in legs:
@@ -334,7 +337,7 @@ def measure_batched(args, P, S, vfs_amd):
         solvers.append(_solver(vfs_amd, P, pair, 5, 10, 50))
         pairs.append(pair)
     print("bench: batched 64 mixed pairs ready", file=sys.stderr, flush=True)
-    if args.no_autotune:
+    if args.batch_no_autotune:
         solvers[0].set_option("autotune", 0)
     b = vfs_amd.Batch(solvers)
     apply_tuning(b, args.batch_tuning)
@@ -381,7 +384,7 @@ def measure_reduced(args, P, S, vfs_amd, pair_checker=True):
     out = {"workload": "16 pairs 848x800, L5 W10 I50, one batch per storage type"}
     for st in [x for x in args.reduced_storage.split(",") if x]:
         solvers = [_solver(vfs_amd, P, p, 5, 10, 50, storage=st) for p in pairs]
-        if args.no_autotune:
+        if args.batch_no_autotune:
             solvers[0].set_option("autotune", 0)
         b = vfs_amd.Batch(solvers)
         apply_tuning(b, args.batch_tuning)
@@ -565,6 +568,7 @@ def measure_served(args, P, S, vfs_amd, plain_pairs_per_s, plain_px):
         sids = [srv.add_stream(b["mask"], b["tv"], b["cv"], 5, 10, 50, 1, 2, 20) for (W, H, b, fr) in strs]
         frames = pinned_frames(srv, sids, strs)
         cold, u0 = prime_streams(srv, sids, strs, frames, track)
+        time.sleep(0.05)  # an idle gap that separates the phases in a kernel trace
         r = rate if rate <= 0 else rate * res["saturation"]["frames_per_s"]
         t_before = srv.stats()
         lat, who, wall, warm, nonfinite, got = run_served(srv, sids, strs, frames, n_timed, r, track=track)
@@ -590,6 +594,7 @@ def measure_served(args, P, S, vfs_amd, plain_pairs_per_s, plain_px):
         res[label] = entry
         print(f"bench: served {label}: {entry}", file=sys.stderr, flush=True)
         srv.close()
+        time.sleep(0.05)
         return u0, got
 
     u0, got = phase("saturation", streams, 0.0, track)

@@ -115,12 +115,16 @@ def _iterk_k(name: str) -> int:
     return int(args[2])
 
 
+_TYPE = {"f32": "float", "f16": "__half", "bf16": "__hip_bfloat16"}
+
+
 def batch_traffic(fetch_csv: str, write_csv: str, storage: str, pairs: int, px: int, copy_bytes: int,
-                  command: str) -> str:
+                  command: str, k: int = 4) -> str:
     """Summary of the HBM bytes per pixel and inner iteration of a batch's level-0
-    multi-iteration launches: the k_iterk dispatches of the largest grid with the largest
-    K, FETCH_SIZE and WRITE_SIZE per dispatch (KiB) scaled by what the same counters read
-    for the k_copy dispatches that move `copy_bytes` each way."""
+    multi-iteration launches: the k_iterk dispatches of the storage type's instance with
+    `k` iterations and the largest grid, FETCH_SIZE and WRITE_SIZE per dispatch (KiB)
+    scaled by what the same counters read for the k_copy dispatches that move
+    `copy_bytes` each way."""
     import statistics as st
     rows = {c: _counter_rows(p, c) for c, p in (("FETCH_SIZE", fetch_csv), ("WRITE_SIZE", write_csv))}
     med = {}
@@ -129,8 +133,8 @@ def batch_traffic(fetch_csv: str, write_csv: str, storage: str, pairs: int, px: 
     for c, rs in rows.items():
         copies = [v for _, n, _, v in rs if "k_copy" in n]
         scale[c] = copy_bytes / 1024.0 / st.median(copies)
-        iterk = [(n, g, v) for _, n, g, v in rs if _iterk_k(n) > 0]
-        kmax = max(_iterk_k(n) for n, _, _ in iterk)
+        iterk = [(n, g, v) for _, n, g, v in rs if _iterk_k(n) > 0 and f"k_iterk<{_TYPE[storage]}," in n]
+        kmax = k
         gmax = max(g for n, g, _ in iterk if _iterk_k(n) == kmax)
         vals = [v for n, g, v in iterk if _iterk_k(n) == kmax and g == gmax]
         sel = (kmax, gmax, len(vals), sorted({n for n, g, _ in iterk if _iterk_k(n) == kmax and g == gmax}))
@@ -153,8 +157,47 @@ def batch_traffic(fetch_csv: str, write_csv: str, storage: str, pairs: int, px: 
     return "\n".join(lines)
 
 
+def trace_segments(trace_csv: str, gap_ms: float = 20.0, after_last: str = "k_copy") -> str:
+    """Segments of a kernel trace: the k_tick / k_iterk dispatches after the last
+    `after_last` dispatch, split where the device idles longer than `gap_ms`; per
+    segment the dispatch count, span, busy share (kernel time / span), idle gaps between
+    dispatches and the average duration of each kernel.  For the served leg the segments
+    are the phases of the bench (stream set-up, cold first frames, timed frames)."""
+    import statistics as st
+    rows = list(csv.DictReader(open(trace_csv)))
+    rows.sort(key=lambda r: int(r["Start_Timestamp"]))
+    last = max([i for i, r in enumerate(rows) if after_last in r["Kernel_Name"]], default=-1)
+    sub = [r for r in rows[last + 1:] if r["Kernel_Name"].startswith(("k_tick", "k_iterk")) or
+           "k_tick" in r["Kernel_Name"] or "k_iterk" in r["Kernel_Name"]]
+    segs, cur = [], [sub[0]]
+    for a, b in zip(sub, sub[1:]):
+        if int(b["Start_Timestamp"]) - int(a["End_Timestamp"]) > gap_ms * 1e6:
+            segs.append(cur)
+            cur = []
+        cur.append(b)
+    segs.append(cur)
+    out = ["| segment | dispatches | span ms | busy | idle gap median / p90 / mean us | k_tick avg us | "
+           "k_iterk avg us |", "|---|---|---|---|---|---|---|"]
+    for i, sg in enumerate(segs):
+        s0 = [int(r["Start_Timestamp"]) for r in sg]
+        e0 = [int(r["End_Timestamp"]) for r in sg]
+        busy = sum(e - s for s, e in zip(s0, e0))
+        span = max(e0[-1] - s0[0], 1)
+        gaps = sorted([s0[j + 1] - e0[j] for j in range(len(sg) - 1)] or [0])
+        dur = lambda k: [int(r["End_Timestamp"]) - int(r["Start_Timestamp"]) for r in sg if k in r["Kernel_Name"]]
+        tk, ik = dur("k_tick"), dur("k_iterk")
+        out.append(f"| {i} | {len(sg)} | {span / 1e6:.1f} | {100 * busy / span:.0f}% | "
+                   f"{st.median(gaps) / 1e3:.1f} / {gaps[int(0.9 * (len(gaps) - 1))] / 1e3:.1f} / "
+                   f"{st.mean(gaps) / 1e3:.1f} | {st.mean(tk) / 1e3 if tk else 0:.1f} | "
+                   f"{st.mean(ik) / 1e3 if ik else 0:.1f} |")
+    return "\n".join(out)
+
+
 def main():
     import sys
+    if len(sys.argv) > 1 and sys.argv[1] == "--segments":
+        print(trace_segments(sys.argv[2]))
+        return
     if len(sys.argv) > 1 and sys.argv[1] == "--batch-traffic":
         bp = argparse.ArgumentParser(prog="profile_ops --batch-traffic")
         bp.add_argument("fetch_csv")
@@ -164,8 +207,9 @@ def main():
         bp.add_argument("--px", type=int, default=848 * 800)
         bp.add_argument("--copy-bytes", type=int, default=512 << 20)
         bp.add_argument("--command", default="")
+        bp.add_argument("--k", type=int, default=4, help="iterations per launch of the level-0 launches")
         b = bp.parse_args(sys.argv[2:])
-        print(batch_traffic(b.fetch_csv, b.write_csv, b.storage, b.pairs, b.px, b.copy_bytes, b.command))
+        print(batch_traffic(b.fetch_csv, b.write_csv, b.storage, b.pairs, b.px, b.copy_bytes, b.command, b.k))
         return
     ap = argparse.ArgumentParser()
     ap.add_argument("trace")
