@@ -561,15 +561,14 @@ __device__ __forceinline__ float thresh_core(float divp, float tau, const Primal
     const float lt = lambda * te;
     const float upper = lt * k.iu2;
     const float lower = -upper;
-    float du = 0.f;
-    if ((rho <= upper) && (rho >= lower)) {
-        du = (Iu == 0.f) ? dun : fmaf(-rho, k.ri, dun);
-    } else if (rho < lower) {
-        du = fmaf(lt, Iu, dun);
-    } else if (rho > upper) {
-        du = fmaf(-lt, Iu, dun);
-    }
-    return upos + du;
+    // The three cases (none for NaN) as selects between candidates that are all computed:
+    // the same values as branches, without the divergent control flow per pixel.
+    const bool inside = (rho <= upper) && (rho >= lower);
+    const float d_in = (Iu == 0.f) ? dun : fmaf(-rho, k.ri, dun);
+    const float d_lo = fmaf(lt, Iu, dun);
+    const float d_hi = fmaf(-lt, Iu, dun);
+    const float d_out = (rho < lower) ? d_lo : ((rho > upper) ? d_hi : 0.f);
+    return upos + (inside ? d_in : d_out);
 }
 
 
@@ -1122,6 +1121,22 @@ __device__ __forceinline__ void op_iterk(const Tile &t, int roles, const float4 
             if (!MASKED && outside_circle(x, y, w, h)) fl[k] |= F_OUT;  // the unmasked form's gate
         }
     }
+    // Per own position, once per launch, the last iteration whose dual / primal region
+    // holds it (-1: none; the dual also none outside the image or the mask on the path
+    // with boundary tests): iteration j, r = K - 1 - j, takes the duals of the tile grown
+    // by 2r + 1 to the left / top and r to the right / bottom, and the primal steps of the
+    // tile grown by 2r and r.  The offsets from (x0, y0) depend on the position index only.
+    int jd[NK], jp[NK];
+#pragma unroll
+    for (int k = 0; k < NK; k++) {
+        const int i = tid + k * NT;
+        const int dx = i % B::RW + B::RO, dy = i / B::RW + B::RO;
+        const int rdx = max(max(0, (-dx) / 2), dx - (B::TW - 1)), rdy = max(max(0, (-dy) / 2), dy - (B::TH - 1));
+        const int rpx = max(max(0, (1 - dx) / 2), dx - (B::TW - 1)), rpy = max(max(0, (1 - dy) / 2), dy - (B::TH - 1));
+        const bool staged = i < B::NR;
+        jd[k] = staged && (full || ((fl[k] & F_ACT) && (fl[k] & F_IN))) ? K - 1 - max(rdx, rdy) : -1;
+        jp[k] = staged ? K - 1 - max(rpx, rpy) : -1;
+    }
 
     auto iterate = [&](auto full_c) {
         constexpr bool FULL = decltype(full_c)::value;
@@ -1143,10 +1158,7 @@ __device__ __forceinline__ void op_iterk(const Tile &t, int roles, const float4 
 #pragma unroll
             for (int k = 0; k < NK; k++) {
                 const int i = tid + k * NT, x = ox[k], y = oy[k];
-                if (i >= B::NR) continue;
-                if (!FULL && !(fl[k] & F_ACT && fl[k] & F_IN)) continue;
-                if (x < x0 - 2 * r - 1 || y < y0 - 2 * r - 1 || x > x0 + B::TW - 1 + r || y > y0 + B::TH - 1 + r)
-                    continue;
+                if (j > jd[k]) continue;
                 const int kR = (FULL || x + 1 < w) ? i + 1 : i, kD = (FULL || y + 1 < h) ? i + B::RW : i;
                 DualIn rr;
                 rr.ok = true;
@@ -1193,8 +1205,7 @@ __device__ __forceinline__ void op_iterk(const Tile &t, int roles, const float4 
 #pragma unroll
             for (int k = 0; k < NK; k++) {
                 const int i = tid + k * NT, x = ox[k], y = oy[k];
-                if (i >= B::NR) continue;
-                if (!FULL && !(fl[k] & F_IN)) continue;
+                if (!FULL && (i >= B::NR || !(fl[k] & F_IN))) continue;
                 const long pos = (long)y * s + x;
                 if (!FULL && MASKED && !(fl[k] & F_ACT)) {
                     if (r == 0) {
@@ -1210,7 +1221,7 @@ __device__ __forceinline__ void op_iterk(const Tile &t, int roles, const float4 
                     }
                     continue;
                 }
-                if (x < x0 - 2 * r || y < y0 - 2 * r || x > x0 + B::TW - 1 + r || y > y0 + B::TH - 1 + r) continue;
+                if (j > jp[k]) continue;
                 float un, u_s;
                 float2 vs;
                 if (!FULL && !MASKED && (fl[k] & F_OUT)) {
