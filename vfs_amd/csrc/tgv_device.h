@@ -1121,12 +1121,15 @@ __device__ __forceinline__ void op_iterk(const Tile &t, int roles, const float4 
             if (!MASKED && outside_circle(x, y, w, h)) fl[k] |= F_OUT;  // the unmasked form's gate
         }
     }
-    // Per own position, once per launch, the last iteration whose dual / primal region
-    // holds it (-1: none; the dual also none outside the image or the mask on the path
-    // with boundary tests): iteration j, r = K - 1 - j, takes the duals of the tile grown
-    // by 2r + 1 to the left / top and r to the right / bottom, and the primal steps of the
+    // Per own position, once per launch, the number of iterations whose dual / primal
+    // region holds it (the dual's none outside the image or the mask on the path with
+    // boundary tests): iteration j, r = K - 1 - j, takes the duals of the tile grown by
+    // 2r + 1 to the left / top and r to the right / bottom, and the primal steps of the
     // tile grown by 2r and r.  The offsets from (x0, y0) depend on the position index only.
-    int jd[NK], jp[NK];
+    // Kernels held to 128 registers (three positions per work-item, half workgroups) test
+    // the regions per phase instead: the counts' registers would spill there.
+    constexpr bool kCounts = NK <= 2 && !HALF;
+    int nd[NK], np[NK];
 #pragma unroll
     for (int k = 0; k < NK; k++) {
         const int i = tid + k * NT;
@@ -1134,8 +1137,8 @@ __device__ __forceinline__ void op_iterk(const Tile &t, int roles, const float4 
         const int rdx = max(max(0, (-dx) / 2), dx - (B::TW - 1)), rdy = max(max(0, (-dy) / 2), dy - (B::TH - 1));
         const int rpx = max(max(0, (1 - dx) / 2), dx - (B::TW - 1)), rpy = max(max(0, (1 - dy) / 2), dy - (B::TH - 1));
         const bool staged = i < B::NR;
-        jd[k] = staged && (full || ((fl[k] & F_ACT) && (fl[k] & F_IN))) ? K - 1 - max(rdx, rdy) : -1;
-        jp[k] = staged ? K - 1 - max(rpx, rpy) : -1;
+        nd[k] = staged && (full || ((fl[k] & F_ACT) && (fl[k] & F_IN))) ? K - max(rdx, rdy) : 0;
+        np[k] = staged ? K - max(rpx, rpy) : 0;
     }
 
     auto iterate = [&](auto full_c) {
@@ -1158,7 +1161,14 @@ __device__ __forceinline__ void op_iterk(const Tile &t, int roles, const float4 
 #pragma unroll
             for (int k = 0; k < NK; k++) {
                 const int i = tid + k * NT, x = ox[k], y = oy[k];
-                if (j > jd[k]) continue;
+                if (kCounts) {
+                    if (nd[k] <= j) continue;
+                } else {
+                    if (i >= B::NR) continue;
+                    if (!FULL && !(fl[k] & F_ACT && fl[k] & F_IN)) continue;
+                    if (x < x0 - 2 * r - 1 || y < y0 - 2 * r - 1 || x > x0 + B::TW - 1 + r || y > y0 + B::TH - 1 + r)
+                        continue;
+                }
                 const int kR = (FULL || x + 1 < w) ? i + 1 : i, kD = (FULL || y + 1 < h) ? i + B::RW : i;
                 DualIn rr;
                 rr.ok = true;
@@ -1205,7 +1215,8 @@ __device__ __forceinline__ void op_iterk(const Tile &t, int roles, const float4 
 #pragma unroll
             for (int k = 0; k < NK; k++) {
                 const int i = tid + k * NT, x = ox[k], y = oy[k];
-                if (!FULL && (i >= B::NR || !(fl[k] & F_IN))) continue;
+                if ((!kCounts || !FULL) && i >= B::NR) continue;
+                if (!FULL && !(fl[k] & F_IN)) continue;
                 const long pos = (long)y * s + x;
                 if (!FULL && MASKED && !(fl[k] & F_ACT)) {
                     if (r == 0) {
@@ -1221,7 +1232,11 @@ __device__ __forceinline__ void op_iterk(const Tile &t, int roles, const float4 
                     }
                     continue;
                 }
-                if (j > jp[k]) continue;
+                if (kCounts) {
+                    if (np[k] <= j) continue;
+                } else if (x < x0 - 2 * r || y < y0 - 2 * r || x > x0 + B::TW - 1 + r || y > y0 + B::TH - 1 + r) {
+                    continue;
+                }
                 float un, u_s;
                 float2 vs;
                 if (!FULL && !MASKED && (fl[k] & F_OUT)) {
