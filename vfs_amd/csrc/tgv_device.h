@@ -525,12 +525,15 @@ __device__ __forceinline__ void op_deriv(const Tile &t, int level)
 
 // Per-pixel constants of the primal step (fixed within a warp: they depend on the
 // tensor only): the reciprocals of eta_u (the threshold's step scale) and of the two
-// eta_v, each 1 where the reference divides by zero only in its zero branch.
+// eta_v, each 1 where the reference divides by zero only in its zero branch.  eta_v2 == 0
+// (alpha0 == 0 and a = c = 0), the reference's other zero branch, is r2 = -0 (the
+// reciprocal itself is never -0) with r1 = 1: kept in the float, so that no per-pixel
+// flag has to live in scalar registers across the iterations.
 struct PrimalCoef {
     float rt, r1, r2;
-    float ri, iu2;  // 1 / Iu (0 for Iu = 0) and Iu^2 of the threshold
-    bool e2z;  // eta_v2 == 0 (alpha0 == 0 and a = c = 0): the reference's zero branch
+    float ri;  // 1 / Iu (0 for Iu = 0) of the threshold
 };
+__device__ __forceinline__ bool eta_v2_zero(const PrimalCoef &k) { return __float_as_uint(k.r2) == 0x80000000u; }
 
 __device__ __forceinline__ PrimalCoef primal_coef(const IterParams &ip, float a, float b, float c, float iu)
 {
@@ -541,11 +544,10 @@ __device__ __forceinline__ PrimalCoef primal_coef(const IterParams &ip, float a,
     const float e2 = fmaf(aa, fmaf(a, a, c * c), a04);
     PrimalCoef k;
     k.rt = etau == 0.f ? 1.0f : __builtin_amdgcn_rcpf(etau);
-    k.r1 = e1 == 0.f ? 1.0f : __builtin_amdgcn_rcpf(e1);
-    k.e2z = e2 == 0.f;
-    k.r2 = k.e2z ? 1.0f : __builtin_amdgcn_rcpf(e2);
+    const bool e2z = e2 == 0.f;
+    k.r1 = (e1 == 0.f || e2z) ? 1.0f : __builtin_amdgcn_rcpf(e1);
+    k.r2 = e2z ? -0.0f : __builtin_amdgcn_rcpf(e2);
     k.ri = iu == 0.f ? 0.f : __builtin_amdgcn_rcpf(iu);
-    k.iu2 = iu * iu;
     return k;
 }
 
@@ -559,12 +561,14 @@ __device__ __forceinline__ float thresh_core(float divp, float tau, const Primal
     const float dun = uhat - upos;
     const float rho = fmaf(Iu, dun, Iz);
     const float lt = lambda * te;
-    const float upper = lt * k.iu2;
+    const float upper = lt * (Iu * Iu);
     const float lower = -upper;
     // The three cases (none for NaN) as selects between candidates that are all computed:
-    // the same values as branches, without the divergent control flow per pixel.
+    // the same values as branches, without the divergent control flow per pixel.  For
+    // Iu = 0 (k.ri = 0) the inside case is dun - rho * 0, dun itself up to the sign of a
+    // zero.
     const bool inside = (rho <= upper) && (rho >= lower);
-    const float d_in = (Iu == 0.f) ? dun : fmaf(-rho, k.ri, dun);
+    const float d_in = fmaf(-rho, k.ri, dun);
     const float d_lo = fmaf(lt, Iu, dun);
     const float d_hi = fmaf(-lt, Iu, dun);
     const float d_out = (rho < lower) ? d_lo : ((rho > upper) ? d_hi : 0.f);
@@ -630,6 +634,13 @@ __device__ __forceinline__ void dual_load(DualIn &r, int x, int y, int w, int h,
     }
 }
 
+// T p with T = [[a, c], [c, b]] (thresholdingL1.cu:242-243): Tp of the dual step, and the v
+// step's tensor term (updatePrimalVariables.cu:216-226), the same value.
+__device__ __forceinline__ float2 tensor_times(float a, float b, float c, float2 p)
+{
+    return make_float2(fmaf(a, p.x, c * p.y), fmaf(c, p.x, b * p.y));
+}
+
 // updateDualVariables.cu:152-236 (masked) / :28-98 (unmasked): ascent on p and q with
 // unit-ball projection, then Tp = T p (thresholdingL1.cu:242-243).  `gate_out`: unmasked
 // form, the pixel lies outside the circle gate (p = q = 0 there).
@@ -673,18 +684,18 @@ __device__ __forceinline__ void dual_step(const DualIn &r, bool gate_out, const 
         const float irq = sq > 1.0f ? __builtin_amdgcn_rsqf(sq) : 1.0f;
         qn = make_float4(q0 * irq, q1 * irq, q2 * irq, q3 * irq);
     }
-    tpn = make_float2(fmaf(r.a, pn.x, r.c * pn.y), fmaf(r.c, pn.x, r.b * pn.y));
+    tpn = tensor_times(r.a, r.b, r.c, pn);
 }
 
 // Thresholding and primal step at one pixel (thresholdingL1.cu:176-216 / :60-86,
 // updatePrimalVariables.cu:165-245 / :30-113) from the new duals: divp is the divergence
-// of Tp, qc4 / pp the pixel's new q / p, ql / qu the new q of its left / upper
-// neighbours (read only where they are inside the mask).  Returns u, u_ and v_ of the
-// next iteration.
-__device__ __forceinline__ void primal_core(const IterParams &ip, const PrimalCoef &k, float a, float b, float c,
-                                            float u_pos, float uu, float iu, float iz, float2 vo, bool hR, bool hD,
-                                            bool hL, bool hU, float divp, float4 qc4, float4 ql, float4 qu,
-                                            float2 pp, float &un, float &u_s, float2 &v_s)
+// of Tp, qc4 / tp the pixel's new q / Tp (T p of the new p: the v step's tensor term,
+// computed once in the dual step), ql / qu the new q of its left / upper neighbours (read
+// only where they are inside the mask).  Returns u, u_ and v_ of the next iteration.
+__device__ __forceinline__ void primal_core(const IterParams &ip, const PrimalCoef &k, float u_pos, float uu,
+                                            float iu, float iz, float2 vo, bool hR, bool hD, bool hL, bool hU,
+                                            float divp, float4 qc4, float4 ql, float4 qu, float2 tp, float &un,
+                                            float &u_s, float2 &v_s)
 {
     un = thresh_core(divp, ip.tau, k, u_pos, uu, iu, iz, ip.lambda);
     float qcx = 0.f, qcw = 0.f, qcy = 0.f, qcz = 0.f;
@@ -707,31 +718,24 @@ __device__ __forceinline__ void primal_core(const IterParams &ip, const PrimalCo
     }
     const float dqx = (qcx - qwx) + (qcz - qnz);
     const float dqy = (qcw - qww) + (qcy - qny);
-    const float tx = fmaf(a, pp.x, c * pp.y);
-    const float tyy = fmaf(c, pp.x, b * pp.y);
-    const float gx = fmaf(ip.alpha1, tx, ip.alpha0 * dqx);
-    const float gy = fmaf(ip.alpha1, tyy, ip.alpha0 * dqy);
+    const float gx = fmaf(ip.alpha1, tp.x, ip.alpha0 * dqx);
+    const float gy = fmaf(ip.alpha1, tp.y, ip.alpha0 * dqy);
     // eta_v1, eta_v2 >= 4 alpha0^2 > 0 for alpha0 != 0; the reference's zero branches
-    // (updatePrimalVariables.cu:228-239, the second writes .x) are kept for alpha0 == 0.
-    float vx = fmaf(ip.tau * k.r1, gx, vo.x);
-    float vy;
-    if (!k.e2z) {
-        vy = fmaf(ip.tau * k.r2, gy, vo.y);
-    } else {
-        vx = fmaf(ip.tau, gx, vo.x);
-        vy = 0.f;
-    }
+    // (updatePrimalVariables.cu:228-239, the second writes .x: there vx takes tau itself,
+    // k.r1 = 1) are kept for alpha0 == 0.
+    const float vx = fmaf(ip.tau * k.r1, gx, vo.x);
+    const float vy = eta_v2_zero(k) ? 0.f : fmaf(ip.tau * k.r2, gy, vo.y);
     u_s = fmaf(ip.mu, un - u_pos, un);
     v_s = make_float2(fmaf(ip.mu, vx - vo.x, vx), fmaf(ip.mu, vy - vo.y, vy));
 }
 
 __device__ __forceinline__ void primal_step(const IterParams &ip, float a, float b, float c, float u_pos, float uu,
                                             float iu, float iz, float2 vo, bool hR, bool hD, bool hL, bool hU,
-                                            float divp, float4 qc4, float4 ql, float4 qu, float2 pp, float &un,
+                                            float divp, float4 qc4, float4 ql, float4 qu, float2 tp, float &un,
                                             float &u_s, float2 &v_s)
 {
-    primal_core(ip, primal_coef(ip, a, b, c, iu), a, b, c, u_pos, uu, iu, iz, vo, hR, hD, hL, hU, divp, qc4, ql, qu,
-                pp, un, u_s, v_s);
+    primal_core(ip, primal_coef(ip, a, b, c, iu), u_pos, uu, iu, iz, vo, hR, hD, hL, hU, divp, qc4, ql, qu, tp, un,
+                u_s, v_s);
 }
 
 // Divergence of Tp at a pixel (thresholdingL1.cu:157-175 masked / :29-45 unmasked): tp0
@@ -899,8 +903,8 @@ __device__ __forceinline__ void op_iter(const Tile &t, int roles, const IterPara
         const float4 qu = hU ? sQ[ey - 1][ex] : make_float4(0.f, 0.f, 0.f, 0.f);
         float un, u_s_v;
         float2 v_s_v;
-        primal_step(ip, a, b, c, q.u0, uu[r], iu[r], iz[r], q.v0, hR, hD, hL, hU, divp, sQ[ey][ex], ql, qu,
-                    sP[ey][ex], un, u_s_v, v_s_v);
+        primal_step(ip, a, b, c, q.u0, uu[r], iu[r], iz[r], q.v0, hR, hD, hL, hU, divp, sQ[ey][ex], ql, qu, tp0, un,
+                    u_s_v, v_s_v);
         us[pos] = un;
         u_s[pos] = u_s_v;
         v_s[pos] = v_s_v;
@@ -1250,11 +1254,15 @@ __device__ __forceinline__ void op_iterk(const Tile &t, int roles, const float4 
                     // the Tp plane's outside it
                     const float tpLx = (FULL || x >= 1) ? L.tp[i - 1].x : 0.f;
                     const float tpUy = (FULL || y >= 1) ? L.tp[i - B::RW].y : 0.f;
-                    const float divp = div_tp<MASKED>(x, y, w, h, hR, hD, hL, hU, L.tp[i], tpLx, tpUy);
+                    const float2 tp0 = L.tp[i];
+                    const float divp = div_tp<MASKED>(x, y, w, h, hR, hD, hL, hU, tp0, tpLx, tpUy);
                     const float4 ql = hL ? L.q[i - 1] : make_float4(0.f, 0.f, 0.f, 0.f);
                     const float4 qu = hU ? L.q[i - B::RW] : make_float4(0.f, 0.f, 0.f, 0.f);
-                    primal_core(ip, kc[k], o[k].a, o[k].b, o[k].c, o[k].ub, o[k].u, o[k].iu, o[k].iz, o[k].v, hR, hD, hL,
-                                hU, divp, L.q[i], ql, qu, o[k].p, un, u_s, vs);
+                    // the v step's T p is the staged Tp (the same value); kernels held to 128
+                    // registers recompute it from their registers instead of keeping it
+                    const float2 tpv = kCounts ? tp0 : tensor_times(o[k].a, o[k].b, o[k].c, o[k].p);
+                    primal_core(ip, kc[k], o[k].ub, o[k].u, o[k].iu, o[k].iz, o[k].v, hR, hD, hL, hU, divp, L.q[i], ql,
+                                qu, tpv, un, u_s, vs);
                 }
                 o[k].u = un;
                 o[k].ub = u_s;
