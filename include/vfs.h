@@ -189,8 +189,17 @@ int vfs_get_energy(vfs_solver *s, int level, int warp, vfs_energy_t *out);
  *             four);
  *   "pair"    the same as "per_launch" -1 / 1 / 2 written as -1 / 0 / 1;
  *   "energy"  0 = no energy records, 1 = after the last warp of every level (default),
- *             2 = after every warp (vfs_get_energy).
- * Results do not depend on "graph", "autotune", "shape", "per_launch" or "pair".
+ *             2 = after every warp (vfs_get_energy);
+ *   "persist" -1 = as "autotune" says (default), 0 = never, 1 = wherever possible: a cold
+ *             solve runs each warp's inner iterations in one launch of persistent
+ *             workgroups, one per tile, that keep their tiles on chip and exchange
+ *             borders every iteration (levels without a mask pixel in column 0, as many
+ *             tiles as the device holds workgroups at once).  If such a launch finds not
+ *             every workgroup resident (other kernels on the device), it gives up, and the
+ *             solve runs again without persistent launches at the next call that waits
+ *             for it; vfs_get_option "persist_reruns" counts those runs.
+ * Results do not depend on "graph", "autotune", "shape", "per_launch", "pair" or
+ * "persist".
  * No counterpart in the reference (FP32 only, stereotgv.h:47-100; its block shape is a
  * constructor argument, stereotgv.cu:32-42). */
 int vfs_set_option(vfs_solver *s, const char *key, int value);
@@ -200,16 +209,21 @@ int vfs_get_option(const vfs_solver *s, const char *key, int *value);
  * tile shape index of the inner iterations (0..6 as option "shape"; with several
  * iterations per launch also 7..17 = 8x8, 4x4, 32x32, 20x20, 36x25, 27x25, 14x13, 7x7,
  * 43x34, 34x40, 53x25, and 1, 3, 4, 6 of the former), whether the inner iteration prefetches its Tp-plane
- * neighbours, and the inner iterations per launch, 1, 2, 4 or 8 (`per_launch` may be
- * NULL).  Runs the tuner first when it has not run for this geometry.
+ * neighbours, and the inner iterations per launch, 1, 2, 4 or 8, or VFS_PERSISTENT when
+ * cold solves run a warp's iterations in one persistent launch (tile shapes 7, 8, 11,
+ * 12, 13, 14, 17 or 18 = 53x50) (`per_launch` may be NULL).  Runs the tuner first when
+ * it has not run for this geometry.
  * No counterpart in the reference. */
 int vfs_get_level_tuning(vfs_solver *s, int masked, int level, int *shape, int *preload, int *per_launch);
 /* Sets the launch choices of one level (reproducible profiling, expert use); they hold
  * until the next initialize or option change.  `per_launch` (1, 2, 4 or 8) may carry
  * VFS_HALF_WORKGROUP: multi-iteration workgroups of 512 work-items, two or three
- * positions each, where the tile's staged region holds 513 .. 1536 positions.
+ * positions each, where the tile's staged region holds 513 .. 1536 positions; or it is
+ * VFS_PERSISTENT: cold solves run each warp's iterations in one persistent launch in
+ * tile shape `shape` (other solves keep the level's other choices).
  * No counterpart in the reference. */
 #define VFS_HALF_WORKGROUP 0x100
+#define VFS_PERSISTENT 0x200
 int vfs_set_level_tuning(vfs_solver *s, int masked, int level, int shape, int preload, int per_launch);
 
 /* Timing with HIP events on the solver's stream: runs `n` solves (masked != 0:

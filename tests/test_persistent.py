@@ -1,0 +1,164 @@
+"""Persistent launches (a warp's inner iterations in one launch of resident workgroups
+that exchange tile borders every iteration) against the other launch forms, through the
+C ABI: every plane bit for bit, in every tile shape, storage type, solve form and
+iteration count, since each pixel runs the same code on the same operands.
+"""
+import numpy as np
+import pytest
+
+from vfs_amd import StereoTgv
+from vfs_amd import synth as S
+
+from test_gpu_parity import make_gpu, small_pair
+
+pytestmark = pytest.mark.gpu
+
+PLANES = ("u", "u_", "v_", "warpUV")
+PERSIST_SHAPES = (7, 8, 11, 12, 13, 14, 17, 18)
+
+
+def solve(pair, levels, warps, iters, masked=True, persist=0, storage="f32", per_launch=None, graph=True):
+    g = make_gpu(pair, levels, warps, iters, storage=storage, graph=graph)
+    g.set_option("persist", persist)
+    if per_launch is not None:
+        g.set_option("per_launch", per_launch)
+    (g.solveStereoForwardMasked if masked else g.solveStereoForward)()
+    return g
+
+
+def planes_of(g, levels=(0,), flow=True):
+    out = {f"{n}@{l}": g.read_plane(n, l) for n in PLANES for l in levels}
+    if flow:
+        out["flow"] = g.copyDisparityToHost()
+    return out
+
+
+def set_persistent(g, shape, masked=True, levels=3):
+    """Fixes persistent launches in `shape` at every level where the device can hold all
+    their tiles at once (at most 1024); returns those levels (at least one)."""
+    from vfs_amd import VfsError
+    done = []
+    for lvl in range(levels):
+        try:
+            g.set_level_tuning(lvl, shape, per_launch=StereoTgv.PERSISTENT, masked=masked)
+            done.append(lvl)
+        except VfsError:
+            w, h, _ = g.level_dims(lvl)
+            tw, th = StereoTgv.SHAPES[shape]
+            assert -(-w // tw) * -(-h // th) > 256, (shape, lvl, "persistent launch refused")
+    assert done, shape
+    return done
+
+
+def assert_same(got, ref, what):
+    for n, v in ref.items():
+        assert np.array_equal(got[n], v, equal_nan=True), (what, n, float(np.nanmax(np.abs(got[n] - v))))
+
+
+@pytest.mark.parametrize("storage", ["f32", "f16", "bf16"])
+@pytest.mark.parametrize("masked", [True, False])
+def test_persistent_equals_one_iteration_per_launch(masked, storage):
+    """Option persist 1 (the shape with the most tiles at each level) against one
+    iteration per launch: 15 iterations per warp = one persistent launch of 14 + one."""
+    pair = small_pair(200, 120, seed=37)
+    ref = planes_of(solve(pair, 3, 2, 15, masked, persist=0, storage=storage, per_launch=1), (0, 1))
+    g = solve(pair, 3, 2, 15, masked, persist=1, storage=storage)
+    print(f"persist masked {masked} storage {storage}", flush=True)
+    for lvl in range(3):
+        assert g.level_tuning(lvl, masked, with_per_launch=True)[2] == StereoTgv.PERSISTENT
+    assert_same(planes_of(g, (0, 1)), ref, "persist 1")
+    assert g.get_option("persist_reruns") == 0
+
+
+@pytest.mark.parametrize("masked", [True, False])
+def test_every_persistent_shape(masked):
+    """Every tile shape of the persistent launch at every level (8x8, 4x4, 36x25, 27x25,
+    14x13, 7x7, 53x25, 53x50: one to three staged positions per work-item, partial tiles
+    at the right and bottom edges) equals one iteration per launch."""
+    pair = small_pair(200, 120, seed=41)
+    ref = planes_of(solve(pair, 3, 2, 11, masked, persist=0, per_launch=1))
+    for shape in PERSIST_SHAPES:
+        g = make_gpu(pair, 3, 2, 11)
+        print(f"persistent shape {shape} masked {masked}", flush=True)
+        levels = set_persistent(g, shape, masked)
+        (g.solveStereoForwardMasked if masked else g.solveStereoForward)()
+        for lvl in levels:
+            assert g.level_tuning(lvl, masked, with_per_launch=True)[::2] == (StereoTgv.SHAPES[shape],
+                                                                              StereoTgv.PERSISTENT)
+        assert_same(planes_of(g), ref, f"shape {shape}")
+        assert g.get_option("persist_reruns") == 0
+
+
+@pytest.mark.parametrize("masked", [True, False])
+def test_persistent_odd_geometry(masked):
+    """An odd-width pair (157x94: row padding, partial tiles in both directions; its
+    trajectory field has a zero vector inside the mask, so the planes are compared
+    without the numeric check of the copy-outs, NaN included)."""
+    pair = small_pair(157, 94, seed=43)
+    ref = planes_of(solve(pair, 3, 2, 9, masked, persist=0, per_launch=1), flow=False)
+    for shape in PERSIST_SHAPES:
+        g = make_gpu(pair, 3, 2, 9)
+        print(f"odd geometry persistent shape {shape} masked {masked}", flush=True)
+        set_persistent(g, shape, masked)
+        (g.solveStereoForwardMasked if masked else g.solveStereoForward)()
+        assert_same(planes_of(g, flow=False), ref, f"odd geometry, shape {shape}")
+
+
+@pytest.mark.parametrize("iters", [1, 2, 3, 4, 130])
+def test_persistent_iteration_counts(iters):
+    """Iteration counts with no persistent launch (1), a launch of only its last two
+    iterations (2), a remainder (3), and beyond the 128-entry step table (130 = 128 + 2)."""
+    pair = small_pair(128, 96, seed=47)
+    ref = planes_of(solve(pair, 2, 2, iters, persist=0, per_launch=1))
+    print(f"persistent iterations {iters}", flush=True)
+    assert_same(planes_of(solve(pair, 2, 2, iters, persist=1)), ref, f"{iters} iterations")
+
+
+def test_persistent_with_mask_in_column_0():
+    """A mask pixel in column 0 of a level forbids multi-iteration launches there (the
+    reference's flat-index left neighbour); the other levels still run persistent ones."""
+    pair = small_pair(160, 96, seed=51)
+    m = pair["mask"].copy()
+    m[40:44, 0:3] = 1.0
+    pair = dict(pair, mask=m)
+    ref = planes_of(solve(pair, 3, 2, 10, persist=0, per_launch=1))
+    g = solve(pair, 3, 2, 10, persist=1)
+    ks = [g.level_tuning(l, True, with_per_launch=True)[2] for l in range(3)]
+    print(f"per_launch by level with column-0 mask: {ks}", flush=True)
+    assert ks[0] != StereoTgv.PERSISTENT
+    assert_same(planes_of(g), ref, "mask in column 0")
+
+
+def test_persistent_then_warm_start_and_graphs():
+    """A warm start after a persistent cold solve continues from the same buffers as after
+    any other launch form; direct launches equal graph replays; a second cold solve (the
+    launch counter advanced, the same images copied in again: a solve without new images
+    calibrates I1 again, as the reference's does) gives the same result again."""
+    pair = small_pair(200, 120, seed=53)
+    outs = []
+    for persist, graph in ((0, True), (1, True), (1, False)):
+        g = solve(pair, 3, 3, 12, persist=persist, per_launch=1 if persist == 0 else None, graph=graph)
+        first = planes_of(g)
+        g.copyImagesToDevice(pair["i0"], pair["i1"])
+        g.solveStereoForwardMasked()
+        again = planes_of(g)
+        assert_same(again, first, f"second cold solve persist {persist}")
+        g.solve_warm(1, 2, 10)
+        outs.append((first, planes_of(g)))
+        print(f"warm after persist {persist} graph {graph}", flush=True)
+    for first, warm in outs[1:]:
+        assert_same(first, outs[0][0], "cold")
+        assert_same(warm, outs[0][1], "warm")
+
+
+def test_persistent_848x800_workload():
+    """The 848x800 L5 W10 I50 masked solve (53x50 tiles at level 0, three staged positions
+    per work-item) with the tuner's choices equals one iteration per launch."""
+    pair = S.make_pair(848, 800, seed=2, radius=400, dmax=40)
+    ref = planes_of(solve(pair, 5, 10, 50, persist=0, per_launch=1))
+    g = solve(pair, 5, 10, 50, persist=1)
+    tun = [g.level_tuning(l, True, with_per_launch=True) for l in range(5)]
+    print(f"848x800 persistent tuning: {tun}", flush=True)
+    assert tun[0][::2] == (StereoTgv.SHAPES[18], StereoTgv.PERSISTENT)
+    assert_same(planes_of(g), ref, "848x800")
+    assert g.get_option("persist_reruns") == 0

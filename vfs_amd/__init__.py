@@ -305,9 +305,9 @@ class StereoTgv:
 
     # tile shapes by index: 0 .. 6 are those of the "shape" option; 8x8, 4x4, 32x32,
     # 20x20, 36x25, 27x25, 14x13, 7x7, 43x34, 34x40 and 53x25 exist for the multi-iteration
-    # launches only
+    # launches only, 53x50 for the persistent launches only
     SHAPES = ((64, 8), (64, 4), (128, 4), (32, 16), (32, 8), (16, 32), (16, 16), (8, 8), (4, 4), (32, 32),
-              (20, 20), (36, 25), (27, 25), (14, 13), (7, 7), (43, 34), (34, 40), (53, 25))
+              (20, 20), (36, 25), (27, 25), (14, 13), (7, 7), (43, 34), (34, 40), (53, 25), (53, 50))
 
     def level_tuning(self, level: int, masked: bool = True, with_per_launch: bool = False):
         """(tile shape (w, h), Tp prefetch[, inner iterations per launch]) used at
@@ -320,11 +320,13 @@ class StereoTgv:
         return self.SHAPES[sh.value], bool(pre.value)
 
     HALF_WORKGROUP = 0x100
+    # per_launch of a level whose cold solves run a warp's iterations in one persistent launch
+    PERSISTENT = 0x200
 
     def set_level_tuning(self, level: int, shape: int, preload: bool = False, per_launch: int = 1,
                          masked: bool = True, half_workgroup: bool = False):
         """Fix the launch choices of one level (shape index as option "shape"; 1, 2, 4 or 8
-        inner iterations per launch, optionally in half workgroups)."""
+        inner iterations per launch, optionally in half workgroups, or PERSISTENT)."""
         k = int(per_launch) | (self.HALF_WORKGROUP if half_workgroup else 0)
         return self._check(self._L.vfs_set_level_tuning(self._h, int(masked), level, shape, int(preload), k),
                            "set_level_tuning")

@@ -22,10 +22,10 @@ HIP_FLAGS = ["--offload-arch=" + ARCH, "-O3", "-std=c++17", "-fPIC", "-Wall", "-
 
 # Translation units of libvfs.so: the launchers and the host solver, plus the op kernels
 # instantiated once per storage type (0 float, 1 __half, 2 __hip_bfloat16) and kernel
-# family (1 = tick kernel; 2, 4, 8 = K-iteration kernel) from tgv_inst.hip, compiled in
-# parallel into objects under vfs_amd/_build/.
+# family (1 = tick kernel; 2, 4, 8 = K-iteration kernel; 0 = whole-warp persistent kernel)
+# from tgv_inst.hip, compiled in parallel into objects under vfs_amd/_build/.
 LIB_SOURCES = ["kernels.hip", "solver.hip"]
-INST_UNITS = [(st, k) for st in (0, 1, 2) for k in (1, 2, 4, 8)]
+INST_UNITS = [(st, k) for st in (0, 1, 2) for k in (1, 2, 4, 8, 0)]
 BUILD_DIR = os.path.join(PKG, "_build")
 
 
@@ -56,7 +56,7 @@ def build_lib(verbose=False) -> str:
         compiles.append([HIPCC] + HIP_FLAGS + [f"-DVFS_INST_STORAGE={st}", f"-DVFS_INST_K={k}", "-c",
                                               os.path.join(CSRC, "tgv_inst.hip"), "-o", obj])
     # the slowest units (multi-iteration kernels) first
-    compiles.sort(key=lambda c: 0 if any(f"-DVFS_INST_K={k}" in c for k in (2, 4, 8)) else 1)
+    compiles.sort(key=lambda c: 0 if any(f"-DVFS_INST_K={k}" in c for k in (0, 2, 4, 8)) else 1)
     with ThreadPoolExecutor(_jobs()) as ex:
         list(ex.map(lambda c: _run(c, verbose), compiles))
     out = os.path.join(PKG, "libvfs.so")

@@ -6,8 +6,8 @@
 namespace vfs {
 
 // Op launches, one per (storage type, kernel family): tgv_inst.hip built with
-// VFS_INST_STORAGE (0 float, 1 __half, 2 __hip_bfloat16) and VFS_INST_K (1: k_tick, else
-// k_iterk with that many iterations).
+// VFS_INST_STORAGE (0 float, 1 __half, 2 __hip_bfloat16) and VFS_INST_K (1: k_tick, 0:
+// k_iterp, else k_iterk with that many iterations).
 #define VFS_INST_DECL(sfx) \
     void launch_ops_##sfx(const PairDesc *desc, const TickArgs &args, int shape, hipStream_t st);
 VFS_INST_DECL(f32_k1)
@@ -22,6 +22,12 @@ VFS_INST_DECL(bf16_k1)
 VFS_INST_DECL(bf16_k2)
 VFS_INST_DECL(bf16_k4)
 VFS_INST_DECL(bf16_k8)
+VFS_INST_DECL(f32_k0)
+VFS_INST_DECL(f16_k0)
+VFS_INST_DECL(bf16_k0)
+int persist_capacity_f32(int shape);
+int persist_capacity_f16(int shape);
+int persist_capacity_bf16(int shape);
 
 namespace {
 
@@ -231,6 +237,16 @@ __global__ void k_widen(const T *__restrict__ src, int nch, float *__restrict__ 
 hipError_t launch_tick(const PairDesc *desc, const TickArgs &args, int storage, int shape, hipStream_t st)
 {
     if (args.n < 1 || args.total < 1) return hipSuccess;
+    if (args.e[0].kind == OP_ITERP) {
+        const int n = iterp_count(args.e[0].it0);
+        if (args.n != 1 || shape < 0 || shape >= kShapes || !persist_shape(shape) || n < 2 || n % 2 ||
+            iterp_first(args.e[0].it0) + n > kMaxSched || args.total > kMaxPersistTiles)
+            return hipErrorInvalidValue;
+        static constexpr void (*kP[3])(const PairDesc *, const TickArgs &, int, hipStream_t) = {
+            launch_ops_f32_k0, launch_ops_f16_k0, launch_ops_bf16_k0};
+        kP[storage == kF16 ? 1 : storage == kBF16 ? 2 : 0](desc, args, shape, st);
+        return hipSuccess;
+    }
     if (op_multi(args.e[0].kind)) {
         const int K = op_iterations(args.e[0].kind), sh = shape_index(shape);
         if (shape < 0 || sh >= kShapes || !iterk_ok(sh, K) ||
@@ -248,6 +264,13 @@ hipError_t launch_tick(const PairDesc *desc, const TickArgs &args, int storage, 
     const int si = storage == kF16 ? 1 : storage == kBF16 ? 2 : 0;
     kFn[si][fam](desc, args, shape, st);
     return hipSuccess;
+}
+
+int persist_capacity(int storage, int shape)
+{
+    if (shape < 0 || shape >= kShapes || !persist_shape(shape)) return 0;
+    return storage == kF16 ? persist_capacity_f16(shape)
+                           : storage == kBF16 ? persist_capacity_bf16(shape) : persist_capacity_f32(shape);
 }
 
 void launch_fill_zero(void *ptr, size_t bytes, hipStream_t st)

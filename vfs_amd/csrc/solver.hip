@@ -47,8 +47,10 @@ struct GraphKey {
     int masked, warm_level, warps, iters, start_roles, monitor;
     float limit;
     std::array<unsigned int, kMaxLevels> tuning;  // Tuning::sig() of the program's launch choices
+    int persist = 0;  // the program runs the tuning's persistent launches
     bool operator<(const GraphKey &o) const
     {
+        if (persist != o.persist) return persist < o.persist;
         if (tuning != o.tuning) return tuning < o.tuning;
         if (masked != o.masked) return masked < o.masked;
         if (warm_level != o.warm_level) return warm_level < o.warm_level;
@@ -101,8 +103,12 @@ struct LevelTune {
     // with four or eight iterations per launch: tile shape and half flag of the
     // two-iteration launches that finish an iteration count not divisible by four
     unsigned char m2shape = 4, m2half = 0;
+    // a warp's iterations in one launch of persistent workgroups (vfs::OP_ITERP), in tile
+    // shape pshape, wherever the solve allows it (a single pair's cold solve); otherwise
+    // the launches above
+    unsigned char persist = 0, pshape = 18;
     // tile shape of the inner iterations
-    int iter_shape() const { return per_launch > 1 ? mshape : shape; }
+    int iter_shape() const { return persist ? pshape : per_launch > 1 ? mshape : shape; }
 };
 
 struct Tuning {
@@ -116,7 +122,8 @@ struct Tuning {
             // shapes 5 bits each (kShapes <= 32) with their half flags
             k[l] = (unsigned int)lv[l].shape | (unsigned int)lv[l].preload << 3 | (unsigned int)code << 4 |
                    (unsigned int)lv[l].mshape << 6 | (unsigned int)lv[l].mhalf << 11 |
-                   (unsigned int)lv[l].m2shape << 12 | (unsigned int)lv[l].m2half << 17;
+                   (unsigned int)lv[l].m2shape << 12 | (unsigned int)lv[l].m2half << 17 |
+                   (unsigned int)lv[l].persist << 18 | (unsigned int)lv[l].pshape << 19;
         }
         return k;
     }
@@ -170,6 +177,7 @@ struct vfs_solver {
     int force_shape = -1;   // option "shape": -1 = per-level tuner
     bool autotune = true;   // option "autotune"
     int force_per_launch = -1;  // option "per_launch": -1 = tuner, else 1, 2, 4 or 8 wherever allowed
+    int persist_opt = -1;   // option "persist": -1 = tuner, 0 = never, 1 = wherever possible
     int monitor = 1;        // option "energy": 0, 1 or 2 (Params::monitor)
     bool mask_col0[kMaxLevels] = {};  // a mask pixel lies in column 0 of the level
     unsigned long long mask_gen = 0;  // copyMaskToDevice calls since create (batches re-tune on change)
@@ -195,6 +203,15 @@ struct vfs_solver {
     // work arena
     float *arena = nullptr;
     long long plane = 0;
+    // exchange words and launch bookkeeping of the persistent op (allocated on first use)
+    unsigned long long *xbuf = nullptr;
+    size_t xwords = 0;
+    vfs::XSync *xsync = nullptr;
+    // the last solve ran persistent launches whose abort flag the host has not read yet;
+    // its solve form, to run it again without them if a launch gave up
+    bool persist_check = false;
+    int persist_masked = 0;
+    int persist_reruns = 0;  // solves run again without persistent launches
     int roles = vfs::kRolesInit;  // buffer roles after the last solve
     bool solved = false;  // a solve has run since initialize
 
@@ -232,6 +249,8 @@ struct vfs_solver {
         for (int i = 0; i < vfs::kMaxSched; i++) d.sched[i].w = alpha1 * d.sched[i].x / 3.0f;
         d.mon_rec = mon;
         d.mon_part = mon ? mon + kMaxLevels * vfs::kMonWarps : nullptr;
+        d.xbuf = xbuf;
+        d.xsync = xsync;
         return d;
     }
 };
@@ -350,9 +369,11 @@ void push_op(Program &pr, int kind, int level, int masked, int roles)
 
 // One level of the solve from the tensor to the end of the last warp
 // (stereotgv.cu:409-494, masked; :275-364 unmasked).  With `sync_spare`, each warp's
-// first multi-iteration op is preceded by OP_SYNC (warm solves above level 0).
+// first multi-iteration op is preceded by OP_SYNC (warm solves above level 0).  With
+// `persist`, a warp's iterations run in OP_ITERP ops (an even count each, within the
+// schedule table) and the rest one at a time.
 void push_level(Program &pr, const Params &P, int masked, int l, int warps, int iters, int &roles, int per_launch,
-                bool sync_spare = false)
+                bool sync_spare = false, bool persist = false)
 {
     std::vector<float4> sched((size_t)std::max(iters, 1));
     iter_schedule(P.timestep_lambda, iters, sched.data());
@@ -373,6 +394,16 @@ void push_level(Program &pr, const Params &P, int masked, int l, int warps, int 
             int k = 1;
             for (int c : {8, 4, 2})
                 if (k == 1 && c <= per_launch && it + c <= iters && it + c <= vfs::kMaxSched) k = c;
+            const int kp = std::min(iters - it, vfs::kMaxSched - it) & ~1;
+            if (persist && !sync_spare && kp >= 2) {
+                o.kind = vfs::OP_ITERP;
+                o.it0 = (kp << 16) | it;
+                o.flags = (unsigned char)((it == 0 ? vfs::EF_FIRST : 0) | (it + kp == iters ? vfs::EF_WRITE_TP_LAST : 0));
+                roles = vfs::rrotate_iter(roles);
+                pr.ops.push_back(o);
+                it += kp;
+                continue;
+            }
             if (k > 1) {
                 // the spare buffers' copy beyond the level's extent runs with the
                 // derivatives (EF_SYNC), which read none of u, u_, v_
@@ -414,6 +445,10 @@ void apply_tuning(Program &pr, const Tuning &tn)
 {
     for (ProgOp &o : pr.ops) {
         const LevelTune &lt = tn.lv[vfs::op_tile_level(o.kind, o.level)];
+        if (o.kind == vfs::OP_ITERP) {
+            o.shape = lt.pshape;
+            continue;
+        }
         const bool multi = vfs::op_multi(o.kind);
         const bool rest = o.kind == vfs::OP_ITER2 && lt.per_launch >= 4;  // the remainder of a warp
         o.shape = multi ? (rest ? lt.m2shape : lt.mshape) : lt.shape;
@@ -430,9 +465,10 @@ void apply_tuning(Program &pr, const Tuning &tn)
 // from zeroed buffers, all levels with the configured warps and iterations.
 // Warm (warm_level >= 0): the previous solve's u, u_ and warpUV are downscaled (x 1/fScale
 // per level) to warm_level and the solve runs from there to level 0 with `warps` and
-// `iters`; nothing is zeroed.
+// `iters`; nothing is zeroed.  `persist`: the levels whose tuning says so run their
+// warps' iterations in persistent launches (single-pair cold solves only).
 Program build_program(const Params &P, int masked, int warm_level, int warps, int iters, int start_roles,
-                      const Tuning &tn)
+                      const Tuning &tn, bool persist = false)
 {
     Program pr;
     int roles = vfs::kRolesInit;
@@ -442,7 +478,8 @@ Program build_program(const Params &P, int masked, int warm_level, int warps, in
         for (int l = 1; l < P.nLevels; l++) push_op(pr, vfs::OP_DOWN, l, masked, 0);
         for (int l = P.nLevels - 1; l >= 0; l--) {
             if (l == P.nLevels - 1) push_op(pr, vfs::OP_CINIT, l, masked, roles);
-            push_level(pr, P, masked, l, P.nWarpIters, P.nSolverIters, roles, tn.lv[l].per_launch);
+            push_level(pr, P, masked, l, P.nWarpIters, P.nSolverIters, roles, tn.lv[l].per_launch, false,
+                       persist && tn.lv[l].persist);
             if (l > 0) {
                 push_op(pr, vfs::OP_UP, l, masked, roles);
                 roles = vfs::rswap(vfs::rswap(roles, vfs::G_U), vfs::G_UB) ^ vfs::RB_UV;
@@ -590,6 +627,10 @@ void release(vfs_solver *s)
     s->staging = nullptr;
     s->staging_bytes = 0;
     s->arena = nullptr;
+    s->xbuf = nullptr;
+    s->xwords = 0;
+    s->xsync = nullptr;
+    s->persist_check = false;
     s->mon = nullptr;
     s->mon_slots.clear();
     s->initialized = s->has_mask = s->has_fields = s->has_images = false;
@@ -647,6 +688,95 @@ int ensure_engine(vfs_solver *s)
     return VFS_OK;
 }
 
+// ---- persistent launches (vfs::OP_ITERP)
+
+// Workgroups of the persistent op that the device holds at once, per (device, storage, shape).
+int persist_cap(int device, int storage, int shape)
+{
+    static std::mutex mu;
+    static std::map<std::array<int, 3>, int> cache;
+    std::lock_guard<std::mutex> g(mu);
+    const std::array<int, 3> k{device, storage, shape};
+    auto it = cache.find(k);
+    if (it != cache.end()) return it->second;
+    const int c = vfs::persist_capacity(storage, shape);
+    cache[k] = c;
+    return c;
+}
+
+// Tiles of the persistent op at `level` in tile shape `sh`, 0 where it cannot run: a
+// shape without the op, a mask pixel in column 0 (as for every multi-iteration op), or more
+// tiles than the device holds workgroups at once (or than kMaxPersistTiles).
+int persist_tiles(const vfs_solver *s, int masked, int level, int sh)
+{
+    if (sh < 0 || sh >= vfs::kShapes || !vfs::persist_shape(sh)) return 0;
+    if (s->lv[level].w < 2 || (masked && s->mask_col0[level])) return 0;
+    const vfs::ShapeDim d = vfs::kShape[sh];
+    const int n = ((s->lv[level].w + d.tw - 1) / d.tw) * ((s->lv[level].h + d.th - 1) / d.th);
+    if (n > vfs::kMaxPersistTiles || n > persist_cap(s->device, s->storage, sh)) return 0;
+    return n;
+}
+
+// The persistent op's shape at `level` when the option forces it: the one with the most
+// tiles the device holds at once; -1 if none.
+int persist_default_shape(const vfs_solver *s, int masked, int level)
+{
+    int best = -1, best_n = 0;
+    for (int sh = 0; sh < vfs::kShapes; sh++) {
+        const int n = persist_tiles(s, masked, level, sh);
+        if (n > best_n) best = sh, best_n = n;
+    }
+    return best;
+}
+
+// Exchange words (both parities) of a persistent launch of `tiles` tiles in shape `sh`.
+size_t persist_words_of(int tiles, int sh) { return (size_t)tiles * 2 * (size_t)vfs::persist_words(sh); }
+
+// Allocates the launch bookkeeping and at least `words` exchange words.  The tags in the
+// words start at launch counter 1, so zeroed words never match.
+int ensure_xbuf(vfs_solver *s, size_t words)
+{
+    if (!s->xsync) {
+        int rc = dev_alloc(s, (void **)&s->xsync, sizeof(vfs::XSync));
+        if (rc) return rc;
+        const vfs::XSync init{1u, 0u, 0u, 0u};
+        HIPCHK(s, hipMemcpyAsync(s->xsync, &init, sizeof init, hipMemcpyHostToDevice, s->stream));
+        HIPCHK(s, hipStreamSynchronize(s->stream));
+        s->desc_dirty = true;
+    }
+    if (words > s->xwords) {
+        HIPCHK(s, hipStreamSynchronize(s->stream));
+        if (s->xbuf) {
+            s->allocs.erase(std::find(s->allocs.begin(), s->allocs.end(), (void *)s->xbuf));
+            HIPCHK(s, hipFree(s->xbuf));
+            s->xbuf = nullptr;
+            s->xwords = 0;
+        }
+        int rc = dev_alloc(s, (void **)&s->xbuf, words * sizeof(unsigned long long));
+        if (rc) return rc;
+        s->xwords = words;
+        s->desc_dirty = true;
+    }
+    return VFS_OK;
+}
+
+// Reads the abort flag of the persistent launches (after the stream's work) and clears
+// it; true if a launch gave up.
+int persist_aborted(vfs_solver *s, bool *aborted)
+{
+    *aborted = false;
+    if (!s->xsync) return VFS_OK;
+    vfs::XSync x;
+    HIPCHK(s, hipMemcpyAsync(&x, s->xsync, sizeof x, hipMemcpyDeviceToHost, s->stream));
+    HIPCHK(s, hipStreamSynchronize(s->stream));
+    if (!x.abort) return VFS_OK;
+    *aborted = true;
+    const unsigned zero = 0;
+    HIPCHK(s, hipMemcpyAsync(&s->xsync->abort, &zero, sizeof zero, hipMemcpyHostToDevice, s->stream));
+    HIPCHK(s, hipStreamSynchronize(s->stream));
+    return VFS_OK;
+}
+
 // ---- per-level launch tuner
 //
 // The inner iteration (the dominant op) of each level is timed in every tile shape, with
@@ -657,11 +787,12 @@ int ensure_engine(vfs_solver *s)
 // kernels compute every pixel identically in every shape.
 struct TuneKey {
     int device, storage, masked, w, h, s, multi_ok;
+    int persist_iters;  // iterations of the persistent candidates (0: none)
     bool operator<(const TuneKey &o) const
     {
-        const int a[7] = {device, storage, masked, w, h, s, multi_ok};
-        const int b[7] = {o.device, o.storage, o.masked, o.w, o.h, o.s, o.multi_ok};
-        return std::lexicographical_compare(a, a + 7, b, b + 7);
+        const int a[8] = {device, storage, masked, w, h, s, multi_ok, persist_iters};
+        const int b[8] = {o.device, o.storage, o.masked, o.w, o.h, o.s, o.multi_ok, o.persist_iters};
+        return std::lexicographical_compare(a, a + 8, b, b + 8);
     }
 };
 
@@ -677,12 +808,23 @@ LevelTune default_tune(int level)
     return t;
 }
 
+// Persistent candidates of a single pair's tuning: the iterations of one launch (a warp's,
+// even, within the schedule table), per level and shape the tiles of the launch (0: not
+// possible there), and the pair's launch bookkeeping.
+struct PersistCands {
+    int iters = 0;
+    std::vector<std::array<int, vfs::kShapes>> tiles;  // [level index][shape]
+    vfs::XSync *xsync = nullptr;
+};
+
 // Times the inner iteration of every level in `levels` in each (tile shape, Tp preload)
 // on the launch set `descs` (their arenas point at scratch memory; `d_desc` is any valid
 // descriptor array of at least descs.size() entries) and stores the fastest choices in tn.
+// With `pc` (a single pair), the persistent op in each of its shapes is a candidate too;
+// if one of its launches gave up (not every workgroup resident), none is taken.
 hipError_t time_launch_choices(hipStream_t st, const vfs::PairDesc *d_desc, const std::vector<vfs::PairDesc> &descs,
                                const Params &P, int masked, const std::vector<int> &levels,
-                               const std::vector<bool> &allow_multi, Tuning &tn)
+                               const std::vector<bool> &allow_multi, Tuning &tn, const PersistCands *pc = nullptr)
 {
     // a short timing of every candidate, then a longer one of those within 10 % of the
     // fastest of their kind (one-iteration, multi-iteration, two-iteration launches)
@@ -690,9 +832,16 @@ hipError_t time_launch_choices(hipStream_t st, const vfs::PairDesc *d_desc, cons
     constexpr float kRefine = 1.10f;
     // candidates: (shape, preload) of one iteration per launch, then two-, four- and
     // eight-iteration launches for the shapes that have them, in full or half workgroups
-    constexpr int kSingle = vfs::kTickShapes * 2, kCand = kSingle + 3 * vfs::kShapes * 2;
+    // (then the persistent op in each shape)
+    constexpr int kSingle = vfs::kTickShapes * 2, kMulti = kSingle + 3 * vfs::kShapes * 2;
+    constexpr int kCand = kMulti + vfs::kShapes;
     auto cand = [&](int c, int &shape, int &pre, int &per_launch, int &half) {
-        if (c < kSingle) {
+        if (c >= kMulti) {
+            shape = c - kMulti;
+            pre = 0;
+            half = 0;
+            per_launch = pc ? pc->iters : 0;
+        } else if (c < kSingle) {
             shape = c / 2;
             pre = c % 2;
             per_launch = 1;
@@ -721,8 +870,11 @@ hipError_t time_launch_choices(hipStream_t st, const vfs::PairDesc *d_desc, cons
             for (int c = 0; c < kCand && r == hipSuccess; c++) {
                 int shape, pre, per_launch, half;
                 cand(c, shape, pre, per_launch, half);
-                if (per_launch > 1 && (!vfs::iterk_ok(shape, per_launch) || !allow_multi[li] ||
-                                       (half && !vfs::half_workgroup_ok(shape, per_launch))))
+                const bool persist = c >= kMulti;
+                if (persist && (!pc || pc->iters < 2 || pc->tiles[li][shape] == 0 || n != 1)) continue;
+                if (!persist && per_launch > 1 &&
+                    (!vfs::iterk_ok(shape, per_launch) || !allow_multi[li] ||
+                     (half && !vfs::half_workgroup_ok(shape, per_launch))))
                     continue;
                 if (round > 0) {
                     const float b = best[li * kCand + c];
@@ -731,7 +883,7 @@ hipError_t time_launch_choices(hipStream_t st, const vfs::PairDesc *d_desc, cons
                     if (!near) continue;
                 }
                 ProgOp o{};
-                o.kind = iter_kind(per_launch);
+                o.kind = persist ? (unsigned char)vfs::OP_ITERP : iter_kind(per_launch);
                 o.level = (unsigned char)levels[li];
                 o.masked = (unsigned char)masked;
                 o.flags = (unsigned char)(pre ? vfs::EF_PRELOAD_TP : 0);
@@ -740,13 +892,13 @@ hipError_t time_launch_choices(hipStream_t st, const vfs::PairDesc *d_desc, cons
                 o.sigma = 1.0f;
                 o.tau = 1.0f;
                 o.mu = 0.8f;
-                o.it0 = 0;
+                o.it0 = persist ? (per_launch << 16) : 0;
                 vfs::TickArgs a = tick_header(P);
                 int acc = 0;
                 for (int k = 0; k < n; k++) {
                     const vfs::PairDesc &d = descs[k];
                     set_entry(a.e[k], o, k, d, acc);
-                    acc += vfs::op_tiles(vfs::OP_ITER, o.level, d.w, d.h, d.s, shape);
+                    acc += vfs::op_tiles(o.kind, o.level, d.w, d.h, d.s, shape);
                 }
                 a.n = n;
                 a.total = acc;
@@ -768,6 +920,21 @@ hipError_t time_launch_choices(hipStream_t st, const vfs::PairDesc *d_desc, cons
     if (e0) (void)hipEventDestroy(e0);
     if (e1) (void)hipEventDestroy(e1);
     if (r != hipSuccess) return r;
+    // a persistent launch that gave up timed nothing: take none
+    if (pc && pc->xsync) {
+        vfs::XSync x;
+        r = hipMemcpyAsync(&x, pc->xsync, sizeof x, hipMemcpyDeviceToHost, st);
+        if (r == hipSuccess) r = hipStreamSynchronize(st);
+        if (r != hipSuccess) return r;
+        if (x.abort) {
+            const unsigned zero = 0;
+            r = hipMemcpyAsync(&pc->xsync->abort, &zero, sizeof zero, hipMemcpyHostToDevice, st);
+            if (r == hipSuccess) r = hipStreamSynchronize(st);
+            if (r != hipSuccess) return r;
+            for (size_t li = 0; li < levels.size(); li++)
+                for (int c = kMulti; c < kCand; c++) best[li * kCand + c] = 1e30f;
+        }
+    }
     for (size_t li = 0; li < levels.size(); li++) {
         const int l = levels[li];
         const LevelTune def = default_tune(l);
@@ -786,7 +953,7 @@ hipError_t time_launch_choices(hipStream_t st, const vfs::PairDesc *d_desc, cons
         }
         float best2 = 1e30f;  // the fastest two-iteration launch, for remainders
         int bestm = -1;       // the fastest multi-iteration launch
-        for (int c = kSingle; c < kCand; c++) {
+        for (int c = kSingle; c < kMulti; c++) {
             const float ms = best[li * kCand + c];
             int shape, pre, per_launch, half;
             cand(c, shape, pre, per_launch, half);
@@ -803,6 +970,16 @@ hipError_t time_launch_choices(hipStream_t st, const vfs::PairDesc *d_desc, cons
             pick.mshape = (unsigned char)shape;
             pick.per_launch = (unsigned char)per_launch;
             pick.mhalf = (unsigned char)half;
+            pick_ms = best[li * kCand + bestm];
+        }
+        // the persistent op where it is clearly faster than both (the launches above stay
+        // the level's choice wherever it cannot run)
+        int bestp = -1;
+        for (int c = kMulti; c < kCand; c++)
+            if (best[li * kCand + c] < 1e29f && (bestp < 0 || best[li * kCand + c] < best[li * kCand + bestp])) bestp = c;
+        if (bestp >= 0 && best[li * kCand + bestp] < 0.98f * pick_ms) {
+            pick.persist = 1;
+            pick.pshape = (unsigned char)(bestp - kMulti);
         }
         tn.lv[l] = pick;
     }
@@ -820,6 +997,28 @@ std::vector<bool> multi_allowed(const vfs_solver *s, int masked, const std::vect
 
 int measure_levels(vfs_solver *s, int masked, const std::vector<int> &levels, Tuning &tn)
 {
+    // persistent candidates (a warp's iterations per launch, within the schedule table)
+    PersistCands pc;
+    if (s->persist_opt != 0) {
+        pc.iters = std::min(s->nSolverIters, vfs::kMaxSched) & ~1;
+        size_t words = 0;
+        for (int l : levels) {
+            std::array<int, vfs::kShapes> t{};
+            for (int sh = 0; sh < vfs::kShapes; sh++) {
+                t[sh] = persist_tiles(s, masked, l, sh);
+                words = std::max(words, persist_words_of(t[sh], sh));
+            }
+            pc.tiles.push_back(t);
+        }
+        if (pc.iters >= 2 && words > 0) {
+            int rc = ensure_xbuf(s, words);
+            if (!rc) rc = ensure_engine(s);
+            if (rc) return rc;
+            pc.xsync = s->xsync;
+        } else {
+            pc.iters = 0;
+        }
+    }
     const size_t bytes = sizeof(float) * (size_t)s->plane * vfs::PL_COUNT;
     float *scratch = nullptr;
     HIPCHK(s, hipMalloc(&scratch, bytes));
@@ -827,7 +1026,7 @@ int measure_levels(vfs_solver *s, int masked, const std::vector<int> &levels, Tu
     std::vector<vfs::PairDesc> d{s->desc()};
     d[0].arena = scratch;
     const hipError_t r = time_launch_choices(s->stream, s->eng.d_desc, d, s->params(), masked, levels,
-                                             multi_allowed(s, masked, levels), tn);
+                                             multi_allowed(s, masked, levels), tn, pc.iters >= 2 ? &pc : nullptr);
     (void)hipStreamSynchronize(s->stream);
     (void)hipFree(scratch);
     if (r != hipSuccess) return fail(s, VFS_ERR_DEVICE, "launch tuner: %s", hipGetErrorString(r));
@@ -848,9 +1047,10 @@ int tune_solver(vfs_solver *s, int masked)
         }
     } else if (s->autotune) {
         std::vector<int> todo;
+        const int piters = s->persist_opt != 0 ? std::min(s->nSolverIters, vfs::kMaxSched) & ~1 : 0;
         auto key = [&](int l) {
             const int ok = multi_allowed(s, m, std::vector<int>{l})[0] ? 1 : 0;
-            return TuneKey{s->device, s->storage, m, s->lv[l].w, s->lv[l].h, s->lv[l].s, ok};
+            return TuneKey{s->device, s->storage, m, s->lv[l].w, s->lv[l].h, s->lv[l].s, ok, piters};
         };
         {
             std::lock_guard<std::mutex> g(g_tune_mu);
@@ -868,19 +1068,94 @@ int tune_solver(vfs_solver *s, int masked)
             for (int l : todo) g_tune[key(l)] = tn.lv[l];
         }
     }
-    // option "per_launch": iterations per launch wherever allowed (1 everywhere else)
+    // option "per_launch": iterations per launch wherever allowed (1 everywhere else), no
+    // persistent launches
     if (s->force_per_launch >= 0) {
         std::vector<int> all;
         for (int l = 0; l < s->nLevels; l++) all.push_back(l);
         const std::vector<bool> ok = multi_allowed(s, m, all);
         for (int l = 0; l < s->nLevels; l++) {
             tn.lv[l].per_launch = (unsigned char)(ok[l] ? s->force_per_launch : 1);
+            tn.lv[l].persist = 0;
             if (!multi_shape(tn.lv[l].mshape)) tn.lv[l].mshape = 4;
             if (!multi_shape(tn.lv[l].m2shape)) tn.lv[l].m2shape = 4;
         }
     }
+    // option "persist": 0 never, 1 wherever the op can run (in the shape with the most tiles)
+    for (int l = 0; l < s->nLevels; l++) {
+        if (s->persist_opt == 0) tn.lv[l].persist = 0;
+        if (s->persist_opt == 1) {
+            const int sh = persist_default_shape(s, m, l);
+            tn.lv[l].persist = sh >= 0 ? 1 : 0;
+            if (sh >= 0) tn.lv[l].pshape = (unsigned char)sh;
+        }
+        // (a fixed or cached choice the device cannot hold at once is not taken)
+        if (tn.lv[l].persist && persist_tiles(s, m, l, tn.lv[l].pshape) == 0) tn.lv[l].persist = 0;
+    }
     s->tuned[m] = true;
     return VFS_OK;
+}
+
+// Whether a cold solve with these launch choices runs persistent launches, and the
+// exchange words they need.
+bool persist_program(const vfs_solver *s, int masked, const Tuning &tn, size_t *words)
+{
+    bool any = false;
+    *words = 0;
+    if (s->persist_opt == 0 || (std::min(s->nSolverIters, vfs::kMaxSched) & ~1) < 2) return false;
+    for (int l = 0; l < s->nLevels; l++) {
+        if (!tn.lv[l].persist) continue;
+        const int n = persist_tiles(s, masked, l, tn.lv[l].pshape);
+        if (n == 0) continue;
+        any = true;
+        *words = std::max(*words, persist_words_of(n, tn.lv[l].pshape));
+    }
+    return any;
+}
+
+// Builds and runs the solve program on the solver's stream and records its end.  A cold
+// solve runs the persistent launches of its tuning when `allow_persist`.
+int run_solve(vfs_solver *s, int masked, int warm_level, int warps, int iters, bool allow_persist)
+{
+    const Params P = s->params();
+    const Tuning &tn = s->tuning[masked ? 1 : 0];
+    size_t words = 0;
+    const bool persist = allow_persist && warm_level < 0 && persist_program(s, masked, tn, &words);
+    int rc;
+    if (persist) {
+        if ((rc = ensure_xbuf(s, words))) return rc;
+        if ((rc = ensure_engine(s))) return rc;
+    }
+    const int start_roles = warm_level < 0 ? 0 : s->roles;
+    const Program pr = build_program(P, masked, warm_level, warps, iters, start_roles, tn, persist);
+    GraphKey key{masked, warm_level, warps, iters, start_roles, P.monitor, P.limitRange, tn.sig()};
+    key.persist = persist ? 1 : 0;
+    rc = engine_run(s->eng, P, pr, key, s->use_graph, &s->err);
+    if (rc) return rc;
+    s->roles = pr.final_roles;
+    s->raw_i1_fresh = false;
+    s->solved = true;
+    s->mon_slots = pr.mon_slots;
+    s->numeric = 0;
+    s->persist_check = persist;
+    s->persist_masked = masked;
+    HIPCHK(s, hipEventRecord(s->done, s->stream));
+    return VFS_OK;
+}
+
+// After a solve with persistent launches: if one gave up (not every workgroup was
+// resident, e.g. beside other kernels on the device), the solve runs again without them,
+// from the same inputs (the calibrated I1 has not replaced the raw one yet).  Waits for the
+// solve when there is something to check.
+int resolve_persist(vfs_solver *s)
+{
+    if (!s->persist_check) return VFS_OK;
+    s->persist_check = false;
+    bool aborted = false;
+    int rc = persist_aborted(s, &aborted);
+    if (rc || !aborted) return rc;
+    s->persist_reruns++;
+    return run_solve(s, s->persist_masked, -1, 0, 0, false);
 }
 
 // warm_level < 0: cold solve; otherwise a warm start at warm_level with warps/iters.
@@ -893,28 +1168,22 @@ int solve_common(vfs_solver *s, int masked, int warm_level = -1, int warps = 0, 
         return fail(s, VFS_ERR_STATE, "warm-started solve before any solve of this pair");
     if (warm_level >= s->nLevels) return fail(s, VFS_ERR_ARG, "warm start level %d >= nLevels", warm_level);
     if ((rc = set_device(s))) return rc;
+    // A warm start reads the previous solve's result: its persistent launches are settled
+    // first.  A cold solve replaces it unread; if one of them gave up, the flag they left
+    // makes this solve's persistent launches give up too, and this solve runs again at the
+    // next wait.
+    if (warm_level >= 0 && (rc = resolve_persist(s))) return rc;
     if ((rc = ensure_engine(s))) return rc;
     HIPCHK(s, refresh_raw_i1(s, s->stream));
-    const Params P = s->params();
     if ((rc = tune_solver(s, masked))) return rc;
-    const Tuning &tn = s->tuning[masked ? 1 : 0];
-    const int start_roles = warm_level < 0 ? 0 : s->roles;
-    const Program pr = build_program(P, masked, warm_level, warps, iters, start_roles, tn);
-    const GraphKey key{masked, warm_level, warps, iters, start_roles, P.monitor, P.limitRange, tn.sig()};
-    rc = engine_run(s->eng, P, pr, key, s->use_graph, &s->err);
-    if (rc) return rc;
-    s->roles = pr.final_roles;
-    s->raw_i1_fresh = false;
-    s->solved = true;
-    s->mon_slots = pr.mon_slots;
-    s->numeric = 0;
-    HIPCHK(s, hipEventRecord(s->done, s->stream));
-    return VFS_OK;
+    return run_solve(s, masked, warm_level, warps, iters, true);
 }
 
 // Waits (on the solver's stream) for the last solve of this pair, wherever it ran.
 int wait_done(vfs_solver *s)
 {
+    int rc = resolve_persist(s);
+    if (rc) return rc;
     HIPCHK(s, hipStreamWaitEvent(s->stream, s->done, 0));
     return VFS_OK;
 }
@@ -1512,6 +1781,12 @@ int vfs_set_option(vfs_solver *s, const char *key, int value)
         s->tuned[0] = s->tuned[1] = false;
         return VFS_OK;
     }
+    if (!strcmp(key, "persist")) {
+        if (value < -1 || value > 1) return fail(s, VFS_ERR_ARG, "persist must be -1, 0 or 1");
+        s->persist_opt = value;
+        s->tuned[0] = s->tuned[1] = false;
+        return VFS_OK;
+    }
     return fail(s, VFS_ERR_ARG, "unknown option '%s'", key);
 }
 
@@ -1524,6 +1799,8 @@ int vfs_get_option(const vfs_solver *s, const char *key, int *value)
     else if (!strcmp(key, "autotune")) *value = s->autotune ? 1 : 0;
     else if (!strcmp(key, "energy")) *value = s->monitor;
     else if (!strcmp(key, "per_launch")) *value = s->force_per_launch;
+    else if (!strcmp(key, "persist")) *value = s->persist_opt;
+    else if (!strcmp(key, "persist_reruns")) *value = s->persist_reruns;
     else if (!strcmp(key, "pair")) *value = s->force_per_launch < 0 ? -1 : s->force_per_launch > 1 ? 1 : 0;
     else return VFS_ERR_ARG;
     return VFS_OK;
@@ -1539,7 +1816,7 @@ int vfs_get_level_tuning(vfs_solver *s, int masked, int level, int *shape, int *
     const LevelTune t = s->tuning[masked ? 1 : 0].lv[level];
     *shape = t.iter_shape();
     *preload = t.preload;
-    if (per_launch) *per_launch = t.per_launch;
+    if (per_launch) *per_launch = t.persist ? VFS_PERSISTENT : t.per_launch;
     return VFS_OK;
 }
 
@@ -1547,6 +1824,20 @@ int vfs_set_level_tuning(vfs_solver *s, int masked, int level, int shape, int pr
 {
     if (!s) return VFS_ERR_ARG;
     if (!s->initialized || level < 0 || level >= s->nLevels) return fail(s, VFS_ERR_ARG, "bad level");
+    if (per_launch == VFS_PERSISTENT) {
+        const int m = masked ? 1 : 0;
+        int rc = set_device(s);
+        if (!rc && !s->tuned[m]) rc = tune_solver(s, m);
+        if (rc) return rc;
+        if (s->persist_opt == 0) return fail(s, VFS_ERR_ARG, "persistent launches are switched off (option persist)");
+        if (persist_tiles(s, m, level, shape) == 0)
+            return fail(s, VFS_ERR_ARG, "no persistent launch in shape %d at level %d (shape, mask column 0, or "
+                                        "more tiles than the device holds at once)", shape, level);
+        LevelTune &t = s->tuning[m].lv[level];
+        t.persist = 1;
+        t.pshape = (unsigned char)shape;
+        return VFS_OK;
+    }
     const bool half = (per_launch & VFS_HALF_WORKGROUP) != 0;
     per_launch &= ~VFS_HALF_WORKGROUP;
     if (!valid_per_launch(per_launch) || (half && (per_launch == 1 || !multi_shape(shape) ||
@@ -1571,6 +1862,7 @@ int vfs_set_level_tuning(vfs_solver *s, int masked, int level, int shape, int pr
         t.preload = (unsigned char)(preload != 0);
     }
     t.per_launch = (unsigned char)per_launch;
+    t.persist = 0;
     return VFS_OK;
 }
 
@@ -1589,6 +1881,13 @@ int vfs_time_solves(vfs_solver *s, int masked, int n, float *ms_total)
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     if (rc) return rc;
+    // persistent launches that gave up did not do their work: the timing does not count
+    // (the last solve runs again, so the solver's result stays right)
+    const int reruns = s->persist_reruns;
+    if ((rc = resolve_persist(s))) return rc;
+    if (s->persist_reruns != reruns)
+        return fail(s, VFS_ERR_DEVICE, "persistent launches gave up during the timed solves (not every workgroup "
+                                       "resident); time with option \"persist\" 0");
     *ms_total = ms;
     return VFS_OK;
 }
@@ -1606,20 +1905,30 @@ int vfs_time_inner_iterations(vfs_solver *s, int level, int reps, float *ms_tota
     const Params P = s->params();
     Program pr;
     int roles = vfs::kRolesInit;
-    const int per_launch = s->tuning[1].lv[level].per_launch;
-    // at least `reps` iterations of the 48-iteration schedule, restarted every 48, in
-    // whole ops
+    int per_launch = s->tuning[1].lv[level].per_launch;
+    // a persistent level: launches of a warp's iterations, as the solve runs them
+    size_t words = 0;
+    const bool persist = persist_program(s, 1, s->tuning[1], &words) && s->tuning[1].lv[level].persist &&
+                         persist_tiles(s, 1, level, s->tuning[1].lv[level].pshape) > 0;
+    const int sched_n = persist ? std::min(s->nSolverIters, vfs::kMaxSched) & ~1 : 48;
+    if (persist) {
+        per_launch = sched_n;
+        if ((rc = ensure_xbuf(s, words))) return rc;
+        if ((rc = ensure_engine(s))) return rc;
+    }
+    // at least `reps` iterations of the sched_n-iteration schedule, restarted every
+    // sched_n, in whole ops
     {
         Params Q = P;
-        Q.nSolverIters = 48;
+        Q.nSolverIters = sched_n;
         Program w;
         int done = 0;
         while (done < reps) {
             w.ops.clear();
-            push_level(w, Q, 1, level, 1, 48, roles, per_launch);
+            push_level(w, Q, 1, level, 1, sched_n, roles, per_launch, false, persist);
             for (const ProgOp &o : w.ops) {
                 if (done >= reps) break;
-                const int k = vfs::op_iterations(o.kind);
+                const int k = o.kind == vfs::OP_ITERP ? vfs::iterp_count(o.it0) : vfs::op_iterations(o.kind);
                 if (k > 0) {
                     pr.ops.push_back(o);
                     done += k;
@@ -1656,6 +1965,12 @@ int vfs_time_inner_iterations(vfs_solver *s, int level, int reps, float *ms_tota
     if (e1) (void)hipEventDestroy(e1);
     (void)hipFree(scratch);
     if (ri != hipSuccess) return fail(s, VFS_ERR_DEVICE, "timing launches: %s", hipGetErrorString(ri));
+    if (persist) {
+        bool aborted = false;
+        if ((rc = persist_aborted(s, &aborted))) return rc;
+        if (aborted)
+            return fail(s, VFS_ERR_DEVICE, "persistent launches gave up while timed (not every workgroup resident)");
+    }
     *ms_total = ms * (float)reps0 / (float)reps;
     if (iters_per_launch) *iters_per_launch = per_launch;
     return VFS_OK;
@@ -1821,6 +2136,9 @@ static int batch_solve(vfs_batch *b, int masked, int warm_level, int warps, int 
         vfs_solver *s = b->members[i];
         std::string why;
         if (ready_to_solve(s, masked, &why)) return failb(b, VFS_ERR_STATE, "member %zu: %s", i, why.c_str());
+        // a member's own last solve is settled before the batch overwrites its state (if
+        // its persistent launches gave up it runs again first, from the same inputs)
+        if (resolve_persist(s)) return failb(b, VFS_ERR_DEVICE, "member %zu: %s", i, s->err.c_str());
         if (s->storage != f->storage || s->limitRange != f->limitRange || s->monitor != f->monitor)
             return failb(b, VFS_ERR_ARG, "member %zu differs in storage, limitRange or the energy option", i);
         if (warm_level >= 0 && (!s->solved || s->roles != f->roles))
@@ -2112,6 +2430,7 @@ int materialize(vfs_server *v, SrvStream &st)
     int rc = vfs_create(&s);
     if (rc) return fails(v, rc, "stream solver: %s", vfs_last_error(nullptr));
     s->storage = v->P.storage;
+    s->persist_opt = 0;  // the server's launches cover many frames: no persistent launches
     rc = vfs_initialize(s, st.width, st.height, v->P.beta, v->P.gamma, v->P.alpha0, v->P.alpha1,
                         v->P.timestep_lambda, v->P.lambda, st.nLevels, v->P.fScale, st.nWarpIters, st.nSolverIters);
     if (!rc) rc = vfs_set_limit_range(s, v->P.limitRange);

@@ -1330,6 +1330,511 @@ void k_iterk(const PairDesc *__restrict__ D, const TickArgs A)
     else op_iterk<SH, K, HALF, false, T>(t, E.roles, sched, ip0, L);
 }
 
+// ------------------------------------------------------------------ a whole warp per launch
+//
+// OP_ITERP runs an even number of inner iterations (normally all of a warp's) in one
+// launch of persistent workgroups, one per tile, all resident at once.  A workgroup stages
+// the tile grown by 2 pixels to the left / top and 1 to the right / bottom (K = 1 of the
+// K-iteration op's geometry) once, and keeps it on chip: each work-item holds what only
+// its positions read in registers, the values neighbours read in LDS.  Iteration j
+// computes the duals over the tile grown by 1 to the left / top, then thresholding and
+// the primal step over the tile.  The duals of the one-pixel ring are computed here from
+// the ring's u_ and v_, p and q as the ring's own tile computes them (the same code, so
+// the same values), so after each iteration the workgroups exchange only u_ and v_: each
+// publishes those of its two outermost columns and rows to the exchange words, and
+// before the next iteration each refreshes its ring from its neighbours' words.  A word
+// is (value bits | tag << 32) with tag = (launch counter << 8) | iteration, written and
+// read by agent-scope atomics, so a reader waits for the tag it needs without fences.
+// Words are double-buffered by the parity of the iteration: a tile writes iteration j + 2's
+// words only after its neighbours have read iteration j's (it needed their j + 1 words).
+// Buffers, stores and out-of-mask values follow the K-iteration op with K = the op's
+// iteration count: iteration n - 2 stores u, u_, v_ to the other buffers, iteration n - 1
+// the whole state to the spare ones, and the host rotates the roles.  Every wait is
+// bounded: a workgroup that finds not every workgroup resident, or waits too long for a
+// neighbour, sets the pair's abort flag and leaves; the others see it and leave too, and
+// the host runs the solve again without this op.
+template <int SH>
+struct BlkP {
+    static constexpr int TW = kShape[SH].tw, TH = kShape[SH].th;
+    static constexpr int RO = -2, RW = TW + 3, RH = TH + 3, NR = RW * RH;  // staged region
+    static constexpr int NT = NR >= 1024 ? 1024 : (NR + 63) / 64 * 64;
+    static constexpr int NK = (NR + NT - 1) / NT;  // staged positions per work-item
+    static_assert(persist_shape(SH) && NK <= 3 && TW >= 3 && TH >= 3, "persistent tile shape");
+};
+
+template <int SH>
+struct LdsP {
+    static constexpr int NR = BlkP<SH>::NR;
+    float ub[NR];
+    float2 v[NR];
+    float4 q[NR];
+    float2 tp[NR];
+    unsigned char mk[NR];
+};
+
+// Exchange words of one tile and parity: u_, v_.x, v_.y (value-major) of column tw - 1
+// (A), column tw - 2 (B), row th - 1 (C), row th - 2 (D), column 0 (E) and row 0 (F).
+// Every position of a tile's ring lies in one of its neighbours' strips.
+template <int SH>
+struct StripP {
+    static constexpr int TW = kShape[SH].tw, TH = kShape[SH].th;
+    static constexpr int A = 0, B = 3 * TH, C = 6 * TH, D = C + 3 * TW, E = D + 3 * TW, F = E + 3 * TH;
+    static constexpr int WORDS = F + 3 * TW;
+    static_assert(WORDS == persist_words(SH), "strip layout");
+};
+
+// Bounded waits of the persistent op, in ticks of the 100 MHz real-time counter.
+constexpr unsigned long long kArriveTicks = 500000ull;  // every workgroup resident: 5 ms
+constexpr unsigned long long kPollTicks = 5000000ull;   // a neighbour's words: 50 ms
+
+__device__ __forceinline__ unsigned long long xload(const unsigned long long *p)
+{
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void xstore(unsigned long long *p, float v, unsigned tag)
+{
+    __hip_atomic_store(p, (unsigned long long)__float_as_uint(v) | ((unsigned long long)tag << 32), __ATOMIC_RELAXED,
+                       __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ unsigned xload32(const unsigned *p)
+{
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void xstore32(unsigned *p, unsigned v)
+{
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ unsigned long long rtime() { return __builtin_amdgcn_s_memrealtime(); }
+
+template <int SH, bool MASKED, typename T>
+__device__ __forceinline__ void op_iterp(const Tile &t, int roles, const float4 *__restrict__ sched,
+                                         const IterParams &ip0, int n_iter, int tile, int ntiles,
+                                         XSync *__restrict__ xs, unsigned long long *__restrict__ xbuf, LdsP<SH> &L,
+                                         unsigned *s_word)
+{
+    using B = BlkP<SH>;
+    using X = StripP<SH>;
+    constexpr int NT = B::NT, NK = B::NK, TW = B::TW, TH = B::TH;
+    const int w = t.w, h = t.h, s = t.s, x0 = t.x0, y0 = t.y0;
+    const int tid = threadIdx.x;
+
+    // ---- every workgroup of the launch must be resident: count the arrivals
+    if (tid == 0) {
+        const unsigned ep = xload32(&xs->epoch);
+        unsigned ab = xload32(&xs->abort);
+        __hip_atomic_fetch_add(&xs->arrive, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (!ab) {
+            const unsigned long long t0 = rtime();
+            while (xload32(&xs->arrive) < (unsigned)ntiles) {
+                if (rtime() - t0 > kArriveTicks) {
+                    xstore32(&xs->abort, 1u);
+                    ab = 1;
+                    break;
+                }
+                __builtin_amdgcn_s_sleep(2);
+            }
+        }
+        s_word[0] = ep;
+        s_word[1] = ab;
+    }
+    __syncthreads();
+    const unsigned epoch = s_word[0];
+
+    if (s_word[1] == 0) {
+        const float *__restrict__ mask = t.mask;
+        const float *__restrict__ u_cur = slot(t, slot_u(rcur(roles, G_U)));
+        const float *__restrict__ ub_cur = slot(t, slot_ub(rcur(roles, G_UB)));
+        const float2 *__restrict__ v_cur = slotp<float2>(t, slot_v(rcur(roles, G_V)));
+        const V2<T> *__restrict__ p_cur = slotp<V2<T>>(t, slot_p(rcur(roles, G_PQ)));
+        const V4<T> *__restrict__ q_cur = slotp<V4<T>>(t, slot_q(rcur(roles, G_PQ)));
+        const float *__restrict__ ub_oth = slot(t, slot_ub(roth(roles, G_UB)));
+        const float2 *__restrict__ v_oth = slotp<float2>(t, slot_v(roth(roles, G_V)));
+        const float2 *__restrict__ Tp = slotp<float2>(t, PL_TP);
+        const T *__restrict__ at = (const T *)slot(t, PL_A);
+        const T *__restrict__ bt = (const T *)slot(t, PL_B);
+        const T *__restrict__ ct = (const T *)slot(t, PL_C);
+        const T *__restrict__ Iu = (const T *)slot(t, PL_IU);
+        const T *__restrict__ Iz = (const T *)slot(t, PL_IZ);
+
+        // ---- stage (as the K-iteration op): own values into registers, shared ones into
+        // LDS.  Outside the mask, q is never read, and its LDS slot holds the other buffers'
+        // u_ and v_ instead (the values read there at odd iterations).
+        OwnK o[NK];
+        int pxy[NK];  // (x + 16) | (y + 16) << 16
+        float omk[NK];
+        float2 otp[NK];
+        float4 oq[NK];
+#pragma unroll
+        for (int k = 0; k < NK; k++) {
+            const int i = tid + k * NT;
+            const int x = x0 + B::RO + i % B::RW, y = y0 + B::RO + i / B::RW;
+            pxy[k] = (x + 16) | ((y + 16) << 16);
+            if (i < B::NR) {
+                const int xc = min(max(x, 0), w - 1), yc = min(max(y, 0), h - 1);
+                const long pos = (long)yc * s + xc;
+                omk[k] = MASKED ? mask[pos] : 1.f;
+                o[k].ub = ub_cur[pos];
+                o[k].v = v_cur[pos];
+                o[k].u = u_cur[pos];
+                o[k].a = ldT(at, pos);
+                o[k].b = ldT(bt, pos);
+                o[k].c = ldT(ct, pos);
+                o[k].iu = ldT(Iu, pos);
+                o[k].iz = ldT(Iz, pos);
+                if (ip0.first) {
+                    o[k].p = make_float2(0.f, 0.f);
+                    oq[k] = make_float4(0.f, 0.f, 0.f, 0.f);
+                } else {
+                    o[k].p = ld2T(p_cur, pos);
+                    oq[k] = ld4T(q_cur, pos);
+                }
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < NK; k++) {
+            const int i = tid + k * NT;
+            otp[k] = make_float2(0.f, 0.f);
+            if (MASKED && i < B::NR && omk[k] == 0.f) {
+                const int x = (pxy[k] & 0xffff) - 16, y = (pxy[k] >> 16) - 16;
+                const int xc = min(max(x, 0), w - 1), yc = min(max(y, 0), h - 1);
+                const long pos = (long)yc * s + xc;
+                const float2 vo = v_oth[pos];
+                oq[k] = make_float4(ub_oth[pos], vo.x, vo.y, 0.f);
+                otp[k] = Tp[pos];
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < NK; k++) {
+            const int i = tid + k * NT;
+            if (i < B::NR) {
+                L.ub[i] = o[k].ub;
+                L.v[i] = o[k].v;
+                L.q[i] = oq[k];
+                L.tp[i] = otp[k];
+                L.mk[i] = omk[k] != 0.f ? 1 : 0;
+            }
+        }
+        // row padding of the last tile column: the spare buffers take the current values
+        if (x0 + TW >= w && s > w) {
+            float *__restrict__ u_spr = slot(t, slot_u(rspare(roles, G_U)));
+            float *__restrict__ ub_spr = slot(t, slot_ub(rspare(roles, G_UB)));
+            float2 *__restrict__ v_spr = slotp<float2>(t, slot_v(rspare(roles, G_V)));
+            const int pw = s - w, rows = min(TH, h - y0);
+            for (int i = tid; i < pw * rows; i += NT) {
+                const long pos = (long)(y0 + i / pw) * s + w + i % pw;
+                u_spr[pos] = u_cur[pos];
+                ub_spr[pos] = ub_cur[pos];
+                v_spr[pos] = v_cur[pos];
+            }
+        }
+        bool inner = true;
+#pragma unroll
+        for (int k = 0; k < NK; k++) {
+            const int i = tid + k * NT;
+            const int x = (pxy[k] & 0xffff) - 16, y = (pxy[k] >> 16) - 16;
+            if (i < B::NR)
+                inner = inner && x >= 0 && y >= 0 && x < w && y < h && omk[k] != 0.f &&
+                        (MASKED || !outside_circle(x, y, w, h));
+        }
+        const bool full = __syncthreads_and(inner) != 0;
+        PrimalCoef kc[NK];
+#pragma unroll
+        for (int k = 0; k < NK; k++) kc[k] = primal_coef(ip0, o[k].a, o[k].b, o[k].c, o[k].iu);
+
+        auto act = [&](int k) { return !MASKED || L.mk[k] != 0; };
+        // Per own position, once per launch: the neighbour flags of the path with boundary
+        // tests (as the K-iteration op) and the regions (duals: the tile grown by 1 to the
+        // left / top; primal steps: the tile).
+        enum : unsigned { F_IN = 1, F_ACT = 2, F_R = 4, F_D = 8, F_L = 16, F_U = 32, F_OUT = 64, F_DUAL = 128, F_TILE = 256 };
+        unsigned fl[NK];
+#pragma unroll
+        for (int k = 0; k < NK; k++) {
+            const int i = tid + k * NT;
+            const int x = (pxy[k] & 0xffff) - 16, y = (pxy[k] >> 16) - 16;
+            const int dx = x - x0, dy = y - y0;
+            fl[k] = 0;
+            if (i >= B::NR) continue;
+            if (dx >= -1 && dy >= -1 && dx <= TW - 1 && dy <= TH - 1) fl[k] |= F_DUAL;
+            if (dx >= 0 && dy >= 0 && dx <= TW - 1 && dy <= TH - 1) fl[k] |= F_TILE;
+            if (!full && x >= 0 && y >= 0 && x < w && y < h) {
+                const int col = i % B::RW, row = i / B::RW;
+                fl[k] |= F_IN | (act(i) ? F_ACT : 0u);
+                if (x + 1 < w && col + 1 < B::RW && act(i + 1)) fl[k] |= F_R;
+                if (y + 1 < h && row + 1 < B::RH && act(i + B::RW)) fl[k] |= F_D;
+                if (x >= 1 && col >= 1 && act(i - 1)) fl[k] |= F_L;
+                if (y >= 1 && row >= 1 && act(i - B::RW)) fl[k] |= F_U;
+                if (!MASKED && outside_circle(x, y, w, h)) fl[k] |= F_OUT;
+            }
+        }
+        // Exchange slots, once per launch: work-item `tid` publishes strip position tid of
+        // the tile (the strips in order A..F: 3 (tw + th) positions) and refreshes ring
+        // position tid (those some dual reads, 3 (tw + th) + 5: columns -2, -1 and tw, rows
+        // -2, -1 and th of the staged region), where that position lies in the image and is
+        // processed.  pub_s: LDS index | strip word << 12; rcv_i: LDS index; rcv_w: the
+        // source tile's strip word (parity 0) | 1 << 30 for a column strip (value stride
+        // th; rows: tw).
+        static_assert(B::NR <= 4096 && X::WORDS < (1 << 20), "slot packing");
+        const int ntx = (w + TW - 1) / TW;
+        int pub_s = -1, rcv_i = -1, rcv_w = 0;
+        if (tid < 3 * (TW + TH)) {
+            int lx, ly, off;
+            if (tid < TH) lx = TW - 1, ly = tid, off = X::A + tid;
+            else if (tid < 2 * TH) lx = TW - 2, ly = tid - TH, off = X::B + ly;
+            else if (tid < 2 * TH + TW) lx = tid - 2 * TH, ly = TH - 1, off = X::C + lx;
+            else if (tid < 2 * TH + 2 * TW) lx = tid - 2 * TH - TW, ly = TH - 2, off = X::D + lx;
+            else if (tid < 3 * TH + 2 * TW) lx = 0, ly = tid - 2 * TH - 2 * TW, off = X::E + ly;
+            else lx = tid - 3 * TH - 2 * TW, ly = 0, off = X::F + lx;
+            const int i = (ly - B::RO) * B::RW + (lx - B::RO);
+            if (x0 + lx < w && y0 + ly < h && act(i)) pub_s = i | (off << 12);
+        }
+        if (tid < 3 * (TW + TH) + 5) {
+            int dx, dy;
+            if (tid < TH + 1) dx = -2, dy = tid - 1;
+            else if (tid < 2 * TH + 4) dx = -1, dy = tid - (TH + 1) - 2;
+            else if (tid < 3 * TH + 5) dx = TW, dy = tid - (2 * TH + 4) - 1;
+            else if (tid < 3 * TH + 5 + TW) dx = tid - (3 * TH + 5), dy = -2;
+            else if (tid < 3 * TH + 5 + 2 * TW) dx = tid - (3 * TH + 5 + TW), dy = -1;
+            else dx = tid - (3 * TH + 5 + 2 * TW), dy = TH;
+            const int x = x0 + dx, y = y0 + dy, i = (dy - B::RO) * B::RW + (dx - B::RO);
+            if (x >= 0 && y >= 0 && x < w && y < h && act(i)) {
+                // its tile's strip holding it
+                const int sx = x / TW, sy = y / TH, lx = x - sx * TW, ly = y - sy * TH;
+                int off, col;
+                if (lx == TW - 1) off = X::A + ly, col = 1;
+                else if (lx == TW - 2) off = X::B + ly, col = 1;
+                else if (ly == TH - 1) off = X::C + lx, col = 0;
+                else if (ly == TH - 2) off = X::D + lx, col = 0;
+                else if (lx == 0) off = X::E + ly, col = 1;
+                else off = X::F + lx, col = 0;  // ly == 0
+                rcv_i = i;
+                rcv_w = ((sy * ntx + sx) * 2 * X::WORDS + off) | (col << 30);
+            }
+        }
+        unsigned long long *__restrict__ mine = xbuf + (long)tile * 2 * X::WORDS;
+        const unsigned tag0 = epoch << 8;
+
+        // One iteration.  MODE 0: an iteration before the op's last two; 1: iteration
+        // n - 2 (u, u_, v_ to the other buffers); 2: the last one (the whole state to the
+        // spare buffers).  Returns false when a wait gave up.
+        auto step = [&](int j, auto full_c, auto mode_c) -> bool {
+            constexpr bool FULL = decltype(full_c)::value;
+            constexpr int MODE = decltype(mode_c)::value;
+            int bad = 0;
+            if (j > 0) {
+                // ---- publish the tile's strips after iteration j - 1 (their values are in
+                // LDS since the barrier that ended it), then refresh the ring from the
+                // neighbours' strips
+                if (pub_s >= 0) {
+                    const int i = pub_s & 4095, off = pub_s >> 12;
+                    const int st = (off < X::C || (off >= X::E && off < X::F)) ? TH : TW;
+                    const unsigned tg = tag0 | (unsigned)(j - 1);
+                    unsigned long long *pw = mine + ((j - 1) & 1) * X::WORDS + off;
+                    const float2 vv = L.v[i];
+                    xstore(pw, L.ub[i], tg);
+                    xstore(pw + st, vv.x, tg);
+                    xstore(pw + 2 * st, vv.y, tg);
+                }
+                if (rcv_i >= 0) {
+                    const unsigned tg = tag0 | (unsigned)(j - 1);
+                    const unsigned long long *pw = xbuf + (long)(rcv_w & 0x3fffffff) + ((j - 1) & 1) * X::WORDS;
+                    const int st = (rcv_w >> 30) ? TH : TW;
+                    unsigned long long a0, a1, a2, t0 = 0;
+                    for (int polls = 0;; polls++) {
+                        a0 = xload(pw);
+                        a1 = xload(pw + st);
+                        a2 = xload(pw + 2 * st);
+                        if ((unsigned)(a0 >> 32) == tg && (unsigned)(a1 >> 32) == tg && (unsigned)(a2 >> 32) == tg)
+                            break;
+                        if (polls == 0) t0 = rtime();
+                        if ((polls & 63) == 63 && (rtime() - t0 > kPollTicks || xload32(&xs->abort) != 0)) {
+                            bad = 1;
+                            break;
+                        }
+                    }
+                    L.ub[rcv_i] = __uint_as_float((unsigned)a0);
+                    L.v[rcv_i] = make_float2(__uint_as_float((unsigned)a1), __uint_as_float((unsigned)a2));
+                }
+            }
+            if (__syncthreads_or(bad)) {
+                if (tid == 0) xstore32(&xs->abort, 1u);
+                return false;
+            }
+            IterParams ip = ip0;
+            const float4 sc = sched[j];
+            ip.k1 = sc.w;
+            ip.k0 = ip0.alpha0 * sc.x / 2.0f;
+            ip.tau = sc.y;
+            ip.mu = sc.z;
+            ip.first = j == 0 && ip0.first;
+            ip.write_tp = MODE == 2 && ip0.write_tp;
+
+            // ---- duals over the tile grown by 1 to the left / top (u_ and v_ of every
+            // position from LDS, where the ring's are refreshed)
+#pragma unroll
+            for (int k = 0; k < NK; k++) {
+                const int i = tid + k * NT;
+                if (!(fl[k] & F_DUAL)) continue;
+                if (!FULL && !(fl[k] & F_ACT && fl[k] & F_IN)) continue;
+                const int x = (pxy[k] & 0xffff) - 16, y = (pxy[k] >> 16) - 16;
+                const int kR = (FULL || x + 1 < w) ? i + 1 : i, kD = (FULL || y + 1 < h) ? i + B::RW : i;
+                DualIn rr;
+                rr.ok = true;
+                rr.act = true;
+                rr.hasR = FULL || (fl[k] & F_R) != 0;
+                rr.hasD = FULL || (fl[k] & F_D) != 0;
+                rr.a = o[k].a;
+                rr.b = o[k].b;
+                rr.c = o[k].c;
+                rr.u0 = L.ub[i];
+                rr.uR = L.ub[kR];
+                rr.uD = L.ub[kD];
+                rr.uL = (!FULL && x >= 1) ? L.ub[i - 1] : 0.f;
+                rr.uU = (!FULL && y >= 1) ? L.ub[i - B::RW] : 0.f;
+                rr.v0 = L.v[i];
+                rr.vR = L.v[kR];
+                rr.vD = L.v[kD];
+                rr.vL = (!FULL && x >= 1) ? L.v[i - 1] : make_float2(0.f, 0.f);
+                rr.vU = (!FULL && y >= 1) ? L.v[i - B::RW] : make_float2(0.f, 0.f);
+                rr.po = make_float2(rT<T>(o[k].p.x), rT<T>(o[k].p.y));
+                const float4 qo = L.q[i];
+                rr.qo = make_float4(rT<T>(qo.x), rT<T>(qo.y), rT<T>(qo.z), rT<T>(qo.w));
+                float2 pn, tpn;
+                float4 qn;
+                dual_step<MASKED>(rr, !FULL && (fl[k] & F_OUT) != 0, ip, pn, qn, tpn);
+                o[k].p = pn;
+                L.q[i] = qn;
+                L.tp[i] = tpn;
+                if (MODE == 2 && (fl[k] & F_TILE)) {
+                    const long pos = (long)y * s + x;
+                    st2T(slotp<V2<T>>(t, slot_p(rspare(roles, G_PQ))), pos, pn);
+                    st4T(slotp<V4<T>>(t, slot_q(rspare(roles, G_PQ))), pos, qn);
+                    if (ip.write_tp) slotp<float2>(t, PL_TP)[pos] = tpn;
+                }
+            }
+            __syncthreads();
+
+            // ---- thresholding + primal step over the tile; outside the mask, the stored
+            // u_ and v_ switch to the buffer the next iteration reads there
+#pragma unroll
+            for (int k = 0; k < NK; k++) {
+                const int i = tid + k * NT;
+                if (FULL && !(fl[k] & F_TILE)) continue;
+                if (!FULL && !(fl[k] & F_IN)) continue;
+                const int x = (pxy[k] & 0xffff) - 16, y = (pxy[k] >> 16) - 16;
+                const long pos = (long)y * s + x;
+                if (!FULL && MASKED && !(fl[k] & F_ACT)) {
+                    if (MODE == 2) {
+                        if (fl[k] & F_TILE) {
+                            slot(t, slot_u(rspare(roles, G_U)))[pos] = o[k].u;
+                            slot(t, slot_ub(rspare(roles, G_UB)))[pos] = o[k].ub;
+                            slotp<float2>(t, slot_v(rspare(roles, G_V)))[pos] = o[k].v;
+                        }
+                    } else if (j & 1) {
+                        L.ub[i] = o[k].ub;
+                        L.v[i] = o[k].v;
+                    } else {
+                        const float4 oth = L.q[i];
+                        L.ub[i] = oth.x;
+                        L.v[i] = make_float2(oth.y, oth.z);
+                    }
+                    continue;
+                }
+                if (!(fl[k] & F_TILE)) continue;
+                float un, u_s;
+                float2 vs;
+                if (!FULL && !MASKED && (fl[k] & F_OUT)) {
+                    un = 0.f;
+                    u_s = 0.f;
+                    vs = make_float2(0.f, 0.f);
+                } else {
+                    const bool hR = FULL || (fl[k] & F_R) != 0, hD = FULL || (fl[k] & F_D) != 0;
+                    const bool hL = FULL || (fl[k] & F_L) != 0, hU = FULL || (fl[k] & F_U) != 0;
+                    const float tpLx = (FULL || x >= 1) ? L.tp[i - 1].x : 0.f;
+                    const float tpUy = (FULL || y >= 1) ? L.tp[i - B::RW].y : 0.f;
+                    const float2 tp0 = L.tp[i];
+                    const float divp = div_tp<MASKED>(x, y, w, h, hR, hD, hL, hU, tp0, tpLx, tpUy);
+                    const float4 ql = hL ? L.q[i - 1] : make_float4(0.f, 0.f, 0.f, 0.f);
+                    const float4 qu = hU ? L.q[i - B::RW] : make_float4(0.f, 0.f, 0.f, 0.f);
+                    primal_core(ip, kc[k], L.ub[i], o[k].u, o[k].iu, o[k].iz, L.v[i], hR, hD, hL, hU, divp, L.q[i], ql,
+                                qu, tp0, un, u_s, vs);
+                }
+                o[k].u = un;
+                L.ub[i] = u_s;
+                L.v[i] = vs;
+                if (MODE == 1) {
+                    slot(t, slot_u(roth(roles, G_U)))[pos] = un;
+                    slot(t, slot_ub(roth(roles, G_UB)))[pos] = u_s;
+                    slotp<float2>(t, slot_v(roth(roles, G_V)))[pos] = vs;
+                } else if (MODE == 2) {
+                    slot(t, slot_u(rspare(roles, G_U)))[pos] = un;
+                    slot(t, slot_ub(rspare(roles, G_UB)))[pos] = u_s;
+                    slotp<float2>(t, slot_v(rspare(roles, G_V)))[pos] = vs;
+                }
+            }
+            // the strips of this iteration are read from LDS after this barrier
+            if (MODE != 2) __syncthreads();
+            return true;
+        };
+        auto run = [&](auto full_c) {
+            using I0 = std::integral_constant<int, 0>;
+            using I1 = std::integral_constant<int, 1>;
+            using I2 = std::integral_constant<int, 2>;
+            for (int j = 0; j < n_iter - 2; j++)
+                if (!step(j, full_c, I0{})) return;
+            if (step(n_iter - 2, full_c, I1{})) step(n_iter - 1, full_c, I2{});
+        };
+        if (full) run(std::true_type{});
+        else run(std::false_type{});
+    }
+
+    // ---- the last workgroup to finish resets the arrivals and advances the launch counter
+    __syncthreads();
+    if (tid == 0) {
+        const unsigned d = __hip_atomic_fetch_add(&xs->done, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (d == (unsigned)ntiles - 1) {
+            xstore32(&xs->arrive, 0u);
+            xstore32(&xs->done, 0u);
+            xstore32(&xs->epoch, epoch + 1u);
+        }
+    }
+}
+
+template <typename T, int SH>
+__global__ __launch_bounds__(BlkP<SH>::NT) void k_iterp(const PairDesc *__restrict__ D, const TickArgs A)
+{
+    using B = BlkP<SH>;
+    __shared__ LdsP<SH> L;
+    __shared__ unsigned s_word[2];
+    // tiles in XCD-contiguous runs (bands of rows), as the other op kernels
+    const int nb = gridDim.x, i = blockIdx.x, xcd = i & 7, q = nb >> 3, r = nb & 7;
+    const int b = (xcd < r ? xcd * (q + 1) : r * (q + 1) + (xcd - r) * q) + (i >> 3);
+    const TickEntry &E = A.e[0];
+    Tile t;
+    t.d = D + E.pair;
+    t.arena = E.arena;
+    t.plane = E.plane;
+    t.mask = E.mask;
+    t.w = E.w;
+    t.h = E.h;
+    t.s = E.s;
+    const int ntx = (t.w + B::TW - 1) / B::TW;
+    t.x0 = (b % ntx) * B::TW;
+    t.y0 = (b / ntx) * B::TH;
+    const float4 *sched = t.d->sched + iterp_first(E.it0);
+    IterParams ip0;
+    ip0.k1 = ip0.k0 = ip0.tau = ip0.mu = 0.f;  // set per iteration
+    ip0.lambda = A.lambda;
+    ip0.alpha0 = A.alpha0;
+    ip0.alpha1 = A.alpha1;
+    ip0.first = (E.flags & EF_FIRST) != 0;
+    ip0.write_tp = (E.flags & EF_WRITE_TP_LAST) != 0;
+    ip0.preload_tp = 0;
+    XSync *xs = t.d->xsync;
+    unsigned long long *xbuf = t.d->xbuf;
+    if (E.masked) op_iterp<SH, true, T>(t, E.roles, sched, ip0, iterp_count(E.it0), b, nb, xs, xbuf, L, s_word);
+    else op_iterp<SH, false, T>(t, E.roles, sched, ip0, iterp_count(E.it0), b, nb, xs, xbuf, L, s_word);
+}
+
 // 13th smallest of 25 (the value medianFilter.cu:307-316's partial selection sort leaves
 // at position 12) by a comparator network: Batcher's odd-even merge sort of 32 inputs,
 // the 7 beyond the window taken as +infinity, pruned to the 113 comparators that
@@ -1808,6 +2313,53 @@ void iterk_storage(const PairDesc *desc, const TickArgs &args, int launch_shape,
     case 16: iterk_shape<T, 16, K>(desc, args, half, st); break;
     case 17: iterk_shape<T, 17, K>(desc, args, half, st); break;
     default: break;  // the host never asks for another shape
+    }
+}
+
+template <typename T, int SH>
+void iterp_shape(const PairDesc *desc, const TickArgs &args, hipStream_t st)
+{
+    if constexpr (persist_shape(SH))
+        k_iterp<T, SH><<<dim3((unsigned)args.total), dim3(BlkP<SH>::NT), 0, st>>>(desc, args);
+}
+
+template <typename T, int SH>
+int iterp_shape_capacity()
+{
+    if constexpr (persist_shape(SH)) {
+        int per = 0, dev = 0, cus = 0;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_iterp<T, SH>, BlkP<SH>::NT, 0) != hipSuccess) return 0;
+        if (hipGetDevice(&dev) != hipSuccess) return 0;
+        if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) return 0;
+        return per * cus;
+    }
+    return 0;
+}
+
+// The persistent op's shapes (persist_shape): 8x8, 4x4, 36x25, 27x25, 14x13, 7x7, 53x25, 53x50.
+#define VFS_PERSIST_SHAPES(X) X(7) X(8) X(11) X(12) X(13) X(14) X(17) X(18)
+
+template <typename T>
+void iterp_storage(const PairDesc *desc, const TickArgs &args, int shape, hipStream_t st)
+{
+    switch (shape) {
+#define VFS_CASE(sh) \
+    case sh: iterp_shape<T, sh>(desc, args, st); break;
+        VFS_PERSIST_SHAPES(VFS_CASE)
+#undef VFS_CASE
+    default: break;  // the host never asks for another shape
+    }
+}
+
+template <typename T>
+int iterp_capacity(int shape)
+{
+    switch (shape) {
+#define VFS_CASE(sh) \
+    case sh: return iterp_shape_capacity<T, sh>();
+        VFS_PERSIST_SHAPES(VFS_CASE)
+#undef VFS_CASE
+    default: return 0;
     }
 }
 

@@ -1,12 +1,13 @@
 // One instantiation unit of the op kernels, compiled once per storage type and kernel
 // family (the library build passes VFS_INST_STORAGE = 0 float, 1 __half, 2
 // __hip_bfloat16, and VFS_INST_K = 1 for the tick kernel, 2, 4 or 8 for the K-iteration
-// kernel), so that the kernels compile in parallel.  Defines launch_ops_<storage>_k<K>,
-// which kernels.hip's launch_tick dispatches to.
+// kernel, 0 for the whole-warp persistent kernel), so that the kernels compile in parallel.
+// Defines launch_ops_<storage>_k<K>, which kernels.hip's launch_tick dispatches to (and
+// for K = 0 persist_capacity_<storage>).
 #include "tgv_device.h"
 
 #if !defined(VFS_INST_STORAGE) || !defined(VFS_INST_K)
-#error "build with -DVFS_INST_STORAGE=<0|1|2> -DVFS_INST_K=<1|2|4|8>"
+#error "build with -DVFS_INST_STORAGE=<0|1|2> -DVFS_INST_K=<0|1|2|4|8>"
 #endif
 
 #if VFS_INST_STORAGE == 0
@@ -30,9 +31,17 @@ void VFS_NAME_X(VFS_INST_SFX, VFS_INST_K)(const PairDesc *desc, const TickArgs &
 {
 #if VFS_INST_K == 1
     tick_storage<VFS_INST_T>(desc, args, shape, st);
+#elif VFS_INST_K == 0
+    iterp_storage<VFS_INST_T>(desc, args, shape, st);
 #else
     iterk_storage<VFS_INST_T, VFS_INST_K>(desc, args, shape, st);
 #endif
 }
+
+#if VFS_INST_K == 0
+#define VFS_CAP_NAME(s) VFS_CAT3(persist_capacity_, s, )
+#define VFS_CAP_NAME_X(s) VFS_CAP_NAME(s)
+int VFS_CAP_NAME_X(VFS_INST_SFX)(int shape) { return iterp_capacity<VFS_INST_T>(shape); }
+#endif
 
 }  // namespace vfs

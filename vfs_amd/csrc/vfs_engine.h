@@ -29,15 +29,17 @@ constexpr int kMaxSched = 128;
 // image into 512, 500, 480 and 768 tiles, two or three workgroups for each of 256 compute
 // units, 53x25 with the smallest staged region of the two-round ones; 27x25, 14x13 and
 // 7x7 split 424x400, 212x200 and 106x100 into 256, 256 and 240: one each).
-constexpr int kShapes = 18, kTickShapes = 7;
+// Shape 18 (53x50: 848x800 in 16 x 16 tiles) serves the whole-warp persistent op only.
+constexpr int kShapes = 19, kTickShapes = 7;
 struct ShapeDim {
     int tw, th;
 };
 constexpr ShapeDim kShape[kShapes] = {{64, 8}, {64, 4}, {128, 4}, {32, 16}, {32, 8}, {16, 32}, {16, 16},
-                                      {8, 8}, {4, 4}, {32, 32}, {20, 20}, {36, 25}, {27, 25}, {14, 13}, {7, 7}, {43, 34}, {34, 40}, {53, 25}};
+                                      {8, 8}, {4, 4}, {32, 32}, {20, 20}, {36, 25}, {27, 25}, {14, 13}, {7, 7}, {43, 34}, {34, 40}, {53, 25},
+                                      {53, 50}};
 __host__ __device__ constexpr bool multi_iter_shape(int shape)
 {
-    return shape == 1 || shape == 3 || (shape >= 4 && shape != 5);
+    return shape == 1 || shape == 3 || (shape >= 4 && shape != 5 && shape < 18);
 }
 // Positions a K-iteration workgroup stages: the tile grown by 2K pixels to the left / top
 // and K to the right / bottom.
@@ -62,6 +64,32 @@ __host__ __device__ constexpr bool iterk_ok(int shape, int K)
 {
     return multi_iter_shape(shape) && (K <= 4 || iterk_region(shape, K) <= 3072);
 }
+
+// Whole-warp persistent op (OP_ITERP): every workgroup of the launch owns one tile for
+// all of the op's iterations and keeps the tile's state on chip; after each iteration it
+// publishes u_ and v_ of its two outermost rows and columns, from which its neighbours
+// refresh the one- and two-pixel rings they stage around their own tiles (the tile grown
+// by 2 pixels to the left / top and 1 to the right / bottom).  All workgroups must be
+// resident at once.  Shapes: staged region of at most 3072 positions (three per
+// work-item), tiles at least 3 pixels on each side.
+__host__ __device__ constexpr int iterp_region(int shape) { return (kShape[shape].tw + 3) * (kShape[shape].th + 3); }
+__host__ __device__ constexpr bool persist_shape(int shape)
+{
+    return (shape == 7 || shape == 8 || (shape >= 11 && shape <= 14) || shape == 17 || shape == 18) &&
+           iterp_region(shape) <= 3072;
+}
+// Exchange words of one tile and parity: u_, v_.x, v_.y of columns tw-1, tw-2, rows
+// th-1, th-2, column 0 and row 0 (StripP in tgv_device.h).
+__host__ __device__ constexpr int persist_words(int shape) { return 9 * (kShape[shape].tw + kShape[shape].th); }
+// Tiles of one persistent launch at most (bounds the exchange buffer).
+constexpr int kMaxPersistTiles = 1024;
+// Launch bookkeeping of the persistent op, one per pair (device memory): the launch
+// counter that makes exchange tags unique, the workgroups that have arrived and
+// finished, and a sticky abort flag set when a workgroup waited too long (not every
+// workgroup resident); the host clears it after re-running the solve without the op.
+struct XSync {
+    unsigned int epoch, arrive, done, abort;
+};
 
 // Storage precision of the image pyramid, warped image, derivatives, tensor and duals.
 // Arithmetic is FP32 in every mode.
@@ -160,6 +188,10 @@ struct PairDesc {
     float4 sched[kMaxSched];
     EnergyPart *mon_rec;   // kMaxLevels x kMonWarps records (mon_slot)
     EnergyPart *mon_part;  // energy_chunks(w0, h0) partial sums of the running OP_ENERGY
+    // OP_ITERP: exchange words ([tile][parity][persist_words], value | tag << 32) and the
+    // launch bookkeeping
+    unsigned long long *xbuf;
+    XSync *xsync;
 };
 
 // Scalars of one inner iteration (schedule of stereotgv.cu:449-477).
@@ -196,14 +228,18 @@ enum OpKind : int {
     OP_ITER8 = 14,  // eight fused TGV iterations (recomputed halo)
     OP_SYNC = 15,   // spare buffers of u, u_, v_ take the current ones beyond the level's extent
     OP_ENERGY = 16, // energy / residual partial sums of the level (kEnergyChunk pixels per workgroup)
-    OP_EFINAL = 17  // one workgroup per pair: sums the partials in order into the (level, warp) record
+    OP_EFINAL = 17, // one workgroup per pair: sums the partials in order into the (level, warp) record
+    OP_ITERP = 18   // an even number of fused TGV iterations (a whole warp), persistent workgroups
 };
 
-// Iterations of an inner-iteration op.
+// Iterations of an inner-iteration op (OP_ITERP: see iterp_count).
 inline int op_iterations(int kind)
 {
     return kind == OP_ITER8 ? 8 : kind == OP_ITER4 ? 4 : kind == OP_ITER2 ? 2 : kind == OP_ITER ? 1 : 0;
 }
+// OP_ITERP entries carry (iterations << 16) | first iteration's index in `it0`.
+__host__ __device__ inline int iterp_count(int it0) { return it0 >> 16; }
+__host__ __device__ inline int iterp_first(int it0) { return it0 & 0xffff; }
 inline bool op_multi(int kind) { return kind == OP_ITER2 || kind == OP_ITER4 || kind == OP_ITER8; }
 
 // Entries per launch: one per lane of a wave64 (the kernel's entry lookup), and 64
@@ -235,6 +271,7 @@ struct TickEntry {
     unsigned int roles;        // buffer roles before the op (see Group)
     float sigma, tau, mu;      // ITER: step sizes of the iteration
     int it0;                   // ITER2 / ITER4 / ITER8: index of the first iteration in the warp;
+                               // ITERP: (iterations << 16) | that index;
                                // EFINAL: the warp of the record
 };
 static_assert(sizeof(TickEntry) == 56, "TickEntry layout");
@@ -278,7 +315,11 @@ inline int op_tiles(int kind, int level, const int *w, const int *h, const int *
 
 // Launches one op for the entries of `args` (all of one kind) in tile shape `shape`;
 // hipErrorInvalidValue (nothing launched) when no kernel exists for that kind and shape.
+// OP_ITERP launches hold one entry.
 hipError_t launch_tick(const PairDesc *desc, const TickArgs &args, int storage, int shape, hipStream_t st);
+// Workgroups of the persistent op (tile shape `shape`) that the device holds at once: its
+// compute units times the workgroups one of them holds; 0 without such a kernel.
+int persist_capacity(int storage, int shape);
 
 // ---- single-plane kernels (load time, outputs) ----
 struct Dims {
