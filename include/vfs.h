@@ -197,7 +197,10 @@ int vfs_get_energy(vfs_solver *s, int level, int warp, vfs_energy_t *out);
  *             tiles as the device holds workgroups at once).  If such a launch finds not
  *             every workgroup resident (other kernels on the device), it gives up, and the
  *             solve runs again without persistent launches at the next call that waits
- *             for it; vfs_get_option "persist_reruns" counts those runs.
+ *             for it; vfs_get_option "persist_reruns" counts those runs, and
+ *             "persist_why" tells what the last launch that gave up waited for (its tile
+ *             count << 12 | 1 arrivals, 2 ring values, 3 a neighbour's progress << 8 |
+ *             iteration).
  * Results do not depend on "graph", "autotune", "shape", "per_launch", "pair" or
  * "persist".
  * No counterpart in the reference (FP32 only, stereotgv.h:47-100; its block shape is a

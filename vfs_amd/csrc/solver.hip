@@ -212,6 +212,7 @@ struct vfs_solver {
     bool persist_check = false;
     int persist_masked = 0;
     int persist_reruns = 0;  // solves run again without persistent launches
+    unsigned persist_why = 0;  // XSync::why of the last launch that gave up
     int roles = vfs::kRolesInit;  // buffer roles after the last solve
     bool solved = false;  // a solve has run since initialize
 
@@ -690,7 +691,11 @@ int ensure_engine(vfs_solver *s)
 
 // ---- persistent launches (vfs::OP_ITERP)
 
-// Workgroups of the persistent op that the device holds at once, per (device, storage, shape).
+// Workgroups of the persistent op that the device surely holds at once, per (device,
+// storage, shape): one per compute unit (where a compute unit holds one at all).  The
+// occupancy calculation can promise more for small workgroups, but whether the hardware
+// places that many depends on how it spreads their waves over the SIMDs; one per compute
+// unit does not, and the faster tilings have at most that many tiles anyway.
 int persist_cap(int device, int storage, int shape)
 {
     static std::mutex mu;
@@ -700,8 +705,11 @@ int persist_cap(int device, int storage, int shape)
     auto it = cache.find(k);
     if (it != cache.end()) return it->second;
     const int c = vfs::persist_capacity(storage, shape);
-    cache[k] = c;
-    return c;
+    int cus = 0;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess) cus = 0;
+    const int r = c > 0 ? std::min(c, cus) : 0;
+    cache[k] = r;
+    return r;
 }
 
 // Tiles of the persistent op at `level` in tile shape `sh`, 0 where it cannot run: a
@@ -739,7 +747,7 @@ int ensure_xbuf(vfs_solver *s, size_t words)
     if (!s->xsync) {
         int rc = dev_alloc(s, (void **)&s->xsync, sizeof(vfs::XSync));
         if (rc) return rc;
-        const vfs::XSync init{1u, 0u, 0u, 0u};
+        const vfs::XSync init{1u, 0u, 0u, 0u, 0u, {0u, 0u, 0u}};
         HIPCHK(s, hipMemcpyAsync(s->xsync, &init, sizeof init, hipMemcpyHostToDevice, s->stream));
         HIPCHK(s, hipStreamSynchronize(s->stream));
         s->desc_dirty = true;
@@ -771,6 +779,7 @@ int persist_aborted(vfs_solver *s, bool *aborted)
     HIPCHK(s, hipStreamSynchronize(s->stream));
     if (!x.abort) return VFS_OK;
     *aborted = true;
+    s->persist_why = x.why;
     const unsigned zero = 0;
     HIPCHK(s, hipMemcpyAsync(&s->xsync->abort, &zero, sizeof zero, hipMemcpyHostToDevice, s->stream));
     HIPCHK(s, hipStreamSynchronize(s->stream));
@@ -1801,6 +1810,7 @@ int vfs_get_option(const vfs_solver *s, const char *key, int *value)
     else if (!strcmp(key, "per_launch")) *value = s->force_per_launch;
     else if (!strcmp(key, "persist")) *value = s->persist_opt;
     else if (!strcmp(key, "persist_reruns")) *value = s->persist_reruns;
+    else if (!strcmp(key, "persist_why")) *value = (int)s->persist_why;
     else if (!strcmp(key, "pair")) *value = s->force_per_launch < 0 ? -1 : s->force_per_launch > 1 ? 1 : 0;
     else return VFS_ERR_ARG;
     return VFS_OK;

@@ -1345,8 +1345,12 @@ void k_iterk(const PairDesc *__restrict__ D, const TickArgs A)
 // before the next iteration each refreshes its ring from its neighbours' words.  A word
 // is (value bits | tag << 32) with tag = (launch counter << 8) | iteration, written and
 // read by agent-scope atomics, so a reader waits for the tag it needs without fences.
-// Words are double-buffered by the parity of the iteration: a tile writes iteration j + 2's
-// words only after its neighbours have read iteration j's (it needed their j + 1 words).
+// Words are double-buffered by the parity of the iteration.  Besides the ring values every
+// tile waits each iteration for a progress word of each of its (up to 8) neighbours, so that
+// no tile runs more than one iteration ahead of a neighbour whatever the mask (a tile reads
+// its upper-left neighbour's corner but not the other way round, and a ring outside the
+// mask reads nothing): a tile writes iteration j + 2's words only after every neighbour has
+// published j + 1, i.e. has read the tile's words of iteration j.
 // Buffers, stores and out-of-mask values follow the K-iteration op with K = the op's
 // iteration count: iteration n - 2 stores u, u_, v_ to the other buffers, iteration n - 1
 // the whole state to the spare ones, and the host rotates the roles.  Every wait is
@@ -1373,13 +1377,15 @@ struct LdsP {
 };
 
 // Exchange words of one tile and parity: u_, v_.x, v_.y (value-major) of column tw - 1
-// (A), column tw - 2 (B), row th - 1 (C), row th - 2 (D), column 0 (E) and row 0 (F).
-// Every position of a tile's ring lies in one of its neighbours' strips.
+// (A), column tw - 2 (B), row th - 1 (C), row th - 2 (D), column 0 (E) and row 0 (F),
+// then the tile's progress word (HB).  Every position of a tile's ring lies in one of its
+// neighbours' strips.
 template <int SH>
 struct StripP {
     static constexpr int TW = kShape[SH].tw, TH = kShape[SH].th;
     static constexpr int A = 0, B = 3 * TH, C = 6 * TH, D = C + 3 * TW, E = D + 3 * TW, F = E + 3 * TH;
-    static constexpr int WORDS = F + 3 * TW;
+    static constexpr int HB = F + 3 * TW;
+    static constexpr int WORDS = HB + 1;
     static_assert(WORDS == persist_words(SH), "strip layout");
 };
 
@@ -1427,7 +1433,8 @@ __device__ __forceinline__ void op_iterp(const Tile &t, int roles, const float4 
             const unsigned long long t0 = rtime();
             while (xload32(&xs->arrive) < (unsigned)ntiles) {
                 if (rtime() - t0 > kArriveTicks) {
-                    xstore32(&xs->abort, 1u);
+                    if (__hip_atomic_exchange(&xs->abort, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0)
+                        xstore32(&xs->why, ((unsigned)ntiles << 12) | (1u << 8));
                     ab = 1;
                     break;
                 }
@@ -1570,11 +1577,13 @@ __device__ __forceinline__ void op_iterp(const Tile &t, int roles, const float4 
         // the tile (the strips in order A..F: 3 (tw + th) positions) and refreshes ring
         // position tid (those some dual reads, 3 (tw + th) + 5: columns -2, -1 and tw, rows
         // -2, -1 and th of the staged region), where that position lies in the image and is
-        // processed.  pub_s: LDS index | strip word << 12; rcv_i: LDS index; rcv_w: the
-        // source tile's strip word (parity 0) | 1 << 30 for a column strip (value stride
-        // th; rows: tw).
-        static_assert(B::NR <= 4096 && X::WORDS < (1 << 20), "slot packing");
-        const int ntx = (w + TW - 1) / TW;
+        // processed; the next 8 work-items wait for the 8 neighbours' progress words.
+        // pub_s: LDS index | strip word << 12; rcv_i: LDS index (-2: a progress word);
+        // rcv_w: the source tile's word (parity 0) | 1 << 30 for a column strip (value
+        // stride th; rows: tw).
+        constexpr int RS = 3 * (TW + TH) + 5;
+        static_assert(B::NR <= 4096 && X::WORDS < (1 << 20) && RS + 8 <= NT, "exchange slots");
+        const int ntx = (w + TW - 1) / TW, nty = (h + TH - 1) / TH;
         int pub_s = -1, rcv_i = -1, rcv_w = 0;
         if (tid < 3 * (TW + TH)) {
             int lx, ly, off;
@@ -1609,6 +1618,14 @@ __device__ __forceinline__ void op_iterp(const Tile &t, int roles, const float4 
                 rcv_i = i;
                 rcv_w = ((sy * ntx + sx) * 2 * X::WORDS + off) | (col << 30);
             }
+        } else if (tid < RS + 8) {
+            const int d = tid - RS, ddx = d < 3 ? d - 1 : d < 5 ? (d == 3 ? -1 : 1) : d - 6;
+            const int ddy = d < 3 ? -1 : d < 5 ? 0 : 1;
+            const int nx = x0 / TW + ddx, ny = y0 / TH + ddy;
+            if (nx >= 0 && ny >= 0 && nx < ntx && ny < nty) {
+                rcv_i = -2;
+                rcv_w = (ny * ntx + nx) * 2 * X::WORDS + X::HB;
+            }
         }
         unsigned long long *__restrict__ mine = xbuf + (long)tile * 2 * X::WORDS;
         const unsigned tag0 = epoch << 8;
@@ -1634,7 +1651,21 @@ __device__ __forceinline__ void op_iterp(const Tile &t, int roles, const float4 
                     xstore(pw + st, vv.x, tg);
                     xstore(pw + 2 * st, vv.y, tg);
                 }
-                if (rcv_i >= 0) {
+                if (tid == NT - 1) xstore(mine + ((j - 1) & 1) * X::WORDS + X::HB, 0.f, tag0 | (unsigned)(j - 1));
+                if (rcv_i == -2) {
+                    // a neighbour's progress word
+                    const unsigned tg = tag0 | (unsigned)(j - 1);
+                    const unsigned long long *pw = xbuf + (long)rcv_w + ((j - 1) & 1) * X::WORDS;
+                    unsigned long long t0 = 0;
+                    for (int polls = 0;; polls++) {
+                        if ((unsigned)(xload(pw) >> 32) == tg) break;
+                        if (polls == 0) t0 = rtime();
+                        if ((polls & 63) == 63 && (rtime() - t0 > kPollTicks || xload32(&xs->abort) != 0)) {
+                            bad = 3;
+                            break;
+                        }
+                    }
+                } else if (rcv_i >= 0) {
                     const unsigned tg = tag0 | (unsigned)(j - 1);
                     const unsigned long long *pw = xbuf + (long)(rcv_w & 0x3fffffff) + ((j - 1) & 1) * X::WORDS;
                     const int st = (rcv_w >> 30) ? TH : TW;
@@ -1647,7 +1678,7 @@ __device__ __forceinline__ void op_iterp(const Tile &t, int roles, const float4 
                             break;
                         if (polls == 0) t0 = rtime();
                         if ((polls & 63) == 63 && (rtime() - t0 > kPollTicks || xload32(&xs->abort) != 0)) {
-                            bad = 1;
+                            bad = 2;
                             break;
                         }
                     }
@@ -1656,7 +1687,10 @@ __device__ __forceinline__ void op_iterp(const Tile &t, int roles, const float4 
                 }
             }
             if (__syncthreads_or(bad)) {
-                if (tid == 0) xstore32(&xs->abort, 1u);
+                // (a work-item that gave up first records why, unless another launch's
+                // workgroup did already)
+                if (bad && __hip_atomic_exchange(&xs->abort, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0)
+                    xstore32(&xs->why, ((unsigned)ntiles << 12) | ((unsigned)bad << 8) | ((unsigned)j & 255u));
                 return false;
             }
             IterParams ip = ip0;

@@ -79,8 +79,8 @@ __host__ __device__ constexpr bool persist_shape(int shape)
            iterp_region(shape) <= 3072;
 }
 // Exchange words of one tile and parity: u_, v_.x, v_.y of columns tw-1, tw-2, rows
-// th-1, th-2, column 0 and row 0 (StripP in tgv_device.h).
-__host__ __device__ constexpr int persist_words(int shape) { return 9 * (kShape[shape].tw + kShape[shape].th); }
+// th-1, th-2, column 0 and row 0, and the tile's progress word (StripP in tgv_device.h).
+__host__ __device__ constexpr int persist_words(int shape) { return 9 * (kShape[shape].tw + kShape[shape].th) + 1; }
 // Tiles of one persistent launch at most (bounds the exchange buffer).
 constexpr int kMaxPersistTiles = 1024;
 // Launch bookkeeping of the persistent op, one per pair (device memory): the launch
@@ -89,6 +89,10 @@ constexpr int kMaxPersistTiles = 1024;
 // workgroup resident); the host clears it after re-running the solve without the op.
 struct XSync {
     unsigned int epoch, arrive, done, abort;
+    // the first give-up since the flag was cleared: the launch's tile count << 12 | what it
+    // waited for (1 arrivals, 2 ring words, 3 a neighbour's progress word) << 8 | the
+    // iteration (low 8 bits)
+    unsigned int why, pad[3];
 };
 
 // Storage precision of the image pyramid, warped image, derivatives, tensor and duals.
