@@ -1393,22 +1393,35 @@ struct StripP {
 constexpr unsigned long long kArriveTicks = 500000ull;  // every workgroup resident: 5 ms
 constexpr unsigned long long kPollTicks = 5000000ull;   // a neighbour's words: 50 ms
 
+// The exchange words and the bookkeeping are device memory reached through descriptor
+// pointers: accessed as global (not flat) memory, so that waiting for LDS operations
+// does not wait for exchange loads in flight as well.
+typedef __attribute__((address_space(1))) unsigned long long g_u64;
+typedef __attribute__((address_space(1))) unsigned g_u32;
 __device__ __forceinline__ unsigned long long xload(const unsigned long long *p)
 {
-    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    return __hip_atomic_load((const g_u64 *)p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 __device__ __forceinline__ void xstore(unsigned long long *p, float v, unsigned tag)
 {
-    __hip_atomic_store(p, (unsigned long long)__float_as_uint(v) | ((unsigned long long)tag << 32), __ATOMIC_RELAXED,
-                       __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_store((g_u64 *)p, (unsigned long long)__float_as_uint(v) | ((unsigned long long)tag << 32),
+                       __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 __device__ __forceinline__ unsigned xload32(const unsigned *p)
 {
-    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    return __hip_atomic_load((const g_u32 *)p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 __device__ __forceinline__ void xstore32(unsigned *p, unsigned v)
 {
-    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_store((g_u32 *)p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ unsigned xadd32(unsigned *p, unsigned v)
+{
+    return __hip_atomic_fetch_add((g_u32 *)p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ unsigned xswap32(unsigned *p, unsigned v)
+{
+    return __hip_atomic_exchange((g_u32 *)p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 __device__ __forceinline__ unsigned long long rtime() { return __builtin_amdgcn_s_memrealtime(); }
 
@@ -1428,12 +1441,12 @@ __device__ __forceinline__ void op_iterp(const Tile &t, int roles, const float4 
     if (tid == 0) {
         const unsigned ep = xload32(&xs->epoch);
         unsigned ab = xload32(&xs->abort);
-        __hip_atomic_fetch_add(&xs->arrive, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        xadd32(&xs->arrive, 1u);
         if (!ab) {
             const unsigned long long t0 = rtime();
             while (xload32(&xs->arrive) < (unsigned)ntiles) {
                 if (rtime() - t0 > kArriveTicks) {
-                    if (__hip_atomic_exchange(&xs->abort, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0)
+                    if (xswap32(&xs->abort, 1u) == 0)
                         xstore32(&xs->why, ((unsigned)ntiles << 12) | (1u << 8));
                     ab = 1;
                     break;
@@ -1689,7 +1702,7 @@ __device__ __forceinline__ void op_iterp(const Tile &t, int roles, const float4 
             if (__syncthreads_or(bad)) {
                 // (a work-item that gave up first records why, unless another launch's
                 // workgroup did already)
-                if (bad && __hip_atomic_exchange(&xs->abort, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0)
+                if (bad && xswap32(&xs->abort, 1u) == 0)
                     xstore32(&xs->why, ((unsigned)ntiles << 12) | ((unsigned)bad << 8) | ((unsigned)j & 255u));
                 return false;
             }
@@ -1824,7 +1837,7 @@ __device__ __forceinline__ void op_iterp(const Tile &t, int roles, const float4 
     // ---- the last workgroup to finish resets the arrivals and advances the launch counter
     __syncthreads();
     if (tid == 0) {
-        const unsigned d = __hip_atomic_fetch_add(&xs->done, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        const unsigned d = xadd32(&xs->done, 1u);
         if (d == (unsigned)ntiles - 1) {
             xstore32(&xs->arrive, 0u);
             xstore32(&xs->done, 0u);
