@@ -29,16 +29,18 @@ HBM_PEAK_GBS = 8000.0
 B_ITER_FP32 = 100.0
 # HBM bytes per launch of the level-0 inner-iteration kernel measured with rocprofv3
 # --pmc (FETCH_SIZE and WRITE_SIZE, scaled by a copy kernel of known size), one summary
-# per launch choice (tile shape, iterations per launch); each holds its command and counters.
-TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "pmc_traffic_k_iterk_{tw}x{th}x{k}_{workload}_L0.md")
+# per launch choice (kernel: k_iterk or the persistent k_iterp, tile shape, iterations
+# per launch); each holds its command and counters.
+TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "pmc_traffic_{kernel}_{tw}x{th}x{k}_{workload}_L0.md")
+PERSISTENT = 0x200  # per_launch of a level whose warps run in persistent launches
 
 
-def profiled_traffic(tw, th, k, workload):
+def profiled_traffic(kernel, tw, th, k, workload):
     """`traffic_bytes_per_launch = N` from the committed PMC summary of this launch choice,
     or None when there is none."""
     import re
     try:
-        with open(TRAFFIC_PROFILE.format(tw=tw, th=th, k=k, workload=workload)) as f:
+        with open(TRAFFIC_PROFILE.format(kernel=kernel, tw=tw, th=th, k=k, workload=workload)) as f:
             m = re.search(r"traffic_bytes_per_launch\s*=\s*([0-9]+)", f.read())
         return int(m.group(1)) if m else None
     except OSError:
@@ -73,8 +75,9 @@ def main():
                     help="force one tile shape (0..6) instead of the per-level tuner (profiling aid)")
     ap.add_argument("--tuning", default="",
                     help="per-level launch choices from level 0, comma-separated (profiling aid): 'S[p]' one "
-                         "iteration per launch in tile shape S (p = Tp prefetch), 'MxK[h]' K = 2 or 4 iterations per "
-                         "launch in tile shape M (h = half workgroups), or 'S[p]/MxK[h]' for both")
+                         "iteration per launch in tile shape S (p = Tp prefetch), 'MxK[h]' K = 2, 4 or 8 iterations "
+                         "per launch in tile shape M (h = half workgroups), 'MxP' a warp per persistent launch in "
+                         "tile shape M, or 'S[p]/MxK[h]' for both")
     ap.add_argument("--no-autotune", action="store_true",
                     help="no launch tuner (fixed defaults; with --tuning for reproducible profiles)")
     ap.add_argument("--iter-only", type=int, default=0,
@@ -169,7 +172,8 @@ def main():
     t_launch = t_iter * per_launch
     bytes_launch = B_ITER_FP32 * w0 * h0 * per_launch
     achieved = bytes_launch / t_launch / 1e9
-    (tw0, th0), _, _ = g.level_tuning(0, True, with_per_launch=True)
+    (tw0, th0), _, k0 = g.level_tuning(0, True, with_per_launch=True)
+    kernel0 = "k_iterp" if k0 == PERSISTENT else "k_iterk" if per_launch > 1 else "k_tick_iter"
 
     # achievable-bandwidth reference: a plain copy kernel, 1 GiB moved per launch
     copy_ms = g.time_copy(512 << 20, 10) / 10
@@ -197,9 +201,12 @@ def main():
                        "p99": round(float(np.percentile(lat, 99)), 4),
                        "mean": round(float(lat.mean()), 4)},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": profiled_traffic(tw0, th0, per_launch, args.workload),
+                     "frac": round(achieved / HBM_PEAK_GBS, 4),
+                     "traffic": profiled_traffic(kernel0, tw0, th0, per_launch, args.workload),
                      "kernel": f"inner TGV iterations at level 0 ({w0}x{h0}): {per_launch} per launch, "
-                               f"{tw0}x{th0} tiles",
+                               f"{tw0}x{th0} tiles" + (" (persistent workgroups: the tiles stay on chip, "
+                                                       "borders exchanged every iteration)"
+                                                       if k0 == PERSISTENT else ""),
                      "us_per_launch": round(t_launch * 1e6, 3), "us_per_iter": round(t_iter * 1e6, 3),
                      "algorithmic_bytes_per_launch": int(bytes_launch),
                      "copy_kernel_GBs": round(copy_gbs, 1)},
@@ -220,8 +227,10 @@ def main():
     for lvl in range(wl["levels"]):
         (tw, th), pre, k = g.level_tuning(lvl, True, with_per_launch=True)
         lw, lh, _ = g.level_dims(lvl)
-        out["tile_tuning"][f"L{lvl} {lw}x{lh}"] = (f"{tw}x{th}" + (" +Tp prefetch" if pre else "")
-                                                   + (f", {k} iterations per launch" if k > 1 else ""))
+        out["tile_tuning"][f"L{lvl} {lw}x{lh}"] = (
+            f"{tw}x{th}" + (" +Tp prefetch" if pre else "")
+            + (", a warp per launch (persistent workgroups)" if k == PERSISTENT
+               else f", {k} iterations per launch" if k > 1 else ""))
     print(f"bench: tile tuning {out['tile_tuning']}", file=sys.stderr, flush=True)
 
     legs = set() if args.quick else ({"served"} if args.serve_only else {l for l in args.legs.split(",") if l})
@@ -248,7 +257,10 @@ def apply_tuning(obj, spec):
         for part in lspec.split("/"):
             if "x" in part:
                 m, k = part.split("x")
-                obj.set_level_tuning(lvl, int(m), per_launch=int(k.rstrip("h")), half_workgroup=k.endswith("h"))
+                if k == "P":
+                    obj.set_level_tuning(lvl, int(m), per_launch=PERSISTENT)
+                else:
+                    obj.set_level_tuning(lvl, int(m), per_launch=int(k.rstrip("h")), half_workgroup=k.endswith("h"))
             else:
                 obj.set_level_tuning(lvl, int(part.rstrip("p")), preload=part.endswith("p"))
 

@@ -13,6 +13,19 @@ def test_inner_ops_greedy_split():
     assert inner_ops(1, 8) == ["ITER"]
 
 
+def test_inner_ops_persistent_split():
+    # an even count per persistent launch within the 128-step table, the rest one at a time
+    assert inner_ops(50, "P") == ["ITERP"]
+    assert inner_ops(15, "P") == ["ITERP", "ITER"]
+    assert inner_ops(1, "P") == ["ITER"]
+    assert inner_ops(130, "P") == ["ITERP", "ITER", "ITER"]
+    assert inner_ops(131, "P") == ["ITERP"] + ["ITER"] * 3
+    # 848x800 L5 W10 I50 with persistent launches at level 0 only: 10 ITERP dispatches
+    ops = program_ops(5, 10, 50, ("P", 4, 8, 8, 8))
+    assert sum(1 for k, l in ops if k == "ITERP") == 10
+    assert all(l == 0 for k, l in ops if k == "ITERP")
+
+
 def test_program_ops_count_matches_traced_solve():
     # a traced 848x800 solve (L5 W10 I50; four iterations per launch at levels 0-2,
     # eight at levels 3-4) issued 651 dispatches with a separate first preparation per

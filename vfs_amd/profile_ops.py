@@ -10,7 +10,9 @@ workgroup adds them up), then the upscale)
 and attributes each dispatch of one solve to its op and level.
 Levels that run several inner iterations per launch (`--per-launch`, one count per level
 from level 0, as the tuner reports them) issue `k_iterk` dispatches: ITER8 / ITER4 / ITER2 while
-they fit the iteration count, the rest one at a time.
+they fit the iteration count, the rest one at a time; `P` marks a level whose warps run
+their iterations in persistent `k_iterp` launches (ITERP: an even count, at most 128, each;
+an odd remainder one at a time).
 
     python -m vfs_amd.profile_ops TRACE.csv --levels 5 --warps 10 --iters 50 --per-launch 4,4,4,4,4 --skip-tail 100
     python -m vfs_amd.profile_ops --batch-traffic FETCH.csv WRITE.csv --storage f16   # HBM bytes per px-iteration
@@ -25,9 +27,18 @@ import collections
 import csv
 
 
-def inner_ops(iters: int, per_launch: int) -> list[str]:
-    """Inner-iteration ops of one warp (the library's greedy split)."""
+def inner_ops(iters: int, per_launch) -> list[str]:
+    """Inner-iteration ops of one warp (the library's greedy split; per_launch "P": the
+    persistent op)."""
     out, it = [], 0
+    if per_launch == "P":
+        while iters - it >= 2:
+            k = min(iters - it, 128 - it) & ~1
+            if k < 2:
+                break
+            out.append("ITERP")
+            it += k
+        return out + ["ITER"] * (iters - it)
     while it < iters:
         k = next((c for c in (8, 4, 2) if c <= per_launch and it + c <= iters), 1)
         out.append({8: "ITER8", 4: "ITER4", 2: "ITER2", 1: "ITER"}[k])
@@ -55,7 +66,8 @@ def program_ops(levels: int, warps: int, iters: int, per_launch=(), energy: int 
 
 def summarize(trace_csv: str, levels: int, warps: int, iters: int, skip_tail: int = 0, per_launch=(),
               energy: int = 1) -> str:
-    rows = [r for r in csv.DictReader(open(trace_csv)) if r["Kernel_Name"].startswith(("k_tick", "k_iterk"))]
+    rows = [r for r in csv.DictReader(open(trace_csv))
+            if r["Kernel_Name"].startswith(("k_tick", "k_iterk", "k_iterp"))]
     rows.sort(key=lambda r: int(r["Start_Timestamp"]))
     seq = program_ops(levels, warps, iters, per_launch, energy)
     n = len(seq)
@@ -64,7 +76,8 @@ def summarize(trace_csv: str, levels: int, warps: int, iters: int, skip_tail: in
         raise ValueError(f"trace holds {end} tick dispatches, one solve needs {n}")
     solve = rows[end - n:end]
     for (kind, lvl), r in zip(seq, solve):
-        if (kind in ("ITER2", "ITER4", "ITER8")) != r["Kernel_Name"].startswith("k_iterk"):
+        if ((kind in ("ITER2", "ITER4", "ITER8")) != r["Kernel_Name"].startswith("k_iterk")
+                or (kind == "ITERP") != r["Kernel_Name"].startswith("k_iterp")):
             raise ValueError(f"dispatch {r['Kernel_Name']} does not match op {kind} at level {lvl}: "
                              "check --per-launch")
     agg = collections.OrderedDict()
@@ -157,6 +170,41 @@ def batch_traffic(fetch_csv: str, write_csv: str, storage: str, pairs: int, px: 
     return "\n".join(lines)
 
 
+def kernel_traffic(fetch_csv: str, write_csv: str, kernel: str, copy_bytes: int, command: str, title: str,
+                   algorithmic: int) -> str:
+    """Summary of the HBM bytes per dispatch of one kernel (name prefix `kernel`, the
+    dispatches with the largest grid): FETCH_SIZE and WRITE_SIZE per dispatch (KiB), median
+    over the dispatches, scaled by what the same counters read for the k_copy dispatches
+    that move `copy_bytes` each way; against `algorithmic` bytes per dispatch."""
+    import statistics as st
+    rows = {c: _counter_rows(p, c) for c, p in (("FETCH_SIZE", fetch_csv), ("WRITE_SIZE", write_csv))}
+    med, scale, sel = {}, {}, None
+    for c, rs in rows.items():
+        copies = [v for _, n, _, v in rs if "k_copy" in n]
+        scale[c] = copy_bytes / 1024.0 / st.median(copies)
+        ks = [(n, g, v) for _, n, g, v in rs if n.startswith(kernel)]
+        gmax = max(g for _, g, _ in ks)
+        vals = [v for _, g, v in ks if g == gmax]
+        sel = (gmax, len(vals), sorted({n for n, g, _ in ks if g == gmax}))
+        med[c] = (st.median(copies), st.median(vals), min(vals), max(vals))
+    grid, n, names = sel
+    kib = med["FETCH_SIZE"][1] * scale["FETCH_SIZE"] + med["WRITE_SIZE"][1] * scale["WRITE_SIZE"]
+    lines = [f"# {title}", "", "Commands (two passes, one counter each; from the repository root on the GPU machine):",
+             "", "    " + command, "",
+             f"Dispatches analysed: {n} `{kernel}` launches with grid {grid} work-items ({', '.join(names)}).", "",
+             f"| counter | k_copy per dispatch (KiB) | scale to bytes moved | {kernel} median per dispatch (KiB) | "
+             "min / max |", "|---|---|---|---|---|"]
+    for c in ("FETCH_SIZE", "WRITE_SIZE"):
+        cm, m, lo, hi = med[c]
+        lines.append(f"| {c} | {cm:.1f} | x {scale[c]:.3f} ({copy_bytes >> 20} MiB each way) | {m:.2f} | "
+                     f"{lo:.1f} / {hi:.1f} |")
+    lines += ["", f"HBM bytes per launch = (FETCH x scale + WRITE x scale):", "",
+              f"traffic_bytes_per_launch = {kib * 1024:.0f}", "",
+              f"Algorithmic bytes per launch: {algorithmic:,}; measured / algorithmic = "
+              f"{kib * 1024 / algorithmic:.4f}.", ""]
+    return "\n".join(lines)
+
+
 def trace_segments(trace_csv: str, gap_ms: float = 20.0, after_last: str = "k_copy") -> str:
     """Segments of a kernel trace: the k_tick / k_iterk dispatches after the last
     `after_last` dispatch, split where the device idles longer than `gap_ms`; per
@@ -198,6 +246,18 @@ def main():
     if len(sys.argv) > 1 and sys.argv[1] == "--segments":
         print(trace_segments(sys.argv[2]))
         return
+    if len(sys.argv) > 1 and sys.argv[1] == "--kernel-traffic":
+        kp = argparse.ArgumentParser(prog="profile_ops --kernel-traffic")
+        kp.add_argument("fetch_csv")
+        kp.add_argument("write_csv")
+        kp.add_argument("--kernel", required=True)
+        kp.add_argument("--copy-bytes", type=int, default=256 << 20)
+        kp.add_argument("--command", default="")
+        kp.add_argument("--title", default="")
+        kp.add_argument("--algorithmic", type=int, required=True)
+        a = kp.parse_args(sys.argv[2:])
+        print(kernel_traffic(a.fetch_csv, a.write_csv, a.kernel, a.copy_bytes, a.command, a.title, a.algorithmic))
+        return
     if len(sys.argv) > 1 and sys.argv[1] == "--batch-traffic":
         bp = argparse.ArgumentParser(prog="profile_ops --batch-traffic")
         bp.add_argument("fetch_csv")
@@ -217,10 +277,11 @@ def main():
     ap.add_argument("--warps", type=int, default=10)
     ap.add_argument("--iters", type=int, default=50)
     ap.add_argument("--skip-tail", type=int, default=0)
-    ap.add_argument("--per-launch", default="", help="inner iterations per launch of each level, from level 0")
+    ap.add_argument("--per-launch", default="",
+                    help="inner iterations per launch of each level, from level 0 (P: persistent launches)")
     ap.add_argument("--energy", type=int, default=1, help="the solver's energy option (0, 1, 2)")
     a = ap.parse_args()
-    per_launch = [int(x) for x in a.per_launch.split(",") if x]
+    per_launch = [x if x == "P" else int(x) for x in a.per_launch.split(",") if x]
     print(summarize(a.trace, a.levels, a.warps, a.iters, a.skip_tail, per_launch, a.energy))
 
 
