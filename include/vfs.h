@@ -213,17 +213,22 @@ int vfs_get_option(const vfs_solver *s, const char *key, int *value);
  * iterations per launch also 7..17 = 8x8, 4x4, 32x32, 20x20, 36x25, 27x25, 14x13, 7x7,
  * 43x34, 34x40, 53x25, and 1, 3, 4, 6 of the former), whether the inner iteration prefetches its Tp-plane
  * neighbours, and the inner iterations per launch, 1, 2, 4 or 8, or VFS_PERSISTENT when
- * cold solves run a warp's iterations in one persistent launch (tile shapes 7, 8, 11,
- * 12, 13, 14, 17 or 18 = 53x50) (`per_launch` may be NULL).  Runs the tuner first when
- * it has not run for this geometry.
+ * cold solves run a warp's iterations in one persistent launch that exchanges tile
+ * borders every iteration (tile shapes 7, 8, 11, 12, 13, 14, 17 or 18 = 53x50), or
+ * VFS_PERSISTENT | K (K = 2, 4, 8) when that launch runs blocks of K iterations with
+ * recomputed halos and exchanges the halo rings between blocks (tile shapes 6, 7, 8, 10,
+ * 12, 13 with sides of at least 2K and a staged region of at most 2048 positions)
+ * (`per_launch` may be NULL).  Runs the tuner first when it has not run for this
+ * geometry.
  * No counterpart in the reference. */
 int vfs_get_level_tuning(vfs_solver *s, int masked, int level, int *shape, int *preload, int *per_launch);
 /* Sets the launch choices of one level (reproducible profiling, expert use); they hold
  * until the next initialize or option change.  `per_launch` (1, 2, 4 or 8) may carry
  * VFS_HALF_WORKGROUP: multi-iteration workgroups of 512 work-items, two or three
  * positions each, where the tile's staged region holds 513 .. 1536 positions; or it is
- * VFS_PERSISTENT: cold solves run each warp's iterations in one persistent launch in
- * tile shape `shape` (other solves keep the level's other choices).
+ * VFS_PERSISTENT, optionally | 2, 4 or 8: cold solves run each warp's iterations in one
+ * persistent launch in tile shape `shape`, exchanging borders every iteration or halo
+ * rings every K iterations (other solves keep the level's other choices).
  * No counterpart in the reference. */
 #define VFS_HALF_WORKGROUP 0x100
 #define VFS_PERSISTENT 0x200

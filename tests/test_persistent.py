@@ -166,3 +166,86 @@ def test_persistent_848x800_workload():
     # progress, so none overwrites exchange words a neighbour has not read)
     g.time_solves(True, 10)
     assert g.get_option("persist_reruns") == 0, hex(g.get_option("persist_why"))
+
+
+# blocked persistent launches (blocks of K iterations with recomputed halos, rings
+# exchanged between blocks): the tile shapes of each K
+BLOCKED = ((2, (6, 7, 8, 10, 12, 13)), (4, (6, 7, 10, 12, 13)), (8, (6, 10)))
+
+
+def set_blocked(g, shape, K, masked=True, levels=3):
+    """Fixes blocked persistent launches (blocks of K) in `shape` at every level where the
+    device holds all their tiles at once; returns those levels (at least one)."""
+    from vfs_amd import VfsError
+    done = []
+    for lvl in range(levels):
+        try:
+            g.set_level_tuning(lvl, shape, per_launch=StereoTgv.PERSISTENT | K, masked=masked)
+            done.append(lvl)
+        except VfsError:
+            w, h, _ = g.level_dims(lvl)
+            tw, th = StereoTgv.SHAPES[shape]
+            assert -(-w // tw) * -(-h // th) > 256, (shape, K, lvl, "blocked persistent launch refused")
+    assert done, (shape, K)
+    return done
+
+
+@pytest.mark.parametrize("masked", [True, False])
+def test_every_blocked_persistent_form(masked):
+    """Blocks of 2, 4 and 8 iterations in every tile shape that has them, at every level
+    (19 iterations per warp: one launch of 18 or 16 and a remainder in other launches),
+    equal one iteration per launch."""
+    pair = small_pair(200, 120, seed=59)
+    ref = planes_of(solve(pair, 3, 2, 19, masked, persist=0, per_launch=1), (0, 1))
+    for K, shapes in BLOCKED:
+        for shape in shapes:
+            g = make_gpu(pair, 3, 2, 19)
+            print(f"blocked persistent K {K} shape {shape} masked {masked}", flush=True)
+            levels = set_blocked(g, shape, K, masked)
+            (g.solveStereoForwardMasked if masked else g.solveStereoForward)()
+            for lvl in levels:
+                assert g.level_tuning(lvl, masked, with_per_launch=True)[::2] == (StereoTgv.SHAPES[shape],
+                                                                                  StereoTgv.PERSISTENT | K)
+            assert_same(planes_of(g, (0, 1)), ref, f"K {K} shape {shape}")
+            assert g.get_option("persist_reruns") == 0, hex(g.get_option("persist_why"))
+
+
+@pytest.mark.parametrize("storage", ["f16", "bf16"])
+def test_blocked_persistent_storage(storage):
+    """Half-precision planes: p and q rounded to the storage type between iterations, the
+    rings carrying the unrounded values (every dual step reads them rounded)."""
+    pair = small_pair(200, 120, seed=61)
+    ref = planes_of(solve(pair, 3, 2, 17, persist=0, storage=storage, per_launch=1), (0, 1))
+    for K, shape in ((4, 10), (8, 6), (2, 13)):
+        g = make_gpu(pair, 3, 2, 17, storage=storage)
+        print(f"blocked persistent {storage} K {K} shape {shape}", flush=True)
+        set_blocked(g, shape, K)
+        g.solveStereoForwardMasked()
+        assert_same(planes_of(g, (0, 1)), ref, f"{storage} K {K} shape {shape}")
+
+
+@pytest.mark.parametrize("masked", [True, False])
+def test_blocked_persistent_odd_geometry(masked):
+    """157x94 (row padding, partial tiles narrower than a ring in both directions)."""
+    pair = small_pair(157, 94, seed=67)
+    ref = planes_of(solve(pair, 3, 2, 12, masked, persist=0, per_launch=1), flow=False)
+    for K, shape in ((2, 8), (4, 7), (4, 12), (8, 6), (8, 10)):
+        g = make_gpu(pair, 3, 2, 12)
+        print(f"odd geometry blocked K {K} shape {shape} masked {masked}", flush=True)
+        set_blocked(g, shape, K, masked)
+        (g.solveStereoForwardMasked if masked else g.solveStereoForward)()
+        assert_same(planes_of(g, flow=False), ref, f"odd geometry, K {K} shape {shape}")
+
+
+@pytest.mark.parametrize("iters", [2, 4, 9, 130])
+def test_blocked_persistent_iteration_counts(iters):
+    """Fewer iterations than a block (2 with blocks of 4: no persistent launch), one block
+    (4), blocks and a remainder (9), beyond the 128-entry step table (130)."""
+    pair = small_pair(128, 96, seed=71)
+    ref = planes_of(solve(pair, 2, 2, iters, persist=0, per_launch=1))
+    for K, shape in ((4, 7), (8, 6)):
+        g = make_gpu(pair, 2, 2, iters)
+        set_blocked(g, shape, K, levels=2)
+        print(f"blocked persistent iterations {iters} K {K}", flush=True)
+        g.solveStereoForwardMasked()
+        assert_same(planes_of(g), ref, f"{iters} iterations, K {K}")

@@ -321,12 +321,14 @@ class StereoTgv:
 
     HALF_WORKGROUP = 0x100
     # per_launch of a level whose cold solves run a warp's iterations in one persistent launch
+    # (| 2, 4 or 8: in blocks of that many iterations, halo rings exchanged between blocks)
     PERSISTENT = 0x200
 
     def set_level_tuning(self, level: int, shape: int, preload: bool = False, per_launch: int = 1,
                          masked: bool = True, half_workgroup: bool = False):
         """Fix the launch choices of one level (shape index as option "shape"; 1, 2, 4 or 8
-        inner iterations per launch, optionally in half workgroups, or PERSISTENT)."""
+        inner iterations per launch, optionally in half workgroups, or PERSISTENT, optionally
+        | 2, 4 or 8)."""
         k = int(per_launch) | (self.HALF_WORKGROUP if half_workgroup else 0)
         return self._check(self._L.vfs_set_level_tuning(self._h, int(masked), level, shape, int(preload), k),
                            "set_level_tuning")

@@ -25,9 +25,20 @@ VFS_INST_DECL(bf16_k8)
 VFS_INST_DECL(f32_k0)
 VFS_INST_DECL(f16_k0)
 VFS_INST_DECL(bf16_k0)
-int persist_capacity_f32(int shape);
-int persist_capacity_f16(int shape);
-int persist_capacity_bf16(int shape);
+#define VFS_CAP_DECL(sfx) int persist_capacity_##sfx(int shape);
+VFS_CAP_DECL(f32_k0)
+VFS_CAP_DECL(f32_k2)
+VFS_CAP_DECL(f32_k4)
+VFS_CAP_DECL(f32_k8)
+VFS_CAP_DECL(f16_k0)
+VFS_CAP_DECL(f16_k2)
+VFS_CAP_DECL(f16_k4)
+VFS_CAP_DECL(f16_k8)
+VFS_CAP_DECL(bf16_k0)
+VFS_CAP_DECL(bf16_k2)
+VFS_CAP_DECL(bf16_k4)
+VFS_CAP_DECL(bf16_k8)
+#undef VFS_CAP_DECL
 
 namespace {
 
@@ -247,6 +258,19 @@ hipError_t launch_tick(const PairDesc *desc, const TickArgs &args, int storage, 
         kP[storage == kF16 ? 1 : storage == kBF16 ? 2 : 0](desc, args, shape, st);
         return hipSuccess;
     }
+    if (args.e[0].kind == OP_ITERPK) {
+        const int n = iterp_count(args.e[0].it0), K = iterp_block(args.e[0].it0);
+        if (args.n != 1 || shape < 0 || shape >= kShapes || (K != 2 && K != 4 && K != 8) ||
+            !persistk_shape(shape, K) || n < K || n % K || iterp_first(args.e[0].it0) + n > kMaxSched ||
+            args.total > kMaxPersistTiles)
+            return hipErrorInvalidValue;
+        using Fn = void (*)(const PairDesc *, const TickArgs &, int, hipStream_t);
+        static constexpr Fn kP[3][3] = {{launch_ops_f32_k2, launch_ops_f32_k4, launch_ops_f32_k8},
+                                        {launch_ops_f16_k2, launch_ops_f16_k4, launch_ops_f16_k8},
+                                        {launch_ops_bf16_k2, launch_ops_bf16_k4, launch_ops_bf16_k8}};
+        kP[storage == kF16 ? 1 : storage == kBF16 ? 2 : 0][K == 2 ? 0 : K == 4 ? 1 : 2](desc, args, shape, st);
+        return hipSuccess;
+    }
     if (op_multi(args.e[0].kind)) {
         const int K = op_iterations(args.e[0].kind), sh = shape_index(shape);
         if (shape < 0 || sh >= kShapes || !iterk_ok(sh, K) ||
@@ -266,11 +290,20 @@ hipError_t launch_tick(const PairDesc *desc, const TickArgs &args, int storage, 
     return hipSuccess;
 }
 
-int persist_capacity(int storage, int shape)
+int persist_capacity(int storage, int shape, int K)
 {
-    if (shape < 0 || shape >= kShapes || !persist_shape(shape)) return 0;
-    return storage == kF16 ? persist_capacity_f16(shape)
-                           : storage == kBF16 ? persist_capacity_bf16(shape) : persist_capacity_f32(shape);
+    if (shape < 0 || shape >= kShapes) return 0;
+    const int si = storage == kF16 ? 1 : storage == kBF16 ? 2 : 0;
+    if (K == 1) {
+        if (!persist_shape(shape)) return 0;
+        static constexpr int (*kC[3])(int) = {persist_capacity_f32_k0, persist_capacity_f16_k0, persist_capacity_bf16_k0};
+        return kC[si](shape);
+    }
+    if ((K != 2 && K != 4 && K != 8) || !persistk_shape(shape, K)) return 0;
+    static constexpr int (*kC[3][3])(int) = {{persist_capacity_f32_k2, persist_capacity_f32_k4, persist_capacity_f32_k8},
+                                             {persist_capacity_f16_k2, persist_capacity_f16_k4, persist_capacity_f16_k8},
+                                             {persist_capacity_bf16_k2, persist_capacity_bf16_k4, persist_capacity_bf16_k8}};
+    return kC[si][K == 2 ? 0 : K == 4 ? 1 : 2](shape);
 }
 
 void launch_fill_zero(void *ptr, size_t bytes, hipStream_t st)

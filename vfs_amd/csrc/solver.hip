@@ -103,10 +103,12 @@ struct LevelTune {
     // with four or eight iterations per launch: tile shape and half flag of the
     // two-iteration launches that finish an iteration count not divisible by four
     unsigned char m2shape = 4, m2half = 0;
-    // a warp's iterations in one launch of persistent workgroups (vfs::OP_ITERP), in tile
-    // shape pshape, wherever the solve allows it (a single pair's cold solve); otherwise
-    // the launches above
-    unsigned char persist = 0, pshape = 18;
+    // a warp's iterations in one launch of persistent workgroups, in tile shape pshape,
+    // wherever the solve allows it (a single pair's cold solve); otherwise the launches
+    // above.  pblock 1: borders exchanged every iteration (vfs::OP_ITERP); 2, 4, 8: blocks
+    // of that many iterations with recomputed halos, rings exchanged between blocks
+    // (vfs::OP_ITERPK)
+    unsigned char persist = 0, pshape = 18, pblock = 1;
     // tile shape of the inner iterations
     int iter_shape() const { return persist ? pshape : per_launch > 1 ? mshape : shape; }
 };
@@ -123,7 +125,8 @@ struct Tuning {
             k[l] = (unsigned int)lv[l].shape | (unsigned int)lv[l].preload << 3 | (unsigned int)code << 4 |
                    (unsigned int)lv[l].mshape << 6 | (unsigned int)lv[l].mhalf << 11 |
                    (unsigned int)lv[l].m2shape << 12 | (unsigned int)lv[l].m2half << 17 |
-                   (unsigned int)lv[l].persist << 18 | (unsigned int)lv[l].pshape << 19;
+                   (unsigned int)lv[l].persist << 18 | (unsigned int)lv[l].pshape << 19 |
+                   (unsigned int)(lv[l].pblock == 8 ? 3 : lv[l].pblock == 4 ? 2 : lv[l].pblock == 2 ? 1 : 0) << 24;
         }
         return k;
     }
@@ -371,10 +374,11 @@ void push_op(Program &pr, int kind, int level, int masked, int roles)
 // One level of the solve from the tensor to the end of the last warp
 // (stereotgv.cu:409-494, masked; :275-364 unmasked).  With `sync_spare`, each warp's
 // first multi-iteration op is preceded by OP_SYNC (warm solves above level 0).  With
-// `persist`, a warp's iterations run in OP_ITERP ops (an even count each, within the
-// schedule table) and the rest one at a time.
+// `pblock` 1, a warp's iterations run in OP_ITERP ops (an even count each, within the
+// schedule table), with 2, 4 or 8 in OP_ITERPK ops (a multiple of pblock each), and the
+// rest as without it.
 void push_level(Program &pr, const Params &P, int masked, int l, int warps, int iters, int &roles, int per_launch,
-                bool sync_spare = false, bool persist = false)
+                bool sync_spare = false, int pblock = 0)
 {
     std::vector<float4> sched((size_t)std::max(iters, 1));
     iter_schedule(P.timestep_lambda, iters, sched.data());
@@ -395,10 +399,10 @@ void push_level(Program &pr, const Params &P, int masked, int l, int warps, int 
             int k = 1;
             for (int c : {8, 4, 2})
                 if (k == 1 && c <= per_launch && it + c <= iters && it + c <= vfs::kMaxSched) k = c;
-            const int kp = std::min(iters - it, vfs::kMaxSched - it) & ~1;
-            if (persist && !sync_spare && kp >= 2) {
-                o.kind = vfs::OP_ITERP;
-                o.it0 = (kp << 16) | it;
+            const int kp = std::min(iters - it, vfs::kMaxSched - it) & ~(std::max(pblock, 2) - 1);
+            if (pblock > 0 && !sync_spare && kp >= std::max(pblock, 2)) {
+                o.kind = pblock > 1 ? vfs::OP_ITERPK : vfs::OP_ITERP;
+                o.it0 = (kp << 16) | (pblock > 1 ? pblock << 8 : 0) | it;
                 o.flags = (unsigned char)((it == 0 ? vfs::EF_FIRST : 0) | (it + kp == iters ? vfs::EF_WRITE_TP_LAST : 0));
                 roles = vfs::rrotate_iter(roles);
                 pr.ops.push_back(o);
@@ -446,7 +450,7 @@ void apply_tuning(Program &pr, const Tuning &tn)
 {
     for (ProgOp &o : pr.ops) {
         const LevelTune &lt = tn.lv[vfs::op_tile_level(o.kind, o.level)];
-        if (o.kind == vfs::OP_ITERP) {
+        if (o.kind == vfs::OP_ITERP || o.kind == vfs::OP_ITERPK) {
             o.shape = lt.pshape;
             continue;
         }
@@ -480,7 +484,7 @@ Program build_program(const Params &P, int masked, int warm_level, int warps, in
         for (int l = P.nLevels - 1; l >= 0; l--) {
             if (l == P.nLevels - 1) push_op(pr, vfs::OP_CINIT, l, masked, roles);
             push_level(pr, P, masked, l, P.nWarpIters, P.nSolverIters, roles, tn.lv[l].per_launch, false,
-                       persist && tn.lv[l].persist);
+                       persist && tn.lv[l].persist ? tn.lv[l].pblock : 0);
             if (l > 0) {
                 push_op(pr, vfs::OP_UP, l, masked, roles);
                 roles = vfs::rswap(vfs::rswap(roles, vfs::G_U), vfs::G_UB) ^ vfs::RB_UV;
@@ -689,22 +693,22 @@ int ensure_engine(vfs_solver *s)
     return VFS_OK;
 }
 
-// ---- persistent launches (vfs::OP_ITERP)
+// ---- persistent launches (vfs::OP_ITERP, and vfs::OP_ITERPK in blocks of K = 2, 4, 8)
 
 // Workgroups of the persistent op that the device surely holds at once, per (device,
-// storage, shape): one per compute unit (where a compute unit holds one at all).  The
+// storage, shape, K): one per compute unit (where a compute unit holds one at all).  The
 // occupancy calculation can promise more for small workgroups, but whether the hardware
 // places that many depends on how it spreads their waves over the SIMDs; one per compute
 // unit does not, and the faster tilings have at most that many tiles anyway.
-int persist_cap(int device, int storage, int shape)
+int persist_cap(int device, int storage, int shape, int K)
 {
     static std::mutex mu;
-    static std::map<std::array<int, 3>, int> cache;
+    static std::map<std::array<int, 4>, int> cache;
     std::lock_guard<std::mutex> g(mu);
-    const std::array<int, 3> k{device, storage, shape};
+    const std::array<int, 4> k{device, storage, shape, K};
     auto it = cache.find(k);
     if (it != cache.end()) return it->second;
-    const int c = vfs::persist_capacity(storage, shape);
+    const int c = vfs::persist_capacity(storage, shape, K);
     int cus = 0;
     if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess) cus = 0;
     const int r = c > 0 ? std::min(c, cus) : 0;
@@ -712,16 +716,18 @@ int persist_cap(int device, int storage, int shape)
     return r;
 }
 
-// Tiles of the persistent op at `level` in tile shape `sh`, 0 where it cannot run: a
-// shape without the op, a mask pixel in column 0 (as for every multi-iteration op), or more
-// tiles than the device holds workgroups at once (or than kMaxPersistTiles).
-int persist_tiles(const vfs_solver *s, int masked, int level, int sh)
+// Tiles of the persistent op (K = 1: OP_ITERP, else OP_ITERPK in blocks of K) at `level`
+// in tile shape `sh`, 0 where it cannot run: a shape without the op, a mask pixel in
+// column 0 (as for every multi-iteration op), or more tiles than the device holds
+// workgroups at once (or than kMaxPersistTiles).
+int persist_tiles(const vfs_solver *s, int masked, int level, int sh, int K = 1)
 {
-    if (sh < 0 || sh >= vfs::kShapes || !vfs::persist_shape(sh)) return 0;
+    if (sh < 0 || sh >= vfs::kShapes) return 0;
+    if (K == 1 ? !vfs::persist_shape(sh) : !vfs::persistk_shape(sh, K)) return 0;
     if (s->lv[level].w < 2 || (masked && s->mask_col0[level])) return 0;
     const vfs::ShapeDim d = vfs::kShape[sh];
     const int n = ((s->lv[level].w + d.tw - 1) / d.tw) * ((s->lv[level].h + d.th - 1) / d.th);
-    if (n > vfs::kMaxPersistTiles || n > persist_cap(s->device, s->storage, sh)) return 0;
+    if (n > vfs::kMaxPersistTiles || n > persist_cap(s->device, s->storage, sh, K)) return 0;
     return n;
 }
 
@@ -738,7 +744,10 @@ int persist_default_shape(const vfs_solver *s, int masked, int level)
 }
 
 // Exchange words (both parities) of a persistent launch of `tiles` tiles in shape `sh`.
-size_t persist_words_of(int tiles, int sh) { return (size_t)tiles * 2 * (size_t)vfs::persist_words(sh); }
+size_t persist_words_of(int tiles, int sh, int K = 1)
+{
+    return (size_t)tiles * 2 * (size_t)(K == 1 ? vfs::persist_words(sh) : vfs::persistk_words(sh));
+}
 
 // Allocates the launch bookkeeping and at least `words` exchange words.  The tags in the
 // words start at launch counter 1, so zeroed words never match.
@@ -817,14 +826,18 @@ LevelTune default_tune(int level)
     return t;
 }
 
-// Persistent candidates of a single pair's tuning: the iterations of one launch (a warp's,
-// even, within the schedule table), per level and shape the tiles of the launch (0: not
-// possible there), and the pair's launch bookkeeping.
+// Persistent candidates of a single pair's tuning: the iterations of a warp (within the
+// schedule table; a launch takes the even count, or the multiple of its K, below it), per
+// level, form (K = 1, 2, 4, 8) and shape the tiles of the launch (0: not possible there),
+// and the pair's launch bookkeeping.
 struct PersistCands {
     int iters = 0;
-    std::vector<std::array<int, vfs::kShapes>> tiles;  // [level index][shape]
+    std::vector<std::array<std::array<int, vfs::kShapes>, 4>> tiles;  // [level index][log2 K][shape]
     vfs::XSync *xsync = nullptr;
 };
+constexpr int kPersistK[4] = {1, 2, 4, 8};
+// Iterations of one persistent launch of blocks of K (1: OP_ITERP) out of `iters`.
+int persist_launch_iters(int iters, int K) { return iters & ~(std::max(K, 2) - 1); }
 
 // Times the inner iteration of every level in `levels` in each (tile shape, Tp preload)
 // on the launch set `descs` (their arenas point at scratch memory; `d_desc` is any valid
@@ -842,14 +855,15 @@ hipError_t time_launch_choices(hipStream_t st, const vfs::PairDesc *d_desc, cons
     // candidates: (shape, preload) of one iteration per launch, then two-, four- and
     // eight-iteration launches for the shapes that have them, in full or half workgroups
     // (then the persistent op in each shape)
+    // (then the persistent ops, OP_ITERP and OP_ITERPK in blocks of 2, 4 and 8, in each shape)
     constexpr int kSingle = vfs::kTickShapes * 2, kMulti = kSingle + 3 * vfs::kShapes * 2;
-    constexpr int kCand = kMulti + vfs::kShapes;
+    constexpr int kCand = kMulti + 4 * vfs::kShapes;
     auto cand = [&](int c, int &shape, int &pre, int &per_launch, int &half) {
         if (c >= kMulti) {
-            shape = c - kMulti;
+            shape = (c - kMulti) % vfs::kShapes;
             pre = 0;
             half = 0;
-            per_launch = pc ? pc->iters : 0;
+            per_launch = pc ? persist_launch_iters(pc->iters, kPersistK[(c - kMulti) / vfs::kShapes]) : 0;
         } else if (c < kSingle) {
             shape = c / 2;
             pre = c % 2;
@@ -880,7 +894,10 @@ hipError_t time_launch_choices(hipStream_t st, const vfs::PairDesc *d_desc, cons
                 int shape, pre, per_launch, half;
                 cand(c, shape, pre, per_launch, half);
                 const bool persist = c >= kMulti;
-                if (persist && (!pc || pc->iters < 2 || pc->tiles[li][shape] == 0 || n != 1)) continue;
+                const int pk = persist ? kPersistK[(c - kMulti) / vfs::kShapes] : 0;
+                if (persist && (!pc || per_launch < std::max(pk, 2) ||
+                                pc->tiles[li][(c - kMulti) / vfs::kShapes][shape] == 0 || n != 1))
+                    continue;
                 if (!persist && per_launch > 1 &&
                     (!vfs::iterk_ok(shape, per_launch) || !allow_multi[li] ||
                      (half && !vfs::half_workgroup_ok(shape, per_launch))))
@@ -892,7 +909,7 @@ hipError_t time_launch_choices(hipStream_t st, const vfs::PairDesc *d_desc, cons
                     if (!near) continue;
                 }
                 ProgOp o{};
-                o.kind = persist ? (unsigned char)vfs::OP_ITERP : iter_kind(per_launch);
+                o.kind = persist ? (unsigned char)(pk > 1 ? vfs::OP_ITERPK : vfs::OP_ITERP) : iter_kind(per_launch);
                 o.level = (unsigned char)levels[li];
                 o.masked = (unsigned char)masked;
                 o.flags = (unsigned char)(pre ? vfs::EF_PRELOAD_TP : 0);
@@ -901,7 +918,7 @@ hipError_t time_launch_choices(hipStream_t st, const vfs::PairDesc *d_desc, cons
                 o.sigma = 1.0f;
                 o.tau = 1.0f;
                 o.mu = 0.8f;
-                o.it0 = persist ? (per_launch << 16) : 0;
+                o.it0 = persist ? (per_launch << 16) | (pk > 1 ? pk << 8 : 0) : 0;
                 vfs::TickArgs a = tick_header(P);
                 int acc = 0;
                 for (int k = 0; k < n; k++) {
@@ -988,7 +1005,8 @@ hipError_t time_launch_choices(hipStream_t st, const vfs::PairDesc *d_desc, cons
             if (best[li * kCand + c] < 1e29f && (bestp < 0 || best[li * kCand + c] < best[li * kCand + bestp])) bestp = c;
         if (bestp >= 0 && best[li * kCand + bestp] < 0.98f * pick_ms) {
             pick.persist = 1;
-            pick.pshape = (unsigned char)(bestp - kMulti);
+            pick.pshape = (unsigned char)((bestp - kMulti) % vfs::kShapes);
+            pick.pblock = (unsigned char)kPersistK[(bestp - kMulti) / vfs::kShapes];
         }
         tn.lv[l] = pick;
     }
@@ -1012,11 +1030,12 @@ int measure_levels(vfs_solver *s, int masked, const std::vector<int> &levels, Tu
         pc.iters = std::min(s->nSolverIters, vfs::kMaxSched) & ~1;
         size_t words = 0;
         for (int l : levels) {
-            std::array<int, vfs::kShapes> t{};
-            for (int sh = 0; sh < vfs::kShapes; sh++) {
-                t[sh] = persist_tiles(s, masked, l, sh);
-                words = std::max(words, persist_words_of(t[sh], sh));
-            }
+            std::array<std::array<int, vfs::kShapes>, 4> t{};
+            for (int kb = 0; kb < 4; kb++)
+                for (int sh = 0; sh < vfs::kShapes; sh++) {
+                    t[kb][sh] = persist_tiles(s, masked, l, sh, kPersistK[kb]);
+                    words = std::max(words, persist_words_of(t[kb][sh], sh, kPersistK[kb]));
+                }
             pc.tiles.push_back(t);
         }
         if (pc.iters >= 2 && words > 0) {
@@ -1097,9 +1116,10 @@ int tune_solver(vfs_solver *s, int masked)
             const int sh = persist_default_shape(s, m, l);
             tn.lv[l].persist = sh >= 0 ? 1 : 0;
             if (sh >= 0) tn.lv[l].pshape = (unsigned char)sh;
+            tn.lv[l].pblock = 1;
         }
         // (a fixed or cached choice the device cannot hold at once is not taken)
-        if (tn.lv[l].persist && persist_tiles(s, m, l, tn.lv[l].pshape) == 0) tn.lv[l].persist = 0;
+        if (tn.lv[l].persist && persist_tiles(s, m, l, tn.lv[l].pshape, tn.lv[l].pblock) == 0) tn.lv[l].persist = 0;
     }
     s->tuned[m] = true;
     return VFS_OK;
@@ -1114,10 +1134,11 @@ bool persist_program(const vfs_solver *s, int masked, const Tuning &tn, size_t *
     if (s->persist_opt == 0 || (std::min(s->nSolverIters, vfs::kMaxSched) & ~1) < 2) return false;
     for (int l = 0; l < s->nLevels; l++) {
         if (!tn.lv[l].persist) continue;
-        const int n = persist_tiles(s, masked, l, tn.lv[l].pshape);
-        if (n == 0) continue;
+        const int K = tn.lv[l].pblock;
+        const int n = persist_tiles(s, masked, l, tn.lv[l].pshape, K);
+        if (n == 0 || persist_launch_iters(std::min(s->nSolverIters, vfs::kMaxSched), K) < std::max(K, 2)) continue;
         any = true;
-        *words = std::max(*words, persist_words_of(n, tn.lv[l].pshape));
+        *words = std::max(*words, persist_words_of(n, tn.lv[l].pshape, K));
     }
     return any;
 }
@@ -1826,7 +1847,7 @@ int vfs_get_level_tuning(vfs_solver *s, int masked, int level, int *shape, int *
     const LevelTune t = s->tuning[masked ? 1 : 0].lv[level];
     *shape = t.iter_shape();
     *preload = t.preload;
-    if (per_launch) *per_launch = t.persist ? VFS_PERSISTENT : t.per_launch;
+    if (per_launch) *per_launch = t.persist ? VFS_PERSISTENT | (t.pblock > 1 ? t.pblock : 0) : t.per_launch;
     return VFS_OK;
 }
 
@@ -1834,18 +1855,22 @@ int vfs_set_level_tuning(vfs_solver *s, int masked, int level, int shape, int pr
 {
     if (!s) return VFS_ERR_ARG;
     if (!s->initialized || level < 0 || level >= s->nLevels) return fail(s, VFS_ERR_ARG, "bad level");
-    if (per_launch == VFS_PERSISTENT) {
+    if (per_launch & VFS_PERSISTENT) {
+        const int K = std::max(per_launch & ~VFS_PERSISTENT, 1);
+        if (K != 1 && K != 2 && K != 4 && K != 8)
+            return fail(s, VFS_ERR_ARG, "VFS_PERSISTENT takes 2, 4 or 8 iterations per block, or none");
         const int m = masked ? 1 : 0;
         int rc = set_device(s);
         if (!rc && !s->tuned[m]) rc = tune_solver(s, m);
         if (rc) return rc;
         if (s->persist_opt == 0) return fail(s, VFS_ERR_ARG, "persistent launches are switched off (option persist)");
-        if (persist_tiles(s, m, level, shape) == 0)
-            return fail(s, VFS_ERR_ARG, "no persistent launch in shape %d at level %d (shape, mask column 0, or "
-                                        "more tiles than the device holds at once)", shape, level);
+        if (persist_tiles(s, m, level, shape, K) == 0)
+            return fail(s, VFS_ERR_ARG, "no persistent launch in shape %d with blocks of %d at level %d (shape, mask "
+                                        "column 0, or more tiles than the device holds at once)", shape, K, level);
         LevelTune &t = s->tuning[m].lv[level];
         t.persist = 1;
         t.pshape = (unsigned char)shape;
+        t.pblock = (unsigned char)K;
         return VFS_OK;
     }
     const bool half = (per_launch & VFS_HALF_WORKGROUP) != 0;
@@ -1918,9 +1943,12 @@ int vfs_time_inner_iterations(vfs_solver *s, int level, int reps, float *ms_tota
     int per_launch = s->tuning[1].lv[level].per_launch;
     // a persistent level: launches of a warp's iterations, as the solve runs them
     size_t words = 0;
-    const bool persist = persist_program(s, 1, s->tuning[1], &words) && s->tuning[1].lv[level].persist &&
-                         persist_tiles(s, 1, level, s->tuning[1].lv[level].pshape) > 0;
-    const int sched_n = persist ? std::min(s->nSolverIters, vfs::kMaxSched) & ~1 : 48;
+    const LevelTune &lt = s->tuning[1].lv[level];
+    const int pb = lt.pblock;
+    const bool persist = persist_program(s, 1, s->tuning[1], &words) && lt.persist &&
+                         persist_tiles(s, 1, level, lt.pshape, pb) > 0 &&
+                         persist_launch_iters(std::min(s->nSolverIters, vfs::kMaxSched), pb) >= std::max(pb, 2);
+    const int sched_n = persist ? persist_launch_iters(std::min(s->nSolverIters, vfs::kMaxSched), pb) : 48;
     if (persist) {
         per_launch = sched_n;
         if ((rc = ensure_xbuf(s, words))) return rc;
@@ -1935,10 +1963,11 @@ int vfs_time_inner_iterations(vfs_solver *s, int level, int reps, float *ms_tota
         int done = 0;
         while (done < reps) {
             w.ops.clear();
-            push_level(w, Q, 1, level, 1, sched_n, roles, per_launch, false, persist);
+            push_level(w, Q, 1, level, 1, sched_n, roles, per_launch, false, persist ? pb : 0);
             for (const ProgOp &o : w.ops) {
                 if (done >= reps) break;
-                const int k = o.kind == vfs::OP_ITERP ? vfs::iterp_count(o.it0) : vfs::op_iterations(o.kind);
+                const int k = o.kind == vfs::OP_ITERP || o.kind == vfs::OP_ITERPK ? vfs::iterp_count(o.it0)
+                                                                                   : vfs::op_iterations(o.kind);
                 if (k > 0) {
                     pr.ops.push_back(o);
                     done += k;

@@ -911,6 +911,108 @@ __device__ __forceinline__ void op_iter(const Tile &t, int roles, const IterPara
     }
 }
 
+// ------------------------------------------------------------------ persistent launches
+//
+// Primitives of the persistent ops (OP_ITERPK below, OP_ITERP further down): exchange words
+// between the workgroups of one launch, and its bookkeeping.
+
+// Bounded waits of the persistent op, in ticks of the 100 MHz real-time counter.
+constexpr unsigned long long kArriveTicks = 500000ull;  // every workgroup resident: 5 ms
+constexpr unsigned long long kPollTicks = 5000000ull;   // a neighbour's words: 50 ms
+
+// The exchange words and the bookkeeping are device memory reached through descriptor
+// pointers: accessed as global (not flat) memory, so that waiting for LDS operations
+// does not wait for exchange loads in flight as well.
+typedef __attribute__((address_space(1))) unsigned long long g_u64;
+typedef __attribute__((address_space(1))) unsigned g_u32;
+__device__ __forceinline__ unsigned long long xload(const unsigned long long *p)
+{
+    return __hip_atomic_load((const g_u64 *)p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void xstore(unsigned long long *p, float v, unsigned tag)
+{
+    __hip_atomic_store((g_u64 *)p, (unsigned long long)__float_as_uint(v) | ((unsigned long long)tag << 32),
+                       __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ unsigned xload32(const unsigned *p)
+{
+    return __hip_atomic_load((const g_u32 *)p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void xstore32(unsigned *p, unsigned v)
+{
+    __hip_atomic_store((g_u32 *)p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ unsigned xadd32(unsigned *p, unsigned v)
+{
+    return __hip_atomic_fetch_add((g_u32 *)p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ unsigned xswap32(unsigned *p, unsigned v)
+{
+    return __hip_atomic_exchange((g_u32 *)p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ unsigned long long rtime() { return __builtin_amdgcn_s_memrealtime(); }
+
+// Every workgroup of a persistent launch must be resident: it counts its arrival and waits
+// (bounded) for the others; s_word[0] receives the launch counter, s_word[1] whether to
+// leave at once (a launch gave up since the host last cleared the flag, or not every
+// workgroup arrived in time).
+__device__ __forceinline__ void persist_arrive(XSync *__restrict__ xs, int ntiles, unsigned *s_word)
+{
+    if (threadIdx.x == 0) {
+        const unsigned ep = xload32(&xs->epoch);
+        unsigned ab = xload32(&xs->abort);
+        xadd32(&xs->arrive, 1u);
+        if (!ab) {
+            const unsigned long long t0 = rtime();
+            while (xload32(&xs->arrive) < (unsigned)ntiles) {
+                if (rtime() - t0 > kArriveTicks) {
+                    if (xswap32(&xs->abort, 1u) == 0) xstore32(&xs->why, ((unsigned)ntiles << 12) | (1u << 8));
+                    ab = 1;
+                    break;
+                }
+                __builtin_amdgcn_s_sleep(2);
+            }
+        }
+        s_word[0] = ep;
+        s_word[1] = ab;
+    }
+    __syncthreads();
+}
+
+// A work-item that gave up waiting (`bad`: 2 ring values, 3 a neighbour's progress word)
+// sets the abort flag, and records why unless another workgroup did first.
+__device__ __forceinline__ void persist_give_up(XSync *__restrict__ xs, int ntiles, int bad, int step)
+{
+    if (bad && xswap32(&xs->abort, 1u) == 0)
+        xstore32(&xs->why, ((unsigned)ntiles << 12) | ((unsigned)bad << 8) | ((unsigned)step & 255u));
+}
+
+// The last workgroup to finish resets the arrivals and advances the launch counter.
+__device__ __forceinline__ void persist_finish(XSync *__restrict__ xs, int ntiles, unsigned epoch)
+{
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const unsigned d = xadd32(&xs->done, 1u);
+        if (d == (unsigned)ntiles - 1) {
+            xstore32(&xs->arrive, 0u);
+            xstore32(&xs->done, 0u);
+            xstore32(&xs->epoch, epoch + 1u);
+        }
+    }
+}
+
+// Waits (bounded) until the word at `p` carries `tag`; false if it gave up (waited too
+// long, or another workgroup gave up).
+__device__ __forceinline__ bool persist_wait_tag(const unsigned long long *p, unsigned tag, XSync *__restrict__ xs)
+{
+    unsigned long long t0 = 0;
+    for (int polls = 0; (unsigned)(xload(p) >> 32) != tag; polls++) {
+        if (polls == 0) t0 = rtime();
+        if ((polls & 63) == 63 && (rtime() - t0 > kPollTicks || xload32(&xs->abort) != 0)) return false;
+    }
+    return true;
+}
+
 // ------------------------------------------------------------------ K iterations per op
 //
 // Temporal blocking of the inner iteration (OP_ITER2, OP_ITER4, OP_ITER8): one workgroup advances
@@ -983,17 +1085,60 @@ __device__ __forceinline__ float rT(float v)
     return Cv<T>::f(Cv<T>::t(v));
 }
 
+// Exchange words of the blocked persistent op (OP_ITERPK), one tile and parity: u, u_, v_.x,
+// v_.y, p.x, p.y, q.x, q.y, q.z, q.w of every tile position (value-major, row-major within
+// a value plane), then the tile's progress word (HB).
+template <int SH>
+struct StripK {
+    static constexpr int TW = kShape[SH].tw, TH = kShape[SH].th;
+    static constexpr int VP = TW * TH, HB = 10 * VP, WORDS = HB + 1;
+    static_assert(WORDS == persistk_words(SH), "strip layout");
+};
+
+// The blocked persistent op's launch: `blocks` blocks of K iterations, this workgroup's tile
+// among the launch's `ntiles`, the pair's exchange words and bookkeeping, and two words of
+// LDS for the arrival.
+struct PersistK {
+    int blocks = 1;
+    int tile = 0, ntiles = 0;
+    XSync *xs = nullptr;
+    unsigned long long *xbuf = nullptr;
+    unsigned *s_word = nullptr;
+};
+
 // The step sizes of iteration j: sched[j] = (sigma, tau, mu, alpha1 sigma / 3); `ip0` holds the
 // schedule-independent fields and the flags of the op.
-template <int SH, int K, bool HALF, bool MASKED, typename T>
+// PERSIST (OP_ITERPK): the workgroups of the launch are persistent, one per tile.  A
+// workgroup stages its tile grown by 2K / K once and runs pk.blocks blocks of K iterations
+// in place.  After each block but the last it publishes the state of the bands of its tile
+// that neighbouring rings cover (columns < K and >= tw - 2K, rows < K and >= th - 2K; the
+// tile is at least 2K on each side, so a ring reads its 8 neighbours only) and then
+// refreshes its own ring (everything staged outside the tile that lies in the image and the
+// mask) from its neighbours' words; outside the mask nothing changes.  Words carry tags,
+// are double-buffered by block parity and every tile waits for all its neighbours'
+// progress words, as in OP_ITERP.  Only the last block stores the state (other and spare
+// buffers, as the K-iteration op does), so the result equals blocks x K iterations run one
+// block per launch.
+template <int SH, int K, bool HALF, bool MASKED, typename T, bool PERSIST = false>
 __device__ __forceinline__ void op_iterk(const Tile &t, int roles, const float4 *__restrict__ sched,
-                                         const IterParams &ip0, LdsK<SH, K> &L)
+                                         const IterParams &ip0, LdsK<SH, K> &L, const PersistK &pk = PersistK())
 {
     static_assert(K % 2 == 0, "K-iteration ops rotate the roles like two iterations");
     using B = BlkK<SH, K, HALF>;
     constexpr int NT = B::NT, NK = B::NK;
+    static_assert(!PERSIST || (!HALF && NK <= 2 && persistk_shape(SH, K)), "persistent K-iteration shape");
     const int w = t.w, h = t.h, s = t.s, x0 = t.x0, y0 = t.y0;
     const int tid = threadIdx.x;
+    unsigned epoch = 0;
+    if constexpr (PERSIST) {
+        // every workgroup of the launch must be resident
+        persist_arrive(pk.xs, pk.ntiles, pk.s_word);
+        epoch = pk.s_word[0];
+        if (pk.s_word[1]) {
+            persist_finish(pk.xs, pk.ntiles, epoch);
+            return;
+        }
+    }
     const float *__restrict__ mask = t.mask;
     const float *__restrict__ u_cur = slot(t, slot_u(rcur(roles, G_U)));
     float *__restrict__ u_oth = slot(t, slot_u(roth(roles, G_U)));
@@ -1145,20 +1290,24 @@ __device__ __forceinline__ void op_iterk(const Tile &t, int roles, const float4 
         np[k] = staged ? K - max(rpx, rpy) : 0;
     }
 
-    auto iterate = [&](auto full_c) {
+    // one block of K iterations (`blk`: its index, iterations blk K ..; `last`: the op's last
+    // block, which stores the state; a compile-time flag, so that no store address is kept
+    // across the blocks before it)
+    auto iterate = [&](auto full_c, int blk, auto last_c) {
         constexpr bool FULL = decltype(full_c)::value;
+        constexpr bool last = decltype(last_c)::value;
         // unrolled: each iteration's regions and stores are known at compile time (fewer
         // registers and tests than a loop over j)
 #pragma unroll
         for (int j = 0; j < K; j++) {
             IterParams ip = ip0;
-            const float4 sc = sched[j];
+            const float4 sc = sched[blk * K + j];
             ip.k1 = sc.w;  // alpha1 sigma / 3, as op_iter forms it
             ip.k0 = ip0.alpha0 * sc.x / 2.0f;
             ip.tau = sc.y;
             ip.mu = sc.z;
-            ip.first = j == 0 && ip0.first;
-            ip.write_tp = j == K - 1 && ip0.write_tp;
+            ip.first = j == 0 && blk == 0 && ip0.first;
+            ip.write_tp = j == K - 1 && last && ip0.write_tp;
             const int r = K - 1 - j;
 
             // ---- duals over the tile grown by 2r + 1 (left / top) and r (right / bottom)
@@ -1204,7 +1353,7 @@ __device__ __forceinline__ void op_iterk(const Tile &t, int roles, const float4 
                 o[k].p = pn;  // the primal step of this iteration reads them unrounded
                 L.q[i] = qn;
                 L.tp[i] = tpn;
-                if (r == 0 && in_tile(x, y)) {
+                if (r == 0 && last && in_tile(x, y)) {
                     const long pos = (long)y * s + x;
                     st2T(p_spr, pos, pn);
                     st4T(q_spr, pos, qn);
@@ -1223,7 +1372,7 @@ __device__ __forceinline__ void op_iterk(const Tile &t, int roles, const float4 
                 if (!FULL && !(fl[k] & F_IN)) continue;
                 const long pos = (long)y * s + x;
                 if (!FULL && MASKED && !(fl[k] & F_ACT)) {
-                    if (r == 0) {
+                    if (r == 0 && last) {
                         // outside the mask: the spare buffers take the current buffers' values
                         if (in_tile(x, y)) {
                             u_spr[pos] = o[k].u;
@@ -1269,11 +1418,11 @@ __device__ __forceinline__ void op_iterk(const Tile &t, int roles, const float4 
                 o[k].v = vs;
                 L.ub[i] = u_s;
                 L.v[i] = vs;
-                if (r == 1 && in_tile(x, y)) {
+                if (last && r == 1 && in_tile(x, y)) {
                     u_oth[pos] = un;
                     ub_oth[pos] = u_s;
                     v_oth[pos] = vs;
-                } else if (r == 0) {
+                } else if (last && r == 0) {
                     u_spr[pos] = un;
                     ub_spr[pos] = u_s;
                     v_spr[pos] = vs;
@@ -1282,8 +1431,119 @@ __device__ __forceinline__ void op_iterk(const Tile &t, int roles, const float4 
             __syncthreads();
         }
     };
-    if (full) iterate(std::true_type{});
-    else iterate(std::false_type{});
+    if constexpr (!PERSIST) {
+        if (full) iterate(std::true_type{}, 0, std::true_type{});
+        else iterate(std::false_type{}, 0, std::true_type{});
+    } else {
+        using X = StripK<SH>;
+        constexpr int TW = B::TW, TH = B::TH;
+        // Exchange roles, once per launch: xr[k] >= 0 for a ring position that neighbours
+        // update (its tile's value-0 word of parity 0), -2 for a tile position in the
+        // published bands, -1 otherwise; hb: the work-items 0..7 wait for the neighbours'
+        // progress words (that neighbour's word of parity 0, or -1 at the image border).
+        const int ntx = (w + TW - 1) / TW, nty = (h + TH - 1) / TH;
+        int xr[NK];
+#pragma unroll
+        for (int k = 0; k < NK; k++) {
+            const int i = tid + k * NT, x = ox[k], y = oy[k];
+            xr[k] = -1;
+            if (i >= B::NR || x < 0 || y < 0 || x >= w || y >= h || !act(i)) continue;
+            const int dx = x - x0, dy = y - y0;
+            if (in_tile(x, y)) {
+                if (dx < K || dx >= TW - 2 * K || dy < K || dy >= TH - 2 * K) xr[k] = -2;
+            } else {
+                const int sx = x / TW, sy = y / TH;
+                xr[k] = (sy * ntx + sx) * 2 * X::WORDS + (y - sy * TH) * TW + (x - sx * TW);
+            }
+        }
+        int hb = -1;
+        if (tid < 8) {
+            const int ddx = tid < 3 ? tid - 1 : tid < 5 ? (tid == 3 ? -1 : 1) : tid - 6;
+            const int ddy = tid < 3 ? -1 : tid < 5 ? 0 : 1;
+            const int nx = x0 / TW + ddx, ny = y0 / TH + ddy;
+            if (nx >= 0 && ny >= 0 && nx < ntx && ny < nty) hb = (ny * ntx + nx) * 2 * X::WORDS + X::HB;
+        }
+        unsigned long long *__restrict__ mine = pk.xbuf + (long)pk.tile * 2 * X::WORDS;
+        const unsigned tag0 = epoch << 8;
+        int blk = 0;
+        bool gave_up = false;
+        for (;; blk++) {
+            if (blk > 0) {
+                const unsigned tg = tag0 | (unsigned)(blk - 1);
+                const int par = (blk - 1) & 1;
+                // ---- publish the bands (the previous block's state: this work-item's
+                // registers and its own q in LDS) and the progress word
+#pragma unroll
+                for (int k = 0; k < NK; k++) {
+                    if (xr[k] != -2) continue;
+                    unsigned long long *pp = mine + par * X::WORDS + (oy[k] - y0) * TW + (ox[k] - x0);
+                    const float4 q = L.q[tid + k * NT];
+                    xstore(pp, o[k].u, tg);
+                    xstore(pp + X::VP, o[k].ub, tg);
+                    xstore(pp + 2 * X::VP, o[k].v.x, tg);
+                    xstore(pp + 3 * X::VP, o[k].v.y, tg);
+                    xstore(pp + 4 * X::VP, o[k].p.x, tg);
+                    xstore(pp + 5 * X::VP, o[k].p.y, tg);
+                    xstore(pp + 6 * X::VP, q.x, tg);
+                    xstore(pp + 7 * X::VP, q.y, tg);
+                    xstore(pp + 8 * X::VP, q.z, tg);
+                    xstore(pp + 9 * X::VP, q.w, tg);
+                }
+                if (tid == NT - 1) xstore(mine + par * X::WORDS + X::HB, 0.f, tg);
+                // ---- the neighbours' progress, then the ring's state
+                int bad = 0;
+                if (hb >= 0 && !persist_wait_tag(pk.xbuf + hb + par * X::WORDS, tg, pk.xs)) bad = 3;
+#pragma unroll
+                for (int k = 0; k < NK; k++) {
+                    if (xr[k] < 0 || bad) continue;
+                    // the ten words in two groups of five (registers: a 64-bit word each)
+                    const unsigned long long *pw = pk.xbuf + xr[k] + par * X::WORDS;
+                    float f[10];
+#pragma unroll
+                    for (int g = 0; g < 2; g++) {
+                        unsigned long long a[5];
+                        unsigned long long t0 = 0;
+                        for (int polls = 0;; polls++) {
+                            bool ok = true;
+#pragma unroll
+                            for (int v = 0; v < 5; v++) a[v] = xload(pw + (g * 5 + v) * X::VP);
+#pragma unroll
+                            for (int v = 0; v < 5; v++) ok = ok && (unsigned)(a[v] >> 32) == tg;
+                            if (ok) break;
+                            if (polls == 0) t0 = rtime();
+                            if ((polls & 63) == 63 && (rtime() - t0 > kPollTicks || xload32(&pk.xs->abort) != 0)) {
+                                bad = 2;
+                                break;
+                            }
+                        }
+#pragma unroll
+                        for (int v = 0; v < 5; v++) f[g * 5 + v] = __uint_as_float((unsigned)a[v]);
+                    }
+                    const int i = tid + k * NT;
+                    o[k].u = f[0];
+                    o[k].ub = f[1];
+                    o[k].v = make_float2(f[2], f[3]);
+                    o[k].p = make_float2(f[4], f[5]);
+                    L.ub[i] = o[k].ub;
+                    L.v[i] = o[k].v;
+                    L.q[i] = make_float4(f[6], f[7], f[8], f[9]);
+                }
+                if (__syncthreads_or(bad)) {
+                    persist_give_up(pk.xs, pk.ntiles, bad, blk);
+                    gave_up = true;
+                    break;
+                }
+            }
+            if (blk == pk.blocks - 1) break;
+            if (full) iterate(std::true_type{}, blk, std::false_type{});
+            else iterate(std::false_type{}, blk, std::false_type{});
+        }
+        if (!gave_up) {
+            if (full) iterate(std::true_type{}, blk, std::true_type{});
+            else iterate(std::false_type{}, blk, std::true_type{});
+        }
+        persist_finish(pk.xs, pk.ntiles, epoch);
+    }
 }
 
 // Half workgroups (512 work-items, two or three positions each) are held to 128 registers, so that
@@ -1328,6 +1588,48 @@ void k_iterk(const PairDesc *__restrict__ D, const TickArgs A)
     ip0.preload_tp = 0;
     if (E.masked) op_iterk<SH, K, HALF, true, T>(t, E.roles, sched, ip0, L);
     else op_iterk<SH, K, HALF, false, T>(t, E.roles, sched, ip0, L);
+}
+
+// The blocked persistent op (OP_ITERPK): one pair per launch, one workgroup per tile (tiles
+// in XCD-contiguous runs as the other op kernels), blocks of K iterations.
+template <typename T, int SH, int K>
+__global__ __launch_bounds__((BlkK<SH, K, false>::NT)) void k_iterkp(const PairDesc *__restrict__ D, const TickArgs A)
+{
+    using B = BlkK<SH, K, false>;
+    __shared__ LdsK<SH, K> L;
+    __shared__ unsigned s_word[2];
+    const int nb = gridDim.x, i = blockIdx.x, xcd = i & 7, q = nb >> 3, r = nb & 7;
+    const int b = (xcd < r ? xcd * (q + 1) : r * (q + 1) + (xcd - r) * q) + (i >> 3);
+    const TickEntry &E = A.e[0];
+    Tile t;
+    t.d = D + E.pair;
+    t.arena = E.arena;
+    t.plane = E.plane;
+    t.mask = E.mask;
+    t.w = E.w;
+    t.h = E.h;
+    t.s = E.s;
+    const int ntx = (t.w + B::TW - 1) / B::TW;
+    t.x0 = (b % ntx) * B::TW;
+    t.y0 = (b / ntx) * B::TH;
+    const float4 *sched = t.d->sched + iterp_first(E.it0);
+    IterParams ip0;
+    ip0.k1 = ip0.k0 = ip0.tau = ip0.mu = 0.f;  // set per iteration
+    ip0.lambda = A.lambda;
+    ip0.alpha0 = A.alpha0;
+    ip0.alpha1 = A.alpha1;
+    ip0.first = (E.flags & EF_FIRST) != 0;
+    ip0.write_tp = (E.flags & EF_WRITE_TP_LAST) != 0;  // the op's last iteration
+    ip0.preload_tp = 0;
+    PersistK pk;
+    pk.blocks = iterp_count(E.it0) / K;
+    pk.tile = b;
+    pk.ntiles = nb;
+    pk.xs = t.d->xsync;
+    pk.xbuf = t.d->xbuf;
+    pk.s_word = s_word;
+    if (E.masked) op_iterk<SH, K, false, true, T, true>(t, E.roles, sched, ip0, L, pk);
+    else op_iterk<SH, K, false, false, T, true>(t, E.roles, sched, ip0, L, pk);
 }
 
 // ------------------------------------------------------------------ a whole warp per launch
@@ -1389,41 +1691,6 @@ struct StripP {
     static_assert(WORDS == persist_words(SH), "strip layout");
 };
 
-// Bounded waits of the persistent op, in ticks of the 100 MHz real-time counter.
-constexpr unsigned long long kArriveTicks = 500000ull;  // every workgroup resident: 5 ms
-constexpr unsigned long long kPollTicks = 5000000ull;   // a neighbour's words: 50 ms
-
-// The exchange words and the bookkeeping are device memory reached through descriptor
-// pointers: accessed as global (not flat) memory, so that waiting for LDS operations
-// does not wait for exchange loads in flight as well.
-typedef __attribute__((address_space(1))) unsigned long long g_u64;
-typedef __attribute__((address_space(1))) unsigned g_u32;
-__device__ __forceinline__ unsigned long long xload(const unsigned long long *p)
-{
-    return __hip_atomic_load((const g_u64 *)p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-__device__ __forceinline__ void xstore(unsigned long long *p, float v, unsigned tag)
-{
-    __hip_atomic_store((g_u64 *)p, (unsigned long long)__float_as_uint(v) | ((unsigned long long)tag << 32),
-                       __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-__device__ __forceinline__ unsigned xload32(const unsigned *p)
-{
-    return __hip_atomic_load((const g_u32 *)p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-__device__ __forceinline__ void xstore32(unsigned *p, unsigned v)
-{
-    __hip_atomic_store((g_u32 *)p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-__device__ __forceinline__ unsigned xadd32(unsigned *p, unsigned v)
-{
-    return __hip_atomic_fetch_add((g_u32 *)p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-__device__ __forceinline__ unsigned xswap32(unsigned *p, unsigned v)
-{
-    return __hip_atomic_exchange((g_u32 *)p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-__device__ __forceinline__ unsigned long long rtime() { return __builtin_amdgcn_s_memrealtime(); }
 
 template <int SH, bool MASKED, typename T>
 __device__ __forceinline__ void op_iterp(const Tile &t, int roles, const float4 *__restrict__ sched,
@@ -1437,27 +1704,8 @@ __device__ __forceinline__ void op_iterp(const Tile &t, int roles, const float4 
     const int w = t.w, h = t.h, s = t.s, x0 = t.x0, y0 = t.y0;
     const int tid = threadIdx.x;
 
-    // ---- every workgroup of the launch must be resident: count the arrivals
-    if (tid == 0) {
-        const unsigned ep = xload32(&xs->epoch);
-        unsigned ab = xload32(&xs->abort);
-        xadd32(&xs->arrive, 1u);
-        if (!ab) {
-            const unsigned long long t0 = rtime();
-            while (xload32(&xs->arrive) < (unsigned)ntiles) {
-                if (rtime() - t0 > kArriveTicks) {
-                    if (xswap32(&xs->abort, 1u) == 0)
-                        xstore32(&xs->why, ((unsigned)ntiles << 12) | (1u << 8));
-                    ab = 1;
-                    break;
-                }
-                __builtin_amdgcn_s_sleep(2);
-            }
-        }
-        s_word[0] = ep;
-        s_word[1] = ab;
-    }
-    __syncthreads();
+    // ---- every workgroup of the launch must be resident
+    persist_arrive(xs, ntiles, s_word);
     const unsigned epoch = s_word[0];
 
     if (s_word[1] == 0) {
@@ -1667,17 +1915,8 @@ __device__ __forceinline__ void op_iterp(const Tile &t, int roles, const float4 
                 if (tid == NT - 1) xstore(mine + ((j - 1) & 1) * X::WORDS + X::HB, 0.f, tag0 | (unsigned)(j - 1));
                 if (rcv_i == -2) {
                     // a neighbour's progress word
-                    const unsigned tg = tag0 | (unsigned)(j - 1);
-                    const unsigned long long *pw = xbuf + (long)rcv_w + ((j - 1) & 1) * X::WORDS;
-                    unsigned long long t0 = 0;
-                    for (int polls = 0;; polls++) {
-                        if ((unsigned)(xload(pw) >> 32) == tg) break;
-                        if (polls == 0) t0 = rtime();
-                        if ((polls & 63) == 63 && (rtime() - t0 > kPollTicks || xload32(&xs->abort) != 0)) {
-                            bad = 3;
-                            break;
-                        }
-                    }
+                    if (!persist_wait_tag(xbuf + (long)rcv_w + ((j - 1) & 1) * X::WORDS, tag0 | (unsigned)(j - 1), xs))
+                        bad = 3;
                 } else if (rcv_i >= 0) {
                     const unsigned tg = tag0 | (unsigned)(j - 1);
                     const unsigned long long *pw = xbuf + (long)(rcv_w & 0x3fffffff) + ((j - 1) & 1) * X::WORDS;
@@ -1700,10 +1939,7 @@ __device__ __forceinline__ void op_iterp(const Tile &t, int roles, const float4 
                 }
             }
             if (__syncthreads_or(bad)) {
-                // (a work-item that gave up first records why, unless another launch's
-                // workgroup did already)
-                if (bad && xswap32(&xs->abort, 1u) == 0)
-                    xstore32(&xs->why, ((unsigned)ntiles << 12) | ((unsigned)bad << 8) | ((unsigned)j & 255u));
+                persist_give_up(xs, ntiles, bad, j);
                 return false;
             }
             IterParams ip = ip0;
@@ -1834,16 +2070,7 @@ __device__ __forceinline__ void op_iterp(const Tile &t, int roles, const float4 
         else run(std::false_type{});
     }
 
-    // ---- the last workgroup to finish resets the arrivals and advances the launch counter
-    __syncthreads();
-    if (tid == 0) {
-        const unsigned d = xadd32(&xs->done, 1u);
-        if (d == (unsigned)ntiles - 1) {
-            xstore32(&xs->arrive, 0u);
-            xstore32(&xs->done, 0u);
-            xstore32(&xs->epoch, epoch + 1u);
-        }
-    }
+    persist_finish(xs, ntiles, epoch);
 }
 
 template <typename T, int SH>
@@ -2360,6 +2587,55 @@ void iterk_storage(const PairDesc *desc, const TickArgs &args, int launch_shape,
     case 16: iterk_shape<T, 16, K>(desc, args, half, st); break;
     case 17: iterk_shape<T, 17, K>(desc, args, half, st); break;
     default: break;  // the host never asks for another shape
+    }
+}
+
+template <typename T, int SH, int K>
+void iterkp_shape(const PairDesc *desc, const TickArgs &args, hipStream_t st)
+{
+    if constexpr (persistk_shape(SH, K))
+        k_iterkp<T, SH, K><<<dim3((unsigned)args.total), dim3(BlkK<SH, K, false>::NT), 0, st>>>(desc, args);
+}
+
+template <typename T, int SH, int K>
+int iterkp_shape_capacity()
+{
+    if constexpr (persistk_shape(SH, K)) {
+        int per = 0, dev = 0, cus = 0;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_iterkp<T, SH, K>, BlkK<SH, K, false>::NT, 0) !=
+            hipSuccess)
+            return 0;
+        if (hipGetDevice(&dev) != hipSuccess) return 0;
+        if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) return 0;
+        return per * cus;
+    }
+    return 0;
+}
+
+// The blocked persistent op's shapes (persistk_shape): 16x16, 8x8, 4x4, 20x20, 27x25, 14x13.
+#define VFS_PERSISTK_SHAPES(X) X(6) X(7) X(8) X(10) X(12) X(13)
+
+template <typename T, int K>
+void iterkp_storage(const PairDesc *desc, const TickArgs &args, int shape, hipStream_t st)
+{
+    switch (shape) {
+#define VFS_CASE(sh) \
+    case sh: iterkp_shape<T, sh, K>(desc, args, st); break;
+        VFS_PERSISTK_SHAPES(VFS_CASE)
+#undef VFS_CASE
+    default: break;  // the host never asks for another shape
+    }
+}
+
+template <typename T, int K>
+int iterkp_capacity(int shape)
+{
+    switch (shape) {
+#define VFS_CASE(sh) \
+    case sh: return iterkp_shape_capacity<T, sh, K>();
+        VFS_PERSISTK_SHAPES(VFS_CASE)
+#undef VFS_CASE
+    default: return 0;
     }
 }
 

@@ -2,8 +2,9 @@
 // family (the library build passes VFS_INST_STORAGE = 0 float, 1 __half, 2
 // __hip_bfloat16, and VFS_INST_K = 1 for the tick kernel, 2, 4 or 8 for the K-iteration
 // kernel, 0 for the whole-warp persistent kernel), so that the kernels compile in parallel.
-// Defines launch_ops_<storage>_k<K>, which kernels.hip's launch_tick dispatches to (and
-// for K = 0 persist_capacity_<storage>).
+// Defines launch_ops_<storage>_k<K>, which kernels.hip's launch_tick dispatches to (the
+// K-iteration units also hold the blocked persistent op), and for K != 1
+// persist_capacity_<storage>_k<K>.
 #include "tgv_device.h"
 
 #if !defined(VFS_INST_STORAGE) || !defined(VFS_INST_K)
@@ -34,14 +35,17 @@ void VFS_NAME_X(VFS_INST_SFX, VFS_INST_K)(const PairDesc *desc, const TickArgs &
 #elif VFS_INST_K == 0
     iterp_storage<VFS_INST_T>(desc, args, shape, st);
 #else
-    iterk_storage<VFS_INST_T, VFS_INST_K>(desc, args, shape, st);
+    if (args.e[0].kind == OP_ITERPK) iterkp_storage<VFS_INST_T, VFS_INST_K>(desc, args, shape, st);
+    else iterk_storage<VFS_INST_T, VFS_INST_K>(desc, args, shape, st);
 #endif
 }
 
+#define VFS_CAP_NAME(s, k) VFS_CAT3(persist_capacity_, s, _k##k)
+#define VFS_CAP_NAME_X(s, k) VFS_CAP_NAME(s, k)
 #if VFS_INST_K == 0
-#define VFS_CAP_NAME(s) VFS_CAT3(persist_capacity_, s, )
-#define VFS_CAP_NAME_X(s) VFS_CAP_NAME(s)
-int VFS_CAP_NAME_X(VFS_INST_SFX)(int shape) { return iterp_capacity<VFS_INST_T>(shape); }
+int VFS_CAP_NAME_X(VFS_INST_SFX, VFS_INST_K)(int shape) { return iterp_capacity<VFS_INST_T>(shape); }
+#elif VFS_INST_K != 1
+int VFS_CAP_NAME_X(VFS_INST_SFX, VFS_INST_K)(int shape) { return iterkp_capacity<VFS_INST_T, VFS_INST_K>(shape); }
 #endif
 
 }  // namespace vfs

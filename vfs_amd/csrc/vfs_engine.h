@@ -81,6 +81,24 @@ __host__ __device__ constexpr bool persist_shape(int shape)
 // Exchange words of one tile and parity: u_, v_.x, v_.y of columns tw-1, tw-2, rows
 // th-1, th-2, column 0 and row 0, and the tile's progress word (StripP in tgv_device.h).
 __host__ __device__ constexpr int persist_words(int shape) { return 9 * (kShape[shape].tw + kShape[shape].th) + 1; }
+
+// Blocked persistent op (OP_ITERPK): the K-iteration op's workgroups made persistent.  A
+// workgroup stages its tile grown by 2K / K once, runs blocks of K iterations as the
+// K-iteration op does, and between blocks refreshes its ring (everything it staged outside
+// its tile) from the state its neighbours publish: u, u_, v_, p and q of the bands of
+// their tiles that neighbouring rings cover.  Shapes: the tile at least 2K pixels on each
+// side (a ring then reads its 8 neighbours only), at most two staged positions per
+// work-item, and one of the tilings that serve the coarse levels.
+__host__ __device__ constexpr bool persistk_shape(int shape, int K)
+{
+    return (K == 2 || K == 4 || K == 8) && shape >= 0 && shape < kShapes &&
+           (shape == 6 || shape == 7 || shape == 8 || shape == 10 || shape == 12 || shape == 13) &&
+           kShape[shape].tw >= 2 * K && kShape[shape].th >= 2 * K && iterk_region(shape, K) <= 2048;
+}
+// Its exchange words of one tile and parity: the tile's u, u_, v_.x, v_.y, p.x, p.y, q.x ..
+// q.w (value-major, row-major within a value; only the bands are written), then the
+// tile's progress word.
+__host__ __device__ constexpr int persistk_words(int shape) { return 10 * kShape[shape].tw * kShape[shape].th + 1; }
 // Tiles of one persistent launch at most (bounds the exchange buffer).
 constexpr int kMaxPersistTiles = 1024;
 // Launch bookkeeping of the persistent op, one per pair (device memory): the launch
@@ -192,8 +210,8 @@ struct PairDesc {
     float4 sched[kMaxSched];
     EnergyPart *mon_rec;   // kMaxLevels x kMonWarps records (mon_slot)
     EnergyPart *mon_part;  // energy_chunks(w0, h0) partial sums of the running OP_ENERGY
-    // OP_ITERP: exchange words ([tile][parity][persist_words], value | tag << 32) and the
-    // launch bookkeeping
+    // OP_ITERP / OP_ITERPK: exchange words ([tile][parity][persist_words or
+    // persistk_words], value | tag << 32) and the launch bookkeeping
     unsigned long long *xbuf;
     XSync *xsync;
 };
@@ -233,7 +251,8 @@ enum OpKind : int {
     OP_SYNC = 15,   // spare buffers of u, u_, v_ take the current ones beyond the level's extent
     OP_ENERGY = 16, // energy / residual partial sums of the level (kEnergyChunk pixels per workgroup)
     OP_EFINAL = 17, // one workgroup per pair: sums the partials in order into the (level, warp) record
-    OP_ITERP = 18   // an even number of fused TGV iterations (a whole warp), persistent workgroups
+    OP_ITERP = 18,  // an even number of fused TGV iterations (a whole warp), persistent workgroups
+    OP_ITERPK = 19  // blocks of K fused TGV iterations (halo recomputed), persistent workgroups
 };
 
 // Iterations of an inner-iteration op (OP_ITERP: see iterp_count).
@@ -241,9 +260,11 @@ inline int op_iterations(int kind)
 {
     return kind == OP_ITER8 ? 8 : kind == OP_ITER4 ? 4 : kind == OP_ITER2 ? 2 : kind == OP_ITER ? 1 : 0;
 }
-// OP_ITERP entries carry (iterations << 16) | first iteration's index in `it0`.
+// OP_ITERP / OP_ITERPK entries carry (iterations << 16) | (ITERPK: K << 8) | first
+// iteration's index in `it0`.
 __host__ __device__ inline int iterp_count(int it0) { return it0 >> 16; }
-__host__ __device__ inline int iterp_first(int it0) { return it0 & 0xffff; }
+__host__ __device__ inline int iterp_first(int it0) { return it0 & 0xff; }
+__host__ __device__ inline int iterp_block(int it0) { return (it0 >> 8) & 0xff; }
 inline bool op_multi(int kind) { return kind == OP_ITER2 || kind == OP_ITER4 || kind == OP_ITER8; }
 
 // Entries per launch: one per lane of a wave64 (the kernel's entry lookup), and 64
@@ -275,7 +296,8 @@ struct TickEntry {
     unsigned int roles;        // buffer roles before the op (see Group)
     float sigma, tau, mu;      // ITER: step sizes of the iteration
     int it0;                   // ITER2 / ITER4 / ITER8: index of the first iteration in the warp;
-                               // ITERP: (iterations << 16) | that index;
+                               // ITERP / ITERPK: (iterations << 16) | (ITERPK: K << 8) |
+                               // that index;
                                // EFINAL: the warp of the record
 };
 static_assert(sizeof(TickEntry) == 56, "TickEntry layout");
@@ -319,11 +341,12 @@ inline int op_tiles(int kind, int level, const int *w, const int *h, const int *
 
 // Launches one op for the entries of `args` (all of one kind) in tile shape `shape`;
 // hipErrorInvalidValue (nothing launched) when no kernel exists for that kind and shape.
-// OP_ITERP launches hold one entry.
+// OP_ITERP / OP_ITERPK launches hold one entry.
 hipError_t launch_tick(const PairDesc *desc, const TickArgs &args, int storage, int shape, hipStream_t st);
-// Workgroups of the persistent op (tile shape `shape`) that the device holds at once: its
-// compute units times the workgroups one of them holds; 0 without such a kernel.
-int persist_capacity(int storage, int shape);
+// Workgroups of the persistent op (tile shape `shape`; K = 1: OP_ITERP, else OP_ITERPK
+// with blocks of K) that the device holds at once: its compute units times the workgroups
+// one of them holds; 0 without such a kernel.
+int persist_capacity(int storage, int shape, int K = 1);
 
 // ---- single-plane kernels (load time, outputs) ----
 struct Dims {
