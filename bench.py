@@ -77,7 +77,8 @@ def main():
                     help="per-level launch choices from level 0, comma-separated (profiling aid): 'S[p]' one "
                          "iteration per launch in tile shape S (p = Tp prefetch), 'MxK[h]' K = 2, 4 or 8 iterations "
                          "per launch in tile shape M (h = half workgroups), 'MxP' a warp per persistent launch in "
-                         "tile shape M, or 'S[p]/MxK[h]' for both")
+                         "tile shape M, 'MxPK' the same in blocks of K = 2, 4 or 8 iterations, or 'S[p]/MxK[h]' "
+                         "for both")
     ap.add_argument("--no-autotune", action="store_true",
                     help="no launch tuner (fixed defaults; with --tuning for reproducible profiles)")
     ap.add_argument("--iter-only", type=int, default=0,
@@ -173,7 +174,8 @@ def main():
     bytes_launch = B_ITER_FP32 * w0 * h0 * per_launch
     achieved = bytes_launch / t_launch / 1e9
     (tw0, th0), _, k0 = g.level_tuning(0, True, with_per_launch=True)
-    kernel0 = "k_iterp" if k0 == PERSISTENT else "k_iterk" if per_launch > 1 else "k_tick_iter"
+    kernel0 = ("k_iterp" if k0 == PERSISTENT else "k_iterkp" if k0 & PERSISTENT else "k_iterk" if per_launch > 1
+               else "k_tick_iter")
 
     # achievable-bandwidth reference: a plain copy kernel, 1 GiB moved per launch
     copy_ms = g.time_copy(512 << 20, 10) / 10
@@ -206,7 +208,9 @@ def main():
                      "kernel": f"inner TGV iterations at level 0 ({w0}x{h0}): {per_launch} per launch, "
                                f"{tw0}x{th0} tiles" + (" (persistent workgroups: the tiles stay on chip, "
                                                        "borders exchanged every iteration)"
-                                                       if k0 == PERSISTENT else ""),
+                                                       if k0 == PERSISTENT else
+                                                       f" (persistent workgroups, blocks of {k0 & 0xff} iterations)"
+                                                       if k0 & PERSISTENT else ""),
                      "us_per_launch": round(t_launch * 1e6, 3), "us_per_iter": round(t_iter * 1e6, 3),
                      "algorithmic_bytes_per_launch": int(bytes_launch),
                      "copy_kernel_GBs": round(copy_gbs, 1)},
@@ -230,7 +234,8 @@ def main():
         out["tile_tuning"][f"L{lvl} {lw}x{lh}"] = (
             f"{tw}x{th}" + (" +Tp prefetch" if pre else "")
             + (", a warp per launch (persistent workgroups)" if k == PERSISTENT
-               else f", {k} iterations per launch" if k > 1 else ""))
+               else f", a warp per launch in blocks of {k & 0xff} iterations (persistent workgroups)"
+               if k & PERSISTENT else f", {k} iterations per launch" if k > 1 else ""))
     print(f"bench: tile tuning {out['tile_tuning']}", file=sys.stderr, flush=True)
 
     legs = set() if args.quick else ({"served"} if args.serve_only else {l for l in args.legs.split(",") if l})
@@ -257,8 +262,8 @@ def apply_tuning(obj, spec):
         for part in lspec.split("/"):
             if "x" in part:
                 m, k = part.split("x")
-                if k == "P":
-                    obj.set_level_tuning(lvl, int(m), per_launch=PERSISTENT)
+                if k.startswith("P"):
+                    obj.set_level_tuning(lvl, int(m), per_launch=PERSISTENT | int(k[1:] or 0))
                 else:
                     obj.set_level_tuning(lvl, int(m), per_launch=int(k.rstrip("h")), half_workgroup=k.endswith("h"))
             else:

@@ -13,7 +13,7 @@ from test_gpu_parity import make_gpu, small_pair
 
 pytestmark = pytest.mark.gpu
 
-PLANES = ("u", "u_", "v_", "warpUV")
+PLANES = ("u", "u_", "v_", "warpUV", "Iu", "Iz")
 PERSIST_SHAPES = (7, 8, 11, 12, 13, 14, 17, 18)
 
 

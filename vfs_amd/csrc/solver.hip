@@ -464,6 +464,23 @@ void apply_tuning(Program &pr, const Tuning &tn)
         if (multi && (rest ? lt.m2half : lt.mhalf) && vfs::half_workgroup_ok(o.shape, vfs::op_iterations(o.kind)))
             o.shape = (unsigned char)(o.shape | vfs::kHalfWorkgroup);
     }
+    // A warp's first inner op computes the warp's derivatives at its staged positions
+    // (EF_DERIV) in place of the OP_DERIV op before it, where that op stages a halo (a
+    // multi-iteration or persistent op) with at most two positions per work-item, and the
+    // OP_DERIV op does not also sync the spare buffers (EF_SYNC).
+    for (size_t i = 0; i + 1 < pr.ops.size(); i++) {
+        const ProgOp &d = pr.ops[i];
+        ProgOp &f = pr.ops[i + 1];
+        if (d.kind != vfs::OP_DERIV || (d.flags & vfs::EF_SYNC)) continue;
+        int K = 0;
+        if (vfs::op_multi(f.kind)) K = vfs::op_iterations(f.kind);
+        else if (f.kind == vfs::OP_ITERP) K = 1;
+        else if (f.kind == vfs::OP_ITERPK) K = vfs::iterp_block(f.it0);
+        if (K == 0 || vfs::staged_per_item(vfs::shape_index(f.shape), K, (f.shape & vfs::kHalfWorkgroup) != 0) > 2)
+            continue;
+        f.flags |= vfs::EF_DERIV;
+        pr.ops.erase(pr.ops.begin() + (long)i);
+    }
 }
 
 // The solve program.  Cold (warm_level < 0): stereotgv.cu:389-515 (masked) or :256-387,

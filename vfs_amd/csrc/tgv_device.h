@@ -481,8 +481,30 @@ __device__ __forceinline__ void op_prep(const Tile &t, int level, int roles)
     }
 }
 
-// computeDerivatives.cu:88-129: 5-point derivative along t = tv/|tv| of I0 and I1w,
-// averaged; Iz = I1w - I0 at the texel centre.
+// computeDerivatives.cu:88-129 at pixel (x, y): Iu, the 5-point derivative along
+// t = tv/|tv| of I0 and I1w, averaged, and Iz = I1w - I0 at the texel centre.
+template <typename T>
+__device__ __forceinline__ float2 deriv_px(const T *__restrict__ I0, const T *__restrict__ I1w,
+                                           const float2 *__restrict__ tv, int w, int h, int s, int x, int y)
+{
+    const long pos = (long)y * s + x;
+    const float2 v = tv[pos];
+    const float r = sqrt_rn(v.x * v.x + v.y * v.y);
+    const float dx = v.x / r, dy = v.y / r;
+    const float X = (float)x, Y = (float)y;
+    float t0 = bil(I0, w, h, s, X - 2.f * dx, Y - 2.f * dy);
+    t0 -= bil(I0, w, h, s, X - dx, Y - dy) * 8.f;
+    t0 += bil(I0, w, h, s, X + dx, Y + dy) * 8.f;
+    t0 -= bil(I0, w, h, s, X + 2.f * dx, Y + 2.f * dy);
+    t0 /= 12.f;
+    float t1 = bil(I1w, w, h, s, X - 2.f * dx, Y - 2.f * dy);
+    t1 -= bil(I1w, w, h, s, X - dx, Y - dy) * 8.f;
+    t1 += bil(I1w, w, h, s, X + dx, Y + dy) * 8.f;
+    t1 -= bil(I1w, w, h, s, X + 2.f * dx, Y + 2.f * dy);
+    t1 /= 12.f;
+    return make_float2((t0 + t1) * 0.5f, ldT(I1w, pos) - ldT(I0, pos));
+}
+
 template <class S, typename T>
 __device__ __forceinline__ void op_deriv(const Tile &t, int level)
 {
@@ -494,24 +516,27 @@ __device__ __forceinline__ void op_deriv(const Tile &t, int level)
     FOR_TILE_PIXELS(S, t, x, y)
     {
         const long pos = (long)y * t.s + x;
-        const float2 v = tv[pos];
-        const float r = sqrt_rn(v.x * v.x + v.y * v.y);
-        const float dx = v.x / r, dy = v.y / r;
-        const float X = (float)x, Y = (float)y;
-        const int w = t.w, h = t.h, s = t.s;
-        float t0 = bil(I0, w, h, s, X - 2.f * dx, Y - 2.f * dy);
-        t0 -= bil(I0, w, h, s, X - dx, Y - dy) * 8.f;
-        t0 += bil(I0, w, h, s, X + dx, Y + dy) * 8.f;
-        t0 -= bil(I0, w, h, s, X + 2.f * dx, Y + 2.f * dy);
-        t0 /= 12.f;
-        float t1 = bil(I1w, w, h, s, X - 2.f * dx, Y - 2.f * dy);
-        t1 -= bil(I1w, w, h, s, X - dx, Y - dy) * 8.f;
-        t1 += bil(I1w, w, h, s, X + dx, Y + dy) * 8.f;
-        t1 -= bil(I1w, w, h, s, X + 2.f * dx, Y + 2.f * dy);
-        t1 /= 12.f;
-        stT(Iu, pos, (t0 + t1) * 0.5f);
-        stT(Iz, pos, ldT(I1w, pos) - ldT(I0, pos));
+        const float2 d = deriv_px(I0, I1w, tv, t.w, t.h, t.s, x, y);
+        stT(Iu, pos, d.x);
+        stT(Iz, pos, d.y);
     }
+}
+
+// The warp's derivatives at staged position (x, y) (clamped into the level) for an
+// inner-iteration op that carries EF_DERIV: computed as op_deriv does, rounded to the
+// storage type as the planes hold them, and stored when the position is in the tile.
+template <typename T>
+__device__ __forceinline__ float2 staged_deriv(const Tile &t, int level, int x, int y, bool in_tile)
+{
+    const float2 d = deriv_px((const T *)t.d->I0[level], (const T *)slot(t, PL_I1W), t.d->tv[level], t.w, t.h,
+                              t.s, x, y);
+    const float iu = Cv<T>::f(Cv<T>::t(d.x)), iz = Cv<T>::f(Cv<T>::t(d.y));
+    if (in_tile) {
+        const long pos = (long)y * t.s + x;
+        stT((T *)slot(t, PL_IU), pos, iu);
+        stT((T *)slot(t, PL_IZ), pos, iz);
+    }
+    return make_float2(iu, iz);
 }
 
 // The inner iteration divides and normalises several times per pixel; it multiplies by
@@ -1162,6 +1187,21 @@ __device__ __forceinline__ void op_iterk(const Tile &t, int roles, const float4 
     // The duals of the `other` buffers are never read (an op reads the current ones and
     // each warp starts from zero): the op stores p and q after its last iteration only.
 
+    // ---- the warp's derivatives (EF_DERIV) at the staged positions, first and on their
+    // own (their registers are free again before the staging), into this work-item's own
+    // Tp slots in LDS until the staging reads them
+    if (ip0.deriv_level >= 0) {
+#pragma unroll
+        for (int k = 0; k < NK; k++) {
+            const int i = tid + k * NT;
+            if (i >= B::NR) continue;
+            const int x = x0 + B::RO + i % B::RW, y = y0 + B::RO + i / B::RW;
+            const int xc = min(max(x, 0), w - 1), yc = min(max(y, 0), h - 1);
+            const bool own = xc == x && yc == y && x >= x0 && x < x0 + B::TW && y >= y0 && y < y0 + B::TH;
+            L.tp[i] = staged_deriv<T>(t, ip0.deriv_level, xc, yc, own);
+        }
+    }
+
     // ---- stage: own values into registers, shared ones into LDS (clamped positions
     // outside the image); every load is issued before the first LDS store
     OwnK o[NK];
@@ -1184,8 +1224,14 @@ __device__ __forceinline__ void op_iterk(const Tile &t, int roles, const float4 
             o[k].a = ldT(at, pos);
             o[k].b = ldT(bt, pos);
             o[k].c = ldT(ct, pos);
-            o[k].iu = ldT(Iu, pos);
-            o[k].iz = ldT(Iz, pos);
+            if (ip0.deriv_level >= 0) {
+                const float2 d = L.tp[i];  // computed above
+                o[k].iu = d.x;
+                o[k].iz = d.y;
+            } else {
+                o[k].iu = ldT(Iu, pos);
+                o[k].iz = ldT(Iz, pos);
+            }
             if (ip0.first) {
                 o[k].p = make_float2(0.f, 0.f);
                 oq[k] = make_float4(0.f, 0.f, 0.f, 0.f);
@@ -1586,6 +1632,7 @@ void k_iterk(const PairDesc *__restrict__ D, const TickArgs A)
     ip0.first = (E.flags & EF_FIRST) != 0;
     ip0.write_tp = (E.flags & EF_WRITE_TP_LAST) != 0;  // the op's last iteration
     ip0.preload_tp = 0;
+    ip0.deriv_level = (E.flags & EF_DERIV) ? (int)E.level : -1;
     if (E.masked) op_iterk<SH, K, HALF, true, T>(t, E.roles, sched, ip0, L);
     else op_iterk<SH, K, HALF, false, T>(t, E.roles, sched, ip0, L);
 }
@@ -1621,6 +1668,7 @@ __global__ __launch_bounds__((BlkK<SH, K, false>::NT)) void k_iterkp(const PairD
     ip0.first = (E.flags & EF_FIRST) != 0;
     ip0.write_tp = (E.flags & EF_WRITE_TP_LAST) != 0;  // the op's last iteration
     ip0.preload_tp = 0;
+    ip0.deriv_level = (E.flags & EF_DERIV) ? (int)E.level : -1;
     PersistK pk;
     pk.blocks = iterp_count(E.it0) / K;
     pk.tile = b;
@@ -1724,6 +1772,19 @@ __device__ __forceinline__ void op_iterp(const Tile &t, int roles, const float4 
         const T *__restrict__ Iu = (const T *)slot(t, PL_IU);
         const T *__restrict__ Iz = (const T *)slot(t, PL_IZ);
 
+        // ---- the warp's derivatives (EF_DERIV), as the K-iteration op computes them
+        if (ip0.deriv_level >= 0) {
+#pragma unroll
+            for (int k = 0; k < NK; k++) {
+                const int i = tid + k * NT;
+                if (i >= B::NR) continue;
+                const int x = x0 + B::RO + i % B::RW, y = y0 + B::RO + i / B::RW;
+                const int xc = min(max(x, 0), w - 1), yc = min(max(y, 0), h - 1);
+                const bool own = xc == x && yc == y && x >= x0 && x < x0 + TW && y >= y0 && y < y0 + TH;
+                L.tp[i] = staged_deriv<T>(t, ip0.deriv_level, xc, yc, own);
+            }
+        }
+
         // ---- stage (as the K-iteration op): own values into registers, shared ones into
         // LDS.  Outside the mask, q is never read, and its LDS slot holds the other buffers'
         // u_ and v_ instead (the values read there at odd iterations).
@@ -1747,8 +1808,14 @@ __device__ __forceinline__ void op_iterp(const Tile &t, int roles, const float4 
                 o[k].a = ldT(at, pos);
                 o[k].b = ldT(bt, pos);
                 o[k].c = ldT(ct, pos);
-                o[k].iu = ldT(Iu, pos);
-                o[k].iz = ldT(Iz, pos);
+                if (ip0.deriv_level >= 0) {
+                    const float2 d = L.tp[i];  // computed above
+                    o[k].iu = d.x;
+                    o[k].iz = d.y;
+                } else {
+                    o[k].iu = ldT(Iu, pos);
+                    o[k].iz = ldT(Iz, pos);
+                }
                 if (ip0.first) {
                     o[k].p = make_float2(0.f, 0.f);
                     oq[k] = make_float4(0.f, 0.f, 0.f, 0.f);
@@ -2103,6 +2170,7 @@ __global__ __launch_bounds__(BlkP<SH>::NT) void k_iterp(const PairDesc *__restri
     ip0.first = (E.flags & EF_FIRST) != 0;
     ip0.write_tp = (E.flags & EF_WRITE_TP_LAST) != 0;
     ip0.preload_tp = 0;
+    ip0.deriv_level = (E.flags & EF_DERIV) ? (int)E.level : -1;
     XSync *xs = t.d->xsync;
     unsigned long long *xbuf = t.d->xbuf;
     if (E.masked) op_iterp<SH, true, T>(t, E.roles, sched, ip0, iterp_count(E.it0), b, nb, xs, xbuf, L, s_word);
@@ -2495,6 +2563,7 @@ __global__ __launch_bounds__(NT) void k_tick(const PairDesc *__restrict__ D, con
         ip.first = E.flags & EF_FIRST;
         ip.write_tp = (E.flags & EF_WRITE_TP) != 0;
         ip.preload_tp = (E.flags & EF_PRELOAD_TP) != 0;
+        ip.deriv_level = -1;
         if (E.masked) op_iter<S, true, T>(t, E.roles, ip, sP, sQ, sT);
         else op_iter<S, false, T>(t, E.roles, ip, sP, sQ, sT);
         break;

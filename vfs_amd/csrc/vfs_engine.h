@@ -57,6 +57,13 @@ __host__ __device__ constexpr bool half_workgroup_ok(int shape, int K)
     return iterk_region(shape, K) > 512 && iterk_region(shape, K) <= 1536;
 }
 __host__ __device__ constexpr int shape_index(int launch_shape) { return launch_shape & (kHalfWorkgroup - 1); }
+// Staged positions per work-item of a K-iteration launch in `shape` (K = 1: the whole-warp
+// persistent op, whose staged region is the tile grown by 2 / 1 as well).
+__host__ __device__ constexpr int staged_per_item(int shape, int K, bool half)
+{
+    const int nr = iterk_region(shape, K), nt = half ? 512 : nr >= 1024 ? 1024 : (nr + 63) / 64 * 64;
+    return (nr + nt - 1) / nt;
+}
 static_assert(kShapes <= kHalfWorkgroup, "shape indices fit below the half-workgroup flag");
 // Tile shapes of K-iteration launches: every multi-iteration shape for K = 2 and 4; for
 // K = 8 those whose staged region holds at most 3072 positions (three per work-item).
@@ -228,6 +235,8 @@ struct IterParams {
     int first;       // p and q were just zeroed (stereotgv.cu:431-432): read them as 0
     int write_tp;    // also store Tp (read later only where a neighbour is outside the mask)
     int preload_tp;  // fetch the Tp-plane neighbours up front (latency-bound levels)
+    int deriv_level;  // >= 0: the warp's derivatives Iu, Iz at that level are computed
+                      // here (EF_DERIV) and stored, instead of loaded
 };
 
 // Ops of a solve program.  A program is the op sequence of one solve (built on the
@@ -280,7 +289,13 @@ enum EntryFlag : int {
     EF_EFINAL = 16,        // END / ENDPREP: one extra workgroup adds the level's energy partials
                            // into record (level, it0), the work of OP_EFINAL
     EF_SYNC = 32,          // DERIV: extra workgroups do OP_SYNC's copy beyond the level's extent
-    EF_PREP = 64           // TENSOR: the same workgroups also run OP_PREP at each pixel
+    EF_PREP = 64,          // TENSOR: the same workgroups also run OP_PREP at each pixel
+    EF_DERIV = 128         // ITER2 / ITER4 / ITER8 / ITERP / ITERPK, a warp's first op: the
+                           // derivatives (OP_DERIV's work) computed at every staged position
+                           // and stored for the tile, in place of an OP_DERIV op (the host
+                           // folds it where the kernel stages at most two positions per
+                           // work-item: with three, the derivatives' dependent loads take
+                           // longer than the OP_DERIV launch)
 };
 
 // One (pair, op) of a launch.  The geometry and arena of the op's tile level travel in

@@ -6,13 +6,17 @@ program's op list (the same sequence the library issues: zero, calibration warp,
 pyramid, then per level the tensor (with the first warp's preparation) and, per warp,
 derivatives / inner iterations / end, each end but the level's last fused with the next
 warp's prep, the energy partials (ENERGY) before the level's last end (whose extra
-workgroup adds them up), then the upscale)
+workgroup adds them up), then the upscale; a warp whose first inner op is a multi-iteration
+or persistent launch that stages at most two positions per work-item computes its
+derivatives there and has no DERIV op: `--fold` lists the levels where that holds)
 and attributes each dispatch of one solve to its op and level.
 Levels that run several inner iterations per launch (`--per-launch`, one count per level
 from level 0, as the tuner reports them) issue `k_iterk` dispatches: ITER8 / ITER4 / ITER2 while
 they fit the iteration count, the rest one at a time; `P` marks a level whose warps run
 their iterations in persistent `k_iterp` launches (ITERP: an even count, at most 128, each;
-an odd remainder one at a time).
+an odd remainder one at a time), `P2` / `P4` / `P8` one in blocked persistent `k_iterkp`
+launches (ITERPK: a multiple of the block, at most 128, the rest as `--per-launch` 8 would
+split it, or as `:n` after it says, e.g. `P4:1`).
 
     python -m vfs_amd.profile_ops TRACE.csv --levels 5 --warps 10 --iters 50 --per-launch 4,4,4,4,4 --skip-tail 100
     python -m vfs_amd.profile_ops --batch-traffic FETCH.csv WRITE.csv --storage f16   # HBM bytes per px-iteration
@@ -29,8 +33,17 @@ import csv
 
 def inner_ops(iters: int, per_launch) -> list[str]:
     """Inner-iteration ops of one warp (the library's greedy split; per_launch "P": the
-    persistent op)."""
+    persistent op, "P2" / "P4" / "P8" the blocked persistent op)."""
     out, it = [], 0
+    if isinstance(per_launch, str) and per_launch[:2] in ("P2", "P4", "P8"):
+        # "P4:1": the level's own iterations per launch (for the remainder) is 1
+        b = int(per_launch[1])
+        rest = int(per_launch.split(":")[1]) if ":" in per_launch else 8
+        k = min(iters, 128) // b * b
+        if k >= b:
+            out.append("ITERPK")
+            it = k
+        return out + inner_ops(iters - it, rest)
     if per_launch == "P":
         while iters - it >= 2:
             k = min(iters - it, 128 - it) & ~1
@@ -46,16 +59,18 @@ def inner_ops(iters: int, per_launch) -> list[str]:
     return out
 
 
-def program_ops(levels: int, warps: int, iters: int, per_launch=(), energy: int = 1) -> list[tuple[str, int]]:
+def program_ops(levels: int, warps: int, iters: int, per_launch=(), energy: int = 1, fold=None) -> list[tuple[str, int]]:
     """Ops of the cold program; `energy` as the solver option (0 none, 1 after each level's
-    last warp, 2 after every warp)."""
+    last warp, 2 after every warp); `fold`: the levels whose warps' first inner op computes
+    the derivatives (default: every level whose first inner op is not a one-iteration op)."""
     ops = [("ZERO", 0), ("CALIB", 0)] + [("DOWN", l) for l in range(1, levels)] + [("CINIT", levels - 1)]
     for l in range(levels - 1, -1, -1):
         ops.append(("TENSOR", l))
         k = per_launch[l] if l < len(per_launch) else 1
         inner = [(name, l) for name in inner_ops(iters, k)]
         for wi in range(warps):
-            ops += [("DERIV", l)] + inner
+            folded = bool(inner) and inner[0][0] != "ITER" and (fold is None or l in fold)
+            ops += ([] if folded else [("DERIV", l)]) + inner
             if energy == 2 or (energy == 1 and wi + 1 == warps):
                 ops.append(("ENERGY", l))
             ops.append(("ENDPREP" if wi + 1 < warps else "END", l))
@@ -65,19 +80,21 @@ def program_ops(levels: int, warps: int, iters: int, per_launch=(), energy: int 
 
 
 def summarize(trace_csv: str, levels: int, warps: int, iters: int, skip_tail: int = 0, per_launch=(),
-              energy: int = 1) -> str:
+              energy: int = 1, fold=None) -> str:
     rows = [r for r in csv.DictReader(open(trace_csv))
-            if r["Kernel_Name"].startswith(("k_tick", "k_iterk", "k_iterp"))]
+            if r["Kernel_Name"].startswith(("k_tick", "k_iterk", "k_iterp"))]  # k_iterk: also k_iterkp
     rows.sort(key=lambda r: int(r["Start_Timestamp"]))
-    seq = program_ops(levels, warps, iters, per_launch, energy)
+    seq = program_ops(levels, warps, iters, per_launch, energy, fold)
     n = len(seq)
     end = len(rows) - skip_tail
     if end < n:
         raise ValueError(f"trace holds {end} tick dispatches, one solve needs {n}")
     solve = rows[end - n:end]
     for (kind, lvl), r in zip(seq, solve):
-        if ((kind in ("ITER2", "ITER4", "ITER8")) != r["Kernel_Name"].startswith("k_iterk")
-                or (kind == "ITERP") != r["Kernel_Name"].startswith("k_iterp")):
+        name = r["Kernel_Name"]
+        if ((kind in ("ITER2", "ITER4", "ITER8")) != (name.startswith("k_iterk") and not name.startswith("k_iterkp"))
+                or (kind == "ITERP") != name.startswith("k_iterp")
+                or (kind == "ITERPK") != name.startswith("k_iterkp")):
             raise ValueError(f"dispatch {r['Kernel_Name']} does not match op {kind} at level {lvl}: "
                              "check --per-launch")
     agg = collections.OrderedDict()
@@ -278,11 +295,16 @@ def main():
     ap.add_argument("--iters", type=int, default=50)
     ap.add_argument("--skip-tail", type=int, default=0)
     ap.add_argument("--per-launch", default="",
-                    help="inner iterations per launch of each level, from level 0 (P: persistent launches)")
+                    help="inner iterations per launch of each level, from level 0 (P: persistent launches, "
+                         "P2 / P4 / P8: blocked persistent launches)")
     ap.add_argument("--energy", type=int, default=1, help="the solver's energy option (0, 1, 2)")
+    ap.add_argument("--fold", default=None,
+                    help="levels whose warps compute the derivatives in their first inner launch, e.g. 1,2,3,4 "
+                         "(default: every level whose first inner launch runs several iterations)")
     a = ap.parse_args()
-    per_launch = [x if x == "P" else int(x) for x in a.per_launch.split(",") if x]
-    print(summarize(a.trace, a.levels, a.warps, a.iters, a.skip_tail, per_launch, a.energy))
+    per_launch = [x if x.startswith("P") else int(x) for x in a.per_launch.split(",") if x]
+    fold = None if a.fold is None else {int(x) for x in a.fold.split(",") if x}
+    print(summarize(a.trace, a.levels, a.warps, a.iters, a.skip_tail, per_launch, a.energy, fold))
 
 
 if __name__ == "__main__":
