@@ -1772,13 +1772,39 @@ __device__ __forceinline__ void op_iterp(const Tile &t, int roles, const float4 
         const T *__restrict__ Iu = (const T *)slot(t, PL_IU);
         const T *__restrict__ Iz = (const T *)slot(t, PL_IZ);
 
+        // ---- the positions of this work-item: position q = tid + k NT in persist_rank's
+        // order (the tile's border at k = 0, so that the work-items that compute it in the
+        // primal step publish it), through a table in the LDS of q (staged over afterwards);
+        // pi[k] = LDS index | column << 12 | row << 20 in the staged region
+        static_assert(persist_border(SH) <= NT && B::NR <= 4096 && B::RW < 256 && X::WORDS < 2047, "positions");
+        int pi[NK];
+        {
+            int *tab = reinterpret_cast<int *>(L.q);
+#pragma unroll
+            for (int k = 0; k < NK; k++) {
+                const int i = tid + k * NT;
+                if (i < B::NR) tab[persist_rank(SH, i)] = i;
+            }
+            __syncthreads();
+#pragma unroll
+            for (int k = 0; k < NK; k++) {
+                const int q = tid + k * NT;
+                const int i = q < B::NR ? tab[q] : 0;
+                pi[k] = i | (i % B::RW) << 12 | (i / B::RW) << 20;
+            }
+            __syncthreads();
+        }
+        auto staged = [&](int k) { return tid + k * NT < B::NR; };
+        auto pos_i = [&](int k) { return pi[k] & 0xfff; };
+        auto pos_x = [&](int k) { return x0 + B::RO + ((pi[k] >> 12) & 0xff); };
+        auto pos_y = [&](int k) { return y0 + B::RO + (pi[k] >> 20); };
+
         // ---- the warp's derivatives (EF_DERIV), as the K-iteration op computes them
         if (ip0.deriv_level >= 0) {
 #pragma unroll
             for (int k = 0; k < NK; k++) {
-                const int i = tid + k * NT;
-                if (i >= B::NR) continue;
-                const int x = x0 + B::RO + i % B::RW, y = y0 + B::RO + i / B::RW;
+                if (!staged(k)) continue;
+                const int i = pos_i(k), x = pos_x(k), y = pos_y(k);
                 const int xc = min(max(x, 0), w - 1), yc = min(max(y, 0), h - 1);
                 const bool own = xc == x && yc == y && x >= x0 && x < x0 + TW && y >= y0 && y < y0 + TH;
                 L.tp[i] = staged_deriv<T>(t, ip0.deriv_level, xc, yc, own);
@@ -1787,18 +1813,16 @@ __device__ __forceinline__ void op_iterp(const Tile &t, int roles, const float4 
 
         // ---- stage (as the K-iteration op): own values into registers, shared ones into
         // LDS.  Outside the mask, q is never read, and its LDS slot holds the other buffers'
-        // u_ and v_ instead (the values read there at odd iterations).
+        // u_ and v_ instead (the values read there at odd iterations; the two buffers'
+        // values trade places between the slots every iteration).
         OwnK o[NK];
-        int pxy[NK];  // (x + 16) | (y + 16) << 16
         float omk[NK];
         float2 otp[NK];
         float4 oq[NK];
 #pragma unroll
         for (int k = 0; k < NK; k++) {
-            const int i = tid + k * NT;
-            const int x = x0 + B::RO + i % B::RW, y = y0 + B::RO + i / B::RW;
-            pxy[k] = (x + 16) | ((y + 16) << 16);
-            if (i < B::NR) {
+            const int i = pos_i(k), x = pos_x(k), y = pos_y(k);
+            if (staged(k)) {
                 const int xc = min(max(x, 0), w - 1), yc = min(max(y, 0), h - 1);
                 const long pos = (long)yc * s + xc;
                 omk[k] = MASKED ? mask[pos] : 1.f;
@@ -1827,10 +1851,9 @@ __device__ __forceinline__ void op_iterp(const Tile &t, int roles, const float4 
         }
 #pragma unroll
         for (int k = 0; k < NK; k++) {
-            const int i = tid + k * NT;
             otp[k] = make_float2(0.f, 0.f);
-            if (MASKED && i < B::NR && omk[k] == 0.f) {
-                const int x = (pxy[k] & 0xffff) - 16, y = (pxy[k] >> 16) - 16;
+            if (MASKED && staged(k) && omk[k] == 0.f) {
+                const int x = pos_x(k), y = pos_y(k);
                 const int xc = min(max(x, 0), w - 1), yc = min(max(y, 0), h - 1);
                 const long pos = (long)yc * s + xc;
                 const float2 vo = v_oth[pos];
@@ -1840,8 +1863,8 @@ __device__ __forceinline__ void op_iterp(const Tile &t, int roles, const float4 
         }
 #pragma unroll
         for (int k = 0; k < NK; k++) {
-            const int i = tid + k * NT;
-            if (i < B::NR) {
+            const int i = pos_i(k);
+            if (staged(k)) {
                 L.ub[i] = o[k].ub;
                 L.v[i] = o[k].v;
                 L.q[i] = oq[k];
@@ -1865,9 +1888,8 @@ __device__ __forceinline__ void op_iterp(const Tile &t, int roles, const float4 
         bool inner = true;
 #pragma unroll
         for (int k = 0; k < NK; k++) {
-            const int i = tid + k * NT;
-            const int x = (pxy[k] & 0xffff) - 16, y = (pxy[k] >> 16) - 16;
-            if (i < B::NR)
+            const int x = pos_x(k), y = pos_y(k);
+            if (staged(k))
                 inner = inner && x >= 0 && y >= 0 && x < w && y < h && omk[k] != 0.f &&
                         (MASKED || !outside_circle(x, y, w, h));
         }
@@ -1884,11 +1906,10 @@ __device__ __forceinline__ void op_iterp(const Tile &t, int roles, const float4 
         unsigned fl[NK];
 #pragma unroll
         for (int k = 0; k < NK; k++) {
-            const int i = tid + k * NT;
-            const int x = (pxy[k] & 0xffff) - 16, y = (pxy[k] >> 16) - 16;
+            const int i = pos_i(k), x = pos_x(k), y = pos_y(k);
             const int dx = x - x0, dy = y - y0;
             fl[k] = 0;
-            if (i >= B::NR) continue;
+            if (!staged(k)) continue;
             if (dx >= -1 && dy >= -1 && dx <= TW - 1 && dy <= TH - 1) fl[k] |= F_DUAL;
             if (dx >= 0 && dy >= 0 && dx <= TW - 1 && dy <= TH - 1) fl[k] |= F_TILE;
             if (!full && x >= 0 && y >= 0 && x < w && y < h) {
@@ -1901,28 +1922,24 @@ __device__ __forceinline__ void op_iterp(const Tile &t, int roles, const float4 
                 if (!MASKED && outside_circle(x, y, w, h)) fl[k] |= F_OUT;
             }
         }
-        // Exchange slots, once per launch: work-item `tid` publishes strip position tid of
-        // the tile (the strips in order A..F: 3 (tw + th) positions) and refreshes ring
-        // position tid (those some dual reads, 3 (tw + th) + 5: columns -2, -1 and tw, rows
-        // -2, -1 and th of the staged region), where that position lies in the image and is
-        // processed; the next 8 work-items wait for the 8 neighbours' progress words.
-        // pub_s: LDS index | strip word << 12; rcv_i: LDS index (-2: a progress word);
-        // rcv_w: the source tile's word (parity 0) | 1 << 30 for a column strip (value
-        // stride th; rows: tw).
+        // Exchange slots, once per launch.  The primal step that computes a position of the
+        // tile's border (this work-item's k = 0 position when tid < persist_border) publishes
+        // its u_ and v_ into every strip holding it, where it lies in the image and is
+        // processed: pub_s = column strip word + 1 | row strip word + 1 << 11 (0: none).
+        // Work-item `tid` refreshes ring position tid (those some dual reads, 3 (tw + th) + 5:
+        // columns -2, -1 and tw, rows -2, -1 and th of the staged region), where that position
+        // lies in the image and is processed; the next 8 work-items wait for the 8 neighbours'
+        // progress words.  rcv_i: LDS index (-2: a progress word); rcv_w: the source tile's
+        // word (parity 0) | 1 << 30 for a column strip (value stride th; rows: tw).
         constexpr int RS = 3 * (TW + TH) + 5;
         static_assert(B::NR <= 4096 && X::WORDS < (1 << 20) && RS + 8 <= NT, "exchange slots");
         const int ntx = (w + TW - 1) / TW, nty = (h + TH - 1) / TH;
-        int pub_s = -1, rcv_i = -1, rcv_w = 0;
-        if (tid < 3 * (TW + TH)) {
-            int lx, ly, off;
-            if (tid < TH) lx = TW - 1, ly = tid, off = X::A + tid;
-            else if (tid < 2 * TH) lx = TW - 2, ly = tid - TH, off = X::B + ly;
-            else if (tid < 2 * TH + TW) lx = tid - 2 * TH, ly = TH - 1, off = X::C + lx;
-            else if (tid < 2 * TH + 2 * TW) lx = tid - 2 * TH - TW, ly = TH - 2, off = X::D + lx;
-            else if (tid < 3 * TH + 2 * TW) lx = 0, ly = tid - 2 * TH - 2 * TW, off = X::E + ly;
-            else lx = tid - 3 * TH - 2 * TW, ly = 0, off = X::F + lx;
-            const int i = (ly - B::RO) * B::RW + (lx - B::RO);
-            if (x0 + lx < w && y0 + ly < h && act(i)) pub_s = i | (off << 12);
+        int pub_s = 0, rcv_i = -1, rcv_w = 0;
+        if (tid < persist_border(SH) && pos_x(0) < w && pos_y(0) < h && act(pos_i(0))) {
+            const int dx = pos_x(0) - x0, dy = pos_y(0) - y0;
+            const int co = dx == TW - 1 ? X::A + dy : dx == TW - 2 ? X::B + dy : dx == 0 ? X::E + dy : -1;
+            const int ro = dy == TH - 1 ? X::C + dx : dy == TH - 2 ? X::D + dx : dy == 0 ? X::F + dx : -1;
+            pub_s = (co + 1) | ((ro + 1) << 11);
         }
         if (tid < 3 * (TW + TH) + 5) {
             int dx, dy;
@@ -1966,20 +1983,8 @@ __device__ __forceinline__ void op_iterp(const Tile &t, int roles, const float4 
             constexpr int MODE = decltype(mode_c)::value;
             int bad = 0;
             if (j > 0) {
-                // ---- publish the tile's strips after iteration j - 1 (their values are in
-                // LDS since the barrier that ended it), then refresh the ring from the
-                // neighbours' strips
-                if (pub_s >= 0) {
-                    const int i = pub_s & 4095, off = pub_s >> 12;
-                    const int st = (off < X::C || (off >= X::E && off < X::F)) ? TH : TW;
-                    const unsigned tg = tag0 | (unsigned)(j - 1);
-                    unsigned long long *pw = mine + ((j - 1) & 1) * X::WORDS + off;
-                    const float2 vv = L.v[i];
-                    xstore(pw, L.ub[i], tg);
-                    xstore(pw + st, vv.x, tg);
-                    xstore(pw + 2 * st, vv.y, tg);
-                }
-                if (tid == NT - 1) xstore(mine + ((j - 1) & 1) * X::WORDS + X::HB, 0.f, tag0 | (unsigned)(j - 1));
+                // ---- refresh the ring from the neighbours' strips of iteration j - 1 (their
+                // primal steps published them)
                 if (rcv_i == -2) {
                     // a neighbour's progress word
                     if (!persist_wait_tag(xbuf + (long)rcv_w + ((j - 1) & 1) * X::WORDS, tag0 | (unsigned)(j - 1), xs))
@@ -2009,6 +2014,9 @@ __device__ __forceinline__ void op_iterp(const Tile &t, int roles, const float4 
                 persist_give_up(xs, ntiles, bad, j);
                 return false;
             }
+            // progress: this tile has read its neighbours' words of iteration j - 1, which
+            // their primal steps of iteration j + 1 overwrite
+            if (MODE != 2 && tid == NT - 1) xstore(mine + (j & 1) * X::WORDS + X::HB, 0.f, tag0 | (unsigned)j);
             IterParams ip = ip0;
             const float4 sc = sched[j];
             ip.k1 = sc.w;
@@ -2022,10 +2030,10 @@ __device__ __forceinline__ void op_iterp(const Tile &t, int roles, const float4 
             // position from LDS, where the ring's are refreshed)
 #pragma unroll
             for (int k = 0; k < NK; k++) {
-                const int i = tid + k * NT;
+                const int i = pos_i(k);
                 if (!(fl[k] & F_DUAL)) continue;
                 if (!FULL && !(fl[k] & F_ACT && fl[k] & F_IN)) continue;
-                const int x = (pxy[k] & 0xffff) - 16, y = (pxy[k] >> 16) - 16;
+                const int x = pos_x(k), y = pos_y(k);
                 const int kR = (FULL || x + 1 < w) ? i + 1 : i, kD = (FULL || y + 1 < h) ? i + B::RW : i;
                 DualIn rr;
                 rr.ok = true;
@@ -2067,25 +2075,28 @@ __device__ __forceinline__ void op_iterp(const Tile &t, int roles, const float4 
             // u_ and v_ switch to the buffer the next iteration reads there
 #pragma unroll
             for (int k = 0; k < NK; k++) {
-                const int i = tid + k * NT;
+                const int i = pos_i(k);
                 if (FULL && !(fl[k] & F_TILE)) continue;
                 if (!FULL && !(fl[k] & F_IN)) continue;
-                const int x = (pxy[k] & 0xffff) - 16, y = (pxy[k] >> 16) - 16;
+                const int x = pos_x(k), y = pos_y(k);
                 const long pos = (long)y * s + x;
                 if (!FULL && MASKED && !(fl[k] & F_ACT)) {
+                    // the two buffers' values trade places between the u_ / v_ slots (read
+                    // by neighbours) and the q slot: the current buffers' after odd
+                    // iterations, the other buffers' after even ones; the last iteration
+                    // finds the current buffers' values in the q slot
+                    const float4 sw = L.q[i];
                     if (MODE == 2) {
                         if (fl[k] & F_TILE) {
                             slot(t, slot_u(rspare(roles, G_U)))[pos] = o[k].u;
-                            slot(t, slot_ub(rspare(roles, G_UB)))[pos] = o[k].ub;
-                            slotp<float2>(t, slot_v(rspare(roles, G_V)))[pos] = o[k].v;
+                            slot(t, slot_ub(rspare(roles, G_UB)))[pos] = sw.x;
+                            slotp<float2>(t, slot_v(rspare(roles, G_V)))[pos] = make_float2(sw.y, sw.z);
                         }
-                    } else if (j & 1) {
-                        L.ub[i] = o[k].ub;
-                        L.v[i] = o[k].v;
                     } else {
-                        const float4 oth = L.q[i];
-                        L.ub[i] = oth.x;
-                        L.v[i] = make_float2(oth.y, oth.z);
+                        const float2 vv = L.v[i];
+                        L.q[i] = make_float4(L.ub[i], vv.x, vv.y, 0.f);
+                        L.ub[i] = sw.x;
+                        L.v[i] = make_float2(sw.y, sw.z);
                     }
                     continue;
                 }
@@ -2111,6 +2122,22 @@ __device__ __forceinline__ void op_iterp(const Tile &t, int roles, const float4 
                 o[k].u = un;
                 L.ub[i] = u_s;
                 L.v[i] = vs;
+                if (MODE != 2 && k == 0 && pub_s != 0) {
+                    // a border position: its strip words of this iteration
+                    const unsigned tg = tag0 | (unsigned)j;
+                    unsigned long long *pw = mine + (j & 1) * X::WORDS;
+                    const int co = (pub_s & 2047) - 1, ro = (pub_s >> 11) - 1;
+                    if (co >= 0) {
+                        xstore(pw + co, u_s, tg);
+                        xstore(pw + co + TH, vs.x, tg);
+                        xstore(pw + co + 2 * TH, vs.y, tg);
+                    }
+                    if (ro >= 0) {
+                        xstore(pw + ro, u_s, tg);
+                        xstore(pw + ro + TW, vs.x, tg);
+                        xstore(pw + ro + 2 * TW, vs.y, tg);
+                    }
+                }
                 if (MODE == 1) {
                     slot(t, slot_u(roth(roles, G_U)))[pos] = un;
                     slot(t, slot_ub(roth(roles, G_UB)))[pos] = u_s;
@@ -2121,7 +2148,7 @@ __device__ __forceinline__ void op_iterp(const Tile &t, int roles, const float4 
                     slotp<float2>(t, slot_v(rspare(roles, G_V)))[pos] = vs;
                 }
             }
-            // the strips of this iteration are read from LDS after this barrier
+            // the next iteration's duals read this one's u_ and v_ from LDS
             if (MODE != 2) __syncthreads();
             return true;
         };

@@ -88,6 +88,31 @@ __host__ __device__ constexpr bool persist_shape(int shape)
 // Exchange words of one tile and parity: u_, v_.x, v_.y of columns tw-1, tw-2, rows
 // th-1, th-2, column 0 and row 0, and the tile's progress word (StripP in tgv_device.h).
 __host__ __device__ constexpr int persist_words(int shape) { return 9 * (kShape[shape].tw + kShape[shape].th) + 1; }
+// The order in which its work-items take the staged positions (position q = tid + k NT):
+// the tile's border first (columns 0, tw - 2, tw - 1 and rows 0, th - 2, th - 1 of the
+// tile, the positions its strips hold), then the rest of the staged region (the tile grown
+// by 2 / 1), each in raster order.  persist_rank maps a raster index of the staged region
+// to its place in that order.
+__host__ __device__ constexpr int persist_border(int shape) { return 3 * (kShape[shape].tw + kShape[shape].th) - 9; }
+__host__ __device__ constexpr int persist_rank(int shape, int i)
+{
+    const int tw = kShape[shape].tw, th = kShape[shape].th, rw = tw + 3;
+    const int dx = i % rw - 2, dy = i / rw - 2;
+    // border positions in tile rows [0, y), and in tile row y before column x (x <= tw)
+    auto rows_before = [=](int y) {
+        const int full = (y > 0) + (y > th - 2) + (y > th - 1);
+        return full * tw + (y - full) * 3;
+    };
+    auto in_row_before = [=](int y, int x) {
+        return (y == 0 || y == th - 2 || y == th - 1) ? x : (x > 0) + (x > tw - 2) + (x > tw - 1);
+    };
+    const bool in_tile = dx >= 0 && dx < tw && dy >= 0 && dy < th;
+    if (in_tile && (dy == 0 || dy == th - 2 || dy == th - 1 || dx == 0 || dx == tw - 2 || dx == tw - 1))
+        return rows_before(dy) + in_row_before(dy, dx);
+    int before = rows_before(dy < 0 ? 0 : dy > th ? th : dy);
+    if (dy >= 0 && dy < th) before += in_row_before(dy, dx < 0 ? 0 : dx);
+    return persist_border(shape) + i - before;
+}
 
 // Blocked persistent op (OP_ITERPK): the K-iteration op's workgroups made persistent.  A
 // workgroup stages its tile grown by 2K / K once, runs blocks of K iterations as the
