@@ -1974,6 +1974,11 @@ __device__ __forceinline__ void op_iterp(const Tile &t, int roles, const float4 
         }
         unsigned long long *__restrict__ mine = xbuf + (long)tile * 2 * X::WORDS;
         const unsigned tag0 = epoch << 8;
+        // this work-item's exchange words, loaded at the end of each iteration (before its
+        // last barrier) and checked at the start of the next, polled again while older
+        const unsigned long long *rw = xbuf + (long)(rcv_w & 0x3fffffff);
+        const int rst = (rcv_w >> 30) ? TH : TW;
+        unsigned long long pa0 = 0, pa1 = 0, pa2 = 0;
 
         // One iteration.  MODE 0: an iteration before the op's last two; 1: iteration
         // n - 2 (u, u_, v_ to the other buffers); 2: the last one (the whole state to the
@@ -1985,21 +1990,19 @@ __device__ __forceinline__ void op_iterp(const Tile &t, int roles, const float4 
             if (j > 0) {
                 // ---- refresh the ring from the neighbours' strips of iteration j - 1 (their
                 // primal steps published them)
+                const unsigned tg = tag0 | (unsigned)(j - 1);
+                const unsigned long long *pw = rw + ((j - 1) & 1) * X::WORDS;
                 if (rcv_i == -2) {
                     // a neighbour's progress word
-                    if (!persist_wait_tag(xbuf + (long)rcv_w + ((j - 1) & 1) * X::WORDS, tag0 | (unsigned)(j - 1), xs))
-                        bad = 3;
+                    if ((unsigned)(pa0 >> 32) != tg && !persist_wait_tag(pw, tg, xs)) bad = 3;
                 } else if (rcv_i >= 0) {
-                    const unsigned tg = tag0 | (unsigned)(j - 1);
-                    const unsigned long long *pw = xbuf + (long)(rcv_w & 0x3fffffff) + ((j - 1) & 1) * X::WORDS;
-                    const int st = (rcv_w >> 30) ? TH : TW;
-                    unsigned long long a0, a1, a2, t0 = 0;
+                    unsigned long long a0 = pa0, a1 = pa1, a2 = pa2, t0 = 0;
                     for (int polls = 0;; polls++) {
-                        a0 = xload(pw);
-                        a1 = xload(pw + st);
-                        a2 = xload(pw + 2 * st);
                         if ((unsigned)(a0 >> 32) == tg && (unsigned)(a1 >> 32) == tg && (unsigned)(a2 >> 32) == tg)
                             break;
+                        a0 = xload(pw);
+                        a1 = xload(pw + rst);
+                        a2 = xload(pw + 2 * rst);
                         if (polls == 0) t0 = rtime();
                         if ((polls & 63) == 63 && (rtime() - t0 > kPollTicks || xload32(&xs->abort) != 0)) {
                             bad = 2;
@@ -2146,6 +2149,16 @@ __device__ __forceinline__ void op_iterp(const Tile &t, int roles, const float4 
                     slot(t, slot_u(rspare(roles, G_U)))[pos] = un;
                     slot(t, slot_ub(rspare(roles, G_UB)))[pos] = u_s;
                     slotp<float2>(t, slot_v(rspare(roles, G_V)))[pos] = vs;
+                }
+            }
+            // the words the next iteration reads (the neighbours published them while
+            // computing this iteration)
+            if (MODE != 2 && rcv_i != -1) {
+                const unsigned long long *pw = rw + (j & 1) * X::WORDS;
+                pa0 = xload(pw);
+                if (rcv_i >= 0) {
+                    pa1 = xload(pw + rst);
+                    pa2 = xload(pw + 2 * rst);
                 }
             }
             // the next iteration's duals read this one's u_ and v_ from LDS
