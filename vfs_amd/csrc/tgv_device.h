@@ -878,7 +878,6 @@ __device__ __forceinline__ void op_iter(const Tile &t, int roles, const IterPara
     // ---- phase A: new duals + Tp into LDS (and the tile's to global memory)
 #pragma unroll
     for (int r = 0; r < RPT; r++) {
-        const int y = y0 + threadIdx.y + BY * r;
         if (!in[r].act) continue;
         float2 pn, tpn;
         float4 qn;
@@ -1336,9 +1335,49 @@ __device__ __forceinline__ void op_iterk(const Tile &t, int roles, const float4 
         np[k] = staged ? K - max(rpx, rpy) : 0;
     }
 
+    // ---- the persistent form's exchange roles, once per launch: xr[k] >= 0 for a ring
+    // position that neighbours update (its tile's value-0 word of parity 0), -2 for a tile
+    // position in the published bands, -1 otherwise; hb: the work-items 0..7 wait for the
+    // neighbours' progress words (that neighbour's word of parity 0, or -1 at the image
+    // border); pf / pfh: with one position per work-item, its ring words and the progress
+    // word loaded at the end of each block but the last and checked at the next block's
+    // start (polled again while older).
+    using X = StripK<SH>;
+    int xr[NK], hb = -1;
+    unsigned long long *__restrict__ mine = nullptr;
+    const unsigned tag0 = epoch << 8;
+    unsigned long long pf[NK == 1 ? 10 : 1], pfh = 0;
+#pragma unroll
+    for (int v = 0; v < (NK == 1 ? 10 : 1); v++) pf[v] = 0;
+    if constexpr (PERSIST) {
+        constexpr int TW = B::TW, TH = B::TH;
+        const int ntx = (w + TW - 1) / TW, nty = (h + TH - 1) / TH;
+#pragma unroll
+        for (int k = 0; k < NK; k++) {
+            const int i = tid + k * NT, x = ox[k], y = oy[k];
+            xr[k] = -1;
+            if (i >= B::NR || x < 0 || y < 0 || x >= w || y >= h || !act(i)) continue;
+            const int dx = x - x0, dy = y - y0;
+            if (in_tile(x, y)) {
+                if (dx < K || dx >= TW - 2 * K || dy < K || dy >= TH - 2 * K) xr[k] = -2;
+            } else {
+                const int sx = x / TW, sy = y / TH;
+                xr[k] = (sy * ntx + sx) * 2 * X::WORDS + (y - sy * TH) * TW + (x - sx * TW);
+            }
+        }
+        if (tid < 8) {
+            const int ddx = tid < 3 ? tid - 1 : tid < 5 ? (tid == 3 ? -1 : 1) : tid - 6;
+            const int ddy = tid < 3 ? -1 : tid < 5 ? 0 : 1;
+            const int nx = x0 / TW + ddx, ny = y0 / TH + ddy;
+            if (nx >= 0 && ny >= 0 && nx < ntx && ny < nty) hb = (ny * ntx + nx) * 2 * X::WORDS + X::HB;
+        }
+        mine = pk.xbuf + (long)pk.tile * 2 * X::WORDS;
+    }
+
     // one block of K iterations (`blk`: its index, iterations blk K ..; `last`: the op's last
     // block, which stores the state; a compile-time flag, so that no store address is kept
-    // across the blocks before it)
+    // across the blocks before it).  The persistent form's blocks but the last publish the
+    // bands in their last iteration's primal step, as each band position is computed.
     auto iterate = [&](auto full_c, int blk, auto last_c) {
         constexpr bool FULL = decltype(full_c)::value;
         constexpr bool last = decltype(last_c)::value;
@@ -1464,6 +1503,26 @@ __device__ __forceinline__ void op_iterk(const Tile &t, int roles, const float4 
                 o[k].v = vs;
                 L.ub[i] = u_s;
                 L.v[i] = vs;
+                if constexpr (PERSIST && !last) {
+                    if (r == 0 && xr[k] == -2) {
+                        // a band position: its state after the block (q: this work-item's own
+                        // LDS slot, written by this iteration's dual step)
+                        constexpr int TW = B::TW;
+                        const unsigned tg = tag0 | (unsigned)blk;
+                        unsigned long long *pp = mine + (blk & 1) * X::WORDS + (y - y0) * TW + (x - x0);
+                        const float4 q = L.q[i];
+                        xstore(pp, un, tg);
+                        xstore(pp + X::VP, u_s, tg);
+                        xstore(pp + 2 * X::VP, vs.x, tg);
+                        xstore(pp + 3 * X::VP, vs.y, tg);
+                        xstore(pp + 4 * X::VP, o[k].p.x, tg);
+                        xstore(pp + 5 * X::VP, o[k].p.y, tg);
+                        xstore(pp + 6 * X::VP, q.x, tg);
+                        xstore(pp + 7 * X::VP, q.y, tg);
+                        xstore(pp + 8 * X::VP, q.z, tg);
+                        xstore(pp + 9 * X::VP, q.w, tg);
+                    }
+                }
                 if (last && r == 1 && in_tile(x, y)) {
                     u_oth[pos] = un;
                     ub_oth[pos] = u_s;
@@ -1474,6 +1533,17 @@ __device__ __forceinline__ void op_iterk(const Tile &t, int roles, const float4 
                     v_spr[pos] = vs;
                 }
             }
+            if constexpr (PERSIST && !last && NK == 1) {
+                // the words the next block's start reads
+                if (r == 0) {
+                    const int par = blk & 1;
+                    if (xr[0] >= 0) {
+#pragma unroll
+                        for (int v = 0; v < 10; v++) pf[v] = xload(pk.xbuf + xr[0] + par * X::WORDS + v * X::VP);
+                    }
+                    if (hb >= 0) pfh = xload(pk.xbuf + hb + par * X::WORDS);
+                }
+            }
             __syncthreads();
         }
     };
@@ -1481,72 +1551,33 @@ __device__ __forceinline__ void op_iterk(const Tile &t, int roles, const float4 
         if (full) iterate(std::true_type{}, 0, std::true_type{});
         else iterate(std::false_type{}, 0, std::true_type{});
     } else {
-        using X = StripK<SH>;
-        constexpr int TW = B::TW, TH = B::TH;
-        // Exchange roles, once per launch: xr[k] >= 0 for a ring position that neighbours
-        // update (its tile's value-0 word of parity 0), -2 for a tile position in the
-        // published bands, -1 otherwise; hb: the work-items 0..7 wait for the neighbours'
-        // progress words (that neighbour's word of parity 0, or -1 at the image border).
-        const int ntx = (w + TW - 1) / TW, nty = (h + TH - 1) / TH;
-        int xr[NK];
-#pragma unroll
-        for (int k = 0; k < NK; k++) {
-            const int i = tid + k * NT, x = ox[k], y = oy[k];
-            xr[k] = -1;
-            if (i >= B::NR || x < 0 || y < 0 || x >= w || y >= h || !act(i)) continue;
-            const int dx = x - x0, dy = y - y0;
-            if (in_tile(x, y)) {
-                if (dx < K || dx >= TW - 2 * K || dy < K || dy >= TH - 2 * K) xr[k] = -2;
-            } else {
-                const int sx = x / TW, sy = y / TH;
-                xr[k] = (sy * ntx + sx) * 2 * X::WORDS + (y - sy * TH) * TW + (x - sx * TW);
-            }
-        }
-        int hb = -1;
-        if (tid < 8) {
-            const int ddx = tid < 3 ? tid - 1 : tid < 5 ? (tid == 3 ? -1 : 1) : tid - 6;
-            const int ddy = tid < 3 ? -1 : tid < 5 ? 0 : 1;
-            const int nx = x0 / TW + ddx, ny = y0 / TH + ddy;
-            if (nx >= 0 && ny >= 0 && nx < ntx && ny < nty) hb = (ny * ntx + nx) * 2 * X::WORDS + X::HB;
-        }
-        unsigned long long *__restrict__ mine = pk.xbuf + (long)pk.tile * 2 * X::WORDS;
-        const unsigned tag0 = epoch << 8;
         int blk = 0;
         bool gave_up = false;
         for (;; blk++) {
             if (blk > 0) {
                 const unsigned tg = tag0 | (unsigned)(blk - 1);
                 const int par = (blk - 1) & 1;
-                // ---- publish the bands (the previous block's state: this work-item's
-                // registers and its own q in LDS) and the progress word
-#pragma unroll
-                for (int k = 0; k < NK; k++) {
-                    if (xr[k] != -2) continue;
-                    unsigned long long *pp = mine + par * X::WORDS + (oy[k] - y0) * TW + (ox[k] - x0);
-                    const float4 q = L.q[tid + k * NT];
-                    xstore(pp, o[k].u, tg);
-                    xstore(pp + X::VP, o[k].ub, tg);
-                    xstore(pp + 2 * X::VP, o[k].v.x, tg);
-                    xstore(pp + 3 * X::VP, o[k].v.y, tg);
-                    xstore(pp + 4 * X::VP, o[k].p.x, tg);
-                    xstore(pp + 5 * X::VP, o[k].p.y, tg);
-                    xstore(pp + 6 * X::VP, q.x, tg);
-                    xstore(pp + 7 * X::VP, q.y, tg);
-                    xstore(pp + 8 * X::VP, q.z, tg);
-                    xstore(pp + 9 * X::VP, q.w, tg);
-                }
-                if (tid == NT - 1) xstore(mine + par * X::WORDS + X::HB, 0.f, tg);
-                // ---- the neighbours' progress, then the ring's state
+                // ---- the neighbours' progress, then the ring's state (both published in
+                // the previous block)
                 int bad = 0;
-                if (hb >= 0 && !persist_wait_tag(pk.xbuf + hb + par * X::WORDS, tg, pk.xs)) bad = 3;
+                if (hb >= 0 && (unsigned)(pfh >> 32) != tg && !persist_wait_tag(pk.xbuf + hb + par * X::WORDS, tg, pk.xs))
+                    bad = 3;
 #pragma unroll
                 for (int k = 0; k < NK; k++) {
                     if (xr[k] < 0 || bad) continue;
-                    // the ten words in two groups of five (registers: a 64-bit word each)
                     const unsigned long long *pw = pk.xbuf + xr[k] + par * X::WORDS;
                     float f[10];
+                    bool have = false;
+                    if constexpr (NK == 1) {
+                        have = true;
 #pragma unroll
-                    for (int g = 0; g < 2; g++) {
+                        for (int v = 0; v < 10; v++) have = have && (unsigned)(pf[v] >> 32) == tg;
+#pragma unroll
+                        for (int v = 0; v < 10; v++) f[v] = __uint_as_float((unsigned)pf[v]);
+                    }
+                    // else the ten words in two groups of five (registers: a 64-bit word each)
+#pragma unroll
+                    for (int g = 0; g < 2 && !have; g++) {
                         unsigned long long a[5];
                         unsigned long long t0 = 0;
                         for (int polls = 0;; polls++) {
@@ -1581,6 +1612,9 @@ __device__ __forceinline__ void op_iterk(const Tile &t, int roles, const float4 
                 }
             }
             if (blk == pk.blocks - 1) break;
+            // progress: this tile has read its neighbours' words of block blk - 1, which
+            // they overwrite in block blk + 1
+            if (tid == NT - 1) xstore(mine + (blk & 1) * X::WORDS + X::HB, 0.f, tag0 | (unsigned)blk);
             if (full) iterate(std::true_type{}, blk, std::false_type{});
             else iterate(std::false_type{}, blk, std::false_type{});
         }
