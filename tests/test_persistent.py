@@ -193,8 +193,8 @@ def set_blocked(g, shape, K, masked=True, levels=3):
 @pytest.mark.parametrize("masked", [True, False])
 def test_every_blocked_persistent_form(masked):
     """Blocks of 2, 4 and 8 iterations in every tile shape that has them, at every level
-    (19 iterations per warp: one launch of 18 or 16 and a remainder in other launches),
-    equal one iteration per launch."""
+    (19 iterations per warp: one launch of 18, whose last block of 4 or 8 is shortened to 2,
+    and one iteration in another launch), equal one iteration per launch."""
     pair = small_pair(200, 120, seed=59)
     ref = planes_of(solve(pair, 3, 2, 19, masked, persist=0, per_launch=1), (0, 1))
     for K, shapes in BLOCKED:
@@ -237,10 +237,11 @@ def test_blocked_persistent_odd_geometry(masked):
         assert_same(planes_of(g, flow=False), ref, f"odd geometry, K {K} shape {shape}")
 
 
-@pytest.mark.parametrize("iters", [2, 4, 9, 130])
+@pytest.mark.parametrize("iters", [2, 4, 6, 9, 130])
 def test_blocked_persistent_iteration_counts(iters):
-    """Fewer iterations than a block (2 with blocks of 4: no persistent launch), one block
-    (4), blocks and a remainder (9), beyond the 128-entry step table (130)."""
+    """Fewer iterations than a block (2: one shortened block), one block (4), a full block
+    of 4 and a shortened one or one shortened block of 8 (6), the same and one iteration in
+    another launch (9), beyond the 128-entry step table (130)."""
     pair = small_pair(128, 96, seed=71)
     ref = planes_of(solve(pair, 2, 2, iters, persist=0, per_launch=1))
     for K, shape in ((4, 7), (8, 6)):

@@ -27,13 +27,15 @@ def test_inner_ops_persistent_split():
 
 
 def test_inner_ops_blocked_persistent_split():
-    # a multiple of the block within the 128-step table, the rest split as with 8 per launch
-    assert inner_ops(50, "P4") == ["ITERPK", "ITER2"]
-    assert inner_ops(50, "P8") == ["ITERPK", "ITER2"]
-    assert inner_ops(50, "P2") == ["ITERPK"]
-    assert inner_ops(19, "P8") == ["ITERPK", "ITER2", "ITER"]
-    assert inner_ops(3, "P4") == ["ITER2", "ITER"]
+    # an even count within the 128-step table (blocks, the last one shortened), the rest
+    # split as with 8 per launch
+    assert inner_ops(50, "P4") == ["ITERPK"]
+    assert inner_ops(50, "P8") == ["ITERPK"]
+    assert inner_ops(19, "P8") == ["ITERPK", "ITER"]
+    assert inner_ops(3, "P4") == ["ITERPK", "ITER"]
+    assert inner_ops(1, "P4") == ["ITER"]
     assert inner_ops(130, "P8") == ["ITERPK", "ITER2"]
+    assert inner_ops(130, "P4:1") == ["ITERPK", "ITER", "ITER"]
 
 
 def test_program_ops_count_matches_traced_solve():

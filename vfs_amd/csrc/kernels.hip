@@ -261,7 +261,7 @@ hipError_t launch_tick(const PairDesc *desc, const TickArgs &args, int storage, 
     if (args.e[0].kind == OP_ITERPK) {
         const int n = iterp_count(args.e[0].it0), K = iterp_block(args.e[0].it0);
         if (args.n != 1 || shape < 0 || shape >= kShapes || (K != 2 && K != 4 && K != 8) ||
-            !persistk_shape(shape, K) || n < K || n % K || iterp_first(args.e[0].it0) + n > kMaxSched ||
+            !persistk_shape(shape, K) || n < 2 || n % 2 || iterp_first(args.e[0].it0) + n > kMaxSched ||
             args.total > kMaxPersistTiles)
             return hipErrorInvalidValue;
         using Fn = void (*)(const PairDesc *, const TickArgs &, int, hipStream_t);

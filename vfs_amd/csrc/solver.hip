@@ -375,8 +375,8 @@ void push_op(Program &pr, int kind, int level, int masked, int roles)
 // (stereotgv.cu:409-494, masked; :275-364 unmasked).  With `sync_spare`, each warp's
 // first multi-iteration op is preceded by OP_SYNC (warm solves above level 0).  With
 // `pblock` 1, a warp's iterations run in OP_ITERP ops (an even count each, within the
-// schedule table), with 2, 4 or 8 in OP_ITERPK ops (a multiple of pblock each), and the
-// rest as without it.
+// schedule table), with 2, 4 or 8 in OP_ITERPK ops (the same counts, in blocks of pblock),
+// and the rest as without it.
 void push_level(Program &pr, const Params &P, int masked, int l, int warps, int iters, int &roles, int per_launch,
                 bool sync_spare = false, int pblock = 0)
 {
@@ -399,8 +399,8 @@ void push_level(Program &pr, const Params &P, int masked, int l, int warps, int 
             int k = 1;
             for (int c : {8, 4, 2})
                 if (k == 1 && c <= per_launch && it + c <= iters && it + c <= vfs::kMaxSched) k = c;
-            const int kp = std::min(iters - it, vfs::kMaxSched - it) & ~(std::max(pblock, 2) - 1);
-            if (pblock > 0 && !sync_spare && kp >= std::max(pblock, 2)) {
+            const int kp = std::min(iters - it, vfs::kMaxSched - it) & ~1;
+            if (pblock > 0 && !sync_spare && kp >= 2) {
                 o.kind = pblock > 1 ? vfs::OP_ITERPK : vfs::OP_ITERP;
                 o.it0 = (kp << 16) | (pblock > 1 ? pblock << 8 : 0) | it;
                 o.flags = (unsigned char)((it == 0 ? vfs::EF_FIRST : 0) | (it + kp == iters ? vfs::EF_WRITE_TP_LAST : 0));
@@ -853,8 +853,9 @@ struct PersistCands {
     vfs::XSync *xsync = nullptr;
 };
 constexpr int kPersistK[4] = {1, 2, 4, 8};
-// Iterations of one persistent launch of blocks of K (1: OP_ITERP) out of `iters`.
-int persist_launch_iters(int iters, int K) { return iters & ~(std::max(K, 2) - 1); }
+// Iterations of one persistent launch (OP_ITERP, or OP_ITERPK in blocks of K with a
+// shorter last block) out of `iters`: the even count.
+int persist_launch_iters(int iters, int /*K*/) { return iters & ~1; }
 
 // Times the inner iteration of every level in `levels` in each (tile shape, Tp preload)
 // on the launch set `descs` (their arenas point at scratch memory; `d_desc` is any valid
@@ -912,7 +913,7 @@ hipError_t time_launch_choices(hipStream_t st, const vfs::PairDesc *d_desc, cons
                 cand(c, shape, pre, per_launch, half);
                 const bool persist = c >= kMulti;
                 const int pk = persist ? kPersistK[(c - kMulti) / vfs::kShapes] : 0;
-                if (persist && (!pc || per_launch < std::max(pk, 2) ||
+                if (persist && (!pc || per_launch < 2 ||
                                 pc->tiles[li][(c - kMulti) / vfs::kShapes][shape] == 0 || n != 1))
                     continue;
                 if (!persist && per_launch > 1 &&
@@ -1153,7 +1154,7 @@ bool persist_program(const vfs_solver *s, int masked, const Tuning &tn, size_t *
         if (!tn.lv[l].persist) continue;
         const int K = tn.lv[l].pblock;
         const int n = persist_tiles(s, masked, l, tn.lv[l].pshape, K);
-        if (n == 0 || persist_launch_iters(std::min(s->nSolverIters, vfs::kMaxSched), K) < std::max(K, 2)) continue;
+        if (n == 0 || persist_launch_iters(std::min(s->nSolverIters, vfs::kMaxSched), K) < 2) continue;
         any = true;
         *words = std::max(*words, persist_words_of(n, tn.lv[l].pshape, K));
     }
@@ -1964,7 +1965,7 @@ int vfs_time_inner_iterations(vfs_solver *s, int level, int reps, float *ms_tota
     const int pb = lt.pblock;
     const bool persist = persist_program(s, 1, s->tuning[1], &words) && lt.persist &&
                          persist_tiles(s, 1, level, lt.pshape, pb) > 0 &&
-                         persist_launch_iters(std::min(s->nSolverIters, vfs::kMaxSched), pb) >= std::max(pb, 2);
+                         persist_launch_iters(std::min(s->nSolverIters, vfs::kMaxSched), pb) >= 2;
     const int sched_n = persist ? persist_launch_iters(std::min(s->nSolverIters, vfs::kMaxSched), pb) : 48;
     if (persist) {
         per_launch = sched_n;

@@ -1119,11 +1119,11 @@ struct StripK {
     static_assert(WORDS == persistk_words(SH), "strip layout");
 };
 
-// The blocked persistent op's launch: `blocks` blocks of K iterations, this workgroup's tile
-// among the launch's `ntiles`, the pair's exchange words and bookkeeping, and two words of
-// LDS for the arrival.
+// The blocked persistent op's launch: `blocks` blocks of K iterations (the last one of K -
+// `skip`), this workgroup's tile among the launch's `ntiles`, the pair's exchange words and
+// bookkeeping, and two words of LDS for the arrival.
 struct PersistK {
-    int blocks = 1;
+    int blocks = 1, skip = 0;
     int tile = 0, ntiles = 0;
     XSync *xs = nullptr;
     unsigned long long *xbuf = nullptr;
@@ -1134,7 +1134,8 @@ struct PersistK {
 // schedule-independent fields and the flags of the op.
 // PERSIST (OP_ITERPK): the workgroups of the launch are persistent, one per tile.  A
 // workgroup stages its tile grown by 2K / K once and runs pk.blocks blocks of K iterations
-// in place.  After each block but the last it publishes the state of the bands of its tile
+// in place, the last one shortened to K - pk.skip iterations (an even number: the block's
+// last iterations, whose regions the staged state covers).  After each block but the last it publishes the state of the bands of its tile
 // that neighbouring rings cover (columns < K and >= tw - 2K, rows < K and >= th - 2K; the
 // tile is at least 2K on each side, so a ring reads its 8 neighbours only) and then
 // refreshes its own ring (everything staged outside the tile that lies in the image and the
@@ -1378,20 +1379,23 @@ __device__ __forceinline__ void op_iterk(const Tile &t, int roles, const float4 
     // block, which stores the state; a compile-time flag, so that no store address is kept
     // across the blocks before it).  The persistent form's blocks but the last publish the
     // bands in their last iteration's primal step, as each band position is computed.
-    auto iterate = [&](auto full_c, int blk, auto last_c) {
+    // `skip`: the block runs iterations skip .. K - 1 only (even: the iterations keep the
+    // parity of their index in the warp, which the out-of-mask values follow)
+    auto iterate = [&](auto full_c, int blk, auto last_c, int skip) {
         constexpr bool FULL = decltype(full_c)::value;
         constexpr bool last = decltype(last_c)::value;
         // unrolled: each iteration's regions and stores are known at compile time (fewer
         // registers and tests than a loop over j)
 #pragma unroll
         for (int j = 0; j < K; j++) {
+            if (j < skip) continue;
             IterParams ip = ip0;
-            const float4 sc = sched[blk * K + j];
+            const float4 sc = sched[blk * K + j - skip];
             ip.k1 = sc.w;  // alpha1 sigma / 3, as op_iter forms it
             ip.k0 = ip0.alpha0 * sc.x / 2.0f;
             ip.tau = sc.y;
             ip.mu = sc.z;
-            ip.first = j == 0 && blk == 0 && ip0.first;
+            ip.first = j == skip && blk == 0 && ip0.first;
             ip.write_tp = j == K - 1 && last && ip0.write_tp;
             const int r = K - 1 - j;
 
@@ -1548,8 +1552,8 @@ __device__ __forceinline__ void op_iterk(const Tile &t, int roles, const float4 
         }
     };
     if constexpr (!PERSIST) {
-        if (full) iterate(std::true_type{}, 0, std::true_type{});
-        else iterate(std::false_type{}, 0, std::true_type{});
+        if (full) iterate(std::true_type{}, 0, std::true_type{}, 0);
+        else iterate(std::false_type{}, 0, std::true_type{}, 0);
     } else {
         int blk = 0;
         bool gave_up = false;
@@ -1615,12 +1619,12 @@ __device__ __forceinline__ void op_iterk(const Tile &t, int roles, const float4 
             // progress: this tile has read its neighbours' words of block blk - 1, which
             // they overwrite in block blk + 1
             if (tid == NT - 1) xstore(mine + (blk & 1) * X::WORDS + X::HB, 0.f, tag0 | (unsigned)blk);
-            if (full) iterate(std::true_type{}, blk, std::false_type{});
-            else iterate(std::false_type{}, blk, std::false_type{});
+            if (full) iterate(std::true_type{}, blk, std::false_type{}, 0);
+            else iterate(std::false_type{}, blk, std::false_type{}, 0);
         }
         if (!gave_up) {
-            if (full) iterate(std::true_type{}, blk, std::true_type{});
-            else iterate(std::false_type{}, blk, std::true_type{});
+            if (full) iterate(std::true_type{}, blk, std::true_type{}, pk.skip);
+            else iterate(std::false_type{}, blk, std::true_type{}, pk.skip);
         }
         persist_finish(pk.xs, pk.ntiles, epoch);
     }
@@ -1704,7 +1708,9 @@ __global__ __launch_bounds__((BlkK<SH, K, false>::NT)) void k_iterkp(const PairD
     ip0.preload_tp = 0;
     ip0.deriv_level = (E.flags & EF_DERIV) ? (int)E.level : -1;
     PersistK pk;
-    pk.blocks = iterp_count(E.it0) / K;
+    const int n = iterp_count(E.it0);
+    pk.blocks = (n + K - 1) / K;
+    pk.skip = pk.blocks * K - n;
     pk.tile = b;
     pk.ntiles = nb;
     pk.xs = t.d->xsync;

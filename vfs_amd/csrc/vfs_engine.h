@@ -116,7 +116,8 @@ __host__ __device__ constexpr int persist_rank(int shape, int i)
 
 // Blocked persistent op (OP_ITERPK): the K-iteration op's workgroups made persistent.  A
 // workgroup stages its tile grown by 2K / K once, runs blocks of K iterations as the
-// K-iteration op does, and between blocks refreshes its ring (everything it staged outside
+// K-iteration op does (an even count in all, the last block shortened to the rest), and
+// between blocks refreshes its ring (everything it staged outside
 // its tile) from the state its neighbours publish: u, u_, v_, p and q of the bands of
 // their tiles that neighbouring rings cover.  Shapes: the tile at least 2K pixels on each
 // side (a ring then reads its 8 neighbours only), at most two staged positions per

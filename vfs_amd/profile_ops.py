@@ -15,8 +15,8 @@ from level 0, as the tuner reports them) issue `k_iterk` dispatches: ITER8 / ITE
 they fit the iteration count, the rest one at a time; `P` marks a level whose warps run
 their iterations in persistent `k_iterp` launches (ITERP: an even count, at most 128, each;
 an odd remainder one at a time), `P2` / `P4` / `P8` one in blocked persistent `k_iterkp`
-launches (ITERPK: a multiple of the block, at most 128, the rest as `--per-launch` 8 would
-split it, or as `:n` after it says, e.g. `P4:1`).
+launches (ITERPK: the same even counts, in blocks, the rest as `--per-launch` 8 would split
+it, or as `:n` after it says, e.g. `P4:1`).
 
     python -m vfs_amd.profile_ops TRACE.csv --levels 5 --warps 10 --iters 50 --per-launch 4,4,4,4,4 --skip-tail 100
     python -m vfs_amd.profile_ops --batch-traffic FETCH.csv WRITE.csv --storage f16   # HBM bytes per px-iteration
@@ -37,10 +37,9 @@ def inner_ops(iters: int, per_launch) -> list[str]:
     out, it = [], 0
     if isinstance(per_launch, str) and per_launch[:2] in ("P2", "P4", "P8"):
         # "P4:1": the level's own iterations per launch (for the remainder) is 1
-        b = int(per_launch[1])
         rest = int(per_launch.split(":")[1]) if ":" in per_launch else 8
-        k = min(iters, 128) // b * b
-        if k >= b:
+        k = min(iters, 128) & ~1
+        if k >= 2:
             out.append("ITERPK")
             it = k
         return out + inner_ops(iters - it, rest)
