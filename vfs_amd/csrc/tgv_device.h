@@ -1751,6 +1751,8 @@ template <int SH>
 struct BlkP {
     static constexpr int TW = kShape[SH].tw, TH = kShape[SH].th;
     static constexpr int RO = -2, RW = TW + 3, RH = TH + 3, NR = RW * RH;  // staged region
+    // LDS rows of an odd pitch: a column of the region spans every bank
+    static constexpr int PITCH = RW | 1, NL = RH * PITCH;
     static constexpr int NT = NR >= 1024 ? 1024 : (NR + 63) / 64 * 64;
     static constexpr int NK = (NR + NT - 1) / NT;  // staged positions per work-item
     static_assert(persist_shape(SH) && NK <= 3 && TW >= 3 && TH >= 3, "persistent tile shape");
@@ -1759,11 +1761,11 @@ struct BlkP {
 template <int SH>
 struct LdsP {
     static constexpr int NR = BlkP<SH>::NR;
-    float ub[NR];
-    float2 v[NR];
-    float4 q[NR];
-    float2 tp[NR];
-    unsigned char mk[NR];
+    float ub[BlkP<SH>::NL];
+    float2 v[BlkP<SH>::NL];
+    float4 q[BlkP<SH>::NL];
+    float2 tp[BlkP<SH>::NL];
+    unsigned char mk[BlkP<SH>::NL];
 };
 
 // Exchange words of one tile and parity: u_, v_.x, v_.y (value-major) of column tw - 1
@@ -1816,7 +1818,7 @@ __device__ __forceinline__ void op_iterp(const Tile &t, int roles, const float4 
         // order (the tile's border at k = 0, so that the work-items that compute it in the
         // primal step publish it), through a table in the LDS of q (staged over afterwards);
         // pi[k] = LDS index | column << 12 | row << 20 in the staged region
-        static_assert(persist_border(SH) <= NT && B::NR <= 4096 && B::RW < 256 && X::WORDS < 2047, "positions");
+        static_assert(persist_border(SH) <= NT && B::NL <= 4096 && B::RW < 256 && X::WORDS < 2047, "positions");
         int pi[NK];
         {
             int *tab = reinterpret_cast<int *>(L.q);
@@ -1829,8 +1831,8 @@ __device__ __forceinline__ void op_iterp(const Tile &t, int roles, const float4 
 #pragma unroll
             for (int k = 0; k < NK; k++) {
                 const int q = tid + k * NT;
-                const int i = q < B::NR ? tab[q] : 0;
-                pi[k] = i | (i % B::RW) << 12 | (i / B::RW) << 20;
+                const int ri = q < B::NR ? tab[q] : 0, c = ri % B::RW, r = ri / B::RW;
+                pi[k] = (r * B::PITCH + c) | c << 12 | r << 20;
             }
             __syncthreads();
         }
@@ -1953,12 +1955,12 @@ __device__ __forceinline__ void op_iterp(const Tile &t, int roles, const float4 
             if (dx >= -1 && dy >= -1 && dx <= TW - 1 && dy <= TH - 1) fl[k] |= F_DUAL;
             if (dx >= 0 && dy >= 0 && dx <= TW - 1 && dy <= TH - 1) fl[k] |= F_TILE;
             if (!full && x >= 0 && y >= 0 && x < w && y < h) {
-                const int col = i % B::RW, row = i / B::RW;
+                const int col = (pi[k] >> 12) & 0xff, row = pi[k] >> 20;
                 fl[k] |= F_IN | (act(i) ? F_ACT : 0u);
                 if (x + 1 < w && col + 1 < B::RW && act(i + 1)) fl[k] |= F_R;
-                if (y + 1 < h && row + 1 < B::RH && act(i + B::RW)) fl[k] |= F_D;
+                if (y + 1 < h && row + 1 < B::RH && act(i + B::PITCH)) fl[k] |= F_D;
                 if (x >= 1 && col >= 1 && act(i - 1)) fl[k] |= F_L;
-                if (y >= 1 && row >= 1 && act(i - B::RW)) fl[k] |= F_U;
+                if (y >= 1 && row >= 1 && act(i - B::PITCH)) fl[k] |= F_U;
                 if (!MASKED && outside_circle(x, y, w, h)) fl[k] |= F_OUT;
             }
         }
@@ -1972,7 +1974,7 @@ __device__ __forceinline__ void op_iterp(const Tile &t, int roles, const float4 
         // progress words.  rcv_i: LDS index (-2: a progress word); rcv_w: the source tile's
         // word (parity 0) | 1 << 30 for a column strip (value stride th; rows: tw).
         constexpr int RS = 3 * (TW + TH) + 5;
-        static_assert(B::NR <= 4096 && X::WORDS < (1 << 20) && RS + 8 <= NT, "exchange slots");
+        static_assert(B::NL <= 4096 && X::WORDS < (1 << 20) && RS + 8 <= NT, "exchange slots");
         const int ntx = (w + TW - 1) / TW, nty = (h + TH - 1) / TH;
         int pub_s = 0, rcv_i = -1, rcv_w = 0;
         if (tid < persist_border(SH) && pos_x(0) < w && pos_y(0) < h && act(pos_i(0))) {
@@ -1989,7 +1991,7 @@ __device__ __forceinline__ void op_iterp(const Tile &t, int roles, const float4 
             else if (tid < 3 * TH + 5 + TW) dx = tid - (3 * TH + 5), dy = -2;
             else if (tid < 3 * TH + 5 + 2 * TW) dx = tid - (3 * TH + 5 + TW), dy = -1;
             else dx = tid - (3 * TH + 5 + 2 * TW), dy = TH;
-            const int x = x0 + dx, y = y0 + dy, i = (dy - B::RO) * B::RW + (dx - B::RO);
+            const int x = x0 + dx, y = y0 + dy, i = (dy - B::RO) * B::PITCH + (dx - B::RO);
             if (x >= 0 && y >= 0 && x < w && y < h && act(i)) {
                 // its tile's strip holding it
                 const int sx = x / TW, sy = y / TH, lx = x - sx * TW, ly = y - sy * TH;
@@ -2077,7 +2079,7 @@ __device__ __forceinline__ void op_iterp(const Tile &t, int roles, const float4 
                 if (!(fl[k] & F_DUAL)) continue;
                 if (!FULL && !(fl[k] & F_ACT && fl[k] & F_IN)) continue;
                 const int x = pos_x(k), y = pos_y(k);
-                const int kR = (FULL || x + 1 < w) ? i + 1 : i, kD = (FULL || y + 1 < h) ? i + B::RW : i;
+                const int kR = (FULL || x + 1 < w) ? i + 1 : i, kD = (FULL || y + 1 < h) ? i + B::PITCH : i;
                 DualIn rr;
                 rr.ok = true;
                 rr.act = true;
@@ -2090,12 +2092,12 @@ __device__ __forceinline__ void op_iterp(const Tile &t, int roles, const float4 
                 rr.uR = L.ub[kR];
                 rr.uD = L.ub[kD];
                 rr.uL = (!FULL && x >= 1) ? L.ub[i - 1] : 0.f;
-                rr.uU = (!FULL && y >= 1) ? L.ub[i - B::RW] : 0.f;
+                rr.uU = (!FULL && y >= 1) ? L.ub[i - B::PITCH] : 0.f;
                 rr.v0 = L.v[i];
                 rr.vR = L.v[kR];
                 rr.vD = L.v[kD];
                 rr.vL = (!FULL && x >= 1) ? L.v[i - 1] : make_float2(0.f, 0.f);
-                rr.vU = (!FULL && y >= 1) ? L.v[i - B::RW] : make_float2(0.f, 0.f);
+                rr.vU = (!FULL && y >= 1) ? L.v[i - B::PITCH] : make_float2(0.f, 0.f);
                 rr.po = make_float2(rT<T>(o[k].p.x), rT<T>(o[k].p.y));
                 const float4 qo = L.q[i];
                 rr.qo = make_float4(rT<T>(qo.x), rT<T>(qo.y), rT<T>(qo.z), rT<T>(qo.w));
@@ -2154,11 +2156,11 @@ __device__ __forceinline__ void op_iterp(const Tile &t, int roles, const float4 
                     const bool hR = FULL || (fl[k] & F_R) != 0, hD = FULL || (fl[k] & F_D) != 0;
                     const bool hL = FULL || (fl[k] & F_L) != 0, hU = FULL || (fl[k] & F_U) != 0;
                     const float tpLx = (FULL || x >= 1) ? L.tp[i - 1].x : 0.f;
-                    const float tpUy = (FULL || y >= 1) ? L.tp[i - B::RW].y : 0.f;
+                    const float tpUy = (FULL || y >= 1) ? L.tp[i - B::PITCH].y : 0.f;
                     const float2 tp0 = L.tp[i];
                     const float divp = div_tp<MASKED>(x, y, w, h, hR, hD, hL, hU, tp0, tpLx, tpUy);
                     const float4 ql = hL ? L.q[i - 1] : make_float4(0.f, 0.f, 0.f, 0.f);
-                    const float4 qu = hU ? L.q[i - B::RW] : make_float4(0.f, 0.f, 0.f, 0.f);
+                    const float4 qu = hU ? L.q[i - B::PITCH] : make_float4(0.f, 0.f, 0.f, 0.f);
                     primal_core(ip, kc[k], L.ub[i], o[k].u, o[k].iu, o[k].iz, L.v[i], hR, hD, hL, hU, divp, L.q[i], ql,
                                 qu, tp0, un, u_s, vs);
                 }
