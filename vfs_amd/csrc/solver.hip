@@ -1016,12 +1016,20 @@ hipError_t time_launch_choices(hipStream_t st, const vfs::PairDesc *d_desc, cons
             pick.mhalf = (unsigned char)half;
             pick_ms = best[li * kCand + bestm];
         }
-        // the persistent op where it is clearly faster than both (the launches above stay
-        // the level's choice wherever it cannot run)
+        // the persistent op where it is clearly faster than both over a warp's iterations
+        // (it runs all of them; the launches above leave a remainder to two-iteration or
+        // one-iteration launches) (the launches above stay the level's choice wherever it
+        // cannot run)
         int bestp = -1;
         for (int c = kMulti; c < kCand; c++)
             if (best[li * kCand + c] < 1e29f && (bestp < 0 || best[li * kCand + c] < best[li * kCand + bestp])) bestp = c;
-        if (bestp >= 0 && best[li * kCand + bestp] < 0.98f * pick_ms) {
+        float warp_ms = pick_ms;  // per iteration, the remainder included
+        if (pc && pc->iters >= 2 && pick.per_launch > 1) {
+            const int w = pc->iters, rem = w % pick.per_launch;
+            warp_ms = (pick_ms * (float)(w - rem) + (rem >= 2 ? best2 * (float)(rem & ~1) : 0.f) +
+                       ((rem & 1) ? best[li * kCand + pick.shape * 2 + pick.preload] : 0.f)) / (float)w;
+        }
+        if (bestp >= 0 && best[li * kCand + bestp] < 0.98f * warp_ms) {
             pick.persist = 1;
             pick.pshape = (unsigned char)((bestp - kMulti) % vfs::kShapes);
             pick.pblock = (unsigned char)kPersistK[(bestp - kMulti) / vfs::kShapes];

@@ -400,6 +400,23 @@ __device__ __forceinline__ float gauss3(const T *__restrict__ I, int w, int h, i
     return r;
 }
 
+// computeDerivatives.cu:88-129: the 5-point derivative along t = tv/|tv| of image `I` at
+// pixel (x, y) (one of the two halves Iu averages).
+template <typename T>
+__device__ __forceinline__ float deriv_along(const T *__restrict__ I, const float2 *__restrict__ tv, int w, int h,
+                                             int s, int x, int y)
+{
+    const float2 v = tv[(long)y * s + x];
+    const float r = sqrt_rn(v.x * v.x + v.y * v.y);
+    const float dx = v.x / r, dy = v.y / r;
+    const float X = (float)x, Y = (float)y;
+    float t = bil(I, w, h, s, X - 2.f * dx, Y - 2.f * dy);
+    t -= bil(I, w, h, s, X - dx, Y - dy) * 8.f;
+    t += bil(I, w, h, s, X + dx, Y + dy) * 8.f;
+    t -= bil(I, w, h, s, X + 2.f * dx, Y + 2.f * dy);
+    return t / 12.f;
+}
+
 // The smoothed image as the tensor kernel samples it: the masked blur writes only
 // mask != 0 pixels of a zeroed plane, and the texture fetch mirrors at the level edge.
 template <typename T, bool MASKED>
@@ -424,10 +441,14 @@ __device__ __forceinline__ void op_tensor(const Tile &t, int level, int xblock, 
     T *at = (T *)slot(t, PL_A);
     T *bt = (T *)slot(t, PL_B);
     T *ct = (T *)slot(t, PL_C);
+    float *i0d = slot(t, PL_I0D);
+    const float2 *tv = t.d->tv[level];
     const int xlimit = xblock * ((t.h + xblock - 1) / xblock);
     FOR_TILE_PIXELS(S, t, x, y)
     {
         const long pos = (long)y * t.s + x;
+        // the I0 half of every warp's derivative at this level (op_deriv)
+        i0d[pos] = deriv_along(I0, tv, t.w, t.h, t.s, x, y);
         float a = 0.f, b = 0.f, c = 0.f;
         const bool active = MASKED ? (mask[pos] != 0.f) : true;
         if (active && x < xlimit) {
@@ -481,28 +502,17 @@ __device__ __forceinline__ void op_prep(const Tile &t, int level, int roles)
     }
 }
 
-// computeDerivatives.cu:88-129 at pixel (x, y): Iu, the 5-point derivative along
-// t = tv/|tv| of I0 and I1w, averaged, and Iz = I1w - I0 at the texel centre.
+// computeDerivatives.cu:88-129 at pixel (x, y): Iu = (I0's derivative along the trajectory,
+// computed once per level by the tensor op into PL_I0D, + I1w's) / 2 and Iz = I1w - I0 at
+// the texel centre.
 template <typename T>
 __device__ __forceinline__ float2 deriv_px(const T *__restrict__ I0, const T *__restrict__ I1w,
-                                           const float2 *__restrict__ tv, int w, int h, int s, int x, int y)
+                                           const float *__restrict__ i0d, const float2 *__restrict__ tv, int w, int h,
+                                           int s, int x, int y)
 {
     const long pos = (long)y * s + x;
-    const float2 v = tv[pos];
-    const float r = sqrt_rn(v.x * v.x + v.y * v.y);
-    const float dx = v.x / r, dy = v.y / r;
-    const float X = (float)x, Y = (float)y;
-    float t0 = bil(I0, w, h, s, X - 2.f * dx, Y - 2.f * dy);
-    t0 -= bil(I0, w, h, s, X - dx, Y - dy) * 8.f;
-    t0 += bil(I0, w, h, s, X + dx, Y + dy) * 8.f;
-    t0 -= bil(I0, w, h, s, X + 2.f * dx, Y + 2.f * dy);
-    t0 /= 12.f;
-    float t1 = bil(I1w, w, h, s, X - 2.f * dx, Y - 2.f * dy);
-    t1 -= bil(I1w, w, h, s, X - dx, Y - dy) * 8.f;
-    t1 += bil(I1w, w, h, s, X + dx, Y + dy) * 8.f;
-    t1 -= bil(I1w, w, h, s, X + 2.f * dx, Y + 2.f * dy);
-    t1 /= 12.f;
-    return make_float2((t0 + t1) * 0.5f, ldT(I1w, pos) - ldT(I0, pos));
+    const float t1 = deriv_along(I1w, tv, w, h, s, x, y);
+    return make_float2((i0d[pos] + t1) * 0.5f, ldT(I1w, pos) - ldT(I0, pos));
 }
 
 template <class S, typename T>
@@ -516,7 +526,7 @@ __device__ __forceinline__ void op_deriv(const Tile &t, int level)
     FOR_TILE_PIXELS(S, t, x, y)
     {
         const long pos = (long)y * t.s + x;
-        const float2 d = deriv_px(I0, I1w, tv, t.w, t.h, t.s, x, y);
+        const float2 d = deriv_px(I0, I1w, slot(t, PL_I0D), tv, t.w, t.h, t.s, x, y);
         stT(Iu, pos, d.x);
         stT(Iz, pos, d.y);
     }
@@ -528,8 +538,8 @@ __device__ __forceinline__ void op_deriv(const Tile &t, int level)
 template <typename T>
 __device__ __forceinline__ float2 staged_deriv(const Tile &t, int level, int x, int y, bool in_tile)
 {
-    const float2 d = deriv_px((const T *)t.d->I0[level], (const T *)slot(t, PL_I1W), t.d->tv[level], t.w, t.h,
-                              t.s, x, y);
+    const float2 d = deriv_px((const T *)t.d->I0[level], (const T *)slot(t, PL_I1W), slot(t, PL_I0D),
+                              t.d->tv[level], t.w, t.h, t.s, x, y);
     const float iu = Cv<T>::f(Cv<T>::t(d.x)), iz = Cv<T>::f(Cv<T>::t(d.y));
     if (in_tile) {
         const long pos = (long)y * t.s + x;

@@ -168,7 +168,8 @@ enum Plane : int {
     PL_TV2 = 27,             // tv2 (float2)
     PL_UV0 = 29, PL_UV1 = 31,  // warpUV ping-pong (float2)
     PL_Q = 33,               // q: 3 buffers (4 x st)
-    PL_COUNT = 45
+    PL_I0D = 45,             // the I0 half of the derivative along the trajectory (float; per level)
+    PL_COUNT = 46
 };
 
 // Ping-pong groups and their roles.  `roles` holds, for group g, the buffer index (0..2)
