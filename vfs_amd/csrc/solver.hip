@@ -466,8 +466,9 @@ void apply_tuning(Program &pr, const Tuning &tn)
     }
     // A warp's first inner op computes the warp's derivatives at its staged positions
     // (EF_DERIV) in place of the OP_DERIV op before it, where that op stages a halo (a
-    // multi-iteration or persistent op) with at most two positions per work-item, and the
-    // OP_DERIV op does not also sync the spare buffers (EF_SYNC).
+    // multi-iteration or persistent op) with at most two positions per work-item (three in
+    // the whole-warp persistent op, which stages once per warp), and the OP_DERIV op does
+    // not also sync the spare buffers (EF_SYNC).
     for (size_t i = 0; i + 1 < pr.ops.size(); i++) {
         const ProgOp &d = pr.ops[i];
         ProgOp &f = pr.ops[i + 1];
@@ -476,7 +477,8 @@ void apply_tuning(Program &pr, const Tuning &tn)
         if (vfs::op_multi(f.kind)) K = vfs::op_iterations(f.kind);
         else if (f.kind == vfs::OP_ITERP) K = 1;
         else if (f.kind == vfs::OP_ITERPK) K = vfs::iterp_block(f.it0);
-        if (K == 0 || vfs::staged_per_item(vfs::shape_index(f.shape), K, (f.shape & vfs::kHalfWorkgroup) != 0) > 2)
+        if (K == 0 || vfs::staged_per_item(vfs::shape_index(f.shape), K, (f.shape & vfs::kHalfWorkgroup) != 0) >
+                          (f.kind == vfs::OP_ITERP ? 3 : 2))
             continue;
         f.flags |= vfs::EF_DERIV;
         pr.ops.erase(pr.ops.begin() + (long)i);

@@ -321,8 +321,9 @@ enum EntryFlag : int {
                            // derivatives (OP_DERIV's work) computed at every staged position
                            // and stored for the tile, in place of an OP_DERIV op (the host
                            // folds it where the kernel stages at most two positions per
-                           // work-item: with three, the derivatives' dependent loads take
-                           // longer than the OP_DERIV launch)
+                           // work-item, three in OP_ITERP: in the K-iteration kernels with
+                           // three the derivatives' dependent loads would be taken before the
+                           // staging of every launch that carries the flag)
 };
 
 // One (pair, op) of a launch.  The geometry and arena of the op's tile level travel in
