@@ -1152,8 +1152,8 @@ struct PersistK {
 // mask) from its neighbours' words; outside the mask nothing changes.  Words carry tags,
 // are double-buffered by block parity and every tile waits for all its neighbours'
 // progress words, as in OP_ITERP.  Only the last block stores the state (other and spare
-// buffers, as the K-iteration op does), so the result equals blocks x K iterations run one
-// block per launch.
+// buffers, as the K-iteration op does), so the result equals the same iterations run one
+// per launch.
 template <int SH, int K, bool HALF, bool MASKED, typename T, bool PERSIST = false>
 __device__ __forceinline__ void op_iterk(const Tile &t, int roles, const float4 *__restrict__ sched,
                                          const IterParams &ip0, LdsK<SH, K> &L, const PersistK &pk = PersistK())
@@ -1686,7 +1686,8 @@ void k_iterk(const PairDesc *__restrict__ D, const TickArgs A)
 }
 
 // The blocked persistent op (OP_ITERPK): one pair per launch, one workgroup per tile (tiles
-// in XCD-contiguous runs as the other op kernels), blocks of K iterations.
+// in XCD-contiguous runs as the other op kernels), an even number of iterations in blocks
+// of K (the last one shortened to the rest).
 template <typename T, int SH, int K>
 __global__ __launch_bounds__((BlkK<SH, K, false>::NT)) void k_iterkp(const PairDesc *__restrict__ D, const TickArgs A)
 {
