@@ -250,3 +250,18 @@ def test_blocked_persistent_iteration_counts(iters):
         print(f"blocked persistent iterations {iters} K {K}", flush=True)
         g.solveStereoForwardMasked()
         assert_same(planes_of(g), ref, f"{iters} iterations, K {K}")
+
+
+def test_tuned_848x800_solve_equals_one_iteration_per_launch():
+    """The tuner's mixture at 848x800 (whole-warp persistent launches at level 0, blocked
+    or whole-warp ones and eight-iteration launches below, each warp's derivatives in its
+    first launch) equals one iteration per launch bit for bit, with no re-run."""
+    pair = S.make_pair(848, 800, seed=5, radius=400, dmax=40)
+    ref = planes_of(solve(pair, 5, 10, 50, persist=0, per_launch=1))
+    g = make_gpu(pair, 5, 10, 50)
+    g.solveStereoForwardMasked()
+    tun = [g.level_tuning(l, True, with_per_launch=True) for l in range(5)]
+    print(f"848x800 tuned launch choices: {tun}", flush=True)
+    assert tun[0][2] & StereoTgv.PERSISTENT
+    assert_same(planes_of(g), ref, "tuned 848x800")
+    assert g.get_option("persist_reruns") == 0, hex(g.get_option("persist_why"))
