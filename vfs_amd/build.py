@@ -89,10 +89,21 @@ def build_dropin(verbose=False) -> str | None:
     return out
 
 
+def build_order_check(verbose=False) -> str | None:
+    """Host program that checks the persistent op's work-item order (vfs_engine.h)."""
+    src = os.path.join(ROOT, "tests", "persist_order", "check_persist_order.cpp")
+    if not os.path.exists(src):
+        return None
+    out = os.path.join(ROOT, "tests", "persist_order", "check_persist_order")
+    _run([HIPCC, "-O1", "-std=c++17", "-Wall", src, "-o", out], verbose)
+    return out
+
+
 def build_all(verbose=False):
     build_lib(verbose)
     build_dllhook(verbose)
     build_dropin(verbose)
+    build_order_check(verbose)
 
 
 if __name__ == "__main__":
