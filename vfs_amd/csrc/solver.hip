@@ -846,9 +846,9 @@ LevelTune default_tune(int level)
 }
 
 // Persistent candidates of a single pair's tuning: the iterations of a warp (within the
-// schedule table; a launch takes the even count, or the multiple of its K, below it), per
-// level, form (K = 1, 2, 4, 8) and shape the tiles of the launch (0: not possible there),
-// and the pair's launch bookkeeping.
+// schedule table; a launch takes the even count below it), per level, form (K = 1, 2, 4, 8)
+// and shape the tiles of the launch (0: not possible there), and the pair's launch
+// bookkeeping.
 struct PersistCands {
     int iters = 0;
     std::vector<std::array<std::array<int, vfs::kShapes>, 4>> tiles;  // [level index][log2 K][shape]
@@ -857,7 +857,7 @@ struct PersistCands {
 constexpr int kPersistK[4] = {1, 2, 4, 8};
 // Iterations of one persistent launch (OP_ITERP, or OP_ITERPK in blocks of K with a
 // shorter last block) out of `iters`: the even count.
-int persist_launch_iters(int iters, int /*K*/) { return iters & ~1; }
+int persist_launch_iters(int iters) { return iters & ~1; }
 
 // Times the inner iteration of every level in `levels` in each (tile shape, Tp preload)
 // on the launch set `descs` (their arenas point at scratch memory; `d_desc` is any valid
@@ -874,7 +874,6 @@ hipError_t time_launch_choices(hipStream_t st, const vfs::PairDesc *d_desc, cons
     constexpr float kRefine = 1.10f;
     // candidates: (shape, preload) of one iteration per launch, then two-, four- and
     // eight-iteration launches for the shapes that have them, in full or half workgroups
-    // (then the persistent op in each shape)
     // (then the persistent ops, OP_ITERP and OP_ITERPK in blocks of 2, 4 and 8, in each shape)
     constexpr int kSingle = vfs::kTickShapes * 2, kMulti = kSingle + 3 * vfs::kShapes * 2;
     constexpr int kCand = kMulti + 4 * vfs::kShapes;
@@ -883,7 +882,7 @@ hipError_t time_launch_choices(hipStream_t st, const vfs::PairDesc *d_desc, cons
             shape = (c - kMulti) % vfs::kShapes;
             pre = 0;
             half = 0;
-            per_launch = pc ? persist_launch_iters(pc->iters, kPersistK[(c - kMulti) / vfs::kShapes]) : 0;
+            per_launch = pc ? persist_launch_iters(pc->iters) : 0;
         } else if (c < kSingle) {
             shape = c / 2;
             pre = c % 2;
@@ -1018,10 +1017,9 @@ hipError_t time_launch_choices(hipStream_t st, const vfs::PairDesc *d_desc, cons
             pick.mhalf = (unsigned char)half;
             pick_ms = best[li * kCand + bestm];
         }
-        // the persistent op where it is clearly faster than both over a warp's iterations
-        // (it runs all of them; the launches above leave a remainder to two-iteration or
-        // one-iteration launches) (the launches above stay the level's choice wherever it
-        // cannot run)
+        // the persistent op where it is clearly faster than both over a warp's iterations:
+        // it runs all of them, where the launches above leave a remainder to two-iteration
+        // or one-iteration launches (those stay the level's choice wherever it cannot run)
         int bestp = -1;
         for (int c = kMulti; c < kCand; c++)
             if (best[li * kCand + c] < 1e29f && (bestp < 0 || best[li * kCand + c] < best[li * kCand + bestp])) bestp = c;
@@ -1164,7 +1162,7 @@ bool persist_program(const vfs_solver *s, int masked, const Tuning &tn, size_t *
         if (!tn.lv[l].persist) continue;
         const int K = tn.lv[l].pblock;
         const int n = persist_tiles(s, masked, l, tn.lv[l].pshape, K);
-        if (n == 0 || persist_launch_iters(std::min(s->nSolverIters, vfs::kMaxSched), K) < 2) continue;
+        if (n == 0 || persist_launch_iters(std::min(s->nSolverIters, vfs::kMaxSched)) < 2) continue;
         any = true;
         *words = std::max(*words, persist_words_of(n, tn.lv[l].pshape, K));
     }
@@ -1975,8 +1973,8 @@ int vfs_time_inner_iterations(vfs_solver *s, int level, int reps, float *ms_tota
     const int pb = lt.pblock;
     const bool persist = persist_program(s, 1, s->tuning[1], &words) && lt.persist &&
                          persist_tiles(s, 1, level, lt.pshape, pb) > 0 &&
-                         persist_launch_iters(std::min(s->nSolverIters, vfs::kMaxSched), pb) >= 2;
-    const int sched_n = persist ? persist_launch_iters(std::min(s->nSolverIters, vfs::kMaxSched), pb) : 48;
+                         persist_launch_iters(std::min(s->nSolverIters, vfs::kMaxSched)) >= 2;
+    const int sched_n = persist ? persist_launch_iters(std::min(s->nSolverIters, vfs::kMaxSched)) : 48;
     if (persist) {
         per_launch = sched_n;
         if ((rc = ensure_xbuf(s, words))) return rc;
