@@ -200,7 +200,10 @@ int vfs_get_energy(vfs_solver *s, int level, int warp, vfs_energy_t *out);
  *             for it; vfs_get_option "persist_reruns" counts those runs, and
  *             "persist_why" tells what the last launch that gave up waited for (its tile
  *             count << 12 | 1 arrivals, 2 ring values, 3 a neighbour's progress << 8 |
- *             iteration).
+ *             iteration);
+ *   "persist_give_up" 1 = the next solve's persistent launches find their abort flag set
+ *             and leave at once, so that the solve runs again without them (tests of
+ *             that path).
  * Results do not depend on "graph", "autotune", "shape", "per_launch", "pair" or
  * "persist".
  * No counterpart in the reference (FP32 only, stereotgv.h:47-100; its block shape is a

@@ -265,3 +265,37 @@ def test_tuned_848x800_solve_equals_one_iteration_per_launch():
     assert tun[0][2] & StereoTgv.PERSISTENT
     assert_same(planes_of(g), ref, "tuned 848x800")
     assert g.get_option("persist_reruns") == 0, hex(g.get_option("persist_why"))
+
+
+@pytest.mark.parametrize("iters", [6, 15])
+def test_persistent_launches_that_give_up_run_again(iters):
+    """A solve whose persistent launches give up (forced from the host: option
+    persist_give_up sets their abort flag, every workgroup leaves at once) runs again
+    without them at the next call that waits for it, and that call reads the re-run's
+    buffers, whichever copy-out it is: 6 iterations per warp are one persistent launch (one
+    role rotation) against a four- and a two-iteration launch (two) in the re-run."""
+    pair = small_pair(200, 120, seed=73)
+    ref = solve(pair, 3, 2, iters, persist=0, per_launch=1)
+    ref_planes, ref_1d = planes_of(ref, (0, 1)), ref.copy1DDisparityToHost()
+    g = make_gpu(pair, 3, 2, iters)
+    for lvl in range(3):
+        g.set_level_tuning(lvl, 4 if lvl < 2 else 6, per_launch=4)
+    set_persistent(g, 7)
+    g.solveStereoForwardMasked()
+    assert g.get_option("persist_reruns") == 0, hex(g.get_option("persist_why"))
+    for round_, first in enumerate(("1d", "flow", "planes")):
+        g.set_option("persist_give_up", 1)
+        g.copyImagesToDevice(pair["i0"], pair["i1"])
+        g.solveStereoForwardMasked()
+        print(f"give up: iterations {iters}, read first: {first}", flush=True)
+        if first == "1d":
+            assert np.array_equal(g.copy1DDisparityToHost(), ref_1d, equal_nan=True)
+        elif first == "flow":
+            assert np.array_equal(g.copyDisparityToHost(), ref_planes["flow"], equal_nan=True)
+        assert_same(planes_of(g, (0, 1)), ref_planes, f"re-run, {first} read first")
+        assert g.get_option("persist_reruns") == round_ + 1
+    # and the next solve runs its persistent launches again
+    g.copyImagesToDevice(pair["i0"], pair["i1"])
+    g.solveStereoForwardMasked()
+    assert_same(planes_of(g, (0, 1)), ref_planes, "after the re-runs")
+    assert g.get_option("persist_reruns") == 3

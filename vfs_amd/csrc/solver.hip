@@ -1586,6 +1586,8 @@ int vfs_copy_disparity_1d(vfs_solver *s, float *out, int pitch)
     if (!out || pitch < s->width) return fail(s, VFS_ERR_ARG, "bad output pointer or pitch");
     int rc = set_device(s);
     if (rc) return rc;
+    // a solve that runs again (persistent launches that gave up) may end with other roles
+    if ((rc = resolve_persist(s))) return rc;
     return copy_result(s, slot_ptr(s, vfs::slot_u(vfs::rcur(s->roles, vfs::G_U))), 1, 0, out, pitch);
 }
 
@@ -1596,6 +1598,7 @@ int vfs_copy_flow(vfs_solver *s, float *out, int pitch)
     if (!out || pitch < 2 * s->width) return fail(s, VFS_ERR_ARG, "bad output pointer or pitch");
     int rc = set_device(s);
     if (rc) return rc;
+    if ((rc = resolve_persist(s))) return rc;  // (roles, as above)
     return copy_result(s, slot_ptr(s, vfs::slot_uv(s->roles, true)), 2, 0, out, pitch);
 }
 
@@ -1724,6 +1727,10 @@ int vfs_read_plane(vfs_solver *s, const char *name, int level, float *out)
     if (!s || !name || !out) return VFS_ERR_ARG;
     if (!s->initialized) return fail(s, VFS_ERR_STATE, "read before initialize");
     if (level < 0 || level >= s->nLevels) return fail(s, VFS_ERR_ARG, "level %d out of range", level);
+    int rc = set_device(s);
+    if (rc) return rc;
+    // a solve that runs again (persistent launches that gave up) may end with other roles
+    if ((rc = resolve_persist(s))) return rc;
     const int r = s->roles;
     const std::string n(name);
     const void *src = nullptr;
@@ -1753,8 +1760,6 @@ int vfs_read_plane(vfs_solver *s, const char *name, int level, float *out)
     else if (n == "v_.other") { src = slot_ptr(s, slot_v(roth(r, G_V))); nch = 2; }
     else if (n == "q") { src = slot_ptr(s, slot_q(rcur(r, G_PQ))); nch = 4; typed = true; }
     if (!src) return fail(s, VFS_ERR_ARG, "unknown plane '%s'", name);
-    int rc = set_device(s);
-    if (rc) return rc;
     rc = typed ? copy_out_st(s, src, nch, level, out, s->lv[level].w * nch)
                : copy_out(s, src, nch, level, out, s->lv[level].w * nch);
     return rc ? rc : nch;
@@ -1841,6 +1846,20 @@ int vfs_set_option(vfs_solver *s, const char *key, int value)
         if (value < -1 || value > 1) return fail(s, VFS_ERR_ARG, "persist must be -1, 0 or 1");
         s->persist_opt = value;
         s->tuned[0] = s->tuned[1] = false;
+        return VFS_OK;
+    }
+    if (!strcmp(key, "persist_give_up")) {
+        // the next solve's persistent launches find the abort flag set and leave at once;
+        // the solve then runs again without them (the re-run path, for tests)
+        if (value != 1) return fail(s, VFS_ERR_ARG, "persist_give_up takes 1");
+        if (!s->initialized) return fail(s, VFS_ERR_STATE, "persist_give_up before initialize");
+        int rc = set_device(s);
+        if (!rc) rc = resolve_persist(s);  // settle the last solve first
+        if (!rc) rc = ensure_xbuf(s, 0);
+        if (rc) return rc;
+        const unsigned one = 1;
+        HIPCHK(s, hipMemcpyAsync(&s->xsync->abort, &one, sizeof one, hipMemcpyHostToDevice, s->stream));
+        HIPCHK(s, hipStreamSynchronize(s->stream));
         return VFS_OK;
     }
     return fail(s, VFS_ERR_ARG, "unknown option '%s'", key);
