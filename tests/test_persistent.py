@@ -294,8 +294,17 @@ def test_persistent_launches_that_give_up_run_again(iters):
             assert np.array_equal(g.copyDisparityToHost(), ref_planes["flow"], equal_nan=True)
         assert_same(planes_of(g, (0, 1)), ref_planes, f"re-run, {first} read first")
         assert g.get_option("persist_reruns") == round_ + 1
+    # a parameter changed before the results are read does not reach the re-run
+    g.set_option("persist_give_up", 1)
+    g.copyImagesToDevice(pair["i0"], pair["i1"])
+    g.solveStereoForwardMasked()
+    limit = g.limitRange
+    g.limitRange = limit * 0.5
+    assert_same(planes_of(g, (0, 1)), ref_planes, "re-run after a parameter change")
+    assert g.get_option("persist_reruns") == 4
+    g.limitRange = limit
     # and the next solve runs its persistent launches again
     g.copyImagesToDevice(pair["i0"], pair["i1"])
     g.solveStereoForwardMasked()
     assert_same(planes_of(g, (0, 1)), ref_planes, "after the re-runs")
-    assert g.get_option("persist_reruns") == 3
+    assert g.get_option("persist_reruns") == 4

@@ -214,6 +214,7 @@ struct vfs_solver {
     // its solve form, to run it again without them if a launch gave up
     bool persist_check = false;
     int persist_masked = 0;
+    Params persist_params{};  // the parameters of that solve (its re-run uses them)
     int persist_reruns = 0;  // solves run again without persistent launches
     unsigned persist_why = 0;  // XSync::why of the last launch that gave up
     int roles = vfs::kRolesInit;  // buffer roles after the last solve
@@ -1171,9 +1172,10 @@ bool persist_program(const vfs_solver *s, int masked, const Tuning &tn, size_t *
 
 // Builds and runs the solve program on the solver's stream and records its end.  A cold
 // solve runs the persistent launches of its tuning when `allow_persist`.
-int run_solve(vfs_solver *s, int masked, int warm_level, int warps, int iters, bool allow_persist)
+int run_solve(vfs_solver *s, int masked, int warm_level, int warps, int iters, bool allow_persist,
+              const Params *params = nullptr)
 {
-    const Params P = s->params();
+    const Params P = params ? *params : s->params();
     const Tuning &tn = s->tuning[masked ? 1 : 0];
     size_t words = 0;
     const bool persist = allow_persist && warm_level < 0 && persist_program(s, masked, tn, &words);
@@ -1195,6 +1197,7 @@ int run_solve(vfs_solver *s, int masked, int warm_level, int warps, int iters, b
     s->numeric = 0;
     s->persist_check = persist;
     s->persist_masked = masked;
+    s->persist_params = P;
     HIPCHK(s, hipEventRecord(s->done, s->stream));
     return VFS_OK;
 }
@@ -1211,7 +1214,7 @@ int resolve_persist(vfs_solver *s)
     int rc = persist_aborted(s, &aborted);
     if (rc || !aborted) return rc;
     s->persist_reruns++;
-    return run_solve(s, s->persist_masked, -1, 0, 0, false);
+    return run_solve(s, s->persist_masked, -1, 0, 0, false, &s->persist_params);
 }
 
 // warm_level < 0: cold solve; otherwise a warm start at warm_level with warps/iters.
@@ -1800,6 +1803,12 @@ int vfs_get_energy(vfs_solver *s, int level, int warp, vfs_energy_t *out)
 int vfs_set_option(vfs_solver *s, const char *key, int value)
 {
     if (!s || !key) return VFS_ERR_ARG;
+    if (s->persist_check) {
+        // a solve whose persistent launches may have given up runs again as it was set up
+        int rc = set_device(s);
+        if (!rc) rc = resolve_persist(s);
+        if (rc) return rc;
+    }
     if (!strcmp(key, "graph")) {
         s->use_graph = value != 0;
         return VFS_OK;
