@@ -308,3 +308,25 @@ def test_persistent_launches_that_give_up_run_again(iters):
     g.solveStereoForwardMasked()
     assert_same(planes_of(g, (0, 1)), ref_planes, "after the re-runs")
     assert g.get_option("persist_reruns") == 4
+
+
+def test_persistent_level_tuning_rejects_what_cannot_run():
+    """VFS_PERSISTENT | K takes K = 2, 4 or 8 in a shape that has the blocked form, no
+    persistent choice while option persist is 0; the solver stays usable after the errors."""
+    from vfs_amd import VfsError
+    pair = small_pair(128, 96, seed=79)
+    ref = planes_of(solve(pair, 2, 1, 6, persist=0, per_launch=1))
+    g = make_gpu(pair, 2, 1, 6)
+    with pytest.raises(VfsError):
+        g.set_level_tuning(0, 7, per_launch=StereoTgv.PERSISTENT | 3)
+    with pytest.raises(VfsError):
+        g.set_level_tuning(0, 8, per_launch=StereoTgv.PERSISTENT | 4)  # 4x4 tiles, blocks of 4
+    with pytest.raises(VfsError):
+        g.set_level_tuning(0, 0, per_launch=StereoTgv.PERSISTENT)  # 64x8: no persistent form
+    g.set_option("persist", 0)
+    with pytest.raises(VfsError):
+        g.set_level_tuning(0, 7, per_launch=StereoTgv.PERSISTENT)
+    g.set_option("persist", -1)
+    g.set_level_tuning(0, 7, per_launch=StereoTgv.PERSISTENT | 2)
+    g.solveStereoForwardMasked()
+    assert_same(planes_of(g), ref, "after the rejected choices")
