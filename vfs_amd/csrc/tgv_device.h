@@ -989,7 +989,8 @@ __device__ __forceinline__ unsigned long long rtime() { return __builtin_amdgcn_
 // Every workgroup of a persistent launch must be resident: it counts its arrival and waits
 // (bounded) for the others; s_word[0] receives the launch counter, s_word[1] whether to
 // leave at once (a launch gave up since the host last cleared the flag, or not every
-// workgroup arrived in time).
+// workgroup arrived in time), and s_word[2], the workgroup's give-up flag
+// (persist_any), starts clear.
 __device__ __forceinline__ void persist_arrive(XSync *__restrict__ xs, int ntiles, unsigned *s_word)
 {
     if (threadIdx.x == 0) {
@@ -1009,8 +1010,22 @@ __device__ __forceinline__ void persist_arrive(XSync *__restrict__ xs, int ntile
         }
         s_word[0] = ep;
         s_word[1] = ab;
+        s_word[2] = 0u;
     }
     __syncthreads();
+}
+
+// Whether some work-item of the workgroup gave up a wait (`bad` != 0), as a barrier: the
+// work-items that did set the workgroup's flag (LDS) before it, and every work-item reads
+// it after it.  The flag is never cleared: the workgroup leaves once it is set.  Every
+// call is followed by another barrier before the next one (the launches' phase barriers),
+// so no work-item sets the flag for a later call while another still reads it for this
+// one.  One barrier, where a workgroup reduction (__syncthreads_or) takes three.
+__device__ __forceinline__ bool persist_any(int bad, unsigned *s_word)
+{
+    if (bad) s_word[2] = 1u;
+    __syncthreads();
+    return s_word[2] != 0u;
 }
 
 // A work-item that gave up waiting (`bad`: 2 ring values, 3 a neighbour's progress word)
@@ -1619,7 +1634,7 @@ __device__ __forceinline__ void op_iterk(const Tile &t, int roles, const float4 
                     L.v[i] = o[k].v;
                     L.q[i] = make_float4(f[6], f[7], f[8], f[9]);
                 }
-                if (__syncthreads_or(bad)) {
+                if (persist_any(bad, pk.s_word)) {
                     persist_give_up(pk.xs, pk.ntiles, bad, blk);
                     gave_up = true;
                     break;
@@ -1693,7 +1708,7 @@ __global__ __launch_bounds__((BlkK<SH, K, false>::NT)) void k_iterkp(const PairD
 {
     using B = BlkK<SH, K, false>;
     __shared__ LdsK<SH, K> L;
-    __shared__ unsigned s_word[2];
+    __shared__ unsigned s_word[3];
     const int nb = gridDim.x, i = blockIdx.x, xcd = i & 7, q = nb >> 3, r = nb & 7;
     const int b = (xcd < r ? xcd * (q + 1) : r * (q + 1) + (xcd - r) * q) + (i >> 3);
     const TickEntry &E = A.e[0];
@@ -2027,52 +2042,17 @@ __device__ __forceinline__ void op_iterp(const Tile &t, int roles, const float4 
         }
         unsigned long long *__restrict__ mine = xbuf + (long)tile * 2 * X::WORDS;
         const unsigned tag0 = epoch << 8;
-        // this work-item's exchange words, loaded at the end of each iteration (before its
-        // last barrier) and checked at the start of the next, polled again while older
+        // this work-item's exchange words, loaded at the end of each iteration
         const unsigned long long *rw = xbuf + (long)(rcv_w & 0x3fffffff);
         const int rst = (rcv_w >> 30) ? TH : TW;
-        unsigned long long pa0 = 0, pa1 = 0, pa2 = 0;
 
         // One iteration.  MODE 0: an iteration before the op's last two; 1: iteration
         // n - 2 (u, u_, v_ to the other buffers); 2: the last one (the whole state to the
-        // spare buffers).  Returns false when a wait gave up.
+        // spare buffers).  Every iteration but the last ends by refreshing the ring for the
+        // next one.  Returns false when a wait gave up.
         auto step = [&](int j, auto full_c, auto mode_c) -> bool {
             constexpr bool FULL = decltype(full_c)::value;
             constexpr int MODE = decltype(mode_c)::value;
-            int bad = 0;
-            if (j > 0) {
-                // ---- refresh the ring from the neighbours' strips of iteration j - 1 (their
-                // primal steps published them)
-                const unsigned tg = tag0 | (unsigned)(j - 1);
-                const unsigned long long *pw = rw + ((j - 1) & 1) * X::WORDS;
-                if (rcv_i == -2) {
-                    // a neighbour's progress word
-                    if ((unsigned)(pa0 >> 32) != tg && !persist_wait_tag(pw, tg, xs)) bad = 3;
-                } else if (rcv_i >= 0) {
-                    unsigned long long a0 = pa0, a1 = pa1, a2 = pa2, t0 = 0;
-                    for (int polls = 0;; polls++) {
-                        if ((unsigned)(a0 >> 32) == tg && (unsigned)(a1 >> 32) == tg && (unsigned)(a2 >> 32) == tg)
-                            break;
-                        a0 = xload(pw);
-                        a1 = xload(pw + rst);
-                        a2 = xload(pw + 2 * rst);
-                        if (polls == 0) t0 = rtime();
-                        if ((polls & 63) == 63 && (rtime() - t0 > kPollTicks || xload32(&xs->abort) != 0)) {
-                            bad = 2;
-                            break;
-                        }
-                    }
-                    L.ub[rcv_i] = __uint_as_float((unsigned)a0);
-                    L.v[rcv_i] = make_float2(__uint_as_float((unsigned)a1), __uint_as_float((unsigned)a2));
-                }
-            }
-            if (__syncthreads_or(bad)) {
-                persist_give_up(xs, ntiles, bad, j);
-                return false;
-            }
-            // progress: this tile has read its neighbours' words of iteration j - 1, which
-            // their primal steps of iteration j + 1 overwrite
-            if (MODE != 2 && tid == NT - 1) xstore(mine + (j & 1) * X::WORDS + X::HB, 0.f, tag0 | (unsigned)j);
             IterParams ip = ip0;
             const float4 sc = sched[j];
             ip.k1 = sc.w;
@@ -2204,18 +2184,43 @@ __device__ __forceinline__ void op_iterp(const Tile &t, int roles, const float4 
                     slotp<float2>(t, slot_v(rspare(roles, G_V)))[pos] = vs;
                 }
             }
-            // the words the next iteration reads (the neighbours published them while
-            // computing this iteration)
-            if (MODE != 2 && rcv_i != -1) {
+            if constexpr (MODE != 2) {
+                // ---- refresh the ring from the neighbours' strips of this iteration (their
+                // primal steps publish them as this one's does); the ring positions are
+                // not read between the barriers around the primal steps.  The barrier of
+                // persist_any then also serves the next iteration's duals, which read this
+                // one's u_ and v_ from LDS.
+                int bad = 0;
+                const unsigned tg = tag0 | (unsigned)j;
                 const unsigned long long *pw = rw + (j & 1) * X::WORDS;
-                pa0 = xload(pw);
-                if (rcv_i >= 0) {
-                    pa1 = xload(pw + rst);
-                    pa2 = xload(pw + 2 * rst);
+                if (rcv_i == -2) {
+                    // a neighbour's progress word
+                    if (!persist_wait_tag(pw, tg, xs)) bad = 3;
+                } else if (rcv_i >= 0) {
+                    unsigned long long a0, a1, a2, t0 = 0;
+                    for (int polls = 0;; polls++) {
+                        a0 = xload(pw);
+                        a1 = xload(pw + rst);
+                        a2 = xload(pw + 2 * rst);
+                        if ((unsigned)(a0 >> 32) == tg && (unsigned)(a1 >> 32) == tg && (unsigned)(a2 >> 32) == tg)
+                            break;
+                        if (polls == 0) t0 = rtime();
+                        if ((polls & 63) == 63 && (rtime() - t0 > kPollTicks || xload32(&xs->abort) != 0)) {
+                            bad = 2;
+                            break;
+                        }
+                    }
+                    L.ub[rcv_i] = __uint_as_float((unsigned)a0);
+                    L.v[rcv_i] = make_float2(__uint_as_float((unsigned)a1), __uint_as_float((unsigned)a2));
                 }
+                if (persist_any(bad, s_word)) {
+                    persist_give_up(xs, ntiles, bad, j + 1);
+                    return false;
+                }
+                // progress: this tile has read its neighbours' words of iteration j, which
+                // their primal steps of iteration j + 2 overwrite
+                if (tid == NT - 1) xstore(mine + ((j + 1) & 1) * X::WORDS + X::HB, 0.f, tag0 | (unsigned)(j + 1));
             }
-            // the next iteration's duals read this one's u_ and v_ from LDS
-            if (MODE != 2) __syncthreads();
             return true;
         };
         auto run = [&](auto full_c) {
@@ -2226,6 +2231,8 @@ __device__ __forceinline__ void op_iterp(const Tile &t, int roles, const float4 
                 if (!step(j, full_c, I0{})) return;
             if (step(n_iter - 2, full_c, I1{})) step(n_iter - 1, full_c, I2{});
         };
+        // progress of iteration 0: no words read yet (the staging covered the ring)
+        if (tid == NT - 1) xstore(mine + X::HB, 0.f, tag0);
         if (full) run(std::true_type{});
         else run(std::false_type{});
     }
@@ -2238,7 +2245,7 @@ __global__ __launch_bounds__(BlkP<SH>::NT) void k_iterp(const PairDesc *__restri
 {
     using B = BlkP<SH>;
     __shared__ LdsP<SH> L;
-    __shared__ unsigned s_word[2];
+    __shared__ unsigned s_word[3];
     // tiles in XCD-contiguous runs (bands of rows), as the other op kernels
     const int nb = gridDim.x, i = blockIdx.x, xcd = i & 7, q = nb >> 3, r = nb & 7;
     const int b = (xcd < r ? xcd * (q + 1) : r * (q + 1) + (xcd - r) * q) + (i >> 3);
