@@ -953,6 +953,14 @@ __device__ __forceinline__ void op_iter(const Tile &t, int roles, const IterPara
 // Bounded waits of the persistent op, in ticks of the 100 MHz real-time counter.
 constexpr unsigned long long kArriveTicks = 500000ull;  // every workgroup resident: 5 ms
 constexpr unsigned long long kPollTicks = 5000000ull;   // a neighbour's words: 50 ms
+// Where a work-item stages one position, its neighbours publish their ring words as it
+// finishes its own step, and a load issued at once mostly finds them stale and costs
+// another round trip through the device-coherent cache (0.6-1 us between two workgroups,
+// measured).  Such work-items first sleep this long (s_sleep units of 64 clocks, about
+// 0.4 us) before their first load: measured at 848x800, 2.9-3.6 -> 2.6-2.7 us per
+// iteration in 27x25 tiles, 3.3 -> 2.3 in 14x13, 2.5-2.7 -> 2.0-2.1 in 8x8 and 7x7 (one
+// iteration per exchange), 1.66 -> 1.49 in 8x8 blocks of four.
+constexpr int kRingDelay = 16;
 
 // The exchange words and the bookkeeping are device memory reached through descriptor
 // pointers: accessed as global (not flat) memory, so that waiting for LDS operations
@@ -1566,6 +1574,7 @@ __device__ __forceinline__ void op_iterk(const Tile &t, int roles, const float4 
                 // the words the next block's start reads
                 if (r == 0) {
                     const int par = blk & 1;
+                    if (xr[0] >= 0) __builtin_amdgcn_s_sleep(kRingDelay);
                     if (xr[0] >= 0) {
 #pragma unroll
                         for (int v = 0; v < 10; v++) pf[v] = xload(pk.xbuf + xr[0] + par * X::WORDS + v * X::VP);
@@ -2198,6 +2207,7 @@ __device__ __forceinline__ void op_iterp(const Tile &t, int roles, const float4 
                     if (!persist_wait_tag(pw, tg, xs)) bad = 3;
                 } else if (rcv_i >= 0) {
                     unsigned long long a0, a1, a2, t0 = 0;
+                    if constexpr (NK == 1) __builtin_amdgcn_s_sleep(kRingDelay);
                     for (int polls = 0;; polls++) {
                         a0 = xload(pw);
                         a1 = xload(pw + rst);
