@@ -1962,6 +1962,11 @@ __device__ __forceinline__ void op_iterp(const Tile &t, int roles, const float4 
                 v_spr[pos] = v_cur[pos];
             }
         }
+        // Interior tiles (staged region inside the image and the mask / circle) run the
+        // iterations without the boundary tests, except in 53x50 tiles, where every tile
+        // takes the path with them: measured 4.38 against 4.53 us per iteration at
+        // 848x800 (in the other shapes the split is the faster one).  Same values either way.
+        constexpr bool kFullPath = SH != 18;
         bool inner = true;
 #pragma unroll
         for (int k = 0; k < NK; k++) {
@@ -1970,7 +1975,7 @@ __device__ __forceinline__ void op_iterp(const Tile &t, int roles, const float4 
                 inner = inner && x >= 0 && y >= 0 && x < w && y < h && omk[k] != 0.f &&
                         (MASKED || !outside_circle(x, y, w, h));
         }
-        const bool full = __syncthreads_and(inner) != 0;
+        const bool full = kFullPath ? __syncthreads_and(inner) != 0 : (__syncthreads(), false);
         PrimalCoef kc[NK];
 #pragma unroll
         for (int k = 0; k < NK; k++) kc[k] = primal_coef(ip0, o[k].a, o[k].b, o[k].c, o[k].iu);
@@ -2243,8 +2248,12 @@ __device__ __forceinline__ void op_iterp(const Tile &t, int roles, const float4 
         };
         // progress of iteration 0: no words read yet (the staging covered the ring)
         if (tid == NT - 1) xstore(mine + X::HB, 0.f, tag0);
-        if (full) run(std::true_type{});
-        else run(std::false_type{});
+        if constexpr (kFullPath) {
+            if (full) run(std::true_type{});
+            else run(std::false_type{});
+        } else {
+            run(std::false_type{});
+        }
     }
 
     persist_finish(xs, ntiles, epoch);
