@@ -83,6 +83,8 @@ def main():
                     help="no launch tuner (fixed defaults; with --tuning for reproducible profiles)")
     ap.add_argument("--iter-only", type=int, default=0,
                     help="profiling aid: run N level-0 inner iterations (after a 256 MiB reference copy) and exit")
+    ap.add_argument("--iter-level", type=int, default=0,
+                    help="profiling aid: the pyramid level whose inner iterations --iter-only runs")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU only (asked for {args.gpus}); the solver shards by "
@@ -117,11 +119,11 @@ def main():
 
     if args.iter_only:
         # A plain device copy of known size (256 MiB read + 256 MiB written per dispatch)
-        # calibrates the HBM counters (FETCH_SIZE / WRITE_SIZE), then only level-0 inner
-        # iterations run.
+        # calibrates the HBM counters (FETCH_SIZE / WRITE_SIZE), then only the inner
+        # iterations of one level run.
         g.time_copy(256 << 20, 3)
-        ms, nl = g.time_inner_iterations(0, args.iter_only)
-        print(f"iter-only: {args.iter_only} level-0 iterations, {ms / args.iter_only * 1e3:.2f} us each "
+        ms, nl = g.time_inner_iterations(args.iter_level, args.iter_only)
+        print(f"iter-only: {args.iter_only} level-{args.iter_level} iterations, {ms / args.iter_only * 1e3:.2f} us each "
               f"({nl} iteration(s) per launch)", flush=True)
         return
 
